@@ -1,0 +1,145 @@
+/*
+ * rnng_amd: C ABI of the MI355X (gfx950) epsilon-neighbor-graph library.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status:
+ * RNNG_OK (0) on success, a negative RNNG_ERR_* code on failure, with a
+ * message available from rnng_last_error() on the calling thread.  No C++
+ * exception crosses this boundary.
+ *
+ * Semantics follow the reference (gabe-raulet/RNNG):
+ *   - Index = int64, Real = float (include/utils.h:24-25).
+ *   - A pair (p, q) is an edge iff L2(p, q) <= eps, where L2 is the float
+ *     sqrtf of the sequential FP32 sum of squared differences over dims
+ *     0..d-1, not fused (drivers/systolic.cpp:449-465), and eps is a float
+ *     (drivers/systolic.cpp:217,421; include/search.hpp:244,399).
+ *   - The returned weight is exactly that float distance.
+ *
+ * Calls are synchronous from the host's point of view (results are complete
+ * when the call returns), as the reference's functor-before-return contract
+ * requires (include/search.hpp:223-264).
+ */
+#ifndef RNNG_AMD_H_
+#define RNNG_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RNNG_OK            0
+#define RNNG_ERR_ARG      -1   /* invalid argument (null pointer, bad size) */
+#define RNNG_ERR_HIP      -2   /* HIP runtime error (message has the details) */
+#define RNNG_ERR_CAPACITY -3   /* caller's output buffer too small; *needed is set */
+#define RNNG_ERR_NODEV    -4   /* no usable gfx950 device */
+#define RNNG_ERR_FORMAT   -5   /* malformed input file */
+#define RNNG_ERR_UNSUPPORTED -6 /* shape the requested mode cannot handle */
+
+/* Distance-stage algorithm. */
+#define RNNG_MODE_PLAIN 0   /* FP32 tiled kernel in the reference's order */
+#define RNNG_MODE_MFMA  1   /* BF16 MFMA candidate filter + FP32 recheck + FP64 band check */
+
+/* Counters and device times of one call.  Times are HIP-event times on the
+ * library's stream, in milliseconds. */
+typedef struct rnng_stats {
+    int64_t pair_evals;    /* unordered pairs covered by the distance stage */
+    int64_t candidates;    /* filter survivors (MFMA mode; 0 in plain mode) */
+    int64_t edges;         /* undirected edges found (i < j for a self join) */
+    int64_t band_pairs;    /* candidates with |d - eps| <= 1e-5 * eps */
+    int64_t band_flips;    /* band pairs whose FP64 decision differs from FP32's */
+    int64_t refills;       /* count-then-fill reruns after an undersized buffer */
+    double prep_ms;        /* BF16 fragment preparation (MFMA mode) */
+    double filter_ms;      /* distance kernel (MFMA filter or plain kernel) */
+    double recheck_ms;     /* FP32 recheck + band check (MFMA mode) */
+    double total_ms;       /* all device work of the call */
+} rnng_stats;
+
+/* Opaque handles. */
+typedef struct rnng_dataset_s *rnng_dataset_t;   /* device-resident point set */
+typedef struct rnng_graph_s *rnng_graph_t;       /* device-resident edge list */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int rnng_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *rnng_last_error(void);
+
+/* Select and initialise the GPU (creates the library's stream).  Called
+ * implicitly with device 0 by every other entry point if not called first. */
+int rnng_init(int device);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident point sets                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Upload n points of d floats (row-major) to HBM.  The PointContainer
+ * layout of the reference (include/point.h:34-100) with fixed dimension:
+ * point i is points[i*d .. i*d+d). */
+int rnng_dataset_create_f32(const float *points, int64_t n, int32_t d, rnng_dataset_t *out);
+
+/* Wrap points that already live in device memory (not copied, not freed). */
+int rnng_dataset_wrap_device_f32(const float *dev_points, int64_t n, int32_t d,
+                                 rnng_dataset_t *out);
+
+int rnng_dataset_destroy(rnng_dataset_t ds);
+
+/* ------------------------------------------------------------------------ */
+/* L2 epsilon graph                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* All pairs i < j of ds with L2 <= eps (the systolic driver's all-pairs
+ * query, drivers/systolic.cpp:329-343, over one GPU's whole point set).
+ * The edge list stays on the device in *out until copied. */
+int rnng_graph_build(rnng_dataset_t ds, float eps, int32_t mode, rnng_graph_t *out,
+                     rnng_stats *stats);
+
+/* All pairs (i, j), i in a, j in b, with L2 <= eps (block x block, cell x
+ * ghosts: include/search.hpp:266-340 over two point sets). */
+int rnng_graph_build_bipartite(rnng_dataset_t a, rnng_dataset_t b, float eps, int32_t mode,
+                               rnng_graph_t *out, rnng_stats *stats);
+
+int rnng_graph_num_edges(rnng_graph_t g, int64_t *n_edges);
+
+/* Copy up to cap edges: ij[2e] = i, ij[2e+1] = j, w[e] = float distance
+ * (either may be NULL).  sorted != 0 orders them by (i, j). */
+int rnng_graph_copy(rnng_graph_t g, int64_t *ij, float *w, int64_t cap, int32_t sorted);
+
+int rnng_graph_destroy(rnng_graph_t g);
+
+/* One-shot host-to-host form with the count-then-fill contract: at most cap
+ * edges are written to (ij, w), sorted by (i, j); *needed receives the total.
+ * Returns RNNG_ERR_CAPACITY if cap < *needed (nothing partial is promised). */
+int rnng_l2_eps_pairs_f32(const float *points, int64_t n, int32_t d, float eps, int32_t mode,
+                          int64_t *ij, float *w, int64_t cap, int64_t *needed,
+                          rnng_stats *stats);
+
+int rnng_l2_eps_bipartite_f32(const float *a, int64_t na, const float *b, int64_t nb,
+                              int32_t d, float eps, int32_t mode, int64_t *ij, float *w,
+                              int64_t cap, int64_t *needed, rnng_stats *stats);
+
+/* Initial sizes (in edges and in filter candidates) of the buffers of the
+ * next builds; 0 restores the library default.  Undersized buffers are safe:
+ * the kernels keep counting past the end, and the call grows the buffer to
+ * the exact count and runs again (count-then-fill), reported in
+ * rnng_stats.refills.  The learned cost model uses this to size buffers. */
+int rnng_set_initial_capacity(int64_t edges, int64_t candidates);
+
+/* Largest float T with sqrtf(T) <= eps (-1 when no value qualifies).  The
+ * kernels compare squared distances against T; sqrtf is monotone, so
+ * sqrtf(v) <= eps  <=>  v <= T. */
+float rnng_sq_threshold(float eps);
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Runs one 32x32x16 BF16 MFMA with inputs that encode row and column
+ * indices and reports, for each (lane, accumulator register), the output
+ * row and column it holds: rows[lane*16 + r], cols[lane*16 + r]. */
+int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

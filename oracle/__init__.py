@@ -1,0 +1,177 @@
+"""CPU oracle for the epsilon-neighbor-graph path (ctypes wrapper of liborc.so).
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this package, and only as the
+checker.  The product (``rnng_amd``) never imports it.
+
+Each function restates a reference routine; the C source (oracle.c / oracle.h)
+cites the reference file:line for each and lists the checks that pin it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+def build() -> None:
+    """Compile liborc.so with the pinned FP flags (make -C oracle)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = C.CDLL(_LIB_PATH)
+    f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    lib.orc_l2sq.argtypes = [f32p, f32p, C.c_int64]
+    lib.orc_l2sq.restype = C.c_float
+    lib.orc_l2.argtypes = [f32p, f32p, C.c_int64]
+    lib.orc_l2.restype = C.c_float
+    lib.orc_sq_threshold.argtypes = [C.c_float]
+    lib.orc_sq_threshold.restype = C.c_float
+    lib.orc_l2_eps_pairs.argtypes = [f32p, C.c_int64, f32p, C.c_int64, C.c_int64, C.c_float,
+                                     C.c_int, C.c_int, vp, vp, C.c_int64]
+    lib.orc_l2_eps_pairs.restype = C.c_int64
+    lib.orc_edit_distance.argtypes = [C.c_char_p, C.c_int64, C.c_char_p, C.c_int64]
+    lib.orc_edit_distance.restype = C.c_int64
+    lib.orc_edit_eps_pairs.argtypes = [vp, i64p, C.c_int64, C.c_float, C.c_int, vp, vp, C.c_int64]
+    lib.orc_edit_eps_pairs.restype = C.c_int64
+    lib.orc_voronoi_assign.argtypes = [f32p, C.c_int64, f32p, C.c_int64, C.c_int64, C.c_int,
+                                       i64p, f32p]
+    lib.orc_voronoi_assign.restype = None
+    lib.orc_ghosts.argtypes = [f32p, C.c_int64, C.c_int64, i64p, f32p, f32p, C.c_int64,
+                               C.c_float, C.c_int, vp, C.c_int64]
+    lib.orc_ghosts.restype = C.c_int64
+    lib.orc_binom.argtypes = [C.c_int64, C.c_int64]
+    lib.orc_binom.restype = C.c_int64
+    lib.orc_simplex_id.argtypes = [i64p, C.c_int64]
+    lib.orc_simplex_id.restype = C.c_int64
+    lib.orc_rips_filtration.argtypes = [f32p, C.c_int64, C.c_int64, C.c_float, C.c_int64,
+                                        C.c_int, vp, vp, C.c_int64]
+    lib.orc_rips_filtration.restype = C.c_int64
+    _lib = lib
+    return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def l2sq(p, q) -> np.float32:
+    p, q = _f32(p), _f32(q)
+    return np.float32(_load().orc_l2sq(p, q, p.shape[-1]))
+
+
+def l2(p, q) -> np.float32:
+    p, q = _f32(p), _f32(q)
+    return np.float32(_load().orc_l2(p, q, p.shape[-1]))
+
+
+def sq_threshold(eps: float) -> np.float32:
+    return np.float32(_load().orc_sq_threshold(np.float32(eps)))
+
+
+def l2_eps_pairs(X, eps, Y=None, nthreads: int = 0):
+    """Brute-force epsilon pairs.  Self join (Y is None): i < j.  Returns
+    (ij int64 [E,2], w float32 [E]) in row-major order."""
+    lib = _load()
+    X = _f32(X)
+    self_join = Y is None
+    Y = X if self_join else _f32(Y)
+    nx, d = X.shape
+    ny = Y.shape[0]
+    eps = np.float32(eps)
+    n = lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, None, None, 0)
+    ij = np.empty((n, 2), np.int64)
+    w = np.empty(n, np.float32)
+    lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, _ptr(ij), _ptr(w), n)
+    return ij, w
+
+
+def l2_eps_pairs_count(X, eps, Y=None, nthreads: int = 0) -> int:
+    lib = _load()
+    X = _f32(X)
+    self_join = Y is None
+    Y = X if self_join else _f32(Y)
+    return int(lib.orc_l2_eps_pairs(X, X.shape[0], Y, Y.shape[0], X.shape[1], np.float32(eps),
+                                    int(self_join), nthreads, None, None, 0))
+
+
+def edit_distance(s: bytes, t: bytes) -> int:
+    return int(_load().orc_edit_distance(s, len(s), t, len(t)))
+
+
+def edit_eps_pairs(atoms: np.ndarray, offsets: np.ndarray, eps, nthreads: int = 0):
+    """Brute-force edit-distance pairs i < j with dist <= eps.  Returns
+    (ij int64 [E,2], dist int32 [E])."""
+    lib = _load()
+    atoms = np.ascontiguousarray(atoms, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    n = offsets.shape[0] - 1
+    eps = np.float32(eps)
+    cnt = lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, None, None, 0)
+    ij = np.empty((cnt, 2), np.int64)
+    dist = np.empty(cnt, np.int32)
+    lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, _ptr(ij), _ptr(dist), cnt)
+    return ij, dist
+
+
+def voronoi_assign(pts, centers, nthreads: int = 0):
+    lib = _load()
+    pts, centers = _f32(pts), _f32(centers)
+    n, d = pts.shape
+    cell = np.empty(n, np.int64)
+    dist = np.empty(n, np.float32)
+    lib.orc_voronoi_assign(pts, n, centers, centers.shape[0], d, nthreads, cell, dist)
+    return cell, dist
+
+
+def ghosts(pts, cell, delta, centers, eps, nthreads: int = 0):
+    """Ghost pairs (cell, point) sorted by (cell, point)."""
+    lib = _load()
+    pts, centers, delta = _f32(pts), _f32(centers), _f32(delta)
+    cell = np.ascontiguousarray(cell, dtype=np.int64)
+    n, d = pts.shape
+    m = centers.shape[0]
+    cnt = lib.orc_ghosts(pts, n, d, cell, delta, centers, m, np.float32(eps), nthreads, None, 0)
+    out = np.empty((cnt, 2), np.int64)
+    lib.orc_ghosts(pts, n, d, cell, delta, centers, m, np.float32(eps), nthreads, _ptr(out), cnt)
+    return out
+
+
+def binom(n: int, k: int) -> int:
+    return int(_load().orc_binom(n, k))
+
+
+def simplex_id(verts) -> int:
+    v = np.ascontiguousarray(verts, dtype=np.int64)
+    return int(_load().orc_simplex_id(v, v.shape[0]))
+
+
+def rips_filtration(pts, eps, maxdim: int = 2, nthreads: int = 0):
+    """Sorted (value float32 [S], id int64 [S]) of the Rips filtration."""
+    lib = _load()
+    pts = _f32(pts)
+    n, d = pts.shape
+    cnt = lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, None, None, 0)
+    val = np.empty(cnt, np.float32)
+    ids = np.empty(cnt, np.int64)
+    lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, _ptr(val), _ptr(ids), cnt)
+    return val, ids

@@ -1,0 +1,114 @@
+"""ctypes binding of librnng_amd.so (the C ABI declared in include/rnng_amd.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+the shared object is missing, or no gfx950 device is present when a compute
+entry point is called, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librnng_amd.so")
+
+RNNG_OK = 0
+RNNG_ERR_ARG = -1
+RNNG_ERR_HIP = -2
+RNNG_ERR_CAPACITY = -3
+RNNG_ERR_NODEV = -4
+RNNG_ERR_FORMAT = -5
+RNNG_ERR_UNSUPPORTED = -6
+
+MODE_PLAIN = 0
+MODE_MFMA = 1
+
+
+class RnngError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rnng error {code}: {msg}")
+        self.code = code
+
+
+class Stats(C.Structure):
+    """Mirror of rnng_stats (include/rnng_amd.h)."""
+    _fields_ = [
+        ("pair_evals", C.c_int64),
+        ("candidates", C.c_int64),
+        ("edges", C.c_int64),
+        ("band_pairs", C.c_int64),
+        ("band_flips", C.c_int64),
+        ("refills", C.c_int64),
+        ("prep_ms", C.c_double),
+        ("filter_ms", C.c_double),
+        ("recheck_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# Every symbol the header declares, with (restype, argtypes).
+_vp = C.c_void_p
+_i64p = C.POINTER(C.c_int64)
+_SIGS = {
+    "rnng_version": (C.c_int, []),
+    "rnng_last_error": (C.c_char_p, []),
+    "rnng_init": (C.c_int, [C.c_int]),
+    "rnng_dataset_create_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "rnng_dataset_wrap_device_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "rnng_dataset_destroy": (C.c_int, [_vp]),
+    "rnng_graph_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp), C.POINTER(Stats)]),
+    "rnng_graph_build_bipartite": (C.c_int, [_vp, _vp, C.c_float, C.c_int32, C.POINTER(_vp),
+                                             C.POINTER(Stats)]),
+    "rnng_graph_num_edges": (C.c_int, [_vp, _i64p]),
+    "rnng_graph_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int32]),
+    "rnng_graph_destroy": (C.c_int, [_vp]),
+    "rnng_l2_eps_pairs_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_float, C.c_int32, _vp, _vp,
+                                        C.c_int64, _i64p, C.POINTER(Stats)]),
+    "rnng_l2_eps_bipartite_f32": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, C.c_int32, C.c_float,
+                                            C.c_int32, _vp, _vp, C.c_int64, _i64p,
+                                            C.POINTER(Stats)]),
+    "rnng_set_initial_capacity": (C.c_int, [C.c_int64, C.c_int64]),
+    "rnng_sq_threshold": (C.c_float, [C.c_float]),
+    "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
+}
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile the HIP sources for gfx950 with hipcc (make -C csrc)."""
+    args = ["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")]
+    if force:
+        subprocess.run(args + ["clean"], check=True)
+    subprocess.run(args, check=True)
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+def check(rc: int) -> None:
+    if rc != RNNG_OK:
+        msg = lib().rnng_last_error()
+        raise RnngError(rc, msg.decode() if msg else "")

@@ -1,0 +1,1154 @@
+// L2 epsilon-range kernels for gfx950 and their host orchestration.
+//
+// Reference semantics (gabe-raulet/RNNG):
+//   L2Distance<float>::operator()  drivers/systolic.cpp:449-465  -- sequential
+//       FP32 sum over dims 0..d-1 of fl(fl(p_k - q_k)^2), not fused, then sqrt
+//   edge test dist <= radius       include/search.hpp:244,399 (radius is float)
+//   all-pairs query over a block   drivers/systolic.cpp:329-343
+//
+// Two distance-stage algorithms:
+//   PLAIN  k_l2_plain: LDS-tiled FP32 kernel that evaluates every pair in the
+//          reference order (sub, mul, add, contraction off) and compares the
+//          squared sum with T(eps), the largest float whose sqrtf is <= eps.
+//   MFMA   k_l2_mfma_filter: BF16 32x32x16 MFMA over augmented vectors that
+//          produces  acc = |x~|^2 + |y~|^2 - 2 x~.y~ - C0  for every pair, where
+//          C0 bounds (from above) what acc can be for any reference edge.  A
+//          pair is a candidate iff acc < 0, tested as the sign bit of the OR of
+//          the accumulators, so the epilogue costs half a VALU op per pair.
+//          k_l2_recheck then recomputes every candidate in the reference's FP32
+//          order (that decides the edge) and sends pairs within 1e-5*eps of eps
+//          to an FP64 check that is counted and reported.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+struct rnng_dataset_s {
+    float* X = nullptr;       // n*d floats in HBM, row-major
+    int64_t n = 0;
+    int32_t d = 0;
+    bool owned = false;
+};
+
+struct rnng_graph_s {
+    rnng::DevBuf ij;          // int2 (i, j)
+    rnng::DevBuf w;           // float distance
+    int64_t n_edges = 0;
+    int64_t cap = 0;
+};
+
+namespace rnng {
+
+float sq_threshold_host(float eps) {
+    if (!(eps >= 0.0f)) return -1.0f;
+    if (std::isinf(eps)) return INFINITY;
+    float v = (float)((double)eps * (double)eps);
+    if (std::isinf(v)) v = FLT_MAX;
+    while (v > 0.0f && std::sqrt(v) > eps) v = std::nextafter(v, -INFINITY);
+    for (;;) {
+        float up = std::nextafter(v, INFINITY);
+        if (std::isinf(up) || std::sqrt(up) > eps) break;
+        v = up;
+    }
+    return v;
+}
+
+namespace {
+
+// ------------------------------------------------------------------------
+// Small utility kernels
+// ------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Order-preserving float <-> uint mapping for atomic min/max.
+__device__ __forceinline__ unsigned f2ord(float f) {
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(unsigned u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// ------------------------------------------------------------------------
+// PLAIN: reference-order FP32 tiled kernel
+// ------------------------------------------------------------------------
+constexpr int PB = 128;   // pairs tile edge per workgroup
+constexpr int PK = 16;    // dims staged per LDS round
+
+// Tile t of the upper triangle (row-major, diagonal included) of an nb x nb grid.
+__device__ __forceinline__ void decode_upper(long long t, int nb, int& bi, int& bj) {
+    const double a = 2.0 * nb + 1.0;
+    long long b = (long long)((a - sqrt(a * a - 8.0 * (double)t)) * 0.5);
+    auto start = [nb](long long r) { return r * nb - r * (r - 1) / 2; };
+    if (b < 0) b = 0;
+    while (b > 0 && start(b) > t) --b;
+    while (b + 1 < nb && start(b + 1) <= t) ++b;
+    bi = (int)b;
+    bj = (int)(b + (t - start(b)));
+}
+
+__global__ __launch_bounds__(256) void k_l2_plain(
+    const float* __restrict__ X, int nx, const float* __restrict__ Y, int ny, int d, float T,
+    int self_join, int nbx, int nby, int2* __restrict__ out_ij, float* __restrict__ out_w,
+    unsigned long long cap, unsigned long long* __restrict__ counter) {
+#pragma clang fp contract(off)
+    __shared__ float As[PK][PB + 4];
+    __shared__ float Bs[PK][PB + 4];
+
+    int bi, bj;
+    if (self_join) {
+        decode_upper((long long)blockIdx.x, nbx, bi, bj);
+    } else {
+        bi = blockIdx.x / nby;
+        bj = blockIdx.x % nby;
+    }
+    const int row0 = bi * PB, col0 = bj * PB;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int lr = tid >> 1, lk = (tid & 1) * 8;   // loader: row and k offset
+
+    float acc[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = 0.0f;
+
+    for (int k0 = 0; k0 < d; k0 += PK) {
+        {
+            const int gr = row0 + lr, gc = col0 + lr;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + lk + e;
+                As[lk + e][lr] = (gr < nx && k < d) ? X[(size_t)gr * d + k] : 0.0f;
+                Bs[lk + e][lr] = (gc < ny && k < d) ? Y[(size_t)gc * d + k] : 0.0f;
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < PK; ++kk) {
+            const float4 a0 = *(const float4*)&As[kk][ty * 4];
+            const float4 a1 = *(const float4*)&As[kk][64 + ty * 4];
+            const float4 b0 = *(const float4*)&Bs[kk][tx * 4];
+            const float4 b1 = *(const float4*)&Bs[kk][64 + tx * 4];
+            const float a[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+            const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float t = a[r] - b[c];
+                    acc[r][c] = acc[r][c] + t * t;
+                }
+        }
+        __syncthreads();
+    }
+
+    // Padding dims contribute exact +0.0, so acc equals the reference sum.
+    unsigned cnt = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = row0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int j = col0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
+            const bool ok = i < nx && j < ny && (!self_join || i < j) && acc[r][c] <= T;
+            cnt += ok ? 1u : 0u;
+        }
+    }
+    if (!__any(cnt != 0)) return;
+    unsigned long long pos = wave_reserve(cnt, counter);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int i = row0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + r - 4);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int j = col0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + c - 4);
+            const bool ok = i < nx && j < ny && (!self_join || i < j) && acc[r][c] <= T;
+            if (ok) {
+                if (pos < cap) {
+                    out_ij[pos] = make_int2(i, j);
+                    out_w[pos] = __fsqrt_rn(acc[r][c]);
+                }
+                ++pos;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// MFMA path: data preparation
+// ------------------------------------------------------------------------
+// Fragment layout (shared by the x-form and the y-form):
+//   tile t = 32 points, k-step s = 16 augmented dims, lane l in 0..63:
+//   frag[(t*KS + s)*64 + l] = 8 bf16 of point 32t + (l & 31),
+//                             dims 16s + 8(l >> 5) .. +7
+// which is exactly the per-lane operand of v_mfma_f32_32x32x16_bf16 (A: row
+// = lane&31; B: column = lane&31; the 16 dims of a step split by lane>>5), so
+// a wave loads one 1 KB fragment with one fully coalesced 16-byte load.
+//
+// Augmented vectors (x = row side, y = column side), dims d..d+5:
+//   x~ = [bf16(x - mu) ..., n1, n2, n3, 1, 1, 1, 0 ...]
+//   y~ = [-2 bf16(y - mu) ..., 1, 1, 1, n1', n2', n3', 0 ...]
+// n1+n2+n3 = |bf16(x - mu)|^2 split into three bf16 pieces, so
+//   x~ . y~ = |x'|^2 + |y'|^2 - 2 x'.y' = |x' - y'|^2
+// and with the accumulator initialised to -C0, acc < 0 marks a candidate.
+// Padding points carry 2^100 in a norm slot so their acc is always positive.
+
+constexpr int NSTAT = 8;
+// stat slots (unsigned, ordered-float or flags) in ds_stat[]
+enum { ST_INT = 0, ST_MAXE = 1, ST_MAXN = 2, ST_MAXRES = 3, ST_MAXDEV = 4 };
+
+struct MfmaParams {
+    float negc0;      // -C0 as float (|negc0| >= C0)
+    int int_mode;     // 1: exact integer filter
+    double c0;        // C0 in double, for reporting
+};
+
+// Per-dimension sums, minima and maxima, and the all-integer flag.
+__global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ X, int n, int d,
+                                                  double* __restrict__ colsum,
+                                                  unsigned* __restrict__ colmin,
+                                                  unsigned* __restrict__ colmax,
+                                                  unsigned* __restrict__ stat) {
+    const int r0 = blockIdx.x * 256;
+    const int r1 = min(n, r0 + 256);
+    bool isint = true;
+    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+        double s = 0.0;
+        float mn = INFINITY, mx = -INFINITY;
+        for (int r = r0; r < r1; ++r) {
+            const float v = X[(size_t)r * d + k];
+            s += (double)v;
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+            isint = isint && (v == rintf(v));
+        }
+        if (r1 > r0) {
+            atomicAdd(&colsum[k], s);
+            atomicMin(&colmin[k], f2ord(mn));
+            atomicMax(&colmax[k], f2ord(mx));
+        }
+    }
+    if (!__all(isint) && lane_id() == 0) atomicAnd(&stat[ST_INT], 0u);
+}
+
+// mu_k = per-dimension mean; rounded to an integer when every value is one so
+// that x - mu stays exact.  Also records the largest |x - mu| and the widest
+// per-dimension range (both as ordered floats).
+__global__ void k_center(const double* __restrict__ colsum, const unsigned* __restrict__ colmin,
+                         const unsigned* __restrict__ colmax, int n, int d,
+                         float* __restrict__ mu, unsigned* __restrict__ stat,
+                         unsigned* __restrict__ range_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    const bool isint = stat[ST_INT] != 0;
+    double m = n > 0 ? colsum[k] / (double)n : 0.0;
+    float muk = isint ? (float)rint(m) : (float)m;
+    mu[k] = muk;
+    const float mn = ord2f(colmin[k]), mx = ord2f(colmax[k]);
+    const float dev = fmaxf(fabsf(mx - muk), fabsf(mn - muk));
+    atomicMax(&stat[ST_MAXDEV], f2ord(dev));
+    atomicMax(range_out, f2ord(mx - mn));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One thread per point: x~ and/or y~ fragments (either output may be null),
+// norm pieces and the error terms.
+//   slot       d      d+1   d+2   d+3   d+4   d+5
+//   x~ real    n1     n2    n3    1     1     1
+//   y~ real    1      1     1     n1    n2    n3
+//   x~ pad     2^100  0     0     1     0     0
+//   y~ pad     1      0     0     2^100 0     0
+// so any pair with a padding point has acc >= 2^100 - C0 > 0.
+__global__ __launch_bounds__(256) void k_prep_frags(
+    const float* __restrict__ X, int n, int n_pad, int d, int KS, const float* __restrict__ mu,
+    uint16_t* __restrict__ xa, uint16_t* __restrict__ yb, unsigned* __restrict__ stat) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float e_max = 0.0f, n_max = 0.0f, r_max = 0.0f;
+    if (i < n_pad) {
+        const int t = i >> 5, l = i & 31;
+        const bool real = i < n;
+        double nrm = 0.0, err = 0.0;
+        const int K = KS * 16;
+        if (real) {
+            for (int k = 0; k < d; ++k) {
+                const float x = X[(size_t)i * d + k];
+                const double exact = (double)x - (double)mu[k];
+                const float xb = bf16_to_f32(f32_to_bf16_rn((float)exact));
+                nrm += (double)xb * (double)xb;
+                const double de = (double)xb - exact;
+                err += de * de;
+            }
+        }
+        float p[3] = {0.f, 0.f, 0.f};
+        double res = 0.0;
+        if (real) {
+            double r = nrm;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                p[q] = bf16_to_f32(f32_to_bf16_rn((float)r));
+                r -= (double)p[q];
+            }
+            res = fabs(r);
+        }
+        const float big = 1.2676506e30f;   // 2^100, exact in bf16
+        for (int k8 = 0; k8 < K; k8 += 8) {
+            const int s = k8 >> 4, h = (k8 >> 3) & 1;
+            const size_t slot = ((size_t)(t * KS + s) * 64 + l + 32 * h) * 8;
+            uint16_t xv[8], yv[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k8 + e;
+                float xf = 0.f, yf = 0.f;
+                if (k < d) {
+                    if (real) {
+                        const float x = X[(size_t)i * d + k];
+                        xf = bf16_to_f32(f32_to_bf16_rn((float)((double)x - (double)mu[k])));
+                        yf = -2.0f * xf;
+                    }
+                } else if (k < d + 3) {
+                    if (real) {
+                        xf = p[k - d];
+                        yf = 1.0f;
+                    } else {
+                        xf = (k == d) ? big : 0.f;
+                        yf = (k == d) ? 1.0f : 0.f;
+                    }
+                } else if (k < d + 6) {
+                    if (real) {
+                        xf = 1.0f;
+                        yf = p[k - d - 3];
+                    } else {
+                        xf = (k == d + 3) ? 1.0f : 0.f;
+                        yf = (k == d + 3) ? big : 0.f;
+                    }
+                }
+                xv[e] = f32_to_bf16_rn(xf);
+                yv[e] = f32_to_bf16_rn(yf);
+            }
+            if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
+            if (yb) *(uint4*)(yb + slot) = *(const uint4*)yv;
+        }
+        if (real) {
+            e_max = (float)sqrt(err) * (1.0f + 1e-6f);
+            n_max = (float)nrm * (1.0f + 1e-6f);
+            r_max = (float)res * 2.0f;
+        }
+    }
+    e_max = wave_max(e_max);
+    n_max = wave_max(n_max);
+    r_max = wave_max(r_max);
+    if (lane_id() == 0) {
+        atomicMax(&stat[ST_MAXE], __float_as_uint(e_max));
+        atomicMax(&stat[ST_MAXN], __float_as_uint(n_max));
+        atomicMax(&stat[ST_MAXRES], __float_as_uint(r_max));
+    }
+}
+
+// C0: an upper bound of what the MFMA can compute for |x~ - y~|^2 for any pair
+// whose reference FP32 distance is <= the band radius.
+__global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned* __restrict__ range,
+                              int d, int KS, float Tb, MfmaParams* __restrict__ out) {
+    const bool isint = stat[ST_INT] != 0;
+    const double maxdev = (double)ord2f(stat[ST_MAXDEV]);
+    const double R = (double)ord2f(*range);
+    const double maxE = (double)__uint_as_float(stat[ST_MAXE]);
+    const double maxN = (double)__uint_as_float(stat[ST_MAXN]);
+    const double maxRes = (double)__uint_as_float(stat[ST_MAXRES]);
+    const double two24 = 16777216.0;
+    double c0;
+    int int_mode = 0;
+    if (isint && maxdev <= 256.0 && (double)d * R * R < two24 &&
+        (double)Tb + 4.0 * maxN + 2.0 < two24) {
+        // All reference partial sums and all MFMA partial sums are integers
+        // below 2^24: both computations are exact, so acc = |x - y|^2 - C0.
+        c0 = floor((double)Tb) + 0.5;
+        int_mode = 1;
+    } else {
+        const double u = 1.0 / two24;
+        const double g_ref = (d + 3) * u * 1.01;
+        const double smax = (double)Tb / (1.0 - g_ref);
+        const double r = sqrt(smax) + 2.0 * maxE;
+        const double base = r * r;
+        const double nterms = KS * 16.0 + 1.0;
+        c0 = base + 2.0 * maxRes + 2.0 * nterms * u * (base + 4.0 * maxN);
+        c0 = c0 * (1.0 + 1.0 / 1048576.0) + 1e-30;
+    }
+    float f = (float)c0;
+    if ((double)f < c0) f = nextafterf(f, INFINITY);
+    out->negc0 = -f;
+    out->int_mode = int_mode;
+    out->c0 = c0;
+}
+
+// ------------------------------------------------------------------------
+// MFMA filter kernel
+// ------------------------------------------------------------------------
+// Workgroup = 4 waves = one row block of 8 row tiles (256 rows); wave w owns
+// row tiles 2w and 2w+1 and keeps their fragments in registers for the whole
+// work item.  The work item's column tiles stream through: 9 coalesced 1 KB
+// loads per tile, 18 MFMAs, 32 ORs, one wave-uniform branch.
+// Output row/column of accumulator register r in lane l (32x32 MFMA):
+//   row = 8*(r/4) + 4*(l/32) + r%4, column = l%32.
+__device__ __forceinline__ int mfma32_row(int lane, int r) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_l2_mfma_filter(
+    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb,
+    const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    const MfmaParams* __restrict__ prm, int2* __restrict__ cand, unsigned long long cap,
+    unsigned long long* __restrict__ counter) {
+    const int4 wi = work[blockIdx.x];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int rt = wi.x * 8 + wave * 2;
+
+    bf16x8_t a0[KS], a1[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        a0[s] = xa[((size_t)rt * KS + s) * 64 + lane];
+        a1[s] = xa[((size_t)(rt + 1) * KS + s) * 64 + lane];
+    }
+    const float negc0 = prm->negc0;
+    const int row_base0 = rt * 32, row_base1 = rt * 32 + 32;
+
+    bf16x8_t b[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) b[s] = yb[((size_t)wi.y * KS + s) * 64 + lane];
+
+    for (int ct = wi.y; ct < wi.z; ++ct) {
+        bf16x8_t bn[KS];
+        const int ctn = (ct + 1 < wi.z) ? ct + 1 : ct;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) bn[s] = yb[((size_t)ctn * KS + s) * 64 + lane];
+
+        f32x16_t c0, c1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            c0[r] = negc0;
+            c1[r] = negc0;
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], b[s], c0, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[s], b[s], c1, 0, 0, 0);
+        }
+        unsigned o = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) o |= __float_as_uint(c0[r]) | __float_as_uint(c1[r]);
+
+        if (__any((int)o < 0)) {
+            const int col = ct * 32 + (lane & 31);
+            unsigned cnt = 0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i0 = row_base0 + mfma32_row(lane, r);
+                const int i1 = row_base1 + mfma32_row(lane, r);
+                const bool p0 = c0[r] < 0.0f && i0 < n_rows && col < n_cols && (!self_join || i0 < col);
+                const bool p1 = c1[r] < 0.0f && i1 < n_rows && col < n_cols && (!self_join || i1 < col);
+                cnt += (p0 ? 1u : 0u) + (p1 ? 1u : 0u);
+            }
+            unsigned long long pos = wave_reserve(cnt, counter);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i0 = row_base0 + mfma32_row(lane, r);
+                const int i1 = row_base1 + mfma32_row(lane, r);
+                const bool p0 = c0[r] < 0.0f && i0 < n_rows && col < n_cols && (!self_join || i0 < col);
+                const bool p1 = c1[r] < 0.0f && i1 < n_rows && col < n_cols && (!self_join || i1 < col);
+                if (p0) {
+                    if (pos < cap) cand[pos] = make_int2(i0, col);
+                    ++pos;
+                }
+                if (p1) {
+                    if (pos < cap) cand[pos] = make_int2(i1, col);
+                    ++pos;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < KS; ++s) b[s] = bn[s];
+    }
+}
+
+// ------------------------------------------------------------------------
+// FP32 recheck in the reference's order + FP64 band check
+// ------------------------------------------------------------------------
+// counters: [0] candidates, [1] edges, [2] band pairs, [3] band flips
+template <bool VEC4>
+__global__ __launch_bounds__(256) void k_l2_recheck(
+    const float* __restrict__ X, const float* __restrict__ Y, int d,
+    const int2* __restrict__ cand, unsigned long long ccap, float T, float eps,
+    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    unsigned long long* __restrict__ counters) {
+#pragma clang fp contract(off)
+    const unsigned long long nc = min(counters[0], ccap);
+    const int lane = lane_id();
+    const double band = 1e-5 * (double)eps;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long c = base + threadIdx.x;
+        bool edge = false;
+        int2 p = make_int2(0, 0);
+        float w = 0.0f;
+        if (c < nc) {
+            p = cand[c];
+            const float* x = X + (size_t)p.x * d;
+            const float* y = Y + (size_t)p.y * d;
+            float acc = 0.0f;
+            if (VEC4) {
+                for (int k = 0; k < d; k += 4) {
+                    const float4 xv = *(const float4*)(x + k);
+                    const float4 yv = *(const float4*)(y + k);
+                    float t;
+                    t = xv.x - yv.x; acc = acc + t * t;
+                    t = xv.y - yv.y; acc = acc + t * t;
+                    t = xv.z - yv.z; acc = acc + t * t;
+                    t = xv.w - yv.w; acc = acc + t * t;
+                }
+            } else {
+                for (int k = 0; k < d; ++k) {
+                    const float t = x[k] - y[k];
+                    acc = acc + t * t;
+                }
+            }
+            edge = acc <= T;
+            w = __fsqrt_rn(acc);
+            if (fabs((double)w - (double)eps) <= band) {
+                atomicAdd(&counters[2], 1ull);
+                double s = 0.0;
+                for (int k = 0; k < d; ++k) {
+                    const double t = (double)x[k] - (double)y[k];
+                    s += t * t;
+                }
+                const bool in64 = sqrt(s) <= (double)eps;
+                if (in64 != edge) atomicAdd(&counters[3], 1ull);
+            }
+        }
+        const unsigned long long m = __ballot(edge);
+        if (m) {
+            unsigned lo = 0, hi = 0;
+            if (lane == 0) {
+                unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)__popcll(m));
+                lo = (unsigned)b0;
+                hi = (unsigned)(b0 >> 32);
+            }
+            lo = __shfl(lo, 0, 64);
+            hi = __shfl(hi, 0, 64);
+            const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
+            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (edge && b0 + rank < ecap) {
+                eij[b0 + rank] = p;
+                ew[b0 + rank] = w;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Debug: MFMA output layout
+// ------------------------------------------------------------------------
+__global__ void k_mfma_layout(int* rows, int* cols) {
+    const int lane = threadIdx.x;
+    bf16x8_t a, b;
+    // A[m][k]: k==0 -> m+1, k==1 -> 1 ; B[k][n]: k==0 -> 1, k==1 -> 64*(n+1)
+    // D[m][n] = (m+1) + 64*(n+1), decodable with exact small integers.
+    const int m = lane & 31, kb = (lane >> 5) * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = kb + e;
+        a[e] = (__bf16)(k == 0 ? (float)(m + 1) : (k == 1 ? 1.0f : 0.0f));
+        b[e] = (__bf16)(k == 0 ? 1.0f : (k == 1 ? 64.0f * (float)(m + 1) : 0.0f));
+    }
+    f32x16_t c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int v = (int)c[r];
+        rows[lane * 16 + r] = (v % 64) - 1;
+        cols[lane * 16 + r] = (v / 64) - 1;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Host orchestration
+// ------------------------------------------------------------------------
+struct Workspace {
+    DevBuf counters;   // u64[8]
+    DevBuf cand;       // int2
+    unsigned long long cand_cap = 0;
+    DevBuf xa, yb;     // fragments
+    DevBuf colsum, colmin, colmax, stat, range, mu, params;
+    DevBuf work;
+    std::vector<int4> work_host;
+    int work_key[4] = {-1, -1, -1, -1};
+};
+
+std::mutex g_ws_mu;
+Workspace g_ws;
+int64_t g_edge_hint = 0;   // initial edge-buffer size of new graphs (0: default)
+int64_t g_cand_hint = 0;   // initial candidate-buffer size (0: default)
+
+inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    k_fill_u64<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+int launch_fill_u32(unsigned* p, int n, unsigned v) {
+    k_fill_u32<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+// Row blocks of 8 tiles (256 rows) x column chunks of CH tiles, chunk-major
+// so that the workgroups resident at one time share column fragments in L2.
+constexpr int CH = 256;
+
+int build_work(Workspace& ws, int n_rowblocks, int n_coltiles, int self_join) {
+    if (ws.work_key[0] == n_rowblocks && ws.work_key[1] == n_coltiles &&
+        ws.work_key[2] == self_join && ws.work_key[3] == CH)
+        return RNNG_OK;
+    ws.work_host.clear();
+    const int nchunks = (n_coltiles + CH - 1) / CH;
+    for (int c = 0; c < nchunks; ++c) {
+        const int c0 = c * CH, c1 = std::min(n_coltiles, c0 + CH);
+        for (int rb = 0; rb < n_rowblocks; ++rb) {
+            int lo = c0;
+            if (self_join) lo = std::max(lo, rb * 8);
+            if (lo >= c1) continue;
+            ws.work_host.push_back(make_int4(rb, lo, c1, 0));
+        }
+    }
+    RNNG_TRY(ws.work.reserve(ws.work_host.size() * sizeof(int4)));
+    RNNG_HIP_TRY(hipMemcpyAsync(ws.work.ptr, ws.work_host.data(), ws.work_host.size() * sizeof(int4),
+                                hipMemcpyHostToDevice, stream()));
+    ws.work_key[0] = n_rowblocks;
+    ws.work_key[1] = n_coltiles;
+    ws.work_key[2] = self_join;
+    ws.work_key[3] = CH;
+    return RNNG_OK;
+}
+
+template <int KS>
+void launch_filter(Workspace& ws, int n_rows, int n_cols, int self_join) {
+    k_l2_mfma_filter<KS><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+        ws.xa.as<bf16x8_t>(), ws.yb.as<bf16x8_t>(), ws.work.as<int4>(), n_rows, n_cols, self_join,
+        ws.params.as<MfmaParams>(), ws.cand.as<int2>(), ws.cand_cap,
+        ws.counters.as<unsigned long long>());
+}
+
+int dispatch_filter(Workspace& ws, int KS, int n_rows, int n_cols, int self_join) {
+    switch (KS) {
+        case 1: launch_filter<1>(ws, n_rows, n_cols, self_join); break;
+        case 2: launch_filter<2>(ws, n_rows, n_cols, self_join); break;
+        case 3: launch_filter<3>(ws, n_rows, n_cols, self_join); break;
+        case 4: launch_filter<4>(ws, n_rows, n_cols, self_join); break;
+        case 5: launch_filter<5>(ws, n_rows, n_cols, self_join); break;
+        case 6: launch_filter<6>(ws, n_rows, n_cols, self_join); break;
+        case 7: launch_filter<7>(ws, n_rows, n_cols, self_join); break;
+        case 8: launch_filter<8>(ws, n_rows, n_cols, self_join); break;
+        case 9: launch_filter<9>(ws, n_rows, n_cols, self_join); break;
+        default:
+            set_error("MFMA mode supports d <= 138");
+            return RNNG_ERR_UNSUPPORTED;
+    }
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+struct Events {
+    hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int create() {
+        for (auto& x : e) RNNG_HIP_TRY(hipEventCreate(&x));
+        return RNNG_OK;
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    double ms(int a, int b) const {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e[a], e[b]) != hipSuccess) return 0.0;
+        return (double)t;
+    }
+};
+
+int edge_buffers(rnng_graph_s* g, int64_t cap) {
+    if (cap < 1) cap = 1;
+    RNNG_TRY(g->ij.reserve((size_t)cap * sizeof(int2)));
+    RNNG_TRY(g->w.reserve((size_t)cap * sizeof(float)));
+    g->cap = cap;
+    return RNNG_OK;
+}
+
+int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+              rnng_graph_s* g, rnng_stats* st) {
+    Workspace& ws = g_ws;
+    const int nx = (int)A->n, ny = (int)B->n, d = A->d;
+    const float T = sq_threshold_host(eps);
+    const int nbx = (nx + PB - 1) / PB, nby = (ny + PB - 1) / PB;
+    const long long nblocks = self_join ? (long long)nbx * (nbx + 1) / 2 : (long long)nbx * nby;
+    if (g->cap == 0) {
+        const int64_t guess = g_edge_hint > 0 ? g_edge_hint
+                                              : std::max<int64_t>(1 << 20, 32 * (int64_t)std::max(nx, ny));
+        RNNG_TRY(edge_buffers(g, guess));
+    }
+    Events ev;
+    RNNG_TRY(ev.create());
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
+        if (nblocks > 0) {
+            k_l2_plain<<<(unsigned)nblocks, 256, 0, stream()>>>(
+                A->X, nx, B->X, ny, d, T, self_join ? 1 : 0, nbx, nby, g->ij.as<int2>(),
+                g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>() + 1);
+            RNNG_HIP_TRY(hipGetLastError());
+        }
+        RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
+        unsigned long long cnt[8];
+        RNNG_HIP_TRY(hipMemcpyAsync(cnt, ws.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        const int64_t ne = (int64_t)cnt[1];
+        if (ne <= g->cap) {
+            g->n_edges = ne;
+            if (st) {
+                st->edges = ne;
+                st->filter_ms += ev.ms(0, 1);
+                st->total_ms += ev.ms(0, 1);
+            }
+            return RNNG_OK;
+        }
+        if (st) st->refills++;
+        RNNG_TRY(edge_buffers(g, ne + ne / 8 + 1024));
+    }
+    set_error("edge buffer refill did not converge");
+    return RNNG_ERR_HIP;
+}
+
+int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+             rnng_graph_s* g, rnng_stats* st) {
+    Workspace& ws = g_ws;
+    const int nx = (int)A->n, ny = (int)B->n, d = A->d;
+    const int KS = (d + 6 + 15) / 16;
+    if (KS > 9) {
+        set_error("MFMA mode supports d <= 138");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    const float T = sq_threshold_host(eps);
+    const float eps_band = (float)((double)eps * (1.0 + 1e-5));
+    float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
+    if (Tb < T) Tb = T;
+
+    const int n_rowtiles = ((nx + 255) / 256) * 8;      // multiple of 8 (row blocks)
+    const int n_coltiles = (ny + 31) / 32;
+    const int n_rowblocks = n_rowtiles / 8;
+    const int npad_x = n_rowtiles * 32;
+    const int npad_y = std::max(n_coltiles * 32, self_join ? npad_x : 0);
+    const size_t frag_bytes_x = (size_t)npad_x * KS * 16 * 2;
+    const size_t frag_bytes_y = (size_t)npad_y * KS * 16 * 2;
+
+    // For a self join the x-form and the y-form come from the same points.
+    RNNG_TRY(ws.xa.reserve(frag_bytes_x));
+    RNNG_TRY(ws.yb.reserve(frag_bytes_y));
+    RNNG_TRY(ws.colsum.reserve(sizeof(double) * d));
+    RNNG_TRY(ws.colmin.reserve(sizeof(unsigned) * d));
+    RNNG_TRY(ws.colmax.reserve(sizeof(unsigned) * d));
+    RNNG_TRY(ws.stat.reserve(sizeof(unsigned) * NSTAT));
+    RNNG_TRY(ws.range.reserve(sizeof(unsigned)));
+    RNNG_TRY(ws.mu.reserve(sizeof(float) * d));
+    RNNG_TRY(ws.params.reserve(sizeof(MfmaParams)));
+    RNNG_TRY(build_work(ws, n_rowblocks, n_coltiles, self_join ? 1 : 0));
+    if (ws.cand_cap == 0) {
+        const unsigned long long guess =
+            g_cand_hint > 0 ? (unsigned long long)g_cand_hint
+                            : std::max<unsigned long long>(1ull << 22,
+                                                           48ull * (unsigned long long)std::max(nx, ny));
+        RNNG_TRY(ws.cand.reserve(guess * sizeof(int2)));
+        ws.cand_cap = guess;
+    }
+    if (g->cap == 0) {
+        const int64_t guess = g_edge_hint > 0 ? g_edge_hint
+                                              : std::max<int64_t>(1 << 20, 32 * (int64_t)std::max(nx, ny));
+        RNNG_TRY(edge_buffers(g, guess));
+    }
+
+    Events ev;
+    RNNG_TRY(ev.create());
+    RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
+
+    // ---- preparation: centering, fragments, error terms, C0 ----
+    {
+        // colmin initialised to f2ord(+inf) = 0xff800000, colmax to f2ord(-inf) = 0x007fffff
+        k_fill_u64<<<blocks_for(d, 256), 256, 0, stream()>>>((unsigned long long*)ws.colsum.ptr, d, 0ull);
+        RNNG_TRY(launch_fill_u32(ws.colmin.as<unsigned>(), d, 0xff800000u));
+        RNNG_TRY(launch_fill_u32(ws.colmax.as<unsigned>(), d, 0x007fffffu));
+        RNNG_TRY(launch_fill_u32(ws.stat.as<unsigned>(), NSTAT, 0u));
+        RNNG_TRY(launch_fill_u32(ws.stat.as<unsigned>() + ST_INT, 1, 1u));
+        RNNG_TRY(launch_fill_u32(ws.stat.as<unsigned>() + ST_MAXDEV, 1, 0x80000000u));
+        RNNG_TRY(launch_fill_u32(ws.range.as<unsigned>(), 1, 0x80000000u));
+    }
+    const rnng_dataset_s* sets[2] = {A, B};
+    const int nsets = self_join ? 1 : 2;
+    int ntot = 0;
+    for (int q = 0; q < nsets; ++q) {
+        const rnng_dataset_s* S = sets[q];
+        if (S->n > 0)
+            k_colstats<<<blocks_for(S->n, 256), 256, 0, stream()>>>(
+                S->X, (int)S->n, d, ws.colsum.as<double>(), ws.colmin.as<unsigned>(),
+                ws.colmax.as<unsigned>(), ws.stat.as<unsigned>());
+        ntot += (int)S->n;
+    }
+    RNNG_HIP_TRY(hipGetLastError());
+    k_center<<<blocks_for(d, 64), 64, 0, stream()>>>(ws.colsum.as<double>(), ws.colmin.as<unsigned>(),
+                                                      ws.colmax.as<unsigned>(), ntot, d,
+                                                      ws.mu.as<float>(), ws.stat.as<unsigned>(),
+                                                      ws.range.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    if (self_join) {
+        // one pass writes both forms for every padded row of the join
+        const int npad = std::max(npad_x, npad_y);
+        k_prep_frags<<<blocks_for(npad, 256), 256, 0, stream()>>>(
+            A->X, nx, npad, d, KS, ws.mu.as<float>(), ws.xa.as<uint16_t>(), ws.yb.as<uint16_t>(),
+            ws.stat.as<unsigned>());
+    } else {
+        k_prep_frags<<<blocks_for(npad_x, 256), 256, 0, stream()>>>(
+            A->X, nx, npad_x, d, KS, ws.mu.as<float>(), ws.xa.as<uint16_t>(), nullptr,
+            ws.stat.as<unsigned>());
+        RNNG_HIP_TRY(hipGetLastError());
+        k_prep_frags<<<blocks_for(npad_y, 256), 256, 0, stream()>>>(
+            B->X, ny, npad_y, d, KS, ws.mu.as<float>(), nullptr, ws.yb.as<uint16_t>(),
+            ws.stat.as<unsigned>());
+    }
+    RNNG_HIP_TRY(hipGetLastError());
+    k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
+                                         ws.params.as<MfmaParams>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
+
+    const bool vec4 = (d % 4) == 0;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
+        RNNG_TRY(dispatch_filter(ws, KS, nx, ny, self_join ? 1 : 0));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+        const unsigned rgrid = 2048;
+        if (vec4)
+            k_l2_recheck<true><<<rgrid, 256, 0, stream()>>>(
+                A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, g->ij.as<int2>(),
+                g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>());
+        else
+            k_l2_recheck<false><<<rgrid, 256, 0, stream()>>>(
+                A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, g->ij.as<int2>(),
+                g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+        unsigned long long cnt[8];
+        RNNG_HIP_TRY(hipMemcpyAsync(cnt, ws.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        const bool cand_over = cnt[0] > ws.cand_cap;
+        const bool edge_over = (int64_t)cnt[1] > g->cap;
+        if (!cand_over && !edge_over) {
+            g->n_edges = (int64_t)cnt[1];
+            if (st) {
+                st->candidates = (int64_t)cnt[0];
+                st->edges = (int64_t)cnt[1];
+                st->band_pairs = (int64_t)cnt[2];
+                st->band_flips = (int64_t)cnt[3];
+                st->prep_ms += ev.ms(0, 1);
+                st->filter_ms += ev.ms(2, 3);
+                st->recheck_ms += ev.ms(3, 4);
+                st->total_ms += ev.ms(0, 1) + ev.ms(2, 4);
+            }
+            return RNNG_OK;
+        }
+        if (st) st->refills++;
+        if (cand_over) {
+            const unsigned long long nc = cnt[0] + cnt[0] / 8 + 1024;
+            RNNG_TRY(ws.cand.reserve(nc * sizeof(int2)));
+            ws.cand_cap = nc;
+        }
+        if (edge_over) RNNG_TRY(edge_buffers(g, (int64_t)cnt[1] + (int64_t)cnt[1] / 8 + 1024));
+    }
+    set_error("candidate/edge buffer refill did not converge");
+    return RNNG_ERR_HIP;
+}
+
+int check_dataset(const rnng_dataset_s* ds) {
+    if (!ds) {
+        set_error("null dataset");
+        return RNNG_ERR_ARG;
+    }
+    if (ds->n > 0x7fffffff - 4096) {
+        set_error("datasets are limited to 2^31 - 4096 points");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    return RNNG_OK;
+}
+
+int graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+                int32_t mode, rnng_graph_t* out, rnng_stats* st) {
+    if (!out) {
+        set_error("null output handle");
+        return RNNG_ERR_ARG;
+    }
+    *out = nullptr;
+    RNNG_TRY(ensure_init());
+    RNNG_TRY(check_dataset(A));
+    RNNG_TRY(check_dataset(B));
+    if (A->d != B->d || A->d <= 0) {
+        set_error("dimension mismatch or d <= 0");
+        return RNNG_ERR_ARG;
+    }
+    if (mode != RNNG_MODE_PLAIN && mode != RNNG_MODE_MFMA) {
+        set_error("unknown mode");
+        return RNNG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    RNNG_TRY(g_ws.counters.reserve(8 * sizeof(unsigned long long)));
+    rnng_graph_s* g = new rnng_graph_s();
+    if (st) {
+        const long long nx = A->n, ny = B->n;
+        st->pair_evals += self_join ? nx * (nx - 1) / 2 : nx * ny;
+    }
+    int rc;
+    if (A->n == 0 || B->n == 0) {
+        rc = edge_buffers(g, 0);
+        g->n_edges = 0;
+    } else if (mode == RNNG_MODE_PLAIN) {
+        rc = run_plain(A, B, self_join, eps, g, st);
+    } else {
+        rc = run_mfma(A, B, self_join, eps, g, st);
+    }
+    if (rc != RNNG_OK) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return RNNG_OK;
+}
+
+}  // namespace
+
+int debug_mfma_layout(int32_t* rows, int32_t* cols) {
+    RNNG_TRY(ensure_init());
+    DevBuf r, c;
+    RNNG_TRY(r.reserve(64 * 16 * sizeof(int)));
+    RNNG_TRY(c.reserve(64 * 16 * sizeof(int)));
+    k_mfma_layout<<<1, 64, 0, stream()>>>(r.as<int>(), c.as<int>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_HIP_TRY(hipMemcpyAsync(rows, r.ptr, 64 * 16 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipMemcpyAsync(cols, c.ptr, 64 * 16 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    r.release();
+    c.release();
+    return RNNG_OK;
+}
+
+}  // namespace rnng
+
+// ===========================================================================
+// C ABI (L2 path)
+// ===========================================================================
+using namespace rnng;
+
+extern "C" {
+
+float rnng_sq_threshold(float eps) { return sq_threshold_host(eps); }
+
+int rnng_dataset_create_f32(const float* points, int64_t n, int32_t d, rnng_dataset_t* out) {
+    if (!out || (n > 0 && !points) || n < 0 || d <= 0) {
+        set_error("rnng_dataset_create_f32: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    *out = nullptr;
+    RNNG_TRY(ensure_init());
+    rnng_dataset_s* ds = new rnng_dataset_s();
+    ds->n = n;
+    ds->d = d;
+    ds->owned = true;
+    const size_t bytes = (size_t)std::max<int64_t>(n, 1) * d * sizeof(float);
+    hipError_t e = hipMalloc(&ds->X, bytes);
+    if (e != hipSuccess) {
+        delete ds;
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return RNNG_ERR_HIP;
+    }
+    if (n > 0) {
+        e = hipMemcpy(ds->X, points, (size_t)n * d * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(ds->X);
+            delete ds;
+            set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
+            return RNNG_ERR_HIP;
+        }
+    }
+    *out = ds;
+    return RNNG_OK;
+}
+
+int rnng_dataset_wrap_device_f32(const float* dev_points, int64_t n, int32_t d, rnng_dataset_t* out) {
+    if (!out || (n > 0 && !dev_points) || n < 0 || d <= 0) {
+        set_error("rnng_dataset_wrap_device_f32: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    rnng_dataset_s* ds = new rnng_dataset_s();
+    ds->X = const_cast<float*>(dev_points);
+    ds->n = n;
+    ds->d = d;
+    ds->owned = false;
+    *out = ds;
+    return RNNG_OK;
+}
+
+int rnng_dataset_destroy(rnng_dataset_t ds) {
+    if (!ds) return RNNG_OK;
+    if (ds->owned && ds->X) (void)hipFree(ds->X);
+    delete ds;
+    return RNNG_OK;
+}
+
+int rnng_graph_build(rnng_dataset_t ds, float eps, int32_t mode, rnng_graph_t* out, rnng_stats* st) {
+    return graph_build(ds, ds, true, eps, mode, out, st);
+}
+
+int rnng_graph_build_bipartite(rnng_dataset_t a, rnng_dataset_t b, float eps, int32_t mode,
+                               rnng_graph_t* out, rnng_stats* st) {
+    return graph_build(a, b, false, eps, mode, out, st);
+}
+
+int rnng_graph_num_edges(rnng_graph_t g, int64_t* n_edges) {
+    if (!g || !n_edges) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    *n_edges = g->n_edges;
+    return RNNG_OK;
+}
+
+int rnng_graph_copy(rnng_graph_t g, int64_t* ij, float* w, int64_t cap, int32_t sorted) {
+    if (!g || cap < 0) {
+        set_error("null graph");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    const int64_t ne = std::min<int64_t>(g->n_edges, cap);
+    if (ne <= 0) return RNNG_OK;
+    std::vector<int2> hij((size_t)g->n_edges);
+    std::vector<float> hw((size_t)g->n_edges);
+    RNNG_HIP_TRY(hipMemcpy(hij.data(), g->ij.ptr, (size_t)g->n_edges * sizeof(int2), hipMemcpyDeviceToHost));
+    RNNG_HIP_TRY(hipMemcpy(hw.data(), g->w.ptr, (size_t)g->n_edges * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<int64_t> order((size_t)g->n_edges);
+    std::iota(order.begin(), order.end(), (int64_t)0);
+    if (sorted) {
+        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+            return hij[a].x != hij[b].x ? hij[a].x < hij[b].x : hij[a].y < hij[b].y;
+        });
+    }
+    for (int64_t e = 0; e < ne; ++e) {
+        const int2 p = hij[(size_t)order[e]];
+        if (ij) {
+            ij[2 * e] = p.x;
+            ij[2 * e + 1] = p.y;
+        }
+        if (w) w[e] = hw[(size_t)order[e]];
+    }
+    return RNNG_OK;
+}
+
+int rnng_graph_destroy(rnng_graph_t g) {
+    if (!g) return RNNG_OK;
+    g->ij.release();
+    g->w.release();
+    delete g;
+    return RNNG_OK;
+}
+
+static int one_shot(const float* a, int64_t na, const float* b, int64_t nb, int32_t d, float eps,
+                    int32_t mode, bool self_join, int64_t* ij, float* w, int64_t cap,
+                    int64_t* needed, rnng_stats* st) {
+    if (!needed || cap < 0 || (cap > 0 && !ij && !w)) {
+        set_error("bad output arguments");
+        return RNNG_ERR_ARG;
+    }
+    rnng_dataset_t A = nullptr, B = nullptr;
+    RNNG_TRY(rnng_dataset_create_f32(a, na, d, &A));
+    int rc = RNNG_OK;
+    if (!self_join) rc = rnng_dataset_create_f32(b, nb, d, &B);
+    rnng_graph_t g = nullptr;
+    if (rc == RNNG_OK) rc = graph_build(A, self_join ? A : B, self_join, eps, mode, &g, st);
+    if (rc == RNNG_OK) {
+        *needed = g->n_edges;
+        if (g->n_edges > cap) {
+            set_error("output capacity too small");
+            rc = RNNG_ERR_CAPACITY;
+        } else {
+            rc = rnng_graph_copy(g, ij, w, cap, 1);
+        }
+    }
+    rnng_graph_destroy(g);
+    rnng_dataset_destroy(A);
+    rnng_dataset_destroy(B);
+    return rc;
+}
+
+int rnng_l2_eps_pairs_f32(const float* points, int64_t n, int32_t d, float eps, int32_t mode,
+                          int64_t* ij, float* w, int64_t cap, int64_t* needed, rnng_stats* st) {
+    return one_shot(points, n, nullptr, 0, d, eps, mode, true, ij, w, cap, needed, st);
+}
+
+int rnng_l2_eps_bipartite_f32(const float* a, int64_t na, const float* b, int64_t nb, int32_t d,
+                              float eps, int32_t mode, int64_t* ij, float* w, int64_t cap,
+                              int64_t* needed, rnng_stats* st) {
+    return one_shot(a, na, b, nb, d, eps, mode, false, ij, w, cap, needed, st);
+}
+
+int rnng_set_initial_capacity(int64_t edges, int64_t candidates) {
+    if (edges < 0 || candidates < 0) {
+        set_error("capacities must be >= 0");
+        return RNNG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_edge_hint = edges;
+    g_cand_hint = candidates;
+    g_ws.cand.release();
+    g_ws.cand_cap = 0;
+    return RNNG_OK;
+}
+
+int rnng_debug_mfma_layout(int32_t* rows, int32_t* cols) {
+    if (!rows || !cols) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    return debug_mfma_layout(rows, cols);
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+"""GPU parity of the L2 epsilon-graph path: the HIP kernels (plain FP32 and
+BF16 MFMA filter + FP32 recheck), called through the C ABI, against the CPU
+oracle on the same seeded inputs.  Integer/index results are bit-exact: the
+sorted (i, j) lists are equal and the float weights are equal bit for bit."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def _l2():
+    from rnng_amd import l2
+    return l2
+
+
+def _check_same(ij, w, ij_ref, w_ref):
+    assert ij.shape == ij_ref.shape, (ij.shape, ij_ref.shape)
+    assert np.array_equal(ij, ij_ref)
+    assert w.tobytes() == w_ref.tobytes()
+
+
+def test_mfma_output_layout():
+    rows, cols = _l2().mfma_layout()
+    lane = np.arange(64)[:, None]
+    r = np.arange(16)[None, :]
+    assert np.array_equal(cols, np.broadcast_to(lane % 32, (64, 16)))
+    assert np.array_equal(rows, 8 * (r // 4) + 4 * (lane // 32) + r % 4)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_small_gmm_config_bit_exact(orc, mode):
+    from rnng_amd import synth
+    pts, eps = synth.points_gmm_small()
+    flush_print(f"gmm_small 10000x16 eps={eps} mode={mode}")
+    ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
+    ij, w = _l2().eps_pairs(pts, eps, mode)
+    _check_same(ij, w, ij_ref, w_ref)
+    assert ij.shape[0] > 50_000
+
+
+@pytest.mark.parametrize("n,d", [(1, 3), (2, 3), (33, 3), (1000, 16), (3001, 17), (2500, 37),
+                                 (2000, 96), (1800, 128), (700, 130)])
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_random_shapes_bit_exact(orc, n, d, mode):
+    rng = np.random.default_rng(n * 7 + d)
+    pts = (rng.standard_normal((n, d)) * 3 + 1).astype(F)
+    eps = F(np.sqrt(2 * d) * 3 * 0.8)
+    flush_print(f"random n={n} d={d} mode={mode}")
+    ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
+    ij, w = _l2().eps_pairs(pts, eps, mode)
+    _check_same(ij, w, ij_ref, w_ref)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_integer_data_and_band(orc, mode):
+    from rnng_amd import synth
+    pts, eps = synth.points_gmm_int(n=20_000)
+    flush_print(f"gmm_int 20000x128 eps={eps} mode={mode}")
+    ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
+    from rnng_amd import Stats
+    st = Stats()
+    ij, w = _l2().eps_pairs(pts, eps, mode, stats=st)
+    _check_same(ij, w, ij_ref, w_ref)
+    if mode == "mfma":
+        assert st.candidates >= st.edges
+        flush_print(f"stats {st.as_dict()}")
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_bipartite_bit_exact(orc, mode):
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((1500, 24)).astype(F)
+    B = rng.standard_normal((2300, 24)).astype(F)
+    eps = F(5.2)
+    flush_print(f"bipartite 1500x2300 d=24 mode={mode}")
+    ij_ref, w_ref = orc.l2_eps_pairs(A, eps, B)
+    ij, w = _l2().eps_bipartite(A, B, eps, mode)
+    _check_same(ij, w, ij_ref, w_ref)
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_edge_cases(orc, mode):
+    l2 = _l2()
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((300, 8)).astype(F)
+    X[7] = X[3]
+    X[250] = X[3]
+    flush_print(f"edge cases mode={mode}")
+    # eps = 0: only exact duplicates
+    ij, w = l2.eps_pairs(X, 0.0, mode)
+    assert [tuple(p) for p in ij] == [(3, 7), (3, 250), (7, 250)]
+    assert np.all(w == 0)
+    # negative eps: nothing
+    assert l2.eps_pairs(X, -1.0, mode)[0].shape[0] == 0
+    # huge eps: every pair
+    ij, w = l2.eps_pairs(X[:100], 1e6, mode)
+    assert ij.shape[0] == 100 * 99 // 2
+    # empty and single-point sets
+    assert l2.eps_pairs(np.zeros((0, 8), F), 1.0, mode)[0].shape[0] == 0
+    assert l2.eps_pairs(X[:1], 1.0, mode)[0].shape[0] == 0
+
+
+@pytest.mark.parametrize("mode", ["plain", "mfma"])
+def test_count_then_fill_refill(orc, mode):
+    from rnng_amd import Stats, _lib
+    l2 = _l2()
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((2000, 16)).astype(F)
+    eps = F(4.0)
+    ij_ref, w_ref = orc.l2_eps_pairs(X, eps)
+    assert ij_ref.shape[0] > 5000
+    flush_print(f"refill mode={mode}")
+    _lib.check(_lib.lib().rnng_set_initial_capacity(100, 100))
+    try:
+        st = Stats()
+        ij, w = l2.eps_pairs(X, eps, mode, stats=st)
+        assert st.refills >= 1
+        _check_same(ij, w, ij_ref, w_ref)
+    finally:
+        _lib.check(_lib.lib().rnng_set_initial_capacity(0, 0))
+    # the one-shot host entry point reports the size when cap is too small
+    ij2, w2 = l2.eps_pairs_oneshot(X, eps, mode)
+    _check_same(ij2, w2, ij_ref, w_ref)
+    L = _lib.lib()
+    needed = C.c_int64()
+    buf = np.empty((10, 2), np.int64)
+    rc = L.rnng_l2_eps_pairs_f32(X.ctypes.data_as(C.c_void_p), 2000, 16, eps, l2._mode(mode),
+                                 buf.ctypes.data_as(C.c_void_p), None, 10, C.byref(needed), None)
+    assert rc == _lib.RNNG_ERR_CAPACITY and needed.value == ij_ref.shape[0]
+
+
+def test_full_size_int_config_mfma_equals_plain():
+    """1,000,000 x 128 integer workload: the MFMA path's edge set equals the
+    plain FP32 kernel's and edges are canonical (i < j, sorted, unique)."""
+    from rnng_amd import Stats, synth
+    l2 = _l2()
+    pts, eps = synth.points_gmm_int()
+    flush_print(f"gmm_int 1Mx128 eps={eps}")
+    ds = l2.Dataset(pts)
+    s1, s2 = Stats(), Stats()
+    g1 = ds.graph(eps, "mfma", s1)
+    g2 = ds.graph(eps, "plain", s2)
+    ij1, w1 = g1.edges()
+    ij2, w2 = g2.edges()
+    flush_print(f"mfma {s1.as_dict()}\nplain {s2.as_dict()}")
+    _check_same(ij1, w1, ij2, w2)
+    assert np.all(ij1[:, 0] < ij1[:, 1])
+    key = ij1[:, 0] * (1 << 32) + ij1[:, 1]
+    assert np.all(np.diff(key) > 0)
+    assert 5_000_000 < ij1.shape[0] < 50_000_000
+    assert s1.candidates >= s1.edges
+    # spot-check a sample of edges with the oracle's distance
+    import oracle
+    rng = np.random.default_rng(0)
+    for e in rng.choice(ij1.shape[0], 200, replace=False):
+        i, j = ij1[e]
+        assert oracle.l2(pts[i], pts[j]) == w1[e]

@@ -1,0 +1,85 @@
+"""CPU-side checks of the product library: it loads, it exports every symbol
+the C header declares, host-only helpers agree with the oracle, error paths
+return codes instead of crashing, and the seeded generators are deterministic.
+No GPU compute is called here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if name.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", name)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            syms |= set(re.findall(r"\b(rnng_[a-z0-9_]+)\s*\(", txt))
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    from rnng_amd import _lib
+    L = _lib.lib()
+    syms = _header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+    # the ctypes table covers the header exactly
+    assert set(_lib.declared_symbols()) == syms
+
+
+def test_library_has_gfx950_code_object():
+    from rnng_amd import _lib
+    blob = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob
+
+
+def test_sq_threshold_matches_oracle(orc):
+    from rnng_amd import l2
+    rng = np.random.default_rng(0)
+    for e in list(rng.uniform(0, 300, 300)) + [0.0, 0.08, 1.5, 256.2889, -1.0, 1e19]:
+        assert l2.sq_threshold(e) == orc.sq_threshold(e)
+
+
+def test_no_device_is_an_error_code_not_a_crash():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present")
+    from rnng_amd import _lib
+    L = _lib.lib()
+    x = np.zeros((4, 3), np.float32)
+    h = C.c_void_p()
+    rc = L.rnng_dataset_create_f32(x.ctypes.data_as(C.c_void_p), 4, 3, C.byref(h))
+    assert rc == _lib.RNNG_ERR_NODEV
+    assert L.rnng_last_error()
+
+
+def test_bad_arguments_are_rejected():
+    from rnng_amd import _lib
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.rnng_dataset_create_f32(None, 4, 3, C.byref(h)) == _lib.RNNG_ERR_ARG
+    assert L.rnng_dataset_create_f32(None, 0, 0, C.byref(h)) == _lib.RNNG_ERR_ARG
+    assert L.rnng_graph_num_edges(None, None) == _lib.RNNG_ERR_ARG
+    assert L.rnng_set_initial_capacity(-1, 0) == _lib.RNNG_ERR_ARG
+    assert L.rnng_set_initial_capacity(0, 0) == _lib.RNNG_OK
+
+
+def test_generators_are_deterministic():
+    from rnng_amd import synth
+    a1, e1 = synth.points_gmm_small()
+    a2, e2 = synth.points_gmm_small()
+    assert a1.tobytes() == a2.tobytes() and e1 == e2
+    assert a1.shape == (10_000, 16) and a1.dtype == np.float32
+    p, e = synth.points_gmm_int(n=5000)
+    assert p.min() >= 0 and p.max() <= 255 and np.all(p == np.rint(p))
+    atoms, offs = synth.dna_mutants(n_seeds=20, per_seed=10)
+    assert offs.shape == (201,) and set(np.unique(atoms).tobytes()) <= set(b"ACGT")
+    lens = np.diff(offs)
+    assert lens.min() >= 80 and lens.max() <= 170
+    r, eps = synth.points_rips(n=1000)
+    assert r.shape == (1000, 3) and eps == np.float32(0.08)
