@@ -124,6 +124,52 @@ int rnng_l2_eps_bipartite_f32(const float *a, int64_t na, const float *b, int64_
  * rnng_stats.refills.  The learned cost model uses this to size buffers. */
 int rnng_set_initial_capacity(int64_t edges, int64_t candidates);
 
+/* ------------------------------------------------------------------------ */
+/* Edit-distance epsilon graph                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rnng_strings_s *rnng_strings_t;   /* device-resident string set */
+
+/* Upload n strings given as a CSR: string i is atoms[offsets[i] ..
+ * offsets[i+1]) (the layout PointContainer<char> keeps after read_seqs,
+ * include/point.hpp:170-205).  Any byte values are allowed. */
+int rnng_strings_create(const char *atoms, const int64_t *offsets, int64_t n,
+                        rnng_strings_t *out);
+
+int rnng_strings_destroy(rnng_strings_t s);
+
+/* All pairs i < j whose Levenshtein distance (unit costs, the reference's
+ * EditDistance functor, drivers/systolic.cpp:467-495) is <= eps; the weight
+ * is the distance as a float.  eps up to 32 (RNNG_ERR_UNSUPPORTED above). */
+int rnng_edit_graph_build(rnng_strings_t s, float eps, rnng_graph_t *out, rnng_stats *stats);
+
+/* One-shot host form with the count-then-fill contract of rnng_l2_eps_pairs_f32. */
+int rnng_edit_eps_pairs(const char *atoms, const int64_t *offsets, int64_t n, float eps,
+                        int64_t *ij, float *w, int64_t cap, int64_t *needed, rnng_stats *stats);
+
+/* ------------------------------------------------------------------------ */
+/* Vietoris-Rips filtration                                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rnng_filtration_s *rnng_filtration_t;   /* device-resident simplices */
+
+/* Rips filtration of the L2 epsilon graph of ds up to dimension maxdim
+ * (0, 1 or 2): every vertex (value 0), edge (value = float L2 distance) and
+ * triangle (value = max of its edges), with the reference's simplex ids
+ * (include/simplex.hpp:5-19), sorted by (value, id) and unique by id -- the
+ * merged output of the rips driver (include/point.hpp:1134-1267,
+ * include/simplex.hpp:130-196).  Triangles need n <= 1,800,000 (60-bit ids). */
+int rnng_rips_build(rnng_dataset_t ds, float eps, int32_t maxdim, rnng_filtration_t *out,
+                    rnng_stats *stats);
+
+/* Total number of simplices, and per dimension (per_dim[0..2], may be NULL). */
+int rnng_filtration_size(rnng_filtration_t f, int64_t *n, int64_t *per_dim);
+
+/* Copy the first cap simplices in filtration order (either array may be NULL). */
+int rnng_filtration_copy(rnng_filtration_t f, float *value, int64_t *id, int64_t cap);
+
+int rnng_filtration_destroy(rnng_filtration_t f);
+
 /* Largest float T with sqrtf(T) <= eps (-1 when no value qualifies).  The
  * kernels compare squared distances against T; sqrtf is monotone, so
  * sqrtf(v) <= eps  <=>  v <= T. */
@@ -137,6 +183,10 @@ float rnng_sq_threshold(float eps);
  * indices and reports, for each (lane, accumulator register), the output
  * row and column it holds: rows[lane*16 + r], cols[lane*16 + r]. */
 int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
+
+/* out[i] = the device square root the kernels use for edge weights, for
+ * checking it against the host's correctly rounded sqrtf. */
+int rnng_debug_sqrt(const float *in, float *out, int64_t n);
 
 #ifdef __cplusplus
 }

@@ -72,8 +72,18 @@ _SIGS = {
                                             C.c_int32, _vp, _vp, C.c_int64, _i64p,
                                             C.POINTER(Stats)]),
     "rnng_set_initial_capacity": (C.c_int, [C.c_int64, C.c_int64]),
+    "rnng_rips_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp), C.POINTER(Stats)]),
+    "rnng_filtration_size": (C.c_int, [_vp, _i64p, _vp]),
+    "rnng_filtration_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "rnng_filtration_destroy": (C.c_int, [_vp]),
+    "rnng_strings_create": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(_vp)]),
+    "rnng_strings_destroy": (C.c_int, [_vp]),
+    "rnng_edit_graph_build": (C.c_int, [_vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),
+    "rnng_edit_eps_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.c_float, _vp, _vp, C.c_int64, _i64p,
+                                      C.POINTER(Stats)]),
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
+    "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
 }
 
 _lib = None

@@ -36,13 +36,22 @@ void set_error(const std::string& msg);
 hipStream_t stream();
 int ensure_init();
 
-// Device buffer that grows on demand and is released by the owner.
+// Device buffer that grows on demand and is released by the owner.  Objects
+// with static storage use it as is (the process exit reclaims the memory);
+// buffers local to a call use LocalBuf, which frees on scope exit.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
     int reserve(size_t need);   // keeps contents only if no growth is needed
     void release();
     template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct LocalBuf : DevBuf {
+    LocalBuf() = default;
+    LocalBuf(const LocalBuf&) = delete;
+    LocalBuf& operator=(const LocalBuf&) = delete;
+    ~LocalBuf() { release(); }
 };
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,58 @@
+// Internal types shared by the library's translation units.
+#pragma once
+
+#include "common.hpp"
+
+struct rnng_dataset_s {
+    float* X = nullptr;       // n*d floats in HBM, row-major
+    int64_t n = 0;
+    int32_t d = 0;
+    bool owned = false;
+};
+
+// Edge list in HBM: (i, j) int32 pairs and float weights.
+struct rnng_graph_s {
+    rnng::DevBuf ij;          // int2 (i, j)
+    rnng::DevBuf w;           // float distance
+    int64_t n_edges = 0;
+    int64_t cap = 0;
+};
+
+namespace rnng {
+
+extern int64_t g_edge_hint;   // initial edge-buffer size of new graphs (0: default)
+int edge_buffers(rnng_graph_s* g, int64_t cap);
+float sq_threshold_host(float eps);
+
+// HIP events on the library stream for stage timing.
+struct Events {
+    hipEvent_t e[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int create() {
+        for (auto& x : e) RNNG_HIP_TRY(hipEventCreate(&x));
+        return RNNG_OK;
+    }
+    ~Events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    double ms(int a, int b) const {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e[a], e[b]) != hipSuccess) return 0.0;
+        return (double)t;
+    }
+};
+
+inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Device sorts (sort.hip).  All run on the library stream and return after
+// the stream has drained.
+int sort_pairs_u64_f32(uint64_t* keys, float* vals, int64_t n);
+int sort_pairs_u64_i64(uint64_t* keys, int64_t* vals, int64_t n);
+int sort_by_value_then_id(float* value, int64_t* id, int64_t n);
+int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n);
+
+// Builds the L2 epsilon graph of a self join (l2_eps.hip).
+int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+                   int32_t mode, rnng_graph_s* g, rnng_stats* st);
+
+}  // namespace rnng

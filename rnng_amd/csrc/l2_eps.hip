@@ -27,24 +27,22 @@
 #include <numeric>
 #include <vector>
 
+#include "internal.hpp"
+
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-struct rnng_dataset_s {
-    float* X = nullptr;       // n*d floats in HBM, row-major
-    int64_t n = 0;
-    int32_t d = 0;
-    bool owned = false;
-};
-
-struct rnng_graph_s {
-    rnng::DevBuf ij;          // int2 (i, j)
-    rnng::DevBuf w;           // float distance
-    int64_t n_edges = 0;
-    int64_t cap = 0;
-};
-
 namespace rnng {
+
+int64_t g_edge_hint = 0;   // initial edge-buffer size of new graphs (0: default)
+
+int edge_buffers(rnng_graph_s* g, int64_t cap) {
+    if (cap < 1) cap = 1;
+    RNNG_TRY(g->ij.reserve((size_t)cap * sizeof(int2)));
+    RNNG_TRY(g->w.reserve((size_t)cap * sizeof(float)));
+    g->cap = cap;
+    return RNNG_OK;
+}
 
 float sq_threshold_host(float eps) {
     if (!(eps >= 0.0f)) return -1.0f;
@@ -139,8 +137,9 @@ __global__ __launch_bounds__(256) void k_l2_plain(
             }
         }
         __syncthreads();
+        const int kend = min(PK, d - k0);   // staged dims beyond d are zero: skip them
 #pragma unroll 4
-        for (int kk = 0; kk < PK; ++kk) {
+        for (int kk = 0; kk < kend; ++kk) {
             const float4 a0 = *(const float4*)&As[kk][ty * 4];
             const float4 a1 = *(const float4*)&As[kk][64 + ty * 4];
             const float4 b0 = *(const float4*)&Bs[kk][tx * 4];
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(256) void k_l2_plain(
             if (ok) {
                 if (pos < cap) {
                     out_ij[pos] = make_int2(i, j);
-                    out_w[pos] = __fsqrt_rn(acc[r][c]);
+                    out_w[pos] = sqrtf(acc[r][c]);
                 }
                 ++pos;
             }
@@ -534,7 +533,7 @@ __global__ __launch_bounds__(256) void k_l2_recheck(
                 }
             }
             edge = acc <= T;
-            w = __fsqrt_rn(acc);
+            w = sqrtf(acc);
             if (fabs((double)w - (double)eps) <= band) {
                 atomicAdd(&counters[2], 1ull);
                 double s = 0.0;
@@ -565,6 +564,25 @@ __global__ __launch_bounds__(256) void k_l2_recheck(
             }
         }
     }
+}
+
+__global__ void k_pack_keys(const int2* __restrict__ ij, uint64_t* __restrict__ keys, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) keys[e] = ((uint64_t)(uint32_t)ij[e].x << 32) | (uint32_t)ij[e].y;
+}
+
+__global__ void k_unpack_keys(const uint64_t* __restrict__ keys, int2* __restrict__ ij, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) ij[e] = make_int2((int)(keys[e] >> 32), (int)(keys[e] & 0xffffffffu));
+}
+
+// sqrtf here is __builtin_sqrtf, which gfx950 code generation expands into
+// v_sqrt_f32 plus an FMA residual correction: the correctly rounded square
+// root the reference gets from std::sqrt(float).  (__fsqrt_rn lowers to a bare
+// scaled v_sqrt_f32, which can be off by one ulp.)
+__global__ void k_debug_sqrt(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = sqrtf(in[i]);
 }
 
 // ------------------------------------------------------------------------
@@ -610,10 +628,8 @@ struct Workspace {
 
 std::mutex g_ws_mu;
 Workspace g_ws;
-int64_t g_edge_hint = 0;   // initial edge-buffer size of new graphs (0: default)
 int64_t g_cand_hint = 0;   // initial candidate-buffer size (0: default)
 
-inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v) {
     k_fill_u64<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
@@ -680,31 +696,6 @@ int dispatch_filter(Workspace& ws, int KS, int n_rows, int n_cols, int self_join
             return RNNG_ERR_UNSUPPORTED;
     }
     RNNG_HIP_TRY(hipGetLastError());
-    return RNNG_OK;
-}
-
-struct Events {
-    hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int create() {
-        for (auto& x : e) RNNG_HIP_TRY(hipEventCreate(&x));
-        return RNNG_OK;
-    }
-    ~Events() {
-        for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
-    }
-    double ms(int a, int b) const {
-        float t = 0.f;
-        if (hipEventElapsedTime(&t, e[a], e[b]) != hipSuccess) return 0.0;
-        return (double)t;
-    }
-};
-
-int edge_buffers(rnng_graph_s* g, int64_t cap) {
-    if (cap < 1) cap = 1;
-    RNNG_TRY(g->ij.reserve((size_t)cap * sizeof(int2)));
-    RNNG_TRY(g->w.reserve((size_t)cap * sizeof(float)));
-    g->cap = cap;
     return RNNG_OK;
 }
 
@@ -912,13 +903,10 @@ int check_dataset(const rnng_dataset_s* ds) {
     return RNNG_OK;
 }
 
-int graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
-                int32_t mode, rnng_graph_t* out, rnng_stats* st) {
-    if (!out) {
-        set_error("null output handle");
-        return RNNG_ERR_ARG;
-    }
-    *out = nullptr;
+}  // namespace
+
+int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+                   int32_t mode, rnng_graph_s* g, rnng_stats* st) {
     RNNG_TRY(ensure_init());
     RNNG_TRY(check_dataset(A));
     RNNG_TRY(check_dataset(B));
@@ -932,21 +920,32 @@ int graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join
     }
     std::lock_guard<std::mutex> lk(g_ws_mu);
     RNNG_TRY(g_ws.counters.reserve(8 * sizeof(unsigned long long)));
-    rnng_graph_s* g = new rnng_graph_s();
     if (st) {
         const long long nx = A->n, ny = B->n;
         st->pair_evals += self_join ? nx * (nx - 1) / 2 : nx * ny;
     }
-    int rc;
     if (A->n == 0 || B->n == 0) {
-        rc = edge_buffers(g, 0);
+        RNNG_TRY(edge_buffers(g, 1));
         g->n_edges = 0;
-    } else if (mode == RNNG_MODE_PLAIN) {
-        rc = run_plain(A, B, self_join, eps, g, st);
-    } else {
-        rc = run_mfma(A, B, self_join, eps, g, st);
+        return RNNG_OK;
     }
+    if (mode == RNNG_MODE_PLAIN) return run_plain(A, B, self_join, eps, g, st);
+    return run_mfma(A, B, self_join, eps, g, st);
+}
+
+namespace {
+int graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
+                int32_t mode, rnng_graph_t* out, rnng_stats* st) {
+    if (!out) {
+        set_error("null output handle");
+        return RNNG_ERR_ARG;
+    }
+    *out = nullptr;
+    rnng_graph_s* g = new rnng_graph_s();
+    const int rc = l2_graph_build(A, B, self_join, eps, mode, g, st);
     if (rc != RNNG_OK) {
+        g->ij.release();
+        g->w.release();
         delete g;
         return rc;
     }
@@ -958,7 +957,7 @@ int graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join
 
 int debug_mfma_layout(int32_t* rows, int32_t* cols) {
     RNNG_TRY(ensure_init());
-    DevBuf r, c;
+    LocalBuf r, c;
     RNNG_TRY(r.reserve(64 * 16 * sizeof(int)));
     RNNG_TRY(c.reserve(64 * 16 * sizeof(int)));
     k_mfma_layout<<<1, 64, 0, stream()>>>(r.as<int>(), c.as<int>());
@@ -1061,25 +1060,26 @@ int rnng_graph_copy(rnng_graph_t g, int64_t* ij, float* w, int64_t cap, int32_t 
     RNNG_TRY(ensure_init());
     const int64_t ne = std::min<int64_t>(g->n_edges, cap);
     if (ne <= 0) return RNNG_OK;
-    std::vector<int2> hij((size_t)g->n_edges);
-    std::vector<float> hw((size_t)g->n_edges);
-    RNNG_HIP_TRY(hipMemcpy(hij.data(), g->ij.ptr, (size_t)g->n_edges * sizeof(int2), hipMemcpyDeviceToHost));
-    RNNG_HIP_TRY(hipMemcpy(hw.data(), g->w.ptr, (size_t)g->n_edges * sizeof(float), hipMemcpyDeviceToHost));
-    std::vector<int64_t> order((size_t)g->n_edges);
-    std::iota(order.begin(), order.end(), (int64_t)0);
+    const int64_t E = g->n_edges;
     if (sorted) {
-        std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-            return hij[a].x != hij[b].x ? hij[a].x < hij[b].x : hij[a].y < hij[b].y;
-        });
+        // sort by (i, j) on the device: key = i << 32 | j
+        LocalBuf keys;
+        RNNG_TRY(keys.reserve((size_t)E * sizeof(uint64_t)));
+        k_pack_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(g->ij.as<int2>(), keys.as<uint64_t>(), E);
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(sort_pairs_u64_f32(keys.as<uint64_t>(), g->w.as<float>(), E));
+        k_unpack_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), g->ij.as<int2>(), E);
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     }
-    for (int64_t e = 0; e < ne; ++e) {
-        const int2 p = hij[(size_t)order[e]];
-        if (ij) {
-            ij[2 * e] = p.x;
-            ij[2 * e + 1] = p.y;
+    std::vector<int2> hij((size_t)ne);
+    RNNG_HIP_TRY(hipMemcpy(hij.data(), g->ij.ptr, (size_t)ne * sizeof(int2), hipMemcpyDeviceToHost));
+    if (w) RNNG_HIP_TRY(hipMemcpy(w, g->w.ptr, (size_t)ne * sizeof(float), hipMemcpyDeviceToHost));
+    if (ij)
+        for (int64_t e = 0; e < ne; ++e) {
+            ij[2 * e] = hij[(size_t)e].x;
+            ij[2 * e + 1] = hij[(size_t)e].y;
         }
-        if (w) w[e] = hw[(size_t)order[e]];
-    }
     return RNNG_OK;
 }
 
@@ -1128,6 +1128,24 @@ int rnng_l2_eps_bipartite_f32(const float* a, int64_t na, const float* b, int64_
                               float eps, int32_t mode, int64_t* ij, float* w, int64_t cap,
                               int64_t* needed, rnng_stats* st) {
     return one_shot(a, na, b, nb, d, eps, mode, false, ij, w, cap, needed, st);
+}
+
+int rnng_debug_sqrt(const float* in, float* out, int64_t n) {
+    if (n < 0 || (n > 0 && (!in || !out))) {
+        set_error("bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    if (n == 0) return RNNG_OK;
+    LocalBuf a, b;
+    RNNG_TRY(a.reserve((size_t)n * 4));
+    RNNG_TRY(b.reserve((size_t)n * 4));
+    RNNG_HIP_TRY(hipMemcpy(a.ptr, in, (size_t)n * 4, hipMemcpyHostToDevice));
+    k_debug_sqrt<<<blocks_for(n, 256), 256, 0, stream()>>>(a.as<float>(), b.as<float>(), n);
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    RNNG_HIP_TRY(hipMemcpy(out, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RNNG_OK;
 }
 
 int rnng_set_initial_capacity(int64_t edges, int64_t candidates) {

@@ -166,3 +166,12 @@ def mfma_layout():
     check(_lib.lib().rnng_debug_mfma_layout(rows.ctypes.data_as(C.c_void_p),
                                             cols.ctypes.data_as(C.c_void_p)))
     return rows, cols
+
+
+def device_sqrt(x) -> np.ndarray:
+    """The device's float square root (diagnostic)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    check(_lib.lib().rnng_debug_sqrt(x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+                                     x.size))
+    return out

@@ -1,0 +1,88 @@
+"""GPU parity of the edit-distance epsilon graph against the CPU oracle's
+reference-order Levenshtein (bit-exact edge lists and distances)."""
+import numpy as np
+import pytest
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def _edit():
+    from rnng_amd import edit
+    return edit
+
+
+@pytest.mark.parametrize("eps", [0, 1, 3, 8, 8.5, 12, 20])
+def test_dna_mutants_subset_bit_exact(orc, eps):
+    from rnng_amd import synth
+    atoms, offs = synth.dna_mutants(n_seeds=30, per_seed=40)
+    flush_print(f"dna 1200 strings eps={eps}")
+    ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, eps)
+    ij, d = _edit().eps_pairs(atoms, offs, eps)
+    assert np.array_equal(ij, ij_ref)
+    assert np.array_equal(d, d_ref)
+    if eps >= 8:
+        assert ij.shape[0] > 1000
+
+
+def test_random_alphabet_and_lengths(orc):
+    rng = np.random.default_rng(3)
+    strs = []
+    for _ in range(400):
+        L = int(rng.integers(0, 60))
+        strs.append(bytes(rng.choice(list(b"abcdxyz"), L)))
+    # near duplicates with edits
+    for s in list(strs[:100]):
+        t = bytearray(s)
+        for _ in range(rng.integers(0, 5)):
+            if t and rng.random() < 0.5:
+                del t[rng.integers(0, len(t))]
+            else:
+                t.insert(int(rng.integers(0, len(t) + 1)), int(rng.choice(list(b"abcdxyz"))))
+        strs.append(bytes(t))
+    atoms, offs = _edit().as_csr(strs)
+    for eps in (2, 6, 15, 32):
+        flush_print(f"random alphabet eps={eps}")
+        ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, eps)
+        ij, d = _edit().eps_pairs(atoms, offs, eps)
+        assert np.array_equal(ij, ij_ref)
+        assert np.array_equal(d, d_ref)
+
+
+def test_edge_cases(orc):
+    edit = _edit()
+    atoms, offs = edit.as_csr([b"", b"", b"A", b"AC", b"ACGT" * 40, b"ACGT" * 40])
+    ij, d = edit.eps_pairs(atoms, offs, 1)
+    assert [tuple(p) for p in ij] == [(0, 1), (0, 2), (1, 2), (2, 3), (4, 5)]
+    assert list(d) == [0, 1, 1, 1, 0]
+    assert edit.eps_pairs(atoms, offs, -1)[0].shape[0] == 0
+    a1, o1 = edit.as_csr([b"ACGT"])
+    assert edit.eps_pairs(a1, o1, 5)[0].shape[0] == 0
+    from rnng_amd import RnngError
+    with pytest.raises(RnngError):
+        edit.eps_pairs(atoms, offs, 40)
+
+
+def test_full_size_dna_properties():
+    """200,000 strings, eps = 8: every reported edge is verified by the
+    oracle's Levenshtein on a sample, the list is canonical, and most edges
+    join mutants of the same seed."""
+    import oracle
+    from rnng_amd import Stats, synth
+    atoms, offs = synth.dna_mutants()
+    st = Stats()
+    flush_print("dna 200000 strings eps=8")
+    ij, d = _edit().eps_pairs(atoms, offs, 8, stats=st)
+    flush_print(f"stats {st.as_dict()}")
+    assert ij.shape[0] > 1_000_000
+    assert np.all(ij[:, 0] < ij[:, 1])
+    key = ij[:, 0] * (1 << 32) + ij[:, 1]
+    assert np.all(np.diff(key) > 0)
+    assert d.min() >= 0 and d.max() <= 8
+    rng = np.random.default_rng(0)
+    for e in rng.choice(ij.shape[0], 300, replace=False):
+        i, j = ij[e]
+        s = atoms[offs[i]:offs[i + 1]].tobytes()
+        t = atoms[offs[j]:offs[j + 1]].tobytes()
+        assert oracle.edit_distance(s, t) == d[e]

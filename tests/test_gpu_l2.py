@@ -32,6 +32,19 @@ def test_mfma_output_layout():
     assert np.array_equal(rows, 8 * (r // 4) + 4 * (lane // 32) + r % 4)
 
 
+def test_device_sqrt_is_correctly_rounded():
+    """The edge weight is sqrtf of the FP32 squared distance; the device root
+    must be the correctly rounded one (numpy's float32 sqrt is IEEE)."""
+    rng = np.random.default_rng(0)
+    bits = rng.integers(0, 0x7f800000, size=4_000_000, dtype=np.uint32)
+    x = bits.view(np.float32)
+    dense = np.arange(65000.0, 66500.0, 1.0 / 64, dtype=np.float32)   # squared radii near 256
+    small = np.nextafter(np.float32(0), np.float32(1)) * np.arange(1, 5000, dtype=np.float32)
+    x = np.concatenate([x, dense, small, np.float32([0.0, 1.0, 2.0, 2.25, 16777216.0])])
+    got = _l2().device_sqrt(x)
+    assert np.array_equal(got.view(np.uint32), np.sqrt(x).view(np.uint32))
+
+
 @pytest.mark.parametrize("mode", ["plain", "mfma"])
 def test_small_gmm_config_bit_exact(orc, mode):
     from rnng_amd import synth
