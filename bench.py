@@ -135,9 +135,10 @@ def main():
         log(f"plain: {tp * 1e3:.1f} ms, {pe} edges, stats {sp.as_dict()}")
         out["plain"] = {"ms": tp * 1e3, "pair_evals_per_s": P / tp, "edges_per_s": pe / tp,
                         "edges_equal_count": pe == ne,
-                        # 3 non-fused VALU ops per dim per pair against the
-                        # non-packed FP32 VALU rate (256 CU x 4 SIMD x 16 lanes x 2.4 GHz)
-                        "valu_frac": (3.0 * d * P / (sp.filter_ms / 1e3)) / 39.3e12}
+                        # 3 non-fused VALU ops per dim per pair against the FP32
+                        # VALU op rate (256 CU x 128 lanes/clk x 2.4 GHz = 7.86e13/s;
+                        # the 157.3 TFLOP/s peak counts an FMA as 2 flops)
+                        "valu_frac": (3.0 * d * P / (sp.filter_ms / 1e3)) / 7.86e13}
         out["speedup_vs_plain"] = tp / (dt / K)
 
     if not args.no_cpu:

@@ -100,20 +100,23 @@ __device__ __forceinline__ void decode_upper(long long t, int nb, int& bi, int& 
     bj = (int)(b + (t - start(b)));
 }
 
+// block_base: a dispatch's grid is limited to 2^32 work-items, so large
+// launches are split into chunks of blocks that each start at block_base.
 __global__ __launch_bounds__(256) void k_l2_plain(
     const float* __restrict__ X, int nx, const float* __restrict__ Y, int ny, int d, float T,
-    int self_join, int nbx, int nby, int2* __restrict__ out_ij, float* __restrict__ out_w,
-    unsigned long long cap, unsigned long long* __restrict__ counter) {
+    int self_join, int nbx, int nby, long long block_base, int2* __restrict__ out_ij,
+    float* __restrict__ out_w, unsigned long long cap, unsigned long long* __restrict__ counter) {
 #pragma clang fp contract(off)
     __shared__ float As[PK][PB + 4];
     __shared__ float Bs[PK][PB + 4];
 
     int bi, bj;
+    const long long blk = block_base + (long long)blockIdx.x;
     if (self_join) {
-        decode_upper((long long)blockIdx.x, nbx, bi, bj);
+        decode_upper(blk, nbx, bi, bj);
     } else {
-        bi = blockIdx.x / nby;
-        bj = blockIdx.x % nby;
+        bi = (int)(blk / nby);
+        bj = (int)(blk % nby);
     }
     const int row0 = bi * PB, col0 = bj * PB;
     const int tid = threadIdx.x;
@@ -716,9 +719,12 @@ int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, 
     for (int attempt = 0; attempt < 2; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
-        if (nblocks > 0) {
-            k_l2_plain<<<(unsigned)nblocks, 256, 0, stream()>>>(
-                A->X, nx, B->X, ny, d, T, self_join ? 1 : 0, nbx, nby, g->ij.as<int2>(),
+        // at most 2^23 blocks (2^31 work-items) per dispatch
+        const long long chunk = 1ll << 23;
+        for (long long b0 = 0; b0 < nblocks; b0 += chunk) {
+            const long long nb = std::min(chunk, nblocks - b0);
+            k_l2_plain<<<(unsigned)nb, 256, 0, stream()>>>(
+                A->X, nx, B->X, ny, d, T, self_join ? 1 : 0, nbx, nby, b0, g->ij.as<int2>(),
                 g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>() + 1);
             RNNG_HIP_TRY(hipGetLastError());
         }
@@ -776,13 +782,15 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     RNNG_TRY(ws.mu.reserve(sizeof(float) * d));
     RNNG_TRY(ws.params.reserve(sizeof(MfmaParams)));
     RNNG_TRY(build_work(ws, n_rowblocks, n_coltiles, self_join ? 1 : 0));
-    if (ws.cand_cap == 0) {
+    {
         const unsigned long long guess =
             g_cand_hint > 0 ? (unsigned long long)g_cand_hint
                             : std::max<unsigned long long>(1ull << 22,
                                                            48ull * (unsigned long long)std::max(nx, ny));
-        RNNG_TRY(ws.cand.reserve(guess * sizeof(int2)));
-        ws.cand_cap = guess;
+        if (ws.cand_cap < guess || g_cand_hint > 0) {
+            RNNG_TRY(ws.cand.reserve(guess * sizeof(int2)));
+            ws.cand_cap = guess;
+        }
     }
     if (g->cap == 0) {
         const int64_t guess = g_edge_hint > 0 ? g_edge_hint
