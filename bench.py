@@ -52,7 +52,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=60_000)
+    ap.add_argument("--cpu-sample", type=int, default=150_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -60,6 +62,7 @@ def main():
         sys.exit(2)
 
     from rnng_amd import Stats, l2, synth
+    from rnng_amd import _lib
 
     t0 = time.perf_counter()
     pts, eps = synth.points_gmm_int(n=args.n)
@@ -67,6 +70,13 @@ def main():
     P = n * (n - 1) // 2
     log(f"generated {n}x{d} eps={eps} in {time.perf_counter() - t0:.1f}s")
     ds = l2.Dataset(pts)
+    if args.copy_calibration:
+        import ctypes
+        ms = ctypes.c_double()
+        nb = args.copy_calibration << 20
+        _lib.check(_lib.lib().rnng_debug_copy(nb, ctypes.byref(ms)))
+        log(f"copy calibration: {nb} bytes read + written in {ms.value:.3f} ms "
+            f"({2 * nb / ms.value / 1e6:.0f} GB/s)")
 
     try:
         import torch

@@ -188,6 +188,11 @@ int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
  * checking it against the host's correctly rounded sqrtf. */
 int rnng_debug_sqrt(const float *in, float *out, int64_t n);
 
+/* Copies nbytes (a multiple of 16) between two device buffers twice with a
+ * plain vector-copy kernel and reports the second copy's time: a kernel of
+ * exactly known HBM traffic for calibrating byte counters. */
+int rnng_debug_copy(int64_t nbytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

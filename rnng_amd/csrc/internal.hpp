@@ -50,6 +50,10 @@ int sort_pairs_u64_f32(uint64_t* keys, float* vals, int64_t n);
 int sort_pairs_u64_i64(uint64_t* keys, int64_t* vals, int64_t n);
 int sort_by_value_then_id(float* value, int64_t* id, int64_t n);
 int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n);
+// Stable compaction of (ij, w) by flag != 0 into (out_ij, out_w); *n_out gets
+// the count.  The outputs must hold n elements.
+int compact_edges(const int2* ij, const float* w, const uint8_t* flag, int64_t n, int2* out_ij,
+                  float* out_w, int64_t* n_out);
 
 // Builds the L2 epsilon graph of a self join (l2_eps.hip).
 int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,

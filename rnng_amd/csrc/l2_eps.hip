@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -274,97 +275,144 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
-// One thread per point: x~ and/or y~ fragments (either output may be null),
-// norm pieces and the error terms.
+// One workgroup per 64 points: the points' rows are staged in LDS with
+// contiguous loads, each point's norm and conversion error is reduced by four
+// lanes, then the x~ and/or y~ fragments (either output may be null) are
+// written 16 bytes per lane, contiguous within each 1 KB fragment.
 //   slot       d      d+1   d+2   d+3   d+4   d+5
 //   x~ real    n1     n2    n3    1     1     1
 //   y~ real    1      1     1     n1    n2    n3
 //   x~ pad     2^100  0     0     1     0     0
 //   y~ pad     1      0     0     2^100 0     0
 // so any pair with a padding point has acc >= 2^100 - C0 > 0.
+// idmap (may be null): padded position -> point index (-1 = padding); without
+// it position p holds point p for p < n.
+constexpr int PP = 64;   // points per prep workgroup
+
 __global__ __launch_bounds__(256) void k_prep_frags(
     const float* __restrict__ X, int n, int n_pad, int d, int KS, const float* __restrict__ mu,
-    uint16_t* __restrict__ xa, uint16_t* __restrict__ yb, unsigned* __restrict__ stat) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int* __restrict__ idmap, uint16_t* __restrict__ xa, uint16_t* __restrict__ yb,
+    unsigned* __restrict__ stat) {
+    extern __shared__ float xs[];          // [PP][d + 1]: x~ = bf16(x - mu) as float
+    __shared__ float pn[PP][3];            // norm pieces
+    __shared__ int preal[PP];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int p0 = blockIdx.x * PP;
+    const int ld = d + 1;
+    if (tid < PP) {
+        const int p = p0 + tid;
+        int src = -1;
+        if (p < n_pad) src = idmap ? idmap[p] : (p < n ? p : -1);
+        preal[tid] = src;
+    }
+    __syncthreads();
+    for (int r = wave; r < PP; r += 4) {
+        const int src = preal[r];
+        for (int k = lane; k < d; k += 64) {
+            float v = 0.0f;
+            if (src >= 0) {
+                const double exact = (double)X[(size_t)src * d + k] - (double)mu[k];
+                v = bf16_to_f32(f32_to_bf16_rn((float)exact));
+            }
+            xs[r * ld + k] = v;
+        }
+    }
+    __syncthreads();
+    // norm and conversion error: 4 lanes per point
     float e_max = 0.0f, n_max = 0.0f, r_max = 0.0f;
-    if (i < n_pad) {
-        const int t = i >> 5, l = i & 31;
-        const bool real = i < n;
+    {
+        const int r = tid >> 2, part = tid & 3;
+        const int src = preal[r];
         double nrm = 0.0, err = 0.0;
-        const int K = KS * 16;
-        if (real) {
-            for (int k = 0; k < d; ++k) {
-                const float x = X[(size_t)i * d + k];
-                const double exact = (double)x - (double)mu[k];
-                const float xb = bf16_to_f32(f32_to_bf16_rn((float)exact));
-                nrm += (double)xb * (double)xb;
-                const double de = (double)xb - exact;
+        if (src >= 0) {
+            for (int k = part; k < d; k += 4) {
+                const double xb = (double)xs[r * ld + k];
+                nrm += xb * xb;
+                const double de = xb - ((double)X[(size_t)src * d + k] - (double)mu[k]);
                 err += de * de;
             }
         }
-        float p[3] = {0.f, 0.f, 0.f};
-        double res = 0.0;
-        if (real) {
-            double r = nrm;
+        nrm += __shfl_xor(nrm, 1, 64);
+        nrm += __shfl_xor(nrm, 2, 64);
+        err += __shfl_xor(err, 1, 64);
+        err += __shfl_xor(err, 2, 64);
+        if (part == 0) {
+            float pc[3] = {0.f, 0.f, 0.f};
+            double rr = nrm;
+            if (src >= 0) {
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                p[q] = bf16_to_f32(f32_to_bf16_rn((float)r));
-                r -= (double)p[q];
-            }
-            res = fabs(r);
-        }
-        const float big = 1.2676506e30f;   // 2^100, exact in bf16
-        for (int k8 = 0; k8 < K; k8 += 8) {
-            const int s = k8 >> 4, h = (k8 >> 3) & 1;
-            const size_t slot = ((size_t)(t * KS + s) * 64 + l + 32 * h) * 8;
-            uint16_t xv[8], yv[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int k = k8 + e;
-                float xf = 0.f, yf = 0.f;
-                if (k < d) {
-                    if (real) {
-                        const float x = X[(size_t)i * d + k];
-                        xf = bf16_to_f32(f32_to_bf16_rn((float)((double)x - (double)mu[k])));
-                        yf = -2.0f * xf;
-                    }
-                } else if (k < d + 3) {
-                    if (real) {
-                        xf = p[k - d];
-                        yf = 1.0f;
-                    } else {
-                        xf = (k == d) ? big : 0.f;
-                        yf = (k == d) ? 1.0f : 0.f;
-                    }
-                } else if (k < d + 6) {
-                    if (real) {
-                        xf = 1.0f;
-                        yf = p[k - d - 3];
-                    } else {
-                        xf = (k == d + 3) ? 1.0f : 0.f;
-                        yf = (k == d + 3) ? big : 0.f;
-                    }
+                for (int q = 0; q < 3; ++q) {
+                    pc[q] = bf16_to_f32(f32_to_bf16_rn((float)rr));
+                    rr -= (double)pc[q];
                 }
-                xv[e] = f32_to_bf16_rn(xf);
-                yv[e] = f32_to_bf16_rn(yf);
+                e_max = (float)sqrt(err) * (1.0f + 1e-6f);
+                n_max = (float)nrm * (1.0f + 1e-6f);
+                r_max = (float)fabs(rr) * 2.0f;
             }
-            if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
-            if (yb) *(uint4*)(yb + slot) = *(const uint4*)yv;
+            pn[r][0] = pc[0];
+            pn[r][1] = pc[1];
+            pn[r][2] = pc[2];
         }
-        if (real) {
-            e_max = (float)sqrt(err) * (1.0f + 1e-6f);
-            n_max = (float)nrm * (1.0f + 1e-6f);
-            r_max = (float)res * 2.0f;
+    }
+    __syncthreads();
+    const float big = 1.2676506e30f;   // 2^100, exact in bf16
+    const int nchunk = (PP / 32) * KS * 64;
+    for (int c = tid; c < nchunk; c += 256) {
+        const int tt = c / (KS * 64), rem = c - tt * (KS * 64);
+        const int st = rem >> 6, l = rem & 63;
+        const int r = tt * 32 + (l & 31);
+        if (p0 + tt * 32 >= n_pad) continue;
+        const bool real = preal[r] >= 0;
+        const int kb = 16 * st + 8 * (l >> 5);
+        uint16_t xv[8], yv[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = kb + e;
+            float xf = 0.f, yf = 0.f;
+            if (k < d) {
+                xf = xs[r * ld + k];
+                yf = -2.0f * xf;
+            } else if (k < d + 3) {
+                if (real) {
+                    xf = pn[r][k - d];
+                    yf = 1.0f;
+                } else {
+                    xf = (k == d) ? big : 0.f;
+                    yf = (k == d) ? 1.0f : 0.f;
+                }
+            } else if (k < d + 6) {
+                if (real) {
+                    xf = 1.0f;
+                    yf = pn[r][k - d - 3];
+                } else {
+                    xf = (k == d + 3) ? 1.0f : 0.f;
+                    yf = (k == d + 3) ? big : 0.f;
+                }
+            }
+            xv[e] = f32_to_bf16_rn(xf);
+            yv[e] = f32_to_bf16_rn(yf);
         }
+        const size_t slot = ((size_t)((p0 >> 5) + tt) * KS * 64 + rem) * 8;
+        if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
+        if (yb) *(uint4*)(yb + slot) = *(const uint4*)yv;
     }
     e_max = wave_max(e_max);
     n_max = wave_max(n_max);
     r_max = wave_max(r_max);
-    if (lane_id() == 0) {
+    if (lane == 0) {
         atomicMax(&stat[ST_MAXE], __float_as_uint(e_max));
         atomicMax(&stat[ST_MAXN], __float_as_uint(n_max));
         atomicMax(&stat[ST_MAXRES], __float_as_uint(r_max));
     }
+}
+
+int launch_prep(const float* X, int n, int n_pad, int d, int KS, const float* mu, const int* idmap,
+                uint16_t* xa, uint16_t* yb, unsigned* stat) {
+    if (n_pad <= 0) return RNNG_OK;
+    const size_t lds = (size_t)PP * (d + 1) * sizeof(float);
+    k_prep_frags<<<blocks_for(n_pad, PP), 256, lds, stream()>>>(X, n, n_pad, d, KS, mu, idmap, xa, yb, stat);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
 }
 
 // C0: an upper bound of what the MFMA can compute for |x~ - y~|^2 for any pair
@@ -406,166 +454,276 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
 // ------------------------------------------------------------------------
 // MFMA filter kernel
 // ------------------------------------------------------------------------
-// Workgroup = 4 waves = one row block of 8 row tiles (256 rows); wave w owns
-// row tiles 2w and 2w+1 and keeps their fragments in registers for the whole
-// work item.  The work item's column tiles stream through: 9 coalesced 1 KB
-// loads per tile, 18 MFMAs, 32 ORs, one wave-uniform branch.
+// Workgroup = 4 waves = one row block of 4*RT row tiles (32 rows each); wave w
+// owns row tiles RT*w .. RT*w+RT-1 and keeps their fragments in registers for
+// the whole work item.  The work item's column tiles stream through.  Per
+// column tile and wave: KS coalesced 1 KB fragment loads, RT*KS MFMAs, an OR
+// over the RT*16 accumulators and one wave-uniform branch.
+//   USE_LDS: the workgroup stages each column tile once in LDS (double
+//   buffered, one barrier per tile) instead of every wave loading it from
+//   L1, cutting vector-cache traffic by 4x; RT = 4 halves it again by using
+//   every B fragment for 4 MFMAs.
 // Output row/column of accumulator register r in lane l (32x32 MFMA):
 //   row = 8*(r/4) + 4*(l/32) + r%4, column = l%32.
-__device__ __forceinline__ int mfma32_row(int lane, int r) {
-    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-}
 
-template <int KS>
+constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
+
+template <int KS, int RT, bool USE_LDS, int CT = 1>
 __global__ __launch_bounds__(256) void k_l2_mfma_filter(
     const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb,
     const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
     const MfmaParams* __restrict__ prm, int2* __restrict__ cand, unsigned long long cap,
     unsigned long long* __restrict__ counter) {
+    // CT column tiles per step (USE_LDS only): RT*CT independent accumulator
+    // chains and one barrier / epilogue per RT*CT*KS MFMAs.
+    constexpr int NCH = KS * 64;                // 16-byte fragments per column tile
+    constexpr int PER = (CT * NCH + 255) / 256; // fragments staged per thread per step
+    __shared__ bf16x8_t bs[USE_LDS ? 2 : 1][USE_LDS ? CT * NCH : 1];
+    // Candidates are staged in LDS (one LDS atomic per ballot) and written out
+    // with a single global reservation per work item; a single global counter
+    // hit once per ballot serialises at one L2 address.
+    __shared__ int2 stage[LSTAGE];
+    __shared__ unsigned scount;
+    __shared__ unsigned long long sbase;
     const int4 wi = work[blockIdx.x];
-    const int wave = threadIdx.x >> 6;
-    const int lane = threadIdx.x & 63;
-    const int rt = wi.x * 8 + wave * 2;
+    // Empty items (padding of the XCD-ordered list) touch no memory: the
+    // whole workgroup leaves before any load, so no barrier is skipped.
+    if (wi.y >= wi.z) return;
+    const int tid = threadIdx.x;
+    if (tid == 0) scount = 0u;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int rt0 = wi.x * (4 * RT) + wave * RT;
 
-    bf16x8_t a0[KS], a1[KS];
+    bf16x8_t a[RT][KS];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        a0[s] = xa[((size_t)rt * KS + s) * 64 + lane];
-        a1[s] = xa[((size_t)(rt + 1) * KS + s) * 64 + lane];
-    }
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
     const float negc0 = prm->negc0;
-    const int row_base0 = rt * 32, row_base1 = rt * 32 + 32;
+    // column tiles beyond wi.z read a valid tile (wi.z - 1) and are masked out
+    const int last = wi.z - 1;
 
-    bf16x8_t b[KS];
+    bf16x8_t b[USE_LDS ? 1 : KS];
+    if (USE_LDS) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) b[s] = yb[((size_t)wi.y * KS + s) * 64 + lane];
-
-    for (int ct = wi.y; ct < wi.z; ++ct) {
-        bf16x8_t bn[KS];
-        const int ctn = (ct + 1 < wi.z) ? ct + 1 : ct;
-#pragma unroll
-        for (int s = 0; s < KS; ++s) bn[s] = yb[((size_t)ctn * KS + s) * 64 + lane];
-
-        f32x16_t c0, c1;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            c0[r] = negc0;
-            c1[r] = negc0;
+        for (int j = 0; j < PER; ++j) {
+            const int c = tid + 256 * j;
+            if (c < CT * NCH) {
+                const int ctile = min(wi.y + c / NCH, last);
+                bs[0][c] = yb[(size_t)ctile * NCH + (c % NCH)];
+            }
         }
+    } else {
+#pragma unroll
+        for (int s = 0; s < KS; ++s) b[s] = yb[((size_t)wi.y * KS + s) * 64 + lane];
+    }
+    __syncthreads();
+
+    for (int ct = wi.y; ct < wi.z; ct += CT) {
+        const int buf = ((ct - wi.y) / CT) & 1;
+        const bool has_next = ct + CT < wi.z;
+        bf16x8_t pre[USE_LDS ? PER : KS];
+        if (USE_LDS) {
+            if (has_next) {
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    const int c = tid + 256 * j;
+                    if (c < CT * NCH) {
+                        const int ctile = min(ct + CT + c / NCH, last);
+                        pre[j] = yb[(size_t)ctile * NCH + (c % NCH)];
+                    }
+                }
+            }
+        } else {
+            const int ctn = has_next ? ct + 1 : ct;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) pre[s] = yb[((size_t)ctn * KS + s) * 64 + lane];
+        }
+
+        f32x16_t acc[CT][RT];
+#pragma unroll
+        for (int cc = 0; cc < CT; ++cc)
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[cc][r][q] = negc0;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0[s], b[s], c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1[s], b[s], c1, 0, 0, 0);
+#pragma unroll
+            for (int cc = 0; cc < CT; ++cc) {
+                const bf16x8_t bb = USE_LDS ? bs[buf][cc * NCH + s * 64 + lane] : b[USE_LDS ? 0 : s];
+#pragma unroll
+                for (int r = 0; r < RT; ++r)
+                    acc[cc][r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[r][s], bb, acc[cc][r], 0, 0, 0);
+            }
         }
         unsigned o = 0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) o |= __float_as_uint(c0[r]) | __float_as_uint(c1[r]);
+        for (int cc = 0; cc < CT; ++cc)
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) o |= __float_as_uint(acc[cc][r][q]);
 
         if (__any((int)o < 0)) {
-            const int col = ct * 32 + (lane & 31);
-            unsigned cnt = 0;
+            // Rare path: one ballot per accumulator register; only registers
+            // holding a candidate reserve output slots (one atomic per
+            // register).  Per-lane bases plus compile-time offsets keep the
+            // row indices out of scalar registers.
+            const int rowb = rt0 * 32 + 4 * (lane >> 5);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i0 = row_base0 + mfma32_row(lane, r);
-                const int i1 = row_base1 + mfma32_row(lane, r);
-                const bool p0 = c0[r] < 0.0f && i0 < n_rows && col < n_cols && (!self_join || i0 < col);
-                const bool p1 = c1[r] < 0.0f && i1 < n_rows && col < n_cols && (!self_join || i1 < col);
-                cnt += (p0 ? 1u : 0u) + (p1 ? 1u : 0u);
-            }
-            unsigned long long pos = wave_reserve(cnt, counter);
+            for (int cc = 0; cc < CT; ++cc) {
+                const int col = (ct + cc) * 32 + (lane & 31);
+                const int lim = (ct + cc < wi.z && col < n_cols)
+                                    ? (self_join ? min(n_rows, col) : n_rows) : 0;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i0 = row_base0 + mfma32_row(lane, r);
-                const int i1 = row_base1 + mfma32_row(lane, r);
-                const bool p0 = c0[r] < 0.0f && i0 < n_rows && col < n_cols && (!self_join || i0 < col);
-                const bool p1 = c1[r] < 0.0f && i1 < n_rows && col < n_cols && (!self_join || i1 < col);
-                if (p0) {
-                    if (pos < cap) cand[pos] = make_int2(i0, col);
-                    ++pos;
-                }
-                if (p1) {
-                    if (pos < cap) cand[pos] = make_int2(i1, col);
-                    ++pos;
-                }
+                for (int r = 0; r < RT; ++r)
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const int row = rowb + (r * 32 + 8 * (q >> 2) + (q & 3));
+                        const bool p = acc[cc][r][q] < 0.0f && row < lim;
+                        const unsigned long long m = __ballot(p);
+                        if (m) {
+                            unsigned slot = 0;
+                            if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
+                            slot = __shfl(slot, 0, 64) +
+                                   __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                            if (p) {
+                                if (slot < LSTAGE) {
+                                    stage[slot] = make_int2(row, col);
+                                } else {
+                                    const unsigned long long g = atomicAdd(counter, 1ull);
+                                    if (g < cap) cand[g] = make_int2(row, col);
+                                }
+                            }
+                        }
+                    }
             }
         }
+        if (USE_LDS) {
+            if (has_next) {
 #pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = bn[s];
+                for (int j = 0; j < PER; ++j) {
+                    const int c = tid + 256 * j;
+                    if (c < CT * NCH) bs[buf ^ 1][c] = pre[j];
+                }
+            }
+            __syncthreads();
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) b[USE_LDS ? 0 : s] = pre[s];
+        }
     }
+    __syncthreads();
+    const unsigned nst = min(scount, (unsigned)LSTAGE);
+    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
+    __syncthreads();
+    for (unsigned i = tid; i < nst; i += 256)
+        if (sbase + i < cap) cand[sbase + i] = stage[i];
 }
 
 // ------------------------------------------------------------------------
 // FP32 recheck in the reference's order + FP64 band check
 // ------------------------------------------------------------------------
 // counters: [0] candidates, [1] edges, [2] band pairs, [3] band flips
+// Each wave takes RC = 16 candidates per round.  The 32 rows they need are
+// first all loaded into registers (two rows per 64-lane instruction when
+// d % 4 == 0), then written to LDS rows padded to a multiple of 4 floats whose
+// stride keeps the 16 computing lanes on distinct banks; then 16 lanes each
+// run one candidate's sequential sum from LDS.  Rounds are uniform over the
+// block (barriers), candidates past the end are masked.
+constexpr int RC = 16;   // candidates per wave per round
+
+__host__ __device__ constexpr int recheck_ld(int d) { return ((d + 3) & ~3) + 4; }
+
+// Output per candidate c: flag[c] = 1 if it is an edge, w[c] = its float
+// distance; a stable device select then compacts the edges (no atomics).
 template <bool VEC4>
 __global__ __launch_bounds__(256) void k_l2_recheck(
     const float* __restrict__ X, const float* __restrict__ Y, int d,
     const int2* __restrict__ cand, unsigned long long ccap, float T, float eps,
-    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    uint8_t* __restrict__ flag, float* __restrict__ cw,
     unsigned long long* __restrict__ counters) {
 #pragma clang fp contract(off)
+    extern __shared__ float rows_all[];   // [4 waves][2*RC][ld]
     const unsigned long long nc = min(counters[0], ccap);
-    const int lane = lane_id();
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ld = recheck_ld(d);
+    float* rows = rows_all + (size_t)wave * (2 * RC) * ld;
     const double band = 1e-5 * (double)eps;
-    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
-         base += (unsigned long long)gridDim.x * blockDim.x) {
-        const unsigned long long c = base + threadIdx.x;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * (4 * RC); base < nc;
+         base += (unsigned long long)gridDim.x * (4 * RC)) {
+        const unsigned long long c0 = base + (unsigned long long)wave * RC;
+        int2 p = make_int2(-1, -1);
+        if (lane < RC && c0 + lane < nc) p = cand[c0 + lane];
+        if (VEC4 && d <= 128) {
+            // row r = 2*i + (lane >> 5), float4 column lane & 31
+            const int half = lane >> 5, k4 = lane & 31;
+            const int nk4 = d >> 2;
+            float4 v[RC];
+#pragma unroll
+            for (int i = 0; i < RC; ++i) {
+                const int r = 2 * i + half;
+                const int srcl = r < RC ? r : r - RC;
+                const int id = __shfl(r < RC ? p.x : p.y, srcl, 64);
+                v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (id >= 0 && k4 < nk4) v[i] = *(const float4*)((r < RC ? X : Y) + (size_t)id * d + 4 * k4);
+            }
+#pragma unroll
+            for (int i = 0; i < RC; ++i) {
+                const int r = 2 * i + half;
+                if (k4 < nk4) *(float4*)(rows + r * ld + 4 * k4) = v[i];
+            }
+        } else {
+            for (int r = 0; r < 2 * RC; ++r) {
+                const int srcl = r < RC ? r : r - RC;
+                const int id = __shfl(r < RC ? p.x : p.y, srcl, 64);
+                if (id < 0) continue;
+                const float* src = (r < RC ? X : Y) + (size_t)id * d;
+                for (int k = lane; k < d; k += 64) rows[r * ld + k] = src[k];
+            }
+        }
+        __syncthreads();
         bool edge = false;
-        int2 p = make_int2(0, 0);
         float w = 0.0f;
-        if (c < nc) {
-            p = cand[c];
-            const float* x = X + (size_t)p.x * d;
-            const float* y = Y + (size_t)p.y * d;
+        if (lane < RC && p.x >= 0) {
+            const float* x = rows + lane * ld;
+            const float* y = rows + (RC + lane) * ld;
             float acc = 0.0f;
-            if (VEC4) {
-                for (int k = 0; k < d; k += 4) {
-                    const float4 xv = *(const float4*)(x + k);
-                    const float4 yv = *(const float4*)(y + k);
-                    float t;
-                    t = xv.x - yv.x; acc = acc + t * t;
-                    t = xv.y - yv.y; acc = acc + t * t;
-                    t = xv.z - yv.z; acc = acc + t * t;
-                    t = xv.w - yv.w; acc = acc + t * t;
-                }
-            } else {
-                for (int k = 0; k < d; ++k) {
-                    const float t = x[k] - y[k];
-                    acc = acc + t * t;
-                }
+            int k = 0;
+            for (; k + 4 <= d; k += 4) {
+                const float4 xv = *(const float4*)(x + k);
+                const float4 yv = *(const float4*)(y + k);
+                float t;
+                t = xv.x - yv.x; acc = acc + t * t;
+                t = xv.y - yv.y; acc = acc + t * t;
+                t = xv.z - yv.z; acc = acc + t * t;
+                t = xv.w - yv.w; acc = acc + t * t;
+            }
+            for (; k < d; ++k) {
+                const float t = x[k] - y[k];
+                acc = acc + t * t;
             }
             edge = acc <= T;
             w = sqrtf(acc);
             if (fabs((double)w - (double)eps) <= band) {
                 atomicAdd(&counters[2], 1ull);
-                double s = 0.0;
-                for (int k = 0; k < d; ++k) {
-                    const double t = (double)x[k] - (double)y[k];
-                    s += t * t;
+                double s2 = 0.0;
+                for (int kk = 0; kk < d; ++kk) {
+                    const double t = (double)x[kk] - (double)y[kk];
+                    s2 += t * t;
                 }
-                const bool in64 = sqrt(s) <= (double)eps;
+                const bool in64 = sqrt(s2) <= (double)eps;
                 if (in64 != edge) atomicAdd(&counters[3], 1ull);
             }
         }
-        const unsigned long long m = __ballot(edge);
-        if (m) {
-            unsigned lo = 0, hi = 0;
-            if (lane == 0) {
-                unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)__popcll(m));
-                lo = (unsigned)b0;
-                hi = (unsigned)(b0 >> 32);
-            }
-            lo = __shfl(lo, 0, 64);
-            hi = __shfl(hi, 0, 64);
-            const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
-            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-            if (edge && b0 + rank < ecap) {
-                eij[b0 + rank] = p;
-                ew[b0 + rank] = w;
-            }
+        if (lane < RC && c0 + lane < nc) {
+            flag[c0 + lane] = edge ? 1 : 0;
+            cw[c0 + lane] = w;
         }
+        __syncthreads();
     }
 }
 
@@ -586,6 +744,15 @@ __global__ void k_unpack_keys(const uint64_t* __restrict__ keys, int2* __restric
 __global__ void k_debug_sqrt(const float* __restrict__ in, float* __restrict__ out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = sqrtf(in[i]);
+}
+
+// Plain device copy of known size, used to calibrate the HBM byte counters
+// (FETCH_SIZE / WRITE_SIZE) against a kernel whose traffic is exactly known.
+__global__ __launch_bounds__(256) void k_debug_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                    int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 // ------------------------------------------------------------------------
@@ -621,13 +788,38 @@ __global__ void k_mfma_layout(int* rows, int* cols) {
 struct Workspace {
     DevBuf counters;   // u64[8]
     DevBuf cand;       // int2
+    DevBuf cflag;      // uint8 per candidate: recheck says edge
+    DevBuf cw;         // float per candidate: its distance
     unsigned long long cand_cap = 0;
     DevBuf xa, yb;     // fragments
     DevBuf colsum, colmin, colmax, stat, range, mu, params;
     DevBuf work;
     std::vector<int4> work_host;
-    int work_key[4] = {-1, -1, -1, -1};
+    int work_key[5] = {-1, -1, -1, -1, -1};
 };
+
+// Filter variants: {row tiles per wave, stage column tiles in LDS, column
+// tiles per step}.
+struct FilterVariant {
+    int rt;
+    bool lds;
+    int ct;
+};
+constexpr FilterVariant kVariants[] = {{2, false, 1}, {2, true, 1}, {4, true, 1}, {3, true, 1},
+                                       {2, true, 2}, {1, true, 4}, {1, true, 2}};
+constexpr int kNumVariants = 7;
+int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
+
+int filter_variant() {
+    if (g_variant < 0) {
+        g_variant = 0;
+        if (const char* e = getenv("RNNG_L2_VARIANT")) {
+            const int v = atoi(e);
+            if (v >= 0 && v < kNumVariants) g_variant = v;
+        }
+    }
+    return g_variant;
+}
 
 std::mutex g_ws_mu;
 Workspace g_ws;
@@ -650,50 +842,89 @@ int launch_fill_u32(unsigned* p, int n, unsigned v) {
 // so that the workgroups resident at one time share column fragments in L2.
 constexpr int CH = 256;
 
-int build_work(Workspace& ws, int n_rowblocks, int n_coltiles, int self_join) {
+// Workgroups are dispatched to the 8 XCDs round-robin by block index, and each
+// XCD has its own L2.  With xcd_remap the list is permuted so that XCD x runs
+// the contiguous x-th eighth of the chunk-major list: the column chunk that
+// its resident workgroups share is then fetched into one L2 instead of eight.
+int g_xcd_remap = -1;
+
+int xcd_remap() {
+    if (g_xcd_remap < 0) {
+        g_xcd_remap = 1;
+        if (const char* e = getenv("RNNG_L2_XCD")) g_xcd_remap = atoi(e) ? 1 : 0;
+    }
+    return g_xcd_remap;
+}
+
+int build_work(Workspace& ws, int n_rowblocks, int rb_tiles, int n_coltiles, int self_join) {
+    const int remap = xcd_remap();
     if (ws.work_key[0] == n_rowblocks && ws.work_key[1] == n_coltiles &&
-        ws.work_key[2] == self_join && ws.work_key[3] == CH)
+        ws.work_key[2] == self_join && ws.work_key[3] == CH * 2 + remap && ws.work_key[4] == rb_tiles)
         return RNNG_OK;
-    ws.work_host.clear();
+    std::vector<int4> list;
     const int nchunks = (n_coltiles + CH - 1) / CH;
     for (int c = 0; c < nchunks; ++c) {
         const int c0 = c * CH, c1 = std::min(n_coltiles, c0 + CH);
         for (int rb = 0; rb < n_rowblocks; ++rb) {
             int lo = c0;
-            if (self_join) lo = std::max(lo, rb * 8);
+            if (self_join) lo = std::max(lo, rb * rb_tiles);
             if (lo >= c1) continue;
-            ws.work_host.push_back(make_int4(rb, lo, c1, 0));
+            list.push_back(make_int4(rb, lo, c1, 0));
         }
     }
-    RNNG_TRY(ws.work.reserve(ws.work_host.size() * sizeof(int4)));
+    if (remap) {
+        const size_t per = (list.size() + 7) / 8;
+        ws.work_host.assign(per * 8, make_int4(0, 0, 0, 0));   // empty items run no tile
+        for (size_t b = 0; b < per * 8; ++b) {
+            const size_t src = (b % 8) * per + b / 8;
+            if (src < list.size()) ws.work_host[b] = list[src];
+        }
+    } else {
+        ws.work_host.swap(list);
+    }
+    RNNG_TRY(ws.work.reserve(std::max<size_t>(1, ws.work_host.size()) * sizeof(int4)));
     RNNG_HIP_TRY(hipMemcpyAsync(ws.work.ptr, ws.work_host.data(), ws.work_host.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, stream()));
     ws.work_key[0] = n_rowblocks;
     ws.work_key[1] = n_coltiles;
     ws.work_key[2] = self_join;
-    ws.work_key[3] = CH;
+    ws.work_key[3] = CH * 2 + remap;
+    ws.work_key[4] = rb_tiles;
     return RNNG_OK;
 }
 
-template <int KS>
+template <int KS, int RT, bool LDS, int CT = 1>
 void launch_filter(Workspace& ws, int n_rows, int n_cols, int self_join) {
-    k_l2_mfma_filter<KS><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+    k_l2_mfma_filter<KS, RT, LDS, CT><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
         ws.xa.as<bf16x8_t>(), ws.yb.as<bf16x8_t>(), ws.work.as<int4>(), n_rows, n_cols, self_join,
         ws.params.as<MfmaParams>(), ws.cand.as<int2>(), ws.cand_cap,
         ws.counters.as<unsigned long long>());
 }
 
-int dispatch_filter(Workspace& ws, int KS, int n_rows, int n_cols, int self_join) {
+template <int KS>
+void launch_filter_v(Workspace& ws, int variant, int n_rows, int n_cols, int self_join) {
+    switch (variant) {
+        case 0: launch_filter<KS, 2, false>(ws, n_rows, n_cols, self_join); break;
+        case 1: launch_filter<KS, 2, true>(ws, n_rows, n_cols, self_join); break;
+        case 3: launch_filter<KS, 3, true>(ws, n_rows, n_cols, self_join); break;
+        case 4: launch_filter<KS, 2, true, 2>(ws, n_rows, n_cols, self_join); break;
+        case 5: launch_filter<KS, 1, true, 4>(ws, n_rows, n_cols, self_join); break;
+        case 6: launch_filter<KS, 1, true, 2>(ws, n_rows, n_cols, self_join); break;
+        default: launch_filter<KS, 4, true>(ws, n_rows, n_cols, self_join); break;
+    }
+}
+
+int dispatch_filter(Workspace& ws, int KS, int variant, int n_rows, int n_cols, int self_join) {
     switch (KS) {
-        case 1: launch_filter<1>(ws, n_rows, n_cols, self_join); break;
-        case 2: launch_filter<2>(ws, n_rows, n_cols, self_join); break;
-        case 3: launch_filter<3>(ws, n_rows, n_cols, self_join); break;
-        case 4: launch_filter<4>(ws, n_rows, n_cols, self_join); break;
-        case 5: launch_filter<5>(ws, n_rows, n_cols, self_join); break;
-        case 6: launch_filter<6>(ws, n_rows, n_cols, self_join); break;
-        case 7: launch_filter<7>(ws, n_rows, n_cols, self_join); break;
-        case 8: launch_filter<8>(ws, n_rows, n_cols, self_join); break;
-        case 9: launch_filter<9>(ws, n_rows, n_cols, self_join); break;
+        case 1: launch_filter_v<1>(ws, variant, n_rows, n_cols, self_join); break;
+        case 2: launch_filter_v<2>(ws, variant, n_rows, n_cols, self_join); break;
+        case 3: launch_filter_v<3>(ws, variant, n_rows, n_cols, self_join); break;
+        case 4: launch_filter_v<4>(ws, variant, n_rows, n_cols, self_join); break;
+        case 5: launch_filter_v<5>(ws, variant, n_rows, n_cols, self_join); break;
+        case 6: launch_filter_v<6>(ws, variant, n_rows, n_cols, self_join); break;
+        case 7: launch_filter_v<7>(ws, variant, n_rows, n_cols, self_join); break;
+        case 8: launch_filter_v<8>(ws, variant, n_rows, n_cols, self_join); break;
+        case 9: launch_filter_v<9>(ws, variant, n_rows, n_cols, self_join); break;
         default:
             set_error("MFMA mode supports d <= 138");
             return RNNG_ERR_UNSUPPORTED;
@@ -736,7 +967,7 @@ int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, 
         if (ne <= g->cap) {
             g->n_edges = ne;
             if (st) {
-                st->edges = ne;
+                st->edges += ne;
                 st->filter_ms += ev.ms(0, 1);
                 st->total_ms += ev.ms(0, 1);
             }
@@ -763,9 +994,12 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
     if (Tb < T) Tb = T;
 
-    const int n_rowtiles = ((nx + 255) / 256) * 8;      // multiple of 8 (row blocks)
+    const int variant = filter_variant();
+    const int rb_tiles = 4 * kVariants[variant].rt;      // row tiles per workgroup
+    const int rb_rows = 32 * rb_tiles;
+    const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
     const int n_coltiles = (ny + 31) / 32;
-    const int n_rowblocks = n_rowtiles / 8;
+    const int n_rowblocks = n_rowtiles / rb_tiles;
     const int npad_x = n_rowtiles * 32;
     const int npad_y = std::max(n_coltiles * 32, self_join ? npad_x : 0);
     const size_t frag_bytes_x = (size_t)npad_x * KS * 16 * 2;
@@ -781,7 +1015,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     RNNG_TRY(ws.range.reserve(sizeof(unsigned)));
     RNNG_TRY(ws.mu.reserve(sizeof(float) * d));
     RNNG_TRY(ws.params.reserve(sizeof(MfmaParams)));
-    RNNG_TRY(build_work(ws, n_rowblocks, n_coltiles, self_join ? 1 : 0));
+    RNNG_TRY(build_work(ws, n_rowblocks, rb_tiles, n_coltiles, self_join ? 1 : 0));
     {
         const unsigned long long guess =
             g_cand_hint > 0 ? (unsigned long long)g_cand_hint
@@ -833,17 +1067,13 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     if (self_join) {
         // one pass writes both forms for every padded row of the join
         const int npad = std::max(npad_x, npad_y);
-        k_prep_frags<<<blocks_for(npad, 256), 256, 0, stream()>>>(
-            A->X, nx, npad, d, KS, ws.mu.as<float>(), ws.xa.as<uint16_t>(), ws.yb.as<uint16_t>(),
-            ws.stat.as<unsigned>());
+        RNNG_TRY(launch_prep(A->X, nx, npad, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
+                             ws.yb.as<uint16_t>(), ws.stat.as<unsigned>()));
     } else {
-        k_prep_frags<<<blocks_for(npad_x, 256), 256, 0, stream()>>>(
-            A->X, nx, npad_x, d, KS, ws.mu.as<float>(), ws.xa.as<uint16_t>(), nullptr,
-            ws.stat.as<unsigned>());
-        RNNG_HIP_TRY(hipGetLastError());
-        k_prep_frags<<<blocks_for(npad_y, 256), 256, 0, stream()>>>(
-            B->X, ny, npad_y, d, KS, ws.mu.as<float>(), nullptr, ws.yb.as<uint16_t>(),
-            ws.stat.as<unsigned>());
+        RNNG_TRY(launch_prep(A->X, nx, npad_x, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
+                             nullptr, ws.stat.as<unsigned>()));
+        RNNG_TRY(launch_prep(B->X, ny, npad_y, d, KS, ws.mu.as<float>(), nullptr, nullptr,
+                             ws.yb.as<uint16_t>(), ws.stat.as<unsigned>()));
     }
     RNNG_HIP_TRY(hipGetLastError());
     k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
@@ -855,45 +1085,55 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        RNNG_TRY(dispatch_filter(ws, KS, nx, ny, self_join ? 1 : 0));
+        RNNG_TRY(dispatch_filter(ws, KS, variant, nx, ny, self_join ? 1 : 0));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
-        const unsigned rgrid = 2048;
-        if (vec4)
-            k_l2_recheck<true><<<rgrid, 256, 0, stream()>>>(
-                A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, g->ij.as<int2>(),
-                g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>());
-        else
-            k_l2_recheck<false><<<rgrid, 256, 0, stream()>>>(
-                A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, g->ij.as<int2>(),
-                g->w.as<float>(), (unsigned long long)g->cap, ws.counters.as<unsigned long long>());
-        RNNG_HIP_TRY(hipGetLastError());
+        unsigned long long ncand = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(&ncand, ws.counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        if (ncand > ws.cand_cap) {
+            // count-then-fill: grow to the exact count and run the filter again
+            if (st) st->refills++;
+            const unsigned long long nc = ncand + ncand / 8 + 1024;
+            RNNG_TRY(ws.cand.reserve(nc * sizeof(int2)));
+            ws.cand_cap = nc;
+            continue;
+        }
+        RNNG_TRY(ws.cflag.reserve(std::max<size_t>(1, ncand)));
+        RNNG_TRY(ws.cw.reserve(std::max<size_t>(1, ncand) * sizeof(float)));
+        if ((int64_t)ncand > g->cap) RNNG_TRY(edge_buffers(g, (int64_t)ncand));
         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+        if (ncand > 0) {
+            const unsigned rgrid = (unsigned)std::min<unsigned long long>(2048, (ncand + 63) / 64);
+            const size_t rlds = (size_t)4 * 2 * RC * recheck_ld(d) * sizeof(float);
+            if (vec4)
+                k_l2_recheck<true><<<rgrid, 256, rlds, stream()>>>(
+                    A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, ws.cflag.as<uint8_t>(),
+                    ws.cw.as<float>(), ws.counters.as<unsigned long long>());
+            else
+                k_l2_recheck<false><<<rgrid, 256, rlds, stream()>>>(
+                    A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, ws.cflag.as<uint8_t>(),
+                    ws.cw.as<float>(), ws.counters.as<unsigned long long>());
+            RNNG_HIP_TRY(hipGetLastError());
+        }
+        int64_t ne = 0;
+        RNNG_TRY(compact_edges(ws.cand.as<int2>(), ws.cw.as<float>(), ws.cflag.as<uint8_t>(), (int64_t)ncand,
+                               g->ij.as<int2>(), g->w.as<float>(), &ne));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[5], stream()));
         unsigned long long cnt[8];
         RNNG_HIP_TRY(hipMemcpyAsync(cnt, ws.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
-        const bool cand_over = cnt[0] > ws.cand_cap;
-        const bool edge_over = (int64_t)cnt[1] > g->cap;
-        if (!cand_over && !edge_over) {
-            g->n_edges = (int64_t)cnt[1];
-            if (st) {
-                st->candidates = (int64_t)cnt[0];
-                st->edges = (int64_t)cnt[1];
-                st->band_pairs = (int64_t)cnt[2];
-                st->band_flips = (int64_t)cnt[3];
-                st->prep_ms += ev.ms(0, 1);
-                st->filter_ms += ev.ms(2, 3);
-                st->recheck_ms += ev.ms(3, 4);
-                st->total_ms += ev.ms(0, 1) + ev.ms(2, 4);
-            }
-            return RNNG_OK;
+        g->n_edges = ne;
+        if (st) {
+            st->candidates += (int64_t)ncand;
+            st->edges += ne;
+            st->band_pairs += (int64_t)cnt[2];
+            st->band_flips += (int64_t)cnt[3];
+            st->prep_ms += ev.ms(0, 1);
+            st->filter_ms += ev.ms(2, 3);
+            st->recheck_ms += ev.ms(4, 5);
+            st->total_ms += ev.ms(0, 1) + ev.ms(2, 3) + ev.ms(4, 5);
         }
-        if (st) st->refills++;
-        if (cand_over) {
-            const unsigned long long nc = cnt[0] + cnt[0] / 8 + 1024;
-            RNNG_TRY(ws.cand.reserve(nc * sizeof(int2)));
-            ws.cand_cap = nc;
-        }
-        if (edge_over) RNNG_TRY(edge_buffers(g, (int64_t)cnt[1] + (int64_t)cnt[1] / 8 + 1024));
+        return RNNG_OK;
     }
     set_error("candidate/edge buffer refill did not converge");
     return RNNG_ERR_HIP;
@@ -1153,6 +1393,28 @@ int rnng_debug_sqrt(const float* in, float* out, int64_t n) {
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     RNNG_HIP_TRY(hipMemcpy(out, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return RNNG_OK;
+}
+
+int rnng_debug_copy(int64_t nbytes, double* ms) {
+    if (nbytes <= 0 || (nbytes & 15)) {
+        set_error("nbytes must be a positive multiple of 16");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    LocalBuf a, b;
+    RNNG_TRY(a.reserve((size_t)nbytes));
+    RNNG_TRY(b.reserve((size_t)nbytes));
+    Events ev;
+    RNNG_TRY(ev.create());
+    const int64_t n16 = nbytes / 16;
+    k_debug_copy<<<4096, 256, 0, stream()>>>(b.as<uint4>(), a.as<uint4>(), n16);   // touch both
+    RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
+    k_debug_copy<<<4096, 256, 0, stream()>>>(a.as<uint4>(), b.as<uint4>(), n16);
+    RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    if (ms) *ms = ev.ms(0, 1);
     return RNNG_OK;
 }
 

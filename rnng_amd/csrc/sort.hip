@@ -105,4 +105,28 @@ int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n) {
     return RNNG_OK;
 }
 
+int compact_edges(const int2* ij, const float* w, const uint8_t* flag, int64_t n, int2* out_ij,
+                  float* out_w, int64_t* n_out) {
+    *n_out = 0;
+    if (n <= 0) return RNNG_OK;
+    LocalBuf cnt, tmp;
+    RNNG_TRY(cnt.reserve(2 * sizeof(unsigned long long)));
+    size_t tb1 = 0, tb2 = 0;
+    unsigned long long* c = cnt.as<unsigned long long>();
+    RNNG_HIP_TRY(rocprim::select(nullptr, tb1, ij, flag, out_ij, c, (size_t)n, stream()));
+    RNNG_HIP_TRY(rocprim::select(nullptr, tb2, w, flag, out_w, c + 1, (size_t)n, stream()));
+    RNNG_TRY(tmp.reserve(std::max(tb1, tb2)));
+    RNNG_HIP_TRY(rocprim::select(tmp.ptr, tb1, ij, flag, out_ij, c, (size_t)n, stream()));
+    RNNG_HIP_TRY(rocprim::select(tmp.ptr, tb2, w, flag, out_w, c + 1, (size_t)n, stream()));
+    unsigned long long h[2] = {0, 0};
+    RNNG_HIP_TRY(hipMemcpyAsync(h, c, sizeof(h), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    if (h[0] != h[1]) {
+        set_error("compact_edges: inconsistent select counts");
+        return RNNG_ERR_HIP;
+    }
+    *n_out = (int64_t)h[0];
+    return RNNG_OK;
+}
+
 }  // namespace rnng
