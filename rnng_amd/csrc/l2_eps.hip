@@ -220,6 +220,7 @@ struct MfmaParams {
     float negc0;      // -C0 as float (|negc0| >= C0)
     int int_mode;     // 1: exact integer filter
     double c0;        // C0 in double, for reporting
+    float c0p[3];     // bf16 pieces with c0p[0] + c0p[1] + c0p[2] >= C0
 };
 
 // Per-dimension sums, minima and maxima, and the all-integer flag.
@@ -279,12 +280,14 @@ __device__ __forceinline__ float wave_max(float v) {
 // contiguous loads, each point's norm and conversion error is reduced by four
 // lanes, then the x~ and/or y~ fragments (either output may be null) are
 // written 16 bytes per lane, contiguous within each 1 KB fragment.
-//   slot       d      d+1   d+2   d+3   d+4   d+5
-//   x~ real    n1     n2    n3    1     1     1
-//   y~ real    1      1     1     n1    n2    n3
-//   x~ pad     2^100  0     0     1     0     0
-//   y~ pad     1      0     0     2^100 0     0
-// so any pair with a padding point has acc >= 2^100 - C0 > 0.
+//   slot       d      d+1   d+2   d+3   d+4   d+5   d+6..d+8
+//   x~ real    n1     n2    n3    1     1     1     1   1   1
+//   y~ real    1      1     1     n1    n2    n3    -c1 -c2 -c3
+//   x~ pad     2^100  0     0     1     0     0     0   0   0
+//   y~ pad     1      0     0     2^100 0     0     0   0   0
+// (c1 + c2 + c3 >= C0, written by k_patch_c0 once C0 is known), so a real
+// pair accumulates |x~|^2 + |y~|^2 - 2 x~.y~ - C0 from 0, and any pair with a
+// padding point has acc >= 2^100 - C0 > 0.
 // idmap (may be null): padded position -> point index (-1 = padding); without
 // it position p holds point p for p < n.
 constexpr int PP = 64;   // points per prep workgroup
@@ -388,6 +391,9 @@ __global__ __launch_bounds__(256) void k_prep_frags(
                     xf = (k == d + 3) ? 1.0f : 0.f;
                     yf = (k == d + 3) ? big : 0.f;
                 }
+            } else if (k < d + 9) {
+                // constant slots: x~ = 1, y~ = -C0 pieces (written by k_patch_c0)
+                xf = real ? 1.0f : 0.0f;
             }
             xv[e] = f32_to_bf16_rn(xf);
             yv[e] = f32_to_bf16_rn(yf);
@@ -431,8 +437,9 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
     if (isint && maxdev <= 256.0 && (double)d * R * R < two24 &&
         (double)Tb + 4.0 * maxN + 2.0 < two24) {
         // All reference partial sums and all MFMA partial sums are integers
-        // below 2^24: both computations are exact, so acc = |x - y|^2 - C0.
-        c0 = floor((double)Tb) + 0.5;
+        // below 2^24: both computations are exact, so acc = |x - y|^2 - C0,
+        // and with the integer C0 = floor(Tb) + 1, acc < 0 <=> |x - y|^2 <= Tb.
+        c0 = floor((double)Tb) + 1.0;
         int_mode = 1;
     } else {
         const double u = 1.0 / two24;
@@ -449,6 +456,43 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
     out->negc0 = -f;
     out->int_mode = int_mode;
     out->c0 = c0;
+    // Three bf16 pieces of C0 (exact for integers below 2^24); the last one is
+    // rounded up so that the pieces never sum below C0.
+    double r = c0;
+    float pc[3];
+    for (int q = 0; q < 2; ++q) {
+        pc[q] = bf16_to_f32(f32_to_bf16_rn((float)r));
+        r -= (double)pc[q];
+    }
+    float l = bf16_to_f32(f32_to_bf16_rn((float)r));
+    if ((double)l < r) {
+        uint32_t u = __float_as_uint(l);
+        // next bf16 toward +inf
+        if (l >= 0.0f) u += 0x10000u; else u -= 0x10000u;
+        l = __uint_as_float(u & 0xffff0000u);
+    }
+    pc[2] = l;
+    out->c0p[0] = pc[0];
+    out->c0p[1] = pc[1];
+    out->c0p[2] = pc[2];
+}
+
+// Writes -C0 into the three constant slots d+6..d+8 of every real point's
+// y-form fragment (the x-form holds 1.0 there), so that every accumulator
+// starts from 0 and still sums to |x~|^2 + |y~|^2 - 2 x~.y~ - C0.
+__global__ void k_patch_c0(uint16_t* __restrict__ yb, int n, int n_pad, int d, int KS,
+                           const int* __restrict__ idmap, const MfmaParams* __restrict__ prm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    const bool real = idmap ? idmap[i] >= 0 : i < n;
+    if (!real) return;
+    const int t = i >> 5, l = i & 31;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int k = d + 6 + m;
+        const int st = k >> 4, h = (k >> 3) & 1, e = k & 7;
+        yb[((size_t)(t * KS + st) * 64 + l + 32 * h) * 8 + e] = f32_to_bf16_rn(-prm->c0p[m]);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -500,7 +544,6 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
-    const float negc0 = prm->negc0;
     // column tiles beyond wi.z read a valid tile (wi.z - 1) and are masked out
     const int last = wi.z - 1;
 
@@ -547,7 +590,7 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[cc][r][q] = negc0;
+                for (int q = 0; q < 16; ++q) acc[cc][r][q] = 0.0f;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -567,39 +610,54 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
                 for (int q = 0; q < 16; ++q) o |= __float_as_uint(acc[cc][r][q]);
 
         if (__any((int)o < 0)) {
-            // Rare path: one ballot per accumulator register; only registers
-            // holding a candidate reserve output slots (one atomic per
-            // register).  Per-lane bases plus compile-time offsets keep the
-            // row indices out of scalar registers.
+            // Rare path: sign bits -> one mask per lane and column tile, then a
+            // rolled loop with one ballot per accumulator register, staged
+            // through LDS (one LDS atomic per ballot).
             const int rowb = rt0 * 32 + 4 * (lane >> 5);
 #pragma unroll
             for (int cc = 0; cc < CT; ++cc) {
-                const int col = (ct + cc) * 32 + (lane & 31);
-                const int lim = (ct + cc < wi.z && col < n_cols)
-                                    ? (self_join ? min(n_rows, col) : n_rows) : 0;
+                unsigned long long mask = 0;
 #pragma unroll
                 for (int r = 0; r < RT; ++r)
 #pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const int row = rowb + (r * 32 + 8 * (q >> 2) + (q & 3));
-                        const bool p = acc[cc][r][q] < 0.0f && row < lim;
-                        const unsigned long long m = __ballot(p);
-                        if (m) {
-                            unsigned slot = 0;
-                            if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
-                            slot = __shfl(slot, 0, 64) +
-                                   __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                            if (p) {
-                                if (slot < LSTAGE) {
-                                    stage[slot] = make_int2(row, col);
-                                } else {
-                                    const unsigned long long g = atomicAdd(counter, 1ull);
-                                    if (g < cap) cand[g] = make_int2(row, col);
-                                }
+                    for (int q = 0; q < 16; ++q)
+                        mask |= (unsigned long long)(__float_as_uint(acc[cc][r][q]) >> 31) << (r * 16 + q);
+                const int col = (ct + cc) * 32 + (lane & 31);
+                const int lim = (ct + cc < wi.z && col < n_cols)
+                                    ? (self_join ? min(n_rows, col) : n_rows) : 0;
+                // only the registers with a sign bit set in some lane are visited
+                unsigned long long any = mask;
+#pragma unroll
+                for (int o2 = 1; o2 < 64; o2 <<= 1) any |= __shfl_xor(any, o2, 64);
+                {
+                    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)any);
+                    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(any >> 32));
+                    any = ((unsigned long long)hi << 32) | lo;
+                }
+#pragma unroll 1
+                while (any) {
+                    const int rq = __builtin_ctzll(any);
+                    any &= any - 1;
+                    const int q = rq & 15;
+                    const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
+                    const bool p = ((mask >> rq) & 1ull) && row < lim;
+                    const unsigned long long m = __ballot(p);
+                    if (m) {
+                        unsigned slot = 0;
+                        if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
+                        slot = __shfl(slot, 0, 64) +
+                               __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                        if (p) {
+                            if (slot < LSTAGE) {
+                                stage[slot] = make_int2(row, col);
+                            } else {
+                                const unsigned long long g = atomicAdd(counter, 1ull);
+                                if (g < cap) cand[g] = make_int2(row, col);
                             }
                         }
                     }
+                }
             }
         }
         if (USE_LDS) {
@@ -615,6 +673,151 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
 #pragma unroll
             for (int s = 0; s < KS; ++s) b[USE_LDS ? 0 : s] = pre[s];
         }
+    }
+    __syncthreads();
+    const unsigned nst = min(scount, (unsigned)LSTAGE);
+    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
+    __syncthreads();
+    for (unsigned i = tid; i < nst; i += 256)
+        if (sbase + i < cap) cand[sbase + i] = stage[i];
+}
+
+// Group variant: the workgroup loads G column tiles per step straight from
+// global memory into LDS (global_load_lds_dwordx4: no staging registers),
+// double buffered, one barrier per G tiles.  Each wave holds its two row
+// tiles' fragments in registers and runs the G tiles from LDS.  The two
+// buffers are distinct LDS objects and the loop is unrolled by two, so the
+// compiler can see that reads of one buffer never alias the asynchronous
+// loads into the other and does not wait for those loads before reading.
+template <int KS, int G>
+__device__ __forceinline__ void filter_g_load(const bf16x8_t* __restrict__ yb, bf16x8_t* dst, int ct0,
+                                              int last, int wave, int lane) {
+    constexpr int NJ = G * KS;
+#pragma unroll
+    for (int jj = 0; jj < (NJ + 3) / 4; ++jj) {
+        const int j = jj * 4 + wave;
+        if (j >= NJ) break;
+        const int tile = min(ct0 + j / KS, last);
+        const bf16x8_t* src = yb + ((size_t)tile * KS + (j % KS)) * 64 + lane;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(dst + j * 64), 16, 0, 0);
+    }
+}
+
+template <int KS, int G, int RT>
+__device__ __forceinline__ void filter_g_compute(
+    const bf16x8_t* bsrc, const bf16x8_t (&a)[RT][KS], int ct, int zend, int lane, int rowb,
+    int n_rows, int n_cols, int self_join, int2* stage, unsigned* scount, int2* __restrict__ cand,
+    unsigned long long cap, unsigned long long* __restrict__ counter) {
+#pragma unroll
+    for (int cc = 0; cc < G; ++cc) {
+        f32x16_t acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const bf16x8_t bb = bsrc[(cc * KS + s) * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+                acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[r][s], bb, acc[r], 0, 0, 0);
+        }
+        unsigned o = 0;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) o |= __float_as_uint(acc[r][q]);
+        if (__any((int)o < 0)) {
+            // Rare path: the sign bits become one 32-bit mask per lane, then a
+            // rolled loop emits one ballot per accumulator register, staged
+            // through LDS.  Keeps the rare path's registers out of the hot loop.
+            unsigned mask = 0;
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) mask |= (__float_as_uint(acc[r][q]) >> 31) << (r * 16 + q);
+            const int col = (ct + cc) * 32 + (lane & 31);
+            const int lim = (ct + cc < zend && col < n_cols) ? (self_join ? min(n_rows, col) : n_rows) : 0;
+            // only the registers with a sign bit set in some lane are visited
+            unsigned any = mask;
+#pragma unroll
+            for (int o2 = 1; o2 < 64; o2 <<= 1) any |= __shfl_xor(any, o2, 64);
+            any = __builtin_amdgcn_readfirstlane(any);
+#pragma unroll 1
+            while (any) {
+                const int rq = __builtin_ctz(any);
+                any &= any - 1;
+                const int q = rq & 15;
+                const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
+                const bool p = ((mask >> rq) & 1u) && row < lim;
+                const unsigned long long m = __ballot(p);
+                if (m) {
+                    unsigned slot = 0;
+                    if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
+                    slot = __shfl(slot, 0, 64) +
+                           __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    if (p) {
+                        if (slot < LSTAGE) {
+                            stage[slot] = make_int2(row, col);
+                        } else {
+                            const unsigned long long g = atomicAdd(counter, 1ull);
+                            if (g < cap) cand[g] = make_int2(row, col);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int KS, int G>
+__global__ __launch_bounds__(256) void k_l2_mfma_filter_g(
+    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb,
+    const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    const MfmaParams* __restrict__ prm, int2* __restrict__ cand, unsigned long long cap,
+    unsigned long long* __restrict__ counter) {
+    constexpr int RT = 2;
+    constexpr int NCH = KS * 64;       // fragments per column tile
+    __shared__ bf16x8_t bsA[G * NCH];
+    __shared__ bf16x8_t bsB[G * NCH];
+    __shared__ int2 stage[LSTAGE];
+    __shared__ unsigned scount;
+    __shared__ unsigned long long sbase;
+    const int4 wi = work[blockIdx.x];
+    if (wi.y >= wi.z) return;   // empty item: no memory is touched
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    if (tid == 0) scount = 0u;
+    const int rt0 = wi.x * (4 * RT) + wave * RT;
+    const int last = wi.z - 1;
+
+    filter_g_load<KS, G>(yb, bsA, wi.y, last, wave, lane);
+    bf16x8_t a[RT][KS];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
+    const int rowb = rt0 * 32 + 4 * (lane >> 5);
+
+    int ct = wi.y;
+    for (;;) {
+        __builtin_amdgcn_s_waitcnt(0x3f70);   // vmcnt(0): this wave's share of the group landed
+        __syncthreads();                        // every share landed; bsB is no longer read
+        if (ct + G < wi.z) filter_g_load<KS, G>(yb, bsB, ct + G, last, wave, lane);
+        filter_g_compute<KS, G, RT>(bsA, a, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
+                                    &scount, cand, cap, counter);
+        ct += G;
+        if (ct >= wi.z) break;
+        __builtin_amdgcn_s_waitcnt(0x3f70);
+        __syncthreads();                        // bsA is no longer read
+        if (ct + G < wi.z) filter_g_load<KS, G>(yb, bsA, ct + G, last, wave, lane);
+        filter_g_compute<KS, G, RT>(bsB, a, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
+                                    &scount, cand, cap, counter);
+        ct += G;
+        if (ct >= wi.z) break;
     }
     __syncthreads();
     const unsigned nst = min(scount, (unsigned)LSTAGE);
@@ -805,14 +1008,16 @@ struct FilterVariant {
     bool lds;
     int ct;
 };
+// Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4.
 constexpr FilterVariant kVariants[] = {{2, false, 1}, {2, true, 1}, {4, true, 1}, {3, true, 1},
-                                       {2, true, 2}, {1, true, 4}, {1, true, 2}};
-constexpr int kNumVariants = 7;
+                                       {2, true, 2}, {1, true, 4}, {1, true, 2},
+                                       {2, true, 1}, {2, true, 2}, {2, true, 3}, {2, true, 4}};
+constexpr int kNumVariants = 11;
 int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
 
 int filter_variant() {
     if (g_variant < 0) {
-        g_variant = 0;
+        g_variant = 1;
         if (const char* e = getenv("RNNG_L2_VARIANT")) {
             const int v = atoi(e);
             if (v >= 0 && v < kNumVariants) g_variant = v;
@@ -901,8 +1106,23 @@ void launch_filter(Workspace& ws, int n_rows, int n_cols, int self_join) {
         ws.counters.as<unsigned long long>());
 }
 
+template <int KS, int G>
+void launch_filter_g(Workspace& ws, int n_rows, int n_cols, int self_join) {
+    k_l2_mfma_filter_g<KS, G><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+        ws.xa.as<bf16x8_t>(), ws.yb.as<bf16x8_t>(), ws.work.as<int4>(), n_rows, n_cols, self_join,
+        ws.params.as<MfmaParams>(), ws.cand.as<int2>(), ws.cand_cap,
+        ws.counters.as<unsigned long long>());
+}
+
 template <int KS>
 void launch_filter_v(Workspace& ws, int variant, int n_rows, int n_cols, int self_join) {
+    switch (variant) {
+        case 7: launch_filter_g<KS, 1>(ws, n_rows, n_cols, self_join); return;
+        case 8: launch_filter_g<KS, 2>(ws, n_rows, n_cols, self_join); return;
+        case 9: launch_filter_g<KS, 3>(ws, n_rows, n_cols, self_join); return;
+        case 10: launch_filter_g<KS, 4>(ws, n_rows, n_cols, self_join); return;
+        default: break;
+    }
     switch (variant) {
         case 0: launch_filter<KS, 2, false>(ws, n_rows, n_cols, self_join); break;
         case 1: launch_filter<KS, 2, true>(ws, n_rows, n_cols, self_join); break;
@@ -926,7 +1146,7 @@ int dispatch_filter(Workspace& ws, int KS, int variant, int n_rows, int n_cols, 
         case 8: launch_filter_v<8>(ws, variant, n_rows, n_cols, self_join); break;
         case 9: launch_filter_v<9>(ws, variant, n_rows, n_cols, self_join); break;
         default:
-            set_error("MFMA mode supports d <= 138");
+            set_error("MFMA mode supports d <= 135");
             return RNNG_ERR_UNSUPPORTED;
     }
     RNNG_HIP_TRY(hipGetLastError());
@@ -984,9 +1204,9 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
              rnng_graph_s* g, rnng_stats* st) {
     Workspace& ws = g_ws;
     const int nx = (int)A->n, ny = (int)B->n, d = A->d;
-    const int KS = (d + 6 + 15) / 16;
+    const int KS = (d + 9 + 15) / 16;
     if (KS > 9) {
-        set_error("MFMA mode supports d <= 138");
+        set_error("MFMA mode supports d <= 135");
         return RNNG_ERR_UNSUPPORTED;
     }
     const float T = sq_threshold_host(eps);
@@ -1079,6 +1299,12 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
                                          ws.params.as<MfmaParams>());
     RNNG_HIP_TRY(hipGetLastError());
+    {
+        const int npy = self_join ? std::max(npad_x, npad_y) : npad_y;
+        k_patch_c0<<<blocks_for(npy, 256), 256, 0, stream()>>>(ws.yb.as<uint16_t>(), self_join ? nx : ny, npy, d,
+                                                                KS, nullptr, ws.params.as<MfmaParams>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
 
     const bool vec4 = (d % 4) == 0;
