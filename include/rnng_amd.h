@@ -52,6 +52,10 @@ typedef struct rnng_stats {
     double filter_ms;      /* distance kernel (MFMA filter or plain kernel) */
     double recheck_ms;     /* FP32 recheck + band check (MFMA mode) */
     double total_ms;       /* all device work of the call */
+    int64_t ghost_tests;      /* (point, center) ghost tests evaluated (Voronoi) */
+    int64_t ghost_candidates; /* (cell, point) pairs passing the center test */
+    int64_t ghosts;           /* candidates with a member within eps: the ghosts */
+    double ghost_ms;          /* ghost center test (Voronoi) */
 } rnng_stats;
 
 /* Opaque handles. */
@@ -169,6 +173,69 @@ int rnng_filtration_size(rnng_filtration_t f, int64_t *n, int64_t *per_dim);
 int rnng_filtration_copy(rnng_filtration_t f, float *value, int64_t *id, int64_t cap);
 
 int rnng_filtration_destroy(rnng_filtration_t f);
+
+/* ------------------------------------------------------------------------ */
+/* Voronoi landmark cells and ghost points                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rnng_voronoi_s *rnng_voronoi_t;   /* cells of a dataset (borrowed) */
+
+/* Knuth's selection sampling of `size` ids from [0, range), ascending, with
+ * the reference's engine and distribution (default_random_engine(seed),
+ * uniform_real_distribution<double>(0, nextafter(1, max))); seed < 0 draws
+ * one from std::random_device.  Host only.
+ * Replaces: selection_sample, include/utils.hpp:1-29. */
+int rnng_selection_sample(int64_t range, int64_t size, int64_t *sample, int32_t seed);
+
+/* Nearest center of every point, reference-order FP32 distances, ties to the
+ * later center; cell[i] in [0, m), dist[i] = that distance.  Host buffers.
+ * Replaces: VoronoiDiagram ctor, include/point.hpp:535-559. */
+int rnng_voronoi_assign_f32(const float *points, int64_t n, const float *centers, int64_t m,
+                            int32_t d, int64_t *cell, float *dist);
+
+/* Voronoi cells of ds around the points centers[0..m) (cell c = centers[c]):
+ * assignment as rnng_voronoi_assign_f32, then the points grouped by cell in
+ * ascending id with the center moved to the front.  ds must outlive the
+ * handle.
+ * Replaces: VoronoiDiagram ctor + coalesce_cells, include/point.hpp:535-758
+ * (one GPU holds every cell, so there is no cell exchange). */
+int rnng_voronoi_create(rnng_dataset_t ds, const int64_t *centers, int64_t m, rnng_voronoi_t *out);
+
+/* As rnng_voronoi_create with m farthest-point landmarks: center 0 is point
+ * `first`, center k+1 the point farthest (exact FP32 distance, ties to the
+ * smaller id) from centers 0..k.  The landmark driver samples centers with
+ * selection_sample instead (drivers/landmark.cpp:110-127); the edge set does
+ * not depend on the centers. */
+int rnng_voronoi_create_fps(rnng_dataset_t ds, int64_t m, int64_t first, rnng_voronoi_t *out);
+
+/* Center point ids in cell order (m). */
+int rnng_voronoi_centers(rnng_voronoi_t v, int64_t *ids);
+
+/* Cell and distance to the own center of every point (either may be NULL). */
+int rnng_voronoi_assignment(rnng_voronoi_t v, int64_t *cell, float *dist);
+
+/* Cell CSR: cell_ptr[m+1], cell_ids[n] (coalesce order; either may be NULL).
+ * Replaces: VoronoiCell membership after coalesce_cells. */
+int rnng_voronoi_cells(rnng_voronoi_t v, int64_t *cell_ptr, int64_t *cell_ids);
+
+/* Epsilon graph through the cells: ghost points (q joins cell b != cell(q)
+ * when L2(center_b, q) <= fl(dist_q + 2 eps) and some member of b is within
+ * eps of q), then every cell's member x member and member x ghost pairs
+ * within eps, deduplicated to undirected edges i < j.  The edge set equals
+ * the all-pairs graph's.  mode must be RNNG_MODE_MFMA (d <= 135).
+ * Replaces: add_ghost_points_systolic (include/point.hpp:760-929), the
+ * per-cell query loop (drivers/landmark.cpp:203-210) and the Graph ctor's
+ * sort/unique (include/graph.hpp:59-116). */
+int rnng_voronoi_graph_build(rnng_voronoi_t v, float eps, int32_t mode, rnng_graph_t *out,
+                             rnng_stats *stats);
+
+/* Ghosts found by the last rnng_voronoi_graph_build: their count, and the
+ * (cell, point) pairs sorted by cell then point into pairs[2*cap].
+ * Replaces: VoronoiCell::ghosts(), include/point.h:120. */
+int rnng_voronoi_num_ghosts(rnng_voronoi_t v, int64_t *n);
+int rnng_voronoi_ghosts(rnng_voronoi_t v, int64_t *pairs, int64_t cap);
+
+int rnng_voronoi_destroy(rnng_voronoi_t v);
 
 /* Largest float T with sqrtf(T) <= eps (-1 when no value qualifies).  The
  * kernels compare squared distances against T; sqrtf is monotone, so

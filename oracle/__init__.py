@@ -55,6 +55,9 @@ def _load():
     lib.orc_ghosts.argtypes = [f32p, C.c_int64, C.c_int64, i64p, f32p, f32p, C.c_int64,
                                C.c_float, C.c_int, vp, C.c_int64]
     lib.orc_ghosts.restype = C.c_int64
+    lib.orc_fps.argtypes = [f32p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int, i64p, i64p,
+                            f32p]
+    lib.orc_fps.restype = None
     lib.orc_binom.argtypes = [C.c_int64, C.c_int64]
     lib.orc_binom.restype = C.c_int64
     lib.orc_simplex_id.argtypes = [i64p, C.c_int64]
@@ -154,6 +157,79 @@ def ghosts(pts, cell, delta, centers, eps, nthreads: int = 0):
     out = np.empty((cnt, 2), np.int64)
     lib.orc_ghosts(pts, n, d, cell, delta, centers, m, np.float32(eps), nthreads, _ptr(out), cnt)
     return out
+
+
+def fps(pts, m: int, first: int = 0, nthreads: int = 0):
+    """Farthest-point landmarks: (centers int64[m], cell int64[n], dist float32[n])."""
+    lib = _load()
+    pts = _f32(pts)
+    n, d = pts.shape
+    centers = np.empty(m, np.int64)
+    cell = np.empty(n, np.int64)
+    dist = np.empty(n, np.float32)
+    lib.orc_fps(pts, n, d, m, first, nthreads, centers, cell, dist)
+    return centers, cell, dist
+
+
+def voronoi_cells(cell, centers, m: int):
+    """Cells in the reference's coalesce order for one rank
+    (include/point.hpp:721-754): members in ascending id, then the center
+    swapped with the first member.  Returns (ptr int64[m+1], ids int64[n])."""
+    cell = np.asarray(cell, dtype=np.int64)
+    order = np.lexsort((np.arange(cell.shape[0]), cell))
+    ptr = np.zeros(m + 1, np.int64)
+    ptr[1:] = np.cumsum(np.bincount(cell, minlength=m))
+    ids = order.astype(np.int64)
+    for c in range(m):
+        lo, hi = ptr[c], ptr[c + 1]
+        seg = ids[lo:hi]
+        hit = np.nonzero(seg == centers[c])[0]
+        if hit.size:
+            k = hit[0]
+            seg[0], seg[k] = seg[k], seg[0]
+    return ptr, ids
+
+
+_MINSTD_M = 2147483647          # minstd_rand0 modulus 2^31 - 1
+_MINSTD_A = 16807               # minstd_rand0 multiplier
+_MINSTD_R = float(_MINSTD_M - 1)  # engine range max - min + 1 (min 1, max 2^31 - 2)
+
+
+def selection_sample(range_: int, size: int, seed: int):
+    """Knuth's Algorithm S as the reference runs it (include/utils.hpp:1-29),
+    restated from the published definitions of its library pieces:
+    std::default_random_engine = minstd_rand0 (x <- 16807 x mod 2^31-1; a
+    seed s becomes s mod (2^31-1), 0 becoming 1), generate_canonical<double,
+    53> over a 2^31-2 range = two draws g1, g2 combined as
+    ((g1-1) + (g2-1) R) / R^2 with R = 2^31-2, all in double, clamped below 1,
+    and uniform_real_distribution(0, b) = canonical * b with
+    b = nextafter(1, +inf).  Python floats are IEEE doubles, so every
+    rounding step is the library's."""
+    import math
+    x = seed % _MINSTD_M
+    if x == 0:
+        x = 1
+    b = math.nextafter(1.0, math.inf)
+    rr = _MINSTD_R * _MINSTD_R
+    out = []
+    pos = 0
+    for i in range(range_):
+        if pos == size:
+            break
+        x = (_MINSTD_A * x) % _MINSTD_M
+        g1 = x
+        x = (_MINSTD_A * x) % _MINSTD_M
+        g2 = x
+        s = float(g1 - 1) * 1.0
+        s = s + float(g2 - 1) * _MINSTD_R
+        c = s / rr
+        if c >= 1.0:
+            c = math.nextafter(1.0, 0.0)
+        u = c * (b - 0.0) + 0.0
+        if float(range_ - i) * u < float(size - pos):
+            out.append(i)
+            pos += 1
+    return np.asarray(out, dtype=np.int64)
 
 
 def binom(n: int, k: int) -> int:

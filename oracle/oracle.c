@@ -244,6 +244,38 @@ void orc_voronoi_assign(const float *pts, int64_t n, const float *centers, int64
 }
 
 /* ------------------------------------------------------------------------ */
+/* Farthest-point landmarks (no reference counterpart: the landmark driver    */
+/* samples centers, drivers/landmark.cpp:110-127).  Center 0 = first; center  */
+/* k+1 = argmax of the distance to the nearest chosen center, ties to the     */
+/* smaller index.  The running (dist, cell) update uses the `<=` rule of      */
+/* include/point.hpp:550-556, so after the last center (cell, dist) is the    */
+/* Voronoi assignment of the centers in selection order.                      */
+/* ------------------------------------------------------------------------ */
+
+void orc_fps(const float *pts, int64_t n, int64_t d, int64_t m, int64_t first, int nthreads,
+             int64_t *centers, int64_t *cell, float *dist)
+{
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    centers[0] = first;
+    for (int64_t k = 0; k < m; ++k) {
+        const float *c = pts + centers[k] * d;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+        for (int64_t i = 0; i < n; ++i) {
+            float dd = orc_l2(c, pts + i * d, d);
+            float md = k == 0 ? FLT_MAX : dist[i];
+            if (dd <= md) { md = dd; cell[i] = k; }
+            dist[i] = md;
+        }
+        if (k + 1 < m) {
+            int64_t bi = 0;
+            for (int64_t i = 1; i < n; ++i)
+                if (dist[i] > dist[bi]) bi = i;
+            centers[k + 1] = bi;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Ghost criterion: include/point.hpp:894-918                                */
 /* ------------------------------------------------------------------------ */
 

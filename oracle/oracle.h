@@ -25,6 +25,10 @@
  *                            definition on tiny inputs
  *   orc_voronoi_assign       brute-force argmin with the later-wins tie rule,
  *                            planted equal-distance ties
+ *   orc_fps                  a numpy restatement on small inputs (farthest
+ *                            point by explicit distance matrix), the
+ *                            by-product assignment equal to orc_voronoi_assign
+ *                            of the chosen centers
  *   orc_ghosts               the covering property: the union of cell-local
  *                            pairs equals the brute-force graph
  *   orc_binom / orc_simplex_id  closed-form C(n,k), the combinatorial
@@ -83,6 +87,11 @@ int64_t orc_edit_eps_pairs(const char *atoms, const int64_t *offsets, int64_t n,
  * if dist <= best then (best, cell) = (dist, c).  Ties go to the later center. */
 void orc_voronoi_assign(const float *pts, int64_t n, const float *centers, int64_t m,
                         int64_t d, int nthreads, int64_t *cell, float *dist);
+
+/* Farthest-point landmarks; (cell, dist) = Voronoi assignment of the chosen
+ * centers in selection order. */
+void orc_fps(const float *pts, int64_t n, int64_t d, int64_t m, int64_t first, int nthreads,
+             int64_t *centers, int64_t *cell, float *dist);
 
 /* Ghost criterion of include/point.hpp:894-918 (single rank): point q with
  * own-center distance delta_q becomes a ghost of cell c != cell(q) iff c has

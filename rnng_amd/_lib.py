@@ -44,6 +44,10 @@ class Stats(C.Structure):
         ("filter_ms", C.c_double),
         ("recheck_ms", C.c_double),
         ("total_ms", C.c_double),
+        ("ghost_tests", C.c_int64),
+        ("ghost_candidates", C.c_int64),
+        ("ghosts", C.c_int64),
+        ("ghost_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -81,6 +85,18 @@ _SIGS = {
     "rnng_edit_graph_build": (C.c_int, [_vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_edit_eps_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.c_float, _vp, _vp, C.c_int64, _i64p,
                                       C.POINTER(Stats)]),
+    "rnng_selection_sample": (C.c_int, [C.c_int64, C.c_int64, _vp, C.c_int32]),
+    "rnng_voronoi_assign_f32": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, C.c_int32, _vp, _vp]),
+    "rnng_voronoi_create": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(_vp)]),
+    "rnng_voronoi_create_fps": (C.c_int, [_vp, C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "rnng_voronoi_centers": (C.c_int, [_vp, _vp]),
+    "rnng_voronoi_assignment": (C.c_int, [_vp, _vp, _vp]),
+    "rnng_voronoi_cells": (C.c_int, [_vp, _vp, _vp]),
+    "rnng_voronoi_graph_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp),
+                                           C.POINTER(Stats)]),
+    "rnng_voronoi_num_ghosts": (C.c_int, [_vp, _i64p]),
+    "rnng_voronoi_ghosts": (C.c_int, [_vp, _vp, C.c_int64]),
+    "rnng_voronoi_destroy": (C.c_int, [_vp]),
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),

@@ -28,10 +28,7 @@
 #include <numeric>
 #include <vector>
 
-#include "internal.hpp"
-
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
+#include "mfma.hpp"
 
 namespace rnng {
 
@@ -216,12 +213,6 @@ constexpr int NSTAT = 8;
 // stat slots (unsigned, ordered-float or flags) in ds_stat[]
 enum { ST_INT = 0, ST_MAXE = 1, ST_MAXN = 2, ST_MAXRES = 3, ST_MAXDEV = 4 };
 
-struct MfmaParams {
-    float negc0;      // -C0 as float (|negc0| >= C0)
-    int int_mode;     // 1: exact integer filter
-    double c0;        // C0 in double, for reporting
-    float c0p[3];     // bf16 pieces with c0p[0] + c0p[1] + c0p[2] >= C0
-};
 
 // Per-dimension sums, minima and maxima, and the all-integer flag.
 __global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ X, int n, int d,
@@ -510,7 +501,6 @@ __global__ void k_patch_c0(uint16_t* __restrict__ yb, int n, int n_pad, int d, i
 // Output row/column of accumulator register r in lane l (32x32 MFMA):
 //   row = 8*(r/4) + 4*(l/32) + r%4, column = l%32.
 
-constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
 
 template <int KS, int RT, bool USE_LDS, int CT = 1>
 __global__ __launch_bounds__(256) void k_l2_mfma_filter(
@@ -1031,18 +1021,6 @@ Workspace g_ws;
 int64_t g_cand_hint = 0;   // initial candidate-buffer size (0: default)
 
 
-int launch_fill_u64(unsigned long long* p, int n, unsigned long long v) {
-    k_fill_u64<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
-    RNNG_HIP_TRY(hipGetLastError());
-    return RNNG_OK;
-}
-
-int launch_fill_u32(unsigned* p, int n, unsigned v) {
-    k_fill_u32<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
-    RNNG_HIP_TRY(hipGetLastError());
-    return RNNG_OK;
-}
-
 // Row blocks of 8 tiles (256 rows) x column chunks of CH tiles, chunk-major
 // so that the workgroups resident at one time share column fragments in L2.
 constexpr int CH = 256;
@@ -1307,7 +1285,6 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     }
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
 
-    const bool vec4 = (d % 4) == 0;
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
@@ -1328,19 +1305,8 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         RNNG_TRY(ws.cw.reserve(std::max<size_t>(1, ncand) * sizeof(float)));
         if ((int64_t)ncand > g->cap) RNNG_TRY(edge_buffers(g, (int64_t)ncand));
         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
-        if (ncand > 0) {
-            const unsigned rgrid = (unsigned)std::min<unsigned long long>(2048, (ncand + 63) / 64);
-            const size_t rlds = (size_t)4 * 2 * RC * recheck_ld(d) * sizeof(float);
-            if (vec4)
-                k_l2_recheck<true><<<rgrid, 256, rlds, stream()>>>(
-                    A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, ws.cflag.as<uint8_t>(),
-                    ws.cw.as<float>(), ws.counters.as<unsigned long long>());
-            else
-                k_l2_recheck<false><<<rgrid, 256, rlds, stream()>>>(
-                    A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, T, eps, ws.cflag.as<uint8_t>(),
-                    ws.cw.as<float>(), ws.counters.as<unsigned long long>());
-            RNNG_HIP_TRY(hipGetLastError());
-        }
+        RNNG_TRY(launch_recheck(A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, ncand, T, eps,
+                                ws.cflag.as<uint8_t>(), ws.cw.as<float>(), ws.counters.as<unsigned long long>()));
         int64_t ne = 0;
         RNNG_TRY(compact_edges(ws.cand.as<int2>(), ws.cw.as<float>(), ws.cflag.as<uint8_t>(), (int64_t)ncand,
                                g->ij.as<int2>(), g->w.as<float>(), &ne));
@@ -1378,6 +1344,79 @@ int check_dataset(const rnng_dataset_s* ds) {
 }
 
 }  // namespace
+
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    k_fill_u64<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+int launch_fill_u32(unsigned* p, int n, unsigned v) {
+    k_fill_u32<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+int launch_recheck(const float* X, const float* Y, int d, const int2* cand, unsigned long long ccap,
+                   unsigned long long ncand, float T, float eps, uint8_t* flag, float* cw,
+                   unsigned long long* counters) {
+    if (ncand == 0) return RNNG_OK;
+    const unsigned rgrid = (unsigned)std::min<unsigned long long>(2048, (ncand + 63) / 64);
+    const size_t rlds = (size_t)4 * 2 * RC * recheck_ld(d) * sizeof(float);
+    if ((d % 4) == 0)
+        k_l2_recheck<true><<<rgrid, 256, rlds, stream()>>>(X, Y, d, cand, ccap, T, eps, flag, cw, counters);
+    else
+        k_l2_recheck<false><<<rgrid, 256, rlds, stream()>>>(X, Y, d, cand, ccap, T, eps, flag, cw, counters);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
+                        int n_rowpad, const int* colmap, int n_colpad) {
+    const int KS = mfma_ksteps(d);
+    if (KS > kMaxKS) {
+        set_error("MFMA mode supports d <= 135");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    P.KS = KS;
+    RNNG_TRY(P.xa.reserve(std::max<size_t>(16, (size_t)n_rowpad * KS * 16 * 2)));
+    RNNG_TRY(P.yb.reserve(std::max<size_t>(16, (size_t)n_colpad * KS * 16 * 2)));
+    RNNG_TRY(P.colsum.reserve(sizeof(double) * d));
+    RNNG_TRY(P.colmin.reserve(sizeof(unsigned) * d));
+    RNNG_TRY(P.colmax.reserve(sizeof(unsigned) * d));
+    RNNG_TRY(P.stat.reserve(sizeof(unsigned) * NSTAT));
+    RNNG_TRY(P.range.reserve(sizeof(unsigned)));
+    RNNG_TRY(P.mu.reserve(sizeof(float) * d));
+    RNNG_TRY(P.params.reserve(sizeof(MfmaParams)));
+    k_fill_u64<<<blocks_for(d, 256), 256, 0, stream()>>>((unsigned long long*)P.colsum.ptr, d, 0ull);
+    RNNG_TRY(launch_fill_u32(P.colmin.as<unsigned>(), d, 0xff800000u));
+    RNNG_TRY(launch_fill_u32(P.colmax.as<unsigned>(), d, 0x007fffffu));
+    RNNG_TRY(launch_fill_u32(P.stat.as<unsigned>(), NSTAT, 0u));
+    RNNG_TRY(launch_fill_u32(P.stat.as<unsigned>() + ST_INT, 1, 1u));
+    RNNG_TRY(launch_fill_u32(P.stat.as<unsigned>() + ST_MAXDEV, 1, 0x80000000u));
+    RNNG_TRY(launch_fill_u32(P.range.as<unsigned>(), 1, 0x80000000u));
+    if (n > 0)
+        k_colstats<<<blocks_for(n, 256), 256, 0, stream()>>>(X, n, d, P.colsum.as<double>(),
+                                                             P.colmin.as<unsigned>(), P.colmax.as<unsigned>(),
+                                                             P.stat.as<unsigned>());
+    k_center<<<blocks_for(d, 64), 64, 0, stream()>>>(P.colsum.as<double>(), P.colmin.as<unsigned>(),
+                                                      P.colmax.as<unsigned>(), n, d, P.mu.as<float>(),
+                                                      P.stat.as<unsigned>(), P.range.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(), nullptr,
+                         P.stat.as<unsigned>()));
+    RNNG_TRY(launch_prep(X, n, n_colpad, d, KS, P.mu.as<float>(), colmap, nullptr, P.yb.as<uint16_t>(),
+                         P.stat.as<unsigned>()));
+    k_finalize_c0<<<1, 1, 0, stream()>>>(P.stat.as<unsigned>(), P.range.as<unsigned>(), d, KS, Tb,
+                                         P.params.as<MfmaParams>());
+    RNNG_HIP_TRY(hipGetLastError());
+    if (n_colpad > 0) {
+        k_patch_c0<<<blocks_for(n_colpad, 256), 256, 0, stream()>>>(P.yb.as<uint16_t>(), n, n_colpad, d, KS, colmap,
+                                                                    P.params.as<MfmaParams>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    return RNNG_OK;
+}
 
 int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
                    int32_t mode, rnng_graph_s* g, rnng_stats* st) {

@@ -1,0 +1,53 @@
+// Shared definitions of the BF16 MFMA candidate filter (l2_eps.hip and the
+// batched Voronoi-cell kernel in voronoi.hip).
+#pragma once
+
+#include "internal.hpp"
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+namespace rnng {
+
+struct MfmaParams {
+    float negc0;      // -C0 as float (|negc0| >= C0)
+    int int_mode;     // 1: exact integer filter
+    double c0;        // C0 in double, for reporting
+    float c0p[3];     // bf16 pieces with c0p[0] + c0p[1] + c0p[2] >= C0
+};
+
+// Fragment buffers and the statistics that size C0.
+struct MfmaPrep {
+    LocalBuf xa, yb;                                   // bf16 fragments (x-form rows, y-form columns)
+    LocalBuf colsum, colmin, colmax, stat, range, mu;  // centering and error statistics
+    LocalBuf params;                                   // MfmaParams
+    int KS = 0;                                      // 16-wide k-steps per fragment
+};
+
+// Number of 16-wide k-steps for dimension d (d data dims + 9 augmented slots).
+inline int mfma_ksteps(int d) { return (d + 9 + 15) / 16; }
+constexpr int kMaxKS = 9;     // d <= 135
+
+// Builds the x-form fragments of the padded row positions (rowmap[p] = point
+// index or -1) and the y-form fragments of the padded column positions
+// (colmap), centred and scaled over all n points of X, with C0 for the
+// squared band radius Tb written into the y-form constant slots.
+int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
+                        int n_rowpad, const int* colmap, int n_colpad);
+
+constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
+
+// Exact FP32 recheck of candidate pairs (ids into X and Y): flag[c] = 1 for an
+// edge (reference-order squared distance <= T), cw[c] = its distance; pairs in
+// the |w - eps| <= 1e-5 eps band are counted in counters[2] and re-decided in
+// FP64 (disagreements in counters[3]).  The candidate count is read from
+// counters[0] on the device (capped at ccap); ncand only sizes the grid.
+int launch_recheck(const float* X, const float* Y, int d, const int2* cand, unsigned long long ccap,
+                   unsigned long long ncand, float T, float eps, uint8_t* flag, float* cw,
+                   unsigned long long* counters);
+
+// Fill kernels usable under stream capture (no memset nodes).
+int launch_fill_u64(unsigned long long* p, int n, unsigned long long v);
+int launch_fill_u32(unsigned* p, int n, unsigned v);
+
+}  // namespace rnng
