@@ -52,10 +52,10 @@ typedef struct rnng_stats {
     double filter_ms;      /* distance kernel (MFMA filter or plain kernel) */
     double recheck_ms;     /* FP32 recheck + band check (MFMA mode) */
     double total_ms;       /* all device work of the call */
-    int64_t ghost_tests;      /* (point, center) ghost tests evaluated (Voronoi) */
-    int64_t ghost_candidates; /* (cell, point) pairs passing the center test */
-    int64_t ghosts;           /* candidates with a member within eps: the ghosts */
-    double ghost_ms;          /* ghost center test (Voronoi) */
+    int64_t ghost_tests;      /* Voronoi: exact center tests of cross-cell edges */
+    int64_t ghost_candidates; /* Voronoi: center tests passed (with repeats) */
+    int64_t ghosts;           /* Voronoi: distinct (cell, point) ghosts */
+    double ghost_ms;          /* Voronoi: cell adjacency and cell-ordered layout */
 } rnng_stats;
 
 /* Opaque handles. */

@@ -1055,16 +1055,8 @@ int build_work(Workspace& ws, int n_rowblocks, int rb_tiles, int n_coltiles, int
             list.push_back(make_int4(rb, lo, c1, 0));
         }
     }
-    if (remap) {
-        const size_t per = (list.size() + 7) / 8;
-        ws.work_host.assign(per * 8, make_int4(0, 0, 0, 0));   // empty items run no tile
-        for (size_t b = 0; b < per * 8; ++b) {
-            const size_t src = (b % 8) * per + b / 8;
-            if (src < list.size()) ws.work_host[b] = list[src];
-        }
-    } else {
-        ws.work_host.swap(list);
-    }
+    if (remap) xcd_permute(list);
+    ws.work_host.swap(list);
     RNNG_TRY(ws.work.reserve(std::max<size_t>(1, ws.work_host.size()) * sizeof(int4)));
     RNNG_HIP_TRY(hipMemcpyAsync(ws.work.ptr, ws.work_host.data(), ws.work_host.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, stream()));
@@ -1077,58 +1069,51 @@ int build_work(Workspace& ws, int n_rowblocks, int rb_tiles, int n_coltiles, int
 }
 
 template <int KS, int RT, bool LDS, int CT = 1>
-void launch_filter(Workspace& ws, int n_rows, int n_cols, int self_join) {
-    k_l2_mfma_filter<KS, RT, LDS, CT><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-        ws.xa.as<bf16x8_t>(), ws.yb.as<bf16x8_t>(), ws.work.as<int4>(), n_rows, n_cols, self_join,
-        ws.params.as<MfmaParams>(), ws.cand.as<int2>(), ws.cand_cap,
-        ws.counters.as<unsigned long long>());
+void launch_filter(const FilterArgs& a) {
+    k_l2_mfma_filter<KS, RT, LDS, CT><<<a.n_items, 256, 0, stream()>>>(
+        a.xa, a.yb, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
 template <int KS, int G>
-void launch_filter_g(Workspace& ws, int n_rows, int n_cols, int self_join) {
-    k_l2_mfma_filter_g<KS, G><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-        ws.xa.as<bf16x8_t>(), ws.yb.as<bf16x8_t>(), ws.work.as<int4>(), n_rows, n_cols, self_join,
-        ws.params.as<MfmaParams>(), ws.cand.as<int2>(), ws.cand_cap,
-        ws.counters.as<unsigned long long>());
+void launch_filter_g(const FilterArgs& a) {
+    k_l2_mfma_filter_g<KS, G><<<a.n_items, 256, 0, stream()>>>(
+        a.xa, a.yb, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
 template <int KS>
-void launch_filter_v(Workspace& ws, int variant, int n_rows, int n_cols, int self_join) {
+void launch_filter_v(const FilterArgs& a, int variant) {
     switch (variant) {
-        case 7: launch_filter_g<KS, 1>(ws, n_rows, n_cols, self_join); return;
-        case 8: launch_filter_g<KS, 2>(ws, n_rows, n_cols, self_join); return;
-        case 9: launch_filter_g<KS, 3>(ws, n_rows, n_cols, self_join); return;
-        case 10: launch_filter_g<KS, 4>(ws, n_rows, n_cols, self_join); return;
+        case 7: launch_filter_g<KS, 1>(a); return;
+        case 8: launch_filter_g<KS, 2>(a); return;
+        case 9: launch_filter_g<KS, 3>(a); return;
+        case 10: launch_filter_g<KS, 4>(a); return;
         default: break;
     }
     switch (variant) {
-        case 0: launch_filter<KS, 2, false>(ws, n_rows, n_cols, self_join); break;
-        case 1: launch_filter<KS, 2, true>(ws, n_rows, n_cols, self_join); break;
-        case 3: launch_filter<KS, 3, true>(ws, n_rows, n_cols, self_join); break;
-        case 4: launch_filter<KS, 2, true, 2>(ws, n_rows, n_cols, self_join); break;
-        case 5: launch_filter<KS, 1, true, 4>(ws, n_rows, n_cols, self_join); break;
-        case 6: launch_filter<KS, 1, true, 2>(ws, n_rows, n_cols, self_join); break;
-        default: launch_filter<KS, 4, true>(ws, n_rows, n_cols, self_join); break;
+        case 0: launch_filter<KS, 2, false>(a); break;
+        case 1: launch_filter<KS, 2, true>(a); break;
+        case 3: launch_filter<KS, 3, true>(a); break;
+        case 4: launch_filter<KS, 2, true, 2>(a); break;
+        case 5: launch_filter<KS, 1, true, 4>(a); break;
+        case 6: launch_filter<KS, 1, true, 2>(a); break;
+        default: launch_filter<KS, 4, true>(a); break;
     }
 }
 
-int dispatch_filter(Workspace& ws, int KS, int variant, int n_rows, int n_cols, int self_join) {
-    switch (KS) {
-        case 1: launch_filter_v<1>(ws, variant, n_rows, n_cols, self_join); break;
-        case 2: launch_filter_v<2>(ws, variant, n_rows, n_cols, self_join); break;
-        case 3: launch_filter_v<3>(ws, variant, n_rows, n_cols, self_join); break;
-        case 4: launch_filter_v<4>(ws, variant, n_rows, n_cols, self_join); break;
-        case 5: launch_filter_v<5>(ws, variant, n_rows, n_cols, self_join); break;
-        case 6: launch_filter_v<6>(ws, variant, n_rows, n_cols, self_join); break;
-        case 7: launch_filter_v<7>(ws, variant, n_rows, n_cols, self_join); break;
-        case 8: launch_filter_v<8>(ws, variant, n_rows, n_cols, self_join); break;
-        case 9: launch_filter_v<9>(ws, variant, n_rows, n_cols, self_join); break;
-        default:
-            set_error("MFMA mode supports d <= 135");
-            return RNNG_ERR_UNSUPPORTED;
-    }
-    RNNG_HIP_TRY(hipGetLastError());
-    return RNNG_OK;
+FilterArgs ws_args(Workspace& ws, int n_rows, int n_cols, int self_join) {
+    FilterArgs a;
+    a.xa = ws.xa.as<bf16x8_t>();
+    a.yb = ws.yb.as<bf16x8_t>();
+    a.work = ws.work.as<int4>();
+    a.n_items = (unsigned)ws.work_host.size();
+    a.n_rows = n_rows;
+    a.n_cols = n_cols;
+    a.self_join = self_join;
+    a.prm = ws.params.as<MfmaParams>();
+    a.cand = ws.cand.as<int2>();
+    a.cap = ws.cand_cap;
+    a.counter = ws.counters.as<unsigned long long>();
+    return a;
 }
 
 int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
@@ -1288,7 +1273,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        RNNG_TRY(dispatch_filter(ws, KS, variant, nx, ny, self_join ? 1 : 0));
+        RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
         unsigned long long ncand = 0;
         RNNG_HIP_TRY(hipMemcpyAsync(&ncand, ws.counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
@@ -1344,6 +1329,44 @@ int check_dataset(const rnng_dataset_s* ds) {
 }
 
 }  // namespace
+
+int filter_dispatch(const FilterArgs& a, int KS, int variant) {
+    if (variant < 0 || variant >= kNumVariants) {
+        set_error("unknown filter variant");
+        return RNNG_ERR_ARG;
+    }
+    if (a.n_items == 0) return RNNG_OK;
+    switch (KS) {
+        case 1: launch_filter_v<1>(a, variant); break;
+        case 2: launch_filter_v<2>(a, variant); break;
+        case 3: launch_filter_v<3>(a, variant); break;
+        case 4: launch_filter_v<4>(a, variant); break;
+        case 5: launch_filter_v<5>(a, variant); break;
+        case 6: launch_filter_v<6>(a, variant); break;
+        case 7: launch_filter_v<7>(a, variant); break;
+        case 8: launch_filter_v<8>(a, variant); break;
+        case 9: launch_filter_v<9>(a, variant); break;
+        default:
+            set_error("MFMA mode supports d <= 135");
+            return RNNG_ERR_UNSUPPORTED;
+    }
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+int filter_block_tiles(int variant) { return 4 * kVariants[variant].rt; }
+int filter_default_variant() { return filter_variant(); }
+int filter_num_variants() { return kNumVariants; }
+
+void xcd_permute(std::vector<int4>& list) {
+    const size_t per = (list.size() + 7) / 8;
+    std::vector<int4> out(per * 8, make_int4(0, 0, 0, 0));   // empty items run no tile
+    for (size_t b = 0; b < per * 8; ++b) {
+        const size_t src = (b % 8) * per + b / 8;
+        if (src < list.size()) out[b] = list[src];
+    }
+    list.swap(out);
+}
 
 int launch_fill_u64(unsigned long long* p, int n, unsigned long long v) {
     k_fill_u64<<<blocks_for(n, 256), 256, 0, stream()>>>(p, n, v);
@@ -1403,10 +1426,16 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                                                       P.colmax.as<unsigned>(), n, d, P.mu.as<float>(),
                                                       P.stat.as<unsigned>(), P.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
-    RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(), nullptr,
-                         P.stat.as<unsigned>()));
-    RNNG_TRY(launch_prep(X, n, n_colpad, d, KS, P.mu.as<float>(), colmap, nullptr, P.yb.as<uint16_t>(),
-                         P.stat.as<unsigned>()));
+    if (rowmap == colmap && n_rowpad == n_colpad) {
+        // one pass writes both forms
+        RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(),
+                             P.yb.as<uint16_t>(), P.stat.as<unsigned>()));
+    } else {
+        RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(), nullptr,
+                             P.stat.as<unsigned>()));
+        RNNG_TRY(launch_prep(X, n, n_colpad, d, KS, P.mu.as<float>(), colmap, nullptr, P.yb.as<uint16_t>(),
+                             P.stat.as<unsigned>()));
+    }
     k_finalize_c0<<<1, 1, 0, stream()>>>(P.stat.as<unsigned>(), P.range.as<unsigned>(), d, KS, Tb,
                                          P.params.as<MfmaParams>());
     RNNG_HIP_TRY(hipGetLastError());

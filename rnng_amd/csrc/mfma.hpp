@@ -2,6 +2,8 @@
 // batched Voronoi-cell kernel in voronoi.hip).
 #pragma once
 
+#include <vector>
+
 #include "internal.hpp"
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -36,6 +38,30 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                         int n_rowpad, const int* colmap, int n_colpad);
 
 constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
+
+// One launch of the MFMA filter over a work list of (row block, first column
+// tile, end column tile, 0) items; a row block is filter_block_tiles(variant)
+// row tiles of 32.  With self_join only pairs row < column are candidates.
+struct FilterArgs {
+    const bf16x8_t* xa = nullptr;
+    const bf16x8_t* yb = nullptr;
+    const int4* work = nullptr;
+    unsigned n_items = 0;
+    int n_rows = 0, n_cols = 0, self_join = 0;
+    const MfmaParams* prm = nullptr;
+    int2* cand = nullptr;
+    unsigned long long cap = 0;
+    unsigned long long* counter = nullptr;
+};
+int filter_dispatch(const FilterArgs& a, int KS, int variant);
+int filter_block_tiles(int variant);
+int filter_default_variant();
+int filter_num_variants();
+constexpr int kFilterChunk = 256;   // column tiles per chunk of the work list
+
+// Reorders a work list so that XCD x (workgroups are dealt to the 8 XCDs
+// round-robin) runs the contiguous x-th eighth; pads with empty items.
+void xcd_permute(std::vector<int4>& list);
 
 // Exact FP32 recheck of candidate pairs (ids into X and Y): flag[c] = 1 for an
 // edge (reference-order squared distance <= T), cw[c] = its distance; pairs in

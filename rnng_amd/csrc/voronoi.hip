@@ -10,17 +10,26 @@
 //   ghosts      : q is a ghost of cell b != cell(q) if L2(center_b, q) <=
 //                 fl(delta_q + 2 eps) and some member p of b has L2(p, q) <= eps
 //                 (include/point.hpp:894-918)
-//   cell query  : member x member and member x ghost pairs within eps of every
-//                 cell (drivers/landmark.cpp:203-210), then the symmetric,
-//                 deduplicated edge set (include/graph.hpp:59-116)
+//   cell query  : the pairs the landmark driver evaluates (members x members
+//                 and members x ghosts of every cell, drivers/landmark.cpp:
+//                 203-210), deduplicated to undirected edges
+//                 (include/graph.hpp:59-116)
 //
-// The ghost center test is pruned per cell with the triangle inequality:
-// L2(c_b, q) >= L2(c_b, c_a) - delta_q for q in cell a, so a center b can pass
-// only if L2(c_a, c_b) <= 2 R_a + 2 eps (R_a = max delta in cell a).  The bound
-// carries a 1e-3 relative margin, far above the FP32 distance error, so the
-// pruning never removes a center that the exact test would accept.  The member
-// existence test of a candidate ghost is a by-product of the cell query: a
-// candidate ghost is a ghost exactly when it has an edge to a member.
+// The cell query runs as one self join over the points in cell order (each
+// cell padded to 32-point tiles) on the BF16 MFMA filter, restricted to the
+// tiles of cell pairs that can hold a reference pair: p in cell a and q in
+// cell b (a != b) are paired by the reference only if q is a ghost of a or p
+// a ghost of b, which needs L2(c_a, q) <= delta_q + 2 eps (or the mirror
+// image) and so L2(c_a, c_b) <= 2 max(R_a, R_b) + 2 eps with R the largest
+// delta of a cell.  Cell pairs beyond that bound (with a 1e-3 relative margin,
+// far above the FP32 distance error) are skipped.  Every evaluated pair is
+// evaluated once; a pair within eps across cells is kept exactly when the
+// reference keeps it (the exact FP32 center test of either side passes), and
+// the passing sides are the ghosts: the ghost existence test of
+// include/point.hpp:912 is the pair test itself.  When eps is large against
+// the cell radii (every center test passes, as for unit vectors in 96 dims
+// at mean degree 25) the pruning keeps every tile and the join is the full
+// upper triangle; the reference's ghost lists then hold almost every point.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -44,10 +53,8 @@ struct rnng_voronoi_s {
     rnng::DevBuf cptr;                    // int32 [m+1] cell CSR
     rnng::DevBuf cids;                    // int32 [n] members, coalesce order
     std::vector<int> cnt;                 // members per cell (host)
-    // candidate ghosts of the last graph build, sorted by (cell, point)
-    rnng::DevBuf gkeys;                   // u64 (cell << 32 | point)
-    rnng::DevBuf greal;                   // uint8 per candidate ghost: is a ghost
-    int64_t n_gcand = 0;
+    // ghosts of the last graph build, (cell << 32 | point) sorted
+    rnng::DevBuf gkeys;
     int64_t n_ghosts = 0;
 };
 
@@ -322,30 +329,29 @@ __global__ void k_cell_radius(const int* __restrict__ cell, const float* __restr
     if (i < n) atomicMax(&rad[cell[i]], f2ord_v(delta[i]));
 }
 
-// Centers b that can hold ghosts of cell a: b != a, b non-empty and
-// L2(c_a, c_b) <= (2 R_a + 2 eps)(1 + 1e-3).  One workgroup per cell a;
-// FILL = 0 counts, FILL = 1 writes the list at lptr[a].
-template <int FILL>
-__global__ __launch_bounds__(256) void k_center_lists(const float* __restrict__ C, int m, int d,
-                                                      const int* __restrict__ cnt,
-                                                      const unsigned* __restrict__ rad, float eps,
-                                                      int* __restrict__ lcnt, const int* __restrict__ lptr,
-                                                      int* __restrict__ list) {
+// Cell adjacency bit matrix adj[a][b / 32] (m x words): cells a != b, both
+// non-empty, with L2(c_a, c_b) <= (2 max(R_a, R_b) + 2 eps)(1 + 1e-3).  For
+// q in cell b and p in cell a, the reference can pair p and q only if
+// L2(c_a, q) <= delta_q + 2 eps or L2(c_b, p) <= delta_p + 2 eps, and either
+// implies L2(c_a, c_b) <= 2 max(R_a, R_b) + 2 eps up to rounding far below the
+// margin.  One workgroup per cell a.
+__global__ __launch_bounds__(256) void k_cell_adjacency(const float* __restrict__ C, int m, int d,
+                                                        const int* __restrict__ cnt,
+                                                        const unsigned* __restrict__ rad, float eps, int words,
+                                                        unsigned* __restrict__ adj) {
+#pragma clang fp contract(off)
     extern __shared__ float ca[];
     const int a = blockIdx.x;
+    unsigned* row = adj + (size_t)a * words;
     if (cnt[a] == 0) {
-        if (!FILL && threadIdx.x == 0) lcnt[a] = 0;
+        for (int w = threadIdx.x; w < words; w += 256) row[w] = 0u;
         return;
     }
     for (int t = threadIdx.x; t < d; t += 256) ca[t] = C[(size_t)a * d + t];
     __syncthreads();
-    const double bound = (2.0 * (double)ord2f_v(rad[a]) + 2.0 * (double)eps) * (1.0 + 1e-3) + 1e-30;
-    __shared__ int wcount;
-    if (threadIdx.x == 0) wcount = 0;
-    __syncthreads();
+    const double ra = (double)ord2f_v(rad[a]);
     const int lane = threadIdx.x & 63;
-    const int base = FILL ? lptr[a] : 0;
-    for (int b0 = 0; b0 < m; b0 += 256) {
+    for (int b0 = 0; b0 < words * 32; b0 += 256) {
         const int b = b0 + threadIdx.x;
         bool keep = false;
         if (b < m && b != a && cnt[b] > 0) {
@@ -355,242 +361,79 @@ __global__ __launch_bounds__(256) void k_center_lists(const float* __restrict__ 
                 const float t = ca[k] - cb[k];
                 acc = acc + t * t;
             }
+            const double rb = (double)ord2f_v(rad[b]);
+            const double bound = (2.0 * fmax(ra, rb) + 2.0 * (double)eps) * (1.0 + 1e-3) + 1e-30;
             keep = (double)sqrtf(acc) <= bound;
         }
         const unsigned long long bal = __ballot(keep);
-        int off = 0;
-        if (lane == 0 && bal) off = atomicAdd(&wcount, __popcll(bal));
-        off = __shfl(off, 0, 64);
-        if (FILL && keep)
-            list[base + off + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = b;
-    }
-    __syncthreads();
-    if (!FILL && threadIdx.x == 0) lcnt[a] = wcount;
-}
-
-// Ghost center test.  Work item (cell a, 128-member tile, 128-center tile of
-// a's list): the exact test L2(c_b, q) <= fl(delta_q + 2 eps) for every pair;
-// passing pairs are written as keys (b << 32 | q).
-__global__ __launch_bounds__(256) void k_ghost_test(const float* __restrict__ X, const float* __restrict__ C,
-                                                    int d, const int4* __restrict__ items,
-                                                    const int* __restrict__ cptr, const int* __restrict__ cids,
-                                                    const int* __restrict__ lptr, const int* __restrict__ lcnt,
-                                                    const int* __restrict__ list, const float* __restrict__ delta,
-                                                    float eps, uint64_t* __restrict__ out, unsigned long long cap,
-                                                    unsigned long long* __restrict__ counter) {
-    __shared__ float As[AK][AB + 4];
-    __shared__ float Bs[AK][AB + 4];
-    const int4 wi = items[blockIdx.x];
-    const int a = wi.x;
-    const int m0 = cptr[a] + wi.y * AB, m1 = cptr[a + 1];
-    const int l0 = lptr[a] + wi.z * AB, l1 = lptr[a] + lcnt[a];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    float acc[8][8];
-    fp32_tile(
-        d, [&](int i) -> const float* { return m0 + i < m1 ? X + (size_t)cids[m0 + i] * d : nullptr; },
-        [&](int j) -> const float* { return l0 + j < l1 ? C + (size_t)list[l0 + j] * d : nullptr; }, As, Bs,
-        acc);
-    const float two_eps = 2.0f * eps;
-    unsigned cnt = 0;
-    int qid[8];
-    float reach[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        const int mi = m0 + micro(ty, r);
-        qid[r] = mi < m1 ? cids[mi] : -1;
-        reach[r] = qid[r] >= 0 ? delta[qid[r]] + two_eps : -1.0f;
-    }
-    int bid[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const int lj = l0 + micro(tx, c);
-        bid[c] = lj < l1 ? list[lj] : -1;
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c) cnt += (qid[r] >= 0 && bid[c] >= 0 && sqrtf(acc[r][c]) <= reach[r]) ? 1u : 0u;
-    if (!__any(cnt != 0)) return;
-    unsigned long long pos = wave_reserve(cnt, counter);
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (qid[r] >= 0 && bid[c] >= 0 && sqrtf(acc[r][c]) <= reach[r]) {
-                if (pos < cap) out[pos] = ((uint64_t)(uint32_t)bid[c] << 32) | (uint32_t)qid[r];
-                ++pos;
-            }
-}
-
-// Per-cell slot maps of the batched query: rows = members padded to 32,
-// columns = members then candidate ghosts, padded to 32 (-1 = padding).
-__global__ __launch_bounds__(256) void k_slot_maps(const int* __restrict__ cptr, const int* __restrict__ cids,
-                                                   const int* __restrict__ gptr, const uint64_t* __restrict__ gkeys,
-                                                   const int* __restrict__ rs, const int* __restrict__ cs,
-                                                   int* __restrict__ rowmap, int* __restrict__ colmap) {
-    const int b = blockIdx.x;
-    const int nb = cptr[b + 1] - cptr[b], gb = gptr[b + 1] - gptr[b];
-    const int nr = rs[b + 1] - rs[b], nc = cs[b + 1] - cs[b];
-    for (int k = threadIdx.x; k < nr; k += 256) rowmap[rs[b] + k] = k < nb ? cids[cptr[b] + k] : -1;
-    for (int k = threadIdx.x; k < nc; k += 256) {
-        int v = -1;
-        if (k < nb) v = cids[cptr[b] + k];
-        else if (k < nb + gb) v = (int)(gkeys[gptr[b] + k - nb] & 0xffffffffull);
-        colmap[cs[b] + k] = v;
-    }
-}
-
-// ------------------------------------------------------------------------
-// Batched cell query (BF16 MFMA filter)
-// ------------------------------------------------------------------------
-// Persistent: each wave takes work items gw, gw + W, ... of a list sorted by
-// decreasing cost.  Item = (first row tile, first column tile, end column
-// tile, cell); the wave keeps RT row tiles' fragments in registers and
-// streams the column tiles from L2 with a one-tile register prefetch.  A
-// candidate (row slot r, column slot c) of cell b needs row r a member
-// (r - rs_b < n_b) and column c either a later member (r - rs_b < c - cs_b <
-// n_b) or a candidate ghost (n_b <= c - cs_b < n_b + g_b).  Candidates are
-// staged per wave in LDS and written with one reservation per flush.
-constexpr int WST = 512;   // staged candidates per wave
-
-struct CellInfo {
-    int rs, cs, nb, dom;   // first row slot, first column slot, members, members + candidate ghosts
-};
-
-template <int KS, int RT>
-__global__ __launch_bounds__(256) void k_cell_filter(const bf16x8_t* __restrict__ xa,
-                                                     const bf16x8_t* __restrict__ yb,
-                                                     const int4* __restrict__ items, int n_items,
-                                                     const CellInfo* __restrict__ info, int2* __restrict__ cand,
-                                                     unsigned long long cap,
-                                                     unsigned long long* __restrict__ counter) {
-    __shared__ int2 stage[4][WST];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nw = gridDim.x * 4;
-    int2* st = stage[wave];
-    unsigned scnt = 0;   // wave-uniform
-    auto flush = [&]() {
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(counter, (unsigned long long)scnt);
-        base = __shfl(base, 0, 64);
-        for (unsigned i = lane; i < scnt; i += 64)
-            if (base + i < cap) cand[base + i] = st[i];
-        scnt = 0;
-    };
-    for (int it = blockIdx.x * 4 + wave; it < n_items; it += nw) {
-        const int4 wi = items[it];
-        const CellInfo ci = info[wi.w];
-        bf16x8_t a[RT][KS];
-#pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-            for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(wi.x + r) * KS + s) * 64 + lane];
-        bf16x8_t b[KS];
-#pragma unroll
-        for (int s = 0; s < KS; ++s) b[s] = yb[((size_t)wi.y * KS + s) * 64 + lane];
-        for (int ct = wi.y; ct < wi.z; ++ct) {
-            const int ctn = ct + 1 < wi.z ? ct + 1 : ct;
-            bf16x8_t pre[KS];
-#pragma unroll
-            for (int s = 0; s < KS; ++s) pre[s] = yb[((size_t)ctn * KS + s) * 64 + lane];
-            f32x16_t acc[RT];
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
-#pragma unroll
-            for (int s = 0; s < KS; ++s)
-#pragma unroll
-                for (int r = 0; r < RT; ++r)
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[r][s], b[s], acc[r], 0, 0, 0);
-            unsigned o = 0;
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int q = 0; q < 16; ++q) o |= __float_as_uint(acc[r][q]);
-            if (__any((int)o < 0)) {
-                unsigned long long mask = 0;
-#pragma unroll
-                for (int r = 0; r < RT; ++r)
-#pragma unroll
-                    for (int q = 0; q < 16; ++q)
-                        mask |= (unsigned long long)(__float_as_uint(acc[r][q]) >> 31) << (r * 16 + q);
-                const int col = ct * 32 + (lane & 31);
-                const int cl = col - ci.cs;
-                unsigned long long any = mask;
-#pragma unroll
-                for (int o2 = 1; o2 < 64; o2 <<= 1) any |= __shfl_xor(any, o2, 64);
-                {
-                    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)any);
-                    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(any >> 32));
-                    any = ((unsigned long long)hi << 32) | lo;
-                }
-#pragma unroll 1
-                while (any) {
-                    const int rq = __builtin_ctzll(any);
-                    any &= any - 1;
-                    const int q = rq & 15;
-                    const int row = (wi.x + (rq >> 4)) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                    const int rl = row - ci.rs;
-                    const bool ok = ((mask >> rq) & 1ull) && rl < ci.nb &&
-                                    ((cl > rl && cl < ci.nb) || (cl >= ci.nb && cl < ci.dom));
-                    const unsigned long long bal = __ballot(ok);
-                    if (bal) {
-                        const unsigned pos =
-                            scnt + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-                        if (ok) st[pos] = make_int2(row, col);
-                        scnt += (unsigned)__popcll(bal);
-                        if (scnt > WST - 64) flush();
-                    }
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < KS; ++s) b[s] = pre[s];
+        // lanes 0 and 32 write the two 32-bit words of this wave's 64 cells
+        if ((lane & 31) == 0) {
+            const int w = (b0 + (threadIdx.x & ~31)) >> 5;
+            if (w < words) row[w] = (unsigned)(bal >> (lane & 32));
         }
     }
-    if (scnt) flush();
 }
 
-// Candidate slots -> point ids (rowmap, colmap).
+// Slot map of the cell-ordered layout: slots rs[b] .. rs[b] + n_b hold the
+// members of cell b in coalesce order, the rest of the cell's 32-padded range
+// is -1.
+__global__ __launch_bounds__(256) void k_slot_map(const int* __restrict__ cptr, const int* __restrict__ cids,
+                                                  const int* __restrict__ rs, int* __restrict__ slot) {
+    const int b = blockIdx.x;
+    const int nb = cptr[b + 1] - cptr[b], nr = rs[b + 1] - rs[b];
+    for (int k = threadIdx.x; k < nr; k += 256) slot[rs[b] + k] = k < nb ? cids[cptr[b] + k] : -1;
+}
+
+// Candidate slots -> point ids.
 __global__ void k_cand_ids(const int2* __restrict__ cand, const unsigned long long* __restrict__ counter,
-                           unsigned long long cap, const int* __restrict__ rowmap,
-                           const int* __restrict__ colmap, int2* __restrict__ ids) {
+                           unsigned long long cap, const int* __restrict__ slot, int2* __restrict__ ids) {
     const unsigned long long nc = min(counter[0], cap);
     for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (unsigned long long)gridDim.x * blockDim.x) {
         const int2 s = cand[c];
-        ids[c] = make_int2(rowmap[s.x], colmap[s.y]);
+        ids[c] = make_int2(slot[s.x], slot[s.y]);
     }
 }
 
-// After the recheck: marks candidate ghosts with an edge to a member, counts
-// candidates and edges per cell, and writes each edge as (min, max) key.
-__global__ void k_cell_edges(const int2* __restrict__ cand, const int2* __restrict__ ids,
-                             const uint8_t* __restrict__ flag, const unsigned long long* __restrict__ counter,
-                             unsigned long long cap, const int* __restrict__ colcell,
-                             const CellInfo* __restrict__ info, uint8_t* __restrict__ colreal,
-                             int* __restrict__ ccand, int* __restrict__ cedge) {
-    const unsigned long long nc = min(counter[0], cap);
+// The reference's semantics for an edge (p, q) within eps found across cells
+// a = cell(p) != b = cell(q): the landmark driver emits it iff q is a ghost of
+// a (L2(c_a, q) <= fl(delta_q + 2 eps)) or p a ghost of b.  Clears the edge
+// flag otherwise; writes each ghost (cell << 32 | point) found; edges are
+// made (min, max).  counters[4] = center tests, counters[5] = ghost hits.
+__global__ void k_ref_semantics(const float* __restrict__ X, const float* __restrict__ C, int d,
+                                const int* __restrict__ cell, const float* __restrict__ delta, float eps,
+                                int2* __restrict__ ids, uint8_t* __restrict__ flag,
+                                unsigned long long cap, unsigned long long* __restrict__ counters,
+                                uint64_t* __restrict__ gkeys) {
+#pragma clang fp contract(off)
+    const unsigned long long nc = min(counters[0], cap);
+    const float two_eps = 2.0f * eps;
     for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (unsigned long long)gridDim.x * blockDim.x) {
-        const int col = cand[c].y;
-        const int b = colcell[col >> 5];
-        atomicAdd(&ccand[b], 1);
-        if (flag[c]) {
-            atomicAdd(&cedge[b], 1);
-            if (col - info[b].cs >= info[b].nb) colreal[col] = 1;
+        if (!flag[c]) continue;
+        const int2 e = ids[c];
+        ids[c] = make_int2(min(e.x, e.y), max(e.x, e.y));
+        const int a = cell[e.x], b = cell[e.y];
+        if (a == b) continue;
+        bool g[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int cc = s ? b : a, q = s ? e.x : e.y;
+            const float* cp = C + (size_t)cc * d;
+            const float* qp = X + (size_t)q * d;
+            float acc = 0.0f;
+            for (int k = 0; k < d; ++k) {
+                const float t = cp[k] - qp[k];
+                acc = acc + t * t;
+            }
+            g[s] = sqrtf(acc) <= delta[q] + two_eps;
+            if (g[s]) {
+                const unsigned long long o = atomicAdd(&counters[5], 1ull);
+                gkeys[o] = ((uint64_t)(uint32_t)cc << 32) | (uint32_t)q;
+            }
         }
-    }
-}
-
-__global__ void k_edge_keys(const int2* __restrict__ ij, int64_t n, uint64_t* __restrict__ keys) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) {
-        const int i = ij[e].x, j = ij[e].y;
-        const int lo = i < j ? i : j, hi = i < j ? j : i;
-        keys[e] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
+        atomicAdd(&counters[4], 2ull);
+        if (!g[0] && !g[1]) flag[c] = 0;
     }
 }
 
@@ -598,22 +441,6 @@ __global__ void k_edge_keys(const int2* __restrict__ ij, int64_t n, uint64_t* __
 __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uint8_t* __restrict__ flag) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
-}
-
-__global__ void k_keys_to_ij(const uint64_t* __restrict__ keys, int64_t n, int2* __restrict__ ij) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) ij[e] = make_int2((int)(keys[e] >> 32), (int)(keys[e] & 0xffffffffull));
-}
-
-// Real-ghost flag per candidate ghost k of cell b: column slot cs_b + n_b + k.
-__global__ void k_ghost_flags(const uint64_t* __restrict__ gkeys, int64_t ng, const int* __restrict__ gptr,
-                              const CellInfo* __restrict__ info, const uint8_t* __restrict__ colreal,
-                              uint8_t* __restrict__ greal) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < ng) {
-        const int b = (int)(gkeys[e] >> 32);
-        greal[e] = colreal[info[b].cs + info[b].nb + (int)(e - gptr[b])];
-    }
 }
 
 int sort_keys_u64(uint64_t* keys, int64_t n, unsigned end_bit) {
@@ -740,120 +567,61 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Ghost candidates: (cell, point) keys sorted, in v->gkeys; per-cell counts in gcnt.
-int ghost_candidates(rnng_voronoi_s* v, float eps, std::vector<int>& gcnt, rnng_stats* st, double* ms) {
-    const int n = (int)v->ds->n, m = (int)v->m, d = v->ds->d;
-    Events ev;
-    RNNG_TRY(ev.create());
-    RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
-    LocalBuf rad, lcnt, lptr, list, items, counter;
-    RNNG_TRY(rad.reserve((size_t)m * sizeof(unsigned)));
-    RNNG_TRY(lcnt.reserve((size_t)m * sizeof(int)));
-    RNNG_TRY(launch_fill_u32(rad.as<unsigned>(), m, 0u));
-    k_cell_radius<<<blocks_for(n, 256), 256, 0, stream()>>>(v->cell.as<int>(), v->delta.as<float>(), n,
-                                                            rad.as<unsigned>());
-    LocalBuf cnt_dev;
-    RNNG_TRY(upload(cnt_dev, v->cnt));
-    const size_t clds = (size_t)d * sizeof(float);
-    k_center_lists<0><<<m, 256, clds, stream()>>>(v->C.as<float>(), m, d, cnt_dev.as<int>(), rad.as<unsigned>(),
-                                                  eps, lcnt.as<int>(), nullptr, nullptr);
-    RNNG_HIP_TRY(hipGetLastError());
-    std::vector<int> lc;
-    RNNG_TRY(download(lc, lcnt, (size_t)m));
-    const std::vector<int> lp = prefix(lc);
-    RNNG_TRY(upload(lptr, lp));
-    RNNG_TRY(list.reserve((size_t)std::max(1, lp[m]) * sizeof(int)));
-    k_center_lists<1><<<m, 256, clds, stream()>>>(v->C.as<float>(), m, d, cnt_dev.as<int>(), rad.as<unsigned>(),
-                                                  eps, lcnt.as<int>(), lptr.as<int>(), list.as<int>());
-    RNNG_HIP_TRY(hipGetLastError());
-    std::vector<int4> it;
-    int64_t tests = 0;
-    for (int a = 0; a < m; ++a) {
-        const int na = v->cnt[a];
-        if (na == 0 || lc[a] == 0) continue;
-        tests += (int64_t)na * lc[a];
-        for (int t = 0; t * AB < na; ++t)
-            for (int u = 0; u * AB < lc[a]; ++u) it.push_back(make_int4(a, t, u, 0));
-    }
-    RNNG_TRY(upload(items, it));
-    RNNG_TRY(counter.reserve(sizeof(unsigned long long)));
-    unsigned long long cap = std::max<unsigned long long>(1 << 16, (unsigned long long)n);
-    int64_t ng = 0;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        RNNG_TRY(v->gkeys.reserve(cap * sizeof(uint64_t)));
-        RNNG_TRY(launch_fill_u64(counter.as<unsigned long long>(), 1, 0ull));
-        if (!it.empty()) {
-            k_ghost_test<<<(unsigned)it.size(), 256, 0, stream()>>>(
-                v->ds->X, v->C.as<float>(), d, items.as<int4>(), v->cptr.as<int>(), v->cids.as<int>(),
-                lptr.as<int>(), lcnt.as<int>(), list.as<int>(), v->delta.as<float>(), eps, v->gkeys.as<uint64_t>(),
-                cap, counter.as<unsigned long long>());
-            RNNG_HIP_TRY(hipGetLastError());
-        }
-        unsigned long long got = 0;
-        RNNG_HIP_TRY(hipMemcpyAsync(&got, counter.ptr, sizeof(got), hipMemcpyDeviceToHost, stream()));
-        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
-        if (got <= cap) {
-            ng = (int64_t)got;
-            break;
-        }
-        if (st) st->refills++;
-        cap = got + got / 8 + 1024;
-        if (attempt == 1) {
-            set_error("ghost buffer refill did not converge");
-            return RNNG_ERR_HIP;
-        }
-    }
-    RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
-    RNNG_TRY(sort_keys_u64(v->gkeys.as<uint64_t>(), ng, 64));
-    // per-cell candidate-ghost counts from the sorted keys (host side, m small)
-    std::vector<uint64_t> hk;
-    RNNG_TRY(download(hk, v->gkeys, (size_t)ng));
-    gcnt.assign(m, 0);
-    for (uint64_t k : hk) gcnt[(int)(k >> 32)]++;
-    v->n_gcand = ng;
-    RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
-    if (st) {
-        st->ghost_tests += tests;
-        st->ghost_candidates += ng;
-    }
-    *ms = ev.ms(0, 2);
-    return RNNG_OK;
-}
-
-// Cell-query tuning: row tiles per item and the longest column run of an item.
+// Cell-query tuning: filter variant (row block shape) and XCD ordering.
 struct CellTune {
-    int rt = 1;
-    int ct_max = 64;
+    int variant = -1;   // -1: the library default
+    int xcd = 1;
 };
 
-template <int KS>
-int launch_cell_filter(int rt, const DevBuf& xa, const DevBuf& yb, const DevBuf& items, int n_items,
-                       const DevBuf& info, DevBuf& cand, unsigned long long cap, DevBuf& counter) {
-    const unsigned grid = (unsigned)std::max(1, std::min((n_items + 3) / 4, 256 * 8));
-    if (rt == 2)
-        k_cell_filter<KS, 2><<<grid, 256, 0, stream()>>>(xa.as<bf16x8_t>(), yb.as<bf16x8_t>(), items.as<int4>(),
-                                                         n_items, info.as<CellInfo>(), cand.as<int2>(), cap,
-                                                         counter.as<unsigned long long>());
-    else
-        k_cell_filter<KS, 1><<<grid, 256, 0, stream()>>>(xa.as<bf16x8_t>(), yb.as<bf16x8_t>(), items.as<int4>(),
-                                                         n_items, info.as<CellInfo>(), cand.as<int2>(), cap,
-                                                         counter.as<unsigned long long>());
-    RNNG_HIP_TRY(hipGetLastError());
-    return RNNG_OK;
-}
-
-int dispatch_cell_filter(int KS, int rt, const DevBuf& xa, const DevBuf& yb, const DevBuf& items, int n_items,
-                         const DevBuf& info, DevBuf& cand, unsigned long long cap, DevBuf& counter) {
-    switch (KS) {
-#define RNNG_KS(k) \
-    case k: return launch_cell_filter<k>(rt, xa, yb, items, n_items, info, cand, cap, counter);
-        RNNG_KS(1) RNNG_KS(2) RNNG_KS(3) RNNG_KS(4) RNNG_KS(5) RNNG_KS(6) RNNG_KS(7) RNNG_KS(8) RNNG_KS(9)
-#undef RNNG_KS
-        default:
-            set_error("MFMA mode supports d <= 135");
-            return RNNG_ERR_UNSUPPORTED;
+// Work list of the cell-ordered self join: for every row block, the column
+// tiles of its own cells from the block's first tile on and of every cell
+// adjacent to one of its cells, merged into runs and cut at chunk
+// boundaries; chunk-major order.  Returns the pairs the tiles cover.
+int64_t cell_work(const std::vector<int>& rs, const std::vector<unsigned>& adj, int words, int m, int bt,
+                  int n_rowblocks, std::vector<int4>& items) {
+    // tile -> cell
+    const int ntiles = rs[m] / 32;
+    std::vector<int> tcell(std::max(1, ntiles));
+    for (int b = 0; b < m; ++b)
+        for (int t = rs[b] / 32; t < rs[b + 1] / 32; ++t) tcell[t] = b;
+    std::vector<unsigned> need(words);
+    std::vector<std::vector<int4>> chunks((ntiles + kFilterChunk - 1) / kFilterChunk + 1);
+    int64_t covered = 0;
+    for (int rb = 0; rb < n_rowblocks; ++rb) {
+        const int t0 = rb * bt, t1 = std::min(ntiles, t0 + bt);
+        if (t0 >= ntiles) break;
+        std::fill(need.begin(), need.end(), 0u);
+        for (int a = tcell[t0]; a <= tcell[t1 - 1]; ++a) {
+            need[a >> 5] |= 1u << (a & 31);
+            const unsigned* row = adj.data() + (size_t)a * words;
+            for (int w = 0; w < words; ++w) need[w] |= row[w];
+        }
+        // runs of needed cells from the block's first cell on
+        auto next = [&](int x, bool set) {   // first cell >= x whose bit is `set` (m if none)
+            while (x < m) {
+                unsigned w = need[x >> 5] ^ (set ? 0u : ~0u);
+                w &= ~0u << (x & 31);
+                if (w) return std::min(m, (x & ~31) + __builtin_ctz(w));
+                x = (x & ~31) + 32;
+            }
+            return m;
+        };
+        int b = next(tcell[t0], true);
+        while (b < m) {
+            const int e = next(b, false) - 1;
+            int lo = std::max(rs[b] / 32, t0), hi = rs[e + 1] / 32;
+            while (lo < hi) {
+                const int cut = std::min(hi, (lo / kFilterChunk + 1) * kFilterChunk);
+                chunks[lo / kFilterChunk].push_back(make_int4(rb, lo, cut, 0));
+                covered += (int64_t)(cut - lo) * 32 * (t1 - t0) * 32;
+                lo = cut;
+            }
+            b = next(e + 1, true);
+        }
     }
+    items.clear();
+    for (auto& c : chunks) items.insert(items.end(), c.begin(), c.end());
+    return covered;
 }
 
 int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st) {
@@ -867,84 +635,72 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     const float eps_band = (float)((double)eps * (1.0 + 1e-5));
     float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
     if (Tb < T) Tb = T;
-
-    std::vector<int> gcnt;
-    double ghost_ms = 0.0;
-    RNNG_TRY(ghost_candidates(v, eps, gcnt, st, &ghost_ms));
-
-    // ---- slot layout and work items ----
-    const CellTune tune;
-    std::vector<int> R(m), Cc(m);
-    for (int b = 0; b < m; ++b) {
-        R[b] = (v->cnt[b] + 31) & ~31;
-        Cc[b] = (v->cnt[b] + gcnt[b] + 31) & ~31;
-    }
-    const std::vector<int> rs = prefix(R), cs = prefix(Cc);
-    const std::vector<int> gp = prefix(gcnt);
-    std::vector<CellInfo> info(m);
-    std::vector<int> colcell(std::max(1, cs[m] / 32));
-    int64_t pairs = 0;
-    struct Item {
-        int4 w;
-        int64_t cost;
-    };
-    std::vector<Item> items;
-    for (int b = 0; b < m; ++b) {
-        const int nb = v->cnt[b], gb = gcnt[b];
-        info[b] = CellInfo{rs[b], cs[b], nb, nb + gb};
-        for (int t = cs[b] / 32; t < cs[b + 1] / 32; ++t) colcell[t] = b;
-        pairs += (int64_t)nb * (nb - 1) / 2 + (int64_t)nb * gb;
-        if (nb == 0 || (nb == 1 && gb == 0)) continue;
-        const int ntile_r = R[b] / 32, c_end = (cs[b] + nb + gb + 31) / 32;
-        for (int t = 0; t < ntile_r; t += tune.rt) {
-            const int c_begin = cs[b] / 32 + t;   // columns left of the diagonal tile hold no pairs
-            for (int c0 = c_begin; c0 < c_end; c0 += tune.ct_max) {
-                const int c1 = std::min(c_end, c0 + tune.ct_max);
-                items.push_back(Item{make_int4(rs[b] / 32 + t, c0, c1, b), (int64_t)(c1 - c0) * tune.rt});
-            }
-        }
-    }
-    std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-    std::vector<int4> witems(items.size());
-    for (size_t i = 0; i < items.size(); ++i) witems[i] = items[i].w;
-
-    const int n_rowpad = rs[m] + 32 * tune.rt;   // a trailing item may read rt - 1 tiles past the last cell
-    const int n_colpad = std::max(cs[m], 32);
-    LocalBuf d_rs, d_cs, d_gptr, d_rowmap, d_colmap, d_info, d_items, d_colcell;
-    RNNG_TRY(upload(d_rs, rs));
-    RNNG_TRY(upload(d_cs, cs));
-    RNNG_TRY(upload(d_gptr, gp));
-    RNNG_TRY(upload(d_info, info));
-    RNNG_TRY(upload(d_items, witems));
-    RNNG_TRY(upload(d_colcell, colcell));
-    RNNG_TRY(d_rowmap.reserve((size_t)n_rowpad * sizeof(int)));
-    RNNG_TRY(d_colmap.reserve((size_t)n_colpad * sizeof(int)));
-    RNNG_TRY(launch_fill_u32(d_rowmap.as<unsigned>(), n_rowpad, 0xffffffffu));
-    RNNG_TRY(launch_fill_u32(d_colmap.as<unsigned>(), n_colpad, 0xffffffffu));
-    k_slot_maps<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), d_gptr.as<int>(),
-                                         v->gkeys.as<uint64_t>(), d_rs.as<int>(), d_cs.as<int>(), d_rowmap.as<int>(),
-                                         d_colmap.as<int>());
-    RNNG_HIP_TRY(hipGetLastError());
+    CellTune tune;
+    const int variant = tune.variant >= 0 ? tune.variant : filter_default_variant();
+    const int bt = filter_block_tiles(variant);
 
     Events ev;
     RNNG_TRY(ev.create());
     RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
-    MfmaPrep P;
-    RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, d_rowmap.as<int>(), n_rowpad, d_colmap.as<int>(),
-                                 n_colpad));
+    // ---- cell adjacency ----
+    const int words = (m + 31) / 32;
+    LocalBuf rad, cnt_dev, adj_dev;
+    RNNG_TRY(rad.reserve((size_t)m * sizeof(unsigned)));
+    RNNG_TRY(adj_dev.reserve((size_t)m * words * sizeof(unsigned)));
+    RNNG_TRY(launch_fill_u32(rad.as<unsigned>(), m, 0u));
+    k_cell_radius<<<blocks_for(n, 256), 256, 0, stream()>>>(v->cell.as<int>(), v->delta.as<float>(), n,
+                                                            rad.as<unsigned>());
+    RNNG_TRY(upload(cnt_dev, v->cnt));
+    k_cell_adjacency<<<m, 256, (size_t)d * sizeof(float), stream()>>>(
+        v->C.as<float>(), m, d, cnt_dev.as<int>(), rad.as<unsigned>(), eps, words, adj_dev.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    std::vector<unsigned> adj;
+    RNNG_TRY(download(adj, adj_dev, (size_t)m * words));
+    // ---- cell-ordered slot layout and work list ----
+    std::vector<int> R(m);
+    for (int b = 0; b < m; ++b) R[b] = (v->cnt[b] + 31) & ~31;
+    const std::vector<int> rs = prefix(R);
+    const int bt_rows = bt * 32;
+    const int n_pad = std::max(bt_rows, (rs[m] + bt_rows - 1) / bt_rows * bt_rows);
+    const int n_rowblocks = n_pad / bt_rows;
+    std::vector<int4> items;
+    const int64_t covered = cell_work(rs, adj, words, m, bt, n_rowblocks, items);
+    if (tune.xcd) xcd_permute(items);
+    LocalBuf d_rs, slot, d_items;
+    RNNG_TRY(upload(d_rs, rs));
+    RNNG_TRY(upload(d_items, items));
+    RNNG_TRY(slot.reserve((size_t)n_pad * sizeof(int)));
+    RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
+    k_slot_map<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), d_rs.as<int>(), slot.as<int>());
+    RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
+    // ---- BF16 fragments of the cell-ordered points ----
+    MfmaPrep P;
+    RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
+    RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
 
-    LocalBuf counters, cand, cid, cflag, cw, colreal, ccand, cedge;
+    LocalBuf counters, cand, cid, cflag, cw;
     RNNG_TRY(counters.reserve(8 * sizeof(unsigned long long)));
     unsigned long long cap = std::max<unsigned long long>(1 << 20, 16ull * (unsigned long long)n);
     unsigned long long ncand = 0;
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(cand.reserve(cap * sizeof(int2)));
         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
-        RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        RNNG_TRY(dispatch_cell_filter(KS, tune.rt, P.xa, P.yb, d_items, (int)witems.size(), d_info, cand, cap,
-                                      counters));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+        FilterArgs fa;
+        fa.xa = P.xa.as<bf16x8_t>();
+        fa.yb = P.yb.as<bf16x8_t>();
+        fa.work = d_items.as<int4>();
+        fa.n_items = (unsigned)items.size();
+        fa.n_rows = n_pad;
+        fa.n_cols = n_pad;
+        fa.self_join = 1;
+        fa.prm = P.params.as<MfmaParams>();
+        fa.cand = cand.as<int2>();
+        fa.cap = cap;
+        fa.counter = counters.as<unsigned long long>();
+        RNNG_TRY(filter_dispatch(fa, KS, variant));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
         RNNG_HIP_TRY(hipMemcpyAsync(&ncand, counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
         if (ncand <= cap) break;
@@ -955,87 +711,74 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             return RNNG_ERR_HIP;
         }
     }
+    // ---- FP32 recheck, reference semantics across cells, compaction ----
     const size_t nc1 = std::max<size_t>(1, ncand);
     RNNG_TRY(cid.reserve(nc1 * sizeof(int2)));
     RNNG_TRY(cflag.reserve(nc1));
     RNNG_TRY(cw.reserve(nc1 * sizeof(float)));
-    RNNG_TRY(colreal.reserve((size_t)n_colpad));
-    RNNG_TRY(ccand.reserve((size_t)m * sizeof(int)));
-    RNNG_TRY(cedge.reserve((size_t)m * sizeof(int)));
-    RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
-    const unsigned egrid = (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(4096, (ncand + 255) / 256));
+    RNNG_TRY(v->gkeys.reserve(2 * nc1 * sizeof(uint64_t)));
+    const unsigned egrid =
+        (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(8192, (ncand + 255) / 256));
     k_cand_ids<<<egrid, 256, 0, stream()>>>(cand.as<int2>(), counters.as<unsigned long long>(), cap,
-                                            d_rowmap.as<int>(), d_colmap.as<int>(), cid.as<int2>());
+                                            slot.as<int>(), cid.as<int2>());
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_TRY(launch_recheck(v->ds->X, v->ds->X, d, cid.as<int2>(), cap, ncand, T, eps, cflag.as<uint8_t>(),
                             cw.as<float>(), counters.as<unsigned long long>()));
-    RNNG_TRY(launch_fill_u32(ccand.as<unsigned>(), m, 0u));
-    RNNG_TRY(launch_fill_u32(cedge.as<unsigned>(), m, 0u));
-    {
-        // colreal as bytes: clear with a u32 fill over the rounded-up size
-        RNNG_TRY(colreal.reserve((size_t)((n_colpad + 3) & ~3)));
-        RNNG_TRY(launch_fill_u32(colreal.as<unsigned>(), (n_colpad + 3) / 4, 0u));
-    }
-    k_cell_edges<<<egrid, 256, 0, stream()>>>(cand.as<int2>(), cid.as<int2>(), cflag.as<uint8_t>(),
-                                              counters.as<unsigned long long>(), cap, d_colcell.as<int>(),
-                                              d_info.as<CellInfo>(), colreal.as<uint8_t>(), ccand.as<int>(),
-                                              cedge.as<int>());
+    k_ref_semantics<<<egrid, 256, 0, stream()>>>(v->ds->X, v->C.as<float>(), d, v->cell.as<int>(),
+                                                 v->delta.as<float>(), eps, cid.as<int2>(), cflag.as<uint8_t>(),
+                                                 cap, counters.as<unsigned long long>(), v->gkeys.as<uint64_t>());
     RNNG_HIP_TRY(hipGetLastError());
-
-    // ---- edges: compact, canonical (min, max), sort, unique ----
     if ((int64_t)ncand > g->cap) RNNG_TRY(edge_buffers(g, (int64_t)ncand));
     int64_t ne = 0;
     RNNG_TRY(compact_edges(cid.as<int2>(), cw.as<float>(), cflag.as<uint8_t>(), (int64_t)ncand, g->ij.as<int2>(),
                            g->w.as<float>(), &ne));
-    int64_t nu = 0;
-    if (ne > 0) {
-        LocalBuf keys, uflag, tmp_ij;
-        RNNG_TRY(keys.reserve((size_t)ne * sizeof(uint64_t)));
-        RNNG_TRY(uflag.reserve((size_t)ne));
-        RNNG_TRY(tmp_ij.reserve((size_t)ne * sizeof(int2)));
-        k_edge_keys<<<blocks_for(ne, 256), 256, 0, stream()>>>(g->ij.as<int2>(), ne, keys.as<uint64_t>());
-        RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(sort_pairs_u64_f32(keys.as<uint64_t>(), g->w.as<float>(), ne));
-        k_unique_flags<<<blocks_for(ne, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), ne, uflag.as<uint8_t>());
-        k_keys_to_ij<<<blocks_for(ne, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), ne, tmp_ij.as<int2>());
-        RNNG_HIP_TRY(hipGetLastError());
-        LocalBuf tmp_w;
-        RNNG_TRY(tmp_w.reserve((size_t)ne * sizeof(float)));
-        RNNG_HIP_TRY(hipMemcpyAsync(tmp_w.ptr, g->w.ptr, (size_t)ne * sizeof(float), hipMemcpyDeviceToDevice,
-                                    stream()));
-        RNNG_TRY(compact_edges(tmp_ij.as<int2>(), tmp_w.as<float>(), uflag.as<uint8_t>(), ne, g->ij.as<int2>(),
-                               g->w.as<float>(), &nu));
-    }
-    g->n_edges = nu;
-
-    // ---- ghosts: candidates with an edge to a member ----
-    RNNG_TRY(v->greal.reserve((size_t)std::max<int64_t>(1, v->n_gcand)));
-    if (v->n_gcand > 0) {
-        k_ghost_flags<<<blocks_for(v->n_gcand, 256), 256, 0, stream()>>>(
-            v->gkeys.as<uint64_t>(), v->n_gcand, d_gptr.as<int>(), d_info.as<CellInfo>(), colreal.as<uint8_t>(),
-            v->greal.as<uint8_t>());
-        RNNG_HIP_TRY(hipGetLastError());
-    }
-    std::vector<uint8_t> gr;
-    RNNG_TRY(download(gr, v->greal, (size_t)v->n_gcand));
-    v->n_ghosts = 0;
-    for (uint8_t x : gr) v->n_ghosts += x;
-    RNNG_HIP_TRY(hipEventRecord(ev.e[5], stream()));
+    g->n_edges = ne;
     unsigned long long cnt[8];
     RNNG_HIP_TRY(hipMemcpyAsync(cnt, counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, stream()));
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    // ---- ghosts: distinct (cell, point) pairs ----
+    const int64_t nhits = (int64_t)cnt[5];
+    int64_t ng = 0;
+    if (nhits > 0) {
+        RNNG_TRY(sort_keys_u64(v->gkeys.as<uint64_t>(), nhits, 64));
+        LocalBuf uflag, uniq, nsel;
+        RNNG_TRY(uflag.reserve((size_t)nhits));
+        RNNG_TRY(uniq.reserve((size_t)nhits * sizeof(uint64_t)));
+        RNNG_TRY(nsel.reserve(sizeof(unsigned long long)));
+        k_unique_flags<<<blocks_for(nhits, 256), 256, 0, stream()>>>(v->gkeys.as<uint64_t>(), nhits,
+                                                                      uflag.as<uint8_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        size_t tb = 0;
+        RNNG_HIP_TRY(rocprim::select(nullptr, tb, v->gkeys.as<uint64_t>(), uflag.as<uint8_t>(), uniq.as<uint64_t>(),
+                                     nsel.as<unsigned long long>(), (size_t)nhits, stream()));
+        LocalBuf tmp;
+        RNNG_TRY(tmp.reserve(tb));
+        RNNG_HIP_TRY(rocprim::select(tmp.ptr, tb, v->gkeys.as<uint64_t>(), uflag.as<uint8_t>(), uniq.as<uint64_t>(),
+                                     nsel.as<unsigned long long>(), (size_t)nhits, stream()));
+        unsigned long long h = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(&h, nsel.ptr, sizeof(h), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        ng = (int64_t)h;
+        RNNG_HIP_TRY(hipMemcpyAsync(v->gkeys.ptr, uniq.ptr, (size_t)ng * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                                    stream()));
+    }
+    v->n_ghosts = ng;
+    RNNG_HIP_TRY(hipEventRecord(ev.e[5], stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     if (st) {
-        st->pair_evals += pairs;
+        st->pair_evals += covered;
         st->candidates += (int64_t)ncand;
-        st->edges += nu;
+        st->edges += ne;
         st->band_pairs += (int64_t)cnt[2];
         st->band_flips += (int64_t)cnt[3];
-        st->ghosts += v->n_ghosts;
-        st->ghost_ms += ghost_ms;
-        st->prep_ms += ev.ms(0, 1);
-        st->filter_ms += ev.ms(2, 3);
+        st->ghost_tests += (int64_t)cnt[4];
+        st->ghost_candidates += nhits;
+        st->ghosts += ng;
+        st->ghost_ms += ev.ms(0, 1);
+        st->prep_ms += ev.ms(1, 2);
+        st->filter_ms += ev.ms(3, 4);
         st->recheck_ms += ev.ms(4, 5);
-        st->total_ms += ghost_ms + ev.ms(0, 1) + ev.ms(2, 3) + ev.ms(4, 5);
+        st->total_ms += ev.ms(0, 2) + ev.ms(3, 5);
     }
     return RNNG_OK;
 }
@@ -1231,22 +974,17 @@ int rnng_voronoi_ghosts(rnng_voronoi_t v, int64_t* pairs, int64_t cap) {
         return RNNG_ERR_CAPACITY;
     }
     std::vector<uint64_t> k;
-    std::vector<uint8_t> r;
-    RNNG_TRY(download(k, v->gkeys, (size_t)v->n_gcand));
-    RNNG_TRY(download(r, v->greal, (size_t)v->n_gcand));
-    int64_t o = 0;
-    for (size_t e = 0; e < k.size(); ++e)
-        if (r[e]) {
-            pairs[2 * o] = (int64_t)(k[e] >> 32);
-            pairs[2 * o + 1] = (int64_t)(k[e] & 0xffffffffull);
-            ++o;
-        }
+    RNNG_TRY(download(k, v->gkeys, (size_t)v->n_ghosts));
+    for (size_t e = 0; e < k.size(); ++e) {
+        pairs[2 * e] = (int64_t)(k[e] >> 32);
+        pairs[2 * e + 1] = (int64_t)(k[e] & 0xffffffffull);
+    }
     return RNNG_OK;
 }
 
 int rnng_voronoi_destroy(rnng_voronoi_t v) {
     if (!v) return RNNG_OK;
-    for (DevBuf* b : {&v->ctr, &v->C, &v->cell, &v->delta, &v->cptr, &v->cids, &v->gkeys, &v->greal}) b->release();
+    for (DevBuf* b : {&v->ctr, &v->C, &v->cell, &v->delta, &v->cptr, &v->cids, &v->gkeys}) b->release();
     delete v;
     return RNNG_OK;
 }
