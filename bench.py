@@ -46,6 +46,19 @@ def cpu_baseline(pts, eps, sample):
                       f"in {dt:.2f} s; edges/s {edges / dt:.4g}"}
 
 
+def measured_traffic(profile, n):
+    """HBM bytes per filter launch from the committed rocprofv3 summary (two
+    --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
+    size); None when the summary is absent or for another size."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if n != 1_000_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def bench_cells(args):
     """Voronoi-cell workload: 2,000,000 x 96 unit vectors (vMF-like mixture of
     1,024 directions, kappa 50), 4,096 farthest-point landmarks, eps for mean
@@ -105,6 +118,95 @@ def bench_cells(args):
     print(json.dumps(out), flush=True)
 
 
+def _secondary_line(metric, value, unit, K, warmup, ms, workload, extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": K, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "data": "synthetic (seeded)", "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def bench_edit(args):
+    """Edit-distance workload: 200,000 DNA-like strings (2,000 random seeds of
+    length 100-150, 100 mutants each at 1%/0.5%/0.5% substitution/insertion/
+    deletion), integer eps = 8.  One step = the full graph build over the
+    resident strings."""
+    from rnng_amd import Stats, edit, synth
+    t0 = time.perf_counter()
+    atoms, offs = synth.dna_mutants()
+    n = offs.shape[0] - 1
+    log(f"generated {n} strings in {time.perf_counter() - t0:.1f}s")
+    S = edit.StringSet(atoms, offs)
+    eps = 8.0
+    for w in range(args.warmup):
+        g = S.graph(eps)
+        log(f"warmup {w}: {g.num_edges} edges, stats {g.stats.as_dict()}")
+        g.close()
+    st = Stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g = S.graph(eps, st)
+        ne = g.num_edges
+        g.close()
+    dt = (time.perf_counter() - t0) / args.steps
+    K = args.steps
+    out = _secondary_line(
+        "edit-distance ε-graph pair-evals/s and edges/s, 200k strings", st.pair_evals / K / dt, "pair-evals/s",
+        K, args.warmup, dt * 1e3, f"edit-distance eps-graph, {n} DNA-like strings (length 100-150), eps=8",
+        {"dtype": "int32", "edges": ne, "edges_per_s": ne / dt, "pair_evals": st.pair_evals // K,
+         "candidates": st.candidates // K,
+         "stage_ms": {"filter": st.filter_ms / K, "verify": st.recheck_ms / K, "device_total": st.total_ms / K}})
+    if not args.no_cpu:
+        import oracle
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        m = 4000
+        t1 = time.perf_counter()
+        ij, _ = oracle.edit_eps_pairs(atoms[:offs[m]], offs[:m + 1], eps, nthreads=cores)
+        tc = time.perf_counter() - t1
+        pairs = m * (m - 1) // 2
+        out["cpu_baseline"] = {"value": pairs / tc, "unit": "pair-evals/s", "cores": cores, "kind": "port",
+                               "sample": f"first {m} strings ({pairs} pairs, {ij.shape[0]} edges) in {tc:.2f} s"}
+    S.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_rips(args):
+    """Rips workload: 300,000 points in 3 dims (torus, sphere, uniform box,
+    noise 0.02), eps 0.08, 2-skeleton.  One step = epsilon graph + triangles +
+    the sorted filtration."""
+    from rnng_amd import Stats, l2, rips, synth
+    pts, eps = synth.points_rips()
+    ds = l2.Dataset(pts)
+    for w in range(args.warmup):
+        v, i = rips.rips_filtration(ds, eps, 2)
+        log(f"warmup {w}: {v.shape[0]} simplices")
+    st = Stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        v, ids = rips.rips_filtration(ds, eps, 2, st)
+    dt = (time.perf_counter() - t0) / args.steps
+    K = args.steps
+    dims = rips.simplex_dims(ids)
+    counts = [int((dims == k).sum()) for k in range(3)]
+    out = _secondary_line(
+        "Rips 2-skeleton simplices/s, 300k x 3", ids.shape[0] / dt, "simplices/s", K, args.warmup, dt * 1e3,
+        "Vietoris-Rips 2-skeleton, 300000 points x 3 dims (torus/sphere/box), eps=0.08",
+        {"dtype": "fp32 distances, int64 simplex ids", "simplices": counts,
+         "stage_ms": {"graph_filter": st.filter_ms / K, "cliques": st.recheck_ms / K,
+                      "device_total": st.total_ms / K}})
+    if not args.no_cpu:
+        import oracle
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        m = 60000
+        t1 = time.perf_counter()
+        vv, _ = oracle.rips_filtration(pts[:m], eps, 2, nthreads=cores)
+        tc = time.perf_counter() - t1
+        out["cpu_baseline"] = {"value": vv.shape[0] / tc, "unit": "simplices/s", "cores": cores, "kind": "port",
+                               "sample": f"first {m} points ({vv.shape[0]} simplices) in {tc:.2f} s"}
+    ds.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,8 +218,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=150_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--workload", choices=["l2", "cells"], default="l2",
-                    help="l2: 1M x 128 all-pairs (default); cells: 2M x 96 through Voronoi cells")
+    ap.add_argument("--workload", choices=["l2", "cells", "edit", "rips"], default="l2",
+                    help="l2: 1M x 128 all-pairs (default, the headline); cells: 2M x 96 through "
+                         "Voronoi cells; edit: 200k strings; rips: 300k x 3 2-skeleton")
     ap.add_argument("--cells", type=int, default=4096, help="landmarks of the cells workload")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -125,8 +228,8 @@ def main():
               "this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "cells":
-        bench_cells(args)
+    if args.workload != "l2":
+        {"cells": bench_cells, "edit": bench_edit, "rips": bench_rips}[args.workload](args)
         return
 
     from rnng_amd import Stats, l2, synth
@@ -198,7 +301,8 @@ def main():
         "stage_ms": {"prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck": st.recheck_ms / K, "device_total": st.total_ms / K},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
+                     "traffic": measured_traffic("l2_mfma_filter_gmm_int_1Mx128.md", n)},
     }
 
     if not args.no_plain:

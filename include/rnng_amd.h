@@ -10,8 +10,8 @@
  *   - Index = int64, Real = float (include/utils.h:24-25).
  *   - A pair (p, q) is an edge iff L2(p, q) <= eps, where L2 is the float
  *     sqrtf of the sequential FP32 sum of squared differences over dims
- *     0..d-1, not fused (drivers/systolic.cpp:449-465), and eps is a float
- *     (drivers/systolic.cpp:217,421; include/search.hpp:244,399).
+ *     0..d-1, not fused (drivers/systolic.cpp:253-269), and eps is a float
+ *     (drivers/systolic.cpp:225; include/search.hpp:244,399).
  *   - The returned weight is exactly that float distance.
  *
  * Calls are synchronous from the host's point of view (results are complete
@@ -92,7 +92,7 @@ int rnng_dataset_destroy(rnng_dataset_t ds);
 /* ------------------------------------------------------------------------ */
 
 /* All pairs i < j of ds with L2 <= eps (the systolic driver's all-pairs
- * query, drivers/systolic.cpp:329-343, over one GPU's whole point set).
+ * query, drivers/systolic.cpp:123-147, over one GPU's whole point set).
  * The edge list stays on the device in *out until copied. */
 int rnng_graph_build(rnng_dataset_t ds, float eps, int32_t mode, rnng_graph_t *out,
                      rnng_stats *stats);
@@ -143,7 +143,7 @@ int rnng_strings_create(const char *atoms, const int64_t *offsets, int64_t n,
 int rnng_strings_destroy(rnng_strings_t s);
 
 /* All pairs i < j whose Levenshtein distance (unit costs, the reference's
- * EditDistance functor, drivers/systolic.cpp:467-495) is <= eps; the weight
+ * EditDistance functor, drivers/systolic.cpp:271-299) is <= eps; the weight
  * is the distance as a float.  eps up to 32 (RNNG_ERR_UNSUPPORTED above). */
 int rnng_edit_graph_build(rnng_strings_t s, float eps, rnng_graph_t *out, rnng_stats *stats);
 
@@ -204,7 +204,7 @@ int rnng_voronoi_create(rnng_dataset_t ds, const int64_t *centers, int64_t m, rn
 /* As rnng_voronoi_create with m farthest-point landmarks: center 0 is point
  * `first`, center k+1 the point farthest (exact FP32 distance, ties to the
  * smaller id) from centers 0..k.  The landmark driver samples centers with
- * selection_sample instead (drivers/landmark.cpp:110-127); the edge set does
+ * selection_sample instead (drivers/landmark.cpp:111-124); the edge set does
  * not depend on the centers. */
 int rnng_voronoi_create_fps(rnng_dataset_t ds, int64_t m, int64_t first, rnng_voronoi_t *out);
 

@@ -14,7 +14,7 @@
 #include <omp.h>
 
 /* ------------------------------------------------------------------------ */
-/* L2 distance in the reference's order: drivers/systolic.cpp:449-465        */
+/* L2 distance in the reference's order: drivers/systolic.cpp:253-269        */
 /* ------------------------------------------------------------------------ */
 
 float orc_l2sq(const float *p, const float *q, int64_t d)
@@ -169,7 +169,7 @@ int64_t orc_l2_eps_pairs(const float *X, int64_t nx, const float *Y, int64_t ny,
 }
 
 /* ------------------------------------------------------------------------ */
-/* Edit distance: drivers/systolic.cpp:467-495                               */
+/* Edit distance: drivers/systolic.cpp:271-299                               */
 /* ------------------------------------------------------------------------ */
 
 int64_t orc_edit_distance(const char *s, int64_t m, const char *t, int64_t n)
@@ -245,7 +245,7 @@ void orc_voronoi_assign(const float *pts, int64_t n, const float *centers, int64
 
 /* ------------------------------------------------------------------------ */
 /* Farthest-point landmarks (no reference counterpart: the landmark driver    */
-/* samples centers, drivers/landmark.cpp:110-127).  Center 0 = first; center  */
+/* samples centers, drivers/landmark.cpp:111-124).  Center 0 = first; center  */
 /* k+1 = argmax of the distance to the nearest chosen center, ties to the     */
 /* smaller index.  The running (dist, cell) update uses the `<=` rule of      */
 /* include/point.hpp:550-556, so after the last center (cell, dist) is the    */

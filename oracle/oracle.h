@@ -45,7 +45,7 @@
 extern "C" {
 #endif
 
-/* Reference-order squared L2 distance: drivers/systolic.cpp:449-465
+/* Reference-order squared L2 distance: drivers/systolic.cpp:253-269
  * (same functor in drivers/landmark.cpp:335-352, drivers/rips.cpp:326-342).
  * delta = fl(p_k - q_k); val = fl(val + fl(delta*delta)), k = 0..d-1. */
 float orc_l2sq(const float *p, const float *q, int64_t d);
@@ -56,7 +56,7 @@ float orc_l2(const float *p, const float *q, int64_t d);
 /* Largest float T >= 0 with sqrtf(T) <= eps (-1 if none, e.g. eps < 0 or NaN).
  * sqrtf is monotone, so sqrtf(v) <= eps  <=>  v <= T for every float v >= 0.
  * The reference tests dist <= radius with radius held as float
- * (drivers/systolic.cpp:217,421; include/search.hpp:244,399). */
+ * (drivers/systolic.cpp:225; include/search.hpp:244,399). */
 float orc_sq_threshold(float eps);
 
 /* Brute-force epsilon range (include/search.hpp:389-407 semantics) over a
@@ -70,7 +70,7 @@ int64_t orc_l2_eps_pairs(const float *X, int64_t nx, const float *Y, int64_t ny,
                          int64_t *ij, float *w, int64_t cap);
 
 /* Levenshtein distance with unit costs, two-row dynamic programme in int64:
- * drivers/systolic.cpp:467-495 (same in landmark.cpp:354-383, rips.cpp:344-372). */
+ * drivers/systolic.cpp:271-299 (same in landmark.cpp:354-383, rips.cpp:344-372). */
 int64_t orc_edit_distance(const char *s, int64_t m, const char *t, int64_t n);
 
 /* Brute-force edit-distance epsilon graph over strings given as a CSR

@@ -1,7 +1,7 @@
 // Edit-distance epsilon graph for gfx950.
 //
 // Reference semantics (gabe-raulet/RNNG): EditDistance<char>::operator()
-// drivers/systolic.cpp:467-495 -- Levenshtein distance with unit costs,
+// drivers/systolic.cpp:271-299 -- Levenshtein distance with unit costs,
 // returned as a float; a pair is an edge iff that float is <= radius
 // (include/search.hpp:244,399).  Distances are integers, so with
 // k = floor(eps) the test is dist <= k.

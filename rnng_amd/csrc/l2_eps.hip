@@ -1,10 +1,10 @@
 // L2 epsilon-range kernels for gfx950 and their host orchestration.
 //
 // Reference semantics (gabe-raulet/RNNG):
-//   L2Distance<float>::operator()  drivers/systolic.cpp:449-465  -- sequential
+//   L2Distance<float>::operator()  drivers/systolic.cpp:253-269  -- sequential
 //       FP32 sum over dims 0..d-1 of fl(fl(p_k - q_k)^2), not fused, then sqrt
 //   edge test dist <= radius       include/search.hpp:244,399 (radius is float)
-//   all-pairs query over a block   drivers/systolic.cpp:329-343
+//   all-pairs query over a block   drivers/systolic.cpp:123-147
 //
 // Two distance-stage algorithms:
 //   PLAIN  k_l2_plain: LDS-tiled FP32 kernel that evaluates every pair in the

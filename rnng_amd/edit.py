@@ -3,7 +3,7 @@
 Strings are a CSR (atoms uint8, offsets int64[n+1]) -- the layout the
 reference's PointContainer<char> holds after read_seqs
 (include/point.hpp:170-205).  Edges are the pairs i < j whose Levenshtein
-distance (drivers/systolic.cpp:467-495) is <= eps, weight = float(distance).
+distance (drivers/systolic.cpp:271-299) is <= eps, weight = float(distance).
 """
 from __future__ import annotations
 

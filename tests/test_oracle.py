@@ -16,7 +16,7 @@ F = np.float32
 
 
 # ---------------------------------------------------------------------------
-# L2 distance in the reference order (drivers/systolic.cpp:449-465)
+# L2 distance in the reference order (drivers/systolic.cpp:253-269)
 # ---------------------------------------------------------------------------
 
 def test_l2_known_integer_answers(orc):
@@ -189,7 +189,7 @@ def test_eps_pairs_cap_and_count(orc):
 
 
 # ---------------------------------------------------------------------------
-# Edit distance (drivers/systolic.cpp:467-495)
+# Edit distance (drivers/systolic.cpp:271-299)
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("s,t,d", [
