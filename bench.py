@@ -65,7 +65,9 @@ def bench_cells(args):
     degree ~25.  One step = ghost test + batched per-cell query + dedup over
     the resident cells; the landmark selection and the cell grouping are
     timed once."""
-    from rnng_amd import Stats, l2, synth, voronoi
+    from rnng_amd import Stats, costmodel, l2, synth, voronoi
+    have_model = costmodel.use_default()
+    log(f"cost model: {'shipped model' if have_model else 'none (count-then-fill)'}")
     t0 = time.perf_counter()
     n = args.n if args.n != 1_000_000 else 2_000_000
     pts, eps = synth.points_vmf(n=n)
@@ -93,6 +95,7 @@ def bench_cells(args):
     dt = time.perf_counter() - t0
     K = args.steps
     log(f"cells: {K} steps {dt / K * 1e3:.2f} ms/step, {ne} edges, stats {st.as_dict()}")
+    tv, tx, tms = v.tuning()
     P_all = n * (n - 1) // 2
     out = {
         "metric": "ε-graph edges/s via Voronoi cells + ghosts, 2M×96",
@@ -108,7 +111,12 @@ def bench_cells(args):
         "ghost_tests": st.ghost_tests // K, "ghost_candidates": st.ghost_candidates // K,
         "ghosts": st.ghosts // K, "candidates": st.candidates // K,
         "band_pairs": st.band_pairs // K, "band_flips": st.band_flips // K, "refills": st.refills,
+        "cost_model": {"used": have_model, "filter_passes_per_step": st.filter_passes / K,
+                       "predicted_candidates": st.predicted_candidates // K,
+                       "predicted_edges": st.predicted_edges // K},
         "landmarks_and_cells_ms": t_lm * 1e3,
+        "autotuned_filter": {"variant": tv, "xcd_order": tx,
+                             "config_ms_on_an_eighth": [round(x, 3) for x in tms.tolist()]},
         "cell_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
         "stage_ms": {"ghost_test": st.ghost_ms / K, "prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck_dedup": st.recheck_ms / K, "device_total": st.total_ms / K},

@@ -56,6 +56,9 @@ typedef struct rnng_stats {
     int64_t ghost_candidates; /* Voronoi: center tests passed (with repeats) */
     int64_t ghosts;           /* Voronoi: distinct (cell, point) ghosts */
     double ghost_ms;          /* Voronoi: cell adjacency and cell-ordered layout */
+    int64_t filter_passes;    /* Voronoi: filter launches (2 for count-then-fill) */
+    int64_t predicted_candidates; /* Voronoi: cost model's candidate estimate (0 without) */
+    int64_t predicted_edges;      /* Voronoi: cost model's edge estimate (0 without) */
 } rnng_stats;
 
 /* Opaque handles. */
@@ -229,6 +232,20 @@ int rnng_voronoi_cells(rnng_voronoi_t v, int64_t *cell_ptr, int64_t *cell_ids);
 int rnng_voronoi_graph_build(rnng_voronoi_t v, float eps, int32_t mode, rnng_graph_t *out,
                              rnng_stats *stats);
 
+/* Per-handle switches of rnng_voronoi_graph_build (1 on, 0 off, -1 the
+ * default: on unless RNNG_CELL_GRAPH / RNNG_CELL_TUNE is 0): use_graph replays
+ * the cost-model-sized fill pass (filter, recheck, reference semantics,
+ * per-cell counts) as a captured HIP graph; autotune times every tuned filter
+ * configuration on every 8th work item at the next build and keeps the
+ * fastest. */
+int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotune);
+
+/* Filter configuration in use (variant: 1, 3 or 2 = 2, 3 or 4 row tiles per
+ * wave with LDS-staged columns, 8 = direct-to-LDS group loads; xcd_order: the
+ * XCD-ordered work list) and the autotuner's timings of the 8 (variant x
+ * order) candidates in ms (config_ms may be NULL; zeros if not tuned). */
+int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
+
 /* Ghosts found by the last rnng_voronoi_graph_build: their count, and the
  * (cell, point) pairs sorted by cell then point into pairs[2*cap].
  * Replaces: VoronoiCell::ghosts(), include/point.h:120. */
@@ -236,6 +253,42 @@ int rnng_voronoi_num_ghosts(rnng_voronoi_t v, int64_t *n);
 int rnng_voronoi_ghosts(rnng_voronoi_t v, int64_t *pairs, int64_t cap);
 
 int rnng_voronoi_destroy(rnng_voronoi_t v);
+
+/* ------------------------------------------------------------------------ */
+/* Cost model of the cell query (no counterpart in the reference)            */
+/* ------------------------------------------------------------------------ */
+
+/* A 4 -> 64 -> 64 -> 2 perceptron (ReLU, FP32) mapping per-cell features to
+ * log-scaled per-cell candidate and edge counts.  Parameters are packed
+ * W1[64][4], b1[64], W2[64][64], b2[64], W3[2][64], b3[2] (4610 floats).  All
+ * buffers are host memory; the work runs in the library's kernels. */
+#define RNNG_MLP_NPARAMS 4610
+
+/* y[batch][2] = the network on x[batch][4] (the inference kernel). */
+int rnng_mlp_infer(const float *params, const float *x, int64_t batch, float *y);
+
+/* Mean squared error over batch and outputs against target[batch][2], and its
+ * gradient with respect to the parameters (forward + backward kernels). */
+int rnng_mlp_loss_grad(const float *params, const float *x, const float *target, int64_t batch,
+                       float *loss, float *grad);
+
+/* `iters` full-batch Adam steps (lr, betas 0.9/0.999) on the GPU, updating
+ * params in place; final_loss (may be NULL) = loss of the result. */
+int rnng_mlp_train(float *params, const float *x, const float *target, int64_t batch, int32_t iters,
+                   float lr, float *final_loss);
+
+/* Cost model used by rnng_voronoi_graph_build to size its candidate and edge
+ * buffers (one filter pass; a second only if the estimate was short).  NULL
+ * removes it: builds then count first and fill in a second pass. */
+int rnng_set_cost_model(const float *params);
+
+/* Per-cell features (m x 4) and actual (candidates, edges) counts (m x 2) of
+ * the last rnng_voronoi_graph_build, counted by the row point's cell: the
+ * cost model's training data.  Features: log2(1+n)/16, log2(1+paired column
+ * points)/24, log2(1+R/eps)/8 (R = largest member distance to the center),
+ * log2(1+n*k)/24 (k = mean number of members within eps of the first
+ * min(32, n) members); targets are log2(1+count)/24. */
+int rnng_voronoi_cell_stats(rnng_voronoi_t v, float *features, float *counts);
 
 /* Largest float T with sqrtf(T) <= eps (-1 when no value qualifies).  The
  * kernels compare squared distances against T; sqrtf is monotone, so

@@ -251,3 +251,57 @@ def rips_filtration(pts, eps, maxdim: int = 2, nthreads: int = 0):
     ids = np.empty(cnt, np.int64)
     lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, _ptr(val), _ptr(ids), cnt)
     return val, ids
+
+
+# ---------------------------------------------------------------------------
+# Cost-model perceptron (no reference counterpart): float64 numpy restatement
+# of the library's 4 -> 64 -> 64 -> 2 ReLU network and its mean-squared-error
+# gradient.  Pinned by central finite differences (tests/test_costmodel_cpu.py).
+# ---------------------------------------------------------------------------
+MLP_I, MLP_H, MLP_O = 4, 64, 2
+MLP_NPARAMS = MLP_H * MLP_I + MLP_H + MLP_H * MLP_H + MLP_H + MLP_O * MLP_H + MLP_O
+
+
+def mlp_unpack(p):
+    p = np.asarray(p, dtype=np.float64)
+    o = 0
+    out = []
+    for shape in [(MLP_H, MLP_I), (MLP_H,), (MLP_H, MLP_H), (MLP_H,), (MLP_O, MLP_H), (MLP_O,)]:
+        k = int(np.prod(shape))
+        out.append(p[o:o + k].reshape(shape))
+        o += k
+    return out
+
+
+def mlp_forward(p, x):
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p)
+    x = np.asarray(x, dtype=np.float64)
+    h1 = np.maximum(x @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+    return h2 @ W3.T + b3
+
+
+def mlp_loss_grad(p, x, t):
+    """(loss, grad) of mean((y - t)^2) over batch and outputs; ReLU'(0) = 0."""
+    W1, b1, W2, b2, W3, b3 = mlp_unpack(p)
+    x = np.asarray(x, dtype=np.float64)
+    t = np.asarray(t, dtype=np.float64)
+    B = x.shape[0]
+    z1 = x @ W1.T + b1
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W2.T + b2
+    h2 = np.maximum(z2, 0.0)
+    y = h2 @ W3.T + b3
+    r = y - t
+    loss = float(np.mean(r * r))
+    dy = 2.0 * r / (B * MLP_O)
+    gW3 = dy.T @ h2
+    gb3 = dy.sum(0)
+    dz2 = (dy @ W3) * (h2 > 0)
+    gW2 = dz2.T @ h1
+    gb2 = dz2.sum(0)
+    dz1 = (dz2 @ W2) * (h1 > 0)
+    gW1 = dz1.T @ x
+    gb1 = dz1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+    return loss, g

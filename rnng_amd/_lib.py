@@ -48,6 +48,9 @@ class Stats(C.Structure):
         ("ghost_candidates", C.c_int64),
         ("ghosts", C.c_int64),
         ("ghost_ms", C.c_double),
+        ("filter_passes", C.c_int64),
+        ("predicted_candidates", C.c_int64),
+        ("predicted_edges", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -94,9 +97,16 @@ _SIGS = {
     "rnng_voronoi_cells": (C.c_int, [_vp, _vp, _vp]),
     "rnng_voronoi_graph_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp),
                                            C.POINTER(Stats)]),
+    "rnng_voronoi_set_options": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "rnng_voronoi_tuning": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _vp]),
     "rnng_voronoi_num_ghosts": (C.c_int, [_vp, _i64p]),
     "rnng_voronoi_ghosts": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_voronoi_destroy": (C.c_int, [_vp]),
+    "rnng_mlp_infer": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
+    "rnng_mlp_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp]),
+    "rnng_mlp_train": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int32, C.c_float, _vp]),
+    "rnng_set_cost_model": (C.c_int, [_vp]),
+    "rnng_voronoi_cell_stats": (C.c_int, [_vp, _vp, _vp]),
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),

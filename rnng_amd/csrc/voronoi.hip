@@ -41,6 +41,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "mfma.hpp"
+#include "mlp.hpp"
 
 struct rnng_voronoi_s {
     const rnng_dataset_s* ds = nullptr;   // borrowed: must outlive the handle
@@ -56,6 +57,24 @@ struct rnng_voronoi_s {
     // ghosts of the last graph build, (cell << 32 | point) sorted
     rnng::DevBuf gkeys;
     int64_t n_ghosts = 0;
+    // cost-model features (m x 4) and counts (m x 2: candidates, edges) of the
+    // last graph build
+    rnng::DevBuf feat;
+    rnng::DevBuf counts;
+    // graph-build workspace, grown on demand and kept across builds
+    rnng::LocalBuf rad, cnt_dev, adj_dev, rs, slot, items, yd, counters, cand, cid, cflag, cw, tune_items;
+    rnng::MfmaPrep prep;
+    // filter configuration chosen by the autotuner on the first build
+    int tuned_variant = -1, tuned_xcd = 1;
+    double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int use_graph = -1, autotune = -1;   // -1: from RNNG_CELL_GRAPH / RNNG_CELL_TUNE (default on)
+    // captured fill pass (filter, id mapping, recheck, reference semantics,
+    // per-cell counts) and the arguments it was captured with
+    hipGraphExec_t fill_exec = nullptr;
+    std::vector<uint64_t> fill_key;
+    ~rnng_voronoi_s() {
+        if (fill_exec) (void)hipGraphExecDestroy(fill_exec);
+    }
 };
 
 namespace rnng {
@@ -443,6 +462,93 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uin
     if (e < n) flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
 }
 
+// Cost-model features of cell a (one workgroup per cell):
+//   f0 = log2(1 + n) / 16                   members
+//   f1 = log2(1 + n/2 + paired) / 24         column points its tiles pair with
+//                                            (every member of adjacent later cells)
+//   f2 = log2(1 + R / eps) / 8               radius against eps
+//   f3 = log2(1 + n * k) / 24                within-cell neighbour estimate: k =
+//                                            mean count of members within eps of
+//                                            the first min(32, n) members (exact FP32)
+constexpr int FS = 32;   // sampled members per cell
+
+__global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__ X, int d,
+                                                       const unsigned* __restrict__ adj, int words,
+                                                       const int* __restrict__ cnt, const int* __restrict__ cptr,
+                                                       const int* __restrict__ cids, const unsigned* __restrict__ rad,
+                                                       float eps, float T, int m, float* __restrict__ feat) {
+#pragma clang fp contract(off)
+    extern __shared__ float smp[];   // [FS][d] sampled members
+    const int a = blockIdx.x;
+    __shared__ long long scol[256];
+    __shared__ long long snb[256];
+    const int n = cnt[a];
+    const int S = min(FS, n);
+    for (int e = threadIdx.x; e < S * d; e += 256) {
+        const int s = e / d, k = e % d;
+        smp[e] = X[(size_t)cids[cptr[a] + s] * d + k];
+    }
+    long long col = 0;
+    const unsigned* row = adj + (size_t)a * words;
+    for (int w = threadIdx.x; w < words; w += 256) {
+        unsigned bits = row[w];
+        if (w == (a >> 5)) bits &= ~0u << (a & 31);      // later cells only
+        else if (w < (a >> 5)) bits = 0u;
+        while (bits) {
+            const int b = w * 32 + __builtin_ctz(bits);
+            bits &= bits - 1;
+            if (b > a && b < m) col += cnt[b];
+        }
+    }
+    __syncthreads();
+    long long nb = 0;
+    for (int k = cptr[a] + threadIdx.x; k < cptr[a + 1]; k += 256) {
+        const float* q = X + (size_t)cids[k] * d;
+        for (int s = 0; s < S; ++s) {
+            const float* sp = smp + s * d;
+            float acc = 0.0f;
+            for (int j = 0; j < d; ++j) {
+                const float t = sp[j] - q[j];
+                acc = acc + t * t;
+            }
+            nb += acc <= T ? 1 : 0;
+        }
+    }
+    scol[threadIdx.x] = col;
+    snb[threadIdx.x] = nb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            scol[threadIdx.x] += scol[threadIdx.x + o];
+            snb[threadIdx.x] += snb[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double e = fmax((double)eps, 1e-30);
+        const double r = n ? (double)ord2f_v(rad[a]) : 0.0;
+        const double kk = S ? (double)snb[0] / S : 0.0;
+        feat[a * 4 + 0] = (float)(log2(1.0 + n) / 16.0);
+        feat[a * 4 + 1] = (float)(log2(1.0 + 0.5 * n + (double)scol[0]) / 24.0);
+        feat[a * 4 + 2] = (float)(log2(1.0 + fmin(r / e, 1e30)) / 8.0);
+        feat[a * 4 + 3] = (float)(log2(1.0 + n * kk) / 24.0);
+    }
+}
+
+// Per-cell (candidates, edges) by the cell of the candidate's row point.
+__global__ void k_cell_counts(const int2* __restrict__ cand, const uint8_t* __restrict__ flag,
+                              const unsigned long long* __restrict__ counters, unsigned long long cap,
+                              const int* __restrict__ slot, const int* __restrict__ cell,
+                              unsigned* __restrict__ cc) {
+    const unsigned long long nc = min(counters[0], cap);
+    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (unsigned long long)gridDim.x * blockDim.x) {
+        const int a = cell[slot[cand[c].x]];
+        atomicAdd(&cc[2 * a], 1u);
+        if (flag[c]) atomicAdd(&cc[2 * a + 1], 1u);
+    }
+}
+
 int sort_keys_u64(uint64_t* keys, int64_t n, unsigned end_bit) {
     if (n <= 1) return RNNG_OK;
     LocalBuf alt, tmp;
@@ -567,11 +673,29 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Cell-query tuning: filter variant (row block shape) and XCD ordering.
-struct CellTune {
-    int variant = -1;   // -1: the library default
-    int xcd = 1;
-};
+// Autotuned filter configurations: (variant, XCD ordering).  Variant 1 =
+// 2 row tiles per wave, 3 = 3, 2 = 4 (all with LDS-staged column tiles), 8 =
+// the direct-to-LDS group kernel with 2 column tiles per step.
+constexpr int kTuneVariants[4] = {1, 3, 2, 8};
+constexpr int kPadRows = 1536;   // multiple of every tuned row block (256, 384, 512 rows)
+
+int tuning_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        on = 1;
+        if (const char* e = getenv("RNNG_CELL_TUNE")) on = atoi(e) ? 1 : 0;
+    }
+    return on;
+}
+
+int graph_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        on = 1;
+        if (const char* e = getenv("RNNG_CELL_GRAPH")) on = atoi(e) ? 1 : 0;
+    }
+    return on;
+}
 
 // Work list of the cell-ordered self join: for every row block, the column
 // tiles of its own cells from the block's first tile on and of every cell
@@ -635,16 +759,15 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     const float eps_band = (float)((double)eps * (1.0 + 1e-5));
     float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
     if (Tb < T) Tb = T;
-    CellTune tune;
-    const int variant = tune.variant >= 0 ? tune.variant : filter_default_variant();
-    const int bt = filter_block_tiles(variant);
 
     Events ev;
     RNNG_TRY(ev.create());
     RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
     // ---- cell adjacency ----
     const int words = (m + 31) / 32;
-    LocalBuf rad, cnt_dev, adj_dev;
+    LocalBuf& rad = v->rad;
+    LocalBuf& cnt_dev = v->cnt_dev;
+    LocalBuf& adj_dev = v->adj_dev;
     RNNG_TRY(rad.reserve((size_t)m * sizeof(unsigned)));
     RNNG_TRY(adj_dev.reserve((size_t)m * words * sizeof(unsigned)));
     RNNG_TRY(launch_fill_u32(rad.as<unsigned>(), m, 0u));
@@ -660,74 +783,222 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     std::vector<int> R(m);
     for (int b = 0; b < m; ++b) R[b] = (v->cnt[b] + 31) & ~31;
     const std::vector<int> rs = prefix(R);
-    const int bt_rows = bt * 32;
-    const int n_pad = std::max(bt_rows, (rs[m] + bt_rows - 1) / bt_rows * bt_rows);
-    const int n_rowblocks = n_pad / bt_rows;
-    std::vector<int4> items;
-    const int64_t covered = cell_work(rs, adj, words, m, bt, n_rowblocks, items);
-    if (tune.xcd) xcd_permute(items);
-    LocalBuf d_rs, slot, d_items;
+    const int n_pad = std::max(kPadRows, (rs[m] + kPadRows - 1) / kPadRows * kPadRows);
+    LocalBuf& d_rs = v->rs;
+    LocalBuf& slot = v->slot;
+    LocalBuf& d_items = v->items;
     RNNG_TRY(upload(d_rs, rs));
-    RNNG_TRY(upload(d_items, items));
     RNNG_TRY(slot.reserve((size_t)n_pad * sizeof(int)));
     RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
     k_slot_map<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), d_rs.as<int>(), slot.as<int>());
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
     // ---- BF16 fragments of the cell-ordered points ----
-    MfmaPrep P;
+    MfmaPrep& P = v->prep;
     RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
     RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
 
-    LocalBuf counters, cand, cid, cflag, cw;
+    LocalBuf& counters = v->counters;
     RNNG_TRY(counters.reserve(8 * sizeof(unsigned long long)));
-    unsigned long long cap = std::max<unsigned long long>(1 << 20, 16ull * (unsigned long long)n);
-    unsigned long long ncand = 0;
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        RNNG_TRY(cand.reserve(cap * sizeof(int2)));
-        RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
-        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+    auto filter_args = [&](const DevBuf& work, size_t n_items, int2* cand_ptr, unsigned long long cap_) {
         FilterArgs fa;
         fa.xa = P.xa.as<bf16x8_t>();
         fa.yb = P.yb.as<bf16x8_t>();
-        fa.work = d_items.as<int4>();
-        fa.n_items = (unsigned)items.size();
+        fa.work = work.as<int4>();
+        fa.n_items = (unsigned)n_items;
         fa.n_rows = n_pad;
         fa.n_cols = n_pad;
         fa.self_join = 1;
         fa.prm = P.params.as<MfmaParams>();
-        fa.cand = cand.as<int2>();
-        fa.cap = cap;
+        fa.cand = cand_ptr;
+        fa.cap = cap_;
         fa.counter = counters.as<unsigned long long>();
-        RNNG_TRY(filter_dispatch(fa, KS, variant));
-        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
-        RNNG_HIP_TRY(hipMemcpyAsync(&ncand, counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
-        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
-        if (ncand <= cap) break;
-        if (st) st->refills++;
-        cap = ncand + ncand / 8 + 1024;
-        if (attempt == 2) {
-            set_error("candidate buffer refill did not converge");
-            return RNNG_ERR_HIP;
+        return fa;
+    };
+    auto work_list = [&](int var, int xcd, std::vector<int4>& out) {
+        const int bt = filter_block_tiles(var);
+        const int64_t cov = cell_work(rs, adj, words, m, bt, (rs[m] + bt * 32 - 1) / (bt * 32), out);
+        if (xcd) xcd_permute(out);
+        return cov;
+    };
+    // ---- autotuner: every (variant, XCD order) on every 8th work item ----
+    if (v->tuned_variant < 0) {
+        v->tuned_variant = filter_default_variant();
+        v->tuned_xcd = 1;
+        if (v->autotune >= 0 ? v->autotune : tuning_enabled()) {
+            double best = 1e30;
+            int k = 0;
+            for (int var : kTuneVariants) {
+                for (int xcd = 0; xcd < 2; ++xcd, ++k) {
+                    std::vector<int4> full, sub;
+                    work_list(var, 0, full);
+                    for (size_t i = 0; i < full.size(); i += 8) sub.push_back(full[i]);
+                    if (xcd) xcd_permute(sub);
+                    RNNG_TRY(upload(v->tune_items, sub));
+                    double t = 0.0;
+                    for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
+                        RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
+                        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+                        RNNG_TRY(filter_dispatch(filter_args(v->tune_items, sub.size(), nullptr, 0ull), KS, var));
+                        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+                        RNNG_HIP_TRY(hipEventSynchronize(ev.e[4]));
+                        t = ev.ms(3, 4);
+                    }
+                    v->tune_ms[k] = t;
+                    if (t < best) {
+                        best = t;
+                        v->tuned_variant = var;
+                        v->tuned_xcd = xcd;
+                    }
+                }
+            }
         }
     }
-    // ---- FP32 recheck, reference semantics across cells, compaction ----
-    const size_t nc1 = std::max<size_t>(1, ncand);
-    RNNG_TRY(cid.reserve(nc1 * sizeof(int2)));
-    RNNG_TRY(cflag.reserve(nc1));
-    RNNG_TRY(cw.reserve(nc1 * sizeof(float)));
-    RNNG_TRY(v->gkeys.reserve(2 * nc1 * sizeof(uint64_t)));
-    const unsigned egrid =
-        (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(8192, (ncand + 255) / 256));
-    k_cand_ids<<<egrid, 256, 0, stream()>>>(cand.as<int2>(), counters.as<unsigned long long>(), cap,
-                                            slot.as<int>(), cid.as<int2>());
+    const int variant = v->tuned_variant;
+    std::vector<int4> items;
+    const int64_t covered = work_list(variant, v->tuned_xcd, items);
+    RNNG_TRY(upload(d_items, items));
+
+    // ---- cost model: per-cell features -> predicted buffer sizes ----
+    RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
+    k_cell_features<<<m, 256, (size_t)FS * d * sizeof(float), stream()>>>(
+        v->ds->X, d, adj_dev.as<unsigned>(), words, cnt_dev.as<int>(), v->cptr.as<int>(), v->cids.as<int>(),
+        rad.as<unsigned>(), eps, T, m, v->feat.as<float>());
     RNNG_HIP_TRY(hipGetLastError());
-    RNNG_TRY(launch_recheck(v->ds->X, v->ds->X, d, cid.as<int2>(), cap, ncand, T, eps, cflag.as<uint8_t>(),
-                            cw.as<float>(), counters.as<unsigned long long>()));
-    k_ref_semantics<<<egrid, 256, 0, stream()>>>(v->ds->X, v->C.as<float>(), d, v->cell.as<int>(),
-                                                 v->delta.as<float>(), eps, cid.as<int2>(), cflag.as<uint8_t>(),
-                                                 cap, counters.as<unsigned long long>(), v->gkeys.as<uint64_t>());
-    RNNG_HIP_TRY(hipGetLastError());
+    const float* model = cost_model_device();
+    double pred_cand = 0.0, pred_edges = 0.0;
+    if (model) {
+        LocalBuf& yd = v->yd;
+        RNNG_TRY(yd.reserve((size_t)m * 2 * sizeof(float)));
+        RNNG_TRY(mlp_infer_device(model, v->feat.as<float>(), m, yd.as<float>()));
+        std::vector<float> y;
+        RNNG_TRY(download(y, yd, (size_t)m * 2));
+        for (int a = 0; a < m; ++a) {
+            pred_cand += std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
+            pred_edges += std::exp2(24.0 * std::min(std::max((double)y[2 * a + 1], 0.0), 1.5)) - 1.0;
+        }
+    }
+
+    LocalBuf& cand = v->cand;
+    LocalBuf& cid = v->cid;
+    LocalBuf& cflag = v->cflag;
+    LocalBuf& cw = v->cw;
+    // With the model: one pass into buffers sized by its estimate (+25 %), a
+    // second only if that was short.  Without: count-then-fill, a counting
+    // pass (capacity 0) and a filling pass at the exact size.
+    unsigned long long cap = model ? (unsigned long long)(pred_cand * 1.25) + 4096 : 0ull;
+    unsigned long long ncand = 0;
+    int passes = 0;
+    bool graphed = false;
+    // Buffers of a fill pass of capacity c (grown only, so their addresses stay
+    // put across builds of one handle and a captured pass can be replayed).
+    auto reserve_fill = [&](unsigned long long c) -> int {
+        const size_t c1 = (size_t)std::max<unsigned long long>(1, c);
+        RNNG_TRY(cand.reserve(c1 * sizeof(int2)));
+        RNNG_TRY(cid.reserve(c1 * sizeof(int2)));
+        RNNG_TRY(cflag.reserve(c1));
+        RNNG_TRY(cw.reserve(c1 * sizeof(float)));
+        RNNG_TRY(v->gkeys.reserve(2 * c1 * sizeof(uint64_t)));
+        RNNG_TRY(v->counts.reserve((size_t)m * 2 * sizeof(unsigned)));
+        return RNNG_OK;
+    };
+    // The fill pass: filter into capacity c, then candidate ids, FP32 recheck,
+    // reference semantics and per-cell counts, all reading the candidate count
+    // from the device (grids are sized by c), so the pass can be captured.
+    auto fill_pass = [&](unsigned long long c) -> int {
+        RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
+        RNNG_TRY(filter_dispatch(filter_args(d_items, items.size(), cand.as<int2>(), c), KS, variant));
+        const unsigned eg =
+            (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(8192, (c + 255) / 256));
+        k_cand_ids<<<eg, 256, 0, stream()>>>(cand.as<int2>(), counters.as<unsigned long long>(), c, slot.as<int>(),
+                                             cid.as<int2>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(launch_recheck(v->ds->X, v->ds->X, d, cid.as<int2>(), c, c, T, eps, cflag.as<uint8_t>(),
+                                cw.as<float>(), counters.as<unsigned long long>()));
+        k_ref_semantics<<<eg, 256, 0, stream()>>>(v->ds->X, v->C.as<float>(), d, v->cell.as<int>(),
+                                                  v->delta.as<float>(), eps, cid.as<int2>(), cflag.as<uint8_t>(), c,
+                                                  counters.as<unsigned long long>(), v->gkeys.as<uint64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(launch_fill_u32(v->counts.as<unsigned>(), 2 * m, 0u));
+        k_cell_counts<<<eg, 256, 0, stream()>>>(cand.as<int2>(), cflag.as<uint8_t>(),
+                                                counters.as<unsigned long long>(), c, slot.as<int>(),
+                                                v->cell.as<int>(), v->counts.as<unsigned>());
+        RNNG_HIP_TRY(hipGetLastError());
+        return RNNG_OK;
+    };
+    auto read_count = [&]() -> int {
+        RNNG_HIP_TRY(hipMemcpyAsync(&ncand, counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        return RNNG_OK;
+    };
+    if (model && cap > 0) {
+        // one pass sized by the model; captured once and replayed while the
+        // buffers, the work list and eps are unchanged
+        RNNG_TRY(reserve_fill(cap));
+        uint32_t eb, tb;
+        std::memcpy(&eb, &eps, 4);
+        std::memcpy(&tb, &T, 4);
+        const std::vector<uint64_t> key = {
+            (uint64_t)(uintptr_t)cand.ptr, (uint64_t)(uintptr_t)cid.ptr,  (uint64_t)(uintptr_t)cflag.ptr,
+            (uint64_t)(uintptr_t)cw.ptr,   (uint64_t)(uintptr_t)v->gkeys.ptr, (uint64_t)(uintptr_t)v->counts.ptr,
+            (uint64_t)(uintptr_t)d_items.ptr, (uint64_t)items.size(), (uint64_t)(uintptr_t)P.xa.ptr,
+            (uint64_t)(uintptr_t)P.yb.ptr, (uint64_t)(uintptr_t)P.params.ptr, (uint64_t)(uintptr_t)slot.ptr,
+            (uint64_t)(uintptr_t)counters.ptr, (uint64_t)(uintptr_t)v->ds->X, (uint64_t)(uintptr_t)v->C.ptr,
+            (uint64_t)(uintptr_t)v->cell.ptr, (uint64_t)(uintptr_t)v->delta.ptr, cap, (uint64_t)variant,
+            (uint64_t)n_pad, (uint64_t)eb, (uint64_t)tb, (uint64_t)m, (uint64_t)d};
+        if (v->use_graph >= 0 ? v->use_graph : graph_enabled()) {
+            if (!v->fill_exec || v->fill_key != key) {
+                if (v->fill_exec) {
+                    (void)hipGraphExecDestroy(v->fill_exec);
+                    v->fill_exec = nullptr;
+                }
+                RNNG_HIP_TRY(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+                const int fs = fill_pass(cap);
+                hipGraph_t graph = nullptr;
+                const hipError_t ec = hipStreamEndCapture(stream(), &graph);
+                if (fs != RNNG_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return fs;
+                }
+                RNNG_HIP_TRY(ec);
+                const hipError_t ei = hipGraphInstantiate(&v->fill_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                RNNG_HIP_TRY(ei);
+                v->fill_key = key;
+            }
+            RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+            RNNG_HIP_TRY(hipGraphLaunch(v->fill_exec, stream()));
+            graphed = true;
+        } else {
+            RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+            RNNG_TRY(fill_pass(cap));
+        }
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+        ++passes;
+        RNNG_TRY(read_count());
+        if (ncand > cap && st) st->refills++;
+    } else {
+        // count-then-fill: a counting pass (capacity 0, filter only)
+        RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+        RNNG_TRY(filter_dispatch(filter_args(d_items, items.size(), nullptr, 0ull), KS, variant));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+        ++passes;
+        RNNG_TRY(read_count());
+        cap = 0;
+    }
+    if (ncand > cap || cap == 0) {
+        // fill at the exact count (after the counting pass or a short estimate)
+        cap = ncand;
+        RNNG_TRY(reserve_fill(cap));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+        RNNG_TRY(fill_pass(cap));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+        ++passes;
+        RNNG_TRY(read_count());
+    }
+    // the candidate count bounds the edge count (the model's edge estimate is
+    // reported, not trusted with memory safety)
     if ((int64_t)ncand > g->cap) RNNG_TRY(edge_buffers(g, (int64_t)ncand));
     int64_t ne = 0;
     RNNG_TRY(compact_edges(cid.as<int2>(), cw.as<float>(), cflag.as<uint8_t>(), (int64_t)ncand, g->ij.as<int2>(),
@@ -775,10 +1046,14 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         st->ghost_candidates += nhits;
         st->ghosts += ng;
         st->ghost_ms += ev.ms(0, 1);
+        st->filter_passes += passes;
+        st->predicted_candidates += (int64_t)pred_cand;
+        st->predicted_edges += (int64_t)pred_edges;
         st->prep_ms += ev.ms(1, 2);
-        st->filter_ms += ev.ms(3, 4);
-        st->recheck_ms += ev.ms(4, 5);
+        st->filter_ms += ev.ms(3, 4);   // the fill pass (filter through per-cell counts)
+        st->recheck_ms += ev.ms(4, 5);  // compaction and ghost list
         st->total_ms += ev.ms(0, 2) + ev.ms(3, 5);
+        (void)graphed;
     }
     return RNNG_OK;
 }
@@ -957,6 +1232,45 @@ int rnng_voronoi_graph_build(rnng_voronoi_t v, float eps, int32_t mode, rnng_gra
     return RNNG_OK;
 }
 
+int rnng_voronoi_cell_stats(rnng_voronoi_t v, float* features, float* counts) {
+    RNNG_TRY(check_voronoi(v));
+    const size_t m = (size_t)v->m;
+    if (!v->feat.ptr || !v->counts.ptr) {
+        set_error("voronoi_cell_stats: no graph build yet");
+        return RNNG_ERR_ARG;
+    }
+    if (features) {
+        RNNG_HIP_TRY(hipMemcpyAsync(features, v->feat.ptr, m * 4 * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    }
+    std::vector<unsigned> c;
+    RNNG_TRY(download(c, v->counts, m * 2));
+    if (counts)
+        for (size_t i = 0; i < 2 * m; ++i) counts[i] = (float)(std::log2(1.0 + (double)c[i]) / 24.0);
+    return RNNG_OK;
+}
+
+int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotune) {
+    RNNG_TRY(check_voronoi(v));
+    v->use_graph = use_graph < 0 ? -1 : (use_graph ? 1 : 0);
+    v->autotune = autotune < 0 ? -1 : (autotune ? 1 : 0);
+    if (v->use_graph == 0 && v->fill_exec) {
+        (void)hipGraphExecDestroy(v->fill_exec);
+        v->fill_exec = nullptr;
+        v->fill_key.clear();
+    }
+    if (v->autotune >= 0) v->tuned_variant = -1;   // choose again at the next build
+    return RNNG_OK;
+}
+
+int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t* variant, int32_t* xcd_order, double* config_ms) {
+    RNNG_TRY(check_voronoi(v));
+    if (variant) *variant = v->tuned_variant;
+    if (xcd_order) *xcd_order = v->tuned_xcd;
+    if (config_ms)
+        for (int k = 0; k < 8; ++k) config_ms[k] = v->tune_ms[k];
+    return RNNG_OK;
+}
+
 int rnng_voronoi_num_ghosts(rnng_voronoi_t v, int64_t* n) {
     RNNG_TRY(check_voronoi(v));
     if (!n) {
@@ -984,7 +1298,8 @@ int rnng_voronoi_ghosts(rnng_voronoi_t v, int64_t* pairs, int64_t cap) {
 
 int rnng_voronoi_destroy(rnng_voronoi_t v) {
     if (!v) return RNNG_OK;
-    for (DevBuf* b : {&v->ctr, &v->C, &v->cell, &v->delta, &v->cptr, &v->cids, &v->gkeys}) b->release();
+    for (DevBuf* b : {&v->ctr, &v->C, &v->cell, &v->delta, &v->cptr, &v->cids, &v->gkeys, &v->feat, &v->counts})
+        b->release();
     delete v;
     return RNNG_OK;
 }

@@ -90,6 +90,18 @@ class Voronoi:
                                                   C.byref(st)))
         return Graph(h, st)
 
+    def set_options(self, use_graph: int = -1, autotune: int = -1) -> None:
+        """1 on, 0 off, -1 default (on) for the captured fill pass and the
+        filter autotuner of the next builds."""
+        check(_lib.lib().rnng_voronoi_set_options(self._h, int(use_graph), int(autotune)))
+
+    def tuning(self):
+        """(variant, xcd_order, timings ms of the 8 tuned configurations)."""
+        var, xcd = C.c_int32(), C.c_int32()
+        ms = np.zeros(8, np.float64)
+        check(_lib.lib().rnng_voronoi_tuning(self._h, C.byref(var), C.byref(xcd), _vp(ms)))
+        return var.value, xcd.value, ms
+
     def ghosts(self) -> np.ndarray:
         """(cell, point) pairs of the last graph build, sorted."""
         n = C.c_int64()

@@ -1,0 +1,79 @@
+"""Runtime machinery of the Voronoi cell build: the captured fill pass
+(HIP graph) replayed several times with other GPU work in between, each
+replay bit-identical to the eager build, and the filter autotuner, whose
+choice must not change the result."""
+import numpy as np
+import pytest
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(seed=3, n=20000, d=32):
+    from rnng_amd import synth
+    rng = np.random.default_rng(seed)
+    k = 16
+    mu = rng.uniform(0, 8, size=(k, d)).astype(np.float32)
+    pts = (mu[rng.integers(0, k, n)] + 0.8 * rng.standard_normal((n, d))).astype(np.float32)
+    return pts, synth.radius_for_degree(pts, 20.0)
+
+
+def _edges(v, eps):
+    g = v.graph(eps)
+    ij, w = g.edges(sorted=True)
+    st = g.stats
+    g.close()
+    return ij, w, v.ghosts(), st
+
+
+def test_fill_graph_replays_match_eager(orc):
+    from rnng_amd import costmodel, l2, voronoi
+    p = costmodel.load()
+    if p is None:
+        pytest.skip("no shipped cost model")
+    pts, eps = _data()
+    ds = l2.Dataset(pts)
+    other = l2.Dataset(pts[:3000])
+    v = voronoi.Voronoi(ds, centers=voronoi.selection_sample(pts.shape[0], 64, 5))
+    costmodel.set_cost_model(p)
+    try:
+        v.set_options(use_graph=0, autotune=0)
+        ij0, w0, gh0, s0 = _edges(v, eps)
+        ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
+        assert np.array_equal(ij0, ij_ref) and w0.tobytes() == w_ref.tobytes()
+        v.set_options(use_graph=1, autotune=0)
+        for rep in range(4):
+            ij, w, gh, st = _edges(v, eps)
+            flush_print(f"replay {rep}: edges {ij.shape[0]} passes {st.filter_passes}")
+            assert np.array_equal(ij, ij0) and w.tobytes() == w0.tobytes()
+            assert np.array_equal(gh, gh0)
+            assert st.candidates == s0.candidates and st.band_pairs == s0.band_pairs
+            # other GPU work and synchronisation between replays
+            g2 = other.graph(eps, "mfma")
+            assert g2.num_edges > 0
+            g2.close()
+    finally:
+        costmodel.set_cost_model(None)
+        v.close()
+        other.close()
+        ds.close()
+
+
+def test_autotuner_choice_keeps_results(orc):
+    from rnng_amd import l2, voronoi
+    pts, eps = _data(seed=9, n=30000, d=96)
+    ds = l2.Dataset(pts)
+    v = voronoi.Voronoi(ds, fps=128, first=0)
+    v.set_options(use_graph=0, autotune=0)
+    ij0, w0, gh0, _ = _edges(v, eps)
+    var0, _, ms0 = v.tuning()
+    assert var0 == 1 and not ms0.any()
+    v.set_options(use_graph=0, autotune=1)
+    ij1, w1, gh1, _ = _edges(v, eps)
+    var, xcd, ms = v.tuning()
+    flush_print(f"autotuner: variant {var} xcd {xcd} timings {np.round(ms, 3)}")
+    assert var in (1, 3, 2, 8) and xcd in (0, 1) and (ms > 0).all()
+    assert np.array_equal(ij0, ij1) and w0.tobytes() == w1.tobytes() and np.array_equal(gh0, gh1)
+    v.close()
+    ds.close()

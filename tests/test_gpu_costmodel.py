@@ -1,0 +1,120 @@
+"""The cost model's own kernels against the float64 oracle (forward,
+mean-squared-error gradient), training on the GPU, and what the trained model
+gains over count-then-fill in the Voronoi cell build: one filter pass instead
+of two, with the identical edge set."""
+import numpy as np
+import pytest
+
+from conftest import flush_print
+
+pytestmark = pytest.mark.gpu
+
+
+def _cm():
+    from rnng_amd import costmodel
+    return costmodel
+
+
+def test_infer_matches_oracle(orc):
+    cm = _cm()
+    rng = np.random.default_rng(1)
+    p = cm.init_params(3)
+    p[256:320] = rng.standard_normal(64) * 0.1     # non-zero biases
+    x = rng.uniform(0, 1, size=(3000, 4)).astype(np.float32)
+    y = cm.infer(p, x)
+    ref = orc.mlp_forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.allclose(y, ref, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("batch", [1, 31, 32, 1000, 4133])
+def test_loss_grad_matches_oracle(orc, batch):
+    cm = _cm()
+    rng = np.random.default_rng(batch)
+    p = cm.init_params(batch)
+    p += (rng.standard_normal(p.size) * 0.05).astype(np.float32)
+    x = rng.uniform(0, 1, size=(batch, 4)).astype(np.float32)
+    t = rng.uniform(0, 1, size=(batch, 2)).astype(np.float32)
+    loss, g = cm.loss_grad(p, x, t)
+    loss_ref, g_ref = orc.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert np.isclose(loss, loss_ref, rtol=1e-4)
+    # FP32 sums over the batch against float64: tolerance scaled by the gradient size
+    tol = 1e-4 * np.abs(g_ref).max() + 1e-7
+    assert np.max(np.abs(g - g_ref)) <= tol, np.max(np.abs(g - g_ref)) / np.abs(g_ref).max()
+
+
+def test_train_fits_a_smooth_target():
+    cm = _cm()
+    rng = np.random.default_rng(7)
+    x = rng.uniform(0, 1, size=(4000, 4)).astype(np.float32)
+    t = np.stack([0.5 * x[:, 0] + 0.3 * x[:, 1] * x[:, 2], np.sqrt(x[:, 3]) * 0.4 + 0.1], 1).astype(np.float32)
+    p0 = cm.init_params(0)
+    l0, _ = cm.loss_grad(p0, x, t)
+    p, l1 = cm.train(p0, x, t, iters=800, lr=3e-3)
+    flush_print(f"train: loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < 0.02 * l0 and l1 < 2e-4
+    # the returned loss is the loss of the returned parameters
+    l2, _ = cm.loss_grad(p, x, t)
+    assert np.isclose(l1, l2, rtol=1e-5)
+
+
+def _build(pts, eps, m, seed, model):
+    from rnng_amd import l2, voronoi
+    cm = _cm()
+    cm.set_cost_model(model)
+    ds = l2.Dataset(pts)
+    v = voronoi.Voronoi(ds, centers=voronoi.selection_sample(pts.shape[0], m, seed))
+    g = v.graph(eps)
+    ij, w = g.edges(sorted=True)
+    st = g.stats
+    g.close()
+    v.close()
+    ds.close()
+    cm.set_cost_model(None)
+    return ij, w, st
+
+
+def test_cost_model_saves_the_counting_pass():
+    """Trained on per-cell counts of 48 synthetic runs, the model sizes the
+    candidate buffer of held-out runs: the build runs the filter once where
+    count-then-fill runs it twice, and the edge set is identical."""
+    cm = _cm()
+    X, T = cm.collect(runs=48, seed=11)
+    p, loss = cm.train(cm.init_params(2), X, T, iters=2500)
+    flush_print(f"collected {X.shape[0]} cells; training loss {loss:.3e}")
+    rng = np.random.default_rng(99)
+    saved = 0
+    ratios = []
+    for r in range(6):
+        pts, eps, m, _ = cm.synthetic_run(rng)
+        ij0, w0, s0 = _build(pts, eps, m, r, None)
+        ij1, w1, s1 = _build(pts, eps, m, r, p)
+        assert np.array_equal(ij0, ij1) and w0.tobytes() == w1.tobytes()
+        assert s0.filter_passes == 2 and s0.predicted_candidates == 0
+        ratio = s1.predicted_candidates / max(1, s1.candidates)
+        ratios.append(ratio)
+        flush_print(f"held-out {r}: n={pts.shape[0]} d={pts.shape[1]} m={m} candidates={s1.candidates} "
+                    f"predicted={s1.predicted_candidates} passes={s1.filter_passes}")
+        saved += s1.filter_passes == 1
+    assert saved >= 5, ratios
+    assert np.median(ratios) < 3.0, ratios
+
+
+def test_shipped_model_sizes_held_out_runs():
+    """The committed model (trained on 256 runs) on fresh synthetic runs of the
+    same families, including 100k+ point runs."""
+    cm = _cm()
+    p = cm.load()
+    if p is None:
+        pytest.skip("no shipped cost model")
+    rng = np.random.default_rng(2024)
+    saved = 0
+    for r in range(5):
+        pts, eps, m, _ = cm.synthetic_run(rng, big=0.4)
+        ij0, w0, s0 = _build(pts, eps, m, r, None)
+        ij1, w1, s1 = _build(pts, eps, m, r, p)
+        assert np.array_equal(ij0, ij1) and w0.tobytes() == w1.tobytes()
+        flush_print(f"shipped model, run {r}: n={pts.shape[0]} m={m} candidates={s1.candidates} "
+                    f"predicted={s1.predicted_candidates} edges={s1.edges} "
+                    f"predicted_edges={s1.predicted_edges} passes={s1.filter_passes}")
+        saved += s1.filter_passes == 1
+    assert saved >= 4
