@@ -55,7 +55,7 @@ typedef struct rnng_stats {
     int64_t ghost_tests;      /* Voronoi: exact center tests of cross-cell edges */
     int64_t ghost_candidates; /* Voronoi: center tests passed (with repeats) */
     int64_t ghosts;           /* Voronoi: distinct (cell, point) ghosts */
-    double ghost_ms;          /* Voronoi: cell adjacency and cell-ordered layout */
+    double ghost_ms;          /* Voronoi: cell adjacency, layout, cost-model features */
     int64_t filter_passes;    /* Voronoi: filter launches (2 for count-then-fill) */
     int64_t predicted_candidates; /* Voronoi: cost model's candidate estimate (0 without) */
     int64_t predicted_edges;      /* Voronoi: cost model's edge estimate (0 without) */
@@ -267,8 +267,10 @@ int rnng_voronoi_destroy(rnng_voronoi_t v);
 /* y[batch][2] = the network on x[batch][4] (the inference kernel). */
 int rnng_mlp_infer(const float *params, const float *x, int64_t batch, float *y);
 
-/* Mean squared error over batch and outputs against target[batch][2], and its
- * gradient with respect to the parameters (forward + backward kernels). */
+/* Loss = mean over batch and outputs of w (y - target)^2, w = 4 where
+ * y < target (an estimate below the count costs a second filter pass, one
+ * above it only memory), and its gradient with respect to the parameters
+ * (forward + backward kernels). */
 int rnng_mlp_loss_grad(const float *params, const float *x, const float *target, int64_t batch,
                        float *loss, float *grad);
 
@@ -277,8 +279,9 @@ int rnng_mlp_loss_grad(const float *params, const float *x, const float *target,
 int rnng_mlp_train(float *params, const float *x, const float *target, int64_t batch, int32_t iters,
                    float lr, float *final_loss);
 
-/* Cost model used by rnng_voronoi_graph_build to size its candidate and edge
- * buffers (one filter pass; a second only if the estimate was short).  NULL
+/* Cost model used by rnng_voronoi_graph_build to size its candidate buffer
+ * (1.5 x the summed per-cell estimates: one filter pass; a second only if the
+ * estimate was short).  NULL
  * removes it: builds then count first and fill in a second pass. */
 int rnng_set_cost_model(const float *params);
 

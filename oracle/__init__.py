@@ -281,8 +281,12 @@ def mlp_forward(p, x):
     return h2 @ W3.T + b3
 
 
+MLP_UNDER = 4.0   # loss weight of an output below its target
+
+
 def mlp_loss_grad(p, x, t):
-    """(loss, grad) of mean((y - t)^2) over batch and outputs; ReLU'(0) = 0."""
+    """(loss, grad) of mean(w (y - t)^2) over batch and outputs, w = 4 where
+    y < t, else 1; ReLU'(0) = 0."""
     W1, b1, W2, b2, W3, b3 = mlp_unpack(p)
     x = np.asarray(x, dtype=np.float64)
     t = np.asarray(t, dtype=np.float64)
@@ -293,8 +297,9 @@ def mlp_loss_grad(p, x, t):
     h2 = np.maximum(z2, 0.0)
     y = h2 @ W3.T + b3
     r = y - t
-    loss = float(np.mean(r * r))
-    dy = 2.0 * r / (B * MLP_O)
+    wr = np.where(r < 0, MLP_UNDER, 1.0)
+    loss = float(np.mean(wr * r * r))
+    dy = 2.0 * wr * r / (B * MLP_O)
     gW3 = dy.T @ h2
     gb3 = dy.sum(0)
     dz2 = (dy @ W3) * (h2 > 0)

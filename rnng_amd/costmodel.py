@@ -3,7 +3,8 @@
 A 4 -> 64 -> 64 -> 2 ReLU perceptron (FP32) maps per-cell features (members,
 paired column points, radius / eps, a sampled within-cell neighbour count;
 log-scaled) to the cell's log-scaled candidate and edge counts.  The library's own kernels run
-its forward, backward (mean squared error), Adam and inference passes
+its forward, backward (squared error, 4x weight below the target), Adam and
+inference passes
 (rnng_mlp_*).  Once set (set_cost_model), rnng_voronoi_graph_build sizes its
 candidate buffer from the summed per-cell estimates and runs the filter once;
 without it the build counts first and fills in a second pass.
@@ -185,7 +186,7 @@ def main(argv=None):
     ap.add_argument("--runs", type=int, default=256)
     ap.add_argument("--iters", type=int, default=4000)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--big", type=float, default=0.1, help="share of 100k-300k point runs")
+    ap.add_argument("--big", type=float, default=0.25, help="share of 100k-300k point runs")
     a = ap.parse_args(argv)
 
     def log(msg):

@@ -6,7 +6,9 @@
 // needs two.
 //
 // Parameters, packed: W1[64][4], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
-// (row-major, out x in).  Loss: mean over batch and outputs of (y - t)^2.
+// (row-major, out x in).  Loss: mean over batch and outputs of w (y - t)^2
+// with w = MLP_UNDER (4) where y < t: an estimate below the count costs a
+// second filter pass, one above it only buffer memory.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -99,8 +101,9 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ 
 #pragma unroll
         for (int o = 0; o < MLP_O; ++o) {
             const float r = ok ? Y[(size_t)(s0 + s) * MLP_O + o] - T[(size_t)(s0 + s) * MLP_O + o] : 0.0f;
-            dy[s][o] = scale * r;
-            l += r * r;
+            const float wr = r < 0.0f ? MLP_UNDER : 1.0f;
+            dy[s][o] = scale * wr * r;
+            l += wr * r * r;
         }
         lsum[s] = l;
     }

@@ -471,6 +471,7 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uin
 //                                            mean count of members within eps of
 //                                            the first min(32, n) members (exact FP32)
 constexpr int FS = 32;   // sampled members per cell
+constexpr int FM = 64;   // members staged in LDS per round
 
 __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__ X, int d,
                                                        const unsigned* __restrict__ adj, int words,
@@ -478,15 +479,18 @@ __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__
                                                        const int* __restrict__ cids, const unsigned* __restrict__ rad,
                                                        float eps, float T, int m, float* __restrict__ feat) {
 #pragma clang fp contract(off)
-    extern __shared__ float smp[];   // [FS][d] sampled members
+    extern __shared__ float fl[];    // [FS][d] sampled members, then [FM][d + 1] staged members
+    float* smp = fl;
+    float* mem = fl + FS * d;
+    const int ld = d + 1;
     const int a = blockIdx.x;
     __shared__ long long scol[256];
     __shared__ long long snb[256];
-    const int n = cnt[a];
+    const int n = cnt[a], base = cptr[a];
     const int S = min(FS, n);
     for (int e = threadIdx.x; e < S * d; e += 256) {
         const int s = e / d, k = e % d;
-        smp[e] = X[(size_t)cids[cptr[a] + s] * d + k];
+        smp[e] = X[(size_t)cids[base + s] * d + k];
     }
     long long col = 0;
     const unsigned* row = adj + (size_t)a * words;
@@ -500,18 +504,29 @@ __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__
             if (b > a && b < m) col += cnt[b];
         }
     }
-    __syncthreads();
+    // member r = tid / 4 of each round against samples 8 (tid % 4) .. + 7
     long long nb = 0;
-    for (int k = cptr[a] + threadIdx.x; k < cptr[a + 1]; k += 256) {
-        const float* q = X + (size_t)cids[k] * d;
-        for (int s = 0; s < S; ++s) {
-            const float* sp = smp + s * d;
-            float acc = 0.0f;
-            for (int j = 0; j < d; ++j) {
-                const float t = sp[j] - q[j];
-                acc = acc + t * t;
+    const int r = threadIdx.x >> 2, s0 = (threadIdx.x & 3) * 8;
+    for (int m0 = 0; m0 < n; m0 += FM) {
+        const int nm = min(FM, n - m0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nm * d; e += 256) {
+            const int rr = e / d, k = e % d;
+            mem[rr * ld + k] = X[(size_t)cids[base + m0 + rr] * d + k];
+        }
+        __syncthreads();
+        if (r < nm) {
+            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int k = 0; k < d; ++k) {
+                const float q = mem[r * ld + k];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const float t = smp[(s0 + j < S ? s0 + j : 0) * d + k] - q;
+                    acc[j] = acc[j] + t * t;
+                }
             }
-            nb += acc <= T ? 1 : 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) nb += (s0 + j < S && acc[j] <= T) ? 1 : 0;
         }
     }
     scol[threadIdx.x] = col;
@@ -526,11 +541,11 @@ __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__
     }
     if (threadIdx.x == 0) {
         const double e = fmax((double)eps, 1e-30);
-        const double r = n ? (double)ord2f_v(rad[a]) : 0.0;
+        const double rr = n ? (double)ord2f_v(rad[a]) : 0.0;
         const double kk = S ? (double)snb[0] / S : 0.0;
         feat[a * 4 + 0] = (float)(log2(1.0 + n) / 16.0);
         feat[a * 4 + 1] = (float)(log2(1.0 + 0.5 * n + (double)scol[0]) / 24.0);
-        feat[a * 4 + 2] = (float)(log2(1.0 + fmin(r / e, 1e30)) / 8.0);
+        feat[a * 4 + 2] = (float)(log2(1.0 + fmin(rr / e, 1e30)) / 8.0);
         feat[a * 4 + 3] = (float)(log2(1.0 + n * kk) / 24.0);
     }
 }
@@ -792,6 +807,26 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
     k_slot_map<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), d_rs.as<int>(), slot.as<int>());
     RNNG_HIP_TRY(hipGetLastError());
+    // ---- cost model: per-cell features -> predicted buffer sizes ----
+    RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
+    k_cell_features<<<m, 256, (size_t)(FS * d + FM * (d + 1)) * sizeof(float), stream()>>>(
+        v->ds->X, d, adj_dev.as<unsigned>(), words, cnt_dev.as<int>(), v->cptr.as<int>(), v->cids.as<int>(),
+        rad.as<unsigned>(), eps, T, m, v->feat.as<float>());
+    RNNG_HIP_TRY(hipGetLastError());
+    const float* model = cost_model_device();
+    double pred_cand = 0.0, pred_edges = 0.0;
+    if (model) {
+        LocalBuf& yd = v->yd;
+        RNNG_TRY(yd.reserve((size_t)m * 2 * sizeof(float)));
+        RNNG_TRY(mlp_infer_device(model, v->feat.as<float>(), m, yd.as<float>()));
+        std::vector<float> y;
+        RNNG_TRY(download(y, yd, (size_t)m * 2));
+        for (int a = 0; a < m; ++a) {
+            pred_cand += std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
+            pred_edges += std::exp2(24.0 * std::min(std::max((double)y[2 * a + 1], 0.0), 1.5)) - 1.0;
+        }
+    }
+
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
     // ---- BF16 fragments of the cell-ordered points ----
     MfmaPrep& P = v->prep;
@@ -859,34 +894,14 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     const int64_t covered = work_list(variant, v->tuned_xcd, items);
     RNNG_TRY(upload(d_items, items));
 
-    // ---- cost model: per-cell features -> predicted buffer sizes ----
-    RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
-    k_cell_features<<<m, 256, (size_t)FS * d * sizeof(float), stream()>>>(
-        v->ds->X, d, adj_dev.as<unsigned>(), words, cnt_dev.as<int>(), v->cptr.as<int>(), v->cids.as<int>(),
-        rad.as<unsigned>(), eps, T, m, v->feat.as<float>());
-    RNNG_HIP_TRY(hipGetLastError());
-    const float* model = cost_model_device();
-    double pred_cand = 0.0, pred_edges = 0.0;
-    if (model) {
-        LocalBuf& yd = v->yd;
-        RNNG_TRY(yd.reserve((size_t)m * 2 * sizeof(float)));
-        RNNG_TRY(mlp_infer_device(model, v->feat.as<float>(), m, yd.as<float>()));
-        std::vector<float> y;
-        RNNG_TRY(download(y, yd, (size_t)m * 2));
-        for (int a = 0; a < m; ++a) {
-            pred_cand += std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
-            pred_edges += std::exp2(24.0 * std::min(std::max((double)y[2 * a + 1], 0.0), 1.5)) - 1.0;
-        }
-    }
-
     LocalBuf& cand = v->cand;
     LocalBuf& cid = v->cid;
     LocalBuf& cflag = v->cflag;
     LocalBuf& cw = v->cw;
-    // With the model: one pass into buffers sized by its estimate (+25 %), a
+    // With the model: one pass into buffers sized by its estimate (+50 %), a
     // second only if that was short.  Without: count-then-fill, a counting
     // pass (capacity 0) and a filling pass at the exact size.
-    unsigned long long cap = model ? (unsigned long long)(pred_cand * 1.25) + 4096 : 0ull;
+    unsigned long long cap = model ? (unsigned long long)(pred_cand * 1.5) + 4096 : 0ull;
     unsigned long long ncand = 0;
     int passes = 0;
     bool graphed = false;

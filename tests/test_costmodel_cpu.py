@@ -23,6 +23,7 @@ def test_oracle_gradient_matches_finite_differences(orc):
         lm, _ = orc.mlp_loss_grad(p - e, x, t)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 + 1e-5 * abs(g[k]), (k, fd, g[k])
-    # loss is the mean over batch and outputs of the squared residual
+    # loss is the mean over batch and outputs of the weighted squared residual
     y = orc.mlp_forward(p, x)
-    assert np.isclose(loss, np.mean((y - t) ** 2))
+    r = y - t
+    assert np.isclose(loss, np.mean(np.where(r < 0, 4.0, 1.0) * r ** 2))
