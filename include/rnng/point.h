@@ -1,0 +1,342 @@
+// Points, point containers and Voronoi cells (include/point.h, point.hpp) for
+// one process: containers own host atoms as in the reference and keep a
+// device copy (rnng_dataset_t for float points, rnng_strings_t for char
+// strings) created on first use.
+#ifndef RNNG_HOST_POINT_H_
+#define RNNG_HOST_POINT_H_
+
+#include <fstream>
+#include <memory>
+
+#include "simplex.h"
+
+template <class Atom_>
+class Point {
+public:
+    using Atom = Atom_;
+    Point(const Atom* mem, Index dim, Index index) : mem(mem), dim(dim), index(index), container_offset(-1) {}
+    Point(const Atom* mem, Index dim, Index index, Index container_offset)
+        : mem(mem), dim(dim), index(index), container_offset(container_offset) {}
+    Index id() const { return index; }
+    Index size() const { return dim; }
+    Index offset() const { return container_offset; }
+    Atom operator[](Index i) const { return mem[i]; }
+    explicit operator const Atom*() const { return mem; }
+    const Atom* begin() const { return mem; }
+    const Atom* end() const { return mem + dim; }
+
+private:
+    const Atom* mem;
+    Index dim, index, container_offset;
+};
+
+namespace rnng_host {
+
+struct DatasetHandle {
+    rnng_dataset_t h = nullptr;
+    ~DatasetHandle() {
+        if (h) rnng_dataset_destroy(h);
+    }
+};
+struct StringsHandle {
+    rnng_strings_t h = nullptr;
+    ~StringsHandle() {
+        if (h) rnng_strings_destroy(h);
+    }
+};
+
+}  // namespace rnng_host
+
+template <class Atom_>
+class PointContainer {
+public:
+    using Atom = Atom_;
+    using AtomVector = std::vector<Atom>;
+
+    PointContainer() : offsets({0}) {}
+    explicit PointContainer(const std::vector<Point<Atom>>& points) : offsets({0}) {
+        for (const auto& p : points) push_back(p);
+    }
+    // points of equal dimension `dim`, ids given
+    PointContainer(const AtomVector& atoms, Index size, Index dim, const IndexVector& indices)
+        : data(atoms), ids(indices) {
+        offsets.resize((size_t)size + 1);
+        for (Index i = 0; i <= size; ++i) offsets[(size_t)i] = i * dim;
+    }
+    // variable-length points: sizes[i] atoms each
+    PointContainer(const AtomVector& atoms, const IndexVector& sizes, const IndexVector& indices)
+        : data(atoms), ids(indices) {
+        offsets.assign(1, 0);
+        for (Index s : sizes) offsets.push_back(offsets.back() + s);
+    }
+
+    Index num_points() const { return (Index)ids.size(); }
+    Index num_atoms() const { return (Index)data.size(); }
+    Point<Atom> operator[](Index i) const { return Point<Atom>(mem(i), size(i), id(i), i); }
+
+    // fvecs records: int32 dimension then that many float32 (include/point.hpp:112-168).
+    Index read_fvecs(const char* fname) {
+        static_assert(std::is_same<Atom, float>::value, "read_fvecs reads float points");
+        std::ifstream is(fname, std::ios::binary);
+        if (!is) fail(std::string("cannot open ") + fname);
+        is.seekg(0, is.end);
+        const size_t filesize = (size_t)is.tellg();
+        is.seekg(0, is.beg);
+        int d = 0;
+        if (filesize) is.read((char*)&d, sizeof(int));
+        is.seekg(0, is.beg);
+        const size_t rec = sizeof(int) + sizeof(float) * (size_t)d;
+        if (filesize && (d <= 0 || filesize % rec != 0)) fail(std::string("malformed fvecs file ") + fname);
+        const size_t total = filesize ? filesize / rec : 0;
+        data.resize(total * (size_t)d);
+        offsets.resize(total + 1);
+        ids.resize(total);
+        std::vector<char> p(rec);
+        for (size_t i = 0; i < total; ++i) {
+            is.read(p.data(), (std::streamsize)rec);
+            int dt;
+            std::memcpy(&dt, p.data(), sizeof(int));
+            if (dt != d) fail(std::string("mixed dimensions in ") + fname);
+            std::memcpy(&data[i * (size_t)d], p.data() + sizeof(int), sizeof(float) * (size_t)d);
+            offsets[i] = (Index)(i * (size_t)d);
+            ids[i] = (Index)i;
+        }
+        offsets[total] = (Index)(total * (size_t)d);
+        invalidate();
+        return (Index)total;
+    }
+
+    // One string per non-empty line, a trailing '\r' dropped (include/point.hpp:170-205).
+    Index read_seqs(const char* fname) {
+        static_assert(std::is_same<Atom, char>::value, "read_seqs reads strings");
+        std::ifstream is(fname);
+        if (!is) fail(std::string("cannot open ") + fname);
+        offsets.clear();
+        ids.clear();
+        data.clear();
+        std::string line;
+        Index id = 0;
+        while (std::getline(is, line)) {
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            if (line.empty()) continue;
+            offsets.push_back((Index)data.size());
+            ids.push_back(id++);
+            data.insert(data.end(), line.begin(), line.end());
+        }
+        offsets.push_back((Index)data.size());
+        invalidate();
+        return id;
+    }
+
+    void push_back(const Point<Atom>& p) {
+        data.insert(data.end(), p.begin(), p.end());
+        offsets.push_back((Index)data.size());
+        ids.push_back(p.id());
+        invalidate();
+    }
+    void indexed_gather(const PointContainer& src, const IndexVector& index_offsets) {
+        data.clear();
+        ids.clear();
+        offsets.assign(1, 0);
+        for (Index i : index_offsets) push_back(src[i]);
+    }
+    void swap(PointContainer& o) {
+        data.swap(o.data);
+        offsets.swap(o.offsets);
+        ids.swap(o.ids);
+        dev_f.swap(o.dev_f);
+        dev_s.swap(o.dev_s);
+    }
+    typename IndexVector::const_iterator ids_begin() const { return ids.begin(); }
+    typename IndexVector::const_iterator ids_end() const { return ids.end(); }
+
+    // Common dimension of float points (aborts if the points differ in size).
+    Index dim() const {
+        const Index n = num_points();
+        if (n == 0) return 0;
+        const Index d = size(0);
+        for (Index i = 1; i < n; ++i)
+            if (size(i) != d) fail("points of different dimensions");
+        return d;
+    }
+    const AtomVector& atoms() const { return data; }
+    const IndexVector& atom_offsets() const { return offsets; }
+
+    // Device copies (built once, dropped when the container changes).
+    rnng_dataset_t device_points() const {
+        static_assert(std::is_same<Atom, float>::value, "device points are float");
+        if (!dev_f) {
+            auto h = std::make_shared<rnng_host::DatasetHandle>();
+            rnng_host::check(rnng_dataset_create_f32(data.data(), num_points(), (int32_t)dim(), &h->h),
+                             "dataset_create_f32");
+            dev_f = h;
+        }
+        return dev_f->h;
+    }
+    rnng_strings_t device_strings() const {
+        static_assert(std::is_same<Atom, char>::value, "device strings are char");
+        if (!dev_s) {
+            auto h = std::make_shared<rnng_host::StringsHandle>();
+            rnng_host::check(rnng_strings_create(data.data(), offsets.data(), num_points(), &h->h),
+                             "strings_create");
+            dev_s = h;
+        }
+        return dev_s->h;
+    }
+
+protected:
+    AtomVector data;
+    IndexVector offsets, ids;
+    mutable std::shared_ptr<rnng_host::DatasetHandle> dev_f;
+    mutable std::shared_ptr<rnng_host::StringsHandle> dev_s;
+
+    void invalidate() {
+        dev_f.reset();
+        dev_s.reset();
+    }
+    [[noreturn]] static void fail(const std::string& msg) {
+        std::fprintf(stderr, "rnng: %s\n", msg.c_str());
+        std::abort();
+    }
+    const Atom* mem(Index i) const { return data.data() + offsets[(size_t)i]; }
+    Index size(Index i) const { return offsets[(size_t)i + 1] - offsets[(size_t)i]; }
+    Index id(Index i) const {
+        if (!(i >= 0 && i < (Index)ids.size())) fail("point index out of range");
+        return ids[(size_t)i];
+    }
+};
+
+// A Voronoi cell: its members (center first) and ghost points
+// (include/point.h:102-137).
+template <class Atom_>
+class VoronoiCell : public PointContainer<Atom_> {
+public:
+    using Atom = Atom_;
+    VoronoiCell(const std::vector<Point<Atom>>& points, const RealVector& dist_to_centers, Index cell_index)
+        : PointContainer<Atom>(points), cell_index(cell_index), dist_to_centers(dist_to_centers) {}
+    void add_ghost_point(const Point<Atom>& p) { ghost_points.push_back(p); }
+    Index id() const { return cell_index; }
+    Index num_ghosts() const { return ghost_points.num_points(); }
+    Point<Atom> ghost(Index i) const { return ghost_points[i]; }
+    Real dist_to_center(Index i) const { return dist_to_centers[(size_t)i]; }
+    const PointContainer<Atom>& ghosts() const { return ghost_points; }
+
+protected:
+    Index cell_index;
+    RealVector dist_to_centers;
+    PointContainer<Atom> ghost_points;
+};
+
+// The Voronoi diagram of float points around centers taken from the points
+// (the landmark driver gathers its centers by id, drivers/landmark.cpp:111-124):
+// exact assignment with ties to the later center (include/point.hpp:535-559),
+// cells in coalesce order, ghosts and the epsilon graph through the cells on
+// the GPU.
+template <class Atom_>
+class VoronoiDiagram : public PointContainer<Atom_> {
+public:
+    using Atom = Atom_;
+    using Cell = VoronoiCell<Atom>;
+    static_assert(std::is_same<Atom, float>::value, "the GPU Voronoi path is for float points");
+
+    template <class Distance>
+    VoronoiDiagram(const PointContainer<Atom>& points, const PointContainer<Atom>& centers, Distance&)
+        : PointContainer<Atom>(points) {
+        static_assert(rnng_host::is_l2<Distance>(), "the GPU Voronoi path computes L2Distance<float>");
+        IndexVector cid(centers.ids_begin(), centers.ids_end());
+        // center ids are positions in `points` (the whole set, ids 0..n-1)
+        handle_.reset(new Handle());
+        rnng_host::check(rnng_voronoi_create(this->device_points(), cid.data(), (int64_t)cid.size(), &handle_->h),
+                         "voronoi_create");
+        num_centers = (Index)cid.size();
+        cell_indices.resize((size_t)points.num_points());
+        dist_to_centers.resize((size_t)points.num_points());
+        rnng_host::check(rnng_voronoi_assignment(handle_->h, cell_indices.data(), dist_to_centers.data()),
+                         "voronoi_assignment");
+    }
+
+    // Cells in cell order with members in coalesce order (include/point.hpp:561-758).
+    void coalesce_cells(std::vector<Cell>& cells) const {
+        IndexVector ptr((size_t)num_centers + 1), ids((size_t)this->num_points());
+        rnng_host::check(rnng_voronoi_cells(handle_->h, ptr.data(), ids.data()), "voronoi_cells");
+        cells.clear();
+        for (Index c = 0; c < num_centers; ++c) {
+            std::vector<Point<Atom>> pts;
+            RealVector dist;
+            for (Index k = ptr[(size_t)c]; k < ptr[(size_t)c + 1]; ++k) {
+                pts.push_back((*this)[ids[(size_t)k]]);
+                dist.push_back(dist_to_centers[(size_t)ids[(size_t)k]]);
+            }
+            cells.emplace_back(pts, dist, c);
+        }
+    }
+
+    // Ghost points of every cell (include/point.hpp:760-929); also keeps the
+    // epsilon graph through the cells for epsilon_graph().
+    template <class Distance>
+    void add_ghost_points_systolic(std::vector<Cell>& cells, Distance&, Real radius, Real /*cover*/,
+                                   Index /*leaf_size*/) const {
+        static_assert(rnng_host::is_l2<Distance>(), "the GPU Voronoi path computes L2Distance<float>");
+        build(radius);
+        int64_t ng = 0;
+        rnng_host::check(rnng_voronoi_num_ghosts(handle_->h, &ng), "voronoi_num_ghosts");
+        std::vector<int64_t> pairs((size_t)(2 * ng));
+        rnng_host::check(rnng_voronoi_ghosts(handle_->h, pairs.data(), ng), "voronoi_ghosts");
+        for (int64_t g = 0; g < ng; ++g)
+            cells[(size_t)pairs[(size_t)(2 * g)]].add_ghost_point((*this)[pairs[(size_t)(2 * g + 1)]]);
+    }
+
+    // Undirected edges (i < j, weight = reference float distance) of the graph
+    // the landmark driver builds through the cells (drivers/landmark.cpp:185-210).
+    std::vector<std::tuple<Index, Index, Real>> epsilon_graph(Real radius, rnng_stats* stats = nullptr) const {
+        build(radius, stats);
+        int64_t ne = 0;
+        rnng_host::check(rnng_graph_num_edges(graph_->h, &ne), "graph_num_edges");
+        std::vector<int64_t> ij((size_t)(2 * ne));
+        std::vector<float> w((size_t)ne);
+        rnng_host::check(rnng_graph_copy(graph_->h, ij.data(), w.data(), ne, 1), "graph_copy");
+        std::vector<std::tuple<Index, Index, Real>> out((size_t)ne);
+        for (int64_t e = 0; e < ne; ++e) out[(size_t)e] = std::make_tuple(ij[2 * e], ij[2 * e + 1], w[(size_t)e]);
+        return out;
+    }
+
+    rnng_voronoi_t handle() const { return handle_->h; }
+
+    // The epsilon graph through the cells (device edge list, owned by the
+    // diagram), built once per radius.
+    rnng_graph_t graph_handle(Real radius) const {
+        build(radius);
+        return graph_->h;
+    }
+
+private:
+    struct Handle {
+        rnng_voronoi_t h = nullptr;
+        ~Handle() {
+            if (h) rnng_voronoi_destroy(h);
+        }
+    };
+    struct GraphHandle {
+        rnng_graph_t h = nullptr;
+        ~GraphHandle() {
+            if (h) rnng_graph_destroy(h);
+        }
+    };
+    void build(Real radius, rnng_stats* stats = nullptr) const {
+        if (graph_ && built_radius == radius && !stats) return;
+        graph_.reset(new GraphHandle());
+        rnng_host::check(rnng_voronoi_graph_build(handle_->h, radius, RNNG_MODE_MFMA, &graph_->h, stats),
+                         "voronoi_graph_build");
+        built_radius = radius;
+    }
+
+    // declared after the base's device copy it borrows, destroyed before it
+    std::shared_ptr<Handle> handle_;
+    mutable std::shared_ptr<GraphHandle> graph_;
+    mutable Real built_radius = -1;
+    Index num_centers = 0;
+    IndexVector cell_indices;
+    RealVector dist_to_centers;
+};
+
+#endif

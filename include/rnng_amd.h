@@ -111,6 +111,15 @@ int rnng_graph_num_edges(rnng_graph_t g, int64_t *n_edges);
  * (either may be NULL).  sorted != 0 orders them by (i, j). */
 int rnng_graph_copy(rnng_graph_t g, int64_t *ij, float *w, int64_t cap, int32_t sorted);
 
+/* The edge list the reference's drivers hold after Graph::redistribute_edges
+ * on one rank (include/graph.hpp:59-116): for a self-join graph over
+ * n_vertices points, (i, j, w) and (j, i, w) for every edge plus (v, v, 0) for
+ * every vertex (the query of a point finds itself), sorted by (u, v).
+ * *needed = 2 E + n_vertices; with cap < *needed and NULL outputs only the
+ * size is reported (RNNG_OK), with non-NULL outputs RNNG_ERR_CAPACITY. */
+int rnng_graph_copy_directed(rnng_graph_t g, int64_t n_vertices, int64_t *uv, float *w, int64_t cap,
+                             int64_t *needed);
+
 int rnng_graph_destroy(rnng_graph_t g);
 
 /* One-shot host-to-host form with the count-then-fill contract: at most cap

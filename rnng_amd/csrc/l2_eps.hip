@@ -925,6 +925,23 @@ __global__ void k_pack_keys(const int2* __restrict__ ij, uint64_t* __restrict__ 
     if (e < n) keys[e] = ((uint64_t)(uint32_t)ij[e].x << 32) | (uint32_t)ij[e].y;
 }
 
+// Directed form of an undirected edge list: (i, j) and (j, i) per edge and a
+// zero-weight self loop per vertex, as sort keys (u << 32 | v).
+__global__ void k_directed_keys(const int2* __restrict__ ij, const float* __restrict__ w, int64_t E, int64_t n,
+                                uint64_t* __restrict__ keys, float* __restrict__ ww) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2 * E) {
+        const int2 e = ij[t >> 1];
+        const uint32_t a = (uint32_t)((t & 1) ? e.y : e.x), b = (uint32_t)((t & 1) ? e.x : e.y);
+        keys[t] = ((uint64_t)a << 32) | b;
+        ww[t] = w[t >> 1];
+    } else if (t < 2 * E + n) {
+        const uint32_t v = (uint32_t)(t - 2 * E);
+        keys[t] = ((uint64_t)v << 32) | v;
+        ww[t] = 0.0f;
+    }
+}
+
 __global__ void k_unpack_keys(const uint64_t* __restrict__ keys, int2* __restrict__ ij, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) ij[e] = make_int2((int)(keys[e] >> 32), (int)(keys[e] & 0xffffffffu));
@@ -1621,6 +1638,44 @@ int rnng_graph_copy(rnng_graph_t g, int64_t* ij, float* w, int64_t cap, int32_t 
         for (int64_t e = 0; e < ne; ++e) {
             ij[2 * e] = hij[(size_t)e].x;
             ij[2 * e + 1] = hij[(size_t)e].y;
+        }
+    return RNNG_OK;
+}
+
+int rnng_graph_copy_directed(rnng_graph_t g, int64_t n_vertices, int64_t* uv, float* w, int64_t cap,
+                             int64_t* needed) {
+    if (!g || cap < 0 || n_vertices < 0 || n_vertices >= INT32_MAX) {
+        set_error("graph_copy_directed: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    const int64_t E = g->n_edges;
+    const int64_t total = 2 * E + n_vertices;
+    if (needed) *needed = total;
+    if (cap < total) {
+        if (uv || w) {
+            set_error("graph_copy_directed: output holds fewer than 2 E + n edges");
+            return RNNG_ERR_CAPACITY;
+        }
+        return RNNG_OK;
+    }
+    if (total == 0) return RNNG_OK;
+    LocalBuf keys, ww;
+    RNNG_TRY(keys.reserve((size_t)total * sizeof(uint64_t)));
+    RNNG_TRY(ww.reserve((size_t)total * sizeof(float)));
+    k_directed_keys<<<blocks_for(total, 256), 256, 0, stream()>>>(g->ij.as<int2>(), g->w.as<float>(), E, n_vertices,
+                                                                  keys.as<uint64_t>(), ww.as<float>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(sort_pairs_u64_f32(keys.as<uint64_t>(), ww.as<float>(), total));
+    std::vector<uint64_t> hk((size_t)total);
+    RNNG_HIP_TRY(hipMemcpyAsync(hk.data(), keys.ptr, (size_t)total * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                stream()));
+    if (w) RNNG_HIP_TRY(hipMemcpyAsync(w, ww.ptr, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    if (uv)
+        for (int64_t e = 0; e < total; ++e) {
+            uv[2 * e] = (int64_t)(hk[(size_t)e] >> 32);
+            uv[2 * e + 1] = (int64_t)(hk[(size_t)e] & 0xffffffffull);
         }
     return RNNG_OK;
 }
