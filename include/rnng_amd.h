@@ -170,15 +170,18 @@ int rnng_edit_eps_pairs(const char *atoms, const int64_t *offsets, int64_t n, fl
 typedef struct rnng_filtration_s *rnng_filtration_t;   /* device-resident simplices */
 
 /* Rips filtration of the L2 epsilon graph of ds up to dimension maxdim
- * (0, 1 or 2): every vertex (value 0), edge (value = float L2 distance) and
- * triangle (value = max of its edges), with the reference's simplex ids
+ * (0..3): every vertex (value 0), edge (value = float L2 distance), triangle
+ * and tetrahedron (value = max of its edges), with the reference's simplex ids
  * (include/simplex.hpp:5-19), sorted by (value, id) and unique by id -- the
  * merged output of the rips driver (include/point.hpp:1134-1267,
- * include/simplex.hpp:130-196).  Triangles need n <= 1,800,000 (60-bit ids). */
+ * include/simplex.hpp:130-196).  Triangles need n <= 1,800,000 (60-bit ids);
+ * tetrahedra n <= 55,111 (beyond it the reference's int64 binomial
+ * v(v-1)(v-2)(v-3) overflows, include/utils.hpp:126-127); larger inputs give
+ * RNNG_ERR_UNSUPPORTED. */
 int rnng_rips_build(rnng_dataset_t ds, float eps, int32_t maxdim, rnng_filtration_t *out,
                     rnng_stats *stats);
 
-/* Total number of simplices, and per dimension (per_dim[0..2], may be NULL). */
+/* Total number of simplices, and per dimension (per_dim[0..3], may be NULL). */
 int rnng_filtration_size(rnng_filtration_t f, int64_t *n, int64_t *per_dim);
 
 /* Copy the first cap simplices in filtration order (either array may be NULL). */

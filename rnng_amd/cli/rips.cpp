@@ -1,7 +1,8 @@
 // Vietoris-Rips filtration on one GPU: the rips driver's flags and merged
 // output file "<outfile>.txt" (drivers/rips.cpp, include/simplex.hpp:130-196).
 // The union of the per-cell complexes is the Rips complex of the whole point
-// set, built here directly; maxdim up to 2.
+// set, built here directly; maxdim up to 3 (tetrahedra for up to 55,111
+// points, where the reference's simplex ids are exact).
 #include "cli_common.h"
 
 static Real radius = -1;
@@ -19,7 +20,7 @@ static bool use_ids = true;
 static void usage(const char* prog, int err) {
     std::fprintf(stderr, "Usage: %s [options] -i <points> -r <radius>\n", prog);
     std::fprintf(stderr, "Options: -m INT   number of centers [%lld] (accepted; one GPU holds every cell)\n", (long long)num_centers);
-    std::fprintf(stderr, "         -d INT   maximum simplex dimension [%lld] (0, 1 or 2)\n", (long long)maxdim);
+    std::fprintf(stderr, "         -d INT   maximum simplex dimension [%lld] (0..3)\n", (long long)maxdim);
     std::fprintf(stderr, "         -c FLOAT cover tree base [%.2f] (accepted; no tree is built)\n", cover);
     std::fprintf(stderr, "         -l INT   leaf size [%lld] (accepted; no tree is built)\n", (long long)leaf_size);
     std::fprintf(stderr, "         -v INT   verbosity level [%d]\n", verbosity);
@@ -51,8 +52,8 @@ int main(int argc, char* argv[]) {
         else usage(argv[0], 1);
     }
     if (!infile || radius < 0) usage(argv[0], 1);
-    if (std::strcmp(metric, "l2") != 0 || maxdim < 0 || maxdim > 2) {
-        std::fprintf(stderr, "the GPU Rips path computes the l2 metric up to dimension 2 (use -d 0, 1 or 2)\n");
+    if (std::strcmp(metric, "l2") != 0 || maxdim < 0 || maxdim > 3) {
+        std::fprintf(stderr, "the GPU Rips path computes the l2 metric up to dimension 3 (use -d 0..3)\n");
         return 1;
     }
     double t0 = wall(), t;
@@ -62,15 +63,15 @@ int main(int argc, char* argv[]) {
     t = wall();
     rnng_filtration_t f = nullptr;
     rnng_host::check(rnng_rips_build(points.device_points(), radius, (int32_t)maxdim, &f, nullptr), "rips_build");
-    int64_t ns = 0, per[3] = {0, 0, 0};
+    int64_t ns = 0, per[4] = {0, 0, 0, 0};
     rnng_host::check(rnng_filtration_size(f, &ns, per), "filtration_size");
     std::vector<float> val((size_t)ns);
     std::vector<int64_t> ids((size_t)ns);
     rnng_host::check(rnng_filtration_copy(f, val.data(), ids.data(), ns), "filtration_copy");
     rnng_filtration_destroy(f);
     if (verbosity >= 1)
-        std::fprintf(stderr, "[time=%.3f] built filtration [vertices=%lld,edges=%lld,triangles=%lld]\n", wall() - t,
-                     (long long)per[0], (long long)per[1], (long long)per[2]);
+        std::fprintf(stderr, "[time=%.3f] built filtration [vertices=%lld,edges=%lld,triangles=%lld,tetrahedra=%lld]\n",
+                     wall() - t, (long long)per[0], (long long)per[1], (long long)per[2], (long long)per[3]);
     std::fprintf(stderr, "[time=%.3f] complete\n", wall() - t0);
     if (outfile) {
         t = wall();

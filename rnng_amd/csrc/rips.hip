@@ -16,8 +16,9 @@
 //
 // Pipeline: epsilon graph -> device sort by (u, v) -> CSR of upper
 // neighbours -> per-edge merge intersection of the two sorted neighbour lists
-// (count, exclusive scan, fill: deterministic, no atomics) -> filtration
-// arrays -> device sort by (value, id).
+// (triangles) and, for dimension 3, a three-way merge with the third vertex's
+// list (tetrahedra); count, exclusive scan, fill: deterministic, no atomics ->
+// filtration arrays -> device sort by (value, id).
 #include "internal.hpp"
 
 #include <algorithm>
@@ -35,6 +36,13 @@ namespace {
 
 __device__ __forceinline__ int64_t binom2(int64_t v) { return v * (v - 1) / 2; }
 __device__ __forceinline__ int64_t binom3(int64_t v) { return (v * (v - 1) / 2) * (v - 2) / 3; }
+// C(v, 4) = C(v, 3) (v - 3) / 4, exact while C(v, 3) (v - 3) < 2^63
+__device__ __forceinline__ int64_t binom4(int64_t v) { return binom3(v) * (v - 3) / 4; }
+
+// Largest vertex count whose 4-clique ids the reference computes without
+// overflow: its Binom multiplies v (v-1) (v-2) (v-3) in int64
+// (include/utils.hpp:126-127), which stays below 2^63 for v <= 55110.
+constexpr int64_t kMaxTetraVertices = 55111;
 
 // rowptr[u] = first edge of row u; keys sorted by (u << 32 | v).
 __global__ void k_csr_rows(const uint64_t* __restrict__ keys, int64_t E, int n,
@@ -88,6 +96,59 @@ __global__ __launch_bounds__(256) void k_triangles(
     if (!FILL) count[e] = c;
 }
 
+// Tetrahedra (u, v, w, x), u < v < w < x, through edge e = (u, v): for each
+// common upper neighbour w of u and v, the common upper neighbours x > w of
+// u, v and w (a three-way merge of the sorted rows).  Value = the largest of
+// the six edge weights (the max over facets, include/point.hpp:1194-1208).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_tetrahedra(
+    const int* __restrict__ row, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+    const float* __restrict__ wt, int64_t E, int64_t* __restrict__ count,
+    const int64_t* __restrict__ offset, float* __restrict__ tval, int64_t* __restrict__ tid) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int u = row[e], v = col[e];
+    const float wuv = wt[e];
+    const int64_t e1 = rowptr[u + 1], e2 = rowptr[v + 1];
+    int64_t i1 = e + 1, i2 = rowptr[v];
+    int64_t c = 0, out = FILL ? offset[e] : 0;
+    const int64_t base = (int64_t)u + binom2(v) + ((int64_t)3 << 60);
+    while (i1 < e1 && i2 < e2) {
+        const int a = col[i1], b = col[i2];
+        if (a < b) {
+            ++i1;
+            continue;
+        }
+        if (b < a) {
+            ++i2;
+            continue;
+        }
+        const int w = a;
+        const float m3 = fmaxf(wuv, fmaxf(wt[i1], wt[i2]));
+        int64_t j1 = i1 + 1, j2 = i2 + 1, j3 = rowptr[w];
+        const int64_t e3 = rowptr[w + 1];
+        while (j1 < e1 && j2 < e2 && j3 < e3) {
+            const int x1 = col[j1], x2 = col[j2], x3 = col[j3];
+            const int mn = min(x1, min(x2, x3));
+            if (x1 == x2 && x2 == x3) {
+                if (FILL) {
+                    tval[out] = fmaxf(m3, fmaxf(wt[j1], fmaxf(wt[j2], wt[j3])));
+                    tid[out] = base + binom3(w) + binom4(x1);
+                    ++out;
+                } else {
+                    ++c;
+                }
+            }
+            j1 += x1 == mn;
+            j2 += x2 == mn;
+            j3 += x3 == mn;
+        }
+        ++i1;
+        ++i2;
+    }
+    if (!FILL) count[e] = c;
+}
+
 __global__ void k_fill_vertices(float* __restrict__ val, int64_t* __restrict__ id, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -114,8 +175,12 @@ __global__ void k_pack_ij(const int2* __restrict__ ij, uint64_t* __restrict__ ke
 int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_s* f,
                rnng_stats* st) {
     const int n = (int)ds->n;
-    if (maxdim < 0 || maxdim > 2) {
-        set_error("maxdim must be 0, 1 or 2");
+    if (maxdim < 0 || maxdim > 3) {
+        set_error("maxdim must be 0, 1, 2 or 3");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    if (maxdim == 3 && n > kMaxTetraVertices) {
+        set_error("4-clique ids of more than 55,111 vertices overflow the reference's int64 binomials");
         return RNNG_ERR_UNSUPPORTED;
     }
     // C(v, 3) must fit the 60-bit id field (include/simplex.hpp:18)
@@ -170,7 +235,25 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         RNNG_HIP_TRY(hipMemcpy(&last_cnt, cnt.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
         T = last_off + last_cnt;
     }
-    const int64_t total = (int64_t)n + (maxdim >= 1 ? E : 0) + T;
+    // clique counts by the per-edge count / scan / fill of kernel K
+    auto count_cliques = [&](auto kernel, LocalBuf& cbuf, LocalBuf& obuf, int64_t& total_out) -> int {
+        RNNG_TRY(cbuf.reserve((size_t)E * sizeof(int64_t)));
+        RNNG_TRY(obuf.reserve(((size_t)E + 1) * sizeof(int64_t)));
+        kernel<<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(), col.as<int>(),
+                                                         g.w.as<float>(), E, cbuf.as<int64_t>(), nullptr, nullptr,
+                                                         nullptr);
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(exclusive_scan_i64(cbuf.as<int64_t>(), obuf.as<int64_t>(), E));
+        int64_t lo = 0, lc = 0;
+        RNNG_HIP_TRY(hipMemcpy(&lo, obuf.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
+        RNNG_HIP_TRY(hipMemcpy(&lc, cbuf.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
+        total_out = lo + lc;
+        return RNNG_OK;
+    };
+    int64_t Q = 0;
+    LocalBuf cnt4, off4;
+    if (maxdim >= 3 && E > 0) RNNG_TRY(count_cliques(k_tetrahedra<false>, cnt4, off4, Q));
+    const int64_t total = (int64_t)n + (maxdim >= 1 ? E : 0) + T + Q;
     RNNG_TRY(f->value.reserve(std::max<size_t>(1, (size_t)total) * sizeof(float)));
     RNNG_TRY(f->id.reserve(std::max<size_t>(1, (size_t)total) * sizeof(int64_t)));
     float* val = f->value.as<float>();
@@ -183,6 +266,10 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         k_triangles<true><<<blocks_for(E, 256), 256, 0, stream()>>>(
             row.as<int>(), rowptr.as<int64_t>(), col.as<int>(), g.w.as<float>(), E, nullptr,
             off.as<int64_t>(), val + n + E, id + n + E);
+    if (Q > 0)
+        k_tetrahedra<true><<<blocks_for(E, 256), 256, 0, stream()>>>(
+            row.as<int>(), rowptr.as<int64_t>(), col.as<int>(), g.w.as<float>(), E, nullptr,
+            off4.as<int64_t>(), val + n + E + T, id + n + E + T);
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
     RNNG_TRY(sort_by_value_then_id(val, id, total));
@@ -192,6 +279,7 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     f->counts[0] = n;
     f->counts[1] = maxdim >= 1 ? E : 0;
     f->counts[2] = T;
+    f->counts[3] = Q;
     if (st) {
         st->recheck_ms += ev.ms(1, 2);    // CSR + clique stage
         st->total_ms += ev.ms(0, 3);
@@ -231,7 +319,7 @@ int rnng_filtration_size(rnng_filtration_t f, int64_t* n, int64_t* per_dim) {
     }
     *n = f->n;
     if (per_dim)
-        for (int p = 0; p < 3; ++p) per_dim[p] = f->counts[p];
+        for (int p = 0; p < 4; ++p) per_dim[p] = f->counts[p];
     return RNNG_OK;
 }
 

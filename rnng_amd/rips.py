@@ -25,7 +25,7 @@ def rips_filtration(points, eps: float, maxdim: int = 2, stats: Stats | None = N
         check(L.rnng_rips_build(ds.handle, np.float32(eps), int(maxdim), C.byref(h), C.byref(st)))
         try:
             n = C.c_int64()
-            per = np.zeros(3, np.int64)
+            per = np.zeros(4, np.int64)
             check(L.rnng_filtration_size(h, C.byref(n), per.ctypes.data_as(C.c_void_p)))
             val = np.empty(n.value, np.float32)
             ids = np.empty(n.value, np.int64)

@@ -116,8 +116,17 @@ def test_rips_file(orc, tmp_path, use_ids):
     assert (tmp_path / "filt.txt").read_text().splitlines() == want
 
 
-def test_rips_rejects_dimension_three(tmp_path):
+def test_rips_default_dimension_three(orc, tmp_path):
+    """Without -d the driver builds up to tetrahedra (the reference's default)."""
+    rng = np.random.default_rng(8)
+    pts = rng.uniform(0, 1, size=(600, 3)).astype(np.float32)
+    eps = np.float32(0.16)
     f = tmp_path / "p.fvecs"
-    _write_fvecs(f, np.zeros((3, 3), np.float32))
-    r = subprocess.run([os.path.join(BIN, "rnng_rips"), "-i", str(f), "-r", "0.5"], capture_output=True, text=True)
-    assert r.returncode == 1 and "dimension 2" in r.stderr
+    _write_fvecs(f, pts)
+    prefix = tmp_path / "filt"
+    _run([os.path.join(BIN, "rnng_rips"), "-i", str(f), "-r", repr(float(eps)), "-o", str(prefix)])
+    val, ids = orc.rips_filtration(pts, eps, 3)
+    assert (tmp_path / "filt.txt").read_text().splitlines() == ["%f\t%d" % (float(v), int(i)) for v, i in zip(val, ids)]
+    r = subprocess.run([os.path.join(BIN, "rnng_rips"), "-i", str(f), "-r", "0.5", "-d", "4"], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "dimension 3" in r.stderr

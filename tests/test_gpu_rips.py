@@ -1,5 +1,6 @@
-"""GPU parity of the Rips filtration (vertices, edges, triangles) against the
-CPU oracle: identical (value, id) sequences, bit for bit."""
+"""GPU parity of the Rips filtration (vertices, edges, triangles,
+tetrahedra) against the CPU oracle: identical (value, id) sequences, bit for
+bit."""
 import math
 
 import numpy as np
@@ -15,7 +16,7 @@ def _rips():
     return rips
 
 
-@pytest.mark.parametrize("maxdim", [0, 1, 2])
+@pytest.mark.parametrize("maxdim", [0, 1, 2, 3])
 def test_rips_small_bit_exact(orc, maxdim):
     from rnng_amd import synth
     pts, eps = synth.points_rips(n=3000)
@@ -37,11 +38,33 @@ def test_rips_random_bit_exact(orc, n, d, eps):
     assert np.array_equal(i, i_ref)
 
 
+@pytest.mark.parametrize("n,d,eps", [(400, 3, 0.45), (250, 6, 1.6), (2000, 3, 0.35)])
+def test_rips_tetrahedra_bit_exact(orc, n, d, eps):
+    rng = np.random.default_rng(n * d)
+    pts = rng.standard_normal((n, d)).astype(np.float32)
+    flush_print(f"rips dim 3 n={n} d={d}")
+    v_ref, i_ref = orc.rips_filtration(pts, eps, maxdim=3)
+    v, i = _rips().rips_filtration(pts, eps, 3)
+    assert v.tobytes() == v_ref.tobytes()
+    assert np.array_equal(i, i_ref)
+    assert (_rips().simplex_dims(i) == 3).sum() > 0
+
+
 def test_rips_clique_counts():
     pts = (np.random.default_rng(1).standard_normal((30, 3)) * 1e-3).astype(np.float32)
-    v, i = _rips().rips_filtration(pts, 1.0, 2)
+    v, i = _rips().rips_filtration(pts, 1.0, 3)
     dims = _rips().simplex_dims(i)
-    assert [(dims == p).sum() for p in range(3)] == [30, math.comb(30, 2), math.comb(30, 3)]
+    assert [(dims == p).sum() for p in range(4)] == [30, math.comb(30, 2), math.comb(30, 3), math.comb(30, 4)]
+
+
+def test_rips_dimension_three_size_guard():
+    """Above 55,111 vertices the reference's int64 binomial for 4-subsets
+    overflows: the library refuses instead of writing wrong ids."""
+    from rnng_amd._lib import RNNG_ERR_UNSUPPORTED, RnngError
+    pts = np.zeros((55_112, 3), np.float32)
+    with pytest.raises(RnngError) as ei:
+        _rips().rips_filtration(pts, 0.0, 3)
+    assert ei.value.code == RNNG_ERR_UNSUPPORTED
 
 
 def test_rips_full_size_properties():
