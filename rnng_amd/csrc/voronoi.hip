@@ -224,11 +224,36 @@ __global__ void k_gather_rows(const float* __restrict__ X, int d, const int* __r
 // the last step (cell, mind) is the Voronoi assignment of the centers in
 // selection order.  Each workgroup also writes the argmax of the updated
 // distances (ties to the smaller point index) for the next selection.
+//
+// A point i in cell a can move to center k only if L2(c_k, i) <= mind_i,
+// which needs L2(c_k, c_a) <= 2 mind_i by the triangle inequality.  Points
+// with L2(c_k, c_a) >= 2 mind_i (1 + 1e-4) are skipped without reading their
+// row: the margin is far above the FP32 error of the three distances
+// (~(d + 2) 2^-24 relative each), so a skipped point's exact distance is > mind_i.
 constexpr int FPS_K = 16;   // dims per LDS round
 
+// Dc[a] = L2(c_k, c_a) for a < k (any summation order: only a bound), and
+// the new center's row copied to C[k].
+__global__ __launch_bounds__(256) void k_fps_center(const float* __restrict__ X, int d, const int* __restrict__ ctr,
+                                                    int k, float* __restrict__ C, float* __restrict__ Dc) {
+    const float* c = X + (size_t)ctr[k] * d;
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < d; t += 256) C[(size_t)k * d + t] = c[t];
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a < k) {
+        const float* ca = C + (size_t)a * d;
+        float acc = 0.0f;
+        for (int j = 0; j < d; ++j) {
+            const float t = ca[j] - c[j];
+            acc += t * t;
+        }
+        Dc[a] = sqrtf(acc);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X, int n, int d,
-                                                    int* __restrict__ ctr, int k, float* __restrict__ mind,
-                                                    int* __restrict__ cell,
+                                                    const int* __restrict__ ctr, int k, float* __restrict__ mind,
+                                                    int* __restrict__ cell, const float* __restrict__ Dc,
                                                     unsigned long long* __restrict__ part) {
 #pragma clang fp contract(off)
     extern __shared__ float fps_lds[];   // [d] center, then [4 waves][FPS_K][65]
@@ -240,35 +265,70 @@ __global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X,
     __syncthreads();
     const int p0 = blockIdx.x * 256 + wave * 64;
     const int i = p0 + lane;
+    float md = FLT_MAX;
+    int cl = 0;
+    bool need = false;
+    if (i < n) {
+        if (k == 0) {
+            need = true;
+        } else {
+            md = mind[i];
+            cl = cell[i];
+            need = !((double)Dc[cl] >= 2.0 * (double)md * (1.0 + 1e-4) + 1e-30);
+        }
+    }
+    const unsigned long long nm = __ballot(need);
     float acc = 0.0f;
-    for (int k0 = 0; k0 < d; k0 += FPS_K) {
-        // 64 rows x FPS_K dims, 16 consecutive lanes per row segment
+    if (__popcll(nm) > 16) {
+        // most rows of the wave are needed: 64 rows x FPS_K dims through LDS,
+        // 16 consecutive lanes per row segment
+        for (int k0 = 0; k0 < d; k0 += FPS_K) {
 #pragma unroll
-        for (int j = 0; j < FPS_K; ++j) {
-            const int e = lane + 64 * j;
-            const int r = e / FPS_K, kk = e % FPS_K;
-            const int row = p0 + r, kd = k0 + kk;
-            tile[kk * 65 + r] = (row < n && kd < d) ? X[(size_t)row * d + kd] : 0.0f;
+            for (int j = 0; j < FPS_K; ++j) {
+                const int e = lane + 64 * j;
+                const int r = e / FPS_K, kk = e % FPS_K;
+                const int row = p0 + r, kd = k0 + kk;
+                tile[kk * 65 + r] = (row < n && kd < d) ? X[(size_t)row * d + kd] : 0.0f;
+            }
+            // rows are written and read by lanes of this wave only
+            __builtin_amdgcn_wave_barrier();
+            const int kend = min(FPS_K, d - k0);
+            for (int kk = 0; kk < kend; ++kk) {
+                const float t = tile[kk * 65 + lane] - cs[k0 + kk];
+                acc = acc + t * t;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        // rows are written and read by lanes of this wave only
-        __builtin_amdgcn_wave_barrier();
-        const int kend = min(FPS_K, d - k0);
-        for (int kk = 0; kk < kend; ++kk) {
-            const float t = tile[kk * 65 + lane] - cs[k0 + kk];
-            acc = acc + t * t;
+    } else if (need) {
+        // few rows needed: each lane reads its own row
+        const float* x = X + (size_t)i * d;
+        if ((d & 3) == 0) {
+            for (int j = 0; j < d; j += 4) {
+                const float4 q = *(const float4*)(x + j);
+                float t;
+                t = q.x - cs[j];     acc = acc + t * t;
+                t = q.y - cs[j + 1]; acc = acc + t * t;
+                t = q.z - cs[j + 2]; acc = acc + t * t;
+                t = q.w - cs[j + 3]; acc = acc + t * t;
+            }
+        } else {
+            for (int j = 0; j < d; ++j) {
+                const float t = x[j] - cs[j];
+                acc = acc + t * t;
+            }
         }
-        __builtin_amdgcn_wave_barrier();
     }
     unsigned long long key = 0ull;
     if (i < n) {
-        const float dd = sqrtf(acc);
-        if (k == 0) cell[i] = 0;
-        float md = k == 0 ? FLT_MAX : mind[i];
-        if (dd <= md) {
-            md = dd;
-            cell[i] = k;
+        if (need) {
+            const float dd = sqrtf(acc);
+            if (dd <= md) {
+                md = dd;
+                cl = k;
+            }
+            mind[i] = md;
+            cell[i] = cl;
         }
-        mind[i] = md;
         key = ((unsigned long long)f2ord_v(md) << 32) | (unsigned)(0xffffffffu - (unsigned)i);
     }
     // block argmax: wave shuffles, then 4 waves through LDS
@@ -666,9 +726,14 @@ int fps(rnng_voronoi_s* v, int first) {
     RNNG_TRY(part.reserve((size_t)nb * sizeof(unsigned long long)));
     RNNG_HIP_TRY(hipMemcpyAsync(v->ctr.ptr, &first, sizeof(int), hipMemcpyHostToDevice, stream()));
     const size_t lds = (size_t)(((d + 3) & ~3) + 4 * FPS_K * 65) * sizeof(float);
+    LocalBuf dc;
+    RNNG_TRY(v->C.reserve((size_t)m * d * sizeof(float)));
+    RNNG_TRY(dc.reserve((size_t)m * sizeof(float)));
     for (int k = 0; k < m; ++k) {
+        k_fps_center<<<blocks_for(std::max(k, 1), 256), 256, 0, stream()>>>(v->ds->X, d, v->ctr.as<int>(), k,
+                                                                            v->C.as<float>(), dc.as<float>());
         k_fps_update<<<nb, 256, lds, stream()>>>(v->ds->X, n, d, v->ctr.as<int>(), k, v->delta.as<float>(),
-                                                 v->cell.as<int>(), part.as<unsigned long long>());
+                                                 v->cell.as<int>(), dc.as<float>(), part.as<unsigned long long>());
         if (k + 1 < m)
             k_fps_select<<<1, 1024, 0, stream()>>>(part.as<unsigned long long>(), nb, v->ctr.as<int>(), k);
     }
