@@ -237,7 +237,7 @@ int rnng_voronoi_cells(rnng_voronoi_t v, int64_t *cell_ptr, int64_t *cell_ids);
  * when L2(center_b, q) <= fl(dist_q + 2 eps) and some member of b is within
  * eps of q), then every cell's member x member and member x ghost pairs
  * within eps, deduplicated to undirected edges i < j.  The edge set equals
- * the all-pairs graph's.  mode must be RNNG_MODE_MFMA (d <= 135).
+ * the all-pairs graph's.  mode must be RNNG_MODE_MFMA (d <= 144).
  * Replaces: add_ghost_points_systolic (include/point.hpp:760-929), the
  * per-cell query loop (drivers/landmark.cpp:203-210) and the Graph ctor's
  * sort/unique (include/graph.hpp:59-116). */

@@ -194,20 +194,19 @@ __global__ __launch_bounds__(256) void k_l2_plain(
 // MFMA path: data preparation
 // ------------------------------------------------------------------------
 // Fragment layout (shared by the x-form and the y-form):
-//   tile t = 32 points, k-step s = 16 augmented dims, lane l in 0..63:
+//   tile t = 32 points, k-step s = 16 dims, lane l in 0..63:
 //   frag[(t*KS + s)*64 + l] = 8 bf16 of point 32t + (l & 31),
 //                             dims 16s + 8(l >> 5) .. +7
 // which is exactly the per-lane operand of v_mfma_f32_32x32x16_bf16 (A: row
 // = lane&31; B: column = lane&31; the 16 dims of a step split by lane>>5), so
 // a wave loads one 1 KB fragment with one fully coalesced 16-byte load.
 //
-// Augmented vectors (x = row side, y = column side), dims d..d+5:
-//   x~ = [bf16(x - mu) ..., n1, n2, n3, 1, 1, 1, 0 ...]
-//   y~ = [-2 bf16(y - mu) ..., 1, 1, 1, n1', n2', n3', 0 ...]
-// n1+n2+n3 = |bf16(x - mu)|^2 split into three bf16 pieces, so
-//   x~ . y~ = |x'|^2 + |y'|^2 - 2 x'.y' = |x' - y'|^2
-// and with the accumulator initialised to -C0, acc < 0 marks a candidate.
-// Padding points carry 2^100 in a norm slot so their acc is always positive.
+// x~ = bf16(x - mu) (rows), y~ = -2 bf16(y - mu) (columns), dims >= d zero,
+// KS = ceil(d / 16).  The squared norms |x~|^2 and |y~/2|^2 are FP32 arrays
+// xn (rows) and ycn (columns, with -C0 added): the accumulator of pair (i, j)
+// starts at xn[i] + ycn[j] and the MFMAs add x~.y~, so it ends at
+// |x' - y'|^2 - C0 and its sign bit marks a candidate.  Padding points carry
+// 2^100 as their norm, so any pair with one is positive.
 
 constexpr int NSTAT = 8;
 // stat slots (unsigned, ordered-float or flags) in ds_stat[]
@@ -270,25 +269,18 @@ __device__ __forceinline__ float wave_max(float v) {
 // One workgroup per 64 points: the points' rows are staged in LDS with
 // contiguous loads, each point's norm and conversion error is reduced by four
 // lanes, then the x~ and/or y~ fragments (either output may be null) are
-// written 16 bytes per lane, contiguous within each 1 KB fragment.
-//   slot       d      d+1   d+2   d+3   d+4   d+5   d+6..d+8
-//   x~ real    n1     n2    n3    1     1     1     1   1   1
-//   y~ real    1      1     1     n1    n2    n3    -c1 -c2 -c3
-//   x~ pad     2^100  0     0     1     0     0     0   0   0
-//   y~ pad     1      0     0     2^100 0     0     0   0   0
-// (c1 + c2 + c3 >= C0, written by k_patch_c0 once C0 is known), so a real
-// pair accumulates |x~|^2 + |y~|^2 - 2 x~.y~ - C0 from 0, and any pair with a
-// padding point has acc >= 2^100 - C0 > 0.
+// written 16 bytes per lane, contiguous within each 1 KB fragment, and the
+// norms (FP32, 2^100 for padding) to xn and/or yn.
 // idmap (may be null): padded position -> point index (-1 = padding); without
 // it position p holds point p for p < n.
 constexpr int PP = 64;   // points per prep workgroup
+constexpr float kPadNorm = 1.2676506e30f;   // 2^100
 
 __global__ __launch_bounds__(256) void k_prep_frags(
     const float* __restrict__ X, int n, int n_pad, int d, int KS, const float* __restrict__ mu,
     const int* __restrict__ idmap, uint16_t* __restrict__ xa, uint16_t* __restrict__ yb,
-    unsigned* __restrict__ stat) {
+    float* __restrict__ xn, float* __restrict__ yn, unsigned* __restrict__ stat) {
     extern __shared__ float xs[];          // [PP][d + 1]: x~ = bf16(x - mu) as float
-    __shared__ float pn[PP][3];            // norm pieces
     __shared__ int preal[PP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int p0 = blockIdx.x * PP;
@@ -313,7 +305,7 @@ __global__ __launch_bounds__(256) void k_prep_frags(
     }
     __syncthreads();
     // norm and conversion error: 4 lanes per point
-    float e_max = 0.0f, n_max = 0.0f, r_max = 0.0f;
+    float e_max = 0.0f, n_max = 0.0f;
     {
         const int r = tid >> 2, part = tid & 3;
         const int src = preal[r];
@@ -330,64 +322,30 @@ __global__ __launch_bounds__(256) void k_prep_frags(
         nrm += __shfl_xor(nrm, 2, 64);
         err += __shfl_xor(err, 1, 64);
         err += __shfl_xor(err, 2, 64);
-        if (part == 0) {
-            float pc[3] = {0.f, 0.f, 0.f};
-            double rr = nrm;
+        if (part == 0 && p0 + r < n_pad) {
+            const float nv = src >= 0 ? (float)nrm : kPadNorm;
+            if (xn) xn[p0 + r] = nv;
+            if (yn) yn[p0 + r] = nv;
             if (src >= 0) {
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    pc[q] = bf16_to_f32(f32_to_bf16_rn((float)rr));
-                    rr -= (double)pc[q];
-                }
                 e_max = (float)sqrt(err) * (1.0f + 1e-6f);
                 n_max = (float)nrm * (1.0f + 1e-6f);
-                r_max = (float)fabs(rr) * 2.0f;
             }
-            pn[r][0] = pc[0];
-            pn[r][1] = pc[1];
-            pn[r][2] = pc[2];
         }
     }
-    __syncthreads();
-    const float big = 1.2676506e30f;   // 2^100, exact in bf16
     const int nchunk = (PP / 32) * KS * 64;
     for (int c = tid; c < nchunk; c += 256) {
         const int tt = c / (KS * 64), rem = c - tt * (KS * 64);
         const int st = rem >> 6, l = rem & 63;
         const int r = tt * 32 + (l & 31);
         if (p0 + tt * 32 >= n_pad) continue;
-        const bool real = preal[r] >= 0;
         const int kb = 16 * st + 8 * (l >> 5);
         uint16_t xv[8], yv[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int k = kb + e;
-            float xf = 0.f, yf = 0.f;
-            if (k < d) {
-                xf = xs[r * ld + k];
-                yf = -2.0f * xf;
-            } else if (k < d + 3) {
-                if (real) {
-                    xf = pn[r][k - d];
-                    yf = 1.0f;
-                } else {
-                    xf = (k == d) ? big : 0.f;
-                    yf = (k == d) ? 1.0f : 0.f;
-                }
-            } else if (k < d + 6) {
-                if (real) {
-                    xf = 1.0f;
-                    yf = pn[r][k - d - 3];
-                } else {
-                    xf = (k == d + 3) ? 1.0f : 0.f;
-                    yf = (k == d + 3) ? big : 0.f;
-                }
-            } else if (k < d + 9) {
-                // constant slots: x~ = 1, y~ = -C0 pieces (written by k_patch_c0)
-                xf = real ? 1.0f : 0.0f;
-            }
+            const float xf = k < d ? xs[r * ld + k] : 0.0f;
             xv[e] = f32_to_bf16_rn(xf);
-            yv[e] = f32_to_bf16_rn(yf);
+            yv[e] = f32_to_bf16_rn(-2.0f * xf);
         }
         const size_t slot = ((size_t)((p0 >> 5) + tt) * KS * 64 + rem) * 8;
         if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
@@ -395,24 +353,23 @@ __global__ __launch_bounds__(256) void k_prep_frags(
     }
     e_max = wave_max(e_max);
     n_max = wave_max(n_max);
-    r_max = wave_max(r_max);
     if (lane == 0) {
         atomicMax(&stat[ST_MAXE], __float_as_uint(e_max));
         atomicMax(&stat[ST_MAXN], __float_as_uint(n_max));
-        atomicMax(&stat[ST_MAXRES], __float_as_uint(r_max));
     }
 }
 
 int launch_prep(const float* X, int n, int n_pad, int d, int KS, const float* mu, const int* idmap,
-                uint16_t* xa, uint16_t* yb, unsigned* stat) {
+                uint16_t* xa, uint16_t* yb, float* xn, float* yn, unsigned* stat) {
     if (n_pad <= 0) return RNNG_OK;
     const size_t lds = (size_t)PP * (d + 1) * sizeof(float);
-    k_prep_frags<<<blocks_for(n_pad, PP), 256, lds, stream()>>>(X, n, n_pad, d, KS, mu, idmap, xa, yb, stat);
+    k_prep_frags<<<blocks_for(n_pad, PP), 256, lds, stream()>>>(X, n, n_pad, d, KS, mu, idmap, xa, yb, xn, yn,
+                                                                stat);
     RNNG_HIP_TRY(hipGetLastError());
     return RNNG_OK;
 }
 
-// C0: an upper bound of what the MFMA can compute for |x~ - y~|^2 for any pair
+// C0: an upper bound of what the MFMA can compute for |x' - y'|^2 for any pair
 // whose reference FP32 distance is <= the band radius.
 __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned* __restrict__ range,
                               int d, int KS, float Tb, MfmaParams* __restrict__ out) {
@@ -421,15 +378,14 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
     const double R = (double)ord2f(*range);
     const double maxE = (double)__uint_as_float(stat[ST_MAXE]);
     const double maxN = (double)__uint_as_float(stat[ST_MAXN]);
-    const double maxRes = (double)__uint_as_float(stat[ST_MAXRES]);
     const double two24 = 16777216.0;
     double c0;
     int int_mode = 0;
     if (isint && maxdev <= 256.0 && (double)d * R * R < two24 &&
         (double)Tb + 4.0 * maxN + 2.0 < two24) {
-        // All reference partial sums and all MFMA partial sums are integers
-        // below 2^24: both computations are exact, so acc = |x - y|^2 - C0,
-        // and with the integer C0 = floor(Tb) + 1, acc < 0 <=> |x - y|^2 <= Tb.
+        // Every norm, the start value xn + (yn - C0) and every MFMA partial sum
+        // is an integer below 2^24: all exact, so acc = |x - y|^2 - C0 and with
+        // the integer C0 = floor(Tb) + 1, acc < 0 <=> |x - y|^2 <= Tb.
         c0 = floor((double)Tb) + 1.0;
         int_mode = 1;
     } else {
@@ -438,8 +394,10 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
         const double smax = (double)Tb / (1.0 - g_ref);
         const double r = sqrt(smax) + 2.0 * maxE;
         const double base = r * r;
-        const double nterms = KS * 16.0 + 1.0;
-        c0 = base + 2.0 * maxRes + 2.0 * nterms * u * (base + 4.0 * maxN);
+        // FP32 accumulation over KS*16 products and the start value, plus the
+        // roundings of xn, yn, yn - C0 and their sum (each <= u times maxN or C0)
+        const double nterms = KS * 16.0 + 2.0;
+        c0 = base + 2.0 * nterms * u * (base + 4.0 * maxN) + 8.0 * u * (maxN + base);
         c0 = c0 * (1.0 + 1.0 / 1048576.0) + 1e-30;
     }
     float f = (float)c0;
@@ -447,43 +405,16 @@ __global__ void k_finalize_c0(const unsigned* __restrict__ stat, const unsigned*
     out->negc0 = -f;
     out->int_mode = int_mode;
     out->c0 = c0;
-    // Three bf16 pieces of C0 (exact for integers below 2^24); the last one is
-    // rounded up so that the pieces never sum below C0.
-    double r = c0;
-    float pc[3];
-    for (int q = 0; q < 2; ++q) {
-        pc[q] = bf16_to_f32(f32_to_bf16_rn((float)r));
-        r -= (double)pc[q];
-    }
-    float l = bf16_to_f32(f32_to_bf16_rn((float)r));
-    if ((double)l < r) {
-        uint32_t u = __float_as_uint(l);
-        // next bf16 toward +inf
-        if (l >= 0.0f) u += 0x10000u; else u -= 0x10000u;
-        l = __uint_as_float(u & 0xffff0000u);
-    }
-    pc[2] = l;
-    out->c0p[0] = pc[0];
-    out->c0p[1] = pc[1];
-    out->c0p[2] = pc[2];
 }
 
-// Writes -C0 into the three constant slots d+6..d+8 of every real point's
-// y-form fragment (the x-form holds 1.0 there), so that every accumulator
-// starts from 0 and still sums to |x~|^2 + |y~|^2 - 2 x~.y~ - C0.
-__global__ void k_patch_c0(uint16_t* __restrict__ yb, int n, int n_pad, int d, int KS,
-                           const int* __restrict__ idmap, const MfmaParams* __restrict__ prm) {
+// Column start values: ycn[j] = yn[j] - C0 for real points (padding keeps
+// its 2^100).
+__global__ void k_column_bias(const float* __restrict__ yn, float* __restrict__ ycn, int n_pad,
+                              const MfmaParams* __restrict__ prm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
-    const bool real = idmap ? idmap[i] >= 0 : i < n;
-    if (!real) return;
-    const int t = i >> 5, l = i & 31;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        const int k = d + 6 + m;
-        const int st = k >> 4, h = (k >> 3) & 1, e = k & 7;
-        yb[((size_t)(t * KS + st) * 64 + l + 32 * h) * 8 + e] = f32_to_bf16_rn(-prm->c0p[m]);
-    }
+    const float v = yn[i];
+    ycn[i] = v >= kPadNorm ? v : v + prm->negc0;
 }
 
 // ------------------------------------------------------------------------
@@ -504,8 +435,8 @@ __global__ void k_patch_c0(uint16_t* __restrict__ yb, int n, int n_pad, int d, i
 
 template <int KS, int RT, bool USE_LDS, int CT = 1>
 __global__ __launch_bounds__(256) void k_l2_mfma_filter(
-    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb,
-    const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb, const float* __restrict__ xn,
+    const float* __restrict__ ycn, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
     const MfmaParams* __restrict__ prm, int2* __restrict__ cand, unsigned long long cap,
     unsigned long long* __restrict__ counter) {
     // CT column tiles per step (USE_LDS only): RT*CT independent accumulator
@@ -534,8 +465,18 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
+    // row part of the accumulator start: the norm of each register's row
+    float xr[RT][16];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[r][q] = xn[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
     // column tiles beyond wi.z read a valid tile (wi.z - 1) and are masked out
     const int last = wi.z - 1;
+    // column part (norm - C0) of this step's column tiles, loaded a step ahead
+    float yc[CT];
+#pragma unroll
+    for (int cc = 0; cc < CT; ++cc) yc[cc] = ycn[min(wi.y + cc, last) * 32 + (lane & 31)];
 
     bf16x8_t b[USE_LDS ? 1 : KS];
     if (USE_LDS) {
@@ -556,6 +497,9 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
     for (int ct = wi.y; ct < wi.z; ct += CT) {
         const int buf = ((ct - wi.y) / CT) & 1;
         const bool has_next = ct + CT < wi.z;
+        float ycn_next[CT];
+#pragma unroll
+        for (int cc = 0; cc < CT; ++cc) ycn_next[cc] = ycn[min(ct + CT + cc, last) * 32 + (lane & 31)];
         bf16x8_t pre[USE_LDS ? PER : KS];
         if (USE_LDS) {
             if (has_next) {
@@ -580,7 +524,7 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[cc][r][q] = 0.0f;
+                for (int q = 0; q < 16; ++q) acc[cc][r][q] = xr[r][q] + yc[cc];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
 #pragma unroll
@@ -602,8 +546,12 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
         if (__any((int)o < 0)) {
             // Rare path: sign bits -> one mask per lane and column tile, then a
             // rolled loop with one ballot per accumulator register, staged
-            // through LDS (one LDS atomic per ballot).
+            // through LDS (one LDS atomic per ballot).  The mask's shifts
+            // depend on an opaque zero, so the compiler cannot speculate them
+            // into every tile of the hot loop.
             const int rowb = rt0 * 32 + 4 * (lane >> 5);
+            int opaque;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
 #pragma unroll
             for (int cc = 0; cc < CT; ++cc) {
                 unsigned long long mask = 0;
@@ -611,7 +559,7 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
                 for (int r = 0; r < RT; ++r)
 #pragma unroll
                     for (int q = 0; q < 16; ++q)
-                        mask |= (unsigned long long)(__float_as_uint(acc[cc][r][q]) >> 31) << (r * 16 + q);
+                        mask |= (unsigned long long)(__float_as_uint(acc[cc][r][q]) >> 31) << (r * 16 + q + opaque);
                 const int col = (ct + cc) * 32 + (lane & 31);
                 const int lim = (ct + cc < wi.z && col < n_cols)
                                     ? (self_join ? min(n_rows, col) : n_rows) : 0;
@@ -663,6 +611,8 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
 #pragma unroll
             for (int s = 0; s < KS; ++s) b[USE_LDS ? 0 : s] = pre[s];
         }
+#pragma unroll
+        for (int cc = 0; cc < CT; ++cc) yc[cc] = ycn_next[cc];
     }
     __syncthreads();
     const unsigned nst = min(scount, (unsigned)LSTAGE);
@@ -696,16 +646,19 @@ __device__ __forceinline__ void filter_g_load(const bf16x8_t* __restrict__ yb, b
 
 template <int KS, int G, int RT>
 __device__ __forceinline__ void filter_g_compute(
-    const bf16x8_t* bsrc, const bf16x8_t (&a)[RT][KS], int ct, int zend, int lane, int rowb,
-    int n_rows, int n_cols, int self_join, int2* stage, unsigned* scount, int2* __restrict__ cand,
-    unsigned long long cap, unsigned long long* __restrict__ counter) {
+    const bf16x8_t* bsrc, const bf16x8_t (&a)[RT][KS], const float (&xr)[RT][16], const float* __restrict__ ycn,
+    int ct, int zend, int lane, int rowb, int n_rows, int n_cols, int self_join, int2* stage, unsigned* scount,
+    int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
+    float yc[G];
+#pragma unroll
+    for (int cc = 0; cc < G; ++cc) yc[cc] = ycn[min(ct + cc, zend - 1) * 32 + (lane & 31)];
 #pragma unroll
     for (int cc = 0; cc < G; ++cc) {
         f32x16_t acc[RT];
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
-            for (int q = 0; q < 16; ++q) acc[r][q] = 0.0f;
+            for (int q = 0; q < 16; ++q) acc[r][q] = xr[r][q] + yc[cc];
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const bf16x8_t bb = bsrc[(cc * KS + s) * 64 + lane];
@@ -721,12 +674,15 @@ __device__ __forceinline__ void filter_g_compute(
         if (__any((int)o < 0)) {
             // Rare path: the sign bits become one 32-bit mask per lane, then a
             // rolled loop emits one ballot per accumulator register, staged
-            // through LDS.  Keeps the rare path's registers out of the hot loop.
+            // through LDS.  The shifts depend on an opaque zero so that they
+            // stay out of the hot loop.
+            int opaque;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
             unsigned mask = 0;
 #pragma unroll
             for (int r = 0; r < RT; ++r)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) mask |= (__float_as_uint(acc[r][q]) >> 31) << (r * 16 + q);
+                for (int q = 0; q < 16; ++q) mask |= (__float_as_uint(acc[r][q]) >> 31) << (r * 16 + q + opaque);
             const int col = (ct + cc) * 32 + (lane & 31);
             const int lim = (ct + cc < zend && col < n_cols) ? (self_join ? min(n_rows, col) : n_rows) : 0;
             // only the registers with a sign bit set in some lane are visited
@@ -764,8 +720,8 @@ __device__ __forceinline__ void filter_g_compute(
 
 template <int KS, int G>
 __global__ __launch_bounds__(256) void k_l2_mfma_filter_g(
-    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb,
-    const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    const bf16x8_t* __restrict__ xa, const bf16x8_t* __restrict__ yb, const float* __restrict__ xn,
+    const float* __restrict__ ycn, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
     const MfmaParams* __restrict__ prm, int2* __restrict__ cand, unsigned long long cap,
     unsigned long long* __restrict__ counter) {
     constexpr int RT = 2;
@@ -790,6 +746,11 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_g(
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
+    float xr[RT][16];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[r][q] = xn[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
     const int rowb = rt0 * 32 + 4 * (lane >> 5);
 
     int ct = wi.y;
@@ -797,14 +758,14 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_g(
         __builtin_amdgcn_s_waitcnt(0x3f70);   // vmcnt(0): this wave's share of the group landed
         __syncthreads();                        // every share landed; bsB is no longer read
         if (ct + G < wi.z) filter_g_load<KS, G>(yb, bsB, ct + G, last, wave, lane);
-        filter_g_compute<KS, G, RT>(bsA, a, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
+        filter_g_compute<KS, G, RT>(bsA, a, xr, ycn, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
                                     &scount, cand, cap, counter);
         ct += G;
         if (ct >= wi.z) break;
         __builtin_amdgcn_s_waitcnt(0x3f70);
         __syncthreads();                        // bsA is no longer read
         if (ct + G < wi.z) filter_g_load<KS, G>(yb, bsA, ct + G, last, wave, lane);
-        filter_g_compute<KS, G, RT>(bsB, a, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
+        filter_g_compute<KS, G, RT>(bsB, a, xr, ycn, ct, wi.z, lane, rowb, n_rows, n_cols, self_join, stage,
                                     &scount, cand, cap, counter);
         ct += G;
         if (ct >= wi.z) break;
@@ -1002,6 +963,7 @@ struct Workspace {
     DevBuf cw;         // float per candidate: its distance
     unsigned long long cand_cap = 0;
     DevBuf xa, yb;     // fragments
+    DevBuf xn, yn, ycn;   // row norms, column norms, column norms - C0
     DevBuf colsum, colmin, colmax, stat, range, mu, params;
     DevBuf work;
     std::vector<int4> work_host;
@@ -1088,13 +1050,13 @@ int build_work(Workspace& ws, int n_rowblocks, int rb_tiles, int n_coltiles, int
 template <int KS, int RT, bool LDS, int CT = 1>
 void launch_filter(const FilterArgs& a) {
     k_l2_mfma_filter<KS, RT, LDS, CT><<<a.n_items, 256, 0, stream()>>>(
-        a.xa, a.yb, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
+        a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
 template <int KS, int G>
 void launch_filter_g(const FilterArgs& a) {
     k_l2_mfma_filter_g<KS, G><<<a.n_items, 256, 0, stream()>>>(
-        a.xa, a.yb, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
+        a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
 template <int KS>
@@ -1121,6 +1083,8 @@ FilterArgs ws_args(Workspace& ws, int n_rows, int n_cols, int self_join) {
     FilterArgs a;
     a.xa = ws.xa.as<bf16x8_t>();
     a.yb = ws.yb.as<bf16x8_t>();
+    a.xn = ws.xn.as<float>();
+    a.ycn = ws.ycn.as<float>();
     a.work = ws.work.as<int4>();
     a.n_items = (unsigned)ws.work_host.size();
     a.n_rows = n_rows;
@@ -1184,9 +1148,9 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
              rnng_graph_s* g, rnng_stats* st) {
     Workspace& ws = g_ws;
     const int nx = (int)A->n, ny = (int)B->n, d = A->d;
-    const int KS = (d + 9 + 15) / 16;
-    if (KS > 9) {
-        set_error("MFMA mode supports d <= 135");
+    const int KS = mfma_ksteps(d);
+    if (KS > kMaxKS) {
+        set_error("MFMA mode supports d <= 144");
         return RNNG_ERR_UNSUPPORTED;
     }
     const float T = sq_threshold_host(eps);
@@ -1208,6 +1172,9 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     // For a self join the x-form and the y-form come from the same points.
     RNNG_TRY(ws.xa.reserve(frag_bytes_x));
     RNNG_TRY(ws.yb.reserve(frag_bytes_y));
+    RNNG_TRY(ws.xn.reserve((size_t)npad_x * sizeof(float)));
+    RNNG_TRY(ws.yn.reserve((size_t)std::max(npad_x, npad_y) * sizeof(float)));
+    RNNG_TRY(ws.ycn.reserve((size_t)std::max(npad_x, npad_y) * sizeof(float)));
     RNNG_TRY(ws.colsum.reserve(sizeof(double) * d));
     RNNG_TRY(ws.colmin.reserve(sizeof(unsigned) * d));
     RNNG_TRY(ws.colmax.reserve(sizeof(unsigned) * d));
@@ -1265,15 +1232,20 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
                                                       ws.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
     if (self_join) {
-        // one pass writes both forms for every padded row of the join
+        // one pass writes both forms for every padded row of the join (the
+        // row norms cover npad_x rows, the column norms every padded row)
         const int npad = std::max(npad_x, npad_y);
         RNNG_TRY(launch_prep(A->X, nx, npad, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
-                             ws.yb.as<uint16_t>(), ws.stat.as<unsigned>()));
+                             ws.yb.as<uint16_t>(), npad == npad_x ? ws.xn.as<float>() : nullptr,
+                             ws.yn.as<float>(), ws.stat.as<unsigned>()));
+        if (npad != npad_x)
+            RNNG_HIP_TRY(hipMemcpyAsync(ws.xn.ptr, ws.yn.ptr, (size_t)npad_x * sizeof(float),
+                                        hipMemcpyDeviceToDevice, stream()));
     } else {
         RNNG_TRY(launch_prep(A->X, nx, npad_x, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
-                             nullptr, ws.stat.as<unsigned>()));
+                             nullptr, ws.xn.as<float>(), nullptr, ws.stat.as<unsigned>()));
         RNNG_TRY(launch_prep(B->X, ny, npad_y, d, KS, ws.mu.as<float>(), nullptr, nullptr,
-                             ws.yb.as<uint16_t>(), ws.stat.as<unsigned>()));
+                             ws.yb.as<uint16_t>(), nullptr, ws.yn.as<float>(), ws.stat.as<unsigned>()));
     }
     RNNG_HIP_TRY(hipGetLastError());
     k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
@@ -1281,8 +1253,8 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     RNNG_HIP_TRY(hipGetLastError());
     {
         const int npy = self_join ? std::max(npad_x, npad_y) : npad_y;
-        k_patch_c0<<<blocks_for(npy, 256), 256, 0, stream()>>>(ws.yb.as<uint16_t>(), self_join ? nx : ny, npy, d,
-                                                                KS, nullptr, ws.params.as<MfmaParams>());
+        k_column_bias<<<blocks_for(npy, 256), 256, 0, stream()>>>(ws.yn.as<float>(), ws.ycn.as<float>(), npy,
+                                                                   ws.params.as<MfmaParams>());
         RNNG_HIP_TRY(hipGetLastError());
     }
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
@@ -1364,7 +1336,7 @@ int filter_dispatch(const FilterArgs& a, int KS, int variant) {
         case 8: launch_filter_v<8>(a, variant); break;
         case 9: launch_filter_v<9>(a, variant); break;
         default:
-            set_error("MFMA mode supports d <= 135");
+            set_error("MFMA mode supports d <= 144");
             return RNNG_ERR_UNSUPPORTED;
     }
     RNNG_HIP_TRY(hipGetLastError());
@@ -1415,12 +1387,15 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                         int n_rowpad, const int* colmap, int n_colpad) {
     const int KS = mfma_ksteps(d);
     if (KS > kMaxKS) {
-        set_error("MFMA mode supports d <= 135");
+        set_error("MFMA mode supports d <= 144");
         return RNNG_ERR_UNSUPPORTED;
     }
     P.KS = KS;
     RNNG_TRY(P.xa.reserve(std::max<size_t>(16, (size_t)n_rowpad * KS * 16 * 2)));
     RNNG_TRY(P.yb.reserve(std::max<size_t>(16, (size_t)n_colpad * KS * 16 * 2)));
+    RNNG_TRY(P.xn.reserve(std::max<size_t>(16, (size_t)n_rowpad * sizeof(float))));
+    RNNG_TRY(P.yn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
+    RNNG_TRY(P.ycn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
     RNNG_TRY(P.colsum.reserve(sizeof(double) * d));
     RNNG_TRY(P.colmin.reserve(sizeof(unsigned) * d));
     RNNG_TRY(P.colmax.reserve(sizeof(unsigned) * d));
@@ -1446,19 +1421,19 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
     if (rowmap == colmap && n_rowpad == n_colpad) {
         // one pass writes both forms
         RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(),
-                             P.yb.as<uint16_t>(), P.stat.as<unsigned>()));
+                             P.yb.as<uint16_t>(), P.xn.as<float>(), P.yn.as<float>(), P.stat.as<unsigned>()));
     } else {
         RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(), nullptr,
-                             P.stat.as<unsigned>()));
+                             P.xn.as<float>(), nullptr, P.stat.as<unsigned>()));
         RNNG_TRY(launch_prep(X, n, n_colpad, d, KS, P.mu.as<float>(), colmap, nullptr, P.yb.as<uint16_t>(),
-                             P.stat.as<unsigned>()));
+                             nullptr, P.yn.as<float>(), P.stat.as<unsigned>()));
     }
     k_finalize_c0<<<1, 1, 0, stream()>>>(P.stat.as<unsigned>(), P.range.as<unsigned>(), d, KS, Tb,
                                          P.params.as<MfmaParams>());
     RNNG_HIP_TRY(hipGetLastError());
     if (n_colpad > 0) {
-        k_patch_c0<<<blocks_for(n_colpad, 256), 256, 0, stream()>>>(P.yb.as<uint16_t>(), n, n_colpad, d, KS, colmap,
-                                                                    P.params.as<MfmaParams>());
+        k_column_bias<<<blocks_for(n_colpad, 256), 256, 0, stream()>>>(P.yn.as<float>(), P.ycn.as<float>(), n_colpad,
+                                                                       P.params.as<MfmaParams>());
         RNNG_HIP_TRY(hipGetLastError());
     }
     return RNNG_OK;

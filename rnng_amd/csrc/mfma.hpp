@@ -15,25 +15,25 @@ struct MfmaParams {
     float negc0;      // -C0 as float (|negc0| >= C0)
     int int_mode;     // 1: exact integer filter
     double c0;        // C0 in double, for reporting
-    float c0p[3];     // bf16 pieces with c0p[0] + c0p[1] + c0p[2] >= C0
 };
 
-// Fragment buffers and the statistics that size C0.
+// Fragment buffers, start values and the statistics that size C0.
 struct MfmaPrep {
     LocalBuf xa, yb;                                   // bf16 fragments (x-form rows, y-form columns)
+    LocalBuf xn, yn, ycn;                              // row norms, column norms, column norms - C0
     LocalBuf colsum, colmin, colmax, stat, range, mu;  // centering and error statistics
     LocalBuf params;                                   // MfmaParams
     int KS = 0;                                      // 16-wide k-steps per fragment
 };
 
-// Number of 16-wide k-steps for dimension d (d data dims + 9 augmented slots).
-inline int mfma_ksteps(int d) { return (d + 9 + 15) / 16; }
-constexpr int kMaxKS = 9;     // d <= 135
+// Number of 16-wide k-steps for dimension d.
+inline int mfma_ksteps(int d) { return (d + 15) / 16; }
+constexpr int kMaxKS = 9;     // d <= 144
 
-// Builds the x-form fragments of the padded row positions (rowmap[p] = point
-// index or -1) and the y-form fragments of the padded column positions
-// (colmap), centred and scaled over all n points of X, with C0 for the
-// squared band radius Tb written into the y-form constant slots.
+// Builds the x-form fragments and row norms of the padded row positions
+// (rowmap[p] = point index or -1) and the y-form fragments and column start
+// values (norm - C0, C0 for the squared band radius Tb) of the padded column
+// positions (colmap), centred over all n points of X.
 int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
                         int n_rowpad, const int* colmap, int n_colpad);
 
@@ -45,6 +45,8 @@ constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
 struct FilterArgs {
     const bf16x8_t* xa = nullptr;
     const bf16x8_t* yb = nullptr;
+    const float* xn = nullptr;     // row norms (accumulator start, row part)
+    const float* ycn = nullptr;    // column norms - C0 (accumulator start, column part)
     const int4* work = nullptr;
     unsigned n_items = 0;
     int n_rows = 0, n_cols = 0, self_join = 0;

@@ -832,7 +832,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     const int n = (int)v->ds->n, m = (int)v->m, d = v->ds->d;
     const int KS = mfma_ksteps(d);
     if (KS > kMaxKS) {
-        set_error("MFMA mode supports d <= 135");
+        set_error("MFMA mode supports d <= 144");
         return RNNG_ERR_UNSUPPORTED;
     }
     const float T = sq_threshold_host(eps);
@@ -904,6 +904,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         FilterArgs fa;
         fa.xa = P.xa.as<bf16x8_t>();
         fa.yb = P.yb.as<bf16x8_t>();
+        fa.xn = P.xn.as<float>();
+        fa.ycn = P.ycn.as<float>();
         fa.work = work.as<int4>();
         fa.n_items = (unsigned)n_items;
         fa.n_rows = n_pad;
@@ -1022,7 +1024,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             (uint64_t)(uintptr_t)cand.ptr, (uint64_t)(uintptr_t)cid.ptr,  (uint64_t)(uintptr_t)cflag.ptr,
             (uint64_t)(uintptr_t)cw.ptr,   (uint64_t)(uintptr_t)v->gkeys.ptr, (uint64_t)(uintptr_t)v->counts.ptr,
             (uint64_t)(uintptr_t)d_items.ptr, (uint64_t)items.size(), (uint64_t)(uintptr_t)P.xa.ptr,
-            (uint64_t)(uintptr_t)P.yb.ptr, (uint64_t)(uintptr_t)P.params.ptr, (uint64_t)(uintptr_t)slot.ptr,
+            (uint64_t)(uintptr_t)P.yb.ptr, (uint64_t)(uintptr_t)P.xn.ptr, (uint64_t)(uintptr_t)P.ycn.ptr,
+            (uint64_t)(uintptr_t)P.params.ptr, (uint64_t)(uintptr_t)slot.ptr,
             (uint64_t)(uintptr_t)counters.ptr, (uint64_t)(uintptr_t)v->ds->X, (uint64_t)(uintptr_t)v->C.ptr,
             (uint64_t)(uintptr_t)v->cell.ptr, (uint64_t)(uintptr_t)v->delta.ptr, cap, (uint64_t)variant,
             (uint64_t)n_pad, (uint64_t)eb, (uint64_t)tb, (uint64_t)m, (uint64_t)d};
