@@ -1,0 +1,83 @@
+"""Writes the golden fixtures of tests/golden/ (run from the repository root:
+python tests/golden/make_fixtures.py).  Every expected value here comes from
+a closed form, a textbook definition or the oracle at the 10k x 16 workload the
+reference runs on a CPU; the fixtures freeze them so the GPU box (which has
+no oracle build requirements beyond this repository) checks against data.
+
+  l2_known.json        vectors with exactly representable squared distances,
+                       and T(eps) values checked by their defining property
+  edit_known.json      textbook Levenshtein distances
+  selection.json       selection_sample outputs (the oracle's restatement of
+                       the reference's engine and distributions)
+  simplex_ids.json     combinatorial-number-system ids, closed form
+  gmm_small.json       the 10,000 x 16 workload: eps, edge count and SHA-256
+                       of the sorted (i, j) int64 pairs and float32 weights
+"""
+import hashlib
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+
+def main():
+    import oracle
+    from rnng_amd import synth
+
+    # exact sums: integer coordinates, squared distance below 2^24
+    l2 = []
+    rng = np.random.default_rng(7)
+    for d in (1, 3, 16, 128):
+        a = rng.integers(-40, 40, size=d)
+        b = rng.integers(-40, 40, size=d)
+        l2.append({"a": a.tolist(), "b": b.tolist(), "sq": int(((a - b) ** 2).sum())})
+    thr = []
+    for eps in (0.0, 1.0, 2.5, 256.28891, 0.9644456, 1e-3, 3.0e4):
+        e = np.float32(eps)
+        t = np.float32(oracle.sq_threshold(e))
+        # defining property: sqrtf(t) <= eps < sqrtf(nextafter(t, inf))
+        assert np.sqrt(t) <= e and np.sqrt(np.nextafter(t, np.float32(np.inf))) > e
+        thr.append({"eps_bits": int(e.view(np.uint32)), "t_bits": int(t.view(np.uint32))})
+    json.dump({"pairs": l2, "thresholds": thr}, open(os.path.join(HERE, "l2_known.json"), "w"), indent=1)
+
+    edit = [("kitten", "sitting", 3), ("flaw", "lawn", 2), ("", "abc", 3), ("abc", "", 3), ("", "", 0),
+            ("gumbo", "gambol", 2), ("intention", "execution", 5), ("ACGTACGT", "ACGTACGT", 0),
+            ("ACGTACGT", "TCGTACGA", 2), ("AAAA", "AAAAAAAA", 4)]
+    for s, t, k in edit:
+        assert oracle.edit_distance(s.encode(), t.encode()) == k
+    json.dump([{"s": s, "t": t, "d": k} for s, t, k in edit], open(os.path.join(HERE, "edit_known.json"), "w"),
+              indent=1)
+
+    sel = []
+    for rng_, size, seed in ((100, 10, 1), (1000, 37, 0), (2_000_000, 64, 7), (50, 50, 3), (123457, 20, 2147483647)):
+        sel.append({"range": rng_, "size": size, "seed": seed,
+                    "sample": oracle.selection_sample(rng_, size, seed).tolist()})
+    json.dump(sel, open(os.path.join(HERE, "selection.json"), "w"), indent=1)
+
+    simp = []
+    for verts in ([0], [7], [0, 1], [3, 9], [2, 7, 11], [999999, 3, 500000], [1, 5, 9, 40]):
+        v = sorted(verts)
+        sid = sum(math.comb(x, i + 1) for i, x in enumerate(v)) | ((len(v) - 1) << 60)
+        assert sid == oracle.simplex_id(np.array(v))
+        simp.append({"verts": verts, "id": sid})
+    json.dump(simp, open(os.path.join(HERE, "simplex_ids.json"), "w"), indent=1)
+
+    pts, eps = synth.points_gmm_small()
+    ij, w = oracle.l2_eps_pairs(pts, eps)
+    gm = {"n": int(pts.shape[0]), "d": int(pts.shape[1]), "eps_bits": int(np.float32(eps).view(np.uint32)),
+          "edges": int(ij.shape[0]),
+          "sha256_ij_int64": hashlib.sha256(np.ascontiguousarray(ij, np.int64).tobytes()).hexdigest(),
+          "sha256_w_float32": hashlib.sha256(np.ascontiguousarray(w, np.float32).tobytes()).hexdigest(),
+          "sha256_points_float32": hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest()}
+    json.dump(gm, open(os.path.join(HERE, "gmm_small.json"), "w"), indent=1)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()
