@@ -9,8 +9,10 @@ inference passes
 candidate buffer from the summed per-cell estimates and runs the filter once;
 without it the build counts first and fills in a second pass.
 
-`python -m rnng_amd.costmodel OUT.npz [--runs 256]` collects per-cell counts
-from 256 synthetic cell builds on the GPU and trains the network on them.
+`python -m rnng_amd.costmodel OUT.npz` collects per-cell counts from 256
+synthetic cell builds of 3k-300k points and 24 of 0.8M-2M points on the GPU and
+trains the network on them (12,000 full-batch Adam steps, lr 3e-3): the shipped
+cost_model.npz.
 """
 from __future__ import annotations
 
@@ -118,12 +120,20 @@ def decode(y: np.ndarray) -> np.ndarray:
     return np.exp2(24.0 * np.clip(y, 0.0, 1.5)) - 1.0
 
 
-def synthetic_run(rng: np.random.Generator, big: float = 0.0):
+def synthetic_run(rng: np.random.Generator, big: float = 0.0, huge: float = 0.0):
     """One varied cell workload: (points, eps, m, fps?).  With probability
-    `big` the run has 100k-300k points and up to 2,048 cells."""
+    `big` the run has 100k-300k points and up to 2,048 cells, with probability
+    `huge` 0.8M-2M points in 2,048 or 4,096 cells."""
     from . import synth
-    large = rng.random() < big
-    n = int(rng.integers(100_000, 300_001)) if large else int(rng.integers(3000, 30001))
+    u = rng.random()
+    very = u < huge
+    large = very or u < huge + big
+    if very:
+        n = int(rng.integers(800_000, 2_000_001))
+    elif large:
+        n = int(rng.integers(100_000, 300_001))
+    else:
+        n = int(rng.integers(3000, 30001))
     d = int(rng.choice([8, 16, 32, 64, 96, 128]))
     kind = int(rng.integers(0, 4))
     if kind == 3:       # the vMF-like unit-vector mixture of the 2M x 96 cell workload, smaller
@@ -147,12 +157,12 @@ def synthetic_run(rng: np.random.Generator, big: float = 0.0):
     pts = np.ascontiguousarray(pts, dtype=np.float32)
     degree = float(rng.uniform(2.0, 60.0))
     eps = synth.radius_for_degree(pts, degree, sample=min(n, 1500), seed=int(rng.integers(1 << 30)))
-    sizes = [256, 512, 1024, 2048] if large else [8, 16, 32, 64, 128, 256, 512]
+    sizes = [2048, 4096] if very else ([256, 512, 1024, 2048] if large else [8, 16, 32, 64, 128, 256, 512])
     m = int(min(n // 8, rng.choice(sizes)))
-    return pts, eps, max(m, 2), bool(rng.integers(0, 2))
+    return pts, eps, max(m, 2), bool(rng.integers(0, 2)) and not very
 
 
-def collect(runs: int = 256, seed: int = 1, log=None, big: float = 0.0):
+def collect(runs: int = 256, seed: int = 1, log=None, big: float = 0.0, huge: float = 0.0):
     """Per-cell (features, targets) from `runs` synthetic cell builds, each
     run without a model (count-then-fill)."""
     from . import l2, voronoi
@@ -160,7 +170,7 @@ def collect(runs: int = 256, seed: int = 1, log=None, big: float = 0.0):
     rng = np.random.default_rng(seed)
     X, T = [], []
     for r in range(runs):
-        pts, eps, m, use_fps = synthetic_run(rng, big)
+        pts, eps, m, use_fps = synthetic_run(rng, big, huge)
         ds = l2.Dataset(pts)
         if use_fps:
             v = voronoi.Voronoi(ds, fps=m, first=int(rng.integers(0, pts.shape[0])))
@@ -183,8 +193,10 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train the cell-query cost model on the GPU")
     ap.add_argument("out")
-    ap.add_argument("--runs", type=int, default=256)
-    ap.add_argument("--iters", type=int, default=4000)
+    ap.add_argument("--runs", type=int, default=256, help="runs of 3k-300k points")
+    ap.add_argument("--huge-runs", type=int, default=24, help="additional runs of 0.8M-2M points")
+    ap.add_argument("--iters", type=int, default=12000)
+    ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--big", type=float, default=0.25, help="share of 100k-300k point runs")
     a = ap.parse_args(argv)
@@ -194,8 +206,11 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     X, T = collect(a.runs, a.seed, log, big=a.big)
-    log(f"collected {X.shape[0]} cells from {a.runs} runs in {time.perf_counter() - t0:.1f}s")
-    p, loss = train(init_params(a.seed), X, T, iters=a.iters)
+    if a.huge_runs:
+        Xh, Th = collect(a.huge_runs, a.seed + 4, log, huge=1.0)
+        X, T = np.concatenate([X, Xh]), np.concatenate([T, Th])
+    log(f"collected {X.shape[0]} cells from {a.runs + a.huge_runs} runs in {time.perf_counter() - t0:.1f}s")
+    p, loss = train(init_params(a.seed), X, T, iters=a.iters, lr=a.lr)
     log(f"trained: loss {loss:.3e}")
     np.savez(a.out, params=p, features=X, targets=T)
 
