@@ -38,6 +38,9 @@ extern "C" {
 /* Distance-stage algorithm. */
 #define RNNG_MODE_PLAIN 0   /* FP32 tiled kernel in the reference's order */
 #define RNNG_MODE_MFMA  1   /* BF16 MFMA candidate filter + FP32 recheck + FP64 band check */
+#define RNNG_MODE_GRID  2   /* self join, d <= 3: uniform grid of cells of side >= eps; only
+                               pairs in adjacent cells are evaluated, each in the reference's
+                               FP32 order (same edges as PLAIN); pair_evals counts them */
 
 /* Counters and device times of one call.  Times are HIP-event times on the
  * library's stream, in milliseconds. */
@@ -177,7 +180,10 @@ typedef struct rnng_filtration_s *rnng_filtration_t;   /* device-resident simpli
  * include/simplex.hpp:130-196).  Triangles need n <= 1,800,000 (60-bit ids);
  * tetrahedra n <= 55,111 (beyond it the reference's int64 binomial
  * v(v-1)(v-2)(v-3) overflows, include/utils.hpp:126-127); larger inputs give
- * RNNG_ERR_UNSUPPORTED. */
+ * RNNG_ERR_UNSUPPORTED.  The graph runs in RNNG_MODE_GRID for d <= 3,
+ * RNNG_MODE_MFMA for d >= 16 and RNNG_MODE_PLAIN otherwise.  stats: pair
+ * counters of the graph; filter_ms = graph, recheck_ms = CSR and cliques,
+ * prep_ms = (value, id) ordering, total_ms = all. */
 int rnng_rips_build(rnng_dataset_t ds, float eps, int32_t maxdim, rnng_filtration_t *out,
                     rnng_stats *stats);
 

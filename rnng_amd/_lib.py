@@ -23,6 +23,7 @@ RNNG_ERR_UNSUPPORTED = -6
 
 MODE_PLAIN = 0
 MODE_MFMA = 1
+MODE_GRID = 2
 
 
 class RnngError(RuntimeError):

@@ -49,6 +49,11 @@ inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1)
 int sort_pairs_u64_f32(uint64_t* keys, float* vals, int64_t n);
 int sort_pairs_u64_i64(uint64_t* keys, int64_t* vals, int64_t n);
 int sort_by_value_then_id(float* value, int64_t* id, int64_t n);
+// Stable LSD sorts on key bits [0, end_bit).
+int sort_pairs_u32_i32(uint32_t* keys, int32_t* vals, int64_t n, unsigned end_bit);
+int sort_pairs_u32_i64(uint32_t* keys, int64_t* vals, int64_t n, unsigned end_bit);
+int sort_pairs_u32_u32(uint32_t* keys, uint32_t* vals, int64_t n, unsigned end_bit);
+int sort_pairs_u64_f32_bits(uint64_t* keys, float* vals, int64_t n, unsigned end_bit);
 int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n);
 // Stable compaction of (ij, w) by flag != 0 into (out_ij, out_w); *n_out gets
 // the count.  The outputs must hold n elements.
@@ -58,5 +63,8 @@ int compact_edges(const int2* ij, const float* w, const uint8_t* flag, int64_t n
 // Builds the L2 epsilon graph of a self join (l2_eps.hip).
 int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
                    int32_t mode, rnng_graph_s* g, rnng_stats* st);
+
+// Uniform-grid self join for d <= 3 (grid.hip); eps must be finite.
+int grid_graph_build(const rnng_dataset_s* A, float eps, rnng_graph_s* g, rnng_stats* st);
 
 }  // namespace rnng

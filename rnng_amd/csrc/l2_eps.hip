@@ -1448,12 +1448,18 @@ int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_j
         set_error("dimension mismatch or d <= 0");
         return RNNG_ERR_ARG;
     }
-    if (mode != RNNG_MODE_PLAIN && mode != RNNG_MODE_MFMA) {
+    if (mode != RNNG_MODE_PLAIN && mode != RNNG_MODE_MFMA && mode != RNNG_MODE_GRID) {
         set_error("unknown mode");
         return RNNG_ERR_ARG;
     }
+    if (mode == RNNG_MODE_GRID && (!self_join || A->d > 3)) {
+        set_error("grid mode is a self join of points with d <= 3");
+        return RNNG_ERR_UNSUPPORTED;
+    }
     std::lock_guard<std::mutex> lk(g_ws_mu);
     RNNG_TRY(g_ws.counters.reserve(8 * sizeof(unsigned long long)));
+    // an infinite eps makes every pair an edge: the grid has nothing to prune
+    if (mode == RNNG_MODE_GRID && std::isfinite(eps)) return grid_graph_build(A, eps, g, st);
     if (st) {
         const long long nx = A->n, ny = B->n;
         st->pair_evals += self_join ? nx * (nx - 1) / 2 : nx * ny;

@@ -1,5 +1,5 @@
-// Vietoris-Rips filtration (vertices, edges, triangles) of the L2 epsilon
-// graph, for gfx950.
+// Vietoris-Rips filtration (vertices, edges, triangles, tetrahedra) of the
+// L2 epsilon graph, for gfx950.
 //
 // Reference semantics (gabe-raulet/RNNG):
 //   VoronoiComplex::build_filtration  include/point.hpp:1134-1232
@@ -14,11 +14,25 @@
 // point set (every clique lies in the complex of the cell of any of its
 // vertices), so one GPU builds it directly from the global epsilon graph.
 //
-// Pipeline: epsilon graph -> device sort by (u, v) -> CSR of upper
-// neighbours -> per-edge merge intersection of the two sorted neighbour lists
-// (triangles) and, for dimension 3, a three-way merge with the third vertex's
-// list (tetrahedra); count, exclusive scan, fill: deterministic, no atomics ->
-// filtration arrays -> device sort by (value, id).
+// Pipeline:
+//  1. epsilon graph (uniform grid for d <= 3, MFMA filter for d >= 16, the
+//     plain kernel otherwise);
+//  2. lower CSR: row = larger vertex, columns = smaller neighbours ascending.
+//     Its edge order is (larger, smaller) lexicographic, which is the order of
+//     the edge ids C(j, 2) + i ("colex");
+//  3. value groups: edges sorted by (weight, id) with one stable 32-bit sort;
+//     every distinct value gets a dense rank (group), vertices are group 0;
+//  4. cliques in colex order: triangles (u, v, w) through lower edge (v, w)
+//     from the merge of the two lower lists, tetrahedra (u, v, w, x) through
+//     (w, x) from a merge for v and a three-way merge for u.  Edges are visited
+//     in colex order and each edge's cliques come out ascending, so every
+//     clique array is already sorted by id.  A clique's group is the largest
+//     group of its edges (its value is the largest edge weight);
+//  5. order: one stable sort of each clique array by group (ceil(log2 G)
+//     bits) turns id order into (value, id) order; per-group counts and one
+//     scan place vertices, edges, triangles and tetrahedra of each group
+//     (p << 60 in the id puts them in that order inside a group).
+// Count, scan, fill throughout: deterministic, no atomics.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -44,43 +58,84 @@ __device__ __forceinline__ int64_t binom4(int64_t v) { return binom3(v) * (v - 3
 // (include/utils.hpp:126-127), which stays below 2^63 for v <= 55110.
 constexpr int64_t kMaxTetraVertices = 55111;
 
-// rowptr[u] = first edge of row u; keys sorted by (u << 32 | v).
-__global__ void k_csr_rows(const uint64_t* __restrict__ keys, int64_t E, int n,
-                           int64_t* __restrict__ rowptr, int* __restrict__ col) {
+constexpr int64_t kP1 = (int64_t)1 << 60, kP2 = (int64_t)2 << 60, kP3 = (int64_t)3 << 60;
+
+// key = larger vertex << 32 | smaller vertex
+__global__ void k_lower_keys(const int2* __restrict__ ij, uint64_t* __restrict__ keys, int64_t E) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) {
+        const int2 p = ij[e];
+        keys[e] = ((uint64_t)(uint32_t)max(p.x, p.y) << 32) | (uint32_t)min(p.x, p.y);
+    }
+}
+
+// rowptr[r] = first edge of row r, col = smaller vertex, row = larger vertex.
+__global__ void k_lower_csr(const uint64_t* __restrict__ keys, int64_t E, int n, int64_t* __restrict__ rowptr,
+                            int* __restrict__ col, int* __restrict__ row) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e > E) return;
-    const int u = e < E ? (int)(keys[e] >> 32) : n;
-    const int pu = e > 0 ? (int)(keys[e - 1] >> 32) : -1;
-    for (int r = pu + 1; r <= u; ++r) rowptr[r] = e;
-    if (e < E) col[e] = (int)(keys[e] & 0xffffffffu);
+    const int r = e < E ? (int)(keys[e] >> 32) : n;
+    const int pr = e > 0 ? (int)(keys[e - 1] >> 32) : -1;
+    for (int x = pr + 1; x <= r; ++x) rowptr[x] = e;
+    if (e < E) {
+        col[e] = (int)(keys[e] & 0xffffffffu);
+        row[e] = r;
+    }
 }
 
-__global__ void k_edge_rows(const uint64_t* __restrict__ keys, int64_t E, int* __restrict__ row) {
+// Weights are non-negative, so their bit patterns order like the values.
+__global__ void k_weight_keys(const float* __restrict__ w, int64_t E, uint32_t* __restrict__ bits,
+                              uint32_t* __restrict__ perm) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < E) row[e] = (int)(keys[e] >> 32);
+    if (e < E) {
+        bits[e] = __float_as_uint(w[e]);
+        perm[e] = (uint32_t)e;
+    }
 }
 
-// Triangles (u, v, w), u < v < w, through edge e = (u, v): the common upper
-// neighbours w > v of u and v.  FILL = false counts, FILL = true writes.
+__global__ void k_group_flags(const uint32_t* __restrict__ bits, int64_t E, int64_t* __restrict__ flag) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < E) flag[k] = (k == 0 || bits[k] != bits[k - 1]) ? 1 : 0;
+}
+
+// Dense value rank of every edge: group 0 is value 0 (the vertices, and the
+// edges of weight 0 if any); vrS in sorted order, vrE by colex edge index,
+// gval[group] = the group's value.
+__global__ void k_edge_groups(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ perm,
+                              const int64_t* __restrict__ flag, const int64_t* __restrict__ excl, int64_t E,
+                              uint32_t* __restrict__ vrS, uint32_t* __restrict__ vrE, float* __restrict__ gval) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    const int64_t shift = bits[0] != 0u ? 1 : 0;
+    const uint32_t g = (uint32_t)(excl[k] + flag[k] - 1 + shift);
+    vrS[k] = g;
+    vrE[perm[k]] = g;
+    if (flag[k]) gval[g] = __uint_as_float(bits[k]);
+    if (k == 0 && shift) gval[0] = 0.0f;
+}
+
+// Triangles (u, v, w), u < v < w, through lower edge e = (v, w): the common
+// smaller neighbours u < v of w and v.  FILL = false counts, FILL = true
+// writes group and id at offset[e].
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_triangles(
     const int* __restrict__ row, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
-    const float* __restrict__ wt, int64_t E, int64_t* __restrict__ count,
-    const int64_t* __restrict__ offset, float* __restrict__ tval, int64_t* __restrict__ tid) {
+    const uint32_t* __restrict__ vrE, int64_t E, int64_t* __restrict__ count, const int64_t* __restrict__ offset,
+    uint32_t* __restrict__ tg, int64_t* __restrict__ tid) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const int u = row[e], v = col[e];
-    const float wuv = wt[e];
-    int64_t i1 = e + 1, e1 = rowptr[u + 1];
-    int64_t i2 = rowptr[v], e2 = rowptr[v + 1];
+    const int w = row[e], v = col[e];
+    int64_t i1 = rowptr[w], i2 = rowptr[v];
+    const int64_t e1 = e, e2 = rowptr[v + 1];
     int64_t c = 0, out = FILL ? offset[e] : 0;
-    const int64_t base = (int64_t)u + binom2(v) + ((int64_t)2 << 60);
+    const uint32_t gvw = FILL ? vrE[e] : 0u;
+    const int64_t base = binom2(v) + binom3(w) + kP2;
     while (i1 < e1 && i2 < e2) {
         const int a = col[i1], b = col[i2];
         if (a == b) {
             if (FILL) {
-                tval[out] = fmaxf(wuv, fmaxf(wt[i1], wt[i2]));
-                tid[out] = base + binom3(a);
+                tg[out] = max(gvw, max(vrE[i1], vrE[i2]));
+                tid[out] = base + a;
                 ++out;
             } else {
                 ++c;
@@ -96,24 +151,23 @@ __global__ __launch_bounds__(256) void k_triangles(
     if (!FILL) count[e] = c;
 }
 
-// Tetrahedra (u, v, w, x), u < v < w < x, through edge e = (u, v): for each
-// common upper neighbour w of u and v, the common upper neighbours x > w of
-// u, v and w (a three-way merge of the sorted rows).  Value = the largest of
-// the six edge weights (the max over facets, include/point.hpp:1194-1208).
+// Tetrahedra (u, v, w, x), u < v < w < x, through lower edge e = (w, x): for
+// each common smaller neighbour v of x and w (ascending), the common smaller
+// neighbours u < v of x, w and v (ascending).
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_tetrahedra(
     const int* __restrict__ row, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
-    const float* __restrict__ wt, int64_t E, int64_t* __restrict__ count,
-    const int64_t* __restrict__ offset, float* __restrict__ tval, int64_t* __restrict__ tid) {
+    const uint32_t* __restrict__ vrE, int64_t E, int64_t* __restrict__ count, const int64_t* __restrict__ offset,
+    uint32_t* __restrict__ qg, int64_t* __restrict__ qid) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    const int u = row[e], v = col[e];
-    const float wuv = wt[e];
-    const int64_t e1 = rowptr[u + 1], e2 = rowptr[v + 1];
-    int64_t i1 = e + 1, i2 = rowptr[v];
+    const int x = row[e], w = col[e];
+    const int64_t bx = rowptr[x], bw = rowptr[w], ew = rowptr[w + 1];
+    int64_t i1 = bx, i2 = bw;
     int64_t c = 0, out = FILL ? offset[e] : 0;
-    const int64_t base = (int64_t)u + binom2(v) + ((int64_t)3 << 60);
-    while (i1 < e1 && i2 < e2) {
+    const uint32_t gwx = FILL ? vrE[e] : 0u;
+    const int64_t base = binom3(w) + binom4(x) + kP3;
+    while (i1 < e && i2 < ew) {
         const int a = col[i1], b = col[i2];
         if (a < b) {
             ++i1;
@@ -123,17 +177,17 @@ __global__ __launch_bounds__(256) void k_tetrahedra(
             ++i2;
             continue;
         }
-        const int w = a;
-        const float m3 = fmaxf(wuv, fmaxf(wt[i1], wt[i2]));
-        int64_t j1 = i1 + 1, j2 = i2 + 1, j3 = rowptr[w];
-        const int64_t e3 = rowptr[w + 1];
-        while (j1 < e1 && j2 < e2 && j3 < e3) {
+        const int v = a;
+        const uint32_t g3 = FILL ? max(gwx, max(vrE[i1], vrE[i2])) : 0u;
+        int64_t j1 = bx, j2 = bw, j3 = rowptr[v];
+        const int64_t e3 = rowptr[v + 1];
+        while (j1 < i1 && j2 < i2 && j3 < e3) {
             const int x1 = col[j1], x2 = col[j2], x3 = col[j3];
             const int mn = min(x1, min(x2, x3));
             if (x1 == x2 && x2 == x3) {
                 if (FILL) {
-                    tval[out] = fmaxf(m3, fmaxf(wt[j1], fmaxf(wt[j2], wt[j3])));
-                    tid[out] = base + binom3(w) + binom4(x1);
+                    qg[out] = max(g3, max(vrE[j1], max(vrE[j2], vrE[j3])));
+                    qid[out] = base + binom2(v) + x1;
                     ++out;
                 } else {
                     ++c;
@@ -149,27 +203,78 @@ __global__ __launch_bounds__(256) void k_tetrahedra(
     if (!FILL) count[e] = c;
 }
 
-__global__ void k_fill_vertices(float* __restrict__ val, int64_t* __restrict__ id, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// gs[g] = first position of a group-sorted array with group >= g, g = 0..G.
+__global__ void k_group_starts(const uint32_t* __restrict__ grp, int64_t L, uint32_t G, int64_t* __restrict__ gs) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > L) return;
+    const int64_t cur = k < L ? (int64_t)grp[k] : (int64_t)G;
+    const int64_t prev = k > 0 ? (int64_t)grp[k - 1] : -1;
+    for (int64_t g = prev + 1; g <= cur; ++g) gs[g] = k;
+}
+
+// Simplices of group g: vertices (group 0 only), then edges, triangles and
+// tetrahedra of the group.
+__global__ void k_group_sizes(const int64_t* __restrict__ gsE, const int64_t* __restrict__ gsT,
+                              const int64_t* __restrict__ gsQ, int64_t n, uint32_t G, int64_t* __restrict__ size) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    int64_t s = g == 0 ? n : 0;
+    if (gsE) s += gsE[g + 1] - gsE[g];
+    if (gsT) s += gsT[g + 1] - gsT[g];
+    if (gsQ) s += gsQ[g + 1] - gsQ[g];
+    size[g] = s;
+}
+
+__global__ void k_place_vertices(float* __restrict__ val, int64_t* __restrict__ id, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         val[i] = 0.0f;
         id[i] = i;
     }
 }
 
-__global__ void k_fill_edges(const int* __restrict__ row, const int* __restrict__ col,
-                             const float* __restrict__ wt, int64_t E, float* __restrict__ val,
-                             int64_t* __restrict__ id) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < E) {
-        val[e] = wt[e];
-        id[e] = (int64_t)row[e] + binom2(col[e]) + ((int64_t)1 << 60);
-    }
+__global__ void k_place_edges(const uint32_t* __restrict__ vrS, const uint32_t* __restrict__ perm,
+                              const int* __restrict__ row, const int* __restrict__ col, int64_t E,
+                              const int64_t* __restrict__ gsE, const int64_t* __restrict__ goff,
+                              const float* __restrict__ gval, int64_t n, float* __restrict__ val,
+                              int64_t* __restrict__ id) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= E) return;
+    const uint32_t g = vrS[k];
+    const int64_t pos = goff[g] + (g == 0 ? n : 0) + (k - gsE[g]);
+    const uint32_t e = perm[k];
+    val[pos] = gval[g];
+    id[pos] = (int64_t)col[e] + binom2(row[e]) + kP1;
 }
 
-__global__ void k_pack_ij(const int2* __restrict__ ij, uint64_t* __restrict__ keys, int64_t n) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) keys[e] = ((uint64_t)(uint32_t)ij[e].x << 32) | (uint32_t)ij[e].y;
+// Triangles (T = gsQ null) or tetrahedra of a group-sorted array.
+__global__ void k_place_cliques(const uint32_t* __restrict__ grp, const int64_t* __restrict__ cid, int64_t L,
+                                const int64_t* __restrict__ gs, const int64_t* __restrict__ gsE,
+                                const int64_t* __restrict__ gsT, const int64_t* __restrict__ goff,
+                                const float* __restrict__ gval, int64_t n, float* __restrict__ val,
+                                int64_t* __restrict__ id) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L) return;
+    const uint32_t g = grp[k];
+    int64_t pos = goff[g] + (g == 0 ? n : 0) + (gsE[g + 1] - gsE[g]) + (k - gs[g]);
+    if (gsT) pos += gsT[g + 1] - gsT[g];
+    val[pos] = gval[g];
+    id[pos] = cid[k];
+}
+
+unsigned bits_for(uint64_t v) {   // smallest b with v < 2^b (at least 1)
+    unsigned b = 1;
+    while (b < 64 && (1ull << b) <= v) ++b;
+    return b;
+}
+
+int last_total(const int64_t* cnt, const int64_t* off, int64_t L, int64_t* total) {
+    int64_t lo = 0, lc = 0;
+    RNNG_HIP_TRY(hipMemcpyAsync(&lo, off + (L - 1), sizeof(lo), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipMemcpyAsync(&lc, cnt + (L - 1), sizeof(lc), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    *total = lo + lc;
+    return RNNG_OK;
 }
 
 int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_s* f,
@@ -200,88 +305,173 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         }
     } release_g{g};
     int64_t E = 0;
+    rnng_stats gst{};
     if (maxdim >= 1 && n >= 2) {
-        const int mode = ds->d >= 16 ? RNNG_MODE_MFMA : RNNG_MODE_PLAIN;
-        RNNG_TRY(l2_graph_build(ds, ds, true, eps, mode, &g, st));
+        const int mode = ds->d <= 3 ? RNNG_MODE_GRID : ds->d >= 16 ? RNNG_MODE_MFMA : RNNG_MODE_PLAIN;
+        RNNG_TRY(l2_graph_build(ds, ds, true, eps, mode, &g, &gst));
         E = g.n_edges;
     }
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
-    LocalBuf keys, rowptr, col, row, cnt, off;
-    RNNG_TRY(keys.reserve(std::max<size_t>(1, (size_t)E) * sizeof(uint64_t)));
+    const int64_t E1 = std::max<int64_t>(E, 1);
+
+    // 2. lower CSR in colex edge order; weights follow their edges
+    LocalBuf keys, rowptr, col, row, wc;
     RNNG_TRY(rowptr.reserve(((size_t)n + 1) * sizeof(int64_t)));
-    RNNG_TRY(col.reserve(std::max<size_t>(1, (size_t)E) * sizeof(int)));
-    RNNG_TRY(row.reserve(std::max<size_t>(1, (size_t)E) * sizeof(int)));
+    RNNG_TRY(col.reserve((size_t)E1 * sizeof(int)));
+    RNNG_TRY(row.reserve((size_t)E1 * sizeof(int)));
     if (E > 0) {
-        k_pack_ij<<<blocks_for(E, 256), 256, 0, stream()>>>(g.ij.as<int2>(), keys.as<uint64_t>(), E);
+        RNNG_TRY(keys.reserve((size_t)E * sizeof(uint64_t)));
+        k_lower_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(g.ij.as<int2>(), keys.as<uint64_t>(), E);
         RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(sort_pairs_u64_f32(keys.as<uint64_t>(), g.w.as<float>(), E));
+        RNNG_TRY(sort_pairs_u64_f32_bits(keys.as<uint64_t>(), g.w.as<float>(), E, 32 + bits_for((uint64_t)n)));
+        k_lower_csr<<<blocks_for(E + 1, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), E, n, rowptr.as<int64_t>(),
+                                                                 col.as<int>(), row.as<int>());
+        RNNG_HIP_TRY(hipGetLastError());
+        keys.release();
     }
-    k_csr_rows<<<blocks_for(E + 1, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), E, n,
-                                                               rowptr.as<int64_t>(), col.as<int>());
-    k_edge_rows<<<blocks_for(std::max<int64_t>(E, 1), 256), 256, 0, stream()>>>(keys.as<uint64_t>(), E,
-                                                                                row.as<int>());
-    RNNG_HIP_TRY(hipGetLastError());
-    int64_t T = 0;
+
+    // 3. value groups
+    LocalBuf bits, perm, flag, excl, vrS, vrE, gval;
+    uint32_t G = 1;
+    RNNG_TRY(gval.reserve((size_t)(E + 1) * sizeof(float)));
+    if (E > 0) {
+        RNNG_TRY(bits.reserve((size_t)E * sizeof(uint32_t)));
+        RNNG_TRY(perm.reserve((size_t)E * sizeof(uint32_t)));
+        RNNG_TRY(flag.reserve((size_t)E * sizeof(int64_t)));
+        RNNG_TRY(excl.reserve((size_t)E * sizeof(int64_t)));
+        RNNG_TRY(vrS.reserve((size_t)E * sizeof(uint32_t)));
+        RNNG_TRY(vrE.reserve((size_t)E * sizeof(uint32_t)));
+        k_weight_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(g.w.as<float>(), E, bits.as<uint32_t>(),
+                                                                perm.as<uint32_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(sort_pairs_u32_u32(bits.as<uint32_t>(), perm.as<uint32_t>(), E, 32));
+        k_group_flags<<<blocks_for(E, 256), 256, 0, stream()>>>(bits.as<uint32_t>(), E, flag.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(exclusive_scan_i64(flag.as<int64_t>(), excl.as<int64_t>(), E));
+        k_edge_groups<<<blocks_for(E, 256), 256, 0, stream()>>>(bits.as<uint32_t>(), perm.as<uint32_t>(),
+                                                                flag.as<int64_t>(), excl.as<int64_t>(), E,
+                                                                vrS.as<uint32_t>(), vrE.as<uint32_t>(),
+                                                                gval.as<float>());
+        RNNG_HIP_TRY(hipGetLastError());
+        int64_t groups = 0;
+        RNNG_TRY(last_total(flag.as<int64_t>(), excl.as<int64_t>(), E, &groups));
+        uint32_t b0 = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(&b0, bits.ptr, sizeof(b0), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        G = (uint32_t)(groups + (b0 != 0u ? 1 : 0));
+        flag.release();
+        excl.release();
+    }
+    const unsigned gbits = bits_for(G);
+
+    // 4. cliques in colex (= id) order
+    LocalBuf cntT, offT, tg, tid, cntQ, offQ, qg, qid;
+    int64_t T = 0, Q = 0;
     if (maxdim >= 2 && E > 0) {
-        RNNG_TRY(cnt.reserve((size_t)E * sizeof(int64_t)));
-        RNNG_TRY(off.reserve(((size_t)E + 1) * sizeof(int64_t)));
-        k_triangles<false><<<blocks_for(E, 256), 256, 0, stream()>>>(
-            row.as<int>(), rowptr.as<int64_t>(), col.as<int>(), g.w.as<float>(), E, cnt.as<int64_t>(),
-            nullptr, nullptr, nullptr);
+        RNNG_TRY(cntT.reserve((size_t)E * sizeof(int64_t)));
+        RNNG_TRY(offT.reserve((size_t)E * sizeof(int64_t)));
+        k_triangles<false><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
+                                                                     col.as<int>(), nullptr, E, cntT.as<int64_t>(),
+                                                                     nullptr, nullptr, nullptr);
         RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(exclusive_scan_i64(cnt.as<int64_t>(), off.as<int64_t>(), E));
-        int64_t last_off = 0, last_cnt = 0;
-        RNNG_HIP_TRY(hipMemcpy(&last_off, off.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
-        RNNG_HIP_TRY(hipMemcpy(&last_cnt, cnt.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
-        T = last_off + last_cnt;
+        RNNG_TRY(exclusive_scan_i64(cntT.as<int64_t>(), offT.as<int64_t>(), E));
+        RNNG_TRY(last_total(cntT.as<int64_t>(), offT.as<int64_t>(), E, &T));
     }
-    // clique counts by the per-edge count / scan / fill of kernel K
-    auto count_cliques = [&](auto kernel, LocalBuf& cbuf, LocalBuf& obuf, int64_t& total_out) -> int {
-        RNNG_TRY(cbuf.reserve((size_t)E * sizeof(int64_t)));
-        RNNG_TRY(obuf.reserve(((size_t)E + 1) * sizeof(int64_t)));
-        kernel<<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(), col.as<int>(),
-                                                         g.w.as<float>(), E, cbuf.as<int64_t>(), nullptr, nullptr,
-                                                         nullptr);
+    if (maxdim >= 3 && E > 0) {
+        RNNG_TRY(cntQ.reserve((size_t)E * sizeof(int64_t)));
+        RNNG_TRY(offQ.reserve((size_t)E * sizeof(int64_t)));
+        k_tetrahedra<false><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
+                                                                      col.as<int>(), nullptr, E, cntQ.as<int64_t>(),
+                                                                      nullptr, nullptr, nullptr);
         RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(exclusive_scan_i64(cbuf.as<int64_t>(), obuf.as<int64_t>(), E));
-        int64_t lo = 0, lc = 0;
-        RNNG_HIP_TRY(hipMemcpy(&lo, obuf.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
-        RNNG_HIP_TRY(hipMemcpy(&lc, cbuf.as<int64_t>() + (E - 1), sizeof(int64_t), hipMemcpyDeviceToHost));
-        total_out = lo + lc;
-        return RNNG_OK;
-    };
-    int64_t Q = 0;
-    LocalBuf cnt4, off4;
-    if (maxdim >= 3 && E > 0) RNNG_TRY(count_cliques(k_tetrahedra<false>, cnt4, off4, Q));
-    const int64_t total = (int64_t)n + (maxdim >= 1 ? E : 0) + T + Q;
+        RNNG_TRY(exclusive_scan_i64(cntQ.as<int64_t>(), offQ.as<int64_t>(), E));
+        RNNG_TRY(last_total(cntQ.as<int64_t>(), offQ.as<int64_t>(), E, &Q));
+    }
+    if (T > 0) {
+        RNNG_TRY(tg.reserve((size_t)T * sizeof(uint32_t)));
+        RNNG_TRY(tid.reserve((size_t)T * sizeof(int64_t)));
+        k_triangles<true><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
+                                                                    col.as<int>(), vrE.as<uint32_t>(), E, nullptr,
+                                                                    offT.as<int64_t>(), tg.as<uint32_t>(),
+                                                                    tid.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    if (Q > 0) {
+        RNNG_TRY(qg.reserve((size_t)Q * sizeof(uint32_t)));
+        RNNG_TRY(qid.reserve((size_t)Q * sizeof(int64_t)));
+        k_tetrahedra<true><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
+                                                                     col.as<int>(), vrE.as<uint32_t>(), E, nullptr,
+                                                                     offQ.as<int64_t>(), qg.as<uint32_t>(),
+                                                                     qid.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    cntT.release();
+    offT.release();
+    cntQ.release();
+    offQ.release();
+    RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
+
+    // 5. (value, id) order: stable group sorts, per-group counts, placement
+    if (T > 0) RNNG_TRY(sort_pairs_u32_i64(tg.as<uint32_t>(), tid.as<int64_t>(), T, gbits));
+    if (Q > 0) RNNG_TRY(sort_pairs_u32_i64(qg.as<uint32_t>(), qid.as<int64_t>(), Q, gbits));
+    const int64_t Ecount = maxdim >= 1 ? E : 0;
+    const int64_t total = (int64_t)n + Ecount + T + Q;
     RNNG_TRY(f->value.reserve(std::max<size_t>(1, (size_t)total) * sizeof(float)));
     RNNG_TRY(f->id.reserve(std::max<size_t>(1, (size_t)total) * sizeof(int64_t)));
     float* val = f->value.as<float>();
     int64_t* id = f->id.as<int64_t>();
-    if (n > 0) k_fill_vertices<<<blocks_for(n, 256), 256, 0, stream()>>>(val, id, n);
-    if (E > 0)
-        k_fill_edges<<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), col.as<int>(), g.w.as<float>(),
-                                                                E, val + n, id + n);
-    if (T > 0)
-        k_triangles<true><<<blocks_for(E, 256), 256, 0, stream()>>>(
-            row.as<int>(), rowptr.as<int64_t>(), col.as<int>(), g.w.as<float>(), E, nullptr,
-            off.as<int64_t>(), val + n + E, id + n + E);
-    if (Q > 0)
-        k_tetrahedra<true><<<blocks_for(E, 256), 256, 0, stream()>>>(
-            row.as<int>(), rowptr.as<int64_t>(), col.as<int>(), g.w.as<float>(), E, nullptr,
-            off4.as<int64_t>(), val + n + E + T, id + n + E + T);
-    RNNG_HIP_TRY(hipGetLastError());
-    RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-    RNNG_TRY(sort_by_value_then_id(val, id, total));
+    if (n > 0) k_place_vertices<<<blocks_for(n, 256), 256, 0, stream()>>>(val, id, n);
+    if (E > 0) {
+        LocalBuf gsE, gsT, gsQ, gsize, goff;
+        RNNG_TRY(gsE.reserve(((size_t)G + 1) * sizeof(int64_t)));
+        RNNG_TRY(gsize.reserve((size_t)G * sizeof(int64_t)));
+        RNNG_TRY(goff.reserve((size_t)G * sizeof(int64_t)));
+        k_group_starts<<<blocks_for(E + 1, 256), 256, 0, stream()>>>(vrS.as<uint32_t>(), E, G, gsE.as<int64_t>());
+        if (T > 0) {
+            RNNG_TRY(gsT.reserve(((size_t)G + 1) * sizeof(int64_t)));
+            k_group_starts<<<blocks_for(T + 1, 256), 256, 0, stream()>>>(tg.as<uint32_t>(), T, G,
+                                                                       gsT.as<int64_t>());
+        }
+        if (Q > 0) {
+            RNNG_TRY(gsQ.reserve(((size_t)G + 1) * sizeof(int64_t)));
+            k_group_starts<<<blocks_for(Q + 1, 256), 256, 0, stream()>>>(qg.as<uint32_t>(), Q, G,
+                                                                       gsQ.as<int64_t>());
+        }
+        k_group_sizes<<<blocks_for(G, 256), 256, 0, stream()>>>(gsE.as<int64_t>(), T > 0 ? gsT.as<int64_t>() : nullptr,
+                                                               Q > 0 ? gsQ.as<int64_t>() : nullptr, n, G,
+                                                               gsize.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(exclusive_scan_i64(gsize.as<int64_t>(), goff.as<int64_t>(), G));
+        k_place_edges<<<blocks_for(E, 256), 256, 0, stream()>>>(vrS.as<uint32_t>(), perm.as<uint32_t>(), row.as<int>(),
+                                                               col.as<int>(), E, gsE.as<int64_t>(),
+                                                               goff.as<int64_t>(), gval.as<float>(), n, val, id);
+        if (T > 0)
+            k_place_cliques<<<blocks_for(T, 256), 256, 0, stream()>>>(
+                tg.as<uint32_t>(), tid.as<int64_t>(), T, gsT.as<int64_t>(), gsE.as<int64_t>(), nullptr,
+                goff.as<int64_t>(), gval.as<float>(), n, val, id);
+        if (Q > 0)
+            k_place_cliques<<<blocks_for(Q, 256), 256, 0, stream()>>>(
+                qg.as<uint32_t>(), qid.as<int64_t>(), Q, gsQ.as<int64_t>(), gsE.as<int64_t>(),
+                T > 0 ? gsT.as<int64_t>() : nullptr, goff.as<int64_t>(), gval.as<float>(), n, val, id);
+        RNNG_HIP_TRY(hipGetLastError());
+    }
     RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     f->n = total;
     f->counts[0] = n;
-    f->counts[1] = maxdim >= 1 ? E : 0;
+    f->counts[1] = Ecount;
     f->counts[2] = T;
     f->counts[3] = Q;
     if (st) {
-        st->recheck_ms += ev.ms(1, 2);    // CSR + clique stage
+        st->pair_evals += gst.pair_evals;
+        st->candidates += gst.candidates;
+        st->edges += gst.edges;
+        st->band_pairs += gst.band_pairs;
+        st->band_flips += gst.band_flips;
+        st->refills += gst.refills;
+        st->filter_ms += ev.ms(0, 1);     // epsilon graph
+        st->recheck_ms += ev.ms(1, 2);    // CSR, value groups, cliques
+        st->prep_ms += ev.ms(2, 3);       // (value, id) order
         st->total_ms += ev.ms(0, 3);
     }
     return RNNG_OK;

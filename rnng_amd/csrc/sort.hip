@@ -63,6 +63,22 @@ int sort_pairs_u64_i64(uint64_t* keys, int64_t* vals, int64_t n) {
     return sort_pairs_impl<uint64_t, int64_t>(keys, vals, n, 64);
 }
 
+int sort_pairs_u32_i32(uint32_t* keys, int32_t* vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_impl<uint32_t, int32_t>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u32_i64(uint32_t* keys, int64_t* vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_impl<uint32_t, int64_t>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u32_u32(uint32_t* keys, uint32_t* vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_impl<uint32_t, uint32_t>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u64_f32_bits(uint64_t* keys, float* vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_impl<uint64_t, float>(keys, vals, n, end_bit);
+}
+
 // Sorts (value, id) lexicographically: stable LSD passes by id, then by value.
 int sort_by_value_then_id(float* value, int64_t* id, int64_t n) {
     if (n <= 1) return RNNG_OK;

@@ -12,7 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import MODE_MFMA, MODE_PLAIN, Stats, check
+from ._lib import MODE_GRID, MODE_MFMA, MODE_PLAIN, Stats, check
 
 
 def _f32(a) -> np.ndarray:
@@ -24,7 +24,7 @@ def _f32(a) -> np.ndarray:
 
 def _mode(mode) -> int:
     if isinstance(mode, str):
-        return {"plain": MODE_PLAIN, "mfma": MODE_MFMA}[mode]
+        return {"plain": MODE_PLAIN, "mfma": MODE_MFMA, "grid": MODE_GRID}[mode]
     return int(mode)
 
 

@@ -16,27 +16,51 @@ from ._lib import Stats, check
 from .l2 import Dataset
 
 
+class Filtration:
+    """Device-resident filtration (rnng_filtration_t) of `ds` at eps."""
+
+    def __init__(self, ds: Dataset, eps: float, maxdim: int = 2, stats: Stats | None = None):
+        self._h = C.c_void_p()
+        st = stats if stats is not None else Stats()
+        check(_lib.lib().rnng_rips_build(ds.handle, np.float32(eps), int(maxdim), C.byref(self._h),
+                                         C.byref(st)))
+
+    def size(self):
+        """(total simplices, per-dimension counts [4])."""
+        n = C.c_int64()
+        per = np.zeros(4, np.int64)
+        check(_lib.lib().rnng_filtration_size(self._h, C.byref(n), per.ctypes.data_as(C.c_void_p)))
+        return n.value, per
+
+    def copy(self):
+        """(value float32 [S], id int64 [S]) in filtration order."""
+        n, _ = self.size()
+        val = np.empty(n, np.float32)
+        ids = np.empty(n, np.int64)
+        check(_lib.lib().rnng_filtration_copy(self._h, val.ctypes.data_as(C.c_void_p),
+                                              ids.ctypes.data_as(C.c_void_p), n))
+        return val, ids
+
+    def close(self) -> None:
+        if self._h:
+            _lib.lib().rnng_filtration_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def rips_filtration(points, eps: float, maxdim: int = 2, stats: Stats | None = None):
     ds = points if isinstance(points, Dataset) else Dataset(points)
-    st = stats if stats is not None else Stats()
-    h = C.c_void_p()
-    L = _lib.lib()
     try:
-        check(L.rnng_rips_build(ds.handle, np.float32(eps), int(maxdim), C.byref(h), C.byref(st)))
-        try:
-            n = C.c_int64()
-            per = np.zeros(4, np.int64)
-            check(L.rnng_filtration_size(h, C.byref(n), per.ctypes.data_as(C.c_void_p)))
-            val = np.empty(n.value, np.float32)
-            ids = np.empty(n.value, np.int64)
-            check(L.rnng_filtration_copy(h, val.ctypes.data_as(C.c_void_p),
-                                         ids.ctypes.data_as(C.c_void_p), n.value))
-        finally:
-            L.rnng_filtration_destroy(h)
+        with Filtration(ds, eps, maxdim, stats) as f:
+            return f.copy()
     finally:
         if ds is not points:
             ds.close()
-    return val, ids
 
 
 def simplex_dims(ids: np.ndarray) -> np.ndarray:

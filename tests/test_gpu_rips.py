@@ -50,6 +50,24 @@ def test_rips_tetrahedra_bit_exact(orc, n, d, eps):
     assert (_rips().simplex_dims(i) == 3).sum() > 0
 
 
+@pytest.mark.parametrize("maxdim", [1, 2, 3])
+def test_rips_value_ties_bit_exact(orc, maxdim):
+    """A 5x5x5 lattice with some points duplicated: many simplices share a
+    value (1, sqrt 2), edges of weight 0 share the vertices' value group, and
+    lattice distances sit exactly at eps = 1 -- the order inside groups of
+    equal value is (dimension, id)."""
+    g = np.stack(np.meshgrid(*[np.arange(5, dtype=np.float32)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pts = np.concatenate([g, g[::7]]).astype(np.float32)
+    pts = pts[np.random.default_rng(3).permutation(pts.shape[0])]
+    for eps in (1.0, 1.5):
+        flush_print(f"rips lattice ties eps={eps} maxdim={maxdim}")
+        v_ref, i_ref = orc.rips_filtration(pts, eps, maxdim=maxdim)
+        v, i = _rips().rips_filtration(pts, eps, maxdim)
+        assert v.tobytes() == v_ref.tobytes()
+        assert np.array_equal(i, i_ref)
+        assert ((v == 0) & (_rips().simplex_dims(i) == 1)).sum() >= 18
+
+
 def test_rips_clique_counts():
     pts = (np.random.default_rng(1).standard_normal((30, 3)) * 1e-3).astype(np.float32)
     v, i = _rips().rips_filtration(pts, 1.0, 3)
