@@ -34,7 +34,9 @@ void eps_join(const PointContainer<Atom>& a, const PointContainer<Atom>& b, bool
               std::vector<int64_t>& ij, std::vector<float>& w) {
     GraphOwner g;
     if constexpr (std::is_same<Atom, float>::value) {
-        if (self) check(rnng_graph_build(a.device_points(), radius, RNNG_MODE_MFMA, &g.h, nullptr), "graph_build");
+        // d <= 3: uniform grid (same edges, pairs of adjacent cells only)
+        const int32_t mode = a.num_points() > 0 && a[0].size() <= 3 ? RNNG_MODE_GRID : RNNG_MODE_MFMA;
+        if (self) check(rnng_graph_build(a.device_points(), radius, mode, &g.h, nullptr), "graph_build");
         else
             check(rnng_graph_build_bipartite(a.device_points(), b.device_points(), radius, RNNG_MODE_MFMA, &g.h,
                                              nullptr),

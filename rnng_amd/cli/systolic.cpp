@@ -33,8 +33,10 @@ static int run() {
     t = wall();
     rnng_host::GraphOwner g;
     rnng_stats st{};
-    if constexpr (std::is_same<Atom, float>::value)
-        rnng_host::check(rnng_graph_build(points.device_points(), radius, RNNG_MODE_MFMA, &g.h, &st), "graph_build");
+    if constexpr (std::is_same<Atom, float>::value) {
+        const int32_t mode = n > 0 && points[0].size() <= 3 ? RNNG_MODE_GRID : RNNG_MODE_MFMA;
+        rnng_host::check(rnng_graph_build(points.device_points(), radius, mode, &g.h, &st), "graph_build");
+    }
     else
         rnng_host::check(rnng_edit_graph_build(points.device_strings(), radius, &g.h, &st), "edit_graph_build");
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] found neighbors\n", wall() - t);

@@ -399,7 +399,8 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
         if (rc != RNNG_OK) return;
         rc = b.reserve(std::max<size_t>(bytes, 16));
         if (rc == RNNG_OK && bytes) {
-            hipError_t e = hipMemcpy(b.ptr, src, bytes, hipMemcpyHostToDevice);
+            hipError_t e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, stream());
+            if (e == hipSuccess) e = hipStreamSynchronize(stream());
             if (e != hipSuccess) {
                 set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
                 rc = RNNG_ERR_HIP;

@@ -70,7 +70,24 @@ __global__ __launch_bounds__(256) void k_grid_bbox(const float* __restrict__ X, 
         }
         fin += (unsigned)__shfl_down((int)fin, off, 64);
     }
+    // the block's four waves meet in LDS: one set of atomics per block
+    __shared__ unsigned part[4][7];
+    const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        for (int k = 0; k < 3; ++k) {
+            part[wv][k] = mn[k];
+            part[wv][3 + k] = mx[k];
+        }
+        part[wv][6] = fin;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; ++q)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = min(mn[k], part[q][k]);
+                mx[k] = max(mx[k], part[q][3 + k]);
+            }
+        for (int q = 1; q < 4; ++q) fin += part[q][6];
         for (int k = 0; k < d; ++k) {
             atomicMin(&stat[k], mn[k]);
             atomicMax(&stat[3 + k], mx[k]);
@@ -234,7 +251,7 @@ int grid_graph_build(const rnng_dataset_s* A, float eps, rnng_graph_s* g, rnng_s
     LocalBuf stat;
     RNNG_TRY(stat.reserve(8 * sizeof(unsigned)));
     k_grid_clear<<<1, 64, 0, stream()>>>(stat.as<unsigned>());
-    k_grid_bbox<<<std::min<unsigned>(blocks_for(n, 256), 1024), 256, 0, stream()>>>(A->X, n, d, stat.as<unsigned>());
+    k_grid_bbox<<<std::min<unsigned>(blocks_for(n, 256), 256), 256, 0, stream()>>>(A->X, n, d, stat.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
     unsigned hs[8];
     RNNG_HIP_TRY(hipMemcpyAsync(hs, stat.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));

@@ -54,6 +54,11 @@ int sort_pairs_u32_i32(uint32_t* keys, int32_t* vals, int64_t n, unsigned end_bi
 int sort_pairs_u32_i64(uint32_t* keys, int64_t* vals, int64_t n, unsigned end_bit);
 int sort_pairs_u32_u32(uint32_t* keys, uint32_t* vals, int64_t n, unsigned end_bit);
 int sort_pairs_u64_f32_bits(uint64_t* keys, float* vals, int64_t n, unsigned end_bit);
+// The same, leaving the sorted data in whichever of two buffers the sort
+// ended in: keys and vals then own it (no copy back).
+int sort_pairs_u32_i64_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit);
+int sort_pairs_u32_u32_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit);
+int sort_pairs_u64_f32_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit);
 int exclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n);
 // Stable compaction of (ij, w) by flag != 0 into (out_ij, out_w); *n_out gets
 // the count.  The outputs must hold n elements.

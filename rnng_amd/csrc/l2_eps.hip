@@ -1613,8 +1613,9 @@ int rnng_graph_copy(rnng_graph_t g, int64_t* ij, float* w, int64_t cap, int32_t 
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     }
     std::vector<int2> hij((size_t)ne);
-    RNNG_HIP_TRY(hipMemcpy(hij.data(), g->ij.ptr, (size_t)ne * sizeof(int2), hipMemcpyDeviceToHost));
-    if (w) RNNG_HIP_TRY(hipMemcpy(w, g->w.ptr, (size_t)ne * sizeof(float), hipMemcpyDeviceToHost));
+    RNNG_HIP_TRY(hipMemcpyAsync(hij.data(), g->ij.ptr, (size_t)ne * sizeof(int2), hipMemcpyDeviceToHost, stream()));
+    if (w) RNNG_HIP_TRY(hipMemcpyAsync(w, g->w.ptr, (size_t)ne * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     if (ij)
         for (int64_t e = 0; e < ne; ++e) {
             ij[2 * e] = hij[(size_t)e].x;
@@ -1718,11 +1719,12 @@ int rnng_debug_sqrt(const float* in, float* out, int64_t n) {
     LocalBuf a, b;
     RNNG_TRY(a.reserve((size_t)n * 4));
     RNNG_TRY(b.reserve((size_t)n * 4));
-    RNNG_HIP_TRY(hipMemcpy(a.ptr, in, (size_t)n * 4, hipMemcpyHostToDevice));
+    RNNG_HIP_TRY(hipMemcpyAsync(a.ptr, in, (size_t)n * 4, hipMemcpyHostToDevice, stream()));
     k_debug_sqrt<<<blocks_for(n, 256), 256, 0, stream()>>>(a.as<float>(), b.as<float>(), n);
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
-    RNNG_HIP_TRY(hipMemcpy(out, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost));
+    RNNG_HIP_TRY(hipMemcpyAsync(out, b.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }
 

@@ -100,55 +100,77 @@ __global__ void k_group_flags(const uint32_t* __restrict__ bits, int64_t E, int6
 
 // Dense value rank of every edge: group 0 is value 0 (the vertices, and the
 // edges of weight 0 if any); vrS in sorted order, vrE by colex edge index,
-// gval[group] = the group's value.
+// cv = group << 32 | smaller vertex by colex edge index, gval[group] = the
+// group's value.
 __global__ void k_edge_groups(const uint32_t* __restrict__ bits, const uint32_t* __restrict__ perm,
                               const int64_t* __restrict__ flag, const int64_t* __restrict__ excl, int64_t E,
-                              uint32_t* __restrict__ vrS, uint32_t* __restrict__ vrE, float* __restrict__ gval) {
+                              const int* __restrict__ col, uint32_t* __restrict__ vrS, uint32_t* __restrict__ vrE,
+                              uint64_t* __restrict__ cv, float* __restrict__ gval) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= E) return;
     const int64_t shift = bits[0] != 0u ? 1 : 0;
     const uint32_t g = (uint32_t)(excl[k] + flag[k] - 1 + shift);
+    const uint32_t e = perm[k];
     vrS[k] = g;
-    vrE[perm[k]] = g;
+    vrE[e] = g;
+    cv[e] = ((uint64_t)g << 32) | (uint32_t)col[e];
     if (flag[k]) gval[g] = __uint_as_float(bits[k]);
     if (k == 0 && shift) gval[0] = 0.0f;
 }
 
 // Triangles (u, v, w), u < v < w, through lower edge e = (v, w): the common
-// smaller neighbours u < v of w and v.  FILL = false counts, FILL = true
-// writes group and id at offset[e].
+// smaller neighbours u < v of w and v.  Eight lanes per edge: the shorter of
+// the two sorted lists (lower(w) below v, lower(v)) is read in rounds of
+// eight elements, each lane binary-searches its element in the other list,
+// and a round's matches are compacted in lane order, so each edge's
+// triangles come out ascending in u and the group's stores are contiguous.
+// cv packs (group << 32 | vertex), so a probe that matches also yields the
+// edge's value group.  FILL = false counts, FILL = true writes group and id at
+// offset[e].
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_triangles(
-    const int* __restrict__ row, const int64_t* __restrict__ rowptr, const int* __restrict__ col,
-    const uint32_t* __restrict__ vrE, int64_t E, int64_t* __restrict__ count, const int64_t* __restrict__ offset,
-    uint32_t* __restrict__ tg, int64_t* __restrict__ tid) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int w = row[e], v = col[e];
-    int64_t i1 = rowptr[w], i2 = rowptr[v];
-    const int64_t e1 = e, e2 = rowptr[v + 1];
-    int64_t c = 0, out = FILL ? offset[e] : 0;
-    const uint32_t gvw = FILL ? vrE[e] : 0u;
+    const int* __restrict__ row, const int64_t* __restrict__ rowptr, const uint64_t* __restrict__ cv, int64_t E,
+    int64_t* __restrict__ count, const int64_t* __restrict__ offset, uint32_t* __restrict__ tg,
+    int64_t* __restrict__ tid) {
+    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
+    const int j = threadIdx.x & 7;
+    const int gshift = threadIdx.x & 56;   // first lane of the group in the wave
+    if (e >= E) return;                    // the 8 lanes of a group leave together
+    const uint64_t ce = cv[e];
+    const int v = (int)(uint32_t)ce, w = row[e];
+    const uint32_t gvw = (uint32_t)(ce >> 32);
+    const int64_t a0 = rowptr[w], a1 = e, b0 = rowptr[v], b1 = rowptr[v + 1];
+    const bool sw = (a1 - a0) > (b1 - b0);
+    const int64_t s0 = sw ? b0 : a0, s1 = sw ? b1 : a1, l0 = sw ? a0 : b0, l1 = sw ? a1 : b1;
+    int64_t out = FILL ? offset[e] : 0;
     const int64_t base = binom2(v) + binom3(w) + kP2;
-    while (i1 < e1 && i2 < e2) {
-        const int a = col[i1], b = col[i2];
-        if (a == b) {
-            if (FILL) {
-                tg[out] = max(gvw, max(vrE[i1], vrE[i2]));
-                tid[out] = base + a;
-                ++out;
-            } else {
-                ++c;
+    for (int64_t r = s0; r < s1; r += 8) {
+        const int64_t k = r + j;
+        bool hit = false;
+        uint64_t se = 0, le = 0;
+        if (k < s1) {
+            se = cv[k];
+            const uint32_t key = (uint32_t)se;
+            int64_t lo = l0, hi = l1;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((uint32_t)cv[mid] < key) lo = mid + 1;
+                else hi = mid;
             }
-            ++i1;
-            ++i2;
-        } else if (a < b) {
-            ++i1;
-        } else {
-            ++i2;
+            if (lo < l1) {
+                le = cv[lo];
+                hit = (uint32_t)le == key;
+            }
         }
+        const uint32_t gm = (uint32_t)(__ballot(hit) >> gshift) & 0xffu;
+        if (FILL && hit) {
+            const int64_t pos = out + __popc(gm & ((1u << j) - 1u));
+            tg[pos] = max(gvw, max((uint32_t)(se >> 32), (uint32_t)(le >> 32)));
+            tid[pos] = base + (int64_t)(uint32_t)se;
+        }
+        out += __popc(gm);
     }
-    if (!FILL) count[e] = c;
+    if (!FILL && j == 0) count[e] = out;
 }
 
 // Tetrahedra (u, v, w, x), u < v < w < x, through lower edge e = (w, x): for
@@ -323,7 +345,7 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         RNNG_TRY(keys.reserve((size_t)E * sizeof(uint64_t)));
         k_lower_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(g.ij.as<int2>(), keys.as<uint64_t>(), E);
         RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(sort_pairs_u64_f32_bits(keys.as<uint64_t>(), g.w.as<float>(), E, 32 + bits_for((uint64_t)n)));
+        RNNG_TRY(sort_pairs_u64_f32_swap(keys, g.w, E, 32 + bits_for((uint64_t)n)));
         k_lower_csr<<<blocks_for(E + 1, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), E, n, rowptr.as<int64_t>(),
                                                                  col.as<int>(), row.as<int>());
         RNNG_HIP_TRY(hipGetLastError());
@@ -331,7 +353,7 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     }
 
     // 3. value groups
-    LocalBuf bits, perm, flag, excl, vrS, vrE, gval;
+    LocalBuf bits, perm, flag, excl, vrS, vrE, cv, gval;
     uint32_t G = 1;
     RNNG_TRY(gval.reserve((size_t)(E + 1) * sizeof(float)));
     if (E > 0) {
@@ -341,16 +363,18 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         RNNG_TRY(excl.reserve((size_t)E * sizeof(int64_t)));
         RNNG_TRY(vrS.reserve((size_t)E * sizeof(uint32_t)));
         RNNG_TRY(vrE.reserve((size_t)E * sizeof(uint32_t)));
+        RNNG_TRY(cv.reserve((size_t)E * sizeof(uint64_t)));
         k_weight_keys<<<blocks_for(E, 256), 256, 0, stream()>>>(g.w.as<float>(), E, bits.as<uint32_t>(),
                                                                 perm.as<uint32_t>());
         RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(sort_pairs_u32_u32(bits.as<uint32_t>(), perm.as<uint32_t>(), E, 32));
+        RNNG_TRY(sort_pairs_u32_u32_swap(bits, perm, E, 32));
         k_group_flags<<<blocks_for(E, 256), 256, 0, stream()>>>(bits.as<uint32_t>(), E, flag.as<int64_t>());
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(exclusive_scan_i64(flag.as<int64_t>(), excl.as<int64_t>(), E));
         k_edge_groups<<<blocks_for(E, 256), 256, 0, stream()>>>(bits.as<uint32_t>(), perm.as<uint32_t>(),
                                                                 flag.as<int64_t>(), excl.as<int64_t>(), E,
-                                                                vrS.as<uint32_t>(), vrE.as<uint32_t>(),
+                                                                col.as<int>(), vrS.as<uint32_t>(),
+                                                                vrE.as<uint32_t>(), cv.as<uint64_t>(),
                                                                 gval.as<float>());
         RNNG_HIP_TRY(hipGetLastError());
         int64_t groups = 0;
@@ -370,9 +394,9 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     if (maxdim >= 2 && E > 0) {
         RNNG_TRY(cntT.reserve((size_t)E * sizeof(int64_t)));
         RNNG_TRY(offT.reserve((size_t)E * sizeof(int64_t)));
-        k_triangles<false><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
-                                                                     col.as<int>(), nullptr, E, cntT.as<int64_t>(),
-                                                                     nullptr, nullptr, nullptr);
+        k_triangles<false><<<blocks_for(8 * E, 256), 256, 0, stream()>>>(
+            row.as<int>(), rowptr.as<int64_t>(), cv.as<uint64_t>(), E, cntT.as<int64_t>(), nullptr, nullptr,
+            nullptr);
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(exclusive_scan_i64(cntT.as<int64_t>(), offT.as<int64_t>(), E));
         RNNG_TRY(last_total(cntT.as<int64_t>(), offT.as<int64_t>(), E, &T));
@@ -390,10 +414,9 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     if (T > 0) {
         RNNG_TRY(tg.reserve((size_t)T * sizeof(uint32_t)));
         RNNG_TRY(tid.reserve((size_t)T * sizeof(int64_t)));
-        k_triangles<true><<<blocks_for(E, 256), 256, 0, stream()>>>(row.as<int>(), rowptr.as<int64_t>(),
-                                                                    col.as<int>(), vrE.as<uint32_t>(), E, nullptr,
-                                                                    offT.as<int64_t>(), tg.as<uint32_t>(),
-                                                                    tid.as<int64_t>());
+        k_triangles<true><<<blocks_for(8 * E, 256), 256, 0, stream()>>>(
+            row.as<int>(), rowptr.as<int64_t>(), cv.as<uint64_t>(), E, nullptr, offT.as<int64_t>(),
+            tg.as<uint32_t>(), tid.as<int64_t>());
         RNNG_HIP_TRY(hipGetLastError());
     }
     if (Q > 0) {
@@ -412,8 +435,8 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
 
     // 5. (value, id) order: stable group sorts, per-group counts, placement
-    if (T > 0) RNNG_TRY(sort_pairs_u32_i64(tg.as<uint32_t>(), tid.as<int64_t>(), T, gbits));
-    if (Q > 0) RNNG_TRY(sort_pairs_u32_i64(qg.as<uint32_t>(), qid.as<int64_t>(), Q, gbits));
+    if (T > 0) RNNG_TRY(sort_pairs_u32_i64_swap(tg, tid, T, gbits));
+    if (Q > 0) RNNG_TRY(sort_pairs_u32_i64_swap(qg, qid, Q, gbits));
     const int64_t Ecount = maxdim >= 1 ? E : 0;
     const int64_t total = (int64_t)n + Ecount + T + Q;
     RNNG_TRY(f->value.reserve(std::max<size_t>(1, (size_t)total) * sizeof(float)));
@@ -521,8 +544,10 @@ int rnng_filtration_copy(rnng_filtration_t f, float* value, int64_t* id, int64_t
     RNNG_TRY(ensure_init());
     const int64_t k = std::min(cap, f->n);
     if (k <= 0) return RNNG_OK;
-    if (value) RNNG_HIP_TRY(hipMemcpy(value, f->value.ptr, (size_t)k * sizeof(float), hipMemcpyDeviceToHost));
-    if (id) RNNG_HIP_TRY(hipMemcpy(id, f->id.ptr, (size_t)k * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (value)
+        RNNG_HIP_TRY(hipMemcpyAsync(value, f->value.ptr, (size_t)k * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    if (id) RNNG_HIP_TRY(hipMemcpyAsync(id, f->id.ptr, (size_t)k * sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }
 

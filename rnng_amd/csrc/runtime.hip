@@ -2,6 +2,7 @@
 // growable device buffers.
 #include "common.hpp"
 
+#include <map>
 #include <mutex>
 
 namespace rnng {
@@ -61,11 +62,63 @@ int ensure_init() {
 
 hipStream_t stream() { return g_stream; }
 
+// Device blocks are cached after release and handed out again to requests
+// that need between half of a block and all of it, so calls that allocate the
+// same scratch sizes each time (graph builds, filtrations) reuse memory
+// instead of paying hipMalloc/hipFree (which also synchronises the device).
+// Every device access of the library is ordered on its one stream, so a block
+// reused by a later request is only touched after the stream's earlier work.
+// At most kPoolCap bytes stay cached; a failing hipMalloc empties the cache
+// and retries.  The cache is never destroyed (buffers of static objects may
+// still be released during process exit).
+namespace {
+struct BlockPool {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_blocks;
+    size_t cached = 0;
+};
+BlockPool& pool() {
+    static BlockPool* p = new BlockPool();
+    return *p;
+}
+constexpr size_t kPoolCap = (size_t)48 << 30;
+
+size_t round_size(size_t need) {
+    const size_t q = need >= ((size_t)2 << 20) ? ((size_t)2 << 20) : 256;
+    return (need + q - 1) / q * q;
+}
+
+void empty_pool() {
+    BlockPool& p = pool();
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (auto& kv : p.free_blocks) (void)hipFree(kv.second);
+    p.free_blocks.clear();
+    p.cached = 0;
+}
+}  // namespace
+
 int DevBuf::reserve(size_t need) {
     if (need <= bytes && ptr) return RNNG_OK;
     release();
-    size_t alloc = need ? need : 256;
+    const size_t alloc = round_size(need ? need : 256);
+    {
+        BlockPool& p = pool();
+        std::lock_guard<std::mutex> lk(p.mu);
+        auto it = p.free_blocks.lower_bound(alloc);
+        if (it != p.free_blocks.end() && it->first <= 2 * alloc) {
+            ptr = it->second;
+            bytes = it->first;
+            p.cached -= it->first;
+            p.free_blocks.erase(it);
+            return RNNG_OK;
+        }
+    }
     hipError_t e = hipMalloc(&ptr, alloc);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        empty_pool();
+        e = hipMalloc(&ptr, alloc);
+    }
     if (e != hipSuccess) {
         ptr = nullptr;
         bytes = 0;
@@ -77,7 +130,16 @@ int DevBuf::reserve(size_t need) {
 }
 
 void DevBuf::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr) {
+        BlockPool& p = pool();
+        std::lock_guard<std::mutex> lk(p.mu);
+        if (p.cached + bytes <= kPoolCap) {
+            p.free_blocks.emplace(bytes, ptr);
+            p.cached += bytes;
+        } else {
+            (void)hipFree(ptr);
+        }
+    }
     ptr = nullptr;
     bytes = 0;
 }

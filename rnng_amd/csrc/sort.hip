@@ -1,6 +1,7 @@
 // Device-wide sorts and scans (rocPRIM radix sort, LSD and stable) used to
 // canonicalise edge lists and filtrations on the GPU.
 #include <cstring>
+#include <utility>
 
 #include <rocprim/rocprim.hpp>
 
@@ -24,6 +25,30 @@ int sort_pairs_impl(K* keys, V* vals, int64_t n, unsigned end_bit) {
     RNNG_HIP_TRY(hipMemcpyAsync(keys, kalt.ptr, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, stream()));
     RNNG_HIP_TRY(hipMemcpyAsync(vals, valt.ptr, (size_t)n * sizeof(V), hipMemcpyDeviceToDevice, stream()));
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    return RNNG_OK;
+}
+
+template <class K, class V>
+int sort_pairs_swap_impl(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit) {
+    if (n <= 1) return RNNG_OK;
+    LocalBuf kalt, valt, tmp;
+    RNNG_TRY(kalt.reserve((size_t)n * sizeof(K)));
+    RNNG_TRY(valt.reserve((size_t)n * sizeof(V)));
+    rocprim::double_buffer<K> kb(keys.as<K>(), kalt.as<K>());
+    rocprim::double_buffer<V> vb(vals.as<V>(), valt.as<V>());
+    size_t tb = 0;
+    RNNG_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tb, kb, vb, (size_t)n, 0, end_bit, stream()));
+    RNNG_TRY(tmp.reserve(tb));
+    RNNG_HIP_TRY(rocprim::radix_sort_pairs(tmp.ptr, tb, kb, vb, (size_t)n, 0, end_bit, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    if (kb.current() != keys.as<K>()) {
+        std::swap(keys.ptr, kalt.ptr);
+        std::swap(keys.bytes, kalt.bytes);
+    }
+    if (vb.current() != vals.as<V>()) {
+        std::swap(vals.ptr, valt.ptr);
+        std::swap(vals.bytes, valt.bytes);
+    }
     return RNNG_OK;
 }
 
@@ -77,6 +102,18 @@ int sort_pairs_u32_u32(uint32_t* keys, uint32_t* vals, int64_t n, unsigned end_b
 
 int sort_pairs_u64_f32_bits(uint64_t* keys, float* vals, int64_t n, unsigned end_bit) {
     return sort_pairs_impl<uint64_t, float>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u32_i64_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_swap_impl<uint32_t, int64_t>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u32_u32_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_swap_impl<uint32_t, uint32_t>(keys, vals, n, end_bit);
+}
+
+int sort_pairs_u64_f32_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_bit) {
+    return sort_pairs_swap_impl<uint64_t, float>(keys, vals, n, end_bit);
 }
 
 // Sorts (value, id) lexicographically: stable LSD passes by id, then by value.

@@ -54,6 +54,19 @@ def test_systolic_l2_file(orc, tmp_path):
     assert out.read_text().splitlines() == _expected_graph_lines(pts.shape[0], ij, w)
 
 
+def test_systolic_l2_file_3d(orc, tmp_path):
+    """3-D points: the driver's self join runs on the uniform grid."""
+    pts = np.random.default_rng(8).uniform(0, 4, size=(5000, 3)).astype(np.float32)
+    eps = np.float32(0.2)
+    f = tmp_path / "p.fvecs"
+    _write_fvecs(f, pts)
+    out = tmp_path / "g.txt"
+    _run([os.path.join(BIN, "rnng_systolic"), "-i", str(f), "-r", repr(float(eps)), "-o", str(out)])
+    ij, w = orc.l2_eps_pairs(pts, eps)
+    assert ij.shape[0] > 1000
+    assert out.read_text().splitlines() == _expected_graph_lines(pts.shape[0], ij, w)
+
+
 def test_systolic_edit_file(orc, tmp_path):
     from rnng_amd import synth
     atoms, offs = synth.dna_mutants(n_seeds=40, per_seed=25)
