@@ -325,6 +325,11 @@ float rnng_sq_threshold(float eps);
  * row and column it holds: rows[lane*16 + r], cols[lane*16 + r]. */
 int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
 
+/* Runs one 32x32x32 int8 MFMA (the edit filter's) on caller data: a and b
+ * hold 16 int8 per lane (64 lanes), d gets the 16 int32 accumulators of
+ * each lane (started at zero): d[lane*16 + r]. */
+int rnng_debug_mfma_i8(const int8_t *a, const int8_t *b, int32_t *d);
+
 /* out[i] = the device square root the kernels use for edge weights, for
  * checking it against the host's correctly rounded sqrtf. */
 int rnng_debug_sqrt(const float *in, float *out, int64_t n);

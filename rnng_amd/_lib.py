@@ -111,6 +111,7 @@ _SIGS = {
     "rnng_voronoi_cell_stats": (C.c_int, [_vp, _vp, _vp]),
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
+    "rnng_debug_mfma_i8": (C.c_int, [_vp, _vp, _vp]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_debug_copy": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
 }

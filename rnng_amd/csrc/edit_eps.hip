@@ -9,16 +9,20 @@
 // Three stages, all exact:
 //   1. Strings are ordered by length.  |m - n| <= dist, so for row string a
 //      only the contiguous range of strings at most k longer can match.
-//   2. k_edit_filter compares 64-bin byte profiles of hashed 3-grams with
-//      v_sad_u8.  Each edit operation changes at most 2q = 6 q-gram counts,
-//      and merging q-grams into bins (or clamping counts at 255) can only
-//      shrink the L1 distance, so dist >= SAD / 6: pairs with SAD > 6k are
-//      not edges.
+//   2. k_edit_filter bounds the L1 distance of 64-bin 3-gram count profiles
+//      on int8 MFMA.  Each edit operation changes at most 2q = 6 q-gram
+//      counts, and merging q-grams into bins or clamping counts at 4 can only
+//      shrink the L1 distance, so dist >= L1_4 / 6: pairs with L1_4 > 6k are
+//      not edges.  With counts clamped at 4 and written as 4-level
+//      thermometer codes t(p) (level l set iff count > l),
+//      L1_4(p, q) = |p|_4 + |q|_4 - 2 <t(p), t(q)>, a 256-long int8 dot
+//      product per pair.
 //   3. k_edit_verify runs Ukkonen's banded dynamic programme (cells with
 //      |i - j| <= KB, KB >= k).  Any alignment of cost <= k stays inside that
 //      band, so the banded value equals the Levenshtein distance whenever the
 //      distance is <= k and exceeds k otherwise.
 #include "internal.hpp"
+#include "mfma.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -34,14 +38,15 @@ struct rnng_strings_s {
     rnng::DevBuf slen;      // int32 [n]: length by sorted position
     rnng::DevBuf soff;      // int64 [n]: atom offset by sorted position
     rnng::DevBuf prof;      // uint4 [n][4]: 3-gram profile by sorted position
+    rnng::DevBuf mxa, myb;  // int8 thermometer fragments (MFMA layout), rows and columns
+    rnng::DevBuf mnrm;      // int [n_pad]: |p|_4 by sorted position (padding: huge)
+    int64_t n_pad = 0;
     std::vector<int32_t> len_sorted;
 };
 
 namespace rnng {
 namespace {
 
-constexpr int ET = 64;     // filter tile edge (pairs)
-constexpr int ECH = 32;    // column tiles per filter work item
 
 // 2-bit code of a character: exact for A, C, G, T (0, 1, 3, 2); for other
 // alphabets it merges characters, which keeps the profile bound valid.
@@ -69,83 +74,206 @@ __global__ __launch_bounds__(256) void k_profiles(const uint8_t* __restrict__ at
     }
 }
 
-__global__ __launch_bounds__(256) void k_edit_filter(
-    const uint4* __restrict__ prof, const int* __restrict__ slen, const int* __restrict__ order,
-    const int4* __restrict__ work, int n, int k, unsigned maxsad, int2* __restrict__ cand,
-    unsigned long long cap, unsigned long long* __restrict__ counter) {
-    __shared__ uint4 rp[ET][4];
-    __shared__ uint4 cp[ET][4];
-    __shared__ int rl[ET], cl[ET];
-    const int4 wi = work[blockIdx.x];
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int a0 = wi.x * ET;
-    {
-        const int r = tid >> 2, q = tid & 3;
-        const int a = a0 + r;
-        rp[r][q] = a < n ? prof[(size_t)a * 4 + q] : make_uint4(0, 0, 0, 0);
-        if (q == 0) rl[r] = a < n ? slen[a] : 0;
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr int TLEV = 4;              // thermometer levels per 3-gram bin (counts clamped at 4)
+constexpr int EKS = 64 * TLEV / 32;  // int8 k-steps of 32 per pair
+constexpr int ERT = 2;               // row tiles (32 strings) per wave
+constexpr int ERB = 4 * ERT * 32;    // rows per work item (4 waves)
+constexpr int ECT = 64;              // column tiles per work item
+constexpr int EPAD = 1 << 28;        // |p|_4 of padding strings: never a candidate
+constexpr int ESTAGE = 4096;         // candidates staged in LDS per work item
+
+// Fragment layout (as the L2 filter's, with int8 operands): tile t = 32
+// strings, k-step s, lane l: frag[(t*EKS + s)*64 + l] = 16 int8 of string
+// 32t + (l & 31), thermometer slots 32s + 16(l >> 5) + j (bin = slot / 4,
+// level = slot % 4).  Rows carry 1 per set level, columns -2, so the MFMA
+// adds -2 <t(p), t(q)>.  One thread per (string, step, half).
+__global__ __launch_bounds__(256) void k_edit_frags(const uint4* __restrict__ prof, int n, int64_t n_pad,
+                                                    i32x4_t* __restrict__ xa, i32x4_t* __restrict__ yb,
+                                                    int* __restrict__ nrm) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_pad * EKS * 2) return;
+    const int64_t a = g / (EKS * 2);
+    const int s = (int)(g % (EKS * 2)) >> 1, h = (int)(g & 1);
+    // bins 8s + 4h .. +3 are word 2(s & 1) + h of profile chunk s / 2
+    unsigned c4 = 0;
+    if (a < n) {
+        const uint4 ch = prof[a * 4 + (s >> 1)];
+        const int w = 2 * (s & 1) + h;
+        c4 = w == 0 ? ch.x : w == 1 ? ch.y : w == 2 ? ch.z : ch.w;
     }
-    for (int bt = wi.y; bt < wi.z; ++bt) {
-        const int b0 = bt * ET;
-        __syncthreads();
-        {
-            const int r = tid >> 2, q = tid & 3;
-            const int b = b0 + r;
-            cp[r][q] = b < n ? prof[(size_t)b * 4 + q] : make_uint4(0, 0, 0, 0);
-            if (q == 0) cl[r] = b < n ? slen[b] : 0;
-        }
-        __syncthreads();
-        unsigned acc[4][4];
+    i32x4_t ones, twos;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+    for (int j = 0; j < 4; ++j) {
+        const unsigned c = (c4 >> (8 * j)) & 255u;
+        unsigned lv = 0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc[r][c] = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint4 x[4], y[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = rp[ty * 4 + r][q];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) y[c] = cp[tx * 4 + c][q];
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    unsigned s = acc[r][c];
-                    s = __builtin_amdgcn_sad_u8(x[r].x, y[c].x, s);
-                    s = __builtin_amdgcn_sad_u8(x[r].y, y[c].y, s);
-                    s = __builtin_amdgcn_sad_u8(x[r].z, y[c].z, s);
-                    s = __builtin_amdgcn_sad_u8(x[r].w, y[c].w, s);
-                    acc[r][c] = s;
-                }
-        }
-        unsigned cnt = 0;
-        bool ok[4][4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int a = a0 + ty * 4 + r, b = b0 + tx * 4 + c;
-                ok[r][c] = a < b && b < n && (cl[tx * 4 + c] - rl[ty * 4 + r]) <= k &&
-                           acc[r][c] <= maxsad;
-                cnt += ok[r][c] ? 1u : 0u;
+        for (int l = 0; l < TLEV; ++l) lv |= (c > (unsigned)l ? 1u : 0u) << (8 * l);
+        ones[j] = (int)lv;
+        twos[j] = (int)(lv * 0xFEu);   // 0x01 -> 0xFE (-2) per byte, no carries
+    }
+    const int64_t at = (a >> 5) * EKS + s;
+    const int lane = h * 32 + (int)(a & 31);
+    xa[at * 64 + lane] = ones;
+    yb[at * 64 + lane] = twos;
+    if (s == 0 && h == 0) {
+        int sum = EPAD;
+        if (a < n) {
+            sum = 0;
+            for (int q = 0; q < 4; ++q) {
+                const uint4 ch = prof[a * 4 + q];
+                const unsigned w4[4] = {ch.x, ch.y, ch.z, ch.w};
+                for (int w = 0; w < 4; ++w)
+                    for (int j = 0; j < 4; ++j) sum += (int)min((w4[w] >> (8 * j)) & 255u, (unsigned)TLEV);
             }
-        if (__any(cnt != 0)) {
-            unsigned long long pos = wave_reserve(cnt, counter);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if (ok[r][c]) {
-                        if (pos < cap) {
-                            const int i = order[a0 + ty * 4 + r], j = order[b0 + tx * 4 + c];
-                            cand[pos] = make_int2(min(i, j), max(i, j));
-                        }
-                        ++pos;
-                    }
         }
+        nrm[a] = sum;
     }
+}
+
+__global__ void k_debug_mfma_i8(const i32x4_t* __restrict__ a, const i32x4_t* __restrict__ b,
+                                int* __restrict__ d) {
+    const int lane = threadIdx.x;
+    i32x16_t acc;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0;
+    acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[lane], b[lane], acc, 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) d[lane * 16 + q] = acc[q];
+}
+
+// One work item = rows [ERB wi.x, +ERB) (two row tiles per wave, fragments in
+// registers) against column tiles [wi.y, wi.z), which the workgroup stages in
+// LDS (double buffered, one barrier per tile).  The accumulator of pair
+// (a, b) starts at |p_a|_4 + |p_b|_4 - 6k - 1 and the MFMAs add
+// -2 <t(p_a), t(p_b)>, so its sign bit is set iff L1_4 <= 6k.  The hot loop
+// ORs the sign bits; tiles with a set bit take the candidate path, which
+// keeps a < b < n (sorted positions) inside the length window and stages
+// (a, b) in LDS.  It reads no memory: on this filter most tiles hold a
+// candidate, and a load per set bit would stall the whole workgroup at the
+// next barrier, so row lengths sit in the wave's lanes and column lengths
+// are fetched a tile ahead.
+__global__ __launch_bounds__(256) void k_edit_filter(
+    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ nrm,
+    const int* __restrict__ slen, const int4* __restrict__ work, int n, int k, int bias,
+    int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
+    constexpr int NCH = EKS * 64;   // 16-byte fragments per column tile
+    __shared__ i32x4_t bs[2][NCH];
+    __shared__ int2 stage[ESTAGE];
+    __shared__ unsigned scount;
+    __shared__ unsigned long long sbase;
+    const int4 wi = work[blockIdx.x];
+    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves
+    const int tid = threadIdx.x;
+    if (tid == 0) scount = 0u;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int rt0 = wi.x * (4 * ERT) + wave * ERT;
+
+    i32x4_t a[ERT][EKS];
+#pragma unroll
+    for (int r = 0; r < ERT; ++r)
+#pragma unroll
+        for (int s = 0; s < EKS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * EKS + s) * 64 + lane];
+    int xr[ERT][16];
+#pragma unroll
+    for (int r = 0; r < ERT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[r][q] = nrm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+    int yc = nrm[wi.y * 32 + (lane & 31)] + bias;
+    // lane l keeps the length of row rt0*32 + l (the wave's 64 rows), and each
+    // lane the length of its column, fetched a tile ahead
+    const int rlen = rt0 * 32 + lane < n ? slen[rt0 * 32 + lane] : 0;
+    int clen = wi.y * 32 + (lane & 31) < n ? slen[wi.y * 32 + (lane & 31)] : 0;
+    bs[0][tid] = yb[(size_t)wi.y * NCH + tid];
+    bs[0][tid + 256] = yb[(size_t)wi.y * NCH + tid + 256];
+    __syncthreads();
+
+    for (int ct = wi.y; ct < wi.z; ++ct) {
+        const int buf = (ct - wi.y) & 1;
+        const bool has_next = ct + 1 < wi.z;
+        i32x4_t p0 = {0, 0, 0, 0}, p1 = {0, 0, 0, 0};
+        int ycn = 0, clen_next = 0;
+        if (has_next) {
+            p0 = yb[(size_t)(ct + 1) * NCH + tid];
+            p1 = yb[(size_t)(ct + 1) * NCH + tid + 256];
+            const int cn = (ct + 1) * 32 + (lane & 31);
+            ycn = nrm[cn] + bias;
+            clen_next = cn < n ? slen[cn] : 0;
+        }
+        i32x16_t acc[ERT];
+#pragma unroll
+        for (int r = 0; r < ERT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[r][q] = xr[r][q] + yc;
+#pragma unroll
+        for (int s = 0; s < EKS; ++s) {
+            const i32x4_t bb = bs[buf][s * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < ERT; ++r) acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, acc[r], 0, 0, 0);
+        }
+        int o = 0;
+#pragma unroll
+        for (int r = 0; r < ERT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) o |= acc[r][q];
+
+        if (__any(o < 0)) {
+            // Rare path.  The mask's shifts depend on an opaque zero, so the
+            // compiler cannot hoist them into every tile of the hot loop.
+            int opaque;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
+            unsigned mask = 0;
+#pragma unroll
+            for (int r = 0; r < ERT; ++r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) mask |= ((unsigned)acc[r][q] >> 31) << (r * 16 + q + opaque);
+            const int col = ct * 32 + (lane & 31);
+            const int rowb = rt0 * 32 + 4 * (lane >> 5);
+            unsigned any = mask;
+#pragma unroll
+            for (int o2 = 1; o2 < 64; o2 <<= 1) any |= (unsigned)__shfl_xor((int)any, o2, 64);
+            any = __builtin_amdgcn_readfirstlane(any);
+#pragma unroll 1
+            while (any) {
+                const int rq = __builtin_ctz(any);
+                any &= any - 1;
+                const int q = rq & 15;
+                const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
+                const int rl = __shfl(rlen, row - rt0 * 32, 64);
+                const bool p = ((mask >> rq) & 1u) && row < col && col < n && clen - rl <= k;
+                const unsigned long long m = __ballot(p);
+                if (m) {
+                    unsigned slot = 0;
+                    if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
+                    slot = __shfl(slot, 0, 64) +
+                           __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    if (p) {
+                        const int2 pr = make_int2(row, col);
+                        if (slot < ESTAGE) {
+                            stage[slot] = pr;
+                        } else {
+                            const unsigned long long gslot = atomicAdd(counter, 1ull);
+                            if (gslot < cap) cand[gslot] = pr;
+                        }
+                    }
+                }
+            }
+        }
+        if (has_next) {
+            bs[buf ^ 1][tid] = p0;
+            bs[buf ^ 1][tid + 256] = p1;
+        }
+        __syncthreads();
+        yc = ycn;
+        clen = clen_next;
+    }
+    const unsigned nst = min(scount, (unsigned)ESTAGE);
+    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
+    __syncthreads();
+    for (unsigned i = tid; i < nst; i += 256)
+        if (sbase + i < cap) cand[sbase + i] = stage[i];
 }
 
 // Banded Levenshtein, band half-width KB >= k.  D[o + KB] holds the DP cell
@@ -199,10 +327,13 @@ __device__ int banded_edit(const uint8_t* __restrict__ s, int m, const uint8_t* 
 }
 
 template <int KB>
+// Candidates are sorted positions (a, b); soff/slen address the strings in
+// that order and order[] gives the original ids of the output edge.
 __global__ __launch_bounds__(256) void k_edit_verify(
-    const uint8_t* __restrict__ atoms, const int64_t* __restrict__ offsets,
-    const int2* __restrict__ cand, unsigned long long ccap, int k, int2* __restrict__ eij,
-    float* __restrict__ ew, unsigned long long ecap, unsigned long long* __restrict__ counters) {
+    const uint8_t* __restrict__ atoms, const int64_t* __restrict__ soff, const int* __restrict__ slen,
+    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
+    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    unsigned long long* __restrict__ counters) {
     const unsigned long long nc = min(counters[0], ccap);
     const int lane = lane_id();
     for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
@@ -212,11 +343,16 @@ __global__ __launch_bounds__(256) void k_edit_verify(
         int2 p = make_int2(0, 0);
         int dist = 0;
         if (c < nc) {
-            p = cand[c];
-            const int64_t oi = offsets[p.x], oj = offsets[p.y];
-            const int m = (int)(offsets[p.x + 1] - oi), nn = (int)(offsets[p.y + 1] - oj);
-            dist = banded_edit<KB>(atoms + oi, m, atoms + oj, nn, k);
-            edge = dist <= k;
+            const int2 q = cand[c];
+            const int m = slen[q.x], nn = slen[q.y];
+            if (m - nn <= k && nn - m <= k) {   // |m - n| <= dist
+                dist = banded_edit<KB>(atoms + soff[q.x], m, atoms + soff[q.y], nn, k);
+                edge = dist <= k;
+            }
+            if (edge) {
+                const int i = order[q.x], j = order[q.y];
+                p = make_int2(min(i, j), max(i, j));
+            }
         }
         const unsigned long long msk = __ballot(edge);
         if (msk) {
@@ -255,7 +391,8 @@ EditWs g_ews;
 template <int KB>
 void launch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
     k_edit_verify<KB><<<2048, 256, 0, stream()>>>(
-        S->atoms.as<uint8_t>(), S->offsets.as<int64_t>(), ws.cand.as<int2>(), ws.cand_cap, k,
+        S->atoms.as<uint8_t>(), S->soff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
+        ws.cand.as<int2>(), ws.cand_cap, k,
         g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
         ws.counters.as<unsigned long long>());
 }
@@ -289,18 +426,20 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
         set_error("edit-distance radius above 32 is not supported");
         return RNNG_ERR_UNSUPPORTED;
     }
-    // work list: row tile x column-tile chunk over the length-compatible range
+    // work list: row block x column-tile chunk over the length-compatible range
     ws.work_host.clear();
-    const int ntiles = (n + ET - 1) / ET;
-    for (int at = 0; at < ntiles; ++at) {
-        const int a_last = std::min(n - 1, at * ET + ET - 1);
+    const int nrb = (int)(S->n_pad / ERB);
+    for (int rb = 0; rb < nrb; ++rb) {
+        if ((int64_t)rb * ERB >= n) break;
+        const int a_last = std::min(n - 1, rb * ERB + ERB - 1);
         const int maxlen = S->len_sorted[a_last] + k;
         const int b_end = (int)(std::upper_bound(S->len_sorted.begin(), S->len_sorted.end(), maxlen) -
                                 S->len_sorted.begin());
-        const int bt_end = (b_end + ET - 1) / ET;
-        for (int bt = at; bt < bt_end; bt += ECH)
-            ws.work_host.push_back(make_int4(at, bt, std::min(bt_end, bt + ECH), 0));
+        const int ct_end = (b_end + 31) / 32;
+        for (int ct = rb * (ERB / 32); ct < ct_end; ct += ECT)
+            ws.work_host.push_back(make_int4(rb, ct, std::min(ct_end, ct + ECT), 0));
     }
+    xcd_permute(ws.work_host);
     RNNG_TRY(ws.counters.reserve(8 * sizeof(unsigned long long)));
     RNNG_TRY(ws.work.reserve(std::max<size_t>(1, ws.work_host.size()) * sizeof(int4)));
     if (!ws.work_host.empty())
@@ -320,8 +459,9 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
         RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
         if (!ws.work_host.empty())
             k_edit_filter<<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-                S->prof.as<uint4>(), S->slen.as<int>(), S->order.as<int>(), ws.work.as<int4>(), n, k,
-                (unsigned)(6 * k), ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
+                S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(), S->mnrm.as<int>(), S->slen.as<int>(),
+                ws.work.as<int4>(), n, k, -(6 * k + 1), ws.cand.as<int2>(), ws.cand_cap,
+                ws.counters.as<unsigned long long>());
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
         RNNG_TRY(dispatch_verify(S, ws, k, g));
@@ -423,11 +563,50 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
             rc = RNNG_ERR_HIP;
         }
     }
+    // MFMA filter fragments, padded to whole row blocks
+    S->n_pad = (n + ERB - 1) / ERB * ERB;
+    if (rc == RNNG_OK && n > 0) {
+        // 64 bins x TLEV levels = one int8 per slot: n_pad x 256 bytes per form
+        const size_t fb = (size_t)S->n_pad * 64 * TLEV;
+        rc = S->mxa.reserve(fb);
+        if (rc == RNNG_OK) rc = S->myb.reserve(fb);
+        if (rc == RNNG_OK) rc = S->mnrm.reserve((size_t)S->n_pad * sizeof(int));
+        if (rc == RNNG_OK) {
+            k_edit_frags<<<blocks_for(S->n_pad * EKS * 2, 256), 256, 0, stream()>>>(
+                S->prof.as<uint4>(), (int)n, S->n_pad, S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(),
+                S->mnrm.as<int>());
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(stream());
+            if (e != hipSuccess) {
+                set_error(std::string("k_edit_frags: ") + hipGetErrorString(e));
+                rc = RNNG_ERR_HIP;
+            }
+        }
+    }
     if (rc != RNNG_OK) {
         rnng_strings_destroy(S);
         return rc;
     }
     *out = S;
+    return RNNG_OK;
+}
+
+int rnng_debug_mfma_i8(const int8_t* a, const int8_t* b, int32_t* d) {
+    if (!a || !b || !d) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    LocalBuf da, db, dd;
+    RNNG_TRY(da.reserve(1024));
+    RNNG_TRY(db.reserve(1024));
+    RNNG_TRY(dd.reserve(64 * 16 * sizeof(int)));
+    RNNG_HIP_TRY(hipMemcpyAsync(da.ptr, a, 1024, hipMemcpyHostToDevice, stream()));
+    RNNG_HIP_TRY(hipMemcpyAsync(db.ptr, b, 1024, hipMemcpyHostToDevice, stream()));
+    k_debug_mfma_i8<<<1, 64, 0, stream()>>>(da.as<i32x4_t>(), db.as<i32x4_t>(), dd.as<int>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_HIP_TRY(hipMemcpyAsync(d, dd.ptr, 64 * 16 * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }
 
@@ -439,6 +618,9 @@ int rnng_strings_destroy(rnng_strings_t S) {
     S->slen.release();
     S->soff.release();
     S->prof.release();
+    S->mxa.release();
+    S->myb.release();
+    S->mnrm.release();
     delete S;
     return RNNG_OK;
 }

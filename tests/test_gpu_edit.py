@@ -64,6 +64,47 @@ def test_edge_cases(orc):
         edit.eps_pairs(atoms, offs, 40)
 
 
+def _clamped_profiles(atoms, offs):
+    """64-bin 3-gram counts (2-bit codes (c >> 1) & 3, as the filter hashes
+    them), clamped at 4."""
+    code = (atoms.astype(np.int64) >> 1) & 3
+    n = offs.shape[0] - 1
+    P = np.zeros((n, 64), np.int64)
+    for i in range(n):
+        c = code[offs[i]:offs[i + 1]]
+        if c.shape[0] >= 3:
+            P[i] = np.bincount((c[:-2] << 4) | (c[1:-1] << 2) | c[2:], minlength=64)
+    return np.minimum(P, 4)
+
+
+@pytest.mark.parametrize("eps", [0, 3, 8, 12])
+def test_filter_keeps_exactly_the_clamped_qgram_pairs(eps):
+    """The MFMA filter's survivors are exactly the pairs inside the length
+    window whose clamped 3-gram profiles are within L1 distance 6k: the
+    candidate count equals that count computed on the CPU (a wrong operand
+    layout or accumulator start would change it)."""
+    from rnng_amd import Stats, synth
+    atoms, offs = synth.dna_mutants(n_seeds=25, per_seed=80)
+    rng = np.random.default_rng(4)
+    extra = [bytes(rng.choice(list(b"ACGTN"), int(rng.integers(90, 160)))) for _ in range(300)]
+    a2, o2 = _edit().as_csr(extra)
+    atoms = np.concatenate([atoms, a2])
+    offs = np.concatenate([offs, o2[1:] + offs[-1]])
+    n = offs.shape[0] - 1
+    Q = _clamped_profiles(atoms, offs)
+    L = np.diff(offs)
+    k = int(np.floor(eps))
+    want = 0
+    for i in range(n - 1):
+        ok = np.abs(L[i + 1:] - L[i]) <= k
+        want += int((np.abs(Q[i + 1:][ok] - Q[i]).sum(1) <= 6 * k).sum())
+    st = Stats()
+    flush_print(f"filter count n={n} eps={eps}")
+    _edit().eps_pairs(atoms, offs, eps, stats=st)
+    assert st.candidates == want
+    assert st.candidates >= st.edges
+
+
 def test_full_size_dna_properties():
     """200,000 strings, eps = 8: every reported edge is verified by the
     oracle's Levenshtein on a sample, the list is canonical, and most edges
