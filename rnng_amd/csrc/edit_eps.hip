@@ -17,10 +17,20 @@
 //      thermometer codes t(p) (level l set iff count > l),
 //      L1_4(p, q) = |p|_4 + |q|_4 - 2 <t(p), t(q)>, a 256-long int8 dot
 //      product per pair.
-//   3. k_edit_verify runs Ukkonen's banded dynamic programme (cells with
-//      |i - j| <= KB, KB >= k).  Any alignment of cost <= k stays inside that
-//      band, so the banded value equals the Levenshtein distance whenever the
-//      distance is <= k and exceeds k otherwise.
+//   3. k_edit_verify_bp decides each candidate with a bit-parallel
+//      Levenshtein recurrence (Myers 1999, in Hyyro's 2001 form for the edit
+//      distance) restricted to Ukkonen's diagonal band |i - j| <= k: one
+//      (2k+1)-bit window of vertical deltas per column that slides down one
+//      row per column once j > k + 1.  The cell just above the window takes
+//      the value of its left neighbour + 1 and the cell below the window in
+//      the previous column that of its upper neighbour + 1; both are upper
+//      bounds of the true values, so every computed cell is >= the true
+//      distance, and cells on an alignment of cost <= k (all inside the band)
+//      are exact: the result equals the Levenshtein distance whenever that is
+//      <= k and exceeds k otherwise.  The match vector of a column is an AND
+//      over the bit-planes of the string set's dense alphabet codes (2 planes
+//      for DNA).  k = 32 (a 65-bit window) uses k_edit_verify, Ukkonen's
+//      banded dynamic programme with the same band.
 #include "internal.hpp"
 #include "mfma.hpp"
 
@@ -40,6 +50,9 @@ struct rnng_strings_s {
     rnng::DevBuf prof;      // uint4 [n][4]: 3-gram profile by sorted position
     rnng::DevBuf mxa, myb;  // int8 thermometer fragments (MFMA layout), rows and columns
     rnng::DevBuf mnrm;      // int [n_pad]: |p|_4 by sorted position (padding: huge)
+    rnng::DevBuf codes;     // uint8 dense alphabet codes, strings in sorted order, 16-byte aligned
+    rnng::DevBuf coff;      // int64 [n]: offset of each sorted position's codes
+    int planes = 1;         // bits of a code
     int64_t n_pad = 0;
     std::vector<int32_t> len_sorted;
 };
@@ -375,6 +388,140 @@ __global__ __launch_bounds__(256) void k_edit_verify(
     }
 }
 
+// Bit-planes of 32 consecutive codes (two aligned 16-byte loads): bit u of
+// w[q] = bit q of code u.  Per dword, (x >> q) & 0x01010101 leaves one bit per
+// byte and the multiply by 0x01020408 gathers the four into bits 24..27
+// (distinct partial products, no carries).
+template <int NP>
+__device__ __forceinline__ void plane_words(const uint8_t* __restrict__ c32, unsigned (&w)[NP]) {
+    const uint4 a = *reinterpret_cast<const uint4*>(c32);
+    const uint4 b = *reinterpret_cast<const uint4*>(c32 + 16);
+    const unsigned d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        unsigned v = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v |= (((d[r] >> q) & 0x01010101u) * 0x01020408u) >> 24 << (4 * r);
+        w[q] = v;
+    }
+}
+
+// Bit-parallel banded Levenshtein of pattern p (rows, length m) against
+// text t (columns, length n): D[m][n] if <= k, else a value > k.  VT holds
+// the 2k+1-bit window (uint32_t for k <= 15, uint64_t for k <= 31); NP is the
+// number of code bit-planes.  Both strings start 16-byte aligned and may be
+// read up to 128 bytes past their end (codes there only reach rows below m
+// or columns after n, which no kept cell depends on).  Columns run in groups
+// of 32: the text's plane words give each column's code bits, and the
+// pattern rows entering the window (row j + k for column j) come from one
+// funnel shift by k of two consecutive pattern plane words.
+template <class VT, int NP>
+__device__ int banded_edit_bp(const uint8_t* __restrict__ p, int m, const uint8_t* __restrict__ t, int n,
+                              int k) {
+    if (m - n > k || n - m > k) return k + 1;
+    const int w = 2 * k + 1;
+    const VT W = w >= (int)(8 * sizeof(VT)) ? ~(VT)0 : (((VT)1 << w) - 1);
+    unsigned pc[NP], pn[NP];
+    plane_words<NP>(p, pc);        // pattern codes 0..31
+    plane_words<NP>(p + 32, pn);   // 32..63
+    VT pl[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) pl[q] = (sizeof(VT) == 8 ? ((VT)pc[q] | ((VT)pn[q] << 32)) : (VT)pc[q]) & W;
+    VT Pv = W, Mv = 0;
+    int T = 0;       // value of the cell just above the window
+    int R = 1;       // first row of the window
+    for (int g = 0; 32 * g < n; ++g) {
+        unsigned tw[NP], ew[NP];
+        plane_words<NP>(t + 32 * g, tw);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) ew[q] = __builtin_amdgcn_alignbit(pn[q], pc[q], (unsigned)k);
+        const int cols = min(32, n - 32 * g);
+        for (int u = 0; u < cols; ++u) {
+            const int j = 32 * g + u + 1;
+            if (j > k + 1) {
+                T += 1 + (int)(Pv & 1) - (int)(Mv & 1);
+                Pv = (Pv >> 1) | ((VT)1 << (w - 1));
+                Mv >>= 1;
+                ++R;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) pl[q] = (pl[q] >> 1) | ((VT)((ew[q] >> u) & 1u) << (w - 1));
+            } else {
+                T += 1;
+            }
+            VT Eq = W;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) Eq &= pl[q] ^ ((VT)((tw[q] >> u) & 1u) - (VT)1);
+            const VT Xv = Eq | Mv;
+            const VT Xh = ((((Eq & Pv) + Pv) & W) ^ Pv) | Eq;
+            VT Ph = (Mv | ~(Xh | Pv)) & W;
+            VT Mh = Pv & Xh;
+            Ph = ((Ph << 1) | 1) & W;
+            Mh = (Mh << 1) & W;
+            Pv = (Mh | ~(Xv | Ph)) & W;
+            Mv = Ph & Xv;
+        }
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pc[q] = pn[q];
+        plane_words<NP>(p + 32 * (g + 2), pn);
+    }
+    const int nb = m - R + 1;   // rows R..m of the final window
+    const VT mask = nb <= 0 ? (VT)0 : nb >= (int)(8 * sizeof(VT)) ? ~(VT)0 : (((VT)1 << nb) - 1);
+    int pcnt, mcnt;
+    if (sizeof(VT) == 8) {
+        pcnt = __popcll((unsigned long long)(Pv & mask));
+        mcnt = __popcll((unsigned long long)(Mv & mask));
+    } else {
+        pcnt = __popc((unsigned)(Pv & mask));
+        mcnt = __popc((unsigned)(Mv & mask));
+    }
+    return T + pcnt - mcnt;
+}
+
+template <class VT, int NP>
+__global__ __launch_bounds__(256) void k_edit_verify_bp(
+    const uint8_t* __restrict__ codes, const int64_t* __restrict__ coff, const int* __restrict__ slen,
+    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
+    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    unsigned long long* __restrict__ counters) {
+    const unsigned long long nc = min(counters[0], ccap);
+    const int lane = lane_id();
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long c = base + threadIdx.x;
+        bool edge = false;
+        int2 pr = make_int2(0, 0);
+        int dist = 0;
+        if (c < nc) {
+            const int2 q = cand[c];
+            const int m = slen[q.x], nn = slen[q.y];
+            dist = banded_edit_bp<VT, NP>(codes + coff[q.x], m, codes + coff[q.y], nn, k);
+            edge = dist <= k;
+            if (edge) {
+                const int i = order[q.x], j = order[q.y];
+                pr = make_int2(min(i, j), max(i, j));
+            }
+        }
+        const unsigned long long msk = __ballot(edge);
+        if (msk) {
+            unsigned lo = 0, hi = 0;
+            if (lane == 0) {
+                unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)__popcll(msk));
+                lo = (unsigned)b0;
+                hi = (unsigned)(b0 >> 32);
+            }
+            lo = __shfl(lo, 0, 64);
+            hi = __shfl(hi, 0, 64);
+            const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
+            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+            if (edge && b0 + rank < ecap) {
+                eij[b0 + rank] = pr;
+                ew[b0 + rank] = (float)dist;
+            }
+        }
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -397,7 +544,33 @@ void launch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) 
         ws.counters.as<unsigned long long>());
 }
 
+template <class VT, int NP>
+void launch_verify_bp(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    k_edit_verify_bp<VT, NP><<<2048, 256, 0, stream()>>>(
+        S->codes.as<uint8_t>(), S->coff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
+        ws.cand.as<int2>(), ws.cand_cap, k, g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
+        ws.counters.as<unsigned long long>());
+}
+
+template <class VT>
+void launch_verify_planes(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    if (S->planes <= 2) launch_verify_bp<VT, 2>(S, ws, k, g);
+    else if (S->planes <= 3) launch_verify_bp<VT, 3>(S, ws, k, g);
+    else if (S->planes <= 4) launch_verify_bp<VT, 4>(S, ws, k, g);
+    else launch_verify_bp<VT, 8>(S, ws, k, g);
+}
+
 int dispatch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    if (k <= 15) {
+        launch_verify_planes<uint32_t>(S, ws, k, g);
+        RNNG_HIP_TRY(hipGetLastError());
+        return RNNG_OK;
+    }
+    if (k <= 31) {
+        launch_verify_planes<uint64_t>(S, ws, k, g);
+        RNNG_HIP_TRY(hipGetLastError());
+        return RNNG_OK;
+    }
     if (k <= 4) launch_verify<4>(S, ws, k, g);
     else if (k <= 8) launch_verify<8>(S, ws, k, g);
     else if (k <= 12) launch_verify<12>(S, ws, k, g);
@@ -552,6 +725,30 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
     up(S->order, ord.data(), ord.size() * sizeof(int32_t));
     up(S->slen, len.data(), len.size() * sizeof(int32_t));
     up(S->soff, soff.data(), soff.size() * sizeof(int64_t));
+    // dense alphabet codes for the bit-parallel verify (same layout as atoms)
+    {
+        bool seen[256] = {false};
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(atoms) + base;
+        for (int64_t i = 0; i < S->total; ++i) seen[src[i]] = true;
+        uint8_t code[256];
+        int sigma = 0;
+        for (int c = 0; c < 256; ++c) code[c] = seen[c] ? (uint8_t)sigma++ : 0;
+        S->planes = 1;
+        while ((1 << S->planes) < sigma) ++S->planes;
+        // strings in sorted order, each starting on a 16-byte boundary; the
+        // verify kernel reads up to 128 bytes past a string's end
+        std::vector<int64_t> co((size_t)n);
+        int64_t pos = 0;
+        for (int64_t a = 0; a < n; ++a) {
+            co[(size_t)a] = pos;
+            pos += (len[(size_t)a] + 15) / 16 * 16;
+        }
+        std::vector<uint8_t> cs((size_t)pos + 256, 0);
+        for (int64_t a = 0; a < n; ++a)
+            for (int32_t i = 0; i < len[(size_t)a]; ++i) cs[(size_t)(co[(size_t)a] + i)] = code[src[soff[(size_t)a] + i]];
+        up(S->codes, cs.data(), cs.size());
+        up(S->coff, co.data(), co.size() * sizeof(int64_t));
+    }
     if (rc == RNNG_OK) rc = S->prof.reserve(std::max<size_t>(16, (size_t)n * 64));
     if (rc == RNNG_OK && n > 0) {
         k_profiles<<<blocks_for(n, 256), 256, 0, stream()>>>(S->atoms.as<uint8_t>(), S->soff.as<int64_t>(),
@@ -621,6 +818,8 @@ int rnng_strings_destroy(rnng_strings_t S) {
     S->mxa.release();
     S->myb.release();
     S->mnrm.release();
+    S->codes.release();
+    S->coff.release();
     delete S;
     return RNNG_OK;
 }
