@@ -75,6 +75,29 @@ public:
     Point<Atom> operator[](Index i) const { return Point<Atom>(mem(i), size(i), id(i), i); }
 
     // fvecs records: int32 dimension then that many float32 (include/point.hpp:112-168).
+    Index read_bin(const char* fname, bool u8) {
+        static_assert(std::is_same<Atom, float>::value, "binary point files hold float or u8 points");
+        std::ifstream is(fname, std::ios::binary);
+        if (!is) fail(std::string("cannot open ") + fname);
+        uint32_t h[2] = {0, 0};
+        if (!is.read(reinterpret_cast<char*>(h), sizeof(h))) fail(std::string("missing header in ") + fname);
+        const size_t n = h[0], d = h[1], m = n * d;
+        data.resize(m);
+        if (u8) {
+            std::vector<unsigned char> b(m);
+            if (m && !is.read(reinterpret_cast<char*>(b.data()), (std::streamsize)m)) fail(std::string("short file ") + fname);
+            for (size_t i = 0; i < m; ++i) data[i] = (float)b[i];
+        } else if (m && !is.read(reinterpret_cast<char*>(data.data()), (std::streamsize)(m * sizeof(float)))) {
+            fail(std::string("short file ") + fname);
+        }
+        offsets.resize(n + 1);
+        ids.resize(n);
+        for (size_t i = 0; i <= n; ++i) offsets[i] = (Index)(i * d);
+        for (size_t i = 0; i < n; ++i) ids[i] = (Index)i;
+        invalidate();
+        return (Index)n;
+    }
+
     Index read_fvecs(const char* fname) {
         static_assert(std::is_same<Atom, float>::value, "read_fvecs reads float points");
         std::ifstream is(fname, std::ios::binary);
@@ -104,6 +127,24 @@ public:
         offsets[total] = (Index)(total * (size_t)d);
         invalidate();
         return (Index)total;
+    }
+
+    // .fbin / .u8bin (uint32 n, uint32 d, then n*d float32 or uint8;
+    // dataset_io.py:48-60): u8 values become floats.
+    Index read_fbin(const char* fname) { return read_bin(fname, false); }
+    Index read_u8bin(const char* fname) { return read_bin(fname, true); }
+
+    // By extension as dataset_io.py's read_file: .fbin, .u8bin; anything else
+    // is read as fvecs, as the reference drivers read every input.
+    Index read_file(const char* fname) {
+        const std::string f(fname);
+        auto ends = [&](const char* e) {
+            const std::string x(e);
+            return f.size() >= x.size() && f.compare(f.size() - x.size(), x.size(), x) == 0;
+        };
+        if (ends(".fbin")) return read_fbin(fname);
+        if (ends(".u8bin")) return read_u8bin(fname);
+        return read_fvecs(fname);
     }
 
     // One string per non-empty line, a trailing '\r' dropped (include/point.hpp:170-205).

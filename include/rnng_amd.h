@@ -93,6 +93,26 @@ int rnng_dataset_wrap_device_f32(const float *dev_points, int64_t n, int32_t d,
 
 int rnng_dataset_destroy(rnng_dataset_t ds);
 
+/* Points and dimension of a dataset. */
+int rnng_dataset_shape(rnng_dataset_t ds, int64_t *n, int32_t *d);
+
+/* Copy a dataset's n*d floats back to the host. */
+int rnng_dataset_copy_f32(rnng_dataset_t ds, float *out);
+
+/* Point files straight into HBM: .fvecs (include/point.hpp:112-168), .fbin
+ * and .u8bin (dataset_io.py:48-70; u8 values become floats), chosen by
+ * extension as dataset_io.py's read_file does, points [start, start + count)
+ * (count < 0: to the end).  Chunks are read into pinned host buffers and
+ * uploaded asynchronously while the next chunk is read; u8bin travels as
+ * bytes and fvecs as records, converted on the device.  Errors:
+ * RNNG_ERR_FORMAT for an unknown extension, a header that does not match the
+ * file size or mixed fvecs dimensions; RNNG_ERR_ARG for a range beyond the
+ * file. */
+int rnng_dataset_read_file(const char *path, int64_t start, int64_t count, rnng_dataset_t *out);
+
+/* Header of a point file: points, dimension, kind (0 float32, 1 uint8). */
+int rnng_file_info(const char *path, int64_t *n, int32_t *d, int32_t *kind);
+
 /* ------------------------------------------------------------------------ */
 /* L2 epsilon graph                                                          */
 /* ------------------------------------------------------------------------ */

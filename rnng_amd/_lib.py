@@ -68,6 +68,10 @@ _SIGS = {
     "rnng_dataset_create_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "rnng_dataset_wrap_device_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "rnng_dataset_destroy": (C.c_int, [_vp]),
+    "rnng_dataset_shape": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+    "rnng_dataset_copy_f32": (C.c_int, [_vp, _vp]),
+    "rnng_dataset_read_file": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "rnng_file_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "rnng_graph_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_graph_build_bipartite": (C.c_int, [_vp, _vp, C.c_float, C.c_int32, C.POINTER(_vp),
                                              C.POINTER(Stats)]),

@@ -50,7 +50,7 @@ int main(int argc, char* argv[]) {
     }
     double t0 = wall(), t;
     PointContainer<float> points;
-    const Index n = points.read_fvecs(infile);
+    const Index n = points.read_file(infile);
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)n);
     t = wall();
     IndexVector landmarks;

@@ -27,7 +27,7 @@ static int run() {
     double t0 = wall(), t;
     PointContainer<Atom> points;
     Index n;
-    if constexpr (std::is_same<Atom, float>::value) n = points.read_fvecs(infile);
+    if constexpr (std::is_same<Atom, float>::value) n = points.read_file(infile);
     else n = points.read_seqs(infile);
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)n);
     t = wall();

@@ -44,6 +44,22 @@ class Dataset:
                                                  C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_handle(cls, h) -> "Dataset":
+        """Adopt a dataset handle created by the library (e.g. from a file)."""
+        self = cls.__new__(cls)
+        n, d = C.c_int64(), C.c_int32()
+        check(_lib.lib().rnng_dataset_shape(h, C.byref(n), C.byref(d)))
+        self.n, self.d = n.value, d.value
+        self._h = h
+        return self
+
+    def to_numpy(self) -> np.ndarray:
+        """The points copied back from HBM."""
+        out = np.empty((self.n, self.d), np.float32)
+        check(_lib.lib().rnng_dataset_copy_f32(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
     @property
     def handle(self):
         return self._h

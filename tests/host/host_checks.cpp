@@ -37,9 +37,9 @@ int main(int argc, char** argv) {
         g.write_file_stream(argv[3]);
         return 0;
     }
-    if (mode == "fvecs") {
+    if (mode == "fvecs" || mode == "file") {
         PointContainer<float> P;
-        const Index n = P.read_fvecs(argv[2]);
+        const Index n = mode == "file" ? P.read_file(argv[2]) : P.read_fvecs(argv[2]);
         std::cout << n << " " << P.dim() << "\n";
         for (Index i = 0; i < n; ++i) {
             const auto p = P[i];

@@ -67,6 +67,21 @@ def test_systolic_l2_file_3d(orc, tmp_path):
     assert out.read_text().splitlines() == _expected_graph_lines(pts.shape[0], ij, w)
 
 
+def test_systolic_fbin_and_u8bin_inputs(orc, tmp_path):
+    """Inputs named .fbin / .u8bin are read by their header (dataset_io.py
+    formats); the graph is the oracle's on the same points."""
+    from rnng_amd import dataset_io
+    X = np.random.default_rng(9).integers(0, 40, size=(3000, 6)).astype(np.uint8)
+    eps = np.float32(14.5)
+    ij, w = orc.l2_eps_pairs(X.astype(np.float32), eps)
+    for ext in (".fbin", ".u8bin"):
+        f = str(tmp_path / ("p" + ext))
+        dataset_io.write_file(f, X if ext == ".u8bin" else X.astype(np.float32))
+        out = tmp_path / ("g" + ext + ".txt")
+        _run([os.path.join(BIN, "rnng_systolic"), "-i", f, "-r", repr(float(eps)), "-o", str(out)])
+        assert out.read_text().splitlines() == _expected_graph_lines(X.shape[0], ij, w)
+
+
 def test_systolic_edit_file(orc, tmp_path):
     from rnng_amd import synth
     atoms, offs = synth.dna_mutants(n_seeds=40, per_seed=25)

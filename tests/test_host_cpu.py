@@ -82,6 +82,25 @@ def test_fvecs_reader(host, tmp_path):
         assert np.allclose(np.array(vals[1:], dtype=np.float64), X[i], rtol=1e-5)
 
 
+@pytest.mark.parametrize("ext", [".fbin", ".u8bin", ".fvecs"])
+def test_read_file_by_extension(host, tmp_path, ext):
+    """PointContainer::read_file: .fbin and .u8bin headers (uint32 n, d) and
+    bodies, u8 values as floats, anything else as fvecs."""
+    from rnng_amd import dataset_io
+    rng = np.random.default_rng(2)
+    X = rng.integers(0, 256, size=(9, 4))
+    if ext != ".u8bin":
+        X = X / 8.0   # exactly representable, printed exactly
+    p = str(tmp_path / ("x" + ext))
+    dataset_io.write_file(p, X.astype(np.uint8 if ext == ".u8bin" else np.float32))
+    out = subprocess.run([host, "file", p], capture_output=True, text=True, check=True).stdout.splitlines()
+    assert out[0] == "9 4"
+    for i, line in enumerate(out[1:]):
+        vals = line.split()
+        assert int(vals[0]) == i
+        assert np.array_equal(np.array(vals[1:], dtype=np.float64), X[i])
+
+
 def test_seqs_reader(host, tmp_path):
     p = tmp_path / "x.seqs"
     p.write_bytes(b"ACGT\r\n\nGATTACA\n\r\nTT\n")
