@@ -140,6 +140,16 @@ int rnng_graph_copy(rnng_graph_t g, int64_t *ij, float *w, int64_t cap, int32_t 
  * every vertex (the query of a point finds itself), sorted by (u, v).
  * *needed = 2 E + n_vertices; with cap < *needed and NULL outputs only the
  * size is reported (RNNG_OK), with non-NULL outputs RNNG_ERR_CAPACITY. */
+/* The same symmetric list as CSR, built on the device: rowptr[0..n_vertices]
+ * (row v = positions rowptr[v] .. rowptr[v+1]), col/w in (row, column)
+ * order, with the (v, v, 0) loops if self_loops != 0.  *needed = 2 E (+ n);
+ * rowptr may be requested alone; col/w need cap >= *needed
+ * (RNNG_ERR_CAPACITY otherwise).  An edge naming a vertex >= n_vertices is
+ * RNNG_ERR_ARG.  No reference counterpart: the reference keeps the edge list
+ * (include/graph.h:7-27); this is its symmetrised form for CSR consumers. */
+int rnng_graph_copy_csr(rnng_graph_t g, int64_t n_vertices, int32_t self_loops, int64_t *rowptr, int32_t *col,
+                        float *w, int64_t cap, int64_t *needed);
+
 int rnng_graph_copy_directed(rnng_graph_t g, int64_t n_vertices, int64_t *uv, float *w, int64_t cap,
                              int64_t *needed);
 

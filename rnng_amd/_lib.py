@@ -77,6 +77,8 @@ _SIGS = {
                                              C.POINTER(Stats)]),
     "rnng_graph_num_edges": (C.c_int, [_vp, _i64p]),
     "rnng_graph_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int32]),
+    "rnng_graph_copy_csr": (C.c_int, [_vp, C.c_int64, C.c_int32, _vp, _vp, _vp, C.c_int64,
+                                      C.POINTER(C.c_int64)]),
     "rnng_graph_copy_directed": (C.c_int, [_vp, C.c_int64, _vp, _vp, C.c_int64, _i64p]),
     "rnng_graph_destroy": (C.c_int, [_vp]),
     "rnng_l2_eps_pairs_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_float, C.c_int32, _vp, _vp,

@@ -903,6 +903,17 @@ __global__ void k_directed_keys(const int2* __restrict__ ij, const float* __rest
     }
 }
 
+// rowptr[r] = first position of row r in the sorted keys (r = 0..n), col = low words
+__global__ void k_csr_from_keys(const uint64_t* __restrict__ keys, int64_t total, int64_t n,
+                                int64_t* __restrict__ rowptr, int* __restrict__ col) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e > total) return;
+    const int64_t r = e < total ? (int64_t)(keys[e] >> 32) : n;
+    const int64_t pr = e > 0 ? (int64_t)(keys[e - 1] >> 32) : -1;
+    for (int64_t x = pr + 1; x <= r; ++x) rowptr[x] = e;
+    if (e < total) col[e] = (int)(keys[e] & 0xffffffffu);
+}
+
 __global__ void k_unpack_keys(const uint64_t* __restrict__ keys, int2* __restrict__ ij, int64_t n) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e < n) ij[e] = make_int2((int)(keys[e] >> 32), (int)(keys[e] & 0xffffffffu));
@@ -1659,6 +1670,55 @@ int rnng_graph_copy_directed(rnng_graph_t g, int64_t n_vertices, int64_t* uv, fl
             uv[2 * e] = (int64_t)(hk[(size_t)e] >> 32);
             uv[2 * e + 1] = (int64_t)(hk[(size_t)e] & 0xffffffffull);
         }
+    return RNNG_OK;
+}
+
+int rnng_graph_copy_csr(rnng_graph_t g, int64_t n_vertices, int32_t self_loops, int64_t* rowptr, int32_t* col,
+                        float* w, int64_t cap, int64_t* needed) {
+    if (!g || cap < 0 || n_vertices < 0 || n_vertices >= INT32_MAX) {
+        set_error("graph_copy_csr: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    const int64_t E = g->n_edges;
+    const int64_t loops = self_loops ? n_vertices : 0;
+    const int64_t total = 2 * E + loops;
+    if (needed) *needed = total;
+    if (cap < total && (col || w)) {
+        set_error("graph_copy_csr: output holds fewer entries than needed");
+        return RNNG_ERR_CAPACITY;
+    }
+    if (!rowptr && !col && !w) return RNNG_OK;
+    LocalBuf keys, ww, rp, cl;
+    RNNG_TRY(keys.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(uint64_t)));
+    RNNG_TRY(ww.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(float)));
+    RNNG_TRY(rp.reserve(((size_t)n_vertices + 1) * sizeof(int64_t)));
+    RNNG_TRY(cl.reserve((size_t)std::max<int64_t>(total, 1) * sizeof(int)));
+    if (total > 0) {
+        k_directed_keys<<<blocks_for(total, 256), 256, 0, stream()>>>(g->ij.as<int2>(), g->w.as<float>(), E, loops,
+                                                                      keys.as<uint64_t>(), ww.as<float>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(sort_pairs_u64_f32(keys.as<uint64_t>(), ww.as<float>(), total));
+        uint64_t last = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(&last, keys.as<uint64_t>() + (total - 1), sizeof(last), hipMemcpyDeviceToHost,
+                                    stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        if ((int64_t)(last >> 32) >= n_vertices) {
+            set_error("graph_copy_csr: an edge names a vertex >= n_vertices");
+            return RNNG_ERR_ARG;
+        }
+    }
+    k_csr_from_keys<<<blocks_for(total + 1, 256), 256, 0, stream()>>>(keys.as<uint64_t>(), total, n_vertices,
+                                                                      rp.as<int64_t>(), cl.as<int>());
+    RNNG_HIP_TRY(hipGetLastError());
+    if (rowptr)
+        RNNG_HIP_TRY(hipMemcpyAsync(rowptr, rp.ptr, ((size_t)n_vertices + 1) * sizeof(int64_t),
+                                    hipMemcpyDeviceToHost, stream()));
+    if (col && total)
+        RNNG_HIP_TRY(hipMemcpyAsync(col, cl.ptr, (size_t)total * sizeof(int), hipMemcpyDeviceToHost, stream()));
+    if (w && total)
+        RNNG_HIP_TRY(hipMemcpyAsync(w, ww.ptr, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }
 

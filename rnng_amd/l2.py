@@ -113,6 +113,21 @@ class Graph:
                                          w.ctypes.data_as(C.c_void_p), ne, int(sorted)))
         return ij, w
 
+    def csr(self, n_vertices: int, self_loops: bool = False):
+        """Symmetric CSR built on the device: (rowptr int64 [n+1], col int32,
+        w float32), columns ascending within each row."""
+        L = _lib.lib()
+        need = C.c_int64()
+        check(L.rnng_graph_copy_csr(self._h, int(n_vertices), int(self_loops), None, None, None, 0,
+                                    C.byref(need)))
+        rowptr = np.empty(int(n_vertices) + 1, np.int64)
+        col = np.empty(need.value, np.int32)
+        w = np.empty(need.value, np.float32)
+        check(L.rnng_graph_copy_csr(self._h, int(n_vertices), int(self_loops), rowptr.ctypes.data_as(C.c_void_p),
+                                    col.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p), need.value,
+                                    C.byref(need)))
+        return rowptr, col, w
+
     def close(self) -> None:
         if self._h:
             _lib.lib().rnng_graph_destroy(self._h)

@@ -173,3 +173,34 @@ def test_full_size_int_config_mfma_equals_plain():
     for e in rng.choice(ij1.shape[0], 200, replace=False):
         i, j = ij1[e]
         assert oracle.l2(pts[i], pts[j]) == w1[e]
+
+
+@pytest.mark.parametrize("loops", [False, True])
+def test_symmetric_csr(orc, loops):
+    """rnng_graph_copy_csr: rows = vertices, both orientations of every edge
+    (plus the zero loops if asked), columns ascending, weights with them;
+    expanded it equals the directed copy's list."""
+    from rnng_amd import _lib
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((2500, 8)).astype(F)
+    X[100] = X[7]
+    eps = F(2.2)
+    ij_ref, w_ref = orc.l2_eps_pairs(X, eps)
+    n = X.shape[0]
+    u = np.concatenate([ij_ref[:, 0], ij_ref[:, 1]] + ([np.arange(n)] if loops else []))
+    v = np.concatenate([ij_ref[:, 1], ij_ref[:, 0]] + ([np.arange(n)] if loops else []))
+    w = np.concatenate([w_ref, w_ref] + ([np.zeros(n, F)] if loops else []))
+    o = np.lexsort((v, u))
+    ds = _l2().Dataset(X)
+    g = ds.graph(eps, "mfma")
+    rowptr, col, ww = g.csr(n, loops)
+    assert rowptr[0] == 0 and rowptr[-1] == u.shape[0]
+    assert np.array_equal(np.repeat(np.arange(n), np.diff(rowptr)), u[o])
+    assert np.array_equal(col, v[o])
+    assert ww.tobytes() == w[o].tobytes()
+    # an edge beyond n_vertices is refused
+    with pytest.raises(_lib.RnngError) as ei:
+        g.csr(50, loops)
+    assert ei.value.code == _lib.RNNG_ERR_ARG
+    g.close()
+    ds.close()
