@@ -164,10 +164,10 @@ __global__ void k_debug_mfma_i8(const i32x4_t* __restrict__ a, const i32x4_t* __
 // -2 <t(p_a), t(p_b)>, so its sign bit is set iff L1_4 <= 6k.  The hot loop
 // ORs the sign bits; tiles with a set bit take the candidate path, which
 // keeps a < b < n (sorted positions) inside the length window and stages
-// (a, b) in LDS.  It reads no memory: on this filter most tiles hold a
+// (a, b) in LDS.  It reads no global memory: on this filter most tiles hold a
 // candidate, and a load per set bit would stall the whole workgroup at the
-// next barrier, so row lengths sit in the wave's lanes and column lengths
-// are fetched a tile ahead.
+// next barrier, so row lengths sit in LDS and column lengths are fetched a
+// tile ahead.
 __global__ __launch_bounds__(256) void k_edit_filter(
     const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ nrm,
     const int* __restrict__ slen, const int4* __restrict__ work, int n, int k, int bias,
@@ -175,6 +175,7 @@ __global__ __launch_bounds__(256) void k_edit_filter(
     constexpr int NCH = EKS * 64;   // 16-byte fragments per column tile
     __shared__ i32x4_t bs[2][NCH];
     __shared__ int2 stage[ESTAGE];
+    __shared__ int srl[4][64];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
     const int4 wi = work[blockIdx.x];
@@ -195,9 +196,9 @@ __global__ __launch_bounds__(256) void k_edit_filter(
 #pragma unroll
         for (int q = 0; q < 16; ++q) xr[r][q] = nrm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
     int yc = nrm[wi.y * 32 + (lane & 31)] + bias;
-    // lane l keeps the length of row rt0*32 + l (the wave's 64 rows), and each
-    // lane the length of its column, fetched a tile ahead
-    const int rlen = rt0 * 32 + lane < n ? slen[rt0 * 32 + lane] : 0;
+    // the lengths of the wave's 64 rows sit in LDS (read by the candidate
+    // path), each lane keeps the length of its column, fetched a tile ahead
+    srl[wave][lane] = rt0 * 32 + lane < n ? slen[rt0 * 32 + lane] : 0;
     int clen = wi.y * 32 + (lane & 31) < n ? slen[wi.y * 32 + (lane & 31)] : 0;
     bs[0][tid] = yb[(size_t)wi.y * NCH + tid];
     bs[0][tid + 256] = yb[(size_t)wi.y * NCH + tid + 256];
@@ -233,8 +234,11 @@ __global__ __launch_bounds__(256) void k_edit_filter(
             for (int q = 0; q < 16; ++q) o |= acc[r][q];
 
         if (__any(o < 0)) {
-            // Rare path.  The mask's shifts depend on an opaque zero, so the
-            // compiler cannot hoist them into every tile of the hot loop.
+            // Candidate path: each lane walks its own set sign bits (a short
+            // divergent loop: a tile holds a few survivors), reads the row's
+            // length from LDS and takes stage slots with LDS atomics.  The
+            // mask's shifts depend on an opaque zero, so the compiler cannot
+            // hoist them into every tile of the hot loop.
             int opaque;
             asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
             unsigned mask = 0;
@@ -243,33 +247,23 @@ __global__ __launch_bounds__(256) void k_edit_filter(
 #pragma unroll
                 for (int q = 0; q < 16; ++q) mask |= ((unsigned)acc[r][q] >> 31) << (r * 16 + q + opaque);
             const int col = ct * 32 + (lane & 31);
-            const int rowb = rt0 * 32 + 4 * (lane >> 5);
-            unsigned any = mask;
-#pragma unroll
-            for (int o2 = 1; o2 < 64; o2 <<= 1) any |= (unsigned)__shfl_xor((int)any, o2, 64);
-            any = __builtin_amdgcn_readfirstlane(any);
+            const int rowb = 4 * (lane >> 5);
+            if (col >= n) mask = 0;
 #pragma unroll 1
-            while (any) {
-                const int rq = __builtin_ctz(any);
-                any &= any - 1;
+            while (mask) {
+                const int rq = __builtin_ctz(mask);
+                mask &= mask - 1;
                 const int q = rq & 15;
-                const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
-                const int rl = __shfl(rlen, row - rt0 * 32, 64);
-                const bool p = ((mask >> rq) & 1u) && row < col && col < n && clen - rl <= k;
-                const unsigned long long m = __ballot(p);
-                if (m) {
-                    unsigned slot = 0;
-                    if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
-                    slot = __shfl(slot, 0, 64) +
-                           __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    if (p) {
-                        const int2 pr = make_int2(row, col);
-                        if (slot < ESTAGE) {
-                            stage[slot] = pr;
-                        } else {
-                            const unsigned long long gslot = atomicAdd(counter, 1ull);
-                            if (gslot < cap) cand[gslot] = pr;
-                        }
+                const int rl_i = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);   // row within the wave
+                const int row = rt0 * 32 + rl_i;
+                if (row < col && clen - srl[wave][rl_i] <= k) {
+                    const unsigned slot = atomicAdd(&scount, 1u);
+                    const int2 pr = make_int2(row, col);
+                    if (slot < ESTAGE) {
+                        stage[slot] = pr;
+                    } else {
+                        const unsigned long long gslot = atomicAdd(counter, 1ull);
+                        if (gslot < cap) cand[gslot] = pr;
                     }
                 }
             }
