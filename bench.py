@@ -121,6 +121,13 @@ def bench_cells(args):
         "stage_ms": {"ghost_test": st.ghost_ms / K, "prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck_dedup": st.recheck_ms / K, "device_total": st.total_ms / K},
     }
+    # on this workload every cell pair is adjacent, so the algorithmic work is
+    # every unordered pair once at 2 d flops (the padded tiles the filter runs
+    # are cell_pair_evals)
+    ach = 2.0 * d * P_all / (st.filter_ms / K * 1e-3) / 1e12
+    out["roofline"] = {"kernel": "k_l2_mfma_filter (autotuned variant)", "bound": "mfma", "achieved": ach,
+                       "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0, "traffic": None,
+                       "work": "2 d flops per unordered pair n(n-1)/2, fill-pass filter time from HIP events"}
     v.close()
     ds.close()
     print(json.dumps(out), flush=True)
@@ -158,12 +165,21 @@ def bench_edit(args):
         g.close()
     dt = (time.perf_counter() - t0) / args.steps
     K = args.steps
+    # algorithmic work of the filter: 256 int8 multiply-adds (64 bins x 4
+    # thermometer levels) per pair inside the length window |m - n| <= k
+    lens = np.sort(np.diff(offs))
+    win = int((np.searchsorted(lens, lens + int(eps), side="right") - np.arange(1, n + 1)).sum())
+    filt_ms = st.filter_ms / K
+    achieved = 2.0 * 256 * win / (filt_ms * 1e-3) / 1e12
     out = _secondary_line(
         "edit-distance ε-graph pair-evals/s and edges/s, 200k strings", st.pair_evals / K / dt, "pair-evals/s",
         K, args.warmup, dt * 1e3, f"edit-distance eps-graph, {n} DNA-like strings (length 100-150), eps=8",
         {"dtype": "int32", "edges": ne, "edges_per_s": ne / dt, "pair_evals": st.pair_evals // K,
-         "candidates": st.candidates // K,
-         "stage_ms": {"filter": st.filter_ms / K, "verify": st.recheck_ms / K, "device_total": st.total_ms / K}})
+         "length_window_pairs": win, "candidates": st.candidates // K,
+         "stage_ms": {"filter": filt_ms, "verify": st.recheck_ms / K, "device_total": st.total_ms / K},
+         "roofline": {"kernel": "k_edit_filter", "bound": "mfma", "achieved": achieved, "peak": 5000.0,
+                      "unit": "TOP/s (int8)", "frac": achieved / 5000.0, "traffic": None,
+                      "work": "2 x 256 int8 ops per length-window pair, filter time from HIP events"}})
     if not args.no_cpu:
         import oracle
         cores = min(len(os.sched_getaffinity(0)), 16)
