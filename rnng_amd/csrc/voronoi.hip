@@ -230,10 +230,16 @@ __global__ void k_gather_rows(const float* __restrict__ X, int d, const int* __r
 // with L2(c_k, c_a) >= 2 mind_i (1 + 1e-4) are skipped without reading their
 // row: the margin is far above the FP32 error of the three distances
 // (~(d + 2) 2^-24 relative each), so a skipped point's exact distance is > mind_i.
-constexpr int FPS_K = 16;   // dims per LDS round
 
 // Dc[a] = L2(c_k, c_a) for a < k (any summation order: only a bound), and
 // the new center's row copied to C[k].
+// Round-to-nearest-even BF16 bits of x (NaN stays NaN).
+__device__ __forceinline__ unsigned bf16_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
 __global__ __launch_bounds__(256) void k_fps_center(const float* __restrict__ X, int d, const int* __restrict__ ctr,
                                                     int k, float* __restrict__ C, float* __restrict__ Dc) {
     const float* c = X + (size_t)ctr[k] * d;
@@ -251,20 +257,42 @@ __global__ __launch_bounds__(256) void k_fps_center(const float* __restrict__ X,
     }
 }
 
-__global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X, int n, int d,
-                                                    const int* __restrict__ ctr, int k, float* __restrict__ mind,
-                                                    int* __restrict__ cell, const float* __restrict__ Dc,
+// XT is the points transposed (dimension-major, row stride ldt): lane i
+// reads XT[k*ldt + i] for k = 0..d-1, so every dimension is one coalesced
+// stream across the wave, and the FP32 sum keeps the reference's order.
+//
+// Before the exact sum, a point is screened with BF16 copies: XB packs the
+// rounded coordinates two per word (dimension pairs, same layout), ex[i]
+// bounds |bf16(x_i) - x_i| from above.  By the triangle inequality
+// |x - c| >= |bf16(x) - bf16(c)| - ex[x] - ex[c]; when that lower bound
+// exceeds md with a 2^-13 relative margin (every FP32 rounding of the screen
+// and of the reference-order sum, d <= 144) and a 1e-20 absolute one (the
+// reference's squares may underflow: each loses at most 2^-150, so its
+// distance is at least the true one minus sqrt(d 2^-149) < 5e-22), the
+// reference distance cannot be <= md and the point keeps its cell: the
+// result is the exact one.
+__global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X, const float* __restrict__ XT,
+                                                    const unsigned* __restrict__ XB, const float* __restrict__ ex,
+                                                    int64_t ldt, int n, int d, const int* __restrict__ ctr, int k,
+                                                    float* __restrict__ mind, int* __restrict__ cell,
+                                                    const float* __restrict__ Dc,
                                                     unsigned long long* __restrict__ part) {
 #pragma clang fp contract(off)
-    extern __shared__ float fps_lds[];   // [d] center, then [4 waves][FPS_K][65]
+    extern __shared__ float fps_lds[];   // [d] center, then [(d+1)/2] its packed BF16 pairs
     float* cs = fps_lds;
+    unsigned* cb = reinterpret_cast<unsigned*>(fps_lds + d);
+    const int d2 = (d + 1) / 2;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    float* tile = fps_lds + ((d + 3) & ~3) + wave * FPS_K * 65;
     const int c = ctr[k];
     for (int t = tid; t < d; t += 256) cs[t] = X[(size_t)c * d + t];
+    for (int t = tid; t < d2; t += 256) {
+        const unsigned lo = bf16_bits(X[(size_t)c * d + 2 * t]);
+        const unsigned hi = 2 * t + 1 < d ? bf16_bits(X[(size_t)c * d + 2 * t + 1]) : 0u;
+        cb[t] = lo | (hi << 16);
+    }
+    const float ec = ex[c];
     __syncthreads();
-    const int p0 = blockIdx.x * 256 + wave * 64;
-    const int i = p0 + lane;
+    const int i = blockIdx.x * 256 + tid;
     float md = FLT_MAX;
     int cl = 0;
     bool need = false;
@@ -277,45 +305,27 @@ __global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X,
             need = !((double)Dc[cl] >= 2.0 * (double)md * (1.0 + 1e-4) + 1e-30);
         }
     }
-    const unsigned long long nm = __ballot(need);
-    float acc = 0.0f;
-    if (__popcll(nm) > 16) {
-        // most rows of the wave are needed: 64 rows x FPS_K dims through LDS,
-        // 16 consecutive lanes per row segment
-        for (int k0 = 0; k0 < d; k0 += FPS_K) {
-#pragma unroll
-            for (int j = 0; j < FPS_K; ++j) {
-                const int e = lane + 64 * j;
-                const int r = e / FPS_K, kk = e % FPS_K;
-                const int row = p0 + r, kd = k0 + kk;
-                tile[kk * 65 + r] = (row < n && kd < d) ? X[(size_t)row * d + kd] : 0.0f;
-            }
-            // rows are written and read by lanes of this wave only
-            __builtin_amdgcn_wave_barrier();
-            const int kend = min(FPS_K, d - k0);
-            for (int kk = 0; kk < kend; ++kk) {
-                const float t = tile[kk * 65 + lane] - cs[k0 + kk];
-                acc = acc + t * t;
-            }
-            __builtin_amdgcn_wave_barrier();
+    if (need && k > 0) {
+        float sq = 0.0f;
+        const unsigned* xb = XB + i;
+#pragma unroll 8
+        for (int kk = 0; kk < d2; ++kk) {
+            const unsigned w = xb[(size_t)kk * ldt], cw = cb[kk];
+            const float a0 = __uint_as_float(w << 16) - __uint_as_float(cw << 16);
+            const float a1 = __uint_as_float(w & 0xffff0000u) - __uint_as_float(cw & 0xffff0000u);
+            sq = fmaf(a0, a0, sq);
+            sq = fmaf(a1, a1, sq);
         }
-    } else if (need) {
-        // few rows needed: each lane reads its own row
-        const float* x = X + (size_t)i * d;
-        if ((d & 3) == 0) {
-            for (int j = 0; j < d; j += 4) {
-                const float4 q = *(const float4*)(x + j);
-                float t;
-                t = q.x - cs[j];     acc = acc + t * t;
-                t = q.y - cs[j + 1]; acc = acc + t * t;
-                t = q.z - cs[j + 2]; acc = acc + t * t;
-                t = q.w - cs[j + 3]; acc = acc + t * t;
-            }
-        } else {
-            for (int j = 0; j < d; ++j) {
-                const float t = x[j] - cs[j];
-                acc = acc + t * t;
-            }
+        const float lower = sqrtf(sq) * (1.0f - 0x1p-13f) - ex[i] - ec;
+        if (isfinite(sq) && lower * (1.0f - 0x1p-13f) - 1e-20f > md) need = false;
+    }
+    float acc = 0.0f;
+    if (need) {
+        const float* xt = XT + i;
+#pragma unroll 8
+        for (int kk = 0; kk < d; ++kk) {
+            const float t = xt[(size_t)kk * ldt] - cs[kk];
+            acc = acc + t * t;
         }
     }
     unsigned long long key = 0ull;
@@ -345,6 +355,46 @@ __global__ __launch_bounds__(256) void k_fps_update(const float* __restrict__ X,
         for (int w = 1; w < 4; ++w) b = wk[w] > b ? wk[w] : b;
         part[blockIdx.x] = b;
     }
+}
+
+// XT[k*ldt + i] = X[i*d + k] and XB[p*ldt + i] = bf16 pair (2p, 2p+1) of
+// point i: 32 x 32 tiles through LDS (coalesced both ways)
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ X, int n, int d,
+                                                   float* __restrict__ XT, unsigned* __restrict__ XB, int64_t ldt) {
+    __shared__ float t[32][33];
+    const int i0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // ty 0..7
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, kk = k0 + tx;
+        t[r][tx] = (i < n && kk < d) ? X[(size_t)i * d + kk] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int kk = k0 + r, i = i0 + tx;
+        if (kk < d && i < n) XT[(size_t)kk * ldt + i] = t[tx][r];
+    }
+    for (int r = ty; r < 16; r += 8) {
+        const int p = k0 / 2 + r, i = i0 + tx;
+        if (2 * p < d && i < n) XB[(size_t)p * ldt + i] = bf16_bits(t[tx][2 * r]) | (bf16_bits(t[tx][2 * r + 1]) << 16);
+    }
+}
+
+// ex[i] >= |bf16(x_i) - x_i|: each coordinate difference is exact in FP32,
+// the sum runs in FP64 (no underflow of tiny differences) and the result is
+// rounded up to float.
+__global__ void k_bf16_error(const float* __restrict__ X, int n, int d, float* __restrict__ ex) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* x = X + (size_t)i * d;
+    double s = 0.0;
+    for (int k = 0; k < d; ++k) {
+        const double e = (double)(__uint_as_float(bf16_bits(x[k]) << 16) - x[k]);
+        s += e * e;
+    }
+    const double v = sqrt(s) * (1.0 + 1e-12);
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    ex[i] = f;
 }
 
 __global__ __launch_bounds__(1024) void k_fps_select(const unsigned long long* __restrict__ part, int nb,
@@ -725,15 +775,26 @@ int fps(rnng_voronoi_s* v, int first) {
     LocalBuf part;
     RNNG_TRY(part.reserve((size_t)nb * sizeof(unsigned long long)));
     RNNG_HIP_TRY(hipMemcpyAsync(v->ctr.ptr, &first, sizeof(int), hipMemcpyHostToDevice, stream()));
-    const size_t lds = (size_t)(((d + 3) & ~3) + 4 * FPS_K * 65) * sizeof(float);
-    LocalBuf dc;
+    const size_t lds = (size_t)(d + (d + 1) / 2) * sizeof(float);
+    LocalBuf dc, xt, xb, ex;
     RNNG_TRY(v->C.reserve((size_t)m * d * sizeof(float)));
     RNNG_TRY(dc.reserve((size_t)m * sizeof(float)));
+    // dimension-major FP32 and packed BF16 copies of the points for the
+    // per-landmark sweeps, and each point's BF16 rounding error
+    const int64_t ldt = ((int64_t)n + 63) / 64 * 64;
+    RNNG_TRY(xt.reserve((size_t)ldt * d * sizeof(float)));
+    RNNG_TRY(xb.reserve((size_t)ldt * ((d + 1) / 2) * sizeof(unsigned)));
+    RNNG_TRY(ex.reserve((size_t)std::max(n, 1) * sizeof(float)));
+    k_transpose<<<dim3(blocks_for(n, 32), blocks_for(d, 32)), 256, 0, stream()>>>(v->ds->X, n, d, xt.as<float>(),
+                                                                                xb.as<unsigned>(), ldt);
+    k_bf16_error<<<blocks_for(n, 256), 256, 0, stream()>>>(v->ds->X, n, d, ex.as<float>());
     for (int k = 0; k < m; ++k) {
         k_fps_center<<<blocks_for(std::max(k, 1), 256), 256, 0, stream()>>>(v->ds->X, d, v->ctr.as<int>(), k,
                                                                             v->C.as<float>(), dc.as<float>());
-        k_fps_update<<<nb, 256, lds, stream()>>>(v->ds->X, n, d, v->ctr.as<int>(), k, v->delta.as<float>(),
-                                                 v->cell.as<int>(), dc.as<float>(), part.as<unsigned long long>());
+        k_fps_update<<<nb, 256, lds, stream()>>>(v->ds->X, xt.as<float>(), xb.as<unsigned>(), ex.as<float>(), ldt,
+                                                 n, d, v->ctr.as<int>(), k,
+                                                 v->delta.as<float>(), v->cell.as<int>(), dc.as<float>(),
+                                                 part.as<unsigned long long>());
         if (k + 1 < m)
             k_fps_select<<<1, 1024, 0, stream()>>>(part.as<unsigned long long>(), nb, v->ctr.as<int>(), k);
     }

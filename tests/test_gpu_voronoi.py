@@ -60,6 +60,30 @@ def test_fps_matches_oracle(orc, n, d, m, first):
     ds.close()
 
 
+@pytest.mark.parametrize("scale", [1e-22, 3e-3, 1e17])
+def test_fps_screen_extreme_scales(orc, scale):
+    """The BF16 screen of the farthest-point sweep is a proven bound: with
+    coordinates so small that BF16 differences underflow when squared in
+    FP32, or so large that FP32 squares approach overflow, and with integer
+    data full of exact ties, the landmarks, cells and distances equal the
+    exact oracle's."""
+    l2, vor = _mods()
+    rng = np.random.default_rng(5)
+    pts = (rng.standard_normal((6000, 24)) * scale).astype(F)
+    ints = rng.integers(0, 4, size=(3000, 8)).astype(F)
+    for X, m in ((pts, 200), (ints, 40)):
+        flush_print(f"fps screen scale={scale} n={X.shape[0]} d={X.shape[1]}")
+        ds = l2.Dataset(X)
+        v = vor.Voronoi(ds, fps=m, first=1)
+        c_ref, cell_ref, dist_ref = orc.fps(X, m, 1)
+        assert np.array_equal(v.centers(), c_ref)
+        cell, dist = v.assignment()
+        assert np.array_equal(cell, cell_ref)
+        assert dist.tobytes() == dist_ref.tobytes()
+        v.close()
+        ds.close()
+
+
 def _graph_case(orc, pts, centers, eps, check_ghosts=True):
     l2, vor = _mods()
     ds = l2.Dataset(pts)
