@@ -12,6 +12,12 @@ no oracle build requirements beyond this repository) checks against data.
   simplex_ids.json     combinatorial-number-system ids, closed form
   gmm_small.json       the 10,000 x 16 workload: eps, edge count and SHA-256
                        of the sorted (i, j) int64 pairs and float32 weights
+  edit_small.json      600 DNA-like strings (the 200k-string generator at 6
+                       seeds x 100 mutants), eps 8: edge count and SHA-256 of
+                       the sorted pairs and distances
+  rips_small.json      the Rips generator at 5,000 points (eps 0.3, maxdim 2)
+                       and 1,500 points (eps 0.35, maxdim 3): simplex counts
+                       and SHA-256 of the (value, id)-sorted filtration
 """
 import hashlib
 import json
@@ -76,6 +82,27 @@ def main():
           "sha256_w_float32": hashlib.sha256(np.ascontiguousarray(w, np.float32).tobytes()).hexdigest(),
           "sha256_points_float32": hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest()}
     json.dump(gm, open(os.path.join(HERE, "gmm_small.json"), "w"), indent=1)
+
+    atoms, offs = synth.dna_mutants(n_seeds=6, per_seed=100)
+    ij, dist = oracle.edit_eps_pairs(atoms, offs, 8)
+    json.dump({"n_seeds": 6, "per_seed": 100, "eps": 8, "strings": int(offs.shape[0] - 1), "edges": int(ij.shape[0]),
+               "sha256_ij_int64": hashlib.sha256(np.ascontiguousarray(ij, np.int64).tobytes()).hexdigest(),
+               "sha256_dist_float32": hashlib.sha256(np.ascontiguousarray(dist, np.float32).tobytes()).hexdigest(),
+               "sha256_atoms": hashlib.sha256(np.ascontiguousarray(atoms, np.uint8).tobytes()).hexdigest()},
+              open(os.path.join(HERE, "edit_small.json"), "w"), indent=1)
+
+    rips = []
+    for n, maxdim, eps in ((5000, 2, 0.3), (1500, 3, 0.35)):
+        pts, _ = synth.points_rips(n=n)
+        eps = np.float32(eps)
+        val, ids = oracle.rips_filtration(pts, eps, maxdim)
+        dims = (ids >> 60) & 0xF
+        rips.append({"n": n, "maxdim": maxdim, "eps_bits": int(np.float32(eps).view(np.uint32)),
+                     "per_dim": [int((dims == p).sum()) for p in range(maxdim + 1)],
+                     "sha256_value_float32": hashlib.sha256(np.ascontiguousarray(val, np.float32).tobytes()).hexdigest(),
+                     "sha256_id_int64": hashlib.sha256(np.ascontiguousarray(ids, np.int64).tobytes()).hexdigest(),
+                     "sha256_points_float32": hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest()})
+    json.dump(rips, open(os.path.join(HERE, "rips_small.json"), "w"), indent=1)
     print("fixtures written")
 
 

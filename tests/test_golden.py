@@ -102,3 +102,48 @@ def test_l2_known_pairs_device():
             assert ij.tolist() == [[0, 0]] and w[0] == eps
             ij, _ = l2.eps_bipartite(a, b, np.nextafter(eps, np.float32(0)), mode)
             assert ij.shape[0] == (1 if p["sq"] == 0 else 0)
+
+
+def test_edit_small_oracle(orc):
+    from rnng_amd import synth
+    g = _load("edit_small.json")
+    atoms, offs = synth.dna_mutants(n_seeds=g["n_seeds"], per_seed=g["per_seed"])
+    assert hashlib.sha256(np.ascontiguousarray(atoms, np.uint8).tobytes()).hexdigest() == g["sha256_atoms"]
+    ij, d = orc.edit_eps_pairs(atoms, offs, g["eps"])
+    assert ij.shape[0] == g["edges"]
+    assert _sha(ij, np.int64) == g["sha256_ij_int64"] and _sha(d, np.float32) == g["sha256_dist_float32"]
+
+
+@pytest.mark.gpu
+def test_edit_small_device_matches_fixture():
+    from rnng_amd import edit, synth
+    g = _load("edit_small.json")
+    atoms, offs = synth.dna_mutants(n_seeds=g["n_seeds"], per_seed=g["per_seed"])
+    ij, d = edit.eps_pairs(atoms, offs, g["eps"])
+    assert ij.shape[0] == g["edges"]
+    assert _sha(ij, np.int64) == g["sha256_ij_int64"] and _sha(d, np.float32) == g["sha256_dist_float32"]
+
+
+def _rips_case(r):
+    from rnng_amd import synth
+    pts, _ = synth.points_rips(n=r["n"])
+    assert _sha(pts, np.float32) == r["sha256_points_float32"]
+    return pts, np.uint32(r["eps_bits"]).view(np.float32)
+
+
+def test_rips_small_oracle(orc):
+    for r in _load("rips_small.json"):
+        pts, eps = _rips_case(r)
+        val, ids = orc.rips_filtration(pts, eps, r["maxdim"])
+        dims = (ids >> 60) & 0xF
+        assert [int((dims == p).sum()) for p in range(r["maxdim"] + 1)] == r["per_dim"]
+        assert _sha(val, np.float32) == r["sha256_value_float32"] and _sha(ids, np.int64) == r["sha256_id_int64"]
+
+
+@pytest.mark.gpu
+def test_rips_small_device_matches_fixture():
+    from rnng_amd import rips
+    for r in _load("rips_small.json"):
+        pts, eps = _rips_case(r)
+        val, ids = rips.rips_filtration(pts, eps, r["maxdim"])
+        assert _sha(val, np.float32) == r["sha256_value_float32"] and _sha(ids, np.int64) == r["sha256_id_int64"]
