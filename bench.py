@@ -3,14 +3,17 @@
 Workload: 1,000,000 points x 128 dims, float32, integer-valued in [0,255]
 (256-component Gaussian mixture, sigma 20, clipped and rounded, seeded),
 eps set for mean degree ~30.  One step = one full epsilon-graph build over
-the resident points: BF16 fragment preparation, the MFMA candidate filter
-over all 499,999,500,000 unordered pairs, the FP32 reference-order recheck
-and FP64 band check.  The plain FP32 kernel (reference order, every pair) is
+the resident points: fragment preparation, the MFMA candidate filter over all
+499,999,500,000 unordered pairs, the FP32 reference-order recheck and FP64
+band check.  The plain FP32 kernel (reference order, every pair) is
 timed once beside it.
 
 Prints one JSON line (rank 0).  The rates are pair-evaluations per second
 (value) and edges per second; roofline is for the MFMA filter kernel, whose
-per-launch duration is measured with HIP events on the library's stream.
+per-launch duration is measured with HIP events on the library's stream.  On
+this integer-valued data (every dimension spans at most 255) the filter runs
+exactly on int8 MFMA and its roofline is the dense int8 peak; the BF16 filter
+is timed beside it.
 """
 import argparse
 import json
@@ -24,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_INT8_TOPS = 5000.0
 
 
 def log(msg):
@@ -308,6 +312,10 @@ def main():
     filt_s = st.filter_ms / K / 1e3
     flops = 2.0 * d * P
     achieved = flops / filt_s / 1e12
+    # integer data spanning <= 255 per dimension runs the exact int8 filter
+    # (v_mfma_i32_32x32x32_i8): its roofline is the dense int8 peak
+    int8 = st.int8_filter == K
+    peak = PEAK_INT8_TOPS if int8 else PEAK_BF16_TFLOPS
     out = {
         "metric": "ε-graph edges/s + pair-evals/s at 1M×128 fp32, MFMA-filter vs plain, % peak",
         "value": P / (dt / K),
@@ -319,7 +327,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 mfma (fp32 accumulate) + fp32 recheck",
+        "dtype": ("int8 mfma (int32 accumulate, exact on this integer data) + fp32 recheck" if int8
+                  else "bf16 mfma (fp32 accumulate) + fp32 recheck"),
         "data": "synthetic (seeded 256-component Gaussian mixture, integer-valued)",
         "config": {"workload": f"eps-graph self join, {n} points x {d} dims fp32 integer-valued "
                                f"[0,255], eps={float(eps):.6g} (mean degree ~30)",
@@ -331,10 +340,28 @@ def main():
         "band_flips": st.band_flips // K,
         "stage_ms": {"prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck": st.recheck_ms / K, "device_total": st.total_ms / K},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
-                     "traffic": measured_traffic("l2_mfma_filter_gmm_int_1Mx128.md", n)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TOP/s (int8)" if int8 else "TFLOP/s", "frac": achieved / peak,
+                     "traffic": measured_traffic("l2_mfma_filter_i8_gmm_int_1Mx128.md" if int8
+                                                 else "l2_mfma_filter_gmm_int_1Mx128.md", n)},
     }
+
+    if int8 and not args.no_plain:
+        # the BF16 filter on the same points, for comparison (same edges)
+        _lib.check(_lib.lib().rnng_set_int8_filter(0))
+        try:
+            sb = Stats()
+            for _ in range(2):
+                g = ds.graph(eps, "mfma", sb)
+                be = g.num_edges
+                g.close()
+            bf = sb.filter_ms / 2
+            out["bf16_filter"] = {"filter_ms": bf, "achieved_tflops": flops / (bf / 1e3) / 1e12,
+                                  "frac_of_bf16_peak": flops / (bf / 1e3) / 1e12 / PEAK_BF16_TFLOPS,
+                                  "edges_equal": be == ne}
+            log(f"bf16 filter: {bf:.2f} ms, {be} edges")
+        finally:
+            _lib.check(_lib.lib().rnng_set_int8_filter(1))
 
     if not args.no_plain:
         sp = Stats()

@@ -37,7 +37,9 @@ extern "C" {
 
 /* Distance-stage algorithm. */
 #define RNNG_MODE_PLAIN 0   /* FP32 tiled kernel in the reference's order */
-#define RNNG_MODE_MFMA  1   /* BF16 MFMA candidate filter + FP32 recheck + FP64 band check */
+#define RNNG_MODE_MFMA  1   /* MFMA candidate filter (BF16, or exact int8 when every coordinate is an
+                               integer and each dimension spans <= 255) + FP32 recheck + FP64 band
+                               check */
 #define RNNG_MODE_GRID  2   /* self join, d <= 3: uniform grid of cells of side >= eps; only
                                pairs in adjacent cells are evaluated, each in the reference's
                                FP32 order (same edges as PLAIN); pair_evals counts them */
@@ -62,6 +64,7 @@ typedef struct rnng_stats {
     int64_t filter_passes;    /* Voronoi: filter launches (2 for count-then-fill) */
     int64_t predicted_candidates; /* Voronoi: cost model's candidate estimate (0 without) */
     int64_t predicted_edges;      /* Voronoi: cost model's edge estimate (0 without) */
+    int64_t int8_filter;          /* MFMA builds that ran the exact int8 filter (integer data) */
 } rnng_stats;
 
 /* Opaque handles. */
@@ -354,6 +357,11 @@ float rnng_sq_threshold(float eps);
  * indices and reports, for each (lane, accumulator register), the output
  * row and column it holds: rows[lane*16 + r], cols[lane*16 + r]. */
 int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
+
+/* MFMA mode on integer-valued data whose dimensions each span at most 255
+ * uses the exact int8 filter unless disabled here (enable = 0: always BF16).
+ * Same edges either way; a switch for comparisons. */
+int rnng_set_int8_filter(int32_t enable);
 
 /* Runs one 32x32x32 int8 MFMA (the edit filter's) on caller data: a and b
  * hold 16 int8 per lane (64 lanes), d gets the 16 int32 accumulators of

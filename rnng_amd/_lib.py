@@ -52,6 +52,7 @@ class Stats(C.Structure):
         ("filter_passes", C.c_int64),
         ("predicted_candidates", C.c_int64),
         ("predicted_edges", C.c_int64),
+        ("int8_filter", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -118,6 +119,7 @@ _SIGS = {
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
     "rnng_debug_mfma_i8": (C.c_int, [_vp, _vp, _vp]),
+    "rnng_set_int8_filter": (C.c_int, [C.c_int32]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_debug_copy": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
 }

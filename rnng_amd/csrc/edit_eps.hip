@@ -87,9 +87,6 @@ __global__ __launch_bounds__(256) void k_profiles(const uint8_t* __restrict__ at
     }
 }
 
-typedef int i32x4_t __attribute__((ext_vector_type(4)));
-typedef int i32x16_t __attribute__((ext_vector_type(16)));
-
 constexpr int TLEV = 4;              // thermometer levels per 3-gram bin (counts clamped at 4)
 constexpr int EKS = 64 * TLEV / 32;  // int8 k-steps of 32 per pair
 constexpr int ERT = 2;               // row tiles (32 strings) per wave

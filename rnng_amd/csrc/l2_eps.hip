@@ -24,6 +24,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -792,6 +793,194 @@ constexpr int RC = 16;   // candidates per wave per round
 
 __host__ __device__ constexpr int recheck_ld(int d) { return ((d + 3) & ~3) + 4; }
 
+// ------------------------------------------------------------------------
+// Integer data: exact int8 MFMA filter
+// ------------------------------------------------------------------------
+// When every coordinate is an integer and every dimension spans at most 255
+// (o_k = the dimension's minimum), x' = x - o - 128 lies in [-128, 127] and
+// the squared distance |x - y|^2 = h_x + h_y - 2 x'.y' (h = |x'|^2) is an
+// exact integer below 2^24, which is also what the reference's FP32 sum
+// gives; an edge is d^2 <= T' = floor(T(eps)).  Rows carry x' and columns
+// -1 - y' = 127 - (y - o), both int8, so one v_mfma_i32_32x32x32_i8 step
+// adds A.B = -x'.y' - sum(x') over 32 dimensions (half the steps of BF16).
+// With u = h_x and v = h_y - T', d^2 <= T' <=> x'.y' >= ceil((u + v) / 2),
+// and ceil((u + v) / 2) >= floor(u/2) + floor(v/2): the filter keeps every
+// pair with x'.y' >= floor(u/2) + floor(v/2) (all edges, and pairs one unit
+// past the threshold), so the accumulator starts at r_x + c_y with
+// r_x = sum(x') + floor(h_x / 2) - 1 and c_y = floor((h_y - T') / 2), and
+// its sign bit marks a candidate.  Every candidate is rechecked in the
+// reference's FP32 order.
+
+constexpr int I8PAD = 1 << 28;   // start term of padding rows and columns
+
+__device__ __forceinline__ int floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+
+// One thread per padded point: int8 fragments (tile t, k-step s of 32 dims,
+// lane l = 32 h + (a & 31) holds dims 32 s + 16 h .. +15) and the start terms.
+__global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, int n, int npad, int d, int KS8,
+                                                 const unsigned* __restrict__ colmin, int Tq,
+                                                 i32x4_t* __restrict__ xa, i32x4_t* __restrict__ yb,
+                                                 int* __restrict__ rterm, int* __restrict__ cterm) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= npad) return;
+    const bool real = a < n;
+    int h = 0, sx = 0;
+    for (int s = 0; s < KS8; ++s) {
+        for (int half = 0; half < 2; ++half) {
+            unsigned wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int k = 32 * s + 16 * half + j;
+                int xv = 0, yv = 0;
+                if (real && k < d) {
+                    const int xp = (int)X[(size_t)a * d + k] - (int)ord2f(colmin[k]) - 128;
+                    xv = xp;
+                    yv = -1 - xp;
+                    h += xp * xp;
+                    sx += xp;
+                }
+                wa[j >> 2] |= (unsigned)(xv & 0xff) << (8 * (j & 3));
+                wb[j >> 2] |= (unsigned)(yv & 0xff) << (8 * (j & 3));
+            }
+            const size_t idx = ((size_t)(a >> 5) * KS8 + s) * 64 + half * 32 + (a & 31);
+            if (xa) xa[idx] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
+            if (yb) yb[idx] = i32x4_t{(int)wb[0], (int)wb[1], (int)wb[2], (int)wb[3]};
+        }
+    }
+    if (rterm) rterm[a] = real ? sx + (h >> 1) - 1 : I8PAD;
+    if (cterm) cterm[a] = real ? floor_half(h - Tq) : I8PAD;
+}
+
+// The int8 counterpart of k_l2_mfma_filter<KS, 2, true, 1>: same work list,
+// tiling (4 waves x 2 row tiles, column tiles double-buffered in LDS),
+// candidate path and output.
+template <int KS8>
+__global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
+    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ rterm,
+    const int* __restrict__ cterm, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
+    constexpr int RT = 2;
+    constexpr int NCH = KS8 * 64;
+    constexpr int PER = (NCH + 255) / 256;
+    __shared__ i32x4_t bs[2][NCH];
+    __shared__ int2 stage[LSTAGE];
+    __shared__ unsigned scount;
+    __shared__ unsigned long long sbase;
+    const int4 wi = work[blockIdx.x];
+    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves before any access
+    const int tid = threadIdx.x;
+    if (tid == 0) scount = 0u;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int rt0 = wi.x * (4 * RT) + wave * RT;
+
+    i32x4_t a[RT][KS8];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int s = 0; s < KS8; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS8 + s) * 64 + lane];
+    int xr[RT][16];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[r][q] = rterm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+    const int last = wi.z - 1;
+    int yc = cterm[wi.y * 32 + (lane & 31)];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int c = tid + 256 * j;
+        if (c < NCH) bs[0][c] = yb[(size_t)wi.y * NCH + c];
+    }
+    __syncthreads();
+
+    for (int ct = wi.y; ct < wi.z; ++ct) {
+        const int buf = (ct - wi.y) & 1;
+        const bool has_next = ct + 1 < wi.z;
+        const int ycn = cterm[min(ct + 1, last) * 32 + (lane & 31)];
+        i32x4_t pre[PER];
+        if (has_next) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int c = tid + 256 * j;
+                if (c < NCH) pre[j] = yb[(size_t)(ct + 1) * NCH + c];
+            }
+        }
+        i32x16_t acc[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[r][q] = xr[r][q] + yc;
+#pragma unroll
+        for (int s = 0; s < KS8; ++s) {
+            const i32x4_t bb = bs[buf][s * 64 + lane];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, acc[r], 0, 0, 0);
+        }
+        int o = 0;
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) o |= acc[r][q];
+
+        if (__any(o < 0)) {
+            // rare path, as in k_l2_mfma_filter: the mask's shifts depend on an
+            // opaque zero so they stay out of the hot loop
+            const int rowb = rt0 * 32 + 4 * (lane >> 5);
+            int opaque;
+            asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
+            unsigned mask = 0;
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) mask |= ((unsigned)acc[r][q] >> 31) << (r * 16 + q + opaque);
+            const int col = ct * 32 + (lane & 31);
+            const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
+            unsigned any = mask;
+#pragma unroll
+            for (int o2 = 1; o2 < 64; o2 <<= 1) any |= (unsigned)__shfl_xor((int)any, o2, 64);
+            any = __builtin_amdgcn_readfirstlane(any);
+#pragma unroll 1
+            while (any) {
+                const int rq = __builtin_ctz(any);
+                any &= any - 1;
+                const int q = rq & 15;
+                const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
+                const bool p = ((mask >> rq) & 1u) && row < lim;
+                const unsigned long long m = __ballot(p);
+                if (m) {
+                    unsigned slot = 0;
+                    if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
+                    slot = __shfl(slot, 0, 64) +
+                           __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    if (p) {
+                        if (slot < LSTAGE) {
+                            stage[slot] = make_int2(row, col);
+                        } else {
+                            const unsigned long long g = atomicAdd(counter, 1ull);
+                            if (g < cap) cand[g] = make_int2(row, col);
+                        }
+                    }
+                }
+            }
+        }
+        if (has_next) {
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int c = tid + 256 * j;
+                if (c < NCH) bs[buf ^ 1][c] = pre[j];
+            }
+        }
+        __syncthreads();
+        yc = ycn;
+    }
+    const unsigned nst = min(scount, (unsigned)LSTAGE);
+    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
+    __syncthreads();
+    for (unsigned i = tid; i < nst; i += 256)
+        if (sbase + i < cap) cand[sbase + i] = stage[i];
+}
+
+
 // Output per candidate c: flag[c] = 1 if it is an edge, w[c] = its float
 // distance; a stable device select then compacts the edges (no atomics).
 template <bool VEC4>
@@ -1090,6 +1279,38 @@ void launch_filter_v(const FilterArgs& a, int variant) {
     }
 }
 
+bool g_int8_filter = true;   // integer data may use the exact int8 filter
+
+float ord2f_host(unsigned u) {
+    const unsigned v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &v, sizeof(f));
+    return f;
+}
+
+template <int KS8>
+void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
+    k_l2_mfma_filter_i8<KS8><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+        ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx, ny,
+        self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
+}
+
+int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8) {
+    if (ws.work_host.empty()) return RNNG_OK;
+    switch (KS8) {
+        case 1: launch_filter_i8<1>(ws, nx, ny, self_join); break;
+        case 2: launch_filter_i8<2>(ws, nx, ny, self_join); break;
+        case 3: launch_filter_i8<3>(ws, nx, ny, self_join); break;
+        case 4: launch_filter_i8<4>(ws, nx, ny, self_join); break;
+        case 5: launch_filter_i8<5>(ws, nx, ny, self_join); break;
+        default:
+            set_error("int8 filter supports d <= 160");
+            return RNNG_ERR_UNSUPPORTED;
+    }
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
 FilterArgs ws_args(Workspace& ws, int n_rows, int n_cols, int self_join) {
     FilterArgs a;
     a.xa = ws.xa.as<bf16x8_t>();
@@ -1169,23 +1390,6 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
     if (Tb < T) Tb = T;
 
-    const int variant = filter_variant();
-    const int rb_tiles = 4 * kVariants[variant].rt;      // row tiles per workgroup
-    const int rb_rows = 32 * rb_tiles;
-    const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
-    const int n_coltiles = (ny + 31) / 32;
-    const int n_rowblocks = n_rowtiles / rb_tiles;
-    const int npad_x = n_rowtiles * 32;
-    const int npad_y = std::max(n_coltiles * 32, self_join ? npad_x : 0);
-    const size_t frag_bytes_x = (size_t)npad_x * KS * 16 * 2;
-    const size_t frag_bytes_y = (size_t)npad_y * KS * 16 * 2;
-
-    // For a self join the x-form and the y-form come from the same points.
-    RNNG_TRY(ws.xa.reserve(frag_bytes_x));
-    RNNG_TRY(ws.yb.reserve(frag_bytes_y));
-    RNNG_TRY(ws.xn.reserve((size_t)npad_x * sizeof(float)));
-    RNNG_TRY(ws.yn.reserve((size_t)std::max(npad_x, npad_y) * sizeof(float)));
-    RNNG_TRY(ws.ycn.reserve((size_t)std::max(npad_x, npad_y) * sizeof(float)));
     RNNG_TRY(ws.colsum.reserve(sizeof(double) * d));
     RNNG_TRY(ws.colmin.reserve(sizeof(unsigned) * d));
     RNNG_TRY(ws.colmax.reserve(sizeof(unsigned) * d));
@@ -1193,28 +1397,12 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     RNNG_TRY(ws.range.reserve(sizeof(unsigned)));
     RNNG_TRY(ws.mu.reserve(sizeof(float) * d));
     RNNG_TRY(ws.params.reserve(sizeof(MfmaParams)));
-    RNNG_TRY(build_work(ws, n_rowblocks, rb_tiles, n_coltiles, self_join ? 1 : 0));
-    {
-        const unsigned long long guess =
-            g_cand_hint > 0 ? (unsigned long long)g_cand_hint
-                            : std::max<unsigned long long>(1ull << 22,
-                                                           48ull * (unsigned long long)std::max(nx, ny));
-        if (ws.cand_cap < guess || g_cand_hint > 0) {
-            RNNG_TRY(ws.cand.reserve(guess * sizeof(int2)));
-            ws.cand_cap = guess;
-        }
-    }
-    if (g->cap == 0) {
-        const int64_t guess = g_edge_hint > 0 ? g_edge_hint
-                                              : std::max<int64_t>(1 << 20, 32 * (int64_t)std::max(nx, ny));
-        RNNG_TRY(edge_buffers(g, guess));
-    }
 
     Events ev;
     RNNG_TRY(ev.create());
     RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
 
-    // ---- preparation: centering, fragments, error terms, C0 ----
+    // ---- column statistics: integer flag, ranges, centering ----
     {
         // colmin initialised to f2ord(+inf) = 0xff800000, colmax to f2ord(-inf) = 0x007fffff
         k_fill_u64<<<blocks_for(d, 256), 256, 0, stream()>>>((unsigned long long*)ws.colsum.ptr, d, 0ull);
@@ -1242,10 +1430,74 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
                                                       ws.mu.as<float>(), ws.stat.as<unsigned>(),
                                                       ws.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
-    if (self_join) {
+    // integer coordinates spanning at most 255 per dimension: the exact int8 filter
+    bool i8 = false;
+    if (g_int8_filter) {
+        unsigned hs[NSTAT], hr = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(hs, ws.stat.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(&hr, ws.range.ptr, sizeof(hr), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        const float range = ord2f_host(hr);
+        i8 = hs[ST_INT] != 0 && range >= 0.0f && range <= 255.0f;
+    }
+    const int KS8 = (d + 31) / 32;
+
+    const int variant = filter_variant();
+    const int rb_tiles = 4 * (i8 ? 2 : kVariants[variant].rt);   // row tiles per workgroup
+    const int rb_rows = 32 * rb_tiles;
+    const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
+    const int n_coltiles = (ny + 31) / 32;
+    const int n_rowblocks = n_rowtiles / rb_tiles;
+    const int npad_x = n_rowtiles * 32;
+    const int npad_y = std::max(n_coltiles * 32, self_join ? npad_x : 0);
+    const int npad = std::max(npad_x, npad_y);
+    const size_t frag_bytes_x = (size_t)(self_join ? npad : npad_x) * KS * 16 * 2;
+    const size_t frag_bytes_y = (size_t)npad_y * KS * 16 * 2;
+
+    // For a self join the x-form and the y-form come from the same points.
+    // (The int8 forms need half the bytes: KS8 * 32 per point.)
+    RNNG_TRY(ws.xa.reserve(frag_bytes_x));
+    RNNG_TRY(ws.yb.reserve(frag_bytes_y));
+    RNNG_TRY(ws.xn.reserve((size_t)npad * sizeof(float)));
+    RNNG_TRY(ws.yn.reserve((size_t)npad * sizeof(float)));
+    RNNG_TRY(ws.ycn.reserve((size_t)npad * sizeof(float)));
+    RNNG_TRY(build_work(ws, n_rowblocks, rb_tiles, n_coltiles, self_join ? 1 : 0));
+    {
+        const unsigned long long guess =
+            g_cand_hint > 0 ? (unsigned long long)g_cand_hint
+                            : std::max<unsigned long long>(1ull << 22,
+                                                           48ull * (unsigned long long)std::max(nx, ny));
+        if (ws.cand_cap < guess || g_cand_hint > 0) {
+            RNNG_TRY(ws.cand.reserve(guess * sizeof(int2)));
+            ws.cand_cap = guess;
+        }
+    }
+    if (g->cap == 0) {
+        const int64_t guess = g_edge_hint > 0 ? g_edge_hint
+                                              : std::max<int64_t>(1 << 20, 32 * (int64_t)std::max(nx, ny));
+        RNNG_TRY(edge_buffers(g, guess));
+    }
+
+    if (i8) {
+        // T' = floor(T), clamped: squared distances of such data stay below 2^24
+        const int Tq = T >= 1073741823.0f ? 1073741823 : (T < 0.0f ? -1 : (int)std::floor(T));
+        if (self_join) {
+            k_prep_i8<<<blocks_for(npad, 256), 256, 0, stream()>>>(A->X, nx, npad, d, KS8, ws.colmin.as<unsigned>(),
+                                                                  Tq, ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(),
+                                                                  ws.xn.as<int>(), ws.ycn.as<int>());
+        } else {
+            k_prep_i8<<<blocks_for(npad_x, 256), 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8,
+                                                                    ws.colmin.as<unsigned>(), Tq,
+                                                                    ws.xa.as<i32x4_t>(), nullptr, ws.xn.as<int>(),
+                                                                    nullptr);
+            k_prep_i8<<<blocks_for(npad_y, 256), 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8,
+                                                                    ws.colmin.as<unsigned>(), Tq, nullptr,
+                                                                    ws.yb.as<i32x4_t>(), nullptr, ws.ycn.as<int>());
+        }
+        RNNG_HIP_TRY(hipGetLastError());
+    } else if (self_join) {
         // one pass writes both forms for every padded row of the join (the
         // row norms cover npad_x rows, the column norms every padded row)
-        const int npad = std::max(npad_x, npad_y);
         RNNG_TRY(launch_prep(A->X, nx, npad, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
                              ws.yb.as<uint16_t>(), npad == npad_x ? ws.xn.as<float>() : nullptr,
                              ws.yn.as<float>(), ws.stat.as<unsigned>()));
@@ -1259,11 +1511,11 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
                              ws.yb.as<uint16_t>(), nullptr, ws.yn.as<float>(), ws.stat.as<unsigned>()));
     }
     RNNG_HIP_TRY(hipGetLastError());
-    k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
-                                         ws.params.as<MfmaParams>());
-    RNNG_HIP_TRY(hipGetLastError());
-    {
-        const int npy = self_join ? std::max(npad_x, npad_y) : npad_y;
+    if (!i8) {
+        k_finalize_c0<<<1, 1, 0, stream()>>>(ws.stat.as<unsigned>(), ws.range.as<unsigned>(), d, KS, Tb,
+                                             ws.params.as<MfmaParams>());
+        RNNG_HIP_TRY(hipGetLastError());
+        const int npy = self_join ? npad : npad_y;
         k_column_bias<<<blocks_for(npy, 256), 256, 0, stream()>>>(ws.yn.as<float>(), ws.ycn.as<float>(), npy,
                                                                    ws.params.as<MfmaParams>());
         RNNG_HIP_TRY(hipGetLastError());
@@ -1273,7 +1525,8 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
+        if (i8) RNNG_TRY(filter_dispatch_i8(ws, nx, ny, self_join ? 1 : 0, KS8));
+        else RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
         unsigned long long ncand = 0;
         RNNG_HIP_TRY(hipMemcpyAsync(&ncand, ws.counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
@@ -1303,6 +1556,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         if (st) {
             st->candidates += (int64_t)ncand;
             st->edges += ne;
+            st->int8_filter += i8 ? 1 : 0;
             st->band_pairs += (int64_t)cnt[2];
             st->band_flips += (int64_t)cnt[3];
             st->prep_ms += ev.ms(0, 1);
@@ -1531,6 +1785,11 @@ using namespace rnng;
 extern "C" {
 
 float rnng_sq_threshold(float eps) { return sq_threshold_host(eps); }
+
+int rnng_set_int8_filter(int32_t enable) {
+    g_int8_filter = enable != 0;
+    return RNNG_OK;
+}
 
 int rnng_dataset_create_f32(const float* points, int64_t n, int32_t d, rnng_dataset_t* out) {
     if (!out || (n > 0 && !points) || n < 0 || d <= 0) {

@@ -204,3 +204,59 @@ def test_symmetric_csr(orc, loops):
     assert ei.value.code == _lib.RNNG_ERR_ARG
     g.close()
     ds.close()
+
+
+@pytest.mark.parametrize("d,lo,hi", [(3, 0, 255), (17, -100, 155), (100, 0, 255), (128, 0, 255),
+                                     (144, 7, 262), (64, 0, 256), (33, 0, 40)])
+def test_int8_filter_bit_exact(orc, d, lo, hi):
+    """Integer data spanning at most 255 per dimension runs the exact int8
+    filter (int8_filter counted); a span of 256 falls back to BF16; both give
+    the oracle's edges, also with the int8 path switched off."""
+    from rnng_amd import Stats, _lib
+    rng = np.random.default_rng(d * 13 + hi)
+    n = 3000
+    X = rng.integers(lo, hi + 1, size=(n, d)).astype(F)
+    X[:, 0] = lo
+    X[1, 0] = hi
+    X[77] = X[12]
+    D2 = ((X[:40, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(2)
+    eps = F(np.sqrt(np.quantile(D2[D2 > 0], 0.002)))
+    flush_print(f"int8 filter d={d} range [{lo},{hi}] eps={eps}")
+    ij_ref, w_ref = orc.l2_eps_pairs(X, eps)
+    assert ij_ref.shape[0] > n // 10
+    eligible = hi - lo <= 255
+    st = Stats()
+    ij, w = _l2().eps_pairs(X, eps, "mfma", stats=st)
+    _check_same(ij, w, ij_ref, w_ref)
+    assert st.int8_filter == (1 if eligible else 0)
+    Y = X[: n // 2] + 0
+    ijb_ref, wb_ref = orc.l2_eps_pairs(X, eps, Y)
+    stb = Stats()
+    ijb, wb = _l2().eps_bipartite(X, Y, eps, "mfma", stats=stb)
+    _check_same(ijb, wb, ijb_ref, wb_ref)
+    assert stb.int8_filter == (1 if eligible else 0)
+    _lib.check(_lib.lib().rnng_set_int8_filter(0))
+    try:
+        st2 = Stats()
+        ij2, w2 = _l2().eps_pairs(X, eps, "mfma", stats=st2)
+        _check_same(ij2, w2, ij_ref, w_ref)
+        assert st2.int8_filter == 0
+    finally:
+        _lib.check(_lib.lib().rnng_set_int8_filter(1))
+
+
+def test_int8_filter_eps_extremes(orc):
+    from rnng_amd import Stats
+    rng = np.random.default_rng(3)
+    X = rng.integers(0, 256, size=(500, 16)).astype(F)
+    X[10] = X[3]
+    X[400] = X[3]
+    for eps, want in ((0.0, [(3, 10), (3, 400), (10, 400)]), (-1.0, [])):
+        st = Stats()
+        ij, w = _l2().eps_pairs(X, eps, "mfma", stats=st)
+        assert [tuple(p) for p in ij] == want and st.int8_filter == 1
+    st = Stats()
+    ij, w = _l2().eps_pairs(X[:200], 1e9, "mfma", stats=st)
+    assert ij.shape[0] == 200 * 199 // 2 and st.int8_filter == 1
+    ij_ref, w_ref = orc.l2_eps_pairs(X[:200], F(1e9))
+    _check_same(ij, w, ij_ref, w_ref)
