@@ -805,10 +805,10 @@ __host__ __device__ constexpr int recheck_ld(int d) { return ((d + 3) & ~3) + 4;
 // adds A.B = -x'.y' - sum(x') over 32 dimensions (half the steps of BF16).
 // With u = h_x and v = h_y - T', d^2 <= T' <=> x'.y' >= ceil((u + v) / 2),
 // and ceil((u + v) / 2) >= floor(u/2) + floor(v/2): the filter keeps every
-// pair with x'.y' >= floor(u/2) + floor(v/2) (all edges, and pairs one unit
-// past the threshold), so the accumulator starts at r_x + c_y with
-// r_x = sum(x') + floor(h_x / 2) - 1 and c_y = floor((h_y - T') / 2), and
-// its sign bit marks a candidate.  Every candidate is rechecked in the
+// pair with x'.y' >= floor(u/2) + floor(v/2) (all edges, and pairs at most
+// two units past the threshold): with r_x = sum(x') + floor(h_x / 2) - 1 and
+// c_y = floor((h_y - T') / 2) the accumulator starts at r_x, and the pair is
+// a candidate iff it ends below -c_y.  Every candidate is rechecked in the
 // reference's FP32 order.
 
 constexpr int I8PAD = 1 << 28;   // start term of padding rows and columns
@@ -851,15 +851,107 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
     if (cterm) cterm[a] = real ? floor_half(h - Tq) : I8PAD;
 }
 
-// The int8 counterpart of k_l2_mfma_filter<KS, 2, true, 1>: same work list,
-// tiling (4 waves x 2 row tiles, column tiles double-buffered in LDS),
-// candidate path and output.
+// Shared pieces of the int8 filter kernels.  A wave owns RT = 2 row tiles:
+// their fragments (a) and start terms in the accumulator layout (xr, the first
+// k-step's C operand) stay in registers for the whole work item.
+constexpr int I8RT = 2;
+
 template <int KS8>
-__global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
+__device__ __forceinline__ void i8_load_rows(const i32x4_t* __restrict__ xa, const int* __restrict__ rterm, int rt0,
+                                             int lane, i32x4_t (&a)[I8RT][KS8], i32x16_t (&xr)[I8RT]) {
+#pragma unroll
+    for (int r = 0; r < I8RT; ++r)
+#pragma unroll
+        for (int s = 0; s < KS8; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS8 + s) * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < I8RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xr[r][q] = rterm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+}
+
+// acc = r_x - x'.y' - sum(x') ... for one column tile held in LDS (b)
+template <int KS8>
+__device__ __forceinline__ void i8_tile(const i32x4_t* b, const i32x4_t (&a)[I8RT][KS8],
+                                        const i32x16_t (&xr)[I8RT], int lane, i32x16_t (&acc)[I8RT]) {
+#pragma unroll
+    for (int s = 0; s < KS8; ++s) {
+        const i32x4_t bb = b[s * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < I8RT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, s == 0 ? xr[r] : acc[r], 0, 0, 0);
+    }
+}
+
+// A lane's 16 values share one column, so acc + c_y < 0 is acc < -c_y,
+// decided on the minimum of the lane's accumulators.
+__device__ __forceinline__ int i8_min(const i32x16_t (&acc)[I8RT]) {
+    int mn = acc[0][0];
+#pragma unroll
+    for (int r = 0; r < I8RT; ++r)
+#pragma unroll
+        for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = min(mn, acc[r][q]);
+    return mn;
+}
+
+// Rare path: candidates of one tile pair, staged in LDS.  Each lane builds
+// the mask of its 32 values below -c_y (value r*16+q of a lane is row
+// rowb + 8*((r*16+q) >> 2) + (q & 3), rising with the index), drops the rows
+// at or past its limit (the row/column limit of a self join, padding) as a
+// bit count, then the lanes walk their own set bits together: one ballot and
+// one LDS reservation per round.  The mask's shifts depend on an opaque zero
+// so they stay out of the hot loop.
+__device__ __forceinline__ void i8_emit(const i32x16_t (&acc)[I8RT], int nyc, int ct, int rt0, int lane, int n_rows,
+                                        int n_cols, int self_join, int2* stage, unsigned* scount,
+                                        int2* __restrict__ cand, unsigned long long cap,
+                                        unsigned long long* __restrict__ counter) {
+    const int rowb = rt0 * 32 + 4 * (lane >> 5);
+    int opaque;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
+    unsigned mask = 0;
+#pragma unroll
+    for (int r = 0; r < I8RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) mask |= ((unsigned)(acc[r][q] - nyc) >> 31) << (r * 16 + q + opaque);
+    const int col = ct * 32 + (lane & 31);
+    const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
+    // values with row offset 8*(i >> 2) + (i & 3) < L are the first k
+    const int L = lim - rowb;
+    const int k = L <= 0 ? 0 : (L >= 64 ? 32 : 4 * (L >> 3) + min(4, L & 7));
+    mask &= k >= 32 ? ~0u : ((1u << k) - 1u);
+#pragma unroll 1
+    for (;;) {
+        const bool p = mask != 0;
+        const unsigned long long m = __ballot(p);
+        if (m == 0) break;
+        int row = 0;
+        if (p) {
+            const int rq = __builtin_ctz(mask);
+            mask &= mask - 1;
+            row = rowb + 8 * (rq >> 2) + (rq & 3);
+        }
+        unsigned slot = 0;
+        if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
+        slot = __shfl(slot, 0, 64) +
+               __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (p) {
+            if (slot < LSTAGE) {
+                stage[slot] = make_int2(row, col);
+            } else {
+                const unsigned long long g = atomicAdd(counter, 1ull);
+                if (g < cap) cand[g] = make_int2(row, col);
+            }
+        }
+    }
+}
+
+// The int8 counterpart of k_l2_mfma_filter<KS, 2, true, 1>: same work list,
+// tiling (4 waves x 2 row tiles, column tiles double-buffered in LDS through
+// registers, one barrier per tile), candidate path and output.
+template <int KS8>
+__global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
     const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ rterm,
     const int* __restrict__ cterm, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
     int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
-    constexpr int RT = 2;
     constexpr int NCH = KS8 * 64;
     constexpr int PER = (NCH + 255) / 256;
     __shared__ i32x4_t bs[2][NCH];
@@ -872,18 +964,10 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int rt0 = wi.x * (4 * RT) + wave * RT;
-
-    i32x4_t a[RT][KS8];
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int s = 0; s < KS8; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS8 + s) * 64 + lane];
-    int xr[RT][16];
-#pragma unroll
-    for (int r = 0; r < RT; ++r)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) xr[r][q] = rterm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+    const int rt0 = wi.x * (4 * I8RT) + wave * I8RT;
+    i32x4_t a[I8RT][KS8];
+    i32x16_t xr[I8RT];
+    i8_load_rows<KS8>(xa, rterm, rt0, lane, a, xr);
     const int last = wi.z - 1;
     int yc = cterm[wi.y * 32 + (lane & 31)];
 #pragma unroll
@@ -891,6 +975,9 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
         const int c = tid + 256 * j;
         if (c < NCH) bs[0][c] = yb[(size_t)wi.y * NCH + c];
     }
+    // every prologue load done here, so the loop's own waits count only its
+    // prefetches (vmcnt 0; expcnt and lgkmcnt left at their maxima)
+    __builtin_amdgcn_s_waitcnt(0xF70);
     __syncthreads();
 
     for (int ct = wi.y; ct < wi.z; ++ct) {
@@ -905,64 +992,10 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
                 if (c < NCH) pre[j] = yb[(size_t)(ct + 1) * NCH + c];
             }
         }
-        i32x16_t acc[RT];
-#pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[r][q] = xr[r][q] + yc;
-#pragma unroll
-        for (int s = 0; s < KS8; ++s) {
-            const i32x4_t bb = bs[buf][s * 64 + lane];
-#pragma unroll
-            for (int r = 0; r < RT; ++r) acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, acc[r], 0, 0, 0);
-        }
-        int o = 0;
-#pragma unroll
-        for (int r = 0; r < RT; ++r)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) o |= acc[r][q];
-
-        if (__any(o < 0)) {
-            // rare path, as in k_l2_mfma_filter: the mask's shifts depend on an
-            // opaque zero so they stay out of the hot loop
-            const int rowb = rt0 * 32 + 4 * (lane >> 5);
-            int opaque;
-            asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
-            unsigned mask = 0;
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int q = 0; q < 16; ++q) mask |= ((unsigned)acc[r][q] >> 31) << (r * 16 + q + opaque);
-            const int col = ct * 32 + (lane & 31);
-            const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
-            unsigned any = mask;
-#pragma unroll
-            for (int o2 = 1; o2 < 64; o2 <<= 1) any |= (unsigned)__shfl_xor((int)any, o2, 64);
-            any = __builtin_amdgcn_readfirstlane(any);
-#pragma unroll 1
-            while (any) {
-                const int rq = __builtin_ctz(any);
-                any &= any - 1;
-                const int q = rq & 15;
-                const int row = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);
-                const bool p = ((mask >> rq) & 1u) && row < lim;
-                const unsigned long long m = __ballot(p);
-                if (m) {
-                    unsigned slot = 0;
-                    if (lane == 0) slot = atomicAdd(&scount, (unsigned)__popcll(m));
-                    slot = __shfl(slot, 0, 64) +
-                           __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    if (p) {
-                        if (slot < LSTAGE) {
-                            stage[slot] = make_int2(row, col);
-                        } else {
-                            const unsigned long long g = atomicAdd(counter, 1ull);
-                            if (g < cap) cand[g] = make_int2(row, col);
-                        }
-                    }
-                }
-            }
-        }
+        i32x16_t acc[I8RT];
+        i8_tile<KS8>(bs[buf], a, xr, lane, acc);
+        if (__any(i8_min(acc) < -yc))
+            i8_emit(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
         if (has_next) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -980,6 +1013,125 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_i8(
         if (sbase + i < cap) cand[sbase + i] = stage[i];
 }
 
+// Barrier of the direct-to-LDS ring: wait until at most n of this wave's
+// vector-memory operations are outstanding (n <= 4, wave-uniform) and its LDS
+// operations are done, then s_barrier.  A __syncthreads() would wait for every
+// direct-to-LDS load in flight (its fence covers them), including the ones
+// issued for the group after next; the LDS data the loop reads after this
+// barrier are the stages, whose loads the vmcnt wait covers.  The empty asm
+// statements keep the compiler from moving memory operations across it.
+__device__ __forceinline__ void ring_barrier(int n) {
+    switch (n) {
+        case 0: __builtin_amdgcn_s_waitcnt(0x070); break;
+        case 1: __builtin_amdgcn_s_waitcnt(0x071); break;
+        case 2: __builtin_amdgcn_s_waitcnt(0x072); break;
+        case 3: __builtin_amdgcn_s_waitcnt(0x073); break;
+        default: __builtin_amdgcn_s_waitcnt(0x074); break;
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
+// This wave's share of the loads of one group (G column tiles from t0,
+// clamped to the last tile): fragment chunks j = wave, wave + 4, ... straight
+// into LDS (global_load_lds_dwordx4), and, from wave 3, the group's column
+// start terms (global_load_lds_dword, two tiles per instruction).
+template <int KS8, int G>
+__device__ __forceinline__ void i8_group_load(const i32x4_t* __restrict__ yb, const int* __restrict__ cterm,
+                                              i32x4_t* dst, int* cdst, int t0, int last, int wave, int lane) {
+    constexpr int NJ = G * KS8;
+#pragma unroll
+    for (int jj = 0; jj < (NJ + 3) / 4; ++jj) {
+        const int j = jj * 4 + wave;
+        if (j >= NJ) break;
+        const int tile = min(t0 + j / KS8, last);
+        const i32x4_t* src = yb + ((size_t)tile * KS8 + (j % KS8)) * 64 + lane;
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + j * 64),
+                                         16, 0, 0);
+    }
+    if (wave == 3) {
+        const int* src = cterm + min(t0 + (lane >> 5), last) * 32 + (lane & 31);
+        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)cdst, 4, 0, 0);
+    }
+}
+
+// Direct-to-LDS variant: column tiles go from global memory straight into a
+// ring of three LDS stages of G tiles each (no staging registers), issued two
+// groups ahead of the group being computed, so a group's loads have two
+// groups' time to land; one barrier per group.  The stages are distinct LDS
+// objects and the loop is unrolled by three, so the compiler sees that reads
+// of one stage never alias the asynchronous loads into the others.
+template <int KS8, int G>
+__global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8_dma(
+    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ rterm,
+    const int* __restrict__ cterm, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
+    int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
+    constexpr int NCH = KS8 * 64;
+    constexpr int NJ = G * KS8;
+    static_assert(G == 1 || G == 2, "the start-term load covers two tiles");
+    __shared__ i32x4_t bs0[G * NCH], bs1[G * NCH], bs2[G * NCH];
+    __shared__ int cs0[64], cs1[64], cs2[64];
+    __shared__ int2 stage[LSTAGE];
+    __shared__ unsigned scount;
+    __shared__ unsigned long long sbase;
+    const int4 wi = work[blockIdx.x];
+    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves before any access
+    const int tid = threadIdx.x;
+    if (tid == 0) scount = 0u;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const int rt0 = wi.x * (4 * I8RT) + wave * I8RT;
+    const int last = wi.z - 1;
+    const int ng = (wi.z - wi.y + G - 1) / G;
+    // loads this wave issues per group (the same in every lane)
+    const int nld =
+        __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / 4 + 1 : 0) + (wave == 3 ? 1 : 0));
+
+    // issue order: rows, group 0, group 1 (the barrier's count relies on it)
+    i32x4_t a[I8RT][KS8];
+    i32x16_t xr[I8RT];
+    i8_load_rows<KS8>(xa, rterm, rt0, lane, a, xr);
+    asm volatile("" ::: "memory");
+    i8_group_load<KS8, G>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
+    asm volatile("" ::: "memory");
+    if (ng > 1) i8_group_load<KS8, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
+    // rows and group 0 landed (group 1 may still be in flight); scount is set
+    ring_barrier(ng > 1 ? nld : 0);
+
+    // group g from stage cur; the loads of group g + 2 go to stage nxt (group
+    // g - 1's, free since the last barrier); then group g + 1 must have landed
+    auto step = [&](const i32x4_t* cur, const int* ccur, i32x4_t* nxt, int* cnxt, int g) {
+        const int t0 = wi.y + g * G;
+        const bool issue = g + 2 < ng;
+        if (issue) i8_group_load<KS8, G>(yb, cterm, nxt, cnxt, t0 + 2 * G, last, wave, lane);
+#pragma unroll
+        for (int cc = 0; cc < G; ++cc) {
+            if (cc > 0 && t0 + cc > last) break;
+            const int nyc = -ccur[cc * 32 + (lane & 31)];
+            i32x16_t acc[I8RT];
+            i8_tile<KS8>(cur + cc * NCH, a, xr, lane, acc);
+            if (__any(i8_min(acc) < nyc))
+                i8_emit(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
+        }
+        ring_barrier(issue ? nld : 0);
+    };
+    int g = 0;
+    for (;;) {
+        step(bs0, cs0, bs2, cs2, g);
+        if (++g >= ng) break;
+        step(bs1, cs1, bs0, cs0, g);
+        if (++g >= ng) break;
+        step(bs2, cs2, bs1, cs1, g);
+        if (++g >= ng) break;
+    }
+    __syncthreads();   // every wave's staged candidates and count
+    const unsigned nst = min(scount, (unsigned)LSTAGE);
+    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
+    __syncthreads();
+    for (unsigned i = tid; i < nst; i += 256)
+        if (sbase + i < cap) cand[sbase + i] = stage[i];
+}
 
 // Output per candidate c: flag[c] = 1 if it is an edge, w[c] = its float
 // distance; a stable device select then compacts the edges (no atomics).
@@ -1288,21 +1440,51 @@ float ord2f_host(unsigned u) {
     return f;
 }
 
-template <int KS8>
+// G = 0: k_l2_mfma_filter_i8; G = 1, 2: the direct-to-LDS kernel
+template <int KS8, int G>
 void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
-    k_l2_mfma_filter_i8<KS8><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-        ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx, ny,
-        self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
+    if constexpr (G == 0) {
+        k_l2_mfma_filter_i8<KS8><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+            ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
+            ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
+    } else {
+        k_l2_mfma_filter_i8_dma<KS8, G><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+            ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
+            ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
+    }
 }
 
-int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8) {
+int g_i8_group = -1;   // int8 filter kernel: 0 registers-staged, 1 or 2 direct-to-LDS tiles per stage
+
+int i8_group() {
+    if (g_i8_group < 0) {
+        g_i8_group = 2;
+        if (const char* e = getenv("RNNG_L2_I8_GROUP")) {
+            const int v = atoi(e);
+            if (v >= 0 && v <= 2) g_i8_group = v;
+        }
+    }
+    return g_i8_group;
+}
+
+int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8, int G) {
     if (ws.work_host.empty()) return RNNG_OK;
-    switch (KS8) {
-        case 1: launch_filter_i8<1>(ws, nx, ny, self_join); break;
-        case 2: launch_filter_i8<2>(ws, nx, ny, self_join); break;
-        case 3: launch_filter_i8<3>(ws, nx, ny, self_join); break;
-        case 4: launch_filter_i8<4>(ws, nx, ny, self_join); break;
-        case 5: launch_filter_i8<5>(ws, nx, ny, self_join); break;
+    switch (KS8 * 4 + G) {
+        case 1 * 4 + 0: launch_filter_i8<1, 0>(ws, nx, ny, self_join); break;
+        case 2 * 4 + 0: launch_filter_i8<2, 0>(ws, nx, ny, self_join); break;
+        case 3 * 4 + 0: launch_filter_i8<3, 0>(ws, nx, ny, self_join); break;
+        case 4 * 4 + 0: launch_filter_i8<4, 0>(ws, nx, ny, self_join); break;
+        case 5 * 4 + 0: launch_filter_i8<5, 0>(ws, nx, ny, self_join); break;
+        case 1 * 4 + 1: launch_filter_i8<1, 1>(ws, nx, ny, self_join); break;
+        case 2 * 4 + 1: launch_filter_i8<2, 1>(ws, nx, ny, self_join); break;
+        case 3 * 4 + 1: launch_filter_i8<3, 1>(ws, nx, ny, self_join); break;
+        case 4 * 4 + 1: launch_filter_i8<4, 1>(ws, nx, ny, self_join); break;
+        case 5 * 4 + 1: launch_filter_i8<5, 1>(ws, nx, ny, self_join); break;
+        case 1 * 4 + 2: launch_filter_i8<1, 2>(ws, nx, ny, self_join); break;
+        case 2 * 4 + 2: launch_filter_i8<2, 2>(ws, nx, ny, self_join); break;
+        case 3 * 4 + 2: launch_filter_i8<3, 2>(ws, nx, ny, self_join); break;
+        case 4 * 4 + 2: launch_filter_i8<4, 2>(ws, nx, ny, self_join); break;
+        case 5 * 4 + 2: launch_filter_i8<5, 2>(ws, nx, ny, self_join); break;
         default:
             set_error("int8 filter supports d <= 160");
             return RNNG_ERR_UNSUPPORTED;
@@ -1525,7 +1707,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        if (i8) RNNG_TRY(filter_dispatch_i8(ws, nx, ny, self_join ? 1 : 0, KS8));
+        if (i8) RNNG_TRY(filter_dispatch_i8(ws, nx, ny, self_join ? 1 : 0, KS8, i8_group()));
         else RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
         unsigned long long ncand = 0;
