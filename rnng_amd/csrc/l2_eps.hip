@@ -851,67 +851,94 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
     if (cterm) cterm[a] = real ? floor_half(h - Tq) : I8PAD;
 }
 
-// Shared pieces of the int8 filter kernels.  A wave owns RT = 2 row tiles:
-// their fragments (a) and start terms in the accumulator layout (xr, the first
-// k-step's C operand) stay in registers for the whole work item.
-constexpr int I8RT = 2;
+// Shared pieces of the filter kernels with row terms as the first MFMA's C
+// operand.  A wave owns RT = 2 row tiles: their fragments (a) and start terms
+// in the accumulator layout (xr) stay in registers for the whole work item;
+// the accumulator ends at r_x + (products) and the pair is a candidate iff it
+// is below -c_y (c_y: the column's start term).  A lane's 16 values of a tile
+// share one column, so the test is on their minimum.
+//   MfI8:   int8 fragments, exact int32 accumulation (integer data).
+//   MfBF16: BF16 fragments, FP32 accumulation; the C0 bound of the start terms
+//           covers partial sums up to |r_x| + sum |products| <= 3 maxN, and
+//           the final comparison is exact.
+struct MfI8 {
+    using frag = i32x4_t;
+    using accv = i32x16_t;
+    using term = int;
+    __device__ static accv mfma(frag a, frag b, accv c) { return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0); }
+    __device__ static term vmin(term a, term b) { return min(a, b); }
+    // v < t; v - t stays inside int32 for these start terms
+    __device__ static unsigned below(term v, term t) { return (unsigned)(v - t) >> 31; }
+};
+struct MfBF16 {
+    using frag = bf16x8_t;
+    using accv = f32x16_t;
+    using term = float;
+    __device__ static accv mfma(frag a, frag b, accv c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static term vmin(term a, term b) { return fminf(a, b); }
+    __device__ static unsigned below(term v, term t) { return v < t ? 1u : 0u; }
+};
 
-template <int KS8>
-__device__ __forceinline__ void i8_load_rows(const i32x4_t* __restrict__ xa, const int* __restrict__ rterm, int rt0,
-                                             int lane, i32x4_t (&a)[I8RT][KS8], i32x16_t (&xr)[I8RT]) {
+constexpr int MRT = 2;   // row tiles per wave
+
+template <class T, int KS>
+__device__ __forceinline__ void mf_load_rows(const typename T::frag* __restrict__ xa,
+                                             const typename T::term* __restrict__ rterm, int rt0, int lane,
+                                             typename T::frag (&a)[MRT][KS], typename T::accv (&xr)[MRT]) {
 #pragma unroll
-    for (int r = 0; r < I8RT; ++r)
+    for (int r = 0; r < MRT; ++r)
 #pragma unroll
-        for (int s = 0; s < KS8; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS8 + s) * 64 + lane];
+        for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
 #pragma unroll
-    for (int r = 0; r < I8RT; ++r)
+    for (int r = 0; r < MRT; ++r)
 #pragma unroll
         for (int q = 0; q < 16; ++q) xr[r][q] = rterm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
 }
 
-// acc = r_x - x'.y' - sum(x') ... for one column tile held in LDS (b)
-template <int KS8>
-__device__ __forceinline__ void i8_tile(const i32x4_t* b, const i32x4_t (&a)[I8RT][KS8],
-                                        const i32x16_t (&xr)[I8RT], int lane, i32x16_t (&acc)[I8RT]) {
+// one column tile held in LDS (b)
+template <class T, int KS>
+__device__ __forceinline__ void mf_tile(const typename T::frag* b, const typename T::frag (&a)[MRT][KS],
+                                        const typename T::accv (&xr)[MRT], int lane, typename T::accv (&acc)[MRT]) {
 #pragma unroll
-    for (int s = 0; s < KS8; ++s) {
-        const i32x4_t bb = b[s * 64 + lane];
+    for (int s = 0; s < KS; ++s) {
+        const typename T::frag bb = b[s * 64 + lane];
 #pragma unroll
-        for (int r = 0; r < I8RT; ++r)
-            acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, s == 0 ? xr[r] : acc[r], 0, 0, 0);
+        for (int r = 0; r < MRT; ++r) acc[r] = T::mfma(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
     }
 }
 
-// A lane's 16 values share one column, so acc + c_y < 0 is acc < -c_y,
-// decided on the minimum of the lane's accumulators.
-__device__ __forceinline__ int i8_min(const i32x16_t (&acc)[I8RT]) {
-    int mn = acc[0][0];
+template <class T>
+__device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)[MRT]) {
+    typename T::term mn = acc[0][0];
 #pragma unroll
-    for (int r = 0; r < I8RT; ++r)
+    for (int r = 0; r < MRT; ++r)
 #pragma unroll
-        for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = min(mn, acc[r][q]);
+        for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = T::vmin(mn, acc[r][q]);
     return mn;
 }
 
 // Rare path: candidates of one tile pair, staged in LDS.  Each lane builds
-// the mask of its 32 values below -c_y (value r*16+q of a lane is row
-// rowb + 8*((r*16+q) >> 2) + (q & 3), rising with the index), drops the rows
-// at or past its limit (the row/column limit of a self join, padding) as a
-// bit count, then the lanes walk their own set bits together: one ballot and
-// one LDS reservation per round.  The mask's shifts depend on an opaque zero
-// so they stay out of the hot loop.
-__device__ __forceinline__ void i8_emit(const i32x16_t (&acc)[I8RT], int nyc, int ct, int rt0, int lane, int n_rows,
-                                        int n_cols, int self_join, int2* stage, unsigned* scount,
-                                        int2* __restrict__ cand, unsigned long long cap,
+// the mask of its 32 values below nyc (value i = r*16+q of a lane is row
+// rowb + 8*(i >> 2) + (i & 3), rising with i), drops the rows at or past its
+// limit (the row/column limit of a self join, padding) as a bit count, then
+// the lanes walk their own set bits together: one ballot and one LDS
+// reservation per round.  The mask's shifts depend on an opaque zero so they
+// stay out of the hot loop.
+template <class T>
+__device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], typename T::term nyc, int ct, int rt0,
+                                        int lane, int n_rows, int n_cols, int self_join, int2* stage,
+                                        unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
                                         unsigned long long* __restrict__ counter) {
     const int rowb = rt0 * 32 + 4 * (lane >> 5);
     int opaque;
     asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
     unsigned mask = 0;
 #pragma unroll
-    for (int r = 0; r < I8RT; ++r)
+    for (int r = 0; r < MRT; ++r)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) mask |= ((unsigned)(acc[r][q] - nyc) >> 31) << (r * 16 + q + opaque);
+        for (int q = 0; q < 16; ++q) mask |= T::below(acc[r][q], nyc) << (r * 16 + q + opaque);
     const int col = ct * 32 + (lane & 31);
     const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
     // values with row offset 8*(i >> 2) + (i & 3) < L are the first k
@@ -964,10 +991,10 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int rt0 = wi.x * (4 * I8RT) + wave * I8RT;
-    i32x4_t a[I8RT][KS8];
-    i32x16_t xr[I8RT];
-    i8_load_rows<KS8>(xa, rterm, rt0, lane, a, xr);
+    const int rt0 = wi.x * (4 * MRT) + wave * MRT;
+    i32x4_t a[MRT][KS8];
+    i32x16_t xr[MRT];
+    mf_load_rows<MfI8, KS8>(xa, rterm, rt0, lane, a, xr);
     const int last = wi.z - 1;
     int yc = cterm[wi.y * 32 + (lane & 31)];
 #pragma unroll
@@ -992,10 +1019,10 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
                 if (c < NCH) pre[j] = yb[(size_t)(ct + 1) * NCH + c];
             }
         }
-        i32x16_t acc[I8RT];
-        i8_tile<KS8>(bs[buf], a, xr, lane, acc);
-        if (__any(i8_min(acc) < -yc))
-            i8_emit(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
+        i32x16_t acc[MRT];
+        mf_tile<MfI8, KS8>(bs[buf], a, xr, lane, acc);
+        if (__any(mf_min<MfI8>(acc) < -yc))
+            mf_emit<MfI8>(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
         if (has_next) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -1037,41 +1064,46 @@ __device__ __forceinline__ void ring_barrier(int n) {
 // clamped to the last tile): fragment chunks j = wave, wave + 4, ... straight
 // into LDS (global_load_lds_dwordx4), and, from wave 3, the group's column
 // start terms (global_load_lds_dword, two tiles per instruction).
-template <int KS8, int G>
-__device__ __forceinline__ void i8_group_load(const i32x4_t* __restrict__ yb, const int* __restrict__ cterm,
-                                              i32x4_t* dst, int* cdst, int t0, int last, int wave, int lane) {
-    constexpr int NJ = G * KS8;
+template <class T, int KS, int G>
+__device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict__ yb,
+                                              const typename T::term* __restrict__ cterm, typename T::frag* dst,
+                                              typename T::term* cdst, int t0, int last, int wave, int lane) {
+    constexpr int NJ = G * KS;
 #pragma unroll
     for (int jj = 0; jj < (NJ + 3) / 4; ++jj) {
         const int j = jj * 4 + wave;
         if (j >= NJ) break;
-        const int tile = min(t0 + j / KS8, last);
-        const i32x4_t* src = yb + ((size_t)tile * KS8 + (j % KS8)) * 64 + lane;
+        const int tile = min(t0 + j / KS, last);
+        const typename T::frag* src = yb + ((size_t)tile * KS + (j % KS)) * 64 + lane;
         __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + j * 64),
                                          16, 0, 0);
     }
     if (wave == 3) {
-        const int* src = cterm + min(t0 + (lane >> 5), last) * 32 + (lane & 31);
+        const typename T::term* src = cterm + min(t0 + (lane >> 5), last) * 32 + (lane & 31);
         __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)cdst, 4, 0, 0);
     }
 }
 
-// Direct-to-LDS variant: column tiles go from global memory straight into a
-// ring of three LDS stages of G tiles each (no staging registers), issued two
+// Direct-to-LDS ring: column tiles go from global memory straight into a ring
+// of three LDS stages of G tiles each (no staging registers), issued two
 // groups ahead of the group being computed, so a group's loads have two
 // groups' time to land; one barrier per group.  The stages are distinct LDS
 // objects and the loop is unrolled by three, so the compiler sees that reads
 // of one stage never alias the asynchronous loads into the others.
-template <int KS8, int G>
-__global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8_dma(
-    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ rterm,
-    const int* __restrict__ cterm, const int4* __restrict__ work, int n_rows, int n_cols, int self_join,
-    int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
-    constexpr int NCH = KS8 * 64;
-    constexpr int NJ = G * KS8;
+template <class T, int KS, int G, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
+    const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
+    const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
+    const int4* __restrict__ work, int n_rows, int n_cols, int self_join, int2* __restrict__ cand,
+    unsigned long long cap, unsigned long long* __restrict__ counter) {
+    using frag = typename T::frag;
+    using term = typename T::term;
+    using accv = typename T::accv;
+    constexpr int NCH = KS * 64;
+    constexpr int NJ = G * KS;
     static_assert(G == 1 || G == 2, "the start-term load covers two tiles");
-    __shared__ i32x4_t bs0[G * NCH], bs1[G * NCH], bs2[G * NCH];
-    __shared__ int cs0[64], cs1[64], cs2[64];
+    __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[G * NCH];
+    __shared__ term cs0[64], cs1[64], cs2[64];
     __shared__ int2 stage[LSTAGE];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
@@ -1081,7 +1113,7 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8_dma
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int rt0 = wi.x * (4 * I8RT) + wave * I8RT;
+    const int rt0 = wi.x * (4 * MRT) + wave * MRT;
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
     // loads this wave issues per group (the same in every lane)
@@ -1089,30 +1121,31 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8_dma
         __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / 4 + 1 : 0) + (wave == 3 ? 1 : 0));
 
     // issue order: rows, group 0, group 1 (the barrier's count relies on it)
-    i32x4_t a[I8RT][KS8];
-    i32x16_t xr[I8RT];
-    i8_load_rows<KS8>(xa, rterm, rt0, lane, a, xr);
+    frag a[MRT][KS];
+    accv xr[MRT];
+    mf_load_rows<T, KS>(xa, rterm, rt0, lane, a, xr);
     asm volatile("" ::: "memory");
-    i8_group_load<KS8, G>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
+    mf_group_load<T, KS, G>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
     asm volatile("" ::: "memory");
-    if (ng > 1) i8_group_load<KS8, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
+    if (ng > 1) mf_group_load<T, KS, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
     // rows and group 0 landed (group 1 may still be in flight); scount is set
     ring_barrier(ng > 1 ? nld : 0);
 
     // group g from stage cur; the loads of group g + 2 go to stage nxt (group
     // g - 1's, free since the last barrier); then group g + 1 must have landed
-    auto step = [&](const i32x4_t* cur, const int* ccur, i32x4_t* nxt, int* cnxt, int g) {
+    auto step = [&](const frag* cur, const term* ccur, frag* nxt, term* cnxt, int g) {
         const int t0 = wi.y + g * G;
         const bool issue = g + 2 < ng;
-        if (issue) i8_group_load<KS8, G>(yb, cterm, nxt, cnxt, t0 + 2 * G, last, wave, lane);
+        if (issue) mf_group_load<T, KS, G>(yb, cterm, nxt, cnxt, t0 + 2 * G, last, wave, lane);
 #pragma unroll
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
-            const int nyc = -ccur[cc * 32 + (lane & 31)];
-            i32x16_t acc[I8RT];
-            i8_tile<KS8>(cur + cc * NCH, a, xr, lane, acc);
-            if (__any(i8_min(acc) < nyc))
-                i8_emit(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
+            const term nyc = -ccur[cc * 32 + (lane & 31)];
+            accv acc[MRT];
+            mf_tile<T, KS>(cur + cc * NCH, a, xr, lane, acc);
+            if (__any(mf_min<T>(acc) < nyc))
+                mf_emit<T>(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap,
+                           counter);
         }
         ring_barrier(issue ? nld : 0);
     };
@@ -1329,16 +1362,18 @@ struct FilterVariant {
     bool lds;
     int ct;
 };
-// Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4.
+// Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4;
+// 11 and 12 the direct-to-LDS ring (three stages of 1 or 2 tiles).
 constexpr FilterVariant kVariants[] = {{2, false, 1}, {2, true, 1}, {4, true, 1}, {3, true, 1},
                                        {2, true, 2}, {1, true, 4}, {1, true, 2},
-                                       {2, true, 1}, {2, true, 2}, {2, true, 3}, {2, true, 4}};
-constexpr int kNumVariants = 11;
+                                       {2, true, 1}, {2, true, 2}, {2, true, 3}, {2, true, 4},
+                                       {2, true, 1}, {2, true, 2}};
+constexpr int kNumVariants = 13;
 int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
 
 int filter_variant() {
     if (g_variant < 0) {
-        g_variant = 1;
+        g_variant = 11;
         if (const char* e = getenv("RNNG_L2_VARIANT")) {
             const int v = atoi(e);
             if (v >= 0 && v < kNumVariants) g_variant = v;
@@ -1411,9 +1446,17 @@ void launch_filter_g(const FilterArgs& a) {
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
+template <int KS, int G>
+void launch_filter_ring(const FilterArgs& a) {
+    k_l2_mfma_filter_ring<MfBF16, KS, G, 1><<<a.n_items, 256, 0, stream()>>>(
+        a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+}
+
 template <int KS>
 void launch_filter_v(const FilterArgs& a, int variant) {
     switch (variant) {
+        case 11: launch_filter_ring<KS, 1>(a); return;
+        case 12: launch_filter_ring<KS, 2>(a); return;
         case 7: launch_filter_g<KS, 1>(a); return;
         case 8: launch_filter_g<KS, 2>(a); return;
         case 9: launch_filter_g<KS, 3>(a); return;
@@ -1448,7 +1491,7 @@ void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
             ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
             ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
     } else {
-        k_l2_mfma_filter_i8_dma<KS8, G><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
+        k_l2_mfma_filter_ring<MfI8, KS8, G, (KS8 <= 4 ? 4 : 1)><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
             ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
             ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
     }

@@ -814,10 +814,11 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Autotuned filter configurations: (variant, XCD ordering).  Variant 1 =
-// 2 row tiles per wave, 3 = 3, 2 = 4 (all with LDS-staged column tiles), 8 =
-// the direct-to-LDS group kernel with 2 column tiles per step.
-constexpr int kTuneVariants[4] = {1, 3, 2, 8};
+// Autotuned filter configurations: (variant, XCD ordering).  Variants 11 and
+// 12 = the direct-to-LDS ring with 1 or 2 column tiles per stage, 1 = column
+// tiles staged in LDS through registers, 8 = the direct-to-LDS group kernel
+// with 2 column tiles per step (all 2 row tiles per wave).
+constexpr int kTuneVariants[4] = {11, 12, 1, 8};
 constexpr int kPadRows = 1536;   // multiple of every tuned row block (256, 384, 512 rows)
 
 int tuning_enabled() {
