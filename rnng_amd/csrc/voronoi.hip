@@ -814,11 +814,11 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Autotuned filter configurations: (variant, XCD ordering).  Variants 11 and
-// 12 = the direct-to-LDS ring with 1 or 2 column tiles per stage, 1 = column
-// tiles staged in LDS through registers, 8 = the direct-to-LDS group kernel
-// with 2 column tiles per step (all 2 row tiles per wave).
-constexpr int kTuneVariants[4] = {11, 12, 1, 8};
+// Autotuned filter configurations: (variant, XCD ordering).  Variants 12 and
+// 11 = the direct-to-LDS ring with 2 or 1 column tiles per stage (2 row tiles
+// per wave), 2 and 1 = column tiles staged in LDS through registers with 4 or
+// 2 row tiles per wave.
+constexpr int kTuneVariants[4] = {12, 11, 2, 1};
 constexpr int kPadRows = 1536;   // multiple of every tuned row block (256, 384, 512 rows)
 
 int tuning_enabled() {
