@@ -815,40 +815,136 @@ constexpr int I8PAD = 1 << 28;   // start term of padding rows and columns
 
 __device__ __forceinline__ int floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
-// One thread per padded point: int8 fragments (tile t, k-step s of 32 dims,
-// lane l = 32 h + (a & 31) holds dims 32 s + 16 h .. +15) and the start terms.
+// One workgroup per 32-point tile: wave w writes k-steps s = w, w + 4, ...
+// (32 dims each) of the tile's int8 fragments; lane l = 32 h + (a & 31) holds
+// dims 32 s + 16 h .. +15 of point a, read as 64 contiguous bytes (float4
+// loads when rows are 16-byte aligned) and stored as one 16-byte word at
+// fragment index l (coalesced).  The row and column start terms come from the
+// per-lane sums of x'^2 and x', reduced over waves and halves in LDS.
+// Optional: hsq receives the exact |x'|^2 of every point and xrow its x' as
+// point-major int8 rows of KS8 * 32 bytes (for the exact recheck).
 __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, int n, int npad, int d, int KS8,
-                                                 const unsigned* __restrict__ colmin, int Tq,
+                                                 int vec4, const unsigned* __restrict__ colmin, int Tq,
                                                  i32x4_t* __restrict__ xa, i32x4_t* __restrict__ yb,
-                                                 int* __restrict__ rterm, int* __restrict__ cterm) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= npad) return;
+                                                 int* __restrict__ rterm, int* __restrict__ cterm,
+                                                 int* __restrict__ hsq, i32x4_t* __restrict__ xrow) {
+    __shared__ int off[160];      // o_k + 128 per dimension (d <= 160)
+    __shared__ int sh[4][64], ssx[4][64];
+    const int tile = blockIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int a = tile * 32 + (lane & 31);
+    const int half = lane >> 5;
     const bool real = a < n;
+    for (int k = tid; k < d; k += 256) off[k] = (int)ord2f(colmin[k]) + 128;
+    __syncthreads();
     int h = 0, sx = 0;
-    for (int s = 0; s < KS8; ++s) {
-        for (int half = 0; half < 2; ++half) {
-            unsigned wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u};
+    for (int s = wave; s < KS8; s += 4) {
+        const int k0 = 32 * s + 16 * half;
+        float v[16];
+        if (real && vec4 && k0 + 16 <= d) {
+            const float4* src = reinterpret_cast<const float4*>(X + (size_t)a * d + k0);
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                const int k = 32 * s + 16 * half + j;
-                int xv = 0, yv = 0;
-                if (real && k < d) {
-                    const int xp = (int)X[(size_t)a * d + k] - (int)ord2f(colmin[k]) - 128;
-                    xv = xp;
-                    yv = -1 - xp;
-                    h += xp * xp;
-                    sx += xp;
-                }
-                wa[j >> 2] |= (unsigned)(xv & 0xff) << (8 * (j & 3));
-                wb[j >> 2] |= (unsigned)(yv & 0xff) << (8 * (j & 3));
+            for (int q = 0; q < 4; ++q) {
+                const float4 f = src[q];
+                v[4 * q] = f.x;
+                v[4 * q + 1] = f.y;
+                v[4 * q + 2] = f.z;
+                v[4 * q + 3] = f.w;
             }
-            const size_t idx = ((size_t)(a >> 5) * KS8 + s) * 64 + half * 32 + (a & 31);
-            if (xa) xa[idx] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
-            if (yb) yb[idx] = i32x4_t{(int)wb[0], (int)wb[1], (int)wb[2], (int)wb[3]};
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = (real && k0 + j < d) ? X[(size_t)a * d + k0 + j] : 0.0f;
+        }
+        unsigned wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int k = k0 + j;
+            int xv = 0, yv = 0;
+            if (real && k < d) {
+                const int xp = (int)v[j] - off[k];
+                xv = xp;
+                yv = -1 - xp;
+                h += xp * xp;
+                sx += xp;
+            }
+            wa[j >> 2] |= (unsigned)(xv & 0xff) << (8 * (j & 3));
+            wb[j >> 2] |= (unsigned)(yv & 0xff) << (8 * (j & 3));
+        }
+        const size_t idx = ((size_t)tile * KS8 + s) * 64 + lane;
+        if (xa) xa[idx] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
+        if (yb) yb[idx] = i32x4_t{(int)wb[0], (int)wb[1], (int)wb[2], (int)wb[3]};
+        if (xrow) xrow[(size_t)a * (2 * KS8) + 2 * s + half] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
+    }
+    sh[wave][lane] = h;
+    ssx[wave][lane] = sx;
+    __syncthreads();
+    if (tid < 32 && a < npad) {
+        int ht = 0, st = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            ht += sh[w][tid] + sh[w][tid + 32];
+            st += ssx[w][tid] + ssx[w][tid + 32];
+        }
+        if (rterm) rterm[a] = real ? st + (ht >> 1) - 1 : I8PAD;
+        if (cterm) cterm[a] = real ? floor_half(ht - Tq) : I8PAD;
+        if (hsq) hsq[a] = real ? ht : 0;
+    }
+}
+
+// Exact recheck of int8-filter candidates from point-major int8 rows of x'
+// (KS8 * 32 bytes per point, 128 B at d = 128 instead of 512 B of FP32):
+//   d^2 = h_r + h_c - 2 x'_r.x'_c,
+// the exact integer the reference's FP32 sum yields on such data (every
+// partial sum is an integer below 2^24), so the edge test d^2 <= T, the
+// weight sqrtf(d^2) and the FP64 band check (sqrt of the same integer) match
+// the FP32 recheck bit for bit.  Eight lanes per candidate: lane k of a group
+// reads 16-byte words k, k + 8, ... of both rows (one coalesced 128-byte
+// segment per point), and the partial dot products are summed over the group.
+__global__ __launch_bounds__(256) void k_recheck_i8(const i32x4_t* __restrict__ xr8, const i32x4_t* __restrict__ yr8,
+                                                    const int* __restrict__ hr, const int* __restrict__ hc, int KS8,
+                                                    const int2* __restrict__ cand, unsigned long long ccap, int Tq,
+                                                    float eps, uint8_t* __restrict__ flag, float* __restrict__ cw,
+                                                    unsigned long long* __restrict__ counters) {
+    const unsigned long long nc = min(counters[0], ccap);
+    const double band = 1e-5 * (double)eps;
+    const int sub = threadIdx.x & 7;
+    const int nw = 2 * KS8;   // 16-byte words per row
+    for (unsigned long long base = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3; ;
+         base += ((unsigned long long)gridDim.x * blockDim.x) >> 3) {
+        // uniform over the wave: its 8 groups take candidates base .. base + 7
+        if ((base & ~7ull) >= nc) break;
+        const bool live = base < nc;
+        int dot = 0;
+        int2 rc = make_int2(0, 0);
+        if (live) {
+            rc = cand[base];
+            const i32x4_t* pr = xr8 + (size_t)rc.x * nw;
+            const i32x4_t* pc = yr8 + (size_t)rc.y * nw;
+            for (int j = sub; j < nw; j += 8) {
+                const i32x4_t u = pr[j];
+                const i32x4_t v = pc[j];
+                dot = __builtin_amdgcn_sdot4(u.x, v.x, dot, false);
+                dot = __builtin_amdgcn_sdot4(u.y, v.y, dot, false);
+                dot = __builtin_amdgcn_sdot4(u.z, v.z, dot, false);
+                dot = __builtin_amdgcn_sdot4(u.w, v.w, dot, false);
+            }
+        }
+        dot += __shfl_xor(dot, 1, 8);
+        dot += __shfl_xor(dot, 2, 8);
+        dot += __shfl_xor(dot, 4, 8);
+        if (live && sub == 0) {
+            const int d2 = hr[rc.x] + hc[rc.y] - 2 * dot;
+            const bool edge = d2 <= Tq;
+            const float w = sqrtf((float)d2);
+            if (fabs((double)w - (double)eps) <= band) {
+                atomicAdd(&counters[2], 1ull);
+                const bool in64 = sqrt((double)d2) <= (double)eps;
+                if (in64 != edge) atomicAdd(&counters[3], 1ull);
+            }
+            flag[base] = edge ? 1 : 0;
+            cw[base] = w;
         }
     }
-    if (rterm) rterm[a] = real ? sx + (h >> 1) - 1 : I8PAD;
-    if (cterm) cterm[a] = real ? floor_half(h - Tq) : I8PAD;
 }
 
 // Shared pieces of the filter kernels with row terms as the first MFMA's C
@@ -1349,6 +1445,8 @@ struct Workspace {
     unsigned long long cand_cap = 0;
     DevBuf xa, yb;     // fragments
     DevBuf xn, yn, ycn;   // row norms, column norms, column norms - C0
+    DevBuf hx, hy;        // exact |x'|^2 per row / column point (int8 path)
+    DevBuf xr8, yr8;      // x' as point-major int8 rows (int8 path recheck)
     DevBuf colsum, colmin, colmax, stat, range, mu, params;
     DevBuf work;
     std::vector<int4> work_host;
@@ -1554,6 +1652,18 @@ FilterArgs ws_args(Workspace& ws, int n_rows, int n_cols, int self_join) {
     return a;
 }
 
+int launch_recheck_i8(const Workspace& ws, int KS8, bool self_join, unsigned long long ncand, int Tq, float eps,
+                      uint8_t* flag, float* cw) {
+    if (ncand == 0) return RNNG_OK;
+    const unsigned grid = (unsigned)std::min<unsigned long long>(8192, (ncand + 31) / 32);
+    k_recheck_i8<<<grid, 256, 0, stream()>>>(ws.xr8.as<i32x4_t>(), self_join ? ws.xr8.as<i32x4_t>() : ws.yr8.as<i32x4_t>(),
+                                             ws.hx.as<int>(), self_join ? ws.hx.as<int>() : ws.hy.as<int>(), KS8,
+                                             ws.cand.as<int2>(), ws.cand_cap, Tq, eps, flag, cw,
+                                             ws.counters.as<unsigned long long>());
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
 int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
               rnng_graph_s* g, rnng_stats* st) {
     Workspace& ws = g_ws;
@@ -1703,21 +1813,27 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         RNNG_TRY(edge_buffers(g, guess));
     }
 
+    // T' = floor(T), clamped: squared distances of int8-eligible data stay below 2^24
+    const int Tq = T >= 1073741823.0f ? 1073741823 : (T < 0.0f ? -1 : (int)std::floor(T));
     if (i8) {
-        // T' = floor(T), clamped: squared distances of such data stay below 2^24
-        const int Tq = T >= 1073741823.0f ? 1073741823 : (T < 0.0f ? -1 : (int)std::floor(T));
+        RNNG_TRY(ws.hx.reserve((size_t)npad * sizeof(int)));
+        RNNG_TRY(ws.xr8.reserve((size_t)npad * KS8 * 32));
+        if (!self_join) {
+            RNNG_TRY(ws.hy.reserve((size_t)npad_y * sizeof(int)));
+            RNNG_TRY(ws.yr8.reserve((size_t)npad_y * KS8 * 32));
+        }
+        auto vec_ok = [&](const float* X) { return (d % 4 == 0 && ((uintptr_t)X % 16) == 0) ? 1 : 0; };
         if (self_join) {
-            k_prep_i8<<<blocks_for(npad, 256), 256, 0, stream()>>>(A->X, nx, npad, d, KS8, ws.colmin.as<unsigned>(),
-                                                                  Tq, ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(),
-                                                                  ws.xn.as<int>(), ws.ycn.as<int>());
+            k_prep_i8<<<npad / 32, 256, 0, stream()>>>(A->X, nx, npad, d, KS8, vec_ok(A->X), ws.colmin.as<unsigned>(),
+                                                       Tq, ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(),
+                                                       ws.ycn.as<int>(), ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
         } else {
-            k_prep_i8<<<blocks_for(npad_x, 256), 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8,
-                                                                    ws.colmin.as<unsigned>(), Tq,
-                                                                    ws.xa.as<i32x4_t>(), nullptr, ws.xn.as<int>(),
-                                                                    nullptr);
-            k_prep_i8<<<blocks_for(npad_y, 256), 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8,
-                                                                    ws.colmin.as<unsigned>(), Tq, nullptr,
-                                                                    ws.yb.as<i32x4_t>(), nullptr, ws.ycn.as<int>());
+            k_prep_i8<<<npad_x / 32, 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8, vec_ok(A->X),
+                                                         ws.colmin.as<unsigned>(), Tq, ws.xa.as<i32x4_t>(), nullptr,
+                                                         ws.xn.as<int>(), nullptr, ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
+            k_prep_i8<<<npad_y / 32, 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8, vec_ok(B->X),
+                                                         ws.colmin.as<unsigned>(), Tq, nullptr, ws.yb.as<i32x4_t>(),
+                                                         nullptr, ws.ycn.as<int>(), ws.hy.as<int>(), ws.yr8.as<i32x4_t>());
         }
         RNNG_HIP_TRY(hipGetLastError());
     } else if (self_join) {
@@ -1768,8 +1884,11 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         RNNG_TRY(ws.cw.reserve(std::max<size_t>(1, ncand) * sizeof(float)));
         if ((int64_t)ncand > g->cap) RNNG_TRY(edge_buffers(g, (int64_t)ncand));
         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
-        RNNG_TRY(launch_recheck(A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, ncand, T, eps,
-                                ws.cflag.as<uint8_t>(), ws.cw.as<float>(), ws.counters.as<unsigned long long>()));
+        if (i8)
+            RNNG_TRY(launch_recheck_i8(ws, KS8, self_join, ncand, Tq, eps, ws.cflag.as<uint8_t>(), ws.cw.as<float>()));
+        else
+            RNNG_TRY(launch_recheck(A->X, B->X, d, ws.cand.as<int2>(), ws.cand_cap, ncand, T, eps,
+                                    ws.cflag.as<uint8_t>(), ws.cw.as<float>(), ws.counters.as<unsigned long long>()));
         int64_t ne = 0;
         RNNG_TRY(compact_edges(ws.cand.as<int2>(), ws.cw.as<float>(), ws.cflag.as<uint8_t>(), (int64_t)ncand,
                                g->ij.as<int2>(), g->w.as<float>(), &ne));
