@@ -214,26 +214,65 @@ constexpr int NSTAT = 8;
 enum { ST_INT = 0, ST_MAXE = 1, ST_MAXN = 2, ST_MAXRES = 3, ST_MAXDEV = 4 };
 
 
-// Per-dimension sums, minima and maxima, and the all-integer flag.
+// Per-dimension sums, minima and maxima, and the all-integer flag.  A block
+// covers COLSTAT_ROWS rows; for d <= 256 its threads split into P = 256 / d
+// row phases (thread t: column t % d, rows r0 + t / d, + P, ...), so every
+// thread streams coalesced row segments, and the phases are combined in LDS
+// before one set of atomics per column and block.  Wider rows loop over
+// columns with a stride of 256.
+constexpr int COLSTAT_ROWS = 1024;
+
 __global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ X, int n, int d,
                                                   double* __restrict__ colsum,
                                                   unsigned* __restrict__ colmin,
                                                   unsigned* __restrict__ colmax,
                                                   unsigned* __restrict__ stat) {
-    const int r0 = blockIdx.x * 256;
-    const int r1 = min(n, r0 + 256);
+    __shared__ double ssum[256];
+    __shared__ float smin[256], smax[256];
+    const int t = threadIdx.x;
+    const int r0 = blockIdx.x * COLSTAT_ROWS;
+    const int r1 = min(n, r0 + COLSTAT_ROWS);
     bool isint = true;
-    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+    if (d <= 256) {
+        const int P = 256 / d;
+        const int k = t % d, ph = t / d;
         double s = 0.0;
         float mn = INFINITY, mx = -INFINITY;
-        for (int r = r0; r < r1; ++r) {
-            const float v = X[(size_t)r * d + k];
-            s += (double)v;
-            mn = fminf(mn, v);
-            mx = fmaxf(mx, v);
-            isint = isint && (v == rintf(v));
+        if (ph < P) {
+#pragma unroll 4
+            for (int r = r0 + ph; r < r1; r += P) {
+                const float v = X[(size_t)r * d + k];
+                s += (double)v;
+                mn = fminf(mn, v);
+                mx = fmaxf(mx, v);
+                isint = isint && (v == rintf(v));
+            }
         }
-        if (r1 > r0) {
+        ssum[t] = s;
+        smin[t] = mn;
+        smax[t] = mx;
+        __syncthreads();
+        if (t < d) {
+            for (int q = 1; q < P; ++q) {
+                s += ssum[t + q * d];
+                mn = fminf(mn, smin[t + q * d]);
+                mx = fmaxf(mx, smax[t + q * d]);
+            }
+            atomicAdd(&colsum[t], s);
+            atomicMin(&colmin[t], f2ord(mn));
+            atomicMax(&colmax[t], f2ord(mx));
+        }
+    } else {
+        for (int k = t; k < d; k += 256) {
+            double s = 0.0;
+            float mn = INFINITY, mx = -INFINITY;
+            for (int r = r0; r < r1; ++r) {
+                const float v = X[(size_t)r * d + k];
+                s += (double)v;
+                mn = fminf(mn, v);
+                mx = fmaxf(mx, v);
+                isint = isint && (v == rintf(v));
+            }
             atomicAdd(&colsum[k], s);
             atomicMin(&colmin[k], f2ord(mn));
             atomicMax(&colmax[k], f2ord(mx));
@@ -1754,7 +1793,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int q = 0; q < nsets; ++q) {
         const rnng_dataset_s* S = sets[q];
         if (S->n > 0)
-            k_colstats<<<blocks_for(S->n, 256), 256, 0, stream()>>>(
+            k_colstats<<<blocks_for(S->n, COLSTAT_ROWS), 256, 0, stream()>>>(
                 S->X, (int)S->n, d, ws.colsum.as<double>(), ws.colmin.as<unsigned>(),
                 ws.colmax.as<unsigned>(), ws.stat.as<unsigned>());
         ntot += (int)S->n;
@@ -2020,7 +2059,7 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
     RNNG_TRY(launch_fill_u32(P.stat.as<unsigned>() + ST_MAXDEV, 1, 0x80000000u));
     RNNG_TRY(launch_fill_u32(P.range.as<unsigned>(), 1, 0x80000000u));
     if (n > 0)
-        k_colstats<<<blocks_for(n, 256), 256, 0, stream()>>>(X, n, d, P.colsum.as<double>(),
+        k_colstats<<<blocks_for(n, COLSTAT_ROWS), 256, 0, stream()>>>(X, n, d, P.colsum.as<double>(),
                                                              P.colmin.as<unsigned>(), P.colmax.as<unsigned>(),
                                                              P.stat.as<unsigned>());
     k_center<<<blocks_for(d, 64), 64, 0, stream()>>>(P.colsum.as<double>(), P.colmin.as<unsigned>(),
