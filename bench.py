@@ -129,7 +129,8 @@ def bench_cells(args):
     # every unordered pair once at 2 d flops (the padded tiles the filter runs
     # are cell_pair_evals)
     ach = 2.0 * d * P_all / (st.filter_ms / K * 1e-3) / 1e12
-    kname = {11: "k_l2_mfma_filter_ring<bf16, G=1>", 12: "k_l2_mfma_filter_ring<bf16, G=2>",
+    kname = {11: "k_l2_mfma_filter_ring<bf16, G=1, S=3>", 12: "k_l2_mfma_filter_ring<bf16, G=2, S=3>",
+             13: "k_l2_mfma_filter_ring<bf16, G=2, S=2>", 14: "k_l2_mfma_filter_ring<bf16, G=3, S=2>",
              8: "k_l2_mfma_filter_g<G=2>"}.get(tv, "k_l2_mfma_filter")
     out["roofline"] = {"kernel": f"{kname} (autotuned variant {tv})", "bound": "mfma", "achieved": ach,
                        "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0, "traffic": None,

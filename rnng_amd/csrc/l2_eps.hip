@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
 }
 
 // Barrier of the direct-to-LDS ring: wait until at most n of this wave's
-// vector-memory operations are outstanding (n <= 4, wave-uniform) and its LDS
+// vector-memory operations are outstanding (n <= 15, wave-uniform) and its LDS
 // operations are done, then s_barrier.  A __syncthreads() would wait for every
 // direct-to-LDS load in flight (its fence covers them), including the ones
 // issued for the group after next; the LDS data the loop reads after this
@@ -1188,7 +1188,18 @@ __device__ __forceinline__ void ring_barrier(int n) {
         case 1: __builtin_amdgcn_s_waitcnt(0x071); break;
         case 2: __builtin_amdgcn_s_waitcnt(0x072); break;
         case 3: __builtin_amdgcn_s_waitcnt(0x073); break;
-        default: __builtin_amdgcn_s_waitcnt(0x074); break;
+        case 4: __builtin_amdgcn_s_waitcnt(0x074); break;
+        case 5: __builtin_amdgcn_s_waitcnt(0x075); break;
+        case 6: __builtin_amdgcn_s_waitcnt(0x076); break;
+        case 7: __builtin_amdgcn_s_waitcnt(0x077); break;
+        case 8: __builtin_amdgcn_s_waitcnt(0x078); break;
+        case 9: __builtin_amdgcn_s_waitcnt(0x079); break;
+        case 10: __builtin_amdgcn_s_waitcnt(0x07a); break;
+        case 11: __builtin_amdgcn_s_waitcnt(0x07b); break;
+        case 12: __builtin_amdgcn_s_waitcnt(0x07c); break;
+        case 13: __builtin_amdgcn_s_waitcnt(0x07d); break;
+        case 14: __builtin_amdgcn_s_waitcnt(0x07e); break;
+        default: __builtin_amdgcn_s_waitcnt(0x07f); break;
     }
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
@@ -1198,7 +1209,8 @@ __device__ __forceinline__ void ring_barrier(int n) {
 // This wave's share of the loads of one group (G column tiles from t0,
 // clamped to the last tile): fragment chunks j = wave, wave + 4, ... straight
 // into LDS (global_load_lds_dwordx4), and, from wave 3, the group's column
-// start terms (global_load_lds_dword, two tiles per instruction).
+// start terms (global_load_lds_dword, two tiles per instruction, (G + 1) / 2
+// instructions).
 template <class T, int KS, int G>
 __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict__ yb,
                                               const typename T::term* __restrict__ cterm, typename T::frag* dst,
@@ -1214,18 +1226,22 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
                                          16, 0, 0);
     }
     if (wave == 3) {
-        const typename T::term* src = cterm + min(t0 + (lane >> 5), last) * 32 + (lane & 31);
-        __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)cdst, 4, 0, 0);
+#pragma unroll
+        for (int c2 = 0; c2 < (G + 1) / 2; ++c2) {
+            const typename T::term* src = cterm + min(t0 + 2 * c2 + (lane >> 5), last) * 32 + (lane & 31);
+            __builtin_amdgcn_global_load_lds((const void*)src,
+                                             (__attribute__((address_space(3))) void*)(cdst + 64 * c2), 4, 0, 0);
+        }
     }
 }
 
 // Direct-to-LDS ring: column tiles go from global memory straight into a ring
-// of three LDS stages of G tiles each (no staging registers), issued two
-// groups ahead of the group being computed, so a group's loads have two
-// groups' time to land; one barrier per group.  The stages are distinct LDS
-// objects and the loop is unrolled by three, so the compiler sees that reads
-// of one stage never alias the asynchronous loads into the others.
-template <class T, int KS, int G, int MINW>
+// of S LDS stages of G tiles each (no staging registers), issued S - 1 groups
+// ahead of the group being computed, so a group's loads have S - 1 groups'
+// time to land; one barrier per group.  The stages are distinct LDS objects
+// and the loop is unrolled by S, so the compiler sees that reads of one stage
+// never alias the asynchronous loads into the others.
+template <class T, int KS, int G, int MINW, int S = 3>
 __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
     const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
@@ -1236,9 +1252,10 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     using accv = typename T::accv;
     constexpr int NCH = KS * 64;
     constexpr int NJ = G * KS;
-    static_assert(G == 1 || G == 2, "the start-term load covers two tiles");
-    __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[G * NCH];
-    __shared__ term cs0[64], cs1[64], cs2[64];
+    constexpr int NC = 64 * ((G + 1) / 2);   // start terms per stage (two tiles per load)
+    static_assert(G >= 1 && G <= 4 && (S == 2 || S == 3), "ring shape");
+    __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
+    __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
     __shared__ int2 stage[LSTAGE];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
@@ -1252,8 +1269,8 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
     // loads this wave issues per group (the same in every lane)
-    const int nld =
-        __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / 4 + 1 : 0) + (wave == 3 ? 1 : 0));
+    const int nld = __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / 4 + 1 : 0) +
+                                                   (wave == 3 ? (G + 1) / 2 : 0));
 
     // issue order: rows, group 0, group 1 (the barrier's count relies on it)
     frag a[MRT][KS];
@@ -1262,16 +1279,18 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     asm volatile("" ::: "memory");
     mf_group_load<T, KS, G>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
     asm volatile("" ::: "memory");
-    if (ng > 1) mf_group_load<T, KS, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
-    // rows and group 0 landed (group 1 may still be in flight); scount is set
-    ring_barrier(ng > 1 ? nld : 0);
+    if (S == 3 && ng > 1) mf_group_load<T, KS, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
+    // rows and group 0 landed (with three stages group 1 may still be in
+    // flight); scount is set
+    ring_barrier(S == 3 && ng > 1 ? nld : 0);
 
-    // group g from stage cur; the loads of group g + 2 go to stage nxt (group
-    // g - 1's, free since the last barrier); then group g + 1 must have landed
+    // group g from stage cur; the loads of group g + S - 1 go to stage nxt
+    // (group g - 1's, free since the last barrier); then group g + 1 must
+    // have landed (with two stages: the loads just issued)
     auto step = [&](const frag* cur, const term* ccur, frag* nxt, term* cnxt, int g) {
         const int t0 = wi.y + g * G;
-        const bool issue = g + 2 < ng;
-        if (issue) mf_group_load<T, KS, G>(yb, cterm, nxt, cnxt, t0 + 2 * G, last, wave, lane);
+        const bool issue = g + S - 1 < ng;
+        if (issue) mf_group_load<T, KS, G>(yb, cterm, nxt, cnxt, t0 + (S - 1) * G, last, wave, lane);
 #pragma unroll
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
@@ -1282,16 +1301,25 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
                 mf_emit<T>(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap,
                            counter);
         }
-        ring_barrier(issue ? nld : 0);
+        ring_barrier(S == 3 && issue ? nld : 0);
     };
     int g = 0;
-    for (;;) {
-        step(bs0, cs0, bs2, cs2, g);
-        if (++g >= ng) break;
-        step(bs1, cs1, bs0, cs0, g);
-        if (++g >= ng) break;
-        step(bs2, cs2, bs1, cs1, g);
-        if (++g >= ng) break;
+    if constexpr (S == 3) {
+        for (;;) {
+            step(bs0, cs0, bs2, cs2, g);
+            if (++g >= ng) break;
+            step(bs1, cs1, bs0, cs0, g);
+            if (++g >= ng) break;
+            step(bs2, cs2, bs1, cs1, g);
+            if (++g >= ng) break;
+        }
+    } else {
+        for (;;) {
+            step(bs0, cs0, bs1, cs1, g);
+            if (++g >= ng) break;
+            step(bs1, cs1, bs0, cs0, g);
+            if (++g >= ng) break;
+        }
     }
     __syncthreads();   // every wave's staged candidates and count
     const unsigned nst = min(scount, (unsigned)LSTAGE);
@@ -1500,17 +1528,18 @@ struct FilterVariant {
     int ct;
 };
 // Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4;
-// 11 and 12 the direct-to-LDS ring (three stages of 1 or 2 tiles).
+// 11 and 12 the direct-to-LDS ring (three stages of 1 or 2 tiles), 13 and 14
+// the ring with two stages of 2 or 3 tiles.
 constexpr FilterVariant kVariants[] = {{2, false, 1}, {2, true, 1}, {4, true, 1}, {3, true, 1},
                                        {2, true, 2}, {1, true, 4}, {1, true, 2},
                                        {2, true, 1}, {2, true, 2}, {2, true, 3}, {2, true, 4},
-                                       {2, true, 1}, {2, true, 2}};
-constexpr int kNumVariants = 13;
+                                       {2, true, 1}, {2, true, 2}, {2, true, 2}, {2, true, 3}};
+constexpr int kNumVariants = 15;
 int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
 
 int filter_variant() {
     if (g_variant < 0) {
-        g_variant = 11;
+        g_variant = 13;
         if (const char* e = getenv("RNNG_L2_VARIANT")) {
             const int v = atoi(e);
             if (v >= 0 && v < kNumVariants) g_variant = v;
@@ -1583,9 +1612,9 @@ void launch_filter_g(const FilterArgs& a) {
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
-template <int KS, int G>
+template <int KS, int G, int S = 3>
 void launch_filter_ring(const FilterArgs& a) {
-    k_l2_mfma_filter_ring<MfBF16, KS, G, 1><<<a.n_items, 256, 0, stream()>>>(
+    k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S><<<a.n_items, 256, 0, stream()>>>(
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
 }
 
@@ -1594,6 +1623,8 @@ void launch_filter_v(const FilterArgs& a, int variant) {
     switch (variant) {
         case 11: launch_filter_ring<KS, 1>(a); return;
         case 12: launch_filter_ring<KS, 2>(a); return;
+        case 13: launch_filter_ring<KS, 2, 2>(a); return;
+        case 14: launch_filter_ring<KS, 3, 2>(a); return;
         case 7: launch_filter_g<KS, 1>(a); return;
         case 8: launch_filter_g<KS, 2>(a); return;
         case 9: launch_filter_g<KS, 3>(a); return;
@@ -1620,51 +1651,72 @@ float ord2f_host(unsigned u) {
     return f;
 }
 
-// G = 0: k_l2_mfma_filter_i8; G = 1, 2: the direct-to-LDS kernel
-template <int KS8, int G>
+// int8 filter kernels by code: 0 = k_l2_mfma_filter_i8 (column tiles staged
+// through registers); 1, 2 = the direct-to-LDS ring with 1 or 2 tiles per
+// stage and three stages; 3, 4 = 3 or 4 tiles per stage and two stages.
+template <int KS8, int CODE>
 void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
-    if constexpr (G == 0) {
-        k_l2_mfma_filter_i8<KS8><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-            ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
-            ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
-    } else {
-        k_l2_mfma_filter_ring<MfI8, KS8, G, (KS8 <= 4 ? 4 : 1)><<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-            ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(), ws.ycn.as<int>(), ws.work.as<int4>(), nx,
-            ny, self_join, ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
-    }
+    constexpr int MINW = KS8 <= 4 ? 4 : 1;
+    const unsigned nb = (unsigned)ws.work_host.size();
+    const i32x4_t* xa = ws.xa.as<i32x4_t>();
+    const i32x4_t* yb = ws.yb.as<i32x4_t>();
+    const int* xn = ws.xn.as<int>();
+    const int* ycn = ws.ycn.as<int>();
+    const int4* wk = ws.work.as<int4>();
+    int2* cand = ws.cand.as<int2>();
+    unsigned long long* cnt = ws.counters.as<unsigned long long>();
+    if constexpr (CODE == 0)
+        k_l2_mfma_filter_i8<KS8><<<nb, 256, 0, stream()>>>(xa, yb, xn, ycn, wk, nx, ny, self_join, cand, ws.cand_cap,
+                                                           cnt);
+    else if constexpr (CODE <= 2)
+        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<nb, 256, 0, stream()>>>(xa, yb, xn, ycn, wk, nx, ny,
+                                                                                  self_join, cand, ws.cand_cap, cnt);
+    else
+        k_l2_mfma_filter_ring<MfI8, KS8, CODE, (KS8 <= 4 && CODE == 3 ? 4 : 1), 2><<<nb, 256, 0, stream()>>>(
+            xa, yb, xn, ycn, wk, nx, ny, self_join, cand, ws.cand_cap, cnt);
 }
 
-int g_i8_group = -1;   // int8 filter kernel: 0 registers-staged, 1 or 2 direct-to-LDS tiles per stage
+int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
 
 int i8_group() {
     if (g_i8_group < 0) {
-        g_i8_group = 2;
+        g_i8_group = 3;
         if (const char* e = getenv("RNNG_L2_I8_GROUP")) {
             const int v = atoi(e);
-            if (v >= 0 && v <= 2) g_i8_group = v;
+            if (v >= 0 && v <= 4) g_i8_group = v;
         }
     }
     return g_i8_group;
 }
 
-int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8, int G) {
+int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8, int code) {
     if (ws.work_host.empty()) return RNNG_OK;
-    switch (KS8 * 4 + G) {
-        case 1 * 4 + 0: launch_filter_i8<1, 0>(ws, nx, ny, self_join); break;
-        case 2 * 4 + 0: launch_filter_i8<2, 0>(ws, nx, ny, self_join); break;
-        case 3 * 4 + 0: launch_filter_i8<3, 0>(ws, nx, ny, self_join); break;
-        case 4 * 4 + 0: launch_filter_i8<4, 0>(ws, nx, ny, self_join); break;
-        case 5 * 4 + 0: launch_filter_i8<5, 0>(ws, nx, ny, self_join); break;
-        case 1 * 4 + 1: launch_filter_i8<1, 1>(ws, nx, ny, self_join); break;
-        case 2 * 4 + 1: launch_filter_i8<2, 1>(ws, nx, ny, self_join); break;
-        case 3 * 4 + 1: launch_filter_i8<3, 1>(ws, nx, ny, self_join); break;
-        case 4 * 4 + 1: launch_filter_i8<4, 1>(ws, nx, ny, self_join); break;
-        case 5 * 4 + 1: launch_filter_i8<5, 1>(ws, nx, ny, self_join); break;
-        case 1 * 4 + 2: launch_filter_i8<1, 2>(ws, nx, ny, self_join); break;
-        case 2 * 4 + 2: launch_filter_i8<2, 2>(ws, nx, ny, self_join); break;
-        case 3 * 4 + 2: launch_filter_i8<3, 2>(ws, nx, ny, self_join); break;
-        case 4 * 4 + 2: launch_filter_i8<4, 2>(ws, nx, ny, self_join); break;
-        case 5 * 4 + 2: launch_filter_i8<5, 2>(ws, nx, ny, self_join); break;
+    switch (KS8 * 8 + code) {
+        case 1 * 8 + 0: launch_filter_i8<1, 0>(ws, nx, ny, self_join); break;
+        case 2 * 8 + 0: launch_filter_i8<2, 0>(ws, nx, ny, self_join); break;
+        case 3 * 8 + 0: launch_filter_i8<3, 0>(ws, nx, ny, self_join); break;
+        case 4 * 8 + 0: launch_filter_i8<4, 0>(ws, nx, ny, self_join); break;
+        case 5 * 8 + 0: launch_filter_i8<5, 0>(ws, nx, ny, self_join); break;
+        case 1 * 8 + 1: launch_filter_i8<1, 1>(ws, nx, ny, self_join); break;
+        case 2 * 8 + 1: launch_filter_i8<2, 1>(ws, nx, ny, self_join); break;
+        case 3 * 8 + 1: launch_filter_i8<3, 1>(ws, nx, ny, self_join); break;
+        case 4 * 8 + 1: launch_filter_i8<4, 1>(ws, nx, ny, self_join); break;
+        case 5 * 8 + 1: launch_filter_i8<5, 1>(ws, nx, ny, self_join); break;
+        case 1 * 8 + 2: launch_filter_i8<1, 2>(ws, nx, ny, self_join); break;
+        case 2 * 8 + 2: launch_filter_i8<2, 2>(ws, nx, ny, self_join); break;
+        case 3 * 8 + 2: launch_filter_i8<3, 2>(ws, nx, ny, self_join); break;
+        case 4 * 8 + 2: launch_filter_i8<4, 2>(ws, nx, ny, self_join); break;
+        case 5 * 8 + 2: launch_filter_i8<5, 2>(ws, nx, ny, self_join); break;
+        case 1 * 8 + 3: launch_filter_i8<1, 3>(ws, nx, ny, self_join); break;
+        case 2 * 8 + 3: launch_filter_i8<2, 3>(ws, nx, ny, self_join); break;
+        case 3 * 8 + 3: launch_filter_i8<3, 3>(ws, nx, ny, self_join); break;
+        case 4 * 8 + 3: launch_filter_i8<4, 3>(ws, nx, ny, self_join); break;
+        case 5 * 8 + 3: launch_filter_i8<5, 3>(ws, nx, ny, self_join); break;
+        case 1 * 8 + 4: launch_filter_i8<1, 4>(ws, nx, ny, self_join); break;
+        case 2 * 8 + 4: launch_filter_i8<2, 4>(ws, nx, ny, self_join); break;
+        case 3 * 8 + 4: launch_filter_i8<3, 4>(ws, nx, ny, self_join); break;
+        case 4 * 8 + 4: launch_filter_i8<4, 4>(ws, nx, ny, self_join); break;
+        case 5 * 8 + 4: launch_filter_i8<5, 4>(ws, nx, ny, self_join); break;
         default:
             set_error("int8 filter supports d <= 160");
             return RNNG_ERR_UNSUPPORTED;

@@ -814,11 +814,11 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Autotuned filter configurations: (variant, XCD ordering).  Variants 12 and
-// 11 = the direct-to-LDS ring with 2 or 1 column tiles per stage (2 row tiles
-// per wave), 2 and 1 = column tiles staged in LDS through registers with 4 or
-// 2 row tiles per wave.
-constexpr int kTuneVariants[4] = {12, 11, 2, 1};
+// Autotuned filter configurations: (variant, XCD ordering).  Variants 13 and
+// 14 = the direct-to-LDS ring with two stages of 2 or 3 column tiles, 12 =
+// three stages of 2 tiles (all 2 row tiles per wave), 2 = column tiles staged
+// in LDS through registers with 4 row tiles per wave.
+constexpr int kTuneVariants[4] = {13, 14, 12, 2};
 constexpr int kPadRows = 1536;   // multiple of every tuned row block (256, 384, 512 rows)
 
 int tuning_enabled() {
