@@ -1553,9 +1553,21 @@ Workspace g_ws;
 int64_t g_cand_hint = 0;   // initial candidate-buffer size (0: default)
 
 
-// Row blocks of 8 tiles (256 rows) x column chunks of CH tiles, chunk-major
-// so that the workgroups resident at one time share column fragments in L2.
-constexpr int CH = 256;
+// Row blocks of 8 tiles (256 rows) x column chunks of chunk_tiles() tiles,
+// chunk-major so that the workgroups resident at one time share column
+// fragments in L2 (RNNG_L2_CHUNK overrides the default of 256).
+int g_chunk = 0;
+
+int chunk_tiles() {
+    if (g_chunk == 0) {
+        g_chunk = 256;
+        if (const char* e = getenv("RNNG_L2_CHUNK")) {
+            const int v = atoi(e);
+            if (v >= 32 && v <= 4096) g_chunk = v;
+        }
+    }
+    return g_chunk;
+}
 
 // Workgroups are dispatched to the 8 XCDs round-robin by block index, and each
 // XCD has its own L2.  With xcd_remap the list is permuted so that XCD x runs
@@ -1573,6 +1585,7 @@ int xcd_remap() {
 
 int build_work(Workspace& ws, int n_rowblocks, int rb_tiles, int n_coltiles, int self_join) {
     const int remap = xcd_remap();
+    const int CH = chunk_tiles();
     if (ws.work_key[0] == n_rowblocks && ws.work_key[1] == n_coltiles &&
         ws.work_key[2] == self_join && ws.work_key[3] == CH * 2 + remap && ws.work_key[4] == rb_tiles)
         return RNNG_OK;
