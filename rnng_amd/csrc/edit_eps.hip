@@ -157,9 +157,12 @@ __global__ void k_debug_mfma_i8(const i32x4_t* __restrict__ a, const i32x4_t* __
 // One work item = rows [ERB wi.x, +ERB) (two row tiles per wave, fragments in
 // registers) against column tiles [wi.y, wi.z), which the workgroup stages in
 // LDS (double buffered, one barrier per tile).  The accumulator of pair
-// (a, b) starts at |p_a|_4 + |p_b|_4 - 6k - 1 and the MFMAs add
-// -2 <t(p_a), t(p_b)>, so its sign bit is set iff L1_4 <= 6k.  The hot loop
-// ORs the sign bits; tiles with a set bit take the candidate path, which
+// (a, b) is |p_a|_4 + |p_b|_4 - 6k - 1 - 2 <t(p_a), t(p_b)>, negative iff
+// L1_4 <= 6k.  The row part |p_a|_4 is the first MFMA's C operand (no
+// per-value start adds), so the accumulator ends at |p_a|_4 - 2 <t, t> and
+// the pair survives iff that is below -(|p_b|_4 - 6k - 1); a lane's 16 values
+// share one column, so the hot loop tests the minimum of the lane's
+// accumulators.  Tiles with a survivor take the candidate path, which
 // keeps a < b < n (sorted positions) inside the length window and stages
 // (a, b) in LDS.  It reads no global memory: on this filter most tiles hold a
 // candidate, and a load per set bit would stall the whole workgroup at the
@@ -187,7 +190,7 @@ __global__ __launch_bounds__(256) void k_edit_filter(
     for (int r = 0; r < ERT; ++r)
 #pragma unroll
         for (int s = 0; s < EKS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * EKS + s) * 64 + lane];
-    int xr[ERT][16];
+    i32x16_t xr[ERT];
 #pragma unroll
     for (int r = 0; r < ERT; ++r)
 #pragma unroll
@@ -215,22 +218,20 @@ __global__ __launch_bounds__(256) void k_edit_filter(
         }
         i32x16_t acc[ERT];
 #pragma unroll
-        for (int r = 0; r < ERT; ++r)
-#pragma unroll
-            for (int q = 0; q < 16; ++q) acc[r][q] = xr[r][q] + yc;
-#pragma unroll
         for (int s = 0; s < EKS; ++s) {
             const i32x4_t bb = bs[buf][s * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < ERT; ++r) acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, acc[r], 0, 0, 0);
+            for (int r = 0; r < ERT; ++r)
+                acc[r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[r][s], bb, s == 0 ? xr[r] : acc[r], 0, 0, 0);
         }
-        int o = 0;
+        const int nyc = -yc;
+        int mn = acc[0][0];
 #pragma unroll
         for (int r = 0; r < ERT; ++r)
 #pragma unroll
-            for (int q = 0; q < 16; ++q) o |= acc[r][q];
+            for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = min(mn, acc[r][q]);
 
-        if (__any(o < 0)) {
+        if (__any(mn < nyc)) {
             // Candidate path: each lane walks its own set sign bits (a short
             // divergent loop: a tile holds a few survivors), reads the row's
             // length from LDS and takes stage slots with LDS atomics.  The
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(256) void k_edit_filter(
 #pragma unroll
             for (int r = 0; r < ERT; ++r)
 #pragma unroll
-                for (int q = 0; q < 16; ++q) mask |= ((unsigned)acc[r][q] >> 31) << (r * 16 + q + opaque);
+                for (int q = 0; q < 16; ++q) mask |= (acc[r][q] < nyc ? 1u : 0u) << (r * 16 + q + opaque);
             const int col = ct * 32 + (lane & 31);
             const int rowb = 4 * (lane >> 5);
             if (col >= n) mask = 0;
