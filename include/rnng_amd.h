@@ -363,6 +363,17 @@ int rnng_debug_mfma_layout(int32_t *rows, int32_t *cols);
  * Same edges either way; a switch for comparisons. */
 int rnng_set_int8_filter(int32_t enable);
 
+/* Filter kernel shapes, process-wide (no counterpart in the reference: for
+ * measurement and for tests that run every shape).  int8_code: 0 column tiles
+ * staged through registers, 1/2 direct-to-LDS ring with three stages of 1/2
+ * tiles, 3/4 two stages of 3/4 tiles.  bf16_variant 0..14 (11/12 three-stage
+ * ring, 13/14 two-stage ring of 2/3 tiles, the others register-staged or
+ * group-load shapes).  A negative value leaves that setting unchanged; the
+ * defaults (3, 13) come from RNNG_L2_I8_GROUP / RNNG_L2_VARIANT when set.
+ * Every shape gives the same edges. */
+int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant);
+int rnng_get_filter_kernels(int32_t *int8_code, int32_t *bf16_variant);
+
 /* Runs one 32x32x32 int8 MFMA (the edit filter's) on caller data: a and b
  * hold 16 int8 per lane (64 lanes), d gets the 16 int32 accumulators of
  * each lane (started at zero): d[lane*16 + r]. */

@@ -120,6 +120,8 @@ _SIGS = {
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),
     "rnng_debug_mfma_i8": (C.c_int, [_vp, _vp, _vp]),
     "rnng_set_int8_filter": (C.c_int, [C.c_int32]),
+    "rnng_set_filter_kernels": (C.c_int, [C.c_int32, C.c_int32]),
+    "rnng_get_filter_kernels": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_debug_copy": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
 }

@@ -2239,6 +2239,24 @@ int rnng_set_int8_filter(int32_t enable) {
     return RNNG_OK;
 }
 
+int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant) {
+    if (int8_code > 4 || bf16_variant >= kNumVariants) {
+        set_error("rnng_set_filter_kernels: int8 code 0..4, BF16 variant 0..14 (negative: unchanged)");
+        return RNNG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_ws_mu);   // not in the middle of a build
+    if (int8_code >= 0) g_i8_group = int8_code;
+    if (bf16_variant >= 0) g_variant = bf16_variant;
+    return RNNG_OK;
+}
+
+int rnng_get_filter_kernels(int32_t* int8_code, int32_t* bf16_variant) {
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    if (int8_code) *int8_code = i8_group();
+    if (bf16_variant) *bf16_variant = filter_variant();
+    return RNNG_OK;
+}
+
 int rnng_dataset_create_f32(const float* points, int64_t n, int32_t d, rnng_dataset_t* out) {
     if (!out || (n > 0 && !points) || n < 0 || d <= 0) {
         set_error("rnng_dataset_create_f32: bad arguments");

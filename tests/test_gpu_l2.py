@@ -245,6 +245,53 @@ def test_int8_filter_bit_exact(orc, d, lo, hi):
         _lib.check(_lib.lib().rnng_set_int8_filter(1))
 
 
+def test_every_filter_kernel_shape(orc):
+    """Every selectable filter kernel (int8 codes 0-4, BF16 variants 0-14, set
+    through rnng_set_filter_kernels) gives the oracle's edges bit for bit, on
+    integer data (the int8 path) and float data (the BF16 path), for self joins
+    and a bipartite join, at d = 17 (scalar loads, partial k-steps) and 128."""
+    from rnng_amd import Stats, _lib
+    lib = _lib.lib()
+    code0, var0 = C.c_int32(), C.c_int32()
+    _lib.check(lib.rnng_get_filter_kernels(C.byref(code0), C.byref(var0)))
+    rng = np.random.default_rng(77)
+
+    def case(X):
+        D2 = ((X[:40, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(2)
+        eps = F(np.sqrt(np.quantile(D2[D2 > 0], 0.003)))
+        Y = X[:1100] + 0
+        return X, Y, eps, orc.l2_eps_pairs(X, eps), orc.l2_eps_pairs(X, eps, Y)
+
+    ints, floats = [], []
+    for d in (17, 128):
+        Xi = rng.integers(0, 200, size=(2600, d)).astype(F)
+        Xi[5] = Xi[9]
+        ints.append(case(Xi))
+        floats.append(case(rng.standard_normal((2600, d)).astype(F)))
+
+    def check(cases, want_int8):
+        for X, Y, eps, (ij_ref, w_ref), (ijb_ref, wb_ref) in cases:
+            st = Stats()
+            ij, w = _l2().eps_pairs(X, eps, "mfma", stats=st)
+            _check_same(ij, w, ij_ref, w_ref)
+            assert st.int8_filter == want_int8
+            ijb, wb = _l2().eps_bipartite(X, Y, eps, "mfma")
+            _check_same(ijb, wb, ijb_ref, wb_ref)
+
+    try:
+        for code in range(5):
+            flush_print(f"int8 filter kernel code {code}")
+            _lib.check(lib.rnng_set_filter_kernels(code, -1))
+            check(ints, 1)
+        for var in range(15):
+            flush_print(f"BF16 filter variant {var}")
+            _lib.check(lib.rnng_set_filter_kernels(-1, var))
+            check(floats, 0)
+        assert lib.rnng_set_filter_kernels(5, -1) != 0 and lib.rnng_set_filter_kernels(-1, 15) != 0
+    finally:
+        _lib.check(lib.rnng_set_filter_kernels(code0.value, var0.value))
+
+
 def test_int8_filter_eps_extremes(orc):
     from rnng_amd import Stats
     rng = np.random.default_rng(3)
