@@ -307,8 +307,8 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 // One workgroup per 64 points: the points' rows are staged in LDS with
-// contiguous loads, each point's norm and conversion error is reduced by four
-// lanes, then the x~ and/or y~ fragments (either output may be null) are
+// contiguous loads, each point's norm and conversion error reduced over the
+// wave on the way, then the x~ and/or y~ fragments (either output may be null) are
 // written 16 bytes per lane, contiguous within each 1 KB fragment, and the
 // norms (FP32, 2^100 for padding) to xn and/or yn.
 // idmap (may be null): padded position -> point index (-1 = padding); without
@@ -332,46 +332,39 @@ __global__ __launch_bounds__(256) void k_prep_frags(
         preal[tid] = src;
     }
     __syncthreads();
+    // rows to LDS as x~, with each row's norm and conversion error summed over
+    // the wave (in double) on the way: X is read once
+    float e_max = 0.0f, n_max = 0.0f;
     for (int r = wave; r < PP; r += 4) {
         const int src = preal[r];
+        double nrm = 0.0, err = 0.0;
         for (int k = lane; k < d; k += 64) {
             float v = 0.0f;
             if (src >= 0) {
                 const double exact = (double)X[(size_t)src * d + k] - (double)mu[k];
                 v = bf16_to_f32(f32_to_bf16_rn((float)exact));
+                nrm += (double)v * (double)v;
+                const double de = (double)v - exact;
+                err += de * de;
             }
             xs[r * ld + k] = v;
         }
-    }
-    __syncthreads();
-    // norm and conversion error: 4 lanes per point
-    float e_max = 0.0f, n_max = 0.0f;
-    {
-        const int r = tid >> 2, part = tid & 3;
-        const int src = preal[r];
-        double nrm = 0.0, err = 0.0;
-        if (src >= 0) {
-            for (int k = part; k < d; k += 4) {
-                const double xb = (double)xs[r * ld + k];
-                nrm += xb * xb;
-                const double de = xb - ((double)X[(size_t)src * d + k] - (double)mu[k]);
-                err += de * de;
-            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            nrm += __shfl_xor(nrm, o, 64);
+            err += __shfl_xor(err, o, 64);
         }
-        nrm += __shfl_xor(nrm, 1, 64);
-        nrm += __shfl_xor(nrm, 2, 64);
-        err += __shfl_xor(err, 1, 64);
-        err += __shfl_xor(err, 2, 64);
-        if (part == 0 && p0 + r < n_pad) {
+        if (lane == 0 && p0 + r < n_pad) {
             const float nv = src >= 0 ? (float)nrm : kPadNorm;
             if (xn) xn[p0 + r] = nv;
             if (yn) yn[p0 + r] = nv;
             if (src >= 0) {
-                e_max = (float)sqrt(err) * (1.0f + 1e-6f);
-                n_max = (float)nrm * (1.0f + 1e-6f);
+                e_max = fmaxf(e_max, (float)sqrt(err) * (1.0f + 1e-6f));
+                n_max = fmaxf(n_max, (float)nrm * (1.0f + 1e-6f));
             }
         }
     }
+    __syncthreads();
     const int nchunk = (PP / 32) * KS * 64;
     for (int c = tid; c < nchunk; c += 256) {
         const int tt = c / (KS * 64), rem = c - tt * (KS * 64);
