@@ -1056,7 +1056,7 @@ __device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)
 // stay out of the hot loop.
 template <class T>
 __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], typename T::term nyc, int ct, int rt0,
-                                        int lane, int n_rows, int n_cols, int self_join, int2* stage,
+                                        int lane, int n_rows, int n_cols, int self_join, int2* stage, unsigned stage_cap,
                                         unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
                                         unsigned long long* __restrict__ counter) {
     const int rowb = rt0 * 32 + 4 * (lane >> 5);
@@ -1089,7 +1089,7 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], type
         slot = __shfl(slot, 0, 64) +
                __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         if (p) {
-            if (slot < LSTAGE) {
+            if (slot < stage_cap) {
                 stage[slot] = make_int2(row, col);
             } else {
                 const unsigned long long g = atomicAdd(counter, 1ull);
@@ -1150,7 +1150,8 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
         i32x16_t acc[MRT];
         mf_tile<MfI8, KS8>(bs[buf], a, xr, lane, acc);
         if (__any(mf_min<MfI8>(acc) < -yc))
-            mf_emit<MfI8>(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap, counter);
+            mf_emit<MfI8>(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, LSTAGE, &scount, cand, cap,
+                          counter);
         if (has_next) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -1249,7 +1250,10 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     static_assert(G >= 1 && G <= 4 && (S == 2 || S == 3), "ring shape");
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
-    __shared__ int2 stage[LSTAGE];
+    // candidates staged in LDS; the int8 shape with four tiles per stage keeps
+    // 512 so that four workgroups fit a CU (the rest go straight to memory)
+    constexpr unsigned STG = (std::is_same<T, MfI8>::value && G == 4) ? 512u : (unsigned)LSTAGE;
+    __shared__ int2 stage[STG];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
     const int4 wi = work[blockIdx.x];
@@ -1291,8 +1295,8 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
             accv acc[MRT];
             mf_tile<T, KS>(cur + cc * NCH, a, xr, lane, acc);
             if (__any(mf_min<T>(acc) < nyc))
-                mf_emit<T>(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, &scount, cand, cap,
-                           counter);
+                mf_emit<T>(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand,
+                           cap, counter);
         }
         ring_barrier(S == 3 && issue ? nld : 0);
     };
@@ -1315,7 +1319,7 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
         }
     }
     __syncthreads();   // every wave's staged candidates and count
-    const unsigned nst = min(scount, (unsigned)LSTAGE);
+    const unsigned nst = min(scount, STG);
     if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
     __syncthreads();
     for (unsigned i = tid; i < nst; i += 256)
@@ -1678,7 +1682,7 @@ void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<nb, 256, 0, stream()>>>(xa, yb, xn, ycn, wk, nx, ny,
                                                                                   self_join, cand, ws.cand_cap, cnt);
     else
-        k_l2_mfma_filter_ring<MfI8, KS8, CODE, (KS8 <= 4 && CODE == 3 ? 4 : 1), 2><<<nb, 256, 0, stream()>>>(
+        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<nb, 256, 0, stream()>>>(
             xa, yb, xn, ycn, wk, nx, ny, self_join, cand, ws.cand_cap, cnt);
 }
 
