@@ -843,11 +843,19 @@ __host__ __device__ constexpr int recheck_ld(int d) { return ((d + 3) & ~3) + 4;
 // a candidate iff it ends below -c_y.  Every candidate is rechecked in the
 // reference's FP32 order.
 
-constexpr int I8PAD = 1 << 28;   // start term of padding rows and columns
+// Start terms of padding positions.  Real rows end at |acc| <= |r_x| + |A.B|
+// <= 1.2e6 + 2.4e6 (d <= 144) and -c_y <= Tq / 2 + 1 <= 2^29 + 1, so a padding
+// row (acc >= 2^30 - 2.4e6) or a padding column (-c_y = -2^28) is never a
+// candidate, whatever eps: padding positions inside the cell path's padded
+// order are not masked by the row/column limits, and 2^30 + A.B stays below
+// 2^31.
+constexpr int I8PAD_ROW = 1 << 30;
+constexpr int I8PAD_COL = 1 << 28;
 
 __device__ __forceinline__ int floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
-// One workgroup per 32-point tile: wave w writes k-steps s = w, w + 4, ...
+// One workgroup per 32-point tile (padded position a holds point idmap[a],
+// or a itself without a map): wave w writes k-steps s = w, w + 4, ...
 // (32 dims each) of the tile's int8 fragments; lane l = 32 h + (a & 31) holds
 // dims 32 s + 16 h .. +15 of point a, read as 64 contiguous bytes (float4
 // loads when rows are 16-byte aligned) and stored as one 16-byte word at
@@ -856,7 +864,8 @@ __device__ __forceinline__ int floor_half(int v) { return v >= 0 ? v / 2 : -((1 
 // Optional: hsq receives the exact |x'|^2 of every point and xrow its x' as
 // point-major int8 rows of KS8 * 32 bytes (for the exact recheck).
 __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, int n, int npad, int d, int KS8,
-                                                 int vec4, const unsigned* __restrict__ colmin, int Tq,
+                                                 const int* __restrict__ idmap, int vec4,
+                                                 const unsigned* __restrict__ colmin, int Tq,
                                                  i32x4_t* __restrict__ xa, i32x4_t* __restrict__ yb,
                                                  int* __restrict__ rterm, int* __restrict__ cterm,
                                                  int* __restrict__ hsq, i32x4_t* __restrict__ xrow) {
@@ -866,7 +875,9 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int a = tile * 32 + (lane & 31);
     const int half = lane >> 5;
-    const bool real = a < n;
+    // padded position a holds point src (idmap: -1 = padding)
+    const int src = idmap ? idmap[a] : (a < n ? a : -1);
+    const bool real = src >= 0;
     for (int k = tid; k < d; k += 256) off[k] = (int)ord2f(colmin[k]) + 128;
     __syncthreads();
     int h = 0, sx = 0;
@@ -874,10 +885,10 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
         const int k0 = 32 * s + 16 * half;
         float v[16];
         if (real && vec4 && k0 + 16 <= d) {
-            const float4* src = reinterpret_cast<const float4*>(X + (size_t)a * d + k0);
+            const float4* row = reinterpret_cast<const float4*>(X + (size_t)src * d + k0);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float4 f = src[q];
+                const float4 f = row[q];
                 v[4 * q] = f.x;
                 v[4 * q + 1] = f.y;
                 v[4 * q + 2] = f.z;
@@ -885,7 +896,7 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) v[j] = (real && k0 + j < d) ? X[(size_t)a * d + k0 + j] : 0.0f;
+            for (int j = 0; j < 16; ++j) v[j] = (real && k0 + j < d) ? X[(size_t)src * d + k0 + j] : 0.0f;
         }
         unsigned wa[4] = {0u, 0u, 0u, 0u}, wb[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -910,15 +921,15 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
     sh[wave][lane] = h;
     ssx[wave][lane] = sx;
     __syncthreads();
-    if (tid < 32 && a < npad) {
+    if (tid < 32 && a < npad) {   // here a = tile * 32 + tid
         int ht = 0, st = 0;
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
             ht += sh[w][tid] + sh[w][tid + 32];
             st += ssx[w][tid] + ssx[w][tid + 32];
         }
-        if (rterm) rterm[a] = real ? st + (ht >> 1) - 1 : I8PAD;
-        if (cterm) cterm[a] = real ? floor_half(ht - Tq) : I8PAD;
+        if (rterm) rterm[a] = real ? st + (ht >> 1) - 1 : I8PAD_ROW;
+        if (cterm) cterm[a] = real ? floor_half(ht - Tq) : I8PAD_COL;
         if (hsq) hsq[a] = real ? ht : 0;
     }
 }
@@ -1665,25 +1676,17 @@ float ord2f_host(unsigned u) {
 // through registers); 1, 2 = the direct-to-LDS ring with 1 or 2 tiles per
 // stage and three stages; 3, 4 = 3 or 4 tiles per stage and two stages.
 template <int KS8, int CODE>
-void launch_filter_i8(Workspace& ws, int nx, int ny, int self_join) {
+void launch_filter_i8(const I8Args& a) {
     constexpr int MINW = KS8 <= 4 ? 4 : 1;
-    const unsigned nb = (unsigned)ws.work_host.size();
-    const i32x4_t* xa = ws.xa.as<i32x4_t>();
-    const i32x4_t* yb = ws.yb.as<i32x4_t>();
-    const int* xn = ws.xn.as<int>();
-    const int* ycn = ws.ycn.as<int>();
-    const int4* wk = ws.work.as<int4>();
-    int2* cand = ws.cand.as<int2>();
-    unsigned long long* cnt = ws.counters.as<unsigned long long>();
     if constexpr (CODE == 0)
-        k_l2_mfma_filter_i8<KS8><<<nb, 256, 0, stream()>>>(xa, yb, xn, ycn, wk, nx, ny, self_join, cand, ws.cand_cap,
-                                                           cnt);
+        k_l2_mfma_filter_i8<KS8><<<a.n_items, 256, 0, stream()>>>(a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows,
+                                                                  a.n_cols, a.self_join, a.cand, a.cap, a.counter);
     else if constexpr (CODE <= 2)
-        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<nb, 256, 0, stream()>>>(xa, yb, xn, ycn, wk, nx, ny,
-                                                                                  self_join, cand, ws.cand_cap, cnt);
+        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<a.n_items, 256, 0, stream()>>>(
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
     else
-        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<nb, 256, 0, stream()>>>(
-            xa, yb, xn, ycn, wk, nx, ny, self_join, cand, ws.cand_cap, cnt);
+        k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<a.n_items, 256, 0, stream()>>>(
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
 }
 
 int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
@@ -1699,34 +1702,53 @@ int i8_group() {
     return g_i8_group;
 }
 
-int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8, int code) {
-    if (ws.work_host.empty()) return RNNG_OK;
+I8Args i8_args(const Workspace& ws, int nx, int ny, int self_join) {
+    I8Args a;
+    a.xa = ws.xa.as<i32x4_t>();
+    a.yb = ws.yb.as<i32x4_t>();
+    a.rterm = ws.xn.as<int>();
+    a.cterm = ws.ycn.as<int>();
+    a.work = ws.work.as<int4>();
+    a.n_items = (unsigned)ws.work_host.size();
+    a.n_rows = nx;
+    a.n_cols = ny;
+    a.self_join = self_join;
+    a.cand = ws.cand.as<int2>();
+    a.cap = ws.cand_cap;
+    a.counter = ws.counters.as<unsigned long long>();
+    return a;
+}
+
+}  // namespace
+
+int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
+    if (a.n_items == 0) return RNNG_OK;
     switch (KS8 * 8 + code) {
-        case 1 * 8 + 0: launch_filter_i8<1, 0>(ws, nx, ny, self_join); break;
-        case 2 * 8 + 0: launch_filter_i8<2, 0>(ws, nx, ny, self_join); break;
-        case 3 * 8 + 0: launch_filter_i8<3, 0>(ws, nx, ny, self_join); break;
-        case 4 * 8 + 0: launch_filter_i8<4, 0>(ws, nx, ny, self_join); break;
-        case 5 * 8 + 0: launch_filter_i8<5, 0>(ws, nx, ny, self_join); break;
-        case 1 * 8 + 1: launch_filter_i8<1, 1>(ws, nx, ny, self_join); break;
-        case 2 * 8 + 1: launch_filter_i8<2, 1>(ws, nx, ny, self_join); break;
-        case 3 * 8 + 1: launch_filter_i8<3, 1>(ws, nx, ny, self_join); break;
-        case 4 * 8 + 1: launch_filter_i8<4, 1>(ws, nx, ny, self_join); break;
-        case 5 * 8 + 1: launch_filter_i8<5, 1>(ws, nx, ny, self_join); break;
-        case 1 * 8 + 2: launch_filter_i8<1, 2>(ws, nx, ny, self_join); break;
-        case 2 * 8 + 2: launch_filter_i8<2, 2>(ws, nx, ny, self_join); break;
-        case 3 * 8 + 2: launch_filter_i8<3, 2>(ws, nx, ny, self_join); break;
-        case 4 * 8 + 2: launch_filter_i8<4, 2>(ws, nx, ny, self_join); break;
-        case 5 * 8 + 2: launch_filter_i8<5, 2>(ws, nx, ny, self_join); break;
-        case 1 * 8 + 3: launch_filter_i8<1, 3>(ws, nx, ny, self_join); break;
-        case 2 * 8 + 3: launch_filter_i8<2, 3>(ws, nx, ny, self_join); break;
-        case 3 * 8 + 3: launch_filter_i8<3, 3>(ws, nx, ny, self_join); break;
-        case 4 * 8 + 3: launch_filter_i8<4, 3>(ws, nx, ny, self_join); break;
-        case 5 * 8 + 3: launch_filter_i8<5, 3>(ws, nx, ny, self_join); break;
-        case 1 * 8 + 4: launch_filter_i8<1, 4>(ws, nx, ny, self_join); break;
-        case 2 * 8 + 4: launch_filter_i8<2, 4>(ws, nx, ny, self_join); break;
-        case 3 * 8 + 4: launch_filter_i8<3, 4>(ws, nx, ny, self_join); break;
-        case 4 * 8 + 4: launch_filter_i8<4, 4>(ws, nx, ny, self_join); break;
-        case 5 * 8 + 4: launch_filter_i8<5, 4>(ws, nx, ny, self_join); break;
+        case 1 * 8 + 0: launch_filter_i8<1, 0>(a); break;
+        case 2 * 8 + 0: launch_filter_i8<2, 0>(a); break;
+        case 3 * 8 + 0: launch_filter_i8<3, 0>(a); break;
+        case 4 * 8 + 0: launch_filter_i8<4, 0>(a); break;
+        case 5 * 8 + 0: launch_filter_i8<5, 0>(a); break;
+        case 1 * 8 + 1: launch_filter_i8<1, 1>(a); break;
+        case 2 * 8 + 1: launch_filter_i8<2, 1>(a); break;
+        case 3 * 8 + 1: launch_filter_i8<3, 1>(a); break;
+        case 4 * 8 + 1: launch_filter_i8<4, 1>(a); break;
+        case 5 * 8 + 1: launch_filter_i8<5, 1>(a); break;
+        case 1 * 8 + 2: launch_filter_i8<1, 2>(a); break;
+        case 2 * 8 + 2: launch_filter_i8<2, 2>(a); break;
+        case 3 * 8 + 2: launch_filter_i8<3, 2>(a); break;
+        case 4 * 8 + 2: launch_filter_i8<4, 2>(a); break;
+        case 5 * 8 + 2: launch_filter_i8<5, 2>(a); break;
+        case 1 * 8 + 3: launch_filter_i8<1, 3>(a); break;
+        case 2 * 8 + 3: launch_filter_i8<2, 3>(a); break;
+        case 3 * 8 + 3: launch_filter_i8<3, 3>(a); break;
+        case 4 * 8 + 3: launch_filter_i8<4, 3>(a); break;
+        case 5 * 8 + 3: launch_filter_i8<5, 3>(a); break;
+        case 1 * 8 + 4: launch_filter_i8<1, 4>(a); break;
+        case 2 * 8 + 4: launch_filter_i8<2, 4>(a); break;
+        case 3 * 8 + 4: launch_filter_i8<3, 4>(a); break;
+        case 4 * 8 + 4: launch_filter_i8<4, 4>(a); break;
+        case 5 * 8 + 4: launch_filter_i8<5, 4>(a); break;
         default:
             set_error("int8 filter supports d <= 160");
             return RNNG_ERR_UNSUPPORTED;
@@ -1734,6 +1756,10 @@ int filter_dispatch_i8(Workspace& ws, int nx, int ny, int self_join, int KS8, in
     RNNG_HIP_TRY(hipGetLastError());
     return RNNG_OK;
 }
+
+int filter_i8_code() { return i8_group(); }
+
+namespace {
 
 FilterArgs ws_args(Workspace& ws, int n_rows, int n_cols, int self_join) {
     FilterArgs a;
@@ -1925,14 +1951,15 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         }
         auto vec_ok = [&](const float* X) { return (d % 4 == 0 && ((uintptr_t)X % 16) == 0) ? 1 : 0; };
         if (self_join) {
-            k_prep_i8<<<npad / 32, 256, 0, stream()>>>(A->X, nx, npad, d, KS8, vec_ok(A->X), ws.colmin.as<unsigned>(),
+            k_prep_i8<<<npad / 32, 256, 0, stream()>>>(A->X, nx, npad, d, KS8, nullptr, vec_ok(A->X),
+                                                       ws.colmin.as<unsigned>(),
                                                        Tq, ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(),
                                                        ws.ycn.as<int>(), ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
         } else {
-            k_prep_i8<<<npad_x / 32, 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8, vec_ok(A->X),
+            k_prep_i8<<<npad_x / 32, 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8, nullptr, vec_ok(A->X),
                                                          ws.colmin.as<unsigned>(), Tq, ws.xa.as<i32x4_t>(), nullptr,
                                                          ws.xn.as<int>(), nullptr, ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
-            k_prep_i8<<<npad_y / 32, 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8, vec_ok(B->X),
+            k_prep_i8<<<npad_y / 32, 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8, nullptr, vec_ok(B->X),
                                                          ws.colmin.as<unsigned>(), Tq, nullptr, ws.yb.as<i32x4_t>(),
                                                          nullptr, ws.ycn.as<int>(), ws.hy.as<int>(), ws.yr8.as<i32x4_t>());
         }
@@ -1967,7 +1994,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        if (i8) RNNG_TRY(filter_dispatch_i8(ws, nx, ny, self_join ? 1 : 0, KS8, i8_group()));
+        if (i8) RNNG_TRY(filter_dispatch_i8(i8_args(ws, nx, ny, self_join ? 1 : 0), KS8, i8_group()));
         else RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
         unsigned long long ncand = 0;
@@ -2128,6 +2155,40 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                                                       P.colmax.as<unsigned>(), n, d, P.mu.as<float>(),
                                                       P.stat.as<unsigned>(), P.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
+    P.i8 = false;
+    if (g_int8_filter && n > 0) {
+        unsigned hs[NSTAT], hr = 0;
+        RNNG_HIP_TRY(hipMemcpyAsync(hs, P.stat.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(&hr, P.range.ptr, sizeof(hr), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+        const float range = ord2f_host(hr);
+        P.i8 = hs[ST_INT] != 0 && range >= 0.0f && range <= 255.0f;
+    }
+    if (P.i8) {
+        // the int8 forms, candidates for squared distances <= floor(Tb)
+        P.KS8 = (d + 31) / 32;
+        const int Tq = Tb >= 1073741823.0f ? 1073741823 : (Tb < 0.0f ? -1 : (int)std::floor(Tb));
+        const int vec4 = (d % 4 == 0 && ((uintptr_t)X % 16) == 0) ? 1 : 0;
+        if (rowmap == colmap && n_rowpad == n_colpad) {
+            if (n_rowpad > 0)
+                k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,
+                                                               P.colmin.as<unsigned>(), Tq, P.xa.as<i32x4_t>(),
+                                                               P.yb.as<i32x4_t>(), P.xn.as<int>(), P.ycn.as<int>(),
+                                                               nullptr, nullptr);
+        } else {
+            if (n_rowpad > 0)
+                k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,
+                                                               P.colmin.as<unsigned>(), Tq, P.xa.as<i32x4_t>(),
+                                                               nullptr, P.xn.as<int>(), nullptr, nullptr, nullptr);
+            if (n_colpad > 0)
+                k_prep_i8<<<n_colpad / 32, 256, 0, stream()>>>(X, n, n_colpad, d, P.KS8, colmap, vec4,
+                                                               P.colmin.as<unsigned>(), Tq, nullptr,
+                                                               P.yb.as<i32x4_t>(), nullptr, P.ycn.as<int>(), nullptr,
+                                                               nullptr);
+        }
+        RNNG_HIP_TRY(hipGetLastError());
+        return RNNG_OK;
+    }
     if (rowmap == colmap && n_rowpad == n_colpad) {
         // one pass writes both forms
         RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(),

@@ -26,6 +26,10 @@ struct MfmaPrep {
     LocalBuf colsum, colmin, colmax, stat, range, mu;  // centering and error statistics
     LocalBuf params;                                   // MfmaParams
     int KS = 0;                                      // 16-wide k-steps per fragment
+    // integer data spanning <= 255 per dimension: int8 fragments (xa, yb) and
+    // int32 start terms (xn rows, ycn columns) for the exact int8 filter
+    bool i8 = false;
+    int KS8 = 0;                                     // 32-wide int8 k-steps
 };
 
 // Number of 16-wide k-steps for dimension d.
@@ -35,7 +39,9 @@ constexpr int kMaxKS = 9;     // d <= 144
 // Builds the x-form fragments and row norms of the padded row positions
 // (rowmap[p] = point index or -1) and the y-form fragments and column start
 // values (norm - C0, C0 for the squared band radius Tb) of the padded column
-// positions (colmap), centred over all n points of X.
+// positions (colmap), centred over all n points of X.  On integer data whose
+// dimensions each span at most 255 (and unless rnng_set_int8_filter(0)) it
+// builds the int8 forms instead and sets P.i8 (threshold floor(Tb)).
 int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
                         int n_rowpad, const int* colmap, int n_colpad);
 
@@ -58,6 +64,24 @@ struct FilterArgs {
     unsigned long long* counter = nullptr;
 };
 int filter_dispatch(const FilterArgs& a, int KS, int variant);
+
+// The exact int8 filter (integer data, every dimension spanning <= 255):
+// int8 fragments, int32 start terms (rows r_x, columns c_y), candidates as for
+// FilterArgs.  code: the kernel shape (0..4, filter_i8_code() = default).
+struct I8Args {
+    const i32x4_t* xa = nullptr;
+    const i32x4_t* yb = nullptr;
+    const int* rterm = nullptr;
+    const int* cterm = nullptr;
+    const int4* work = nullptr;
+    unsigned n_items = 0;
+    int n_rows = 0, n_cols = 0, self_join = 0;
+    int2* cand = nullptr;
+    unsigned long long cap = 0;
+    unsigned long long* counter = nullptr;
+};
+int filter_dispatch_i8(const I8Args& a, int KS8, int code);
+int filter_i8_code();
 int filter_block_tiles(int variant);
 int filter_default_variant();
 int filter_num_variants();

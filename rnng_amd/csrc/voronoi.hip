@@ -979,8 +979,30 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         fa.counter = counters.as<unsigned long long>();
         return fa;
     };
+    // the filter over a work list: the exact int8 kernel when the prep built
+    // int8 forms, else the BF16 variant var
+    auto run_filter = [&](const DevBuf& work, size_t n_items, int2* cand_ptr, unsigned long long cap_,
+                          int var) -> int {
+        if (P.i8) {
+            I8Args ia;
+            ia.xa = P.xa.as<i32x4_t>();
+            ia.yb = P.yb.as<i32x4_t>();
+            ia.rterm = P.xn.as<int>();
+            ia.cterm = P.ycn.as<int>();
+            ia.work = work.as<int4>();
+            ia.n_items = (unsigned)n_items;
+            ia.n_rows = n_pad;
+            ia.n_cols = n_pad;
+            ia.self_join = 1;
+            ia.cand = cand_ptr;
+            ia.cap = cap_;
+            ia.counter = counters.as<unsigned long long>();
+            return filter_dispatch_i8(ia, P.KS8, filter_i8_code());
+        }
+        return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
+    };
     auto work_list = [&](int var, int xcd, std::vector<int4>& out) {
-        const int bt = filter_block_tiles(var);
+        const int bt = P.i8 ? 8 : filter_block_tiles(var);   // int8 kernels: 8 row tiles per block
         const int64_t cov = cell_work(rs, adj, words, m, bt, (rs[m] + bt * 32 - 1) / (bt * 32), out);
         if (xcd) xcd_permute(out);
         return cov;
@@ -989,7 +1011,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     if (v->tuned_variant < 0) {
         v->tuned_variant = filter_default_variant();
         v->tuned_xcd = 1;
-        if (v->autotune >= 0 ? v->autotune : tuning_enabled()) {
+        // the int8 filter has one default shape; the tuner times BF16 variants
+        if (!P.i8 && (v->autotune >= 0 ? v->autotune : tuning_enabled())) {
             double best = 1e30;
             int k = 0;
             for (int var : kTuneVariants) {
@@ -1003,7 +1026,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
                     for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
                         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
                         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
-                        RNNG_TRY(filter_dispatch(filter_args(v->tune_items, sub.size(), nullptr, 0ull), KS, var));
+                        RNNG_TRY(run_filter(v->tune_items, sub.size(), nullptr, 0ull, var));
                         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
                         RNNG_HIP_TRY(hipEventSynchronize(ev.e[4]));
                         t = ev.ms(3, 4);
@@ -1051,7 +1074,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     // from the device (grids are sized by c), so the pass can be captured.
     auto fill_pass = [&](unsigned long long c) -> int {
         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
-        RNNG_TRY(filter_dispatch(filter_args(d_items, items.size(), cand.as<int2>(), c), KS, variant));
+        RNNG_TRY(run_filter(d_items, items.size(), cand.as<int2>(), c, variant));
         const unsigned eg =
             (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(8192, (c + 255) / 256));
         k_cand_ids<<<eg, 256, 0, stream()>>>(cand.as<int2>(), counters.as<unsigned long long>(), c, slot.as<int>(),
@@ -1126,7 +1149,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         // count-then-fill: a counting pass (capacity 0, filter only)
         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
-        RNNG_TRY(filter_dispatch(filter_args(d_items, items.size(), nullptr, 0ull), KS, variant));
+        RNNG_TRY(run_filter(d_items, items.size(), nullptr, 0ull, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
         ++passes;
         RNNG_TRY(read_count());
@@ -1184,6 +1207,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     if (st) {
         st->pair_evals += covered;
         st->candidates += (int64_t)ncand;
+        st->int8_filter += P.i8 ? 1 : 0;
         st->edges += ne;
         st->band_pairs += (int64_t)cnt[2];
         st->band_flips += (int64_t)cnt[3];

@@ -144,15 +144,28 @@ def test_integer_landmark_graph(orc):
     eps = synth.radius_for_degree(pts, 25.0)
     centers = np.sort(rng.choice(pts.shape[0], size=50, replace=False))
     flush_print(f"landmark int 8000x128 m=50 eps={eps}")
-    _graph_case(orc, pts, centers, eps)
+    st = _graph_case(orc, pts, centers, eps)
+    assert st.int8_filter == 1   # integer data spanning <= 255: the exact int8 cell filter
+    from rnng_amd import _lib
+    _lib.check(_lib.lib().rnng_set_int8_filter(0))
+    try:
+        st = _graph_case(orc, pts, centers, eps, check_ghosts=False)
+        assert st.int8_filter == 0
+    finally:
+        _lib.check(_lib.lib().rnng_set_int8_filter(1))
 
 
-@pytest.mark.parametrize("case", ["one_cell", "singletons", "eps_zero", "dup_centers"])
+@pytest.mark.parametrize("case", ["one_cell", "singletons", "eps_zero", "dup_centers", "huge_eps"])
 def test_landmark_edge_cases(orc, case):
     rng = np.random.default_rng(5)
     pts = rng.integers(0, 4, size=(400, 6)).astype(F)
     eps = F(1.5)
-    if case == "one_cell":
+    if case == "huge_eps":
+        # every pair is an edge and the threshold sits at its clamp: padding
+        # positions between the cells must still never be candidates
+        centers = np.arange(0, 400, 37)
+        eps = F(1e6)
+    elif case == "one_cell":
         centers = np.array([7])
     elif case == "singletons":
         centers = np.arange(400)
