@@ -1099,11 +1099,18 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], type
         if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
         slot = __shfl(slot, 0, 64) +
                __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (p) {
-            if (slot < stage_cap) {
-                stage[slot] = make_int2(row, col);
-            } else {
-                const unsigned long long g = atomicAdd(counter, 1ull);
+        if (p && slot < stage_cap) stage[slot] = make_int2(row, col);
+        // past the stage (dense tiles, e.g. clusters in cell order): one global
+        // reservation per round for the wave's overflowing candidates
+        const bool ovf = p && slot >= stage_cap;
+        const unsigned long long mo = __ballot(ovf);
+        if (mo) {
+            unsigned long long gb = 0;
+            if (lane == 0) gb = atomicAdd(counter, (unsigned long long)__popcll(mo));
+            gb = __shfl(gb, 0, 64);
+            if (ovf) {
+                const unsigned long long g =
+                    gb + __builtin_amdgcn_mbcnt_hi((unsigned)(mo >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mo, 0u));
                 if (g < cap) cand[g] = make_int2(row, col);
             }
         }
