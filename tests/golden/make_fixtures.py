@@ -12,6 +12,9 @@ no oracle build requirements beyond this repository) checks against data.
   simplex_ids.json     combinatorial-number-system ids, closed form
   gmm_small.json       the 10,000 x 16 workload: eps, edge count and SHA-256
                        of the sorted (i, j) int64 pairs and float32 weights
+  gmm_int_small.json   the integer-valued 1M x 128 generator at 20,000 points
+                       (coordinates in [0, 255]: the exact int8 filter path):
+                       eps, edge count and SHA-256 of the sorted pairs and weights
   edit_small.json      600 DNA-like strings (the 200k-string generator at 6
                        seeds x 100 mutants), eps 8: edge count and SHA-256 of
                        the sorted pairs and distances
@@ -82,6 +85,15 @@ def main():
           "sha256_w_float32": hashlib.sha256(np.ascontiguousarray(w, np.float32).tobytes()).hexdigest(),
           "sha256_points_float32": hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest()}
     json.dump(gm, open(os.path.join(HERE, "gmm_small.json"), "w"), indent=1)
+
+    pts, eps = synth.points_gmm_int(n=20000)
+    ij, w = oracle.l2_eps_pairs(pts, eps)
+    gi = {"n": int(pts.shape[0]), "d": int(pts.shape[1]), "eps_bits": int(np.float32(eps).view(np.uint32)),
+          "edges": int(ij.shape[0]),
+          "sha256_ij_int64": hashlib.sha256(np.ascontiguousarray(ij, np.int64).tobytes()).hexdigest(),
+          "sha256_w_float32": hashlib.sha256(np.ascontiguousarray(w, np.float32).tobytes()).hexdigest(),
+          "sha256_points_float32": hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest()}
+    json.dump(gi, open(os.path.join(HERE, "gmm_int_small.json"), "w"), indent=1)
 
     atoms, offs = synth.dna_mutants(n_seeds=6, per_seed=100)
     ij, dist = oracle.edit_eps_pairs(atoms, offs, 8)

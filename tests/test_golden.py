@@ -90,6 +90,51 @@ def test_gmm_small_device_paths_match_fixture(path):
     assert _sha(ij, np.int64) == g["sha256_ij_int64"] and _sha(w, np.float32) == g["sha256_w_float32"]
 
 
+def test_gmm_int_small_generator_and_oracle(orc):
+    from rnng_amd import synth
+    g = _load("gmm_int_small.json")
+    pts, eps = synth.points_gmm_int(n=g["n"])
+    assert _sha(pts, np.float32) == g["sha256_points_float32"]
+    assert int(np.float32(eps).view(np.uint32)) == g["eps_bits"]
+    ij, w = orc.l2_eps_pairs(pts, eps)
+    assert ij.shape[0] == g["edges"]
+    assert _sha(ij, np.int64) == g["sha256_ij_int64"] and _sha(w, np.float32) == g["sha256_w_float32"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["plain", "mfma", "mfma_bf16", "cells"])
+def test_gmm_int_small_device_paths_match_fixture(path):
+    """Integer-valued data: the MFMA and cell paths take the exact int8 filter
+    (and the exact integer recheck); with the int8 filter switched off they
+    take BF16; every path gives the frozen edge set."""
+    from rnng_amd import Stats, _lib, l2, synth, voronoi
+    g = _load("gmm_int_small.json")
+    pts, _ = synth.points_gmm_int(n=g["n"])
+    eps = np.array([g["eps_bits"]], np.uint32).view(np.float32)[0]
+    flush_print(f"golden gmm_int_small via {path}")
+    st = Stats()
+    if path == "cells":
+        ds = l2.Dataset(pts)
+        v = voronoi.Voronoi(ds, fps=64, first=0)
+        gr = v.graph(eps)
+        ij, w = gr.edges(sorted=True)
+        st = gr.stats
+        gr.close()
+        v.close()
+        ds.close()
+    elif path == "mfma_bf16":
+        _lib.check(_lib.lib().rnng_set_int8_filter(0))
+        try:
+            ij, w = l2.eps_pairs(pts, eps, "mfma", stats=st)
+        finally:
+            _lib.check(_lib.lib().rnng_set_int8_filter(1))
+    else:
+        ij, w = l2.eps_pairs(pts, eps, path, stats=st)
+    assert st.int8_filter == (1 if path in ("mfma", "cells") else 0)
+    assert ij.shape[0] == g["edges"]
+    assert _sha(ij, np.int64) == g["sha256_ij_int64"] and _sha(w, np.float32) == g["sha256_w_float32"]
+
+
 @pytest.mark.gpu
 def test_l2_known_pairs_device():
     from rnng_amd import l2
