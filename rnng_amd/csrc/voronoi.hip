@@ -533,36 +533,69 @@ __global__ void k_ref_semantics(const float* __restrict__ X, const float* __rest
                                 const int* __restrict__ cell, const float* __restrict__ delta, float eps,
                                 int2* __restrict__ ids, uint8_t* __restrict__ flag,
                                 unsigned long long cap, unsigned long long* __restrict__ counters,
-                                uint64_t* __restrict__ gkeys) {
+                                uint64_t* __restrict__ gkeys, int vec4) {
 #pragma clang fp contract(off)
     const unsigned long long nc = min(counters[0], cap);
     const float two_eps = 2.0f * eps;
-    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (unsigned long long)gridDim.x * blockDim.x) {
-        if (!flag[c]) continue;
-        const int2 e = ids[c];
-        ids[c] = make_int2(min(e.x, e.y), max(e.x, e.y));
-        const int a = cell[e.x], b = cell[e.y];
-        if (a == b) continue;
-        bool g[2];
+    const int lane = threadIdx.x & 63;
+    // rounds are uniform over the wave (its 64 candidates), so the counters and
+    // ghost slots are reserved once per wave (ballot + mbcnt)
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long c = base + threadIdx.x;
+        bool tested = false, g[2] = {false, false};
+        int cab[2] = {0, 0}, qv[2] = {0, 0};
+        if (c < nc && flag[c]) {
+            const int2 e = ids[c];
+            ids[c] = make_int2(min(e.x, e.y), max(e.x, e.y));
+            const int a = cell[e.x], b = cell[e.y];
+            if (a != b) {
+                tested = true;
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int cc = s ? b : a, q = s ? e.x : e.y;
-            const float* cp = C + (size_t)cc * d;
-            const float* qp = X + (size_t)q * d;
-            float acc = 0.0f;
-            for (int k = 0; k < d; ++k) {
-                const float t = cp[k] - qp[k];
-                acc = acc + t * t;
-            }
-            g[s] = sqrtf(acc) <= delta[q] + two_eps;
-            if (g[s]) {
-                const unsigned long long o = atomicAdd(&counters[5], 1ull);
-                gkeys[o] = ((uint64_t)(uint32_t)cc << 32) | (uint32_t)q;
+                for (int s = 0; s < 2; ++s) {
+                    const int cc = s ? b : a, q = s ? e.x : e.y;
+                    const float* cp = C + (size_t)cc * d;
+                    const float* qp = X + (size_t)q * d;
+                    float acc = 0.0f;   // the reference's order: sequential, not fused
+                    if (vec4) {
+                        for (int k = 0; k < d; k += 4) {
+                            const float4 u = *reinterpret_cast<const float4*>(cp + k);
+                            const float4 v = *reinterpret_cast<const float4*>(qp + k);
+                            float t = u.x - v.x;
+                            acc = acc + t * t;
+                            t = u.y - v.y;
+                            acc = acc + t * t;
+                            t = u.z - v.z;
+                            acc = acc + t * t;
+                            t = u.w - v.w;
+                            acc = acc + t * t;
+                        }
+                    } else {
+                        for (int k = 0; k < d; ++k) {
+                            const float t = cp[k] - qp[k];
+                            acc = acc + t * t;
+                        }
+                    }
+                    g[s] = sqrtf(acc) <= delta[q] + two_eps;
+                    cab[s] = cc;
+                    qv[s] = q;
+                }
+                if (!g[0] && !g[1]) flag[c] = 0;
             }
         }
-        atomicAdd(&counters[4], 2ull);
-        if (!g[0] && !g[1]) flag[c] = 0;
+        const unsigned long long mt = __ballot(tested);
+        if (mt && lane == 0) atomicAdd(&counters[4], 2ull * (unsigned long long)__popcll(mt));
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const unsigned long long mg = __ballot(g[s]);
+            if (mg) {
+                unsigned long long o = 0;
+                if (lane == 0) o = atomicAdd(&counters[5], (unsigned long long)__popcll(mg));
+                o = __shfl(o, 0, 64) +
+                    __builtin_amdgcn_mbcnt_hi((unsigned)(mg >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mg, 0u));
+                if (g[s]) gkeys[o] = ((uint64_t)(uint32_t)cab[s] << 32) | (uint32_t)qv[s];
+            }
+        }
     }
 }
 
@@ -666,11 +699,26 @@ __global__ void k_cell_counts(const int2* __restrict__ cand, const uint8_t* __re
                               const int* __restrict__ slot, const int* __restrict__ cell,
                               unsigned* __restrict__ cc) {
     const unsigned long long nc = min(counters[0], cap);
-    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (unsigned long long)gridDim.x * blockDim.x) {
-        const int a = cell[slot[cand[c].x]];
-        atomicAdd(&cc[2 * a], 1u);
-        if (flag[c]) atomicAdd(&cc[2 * a + 1], 1u);
+    const int lane = threadIdx.x & 63;
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long c = base + threadIdx.x;
+        const bool live = c < nc;
+        const int a = live ? cell[slot[cand[c].x]] : -1;
+        const bool f = live && flag[c];
+        // candidates come grouped by row block, so a wave's rows usually share
+        // one cell: one pair of atomics for the wave then
+        const int a0 = __builtin_amdgcn_readfirstlane(a);
+        if (__all(!live || a == a0)) {
+            const unsigned long long ml = __ballot(live), mf = __ballot(f);
+            if (lane == 0 && ml) {   // a wave past the end has no live lane (a0 = -1)
+                atomicAdd(&cc[2 * a0], (unsigned)__popcll(ml));
+                if (mf) atomicAdd(&cc[2 * a0 + 1], (unsigned)__popcll(mf));
+            }
+        } else if (live) {
+            atomicAdd(&cc[2 * a], 1u);
+            if (f) atomicAdd(&cc[2 * a + 1], 1u);
+        }
     }
 }
 
@@ -1082,9 +1130,11 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(launch_recheck(v->ds->X, v->ds->X, d, cid.as<int2>(), c, c, T, eps, cflag.as<uint8_t>(),
                                 cw.as<float>(), counters.as<unsigned long long>()));
+        const int rs_vec4 = (d % 4 == 0 && ((uintptr_t)v->ds->X % 16) == 0 && ((uintptr_t)v->C.ptr % 16) == 0) ? 1 : 0;
         k_ref_semantics<<<eg, 256, 0, stream()>>>(v->ds->X, v->C.as<float>(), d, v->cell.as<int>(),
                                                   v->delta.as<float>(), eps, cid.as<int2>(), cflag.as<uint8_t>(), c,
-                                                  counters.as<unsigned long long>(), v->gkeys.as<uint64_t>());
+                                                  counters.as<unsigned long long>(), v->gkeys.as<uint64_t>(),
+                                                  rs_vec4);
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(launch_fill_u32(v->counts.as<unsigned>(), 2 * m, 0u));
         k_cell_counts<<<eg, 256, 0, stream()>>>(cand.as<int2>(), cflag.as<uint8_t>(),
