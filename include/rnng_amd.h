@@ -65,6 +65,8 @@ typedef struct rnng_stats {
     int64_t predicted_candidates; /* Voronoi: cost model's candidate estimate (0 without) */
     int64_t predicted_edges;      /* Voronoi: cost model's edge estimate (0 without) */
     int64_t int8_filter;          /* MFMA builds that ran the exact int8 filter (integer data) */
+    int64_t plain_fallbacks;      /* MFMA builds run by the plain kernel: coordinates no filter can
+                                     bound (NaN, infinities, d * range^2 > 1e36) */
 } rnng_stats;
 
 /* Opaque handles. */

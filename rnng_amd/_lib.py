@@ -53,6 +53,7 @@ class Stats(C.Structure):
         ("predicted_candidates", C.c_int64),
         ("predicted_edges", C.c_int64),
         ("int8_filter", C.c_int64),
+        ("plain_fallbacks", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_l2_plain(
 
 constexpr int NSTAT = 8;
 // stat slots (unsigned, ordered-float or flags) in ds_stat[]
-enum { ST_INT = 0, ST_MAXE = 1, ST_MAXN = 2, ST_MAXRES = 3, ST_MAXDEV = 4 };
+enum { ST_INT = 0, ST_MAXE = 1, ST_MAXN = 2, ST_MAXRES = 3, ST_MAXDEV = 4, ST_NONFIN = 5 };
 
 
 // Per-dimension sums, minima and maxima, and the all-integer flag.  A block
@@ -283,7 +283,9 @@ __global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ X, i
 
 // mu_k = per-dimension mean; rounded to an integer when every value is one so
 // that x - mu stays exact.  Also records the largest |x - mu| and the widest
-// per-dimension range (both as ordered floats).
+// per-dimension range (both as ordered floats), and flags a column whose sum is
+// not finite (it holds a NaN or an infinity: the double sum of floats cannot
+// overflow).
 __global__ void k_center(const double* __restrict__ colsum, const unsigned* __restrict__ colmin,
                          const unsigned* __restrict__ colmax, int n, int d,
                          float* __restrict__ mu, unsigned* __restrict__ stat,
@@ -292,6 +294,7 @@ __global__ void k_center(const double* __restrict__ colsum, const unsigned* __re
     if (k >= d) return;
     const bool isint = stat[ST_INT] != 0;
     double m = n > 0 ? colsum[k] / (double)n : 0.0;
+    if (!isfinite(m)) atomicOr(&stat[ST_NONFIN], 1u);
     float muk = isint ? (float)rint(m) : (float)m;
     mu[k] = muk;
     const float mn = ord2f(colmin[k]), mx = ord2f(colmax[k]);
@@ -1798,6 +1801,13 @@ int launch_recheck_i8(const Workspace& ws, int KS8, bool self_join, unsigned lon
     return RNNG_OK;
 }
 
+// The MFMA filters bound |x - y|^2 from centered BF16 or int8 forms; that
+// needs finite coordinates, and FP32 sums of d squared differences of at most
+// the widest per-dimension range that stay far from overflow.
+bool mfma_filter_applies(const unsigned* hs, float range, int d) {
+    return hs[ST_NONFIN] == 0 && range >= 0.0f && (double)d * range * range <= 1e36;
+}
+
 int run_plain(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
               rnng_graph_s* g, rnng_stats* st) {
     Workspace& ws = g_ws;
@@ -1899,15 +1909,21 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
                                                       ws.mu.as<float>(), ws.stat.as<unsigned>(),
                                                       ws.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
-    // integer coordinates spanning at most 255 per dimension: the exact int8 filter
+    // integer coordinates spanning at most 255 per dimension: the exact int8
+    // filter; coordinates no filter can bound (NaN, infinities, ranges whose
+    // squares approach FP32 overflow): the plain kernel, exact for any input
     bool i8 = false;
-    if (g_int8_filter) {
+    {
         unsigned hs[NSTAT], hr = 0;
         RNNG_HIP_TRY(hipMemcpyAsync(hs, ws.stat.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipMemcpyAsync(&hr, ws.range.ptr, sizeof(hr), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
         const float range = ord2f_host(hr);
-        i8 = hs[ST_INT] != 0 && range >= 0.0f && range <= 255.0f;
+        if (!mfma_filter_applies(hs, range, d)) {
+            if (st) ++st->plain_fallbacks;
+            return run_plain(A, B, self_join, eps, g, st);
+        }
+        i8 = g_int8_filter && hs[ST_INT] != 0 && range <= 255.0f;
     }
     const int KS8 = (d + 31) / 32;
 
@@ -2127,19 +2143,8 @@ int launch_recheck(const float* X, const float* Y, int d, const int2* cand, unsi
     return RNNG_OK;
 }
 
-int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
-                        int n_rowpad, const int* colmap, int n_colpad) {
-    const int KS = mfma_ksteps(d);
-    if (KS > kMaxKS) {
-        set_error("MFMA mode supports d <= 144");
-        return RNNG_ERR_UNSUPPORTED;
-    }
-    P.KS = KS;
-    RNNG_TRY(P.xa.reserve(std::max<size_t>(16, (size_t)n_rowpad * KS * 16 * 2)));
-    RNNG_TRY(P.yb.reserve(std::max<size_t>(16, (size_t)n_colpad * KS * 16 * 2)));
-    RNNG_TRY(P.xn.reserve(std::max<size_t>(16, (size_t)n_rowpad * sizeof(float))));
-    RNNG_TRY(P.yn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
-    RNNG_TRY(P.ycn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
+int mfma_input_check(MfmaPrep& P, const float* X, int n, int d) {
+    P.stats_ready = false;
     RNNG_TRY(P.colsum.reserve(sizeof(double) * d));
     RNNG_TRY(P.colmin.reserve(sizeof(unsigned) * d));
     RNNG_TRY(P.colmax.reserve(sizeof(unsigned) * d));
@@ -2163,14 +2168,38 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                                                       P.stat.as<unsigned>(), P.range.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
     P.i8 = false;
-    if (g_int8_filter && n > 0) {
+    if (n > 0) {
         unsigned hs[NSTAT], hr = 0;
         RNNG_HIP_TRY(hipMemcpyAsync(hs, P.stat.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipMemcpyAsync(&hr, P.range.ptr, sizeof(hr), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
         const float range = ord2f_host(hr);
-        P.i8 = hs[ST_INT] != 0 && range >= 0.0f && range <= 255.0f;
+        if (!mfma_filter_applies(hs, range, d)) {
+            set_error("the cell filter needs finite coordinates with d * (widest range)^2 <= 1e36 "
+                      "(the all-pairs graph takes any input)");
+            return RNNG_ERR_UNSUPPORTED;
+        }
+        P.i8 = g_int8_filter && hs[ST_INT] != 0 && range <= 255.0f;
     }
+    P.stats_ready = true;
+    return RNNG_OK;
+}
+
+int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
+                        int n_rowpad, const int* colmap, int n_colpad) {
+    const int KS = mfma_ksteps(d);
+    if (KS > kMaxKS) {
+        set_error("MFMA mode supports d <= 144");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    if (!P.stats_ready) RNNG_TRY(mfma_input_check(P, X, n, d));
+    P.stats_ready = false;
+    P.KS = KS;
+    RNNG_TRY(P.xa.reserve(std::max<size_t>(16, (size_t)n_rowpad * KS * 16 * 2)));
+    RNNG_TRY(P.yb.reserve(std::max<size_t>(16, (size_t)n_colpad * KS * 16 * 2)));
+    RNNG_TRY(P.xn.reserve(std::max<size_t>(16, (size_t)n_rowpad * sizeof(float))));
+    RNNG_TRY(P.yn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
+    RNNG_TRY(P.ycn.reserve(std::max<size_t>(16, (size_t)n_colpad * sizeof(float))));
     if (P.i8) {
         // the int8 forms, candidates for squared distances <= floor(Tb)
         P.KS8 = (d + 31) / 32;

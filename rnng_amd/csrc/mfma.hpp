@@ -30,11 +30,18 @@ struct MfmaPrep {
     // int32 start terms (xn rows, ycn columns) for the exact int8 filter
     bool i8 = false;
     int KS8 = 0;                                     // 32-wide int8 k-steps
+    bool stats_ready = false;                        // mfma_input_check ran for the next prepare
 };
 
 // Number of 16-wide k-steps for dimension d.
 inline int mfma_ksteps(int d) { return (d + 15) / 16; }
 constexpr int kMaxKS = 9;     // d <= 144
+
+// Column statistics of the n points of X (centering, integer flag, ranges),
+// read back to decide the filter: RNNG_ERR_UNSUPPORTED when no filter can bound
+// the distances (a NaN or an infinity, or d * (widest range)^2 > 1e36), else
+// sets P.i8 and marks the statistics ready for the next mfma_prepare_mapped.
+int mfma_input_check(MfmaPrep& P, const float* X, int n, int d);
 
 // Builds the x-form fragments and row norms of the padded row positions
 // (rowmap[p] = point index or -1) and the y-form fragments and column start

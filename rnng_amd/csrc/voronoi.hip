@@ -953,6 +953,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     Events ev;
     RNNG_TRY(ev.create());
     RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
+    // coordinates the cell filter (and the cost model's features) can use
+    RNNG_TRY(mfma_input_check(v->prep, v->ds->X, n, d));
     // ---- cell adjacency ----
     const int words = (m + 31) / 32;
     LocalBuf& rad = v->rad;

@@ -307,3 +307,60 @@ def test_int8_filter_eps_extremes(orc):
     assert ij.shape[0] == 200 * 199 // 2 and st.int8_filter == 1
     ij_ref, w_ref = orc.l2_eps_pairs(X[:200], F(1e9))
     _check_same(ij, w, ij_ref, w_ref)
+
+
+def test_nonfinite_and_huge_coordinates(orc):
+    """The MFMA filters bound distances from centered BF16 or int8 forms, which
+    needs finite coordinates whose squared differences stay far from FP32
+    overflow.  Other inputs (NaN, infinities, a span of 1e20) run the plain
+    kernel, which follows the reference's FP32 arithmetic for any input: a NaN
+    distance is never an edge, an infinite one only for eps = inf.  The cell
+    path refuses such inputs with an error."""
+    from rnng_amd import Stats, _lib
+    from rnng_amd.l2 import Dataset
+    from rnng_amd.voronoi import Voronoi
+    rng = np.random.default_rng(21)
+
+    def eps_for(M):
+        S = M[:200].astype(np.float64)
+        D2 = ((S[:, None, :] - S[None, :, :]) ** 2).sum(2)
+        return F(np.sqrt(np.quantile(D2[D2 > 0], 0.01)))
+
+    X = rng.standard_normal((2000, 16)).astype(F)
+    Xi = rng.integers(0, 256, size=(2000, 24)).astype(F)
+    eps_x, eps_i = eps_for(X), eps_for(Xi)
+    A = X.copy()
+    A[5, 0] = np.nan
+    A[17, 3] = A[18, 3] = np.inf
+    A[40, 1] = -np.inf
+    Bi = Xi.copy()
+    Bi[7, 2] = Bi[9, 2] = np.inf          # integer-valued except two infinities
+    H = X.copy()
+    H[:50] *= F(1e20)                      # squares of this span overflow FP32
+    for name, M, eps in (("nan/inf", A, eps_x), ("int+inf", Bi, eps_i), ("huge", H, eps_x),
+                         ("inf eps", A[:300], F(np.inf))):
+        flush_print(f"non-finite coordinates: {name}")
+        st = Stats()
+        ij, w = _l2().eps_pairs(M, eps, "mfma", stats=st)
+        ij_ref, w_ref = orc.l2_eps_pairs(M, eps)
+        _check_same(ij, w, ij_ref, w_ref)
+        assert st.plain_fallbacks == 1 and st.int8_filter == 0 and st.candidates == 0
+        assert ij.shape[0] > 0
+    assert not np.isin([5, 17, 18, 40], _l2().eps_pairs(A, eps_x, "mfma")[0]).any()
+    st = Stats()
+    ij, w = _l2().eps_bipartite(A[:1000], A[1000:], eps_x, "mfma", stats=st)
+    ij_ref, w_ref = orc.l2_eps_pairs(A[:1000], eps_x, A[1000:])
+    _check_same(ij, w, ij_ref, w_ref)
+    assert st.plain_fallbacks == 1
+    st = Stats()
+    ij, w = _l2().eps_pairs(X, eps_x, "mfma", stats=st)
+    assert st.plain_fallbacks == 0 and st.candidates > 0
+    flush_print("non-finite coordinates: cell path")
+    ds = Dataset(A)
+    v = Voronoi(ds, fps=16)
+    try:
+        with pytest.raises(_lib.RnngError):
+            v.graph(eps_x).close()
+    finally:
+        v.close()
+        ds.close()
