@@ -194,7 +194,8 @@ int rnng_strings_destroy(rnng_strings_t s);
 
 /* All pairs i < j whose Levenshtein distance (unit costs, the reference's
  * EditDistance functor, drivers/systolic.cpp:271-299) is <= eps; the weight
- * is the distance as a float.  eps up to 32 (RNNG_ERR_UNSUPPORTED above). */
+ * is the distance as a float.  Any eps; above 32 every string must have at most
+ * 512 atoms (RNNG_ERR_UNSUPPORTED otherwise). */
 int rnng_edit_graph_build(rnng_strings_t s, float eps, rnng_graph_t *out, rnng_stats *stats);
 
 /* One-shot host form with the count-then-fill contract of rnng_l2_eps_pairs_f32. */

@@ -30,7 +30,9 @@
 //      <= k and exceeds k otherwise.  The match vector of a column is an AND
 //      over the bit-planes of the string set's dense alphabet codes (2 planes
 //      for DNA).  k = 32 (a 65-bit window) uses k_edit_verify, Ukkonen's
-//      banded dynamic programme with the same band.
+//      banded dynamic programme with the same band; k > 32 uses
+//      k_edit_verify_myers, the same recurrence unbanded over all rows of the
+//      shorter string (up to 8 64-bit words: strings of at most 512 atoms).
 #include "internal.hpp"
 #include "mfma.hpp"
 
@@ -514,6 +516,124 @@ __global__ __launch_bounds__(256) void k_edit_verify_bp(
     }
 }
 
+// Exact Levenshtein distance of pattern p (rows, length m <= 64 W) and text t
+// (columns, length n), for radii beyond the banded kernels: Myers' bit-parallel
+// column step over all m rows, W 64-bit words of vertical deltas (Pv = +1,
+// Mv = -1) with the carries of the addition and of the shifts passed word to
+// word; D[0][j] = j enters as the +1 shifted into row 1's horizontal delta, and
+// the score follows row m (D[m][0] = m).  Rows past m (pattern codes read past
+// its end) only feed rows below them, so they never reach row m.  Returns
+// D[m][n], or a value > k as soon as D[m][j] - (n - j) > k after a group of
+// 32 columns (D[m][n] >= D[m][j] - (n - j)).  Same read-past-end allowance as
+// banded_edit_bp (at most 31 bytes past either string).
+template <int W, int NP>
+__device__ int myers_edit(const uint8_t* __restrict__ p, int m, const uint8_t* __restrict__ t, int n, int k) {
+    if (m == 0) return n;
+    uint64_t pl[NP][W];
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pl[q][w] = 0;
+#pragma unroll
+    for (int u = 0; u < 2 * W; ++u) {
+        if (32 * u < m) {
+            unsigned pw[NP];
+            plane_words<NP>(p + 32 * u, pw);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pl[q][u >> 1] |= (uint64_t)pw[q] << (32 * (u & 1));
+        }
+    }
+    uint64_t Pv[W], Mv[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        Pv[w] = ~0ull;
+        Mv[w] = 0ull;
+    }
+    const int wl = (m - 1) >> 6;   // word and bit of row m
+    const int bl = (m - 1) & 63;
+    int score = m;
+    for (int g = 0; 32 * g < n; ++g) {
+        unsigned tw[NP];
+        plane_words<NP>(t + 32 * g, tw);
+        const int cols = min(32, n - 32 * g);
+        for (int u = 0; u < cols; ++u) {
+            uint64_t cy = 0, ph_in = 1, mh_in = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint64_t Eq = ~0ull;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) Eq &= pl[q][w] ^ ((uint64_t)((tw[q] >> u) & 1u) - 1ull);
+                const uint64_t Xv = Eq | Mv[w];
+                const uint64_t a = Eq & Pv[w];
+                const uint64_t s1 = a + Pv[w];
+                const uint64_t s2 = s1 + cy;
+                cy = (uint64_t)((s1 < a) | (s2 < s1));
+                const uint64_t Xh = (s2 ^ Pv[w]) | Eq;
+                const uint64_t Ph = Mv[w] | ~(Xh | Pv[w]);
+                const uint64_t Mh = Pv[w] & Xh;
+                if (w == wl) score += (int)((Ph >> bl) & 1ull) - (int)((Mh >> bl) & 1ull);
+                const uint64_t Phs = (Ph << 1) | ph_in;
+                const uint64_t Mhs = (Mh << 1) | mh_in;
+                ph_in = Ph >> 63;
+                mh_in = Mh >> 63;
+                Pv[w] = Mhs | ~(Xv | Phs);
+                Mv[w] = Phs & Xv;
+            }
+        }
+        if (score - (n - 32 * g - cols) > k) return k + 1;
+    }
+    return score;
+}
+
+template <int W, int NP>
+__global__ __launch_bounds__(256) void k_edit_verify_myers(
+    const uint8_t* __restrict__ codes, const int64_t* __restrict__ coff, const int* __restrict__ slen,
+    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
+    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    unsigned long long* __restrict__ counters) {
+    const unsigned long long nc = min(counters[0], ccap);
+    const int lane = lane_id();
+    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
+         base += (unsigned long long)gridDim.x * blockDim.x) {
+        const unsigned long long c = base + threadIdx.x;
+        bool edge = false;
+        int2 pr = make_int2(0, 0);
+        int dist = 0;
+        if (c < nc) {
+            const int2 q = cand[c];
+            const int m = slen[q.x], nn = slen[q.y];
+            if (m - nn <= k && nn - m <= k) {
+                // the shorter string is the pattern (the distance is symmetric)
+                dist = m <= nn ? myers_edit<W, NP>(codes + coff[q.x], m, codes + coff[q.y], nn, k)
+                               : myers_edit<W, NP>(codes + coff[q.y], nn, codes + coff[q.x], m, k);
+                edge = dist <= k;
+            }
+            if (edge) {
+                const int i = order[q.x], j = order[q.y];
+                pr = make_int2(min(i, j), max(i, j));
+            }
+        }
+        const unsigned long long msk = __ballot(edge);
+        if (msk) {
+            unsigned lo = 0, hi = 0;
+            if (lane == 0) {
+                unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)__popcll(msk));
+                lo = (unsigned)b0;
+                hi = (unsigned)(b0 >> 32);
+            }
+            lo = __shfl(lo, 0, 64);
+            hi = __shfl(hi, 0, 64);
+            const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
+            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+            if (edge && b0 + rank < ecap) {
+                eij[b0 + rank] = pr;
+                ew[b0 + rank] = (float)dist;
+            }
+        }
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -552,7 +672,38 @@ void launch_verify_planes(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph
     else launch_verify_bp<VT, 8>(S, ws, k, g);
 }
 
+template <int W, int NP>
+void launch_verify_myers(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    k_edit_verify_myers<W, NP><<<2048, 256, 0, stream()>>>(
+        S->codes.as<uint8_t>(), S->coff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
+        ws.cand.as<int2>(), ws.cand_cap, k, g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
+        ws.counters.as<unsigned long long>());
+}
+
+template <int W>
+void launch_myers_planes(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    if (S->planes <= 2) launch_verify_myers<W, 2>(S, ws, k, g);
+    else if (S->planes <= 3) launch_verify_myers<W, 3>(S, ws, k, g);
+    else if (S->planes <= 4) launch_verify_myers<W, 4>(S, ws, k, g);
+    else launch_verify_myers<W, 8>(S, ws, k, g);
+}
+
+constexpr int kMyersMaxLen = 512;   // longest string of a radius above 32 (8 words)
+
 int dispatch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+    if (k > 32) {
+        const int maxlen = S->len_sorted.empty() ? 0 : S->len_sorted.back();
+        if (maxlen <= 64) launch_myers_planes<1>(S, ws, k, g);
+        else if (maxlen <= 128) launch_myers_planes<2>(S, ws, k, g);
+        else if (maxlen <= 256) launch_myers_planes<4>(S, ws, k, g);
+        else if (maxlen <= kMyersMaxLen) launch_myers_planes<8>(S, ws, k, g);
+        else {
+            set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
+            return RNNG_ERR_UNSUPPORTED;
+        }
+        RNNG_HIP_TRY(hipGetLastError());
+        return RNNG_OK;
+    }
     if (k <= 15) {
         launch_verify_planes<uint32_t>(S, ws, k, g);
         RNNG_HIP_TRY(hipGetLastError());
@@ -587,17 +738,21 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
         return RNNG_OK;
     }
     const int k = eps >= 1e9f ? 1000000000 : (int)floorf(eps);
-    if (k > 32) {
-        set_error("edit-distance radius above 32 is not supported");
+    if (k > 32 && !S->len_sorted.empty() && S->len_sorted.back() > kMyersMaxLen) {
+        set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
         return RNNG_ERR_UNSUPPORTED;
     }
+    // the clamped 3-gram L1 of two strings is at most 512, so any radius above
+    // 100 passes every length-compatible pair (and -(6k + 1) stays in range)
+    const int kf = std::min(k, 100);
+    const int kw = std::min(k, 1 << 30);
     // work list: row block x column-tile chunk over the length-compatible range
     ws.work_host.clear();
     const int nrb = (int)(S->n_pad / ERB);
     for (int rb = 0; rb < nrb; ++rb) {
         if ((int64_t)rb * ERB >= n) break;
         const int a_last = std::min(n - 1, rb * ERB + ERB - 1);
-        const int maxlen = S->len_sorted[a_last] + k;
+        const int maxlen = (int)std::min<int64_t>((int64_t)S->len_sorted[a_last] + kw, INT32_MAX);
         const int b_end = (int)(std::upper_bound(S->len_sorted.begin(), S->len_sorted.end(), maxlen) -
                                 S->len_sorted.begin());
         const int ct_end = (b_end + 31) / 32;
@@ -625,7 +780,7 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
         if (!ws.work_host.empty())
             k_edit_filter<<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
                 S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(), S->mnrm.as<int>(), S->slen.as<int>(),
-                ws.work.as<int4>(), n, k, -(6 * k + 1), ws.cand.as<int2>(), ws.cand_cap,
+                ws.work.as<int4>(), n, k, -(6 * kf + 1), ws.cand.as<int2>(), ws.cand_cap,
                 ws.counters.as<unsigned long long>());
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));

@@ -73,3 +73,76 @@ def test_banded_bit_parallel_equals_full_dp():
             assert b > k, (s, t, k, d, b)
         checked += 1
     assert within > 1000
+
+
+M64 = (1 << 64) - 1
+
+
+def multiword_bit_parallel(p: bytes, t: bytes, k: int, words: int, past_end: bytes) -> int:
+    """The recurrence of k_edit_verify_myers (radii above 32): the same column
+    step over every row of the pattern, held in `words` 64-bit words with the
+    carries of the addition and of the shifts passed word to word, bits past
+    the pattern's end taken from `past_end` (codes read past the string), the
+    score following row m, and the early exit after each group of 32 columns."""
+    m, n = len(p), len(t)
+    if m == 0:
+        return n
+    pp = p + past_end[: 64 * words - m]
+    Pv, Mv = [M64] * words, [0] * words
+    wl, bl = (m - 1) >> 6, (m - 1) & 63
+    score = m
+    for g in range((n + 31) // 32):
+        cols = min(32, n - 32 * g)
+        for u in range(cols):
+            c = t[32 * g + u]
+            cy, ph_in, mh_in = 0, 1, 0
+            for w in range(words):
+                Eq = sum(1 << b for b in range(64) if pp[64 * w + b] == c)
+                Xv = Eq | Mv[w]
+                a = Eq & Pv[w]
+                s1 = (a + Pv[w]) & M64
+                s2 = (s1 + cy) & M64
+                cy = int(s1 < a or s2 < s1)
+                Xh = (s2 ^ Pv[w]) | Eq
+                Ph = (Mv[w] | ~(Xh | Pv[w])) & M64
+                Mh = Pv[w] & Xh
+                if w == wl:
+                    score += ((Ph >> bl) & 1) - ((Mh >> bl) & 1)
+                Phs = ((Ph << 1) | ph_in) & M64
+                Mhs = ((Mh << 1) | mh_in) & M64
+                ph_in, mh_in = Ph >> 63, Mh >> 63
+                Pv[w] = (Mhs | ~(Xv | Phs)) & M64
+                Mv[w] = Phs & Xv
+        if score - (n - 32 * g - cols) > k:
+            return k + 1
+    return score
+
+
+def test_multiword_bit_parallel_equals_full_dp():
+    """Equal to the full dynamic programme whenever the distance is <= k,
+    above k otherwise, for patterns filling 1, 2 or 3 words (carries across
+    word boundaries), garbage codes past the pattern's end and radii above
+    32 (the early exit fires on the far pairs)."""
+    rng = random.Random(7)
+    within = 0
+    for _ in range(500):
+        alpha = rng.choice([b"AC", b"ACGT", b"abcdefghijklmnopqrstu"])
+        words = rng.choice([1, 2, 3])
+        m = rng.randint(0, 64 * words)
+        s = bytes(rng.choice(alpha) for _ in range(m))
+        if rng.random() < 0.6:
+            t = _mutate(rng, s, alpha, rng.randint(0, 50))
+        else:
+            t = bytes(rng.choice(alpha) for _ in range(rng.randint(0, 64 * words + 30)))
+        if len(t) < len(s):
+            s, t = t, s                     # the shorter string is the pattern
+        k = rng.choice([33, 40, 64, 10 ** 9])
+        past_end = bytes(rng.choice(alpha) for _ in range(64 * words))
+        d = oracle.edit_distance(s, t)
+        b = multiword_bit_parallel(s, t, k, words, past_end)
+        if d <= k:
+            assert b == d, (s, t, k, d, b)
+            within += 1
+        else:
+            assert b > k, (s, t, k, d, b)
+    assert within > 150

@@ -59,9 +59,47 @@ def test_edge_cases(orc):
     assert edit.eps_pairs(atoms, offs, -1)[0].shape[0] == 0
     a1, o1 = edit.as_csr([b"ACGT"])
     assert edit.eps_pairs(a1, o1, 5)[0].shape[0] == 0
+    # radii above 32 verify with the unbanded bit-parallel kernel
+    ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, 40)
+    ij, d = edit.eps_pairs(atoms, offs, 40)
+    assert np.array_equal(ij, ij_ref) and np.array_equal(d, d_ref)
+    # ... for strings of at most 512 atoms
     from rnng_amd import RnngError
+    a2, o2 = edit.as_csr([b"A" * 513, b"A" * 500])
     with pytest.raises(RnngError):
-        edit.eps_pairs(atoms, offs, 40)
+        edit.eps_pairs(a2, o2, 40)
+    assert edit.eps_pairs(a2, o2, 20)[0].shape[0] == 1
+
+
+@pytest.mark.parametrize("alphabet,maxlen", [(b"ACGT", 150), (b"ACGT", 300), (b"abcdefghijklmnopqrstu", 64),
+                                             (b"abcdefghijklmnopqrstu", 512)])
+def test_radius_above_32(orc, alphabet, maxlen):
+    """Radii above 32 (beyond the banded kernels): the multiword bit-parallel
+    verify gives the oracle's pairs and distances, with 1, 2, 4 or 8 words and
+    2 or 5 code bit-planes; an infinite radius links every pair."""
+    rng = np.random.default_rng(maxlen)
+    strs = [rng.choice(list(alphabet), int(rng.integers(0, maxlen + 1))).astype(np.uint8).tobytes()
+            for _ in range(120)]
+    for s in list(strs[:60]):
+        t = bytearray(s)
+        for _ in range(int(rng.integers(0, 60))):
+            r = rng.random()
+            if t and r < 0.33:
+                t[int(rng.integers(0, len(t)))] = int(rng.choice(list(alphabet)))
+            elif t and r < 0.66:
+                del t[int(rng.integers(0, len(t)))]
+            elif len(t) < maxlen:
+                t.insert(int(rng.integers(0, len(t) + 1)), int(rng.choice(list(alphabet))))
+        strs.append(bytes(t))
+    atoms, offs = _edit().as_csr(strs)
+    for eps in (33, 47.5, 90, float("inf")):
+        flush_print(f"radius above 32: alphabet {len(alphabet)} maxlen {maxlen} eps {eps}")
+        ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, eps)
+        ij, d = _edit().eps_pairs(atoms, offs, eps)
+        assert np.array_equal(ij, ij_ref)
+        assert np.array_equal(d, d_ref)
+        assert ij.shape[0] > 0
+    assert ij.shape[0] == len(strs) * (len(strs) - 1) // 2
 
 
 def _clamped_profiles(atoms, offs):
