@@ -39,7 +39,8 @@ extern "C" {
 #define RNNG_MODE_PLAIN 0   /* FP32 tiled kernel in the reference's order */
 #define RNNG_MODE_MFMA  1   /* MFMA candidate filter (BF16, or exact int8 when every coordinate is an
                                integer and each dimension spans <= 255) + FP32 recheck + FP64 band
-                               check */
+                               check; d > 144 or coordinates no filter can bound (NaN, infinities,
+                               d * range^2 > 1e36) run the PLAIN kernel (stats: plain_fallbacks) */
 #define RNNG_MODE_GRID  2   /* self join, d <= 3: uniform grid of cells of side >= eps; only
                                pairs in adjacent cells are evaluated, each in the reference's
                                FP32 order (same edges as PLAIN); pair_evals counts them */

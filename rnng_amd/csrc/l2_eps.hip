@@ -1860,10 +1860,6 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     Workspace& ws = g_ws;
     const int nx = (int)A->n, ny = (int)B->n, d = A->d;
     const int KS = mfma_ksteps(d);
-    if (KS > kMaxKS) {
-        set_error("MFMA mode supports d <= 144");
-        return RNNG_ERR_UNSUPPORTED;
-    }
     const float T = sq_threshold_host(eps);
     const float eps_band = (float)((double)eps * (1.0 + 1e-5));
     float Tb = sq_threshold_host(std::nextafter(eps_band, INFINITY));
@@ -1911,7 +1907,8 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     RNNG_HIP_TRY(hipGetLastError());
     // integer coordinates spanning at most 255 per dimension: the exact int8
     // filter; coordinates no filter can bound (NaN, infinities, ranges whose
-    // squares approach FP32 overflow): the plain kernel, exact for any input
+    // squares approach FP32 overflow) or d > 144 (row fragments held in
+    // registers): the plain kernel, exact for any input
     bool i8 = false;
     {
         unsigned hs[NSTAT], hr = 0;
@@ -1919,7 +1916,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         RNNG_HIP_TRY(hipMemcpyAsync(&hr, ws.range.ptr, sizeof(hr), hipMemcpyDeviceToHost, stream()));
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
         const float range = ord2f_host(hr);
-        if (!mfma_filter_applies(hs, range, d)) {
+        if (!mfma_filter_applies(hs, range, d) || KS > kMaxKS) {
             if (st) ++st->plain_fallbacks;
             return run_plain(A, B, self_join, eps, g, st);
         }

@@ -364,3 +364,21 @@ def test_nonfinite_and_huge_coordinates(orc):
     finally:
         v.close()
         ds.close()
+
+
+def test_mfma_mode_above_d144_runs_plain(orc):
+    """The filters hold row fragments in registers up to d = 144; MFMA mode
+    on wider points (d = 200 here, like GIST-style vectors) runs the plain
+    kernel and gives the oracle's edges."""
+    from rnng_amd import Stats
+    rng = np.random.default_rng(200)
+    X = rng.standard_normal((1500, 200)).astype(F)
+    X[9] = X[4]
+    D2 = ((X[:40, None, :].astype(np.float64) - X[None, :, :]) ** 2).sum(2)
+    eps = F(np.sqrt(np.quantile(D2[D2 > 0], 0.01)))
+    flush_print(f"d=200 mfma mode eps={eps}")
+    st = Stats()
+    ij, w = _l2().eps_pairs(X, eps, "mfma", stats=st)
+    ij_ref, w_ref = orc.l2_eps_pairs(X, eps)
+    _check_same(ij, w, ij_ref, w_ref)
+    assert st.plain_fallbacks == 1 and ij.shape[0] > 100
