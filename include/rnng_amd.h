@@ -370,11 +370,13 @@ int rnng_set_int8_filter(int32_t enable);
 /* Filter kernel shapes, process-wide (no counterpart in the reference: for
  * measurement and for tests that run every shape).  int8_code: 0 column tiles
  * staged through registers, 1/2 direct-to-LDS ring with three stages of 1/2
- * tiles, 3/4 two stages of 3/4 tiles.  bf16_variant 0..14 (11/12 three-stage
- * ring, 13/14 two-stage ring of 2/3 tiles, the others register-staged or
- * group-load shapes).  A negative value leaves that setting unchanged; the
- * defaults (3, 13) come from RNNG_L2_I8_GROUP / RNNG_L2_VARIANT when set.
- * Every shape gives the same edges. */
+ * tiles, 3/4 two stages of 3/4 tiles, 5/6/7 two stages of 4/6/8 tiles shared
+ * by eight waves (all-pairs builds; the cell path runs code 3 for codes above
+ * 4).  bf16_variant 0..14 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
+ * tiles, the others register-staged or group-load shapes).  A negative value
+ * leaves that setting unchanged; the defaults (7, 13) come from
+ * RNNG_L2_I8_GROUP / RNNG_L2_VARIANT when set.  Every shape gives the same
+ * edges. */
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant);
 int rnng_get_filter_kernels(int32_t *int8_code, int32_t *bf16_variant);
 

@@ -1226,21 +1226,21 @@ __device__ __forceinline__ void ring_barrier(int n) {
 // into LDS (global_load_lds_dwordx4), and, from wave 3, the group's column
 // start terms (global_load_lds_dword, two tiles per instruction, (G + 1) / 2
 // instructions).
-template <class T, int KS, int G>
+template <class T, int KS, int G, int NW = 4>
 __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict__ yb,
                                               const typename T::term* __restrict__ cterm, typename T::frag* dst,
                                               typename T::term* cdst, int t0, int last, int wave, int lane) {
     constexpr int NJ = G * KS;
 #pragma unroll
-    for (int jj = 0; jj < (NJ + 3) / 4; ++jj) {
-        const int j = jj * 4 + wave;
+    for (int jj = 0; jj < (NJ + NW - 1) / NW; ++jj) {
+        const int j = jj * NW + wave;
         if (j >= NJ) break;
         const int tile = min(t0 + j / KS, last);
         const typename T::frag* src = yb + ((size_t)tile * KS + (j % KS)) * 64 + lane;
         __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + j * 64),
                                          16, 0, 0);
     }
-    if (wave == 3) {
+    if (wave == NW - 1) {
 #pragma unroll
         for (int c2 = 0; c2 < (G + 1) / 2; ++c2) {
             const typename T::term* src = cterm + min(t0 + 2 * c2 + (lane >> 5), last) * 32 + (lane & 31);
@@ -1256,8 +1256,8 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
 // time to land; one barrier per group.  The stages are distinct LDS objects
 // and the loop is unrolled by S, so the compiler sees that reads of one stage
 // never alias the asynchronous loads into the others.
-template <class T, int KS, int G, int MINW, int S = 3>
-__global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
+template <class T, int KS, int G, int MINW, int S = 3, int NW = 4>
+__global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
     const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
     const int4* __restrict__ work, int n_rows, int n_cols, int self_join, int2* __restrict__ cand,
@@ -1268,12 +1268,12 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     constexpr int NCH = KS * 64;
     constexpr int NJ = G * KS;
     constexpr int NC = 64 * ((G + 1) / 2);   // start terms per stage (two tiles per load)
-    static_assert(G >= 1 && G <= 4 && (S == 2 || S == 3), "ring shape");
+    static_assert(G >= 1 && G <= 16 && (S == 2 || S == 3), "ring shape");
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
     // candidates staged in LDS; the int8 shape with four tiles per stage keeps
     // 512 so that four workgroups fit a CU (the rest go straight to memory)
-    constexpr unsigned STG = (std::is_same<T, MfI8>::value && G == 4) ? 512u : (unsigned)LSTAGE;
+    constexpr unsigned STG = (std::is_same<T, MfI8>::value && G >= 4) ? 512u : (unsigned)LSTAGE;
     __shared__ int2 stage[STG];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
@@ -1283,21 +1283,21 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
     const int lane = tid & 63;
-    const int rt0 = wi.x * (4 * MRT) + wave * MRT;
+    const int rt0 = wi.x * (NW * MRT) + wave * MRT;
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
     // loads this wave issues per group (the same in every lane)
-    const int nld = __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / 4 + 1 : 0) +
-                                                   (wave == 3 ? (G + 1) / 2 : 0));
+    const int nld = __builtin_amdgcn_readfirstlane((wave < NJ ? (NJ - 1 - wave) / NW + 1 : 0) +
+                                                   (wave == NW - 1 ? (G + 1) / 2 : 0));
 
     // issue order: rows, group 0, group 1 (the barrier's count relies on it)
     frag a[MRT][KS];
     accv xr[MRT];
     mf_load_rows<T, KS>(xa, rterm, rt0, lane, a, xr);
     asm volatile("" ::: "memory");
-    mf_group_load<T, KS, G>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
+    mf_group_load<T, KS, G, NW>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
     asm volatile("" ::: "memory");
-    if (S == 3 && ng > 1) mf_group_load<T, KS, G>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
+    if (S == 3 && ng > 1) mf_group_load<T, KS, G, NW>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
     // rows and group 0 landed (with three stages group 1 may still be in
     // flight); scount is set
     ring_barrier(S == 3 && ng > 1 ? nld : 0);
@@ -1308,7 +1308,7 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     auto step = [&](const frag* cur, const term* ccur, frag* nxt, term* cnxt, int g) {
         const int t0 = wi.y + g * G;
         const bool issue = g + S - 1 < ng;
-        if (issue) mf_group_load<T, KS, G>(yb, cterm, nxt, cnxt, t0 + (S - 1) * G, last, wave, lane);
+        if (issue) mf_group_load<T, KS, G, NW>(yb, cterm, nxt, cnxt, t0 + (S - 1) * G, last, wave, lane);
 #pragma unroll
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
@@ -1343,7 +1343,7 @@ __global__ __launch_bounds__(256, MINW) void k_l2_mfma_filter_ring(
     const unsigned nst = min(scount, STG);
     if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
     __syncthreads();
-    for (unsigned i = tid; i < nst; i += 256)
+    for (unsigned i = tid; i < nst; i += 64 * NW)
         if (sbase + i < cap) cand[sbase + i] = stage[i];
 }
 
@@ -1684,7 +1684,9 @@ float ord2f_host(unsigned u) {
 
 // int8 filter kernels by code: 0 = k_l2_mfma_filter_i8 (column tiles staged
 // through registers); 1, 2 = the direct-to-LDS ring with 1 or 2 tiles per
-// stage and three stages; 3, 4 = 3 or 4 tiles per stage and two stages.
+// stage and three stages; 3, 4 = 3 or 4 tiles per stage and two stages; 5, 6,
+// 7 = two stages of 4, 6 or 8 tiles shared by eight waves (512 threads,
+// 16 row tiles per work item; 6 tiles at KS8 = 5 for two workgroups per CU).
 template <int KS8, int CODE>
 void launch_filter_i8(const I8Args& a) {
     constexpr int MINW = KS8 <= 4 ? 4 : 1;
@@ -1694,19 +1696,29 @@ void launch_filter_i8(const I8Args& a) {
     else if constexpr (CODE <= 2)
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<a.n_items, 256, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
-    else
+    else if constexpr (CODE <= 4)
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<a.n_items, 256, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+    else {
+        constexpr int G = CODE == 5 ? 4 : (CODE == 6 || KS8 >= 5) ? 6 : 8;
+        k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8><<<a.n_items, 512, 0, stream()>>>(
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+    }
 }
+
+constexpr int kI8Codes = 8;
+
+// row tiles per work item of an int8 kernel code (two per wave)
+int i8_row_tiles(int code) { return code >= 5 ? 16 : 8; }
 
 int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
 
 int i8_group() {
     if (g_i8_group < 0) {
-        g_i8_group = 3;
+        g_i8_group = 7;
         if (const char* e = getenv("RNNG_L2_I8_GROUP")) {
             const int v = atoi(e);
-            if (v >= 0 && v <= 4) g_i8_group = v;
+            if (v >= 0 && v < kI8Codes) g_i8_group = v;
         }
     }
     return g_i8_group;
@@ -1733,32 +1745,47 @@ I8Args i8_args(const Workspace& ws, int nx, int ny, int self_join) {
 
 int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
     if (a.n_items == 0) return RNNG_OK;
-    switch (KS8 * 8 + code) {
-        case 1 * 8 + 0: launch_filter_i8<1, 0>(a); break;
-        case 2 * 8 + 0: launch_filter_i8<2, 0>(a); break;
-        case 3 * 8 + 0: launch_filter_i8<3, 0>(a); break;
-        case 4 * 8 + 0: launch_filter_i8<4, 0>(a); break;
-        case 5 * 8 + 0: launch_filter_i8<5, 0>(a); break;
-        case 1 * 8 + 1: launch_filter_i8<1, 1>(a); break;
-        case 2 * 8 + 1: launch_filter_i8<2, 1>(a); break;
-        case 3 * 8 + 1: launch_filter_i8<3, 1>(a); break;
-        case 4 * 8 + 1: launch_filter_i8<4, 1>(a); break;
-        case 5 * 8 + 1: launch_filter_i8<5, 1>(a); break;
-        case 1 * 8 + 2: launch_filter_i8<1, 2>(a); break;
-        case 2 * 8 + 2: launch_filter_i8<2, 2>(a); break;
-        case 3 * 8 + 2: launch_filter_i8<3, 2>(a); break;
-        case 4 * 8 + 2: launch_filter_i8<4, 2>(a); break;
-        case 5 * 8 + 2: launch_filter_i8<5, 2>(a); break;
-        case 1 * 8 + 3: launch_filter_i8<1, 3>(a); break;
-        case 2 * 8 + 3: launch_filter_i8<2, 3>(a); break;
-        case 3 * 8 + 3: launch_filter_i8<3, 3>(a); break;
-        case 4 * 8 + 3: launch_filter_i8<4, 3>(a); break;
-        case 5 * 8 + 3: launch_filter_i8<5, 3>(a); break;
-        case 1 * 8 + 4: launch_filter_i8<1, 4>(a); break;
-        case 2 * 8 + 4: launch_filter_i8<2, 4>(a); break;
-        case 3 * 8 + 4: launch_filter_i8<3, 4>(a); break;
-        case 4 * 8 + 4: launch_filter_i8<4, 4>(a); break;
-        case 5 * 8 + 4: launch_filter_i8<5, 4>(a); break;
+    switch (KS8 * 16 + code) {
+        case 1 * 16 + 0: launch_filter_i8<1, 0>(a); break;
+        case 1 * 16 + 1: launch_filter_i8<1, 1>(a); break;
+        case 1 * 16 + 2: launch_filter_i8<1, 2>(a); break;
+        case 1 * 16 + 3: launch_filter_i8<1, 3>(a); break;
+        case 1 * 16 + 4: launch_filter_i8<1, 4>(a); break;
+        case 1 * 16 + 5: launch_filter_i8<1, 5>(a); break;
+        case 1 * 16 + 6: launch_filter_i8<1, 6>(a); break;
+        case 1 * 16 + 7: launch_filter_i8<1, 7>(a); break;
+        case 2 * 16 + 0: launch_filter_i8<2, 0>(a); break;
+        case 2 * 16 + 1: launch_filter_i8<2, 1>(a); break;
+        case 2 * 16 + 2: launch_filter_i8<2, 2>(a); break;
+        case 2 * 16 + 3: launch_filter_i8<2, 3>(a); break;
+        case 2 * 16 + 4: launch_filter_i8<2, 4>(a); break;
+        case 2 * 16 + 5: launch_filter_i8<2, 5>(a); break;
+        case 2 * 16 + 6: launch_filter_i8<2, 6>(a); break;
+        case 2 * 16 + 7: launch_filter_i8<2, 7>(a); break;
+        case 3 * 16 + 0: launch_filter_i8<3, 0>(a); break;
+        case 3 * 16 + 1: launch_filter_i8<3, 1>(a); break;
+        case 3 * 16 + 2: launch_filter_i8<3, 2>(a); break;
+        case 3 * 16 + 3: launch_filter_i8<3, 3>(a); break;
+        case 3 * 16 + 4: launch_filter_i8<3, 4>(a); break;
+        case 3 * 16 + 5: launch_filter_i8<3, 5>(a); break;
+        case 3 * 16 + 6: launch_filter_i8<3, 6>(a); break;
+        case 3 * 16 + 7: launch_filter_i8<3, 7>(a); break;
+        case 4 * 16 + 0: launch_filter_i8<4, 0>(a); break;
+        case 4 * 16 + 1: launch_filter_i8<4, 1>(a); break;
+        case 4 * 16 + 2: launch_filter_i8<4, 2>(a); break;
+        case 4 * 16 + 3: launch_filter_i8<4, 3>(a); break;
+        case 4 * 16 + 4: launch_filter_i8<4, 4>(a); break;
+        case 4 * 16 + 5: launch_filter_i8<4, 5>(a); break;
+        case 4 * 16 + 6: launch_filter_i8<4, 6>(a); break;
+        case 4 * 16 + 7: launch_filter_i8<4, 7>(a); break;
+        case 5 * 16 + 0: launch_filter_i8<5, 0>(a); break;
+        case 5 * 16 + 1: launch_filter_i8<5, 1>(a); break;
+        case 5 * 16 + 2: launch_filter_i8<5, 2>(a); break;
+        case 5 * 16 + 3: launch_filter_i8<5, 3>(a); break;
+        case 5 * 16 + 4: launch_filter_i8<5, 4>(a); break;
+        case 5 * 16 + 5: launch_filter_i8<5, 5>(a); break;
+        case 5 * 16 + 6: launch_filter_i8<5, 6>(a); break;
+        case 5 * 16 + 7: launch_filter_i8<5, 7>(a); break;
         default:
             set_error("int8 filter supports d <= 160");
             return RNNG_ERR_UNSUPPORTED;
@@ -1925,7 +1952,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     const int KS8 = (d + 31) / 32;
 
     const int variant = filter_variant();
-    const int rb_tiles = 4 * (i8 ? 2 : kVariants[variant].rt);   // row tiles per workgroup
+    const int rb_tiles = i8 ? i8_row_tiles(i8_group()) : 4 * kVariants[variant].rt;   // row tiles per workgroup
     const int rb_rows = 32 * rb_tiles;
     const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
     const int n_coltiles = (ny + 31) / 32;
@@ -2331,7 +2358,7 @@ int rnng_set_int8_filter(int32_t enable) {
 }
 
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant) {
-    if (int8_code > 4 || bf16_variant >= kNumVariants) {
+    if (int8_code >= kI8Codes || bf16_variant >= kNumVariants) {
         set_error("rnng_set_filter_kernels: int8 code 0..4, BF16 variant 0..14 (negative: unchanged)");
         return RNNG_ERR_ARG;
     }

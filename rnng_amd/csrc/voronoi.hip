@@ -1047,7 +1047,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             ia.cand = cand_ptr;
             ia.cap = cap_;
             ia.counter = counters.as<unsigned long long>();
-            return filter_dispatch_i8(ia, P.KS8, filter_i8_code());
+            return filter_dispatch_i8(ia, P.KS8, filter_i8_code() > 4 ? 3 : filter_i8_code());
         }
         return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
     };
