@@ -131,6 +131,8 @@ def bench_cells(args):
     ach = 2.0 * d * P_all / (st.filter_ms / K * 1e-3) / 1e12
     kname = {11: "k_l2_mfma_filter_ring<bf16, G=1, S=3>", 12: "k_l2_mfma_filter_ring<bf16, G=2, S=3>",
              13: "k_l2_mfma_filter_ring<bf16, G=2, S=2>", 14: "k_l2_mfma_filter_ring<bf16, G=3, S=2>",
+             16: "k_l2_mfma_filter_ring<bf16, G=4, S=2, 12 waves>",
+             17: "k_l2_mfma_filter_ring<bf16, G=8, S=2, 12 waves>",
              8: "k_l2_mfma_filter_g<G=2>"}.get(tv, "k_l2_mfma_filter")
     out["roofline"] = {"kernel": f"{kname} (autotuned variant {tv})", "bound": "mfma", "achieved": ach,
                        "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0, "traffic": None,

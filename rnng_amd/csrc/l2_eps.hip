@@ -1539,25 +1539,28 @@ struct Workspace {
 };
 
 // Filter variants: {row tiles per wave, stage column tiles in LDS, column
-// tiles per step}.
+// tiles per step, waves per workgroup}.
 struct FilterVariant {
     int rt;
     bool lds;
     int ct;
+    int nw;
 };
 // Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4;
 // 11 and 12 the direct-to-LDS ring (three stages of 1 or 2 tiles), 13 and 14
-// the ring with two stages of 2 or 3 tiles.
-constexpr FilterVariant kVariants[] = {{2, false, 1}, {2, true, 1}, {4, true, 1}, {3, true, 1},
-                                       {2, true, 2}, {1, true, 4}, {1, true, 2},
-                                       {2, true, 1}, {2, true, 2}, {2, true, 3}, {2, true, 4},
-                                       {2, true, 1}, {2, true, 2}, {2, true, 2}, {2, true, 3}};
-constexpr int kNumVariants = 15;
+// the ring with two stages of 2 or 3 tiles; 15 two stages of 4 tiles shared by
+// 8 waves, 16 and 17 two stages of 4 or 8 tiles shared by 12 waves.
+constexpr FilterVariant kVariants[] = {{2, false, 1, 4}, {2, true, 1, 4}, {4, true, 1, 4}, {3, true, 1, 4},
+                                       {2, true, 2, 4},  {1, true, 4, 4}, {1, true, 2, 4},
+                                       {2, true, 1, 4},  {2, true, 2, 4}, {2, true, 3, 4}, {2, true, 4, 4},
+                                       {2, true, 1, 4},  {2, true, 2, 4}, {2, true, 2, 4}, {2, true, 3, 4},
+                                       {2, true, 4, 8},  {2, true, 4, 12}, {2, true, 8, 12}};
+constexpr int kNumVariants = 18;
 int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
 
 int filter_variant() {
     if (g_variant < 0) {
-        g_variant = 13;
+        g_variant = 17;
         if (const char* e = getenv("RNNG_L2_VARIANT")) {
             const int v = atoi(e);
             if (v >= 0 && v < kNumVariants) g_variant = v;
@@ -1643,9 +1646,9 @@ void launch_filter_g(const FilterArgs& a) {
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
-template <int KS, int G, int S = 3>
+template <int KS, int G, int S = 3, int NW = 4>
 void launch_filter_ring(const FilterArgs& a) {
-    k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S><<<a.n_items, 256, 0, stream()>>>(
+    k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S, NW><<<a.n_items, 64 * NW, 0, stream()>>>(
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
 }
 
@@ -1656,6 +1659,9 @@ void launch_filter_v(const FilterArgs& a, int variant) {
         case 12: launch_filter_ring<KS, 2>(a); return;
         case 13: launch_filter_ring<KS, 2, 2>(a); return;
         case 14: launch_filter_ring<KS, 3, 2>(a); return;
+        case 15: launch_filter_ring<KS, 4, 2, 8>(a); return;
+        case 16: launch_filter_ring<KS, 4, 2, 12>(a); return;
+        case 17: launch_filter_ring<KS, (KS >= 9 ? 6 : 8), 2, 12>(a); return;
         case 7: launch_filter_g<KS, 1>(a); return;
         case 8: launch_filter_g<KS, 2>(a); return;
         case 9: launch_filter_g<KS, 3>(a); return;
@@ -1952,7 +1958,7 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     const int KS8 = (d + 31) / 32;
 
     const int variant = filter_variant();
-    const int rb_tiles = i8 ? i8_row_tiles(i8_group()) : 4 * kVariants[variant].rt;   // row tiles per workgroup
+    const int rb_tiles = i8 ? i8_row_tiles(i8_group()) : filter_block_tiles(variant);   // row tiles per workgroup
     const int rb_rows = 32 * rb_tiles;
     const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
     const int n_coltiles = (ny + 31) / 32;
@@ -2127,7 +2133,7 @@ int filter_dispatch(const FilterArgs& a, int KS, int variant) {
     return RNNG_OK;
 }
 
-int filter_block_tiles(int variant) { return 4 * kVariants[variant].rt; }
+int filter_block_tiles(int variant) { return kVariants[variant].nw * kVariants[variant].rt; }
 int filter_default_variant() { return filter_variant(); }
 int filter_num_variants() { return kNumVariants; }
 

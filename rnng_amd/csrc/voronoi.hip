@@ -862,12 +862,12 @@ int check_voronoi(const rnng_voronoi_s* v) {
     return RNNG_OK;
 }
 
-// Autotuned filter configurations: (variant, XCD ordering).  Variants 13 and
-// 14 = the direct-to-LDS ring with two stages of 2 or 3 column tiles, 12 =
-// three stages of 2 tiles (all 2 row tiles per wave), 2 = column tiles staged
-// in LDS through registers with 4 row tiles per wave.
-constexpr int kTuneVariants[4] = {13, 14, 12, 2};
-constexpr int kPadRows = 1536;   // multiple of every tuned row block (256, 384, 512 rows)
+// Autotuned filter configurations: (variant, XCD ordering).  Variant 17 = the
+// direct-to-LDS ring with two stages of 8 column tiles shared by 12 waves (24
+// row tiles per work item), 13 and 14 = two stages of 2 or 3 tiles shared by 4
+// waves, 12 = three stages of 2 tiles (all 2 row tiles per wave).
+constexpr int kTuneVariants[4] = {17, 13, 14, 12};
+constexpr int kPadRows = 1536;   // multiple of every row block (256, 512, 768 rows)
 
 int tuning_enabled() {
     static int on = -1;
