@@ -1801,6 +1801,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
 }
 
 int filter_i8_code() { return i8_group(); }
+int filter_i8_row_tiles(int code) { return i8_row_tiles(code); }
 
 namespace {
 

@@ -90,6 +90,7 @@ struct I8Args {
 int filter_dispatch_i8(const I8Args& a, int KS8, int code);
 int filter_i8_code();
 int filter_block_tiles(int variant);
+int filter_i8_row_tiles(int code);   // row tiles per work item of an int8 kernel code
 int filter_default_variant();
 int filter_num_variants();
 constexpr int kFilterChunk = 256;   // column tiles per chunk of the work list

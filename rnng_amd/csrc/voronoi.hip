@@ -869,6 +869,12 @@ int check_voronoi(const rnng_voronoi_s* v) {
 constexpr int kTuneVariants[4] = {17, 13, 14, 12};
 constexpr int kPadRows = 1536;   // multiple of every row block (256, 512, 768 rows)
 
+// int8 kernel of the cell path: the eight-wave shapes (codes above 4) cover
+// 512-row blocks, whose column-tile unions over short cell runs cost more than
+// their longer groups save (1M x 128 integer points in 4,096 cells: 43.0 ms
+// against 34.7 ms for code 3), so those codes map to code 3 here.
+int cell_i8_code() { return filter_i8_code() > 4 ? 3 : filter_i8_code(); }
+
 int tuning_enabled() {
     static int on = -1;
     if (on < 0) {
@@ -1047,12 +1053,12 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             ia.cand = cand_ptr;
             ia.cap = cap_;
             ia.counter = counters.as<unsigned long long>();
-            return filter_dispatch_i8(ia, P.KS8, filter_i8_code() > 4 ? 3 : filter_i8_code());
+            return filter_dispatch_i8(ia, P.KS8, cell_i8_code());
         }
         return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
     };
     auto work_list = [&](int var, int xcd, std::vector<int4>& out) {
-        const int bt = P.i8 ? 8 : filter_block_tiles(var);   // int8 kernels: 8 row tiles per block
+        const int bt = P.i8 ? filter_i8_row_tiles(cell_i8_code()) : filter_block_tiles(var);
         const int64_t cov = cell_work(rs, adj, words, m, bt, (rs[m] + bt * 32 - 1) / (bt * 32), out);
         if (xcd) xcd_permute(out);
         return cov;
