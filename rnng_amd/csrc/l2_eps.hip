@@ -1574,14 +1574,14 @@ Workspace g_ws;
 int64_t g_cand_hint = 0;   // initial candidate-buffer size (0: default)
 
 
-// Row blocks of 8 tiles (256 rows) x column chunks of chunk_tiles() tiles,
+// Row blocks of the kernel's row tiles x column chunks of chunk_tiles() tiles,
 // chunk-major so that the workgroups resident at one time share column
-// fragments in L2 (RNNG_L2_CHUNK overrides the default of 256).
+// fragments in L2 (RNNG_L2_CHUNK overrides the default of 192).
 int g_chunk = 0;
 
 int chunk_tiles() {
     if (g_chunk == 0) {
-        g_chunk = 256;
+        g_chunk = 192;
         if (const char* e = getenv("RNNG_L2_CHUNK")) {
             const int v = atoi(e);
             if (v >= 32 && v <= 4096) g_chunk = v;
