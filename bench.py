@@ -262,6 +262,8 @@ def main():
                     help="l2: 1M x 128 all-pairs (default, the headline); cells: 2M x 96 through "
                          "Voronoi cells; edit: 200k strings; rips: 300k x 3 2-skeleton")
     ap.add_argument("--cells", type=int, default=4096, help="landmarks of the cells workload")
+    ap.add_argument("--bf16", action="store_true",
+                    help="run the timed steps on the BF16 filter even where the exact int8 one applies")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -295,6 +297,8 @@ def main():
     except Exception:
         sync = lambda: None  # noqa: E731
 
+    if args.bf16:
+        _lib.check(_lib.lib().rnng_set_int8_filter(0))
     for w in range(args.warmup):
         g = ds.graph(eps, "mfma")
         log(f"warmup {w}: {g.num_edges} edges, {g.stats.total_ms:.1f} ms device")

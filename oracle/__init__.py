@@ -91,9 +91,11 @@ def sq_threshold(eps: float) -> np.float32:
     return np.float32(_load().orc_sq_threshold(np.float32(eps)))
 
 
-def l2_eps_pairs(X, eps, Y=None, nthreads: int = 0):
+def l2_eps_pairs(X, eps, Y=None, nthreads: int = 0, cap: int = 0):
     """Brute-force epsilon pairs.  Self join (Y is None): i < j.  Returns
-    (ij int64 [E,2], w float32 [E]) in row-major order."""
+    (ij int64 [E,2], w float32 [E]) in row-major order.  cap > 0 is a size
+    guess: when the count fits, the pairs are computed in one pass instead of
+    a counting pass and a filling pass."""
     lib = _load()
     X = _f32(X)
     self_join = Y is None
@@ -101,7 +103,14 @@ def l2_eps_pairs(X, eps, Y=None, nthreads: int = 0):
     nx, d = X.shape
     ny = Y.shape[0]
     eps = np.float32(eps)
-    n = lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, None, None, 0)
+    if cap > 0:
+        ij = np.empty((cap, 2), np.int64)
+        w = np.empty(cap, np.float32)
+        n = lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, _ptr(ij), _ptr(w), cap)
+        if n <= cap:
+            return ij[:n].copy(), w[:n].copy()
+    else:
+        n = lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, None, None, 0)
     ij = np.empty((n, 2), np.int64)
     w = np.empty(n, np.float32)
     lib.orc_l2_eps_pairs(X, nx, Y, ny, d, eps, int(self_join), nthreads, _ptr(ij), _ptr(w), n)
@@ -121,15 +130,22 @@ def edit_distance(s: bytes, t: bytes) -> int:
     return int(_load().orc_edit_distance(s, len(s), t, len(t)))
 
 
-def edit_eps_pairs(atoms: np.ndarray, offsets: np.ndarray, eps, nthreads: int = 0):
+def edit_eps_pairs(atoms: np.ndarray, offsets: np.ndarray, eps, nthreads: int = 0, cap: int = 0):
     """Brute-force edit-distance pairs i < j with dist <= eps.  Returns
-    (ij int64 [E,2], dist int32 [E])."""
+    (ij int64 [E,2], dist int32 [E]).  cap > 0: one pass when the count fits."""
     lib = _load()
     atoms = np.ascontiguousarray(atoms, dtype=np.uint8)
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     n = offsets.shape[0] - 1
     eps = np.float32(eps)
-    cnt = lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, None, None, 0)
+    if cap > 0:
+        ij = np.empty((cap, 2), np.int64)
+        dist = np.empty(cap, np.int32)
+        cnt = lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, _ptr(ij), _ptr(dist), cap)
+        if cnt <= cap:
+            return ij[:cnt].copy(), dist[:cnt].copy()
+    else:
+        cnt = lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, None, None, 0)
     ij = np.empty((cnt, 2), np.int64)
     dist = np.empty(cnt, np.int32)
     lib.orc_edit_eps_pairs(_ptr(atoms), offsets, n, eps, nthreads, _ptr(ij), _ptr(dist), cnt)
@@ -241,12 +257,20 @@ def simplex_id(verts) -> int:
     return int(_load().orc_simplex_id(v, v.shape[0]))
 
 
-def rips_filtration(pts, eps, maxdim: int = 2, nthreads: int = 0):
-    """Sorted (value float32 [S], id int64 [S]) of the Rips filtration."""
+def rips_filtration(pts, eps, maxdim: int = 2, nthreads: int = 0, cap: int = 0):
+    """Sorted (value float32 [S], id int64 [S]) of the Rips filtration.
+    cap > 0: one pass when the count fits."""
     lib = _load()
     pts = _f32(pts)
     n, d = pts.shape
-    cnt = lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, None, None, 0)
+    if cap > 0:
+        val = np.empty(cap, np.float32)
+        ids = np.empty(cap, np.int64)
+        cnt = lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, _ptr(val), _ptr(ids), cap)
+        if cnt <= cap:
+            return val[:cnt].copy(), ids[:cnt].copy()
+    else:
+        cnt = lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, None, None, 0)
     val = np.empty(cnt, np.float32)
     ids = np.empty(cnt, np.int64)
     lib.orc_rips_filtration(pts, n, d, np.float32(eps), maxdim, nthreads, _ptr(val), _ptr(ids), cnt)
