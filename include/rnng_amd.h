@@ -172,6 +172,27 @@ int rnng_l2_eps_bipartite_f32(const float *a, int64_t na, const float *b, int64_
                               int32_t d, float eps, int32_t mode, int64_t *ij, float *w,
                               int64_t cap, int64_t *needed, rnng_stats *stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rnng_l2_eps_pairs_f32's self join, computed by its PLAIN
+ * path's kernels, on device-resident buffers and with no allocation.
+ * dev_points holds n x d floats (row-major) in device memory of the library's
+ * device; the first *needed edges go to dev_ij[2e] = i, dev_ij[2e+1] = j and
+ * dev_w[e] (device memory; either may be NULL), sorted by (i, j).  Every
+ * intermediate lives in a fixed array of RNNG_NOALLOC_SCRATCH_BYTES in device
+ * memory that each GPU holds until the process ends; one call handles at most
+ * RNNG_NOALLOC_MAX_EDGES edges: a larger count returns RNNG_ERR_UNSUPPORTED
+ * with *needed set, cap < *needed returns RNNG_ERR_CAPACITY (nothing written).
+ * Every call on a GPU shares that array; the library serialises its calls, and
+ * a call returns when its outputs are complete.  Pointers that are not device
+ * memory of the library's device are RNNG_ERR_ARG.  stats (may be NULL):
+ * pair_evals and edges. */
+#define RNNG_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define RNNG_NOALLOC_MAX_EDGES (1ll << 21)
+int rnng_l2_eps_pairs_f32_noalloc(const float *dev_points, int64_t n, int32_t d, float eps,
+                                  int64_t *dev_ij, float *dev_w, int64_t cap, int64_t *needed,
+                                  rnng_stats *stats);
+
 /* Initial sizes (in edges and in filter candidates) of the buffers of the
  * next builds; 0 restores the library default.  Undersized buffers are safe:
  * the kernels keep counting past the end, and the call grows the buffer to

@@ -88,6 +88,8 @@ _SIGS = {
     "rnng_l2_eps_bipartite_f32": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, C.c_int32, C.c_float,
                                             C.c_int32, _vp, _vp, C.c_int64, _i64p,
                                             C.POINTER(Stats)]),
+    "rnng_l2_eps_pairs_f32_noalloc": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_float, _vp, _vp, C.c_int64,
+                                                _i64p, C.POINTER(Stats)]),
     "rnng_set_initial_capacity": (C.c_int, [C.c_int64, C.c_int64]),
     "rnng_rips_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_filtration_size": (C.c_int, [_vp, _i64p, _vp]),

@@ -69,6 +69,16 @@ int compact_edges(const int2* ij, const float* w, const uint8_t* flag, int64_t n
 int l2_graph_build(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, float eps,
                    int32_t mode, rnng_graph_s* g, rnng_stats* st);
 
+// The plain kernel over the upper triangle of one point set (i < j): writes the
+// first `cap` edges as (i, j) and float distance, counts every edge in
+// *counter (which the caller zeroes).  Enqueued on the library stream.
+int launch_plain_self(const float* X, int n, int d, float T, int2* ij, float* w, unsigned long long cap,
+                      unsigned long long* counter);
+
+// Per-device address of the no-allocation entry point's static scratch array
+// (noalloc.hip); called by device initialisation with that device current.
+int noalloc_attach(int device);
+
 // Uniform-grid self join for d <= 3 (grid.hip); eps must be finite.
 int grid_graph_build(const rnng_dataset_s* A, float eps, rnng_graph_s* g, rnng_stats* st);
 

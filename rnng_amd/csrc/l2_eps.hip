@@ -2110,6 +2110,20 @@ int check_dataset(const rnng_dataset_s* ds) {
 
 }  // namespace
 
+int launch_plain_self(const float* X, int n, int d, float T, int2* ij, float* w, unsigned long long cap,
+                      unsigned long long* counter) {
+    if (n <= 1) return RNNG_OK;
+    const int nb = (n + PB - 1) / PB;
+    const long long nblocks = (long long)nb * (nb + 1) / 2;
+    const long long chunk = 1ll << 23;   // at most 2^31 work-items per dispatch
+    for (long long b0 = 0; b0 < nblocks; b0 += chunk) {
+        const long long nbk = std::min(chunk, nblocks - b0);
+        k_l2_plain<<<(unsigned)nbk, 256, 0, stream()>>>(X, n, X, n, d, T, 1, nb, nb, b0, ij, w, cap, counter);
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    return RNNG_OK;
+}
+
 int filter_dispatch(const FilterArgs& a, int KS, int variant) {
     if (variant < 0 || variant >= kNumVariants) {
         set_error("unknown filter variant");

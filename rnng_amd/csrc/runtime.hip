@@ -16,6 +16,8 @@ hipStream_t g_stream = nullptr;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+int noalloc_attach(int device);
+
 const char* last_error() { return g_last_error.c_str(); }
 
 static int init_locked(int device) {
@@ -37,6 +39,8 @@ static int init_locked(int device) {
         return RNNG_ERR_NODEV;
     }
     RNNG_HIP_TRY(hipSetDevice(device));
+    // the no-allocation entry point's scratch array: its address on this device
+    RNNG_TRY(noalloc_attach(device));
     if (g_stream) {
         (void)hipStreamDestroy(g_stream);
         g_stream = nullptr;
