@@ -316,10 +316,13 @@ int rnng_voronoi_graph_build(rnng_voronoi_t v, float eps, int32_t mode, rnng_gra
  * fastest. */
 int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotune);
 
-/* Filter configuration in use (variant: 1, 3 or 2 = 2, 3 or 4 row tiles per
- * wave with LDS-staged columns, 8 = direct-to-LDS group loads; xcd_order: the
- * XCD-ordered work list) and the autotuner's timings of the 8 (variant x
- * order) candidates in ms (config_ms may be NULL; zeros if not tuned). */
+/* Filter configuration in use (the BF16 variant of rnng_set_filter_kernels:
+ * the autotuner times 17 = twelve waves sharing two LDS stages of 8 column
+ * tiles, 13 / 14 = four waves with two stages of 2 / 3 tiles, 12 = four waves
+ * with three stages of 2 tiles; xcd_order: the XCD-ordered work list) and the
+ * autotuner's timings of the 8 (variant x order) candidates in ms, in that
+ * variant order, each without and with the XCD order (config_ms may be NULL;
+ * zeros if not tuned; integer data runs the int8 filter untuned). */
 int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
 
 /* Ghosts found by the last rnng_voronoi_graph_build: their count, and the
@@ -401,6 +404,15 @@ int rnng_set_int8_filter(int32_t enable);
  * edges. */
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant);
 int rnng_get_filter_kernels(int32_t *int8_code, int32_t *bf16_variant);
+
+/* The XCD ordering of a filter work list, on the host (no device work):
+ * items holds n work items of 4 int32 (row block, first column tile, end
+ * column tile, flags); out receives *n_out = 8 ceil(n / 8) items in launch
+ * order, workgroup b taking item (b mod 8) ceil(n / 8) + b / 8, so that XCD x
+ * (which the hardware gives every 8th workgroup) runs the x-th contiguous
+ * eighth of the list; positions past the list hold empty items (first tile
+ * not below the end tile) that the filter kernels skip. */
+int rnng_debug_xcd_permute(const int32_t *items, int64_t n, int32_t *out, int64_t cap, int64_t *n_out);
 
 /* Runs one 32x32x32 int8 MFMA (the edit filter's) on caller data: a and b
  * hold 16 int8 per lane (64 lanes), d gets the 16 int32 accumulators of

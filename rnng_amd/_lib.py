@@ -127,6 +127,7 @@ _SIGS = {
     "rnng_get_filter_kernels": (C.c_int, [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_debug_copy": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
+    "rnng_debug_xcd_permute": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _i64p]),
 }
 
 _lib = None

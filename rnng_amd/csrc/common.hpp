@@ -59,6 +59,17 @@ struct LocalBuf : DevBuf {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Work items of the filter kernels are int4 (row block, first column tile,
+// end column tile, flags).  An item whose first tile is not below its end
+// covers nothing: the XCD-ordered lists are padded with such items.  Every
+// filter kernel reads its item through this helper and, when it returns
+// false, leaves before any other memory access (the whole workgroup takes the
+// same branch, so no barrier is skipped by part of it).
+__device__ __forceinline__ bool load_work_item(const int4* __restrict__ work, int4& wi) {
+    wi = work[blockIdx.x];
+    return wi.y < wi.z;
+}
+
 // Exclusive prefix of per-lane counts over the 64-lane wave and one atomic
 // reservation per wave.  Every lane of the wave must call it (converged).
 // Returns the global base for this lane's first item.

@@ -180,8 +180,8 @@ __global__ __launch_bounds__(256) void k_edit_filter(
     __shared__ int srl[4][64];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    const int4 wi = work[blockIdx.x];
-    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves
+    int4 wi;
+    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6, lane = tid & 63;

@@ -486,10 +486,8 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter(
     __shared__ int2 stage[LSTAGE];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    const int4 wi = work[blockIdx.x];
-    // Empty items (padding of the XCD-ordered list) touch no memory: the
-    // whole workgroup leaves before any load, so no barrier is skipped.
-    if (wi.y >= wi.z) return;
+    int4 wi;
+    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
@@ -767,8 +765,8 @@ __global__ __launch_bounds__(256) void k_l2_mfma_filter_g(
     __shared__ int2 stage[LSTAGE];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    const int4 wi = work[blockIdx.x];
-    if (wi.y >= wi.z) return;   // empty item: no memory is touched
+    int4 wi;
+    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     const int lane = tid & 63;
@@ -1134,8 +1132,8 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
     __shared__ int2 stage[LSTAGE];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    const int4 wi = work[blockIdx.x];
-    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves before any access
+    int4 wi;
+    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
@@ -1277,8 +1275,8 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     __shared__ int2 stage[STG];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    const int4 wi = work[blockIdx.x];
-    if (wi.y >= wi.z) return;   // padding item: the whole workgroup leaves before any access
+    int4 wi;
+    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
     if (tid == 0) scount = 0u;
     const int wave = tid >> 6;
@@ -2684,6 +2682,29 @@ int rnng_set_initial_capacity(int64_t edges, int64_t candidates) {
     g_cand_hint = candidates;
     g_ws.cand.release();
     g_ws.cand_cap = 0;
+    return RNNG_OK;
+}
+
+int rnng_debug_xcd_permute(const int32_t* items, int64_t n, int32_t* out, int64_t cap, int64_t* n_out) {
+    if (n < 0 || cap < 0 || !n_out || (n > 0 && !items) || (cap > 0 && !out)) {
+        set_error("rnng_debug_xcd_permute: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    std::vector<int4> list((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        list[(size_t)i] = make_int4(items[4 * i], items[4 * i + 1], items[4 * i + 2], items[4 * i + 3]);
+    xcd_permute(list);
+    *n_out = (int64_t)list.size();
+    if ((int64_t)list.size() > cap) {
+        set_error("rnng_debug_xcd_permute: output capacity too small");
+        return RNNG_ERR_CAPACITY;
+    }
+    for (size_t i = 0; i < list.size(); ++i) {
+        out[4 * i] = list[i].x;
+        out[4 * i + 1] = list[i].y;
+        out[4 * i + 2] = list[i].z;
+        out[4 * i + 3] = list[i].w;
+    }
     return RNNG_OK;
 }
 

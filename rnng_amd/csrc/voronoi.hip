@@ -1008,6 +1008,11 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             pred_cand += std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
             pred_edges += std::exp2(24.0 * std::min(std::max((double)y[2 * a + 1], 0.0), 1.5)) - 1.0;
         }
+        // a non-finite output (NaN passes the clamps) sizes nothing: count-then-fill
+        if (!std::isfinite(pred_cand) || !std::isfinite(pred_edges)) {
+            model = nullptr;
+            pred_cand = pred_edges = 0.0;
+        }
     }
 
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
@@ -1109,7 +1114,9 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     // With the model: one pass into buffers sized by its estimate (+50 %), a
     // second only if that was short.  Without: count-then-fill, a counting
     // pass (capacity 0) and a filling pass at the exact size.
-    unsigned long long cap = model ? (unsigned long long)(pred_cand * 1.5) + 4096 : 0ull;
+    // (never above the pairs the work list covers, which bound the candidates)
+    unsigned long long cap =
+        model ? (unsigned long long)std::min(pred_cand * 1.5 + 4096.0, (double)std::max<int64_t>(covered, 1)) : 0ull;
     unsigned long long ncand = 0;
     int passes = 0;
     bool graphed = false;
@@ -1156,10 +1163,14 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         RNNG_HIP_TRY(hipStreamSynchronize(stream()));
         return RNNG_OK;
     };
+    // buffers the estimate cannot get: the count-then-fill passes below
+    if (model && cap > 0 && reserve_fill(cap) != RNNG_OK) {
+        (void)hipGetLastError();
+        cap = 0;
+    }
     if (model && cap > 0) {
         // one pass sized by the model; captured once and replayed while the
         // buffers, the work list and eps are unchanged
-        RNNG_TRY(reserve_fill(cap));
         uint32_t eb, tb;
         std::memcpy(&eb, &eps, 4);
         std::memcpy(&tb, &T, 4);

@@ -118,3 +118,28 @@ def test_shipped_model_sizes_held_out_runs():
                     f"predicted_edges={s1.predicted_edges} passes={s1.filter_passes}")
         saved += s1.filter_passes == 1
     assert saved >= 4
+
+
+def test_wild_model_outputs_keep_the_edge_set():
+    """A model whose outputs are NaN sizes nothing (count-then-fill, two
+    passes); one whose outputs sit at the clamp (2^36 candidates per cell) is
+    capped by the pairs the work list covers (one pass).  Either way the edges
+    equal the plain kernel's."""
+    from rnng_amd import l2
+    cm = _cm()
+    rng = np.random.default_rng(5)
+    pts = (rng.standard_normal((6000, 24)) + 4 * rng.integers(0, 6, size=(6000, 1))).astype(np.float32)
+    eps = np.float32(2.2)
+    ij_ref, w_ref = l2.eps_pairs(pts, eps, "plain")
+    nan_model = np.full(cm.NPARAMS, np.nan, np.float32)
+    huge_model = np.zeros(cm.NPARAMS, np.float32)
+    huge_model[-2:] = 10.0   # output biases: every cell at the clamp
+    for name, model, passes in (("nan", nan_model, 2), ("huge", huge_model, 1)):
+        flush_print(f"cost model outputs: {name}")
+        ij, w, st = _build(pts, eps, 24, 3, model)
+        assert np.array_equal(ij, ij_ref) and w.tobytes() == w_ref.tobytes(), name
+        assert st.filter_passes == passes, (name, st.filter_passes)
+        if name == "nan":
+            assert st.predicted_candidates == 0
+        else:
+            assert 0 < st.predicted_candidates and st.refills == 0

@@ -83,3 +83,39 @@ def test_generators_are_deterministic():
     assert lens.min() >= 80 and lens.max() <= 170
     r, eps = synth.points_rips(n=1000)
     assert r.shape == (1000, 3) and eps == np.float32(0.08)
+
+
+def test_xcd_permute_pads_with_empty_items_and_keeps_every_item():
+    """The XCD-ordered work list (host code, no device): every input item once,
+    the length a multiple of 8, every padding item empty (first column tile
+    not below the end tile, which each filter kernel skips before touching
+    memory), and XCD x's workgroups (every 8th, from x) a contiguous run of the
+    input in order."""
+    from rnng_amd import _lib
+    L = _lib.lib()
+    rng = np.random.default_rng(3)
+    for n in list(range(0, 20)) + [63, 64, 65, 1000, 4097]:
+        rb = rng.integers(0, 50, n)
+        lo = rng.integers(0, 100, n)
+        items = np.stack([rb, lo, lo + rng.integers(1, 9, n), np.zeros(n, np.int64)], 1).astype(np.int32)
+        items = np.ascontiguousarray(items)
+        cap = 8 * ((n + 7) // 8)
+        out = np.full((max(cap, 1), 4), -9, np.int32)
+        n_out = C.c_int64()
+        assert L.rnng_debug_xcd_permute(items.ctypes.data_as(C.c_void_p), n, out.ctypes.data_as(C.c_void_p), cap,
+                                        C.byref(n_out)) == 0
+        m = n_out.value
+        assert m == cap and m % 8 == 0
+        out = out[:m]
+        real = out[:, 1] < out[:, 2]
+        assert int(real.sum()) == n
+        assert (out[~real, 1] >= out[~real, 2]).all()
+        assert sorted(map(tuple, out[real].tolist())) == sorted(map(tuple, items.tolist()))
+        per = m // 8
+        for x in range(8):
+            share = out[x::8]
+            share = share[share[:, 1] < share[:, 2]]
+            assert np.array_equal(share, items[x * per: x * per + share.shape[0]])
+        if n:
+            assert L.rnng_debug_xcd_permute(items.ctypes.data_as(C.c_void_p), n, out.ctypes.data_as(C.c_void_p),
+                                            m - 1, C.byref(n_out)) == _lib.RNNG_ERR_CAPACITY

@@ -1,0 +1,127 @@
+"""The oracle against the reference itself: the reference's three drivers,
+built from their sources (make -C oracle ref) and run under MPI with 8 ranks
+by tests/golden/make_ref_fixtures.py, wrote output files whose SHA-256 digests
+are frozen in tests/golden/ref_outputs.json together with the digests of the
+input files they read.  Here the oracle's answer on the same seeded inputs,
+formatted as the reference's writers format it (include/graph.hpp:20-57 after
+redistribute_edges, include/simplex.hpp:175-192), must give the same bytes;
+at full size the sorted upper-triangle pairs must give the same digest as the
+oracle's fixture.  The systolic driver's ring of remote blocks and the
+landmark driver's ghost queries run the reference's cover tree on non-member
+queries, so these equalities also pin the cover tree's float pruning on these
+inputs."""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+REF = json.load(open(os.path.join(GOLD, "ref_outputs.json")))
+
+
+def _sha(b: bytes) -> str:
+    return hashlib.sha256(b).hexdigest()
+
+
+def _fvecs_bytes(X):
+    X = np.ascontiguousarray(X, np.float32)
+    rec = np.empty((X.shape[0], X.shape[1] + 1), np.float32)
+    rec[:, 0] = np.array([X.shape[1]], np.int32).view(np.float32)[0]
+    rec[:, 1:] = X
+    return rec.tobytes()
+
+
+def _seqs_bytes(atoms, offs):
+    return b"".join(atoms[offs[i]:offs[i + 1]].tobytes() + b"\n" for i in range(offs.shape[0] - 1))
+
+
+def _graph_text(n, ij, w):
+    """One rank after redistribute_edges: both orientations and a zero loop
+    per vertex, sorted by (u, v), weights with setprecision(6)."""
+    u = np.concatenate([ij[:, 0], ij[:, 1], np.arange(n)])
+    v = np.concatenate([ij[:, 1], ij[:, 0], np.arange(n)])
+    ww = np.concatenate([w, w, np.zeros(n, np.float32)])
+    o = np.lexsort((v, u))
+    lines = [f"% {n} {n} {o.size}"]
+    lines += ["%d %d %s" % (a, b, "%.6g" % float(c)) for a, b, c in zip(u[o], v[o], ww[o])]
+    return ("\n".join(lines) + "\n").encode()
+
+
+def _verts(sid, n):
+    """Vertices of a simplex id (combinatorial number system), ascending."""
+    dim = (sid >> 60) & 0xF
+    uid = sid & ((1 << 60) - 1)
+    out = []
+    for i in range(dim, -1, -1):
+        lo, hi = 0, n
+        while lo < hi:
+            mid = (lo + hi) // 2
+            if math.comb(mid, i + 1) > uid:
+                hi = mid
+            else:
+                lo = mid + 1
+        out.append(hi - 1)
+        uid -= math.comb(hi - 1, i + 1)
+    return sorted(out)
+
+
+def test_gmm_small_oracle_equals_reference_files(orc):
+    from rnng_amd import synth
+    pts, eps = synth.points_gmm_small()
+    for key in ("systolic_gmm_small_10kx16", "landmark_gmm_small_10kx16"):
+        assert _sha(_fvecs_bytes(pts)) == REF[key]["input_sha256"]
+    ij, w = orc.l2_eps_pairs(pts, eps)
+    out = _sha(_graph_text(pts.shape[0], ij, w))
+    assert out == REF["systolic_gmm_small_10kx16"]["output_sha256"]
+    assert out == REF["landmark_gmm_small_10kx16"]["output_sha256"]
+
+
+def test_edit_oracle_equals_reference_files(orc):
+    from rnng_amd import synth
+    atoms, offs = synth.dna_mutants(n_seeds=6, per_seed=100)
+    assert _sha(_seqs_bytes(atoms, offs)) == REF["systolic_edit_600"]["input_sha256"]
+    ij, d = orc.edit_eps_pairs(atoms, offs, 8)
+    out = _sha(_graph_text(offs.shape[0] - 1, ij, d.astype(np.float32)))
+    assert out == REF["systolic_edit_600"]["output_sha256"]
+    assert out == REF["landmark_edit_600_m3"]["output_sha256"]
+    # With 24 landmarks two centers are the same string; the earlier one's
+    # cell is empty (ties go to the later center), and the reference's ghost
+    # pass numbers centers over the non-empty cells only
+    # (include/point.hpp:812-819) but indexes every cell with that number
+    # (:903-909): its landmark output then misses edges.  The library's cell
+    # path keeps the all-pairs edge set (the systolic output) for any centers.
+    lm = REF["landmark_edit_600"]
+    assert lm["edges_undirected"] < REF["systolic_edit_600"]["edges_undirected"] == ij.shape[0]
+
+
+def test_rips_oracle_equals_reference_files(orc):
+    from rnng_amd import synth
+    for n, eps, dim, key, ids in ((5000, 0.3, 2, "rips_5000_d2", True), (1500, 0.35, 3, "rips_1500_d3", False)):
+        pts, _ = synth.points_rips(n=n)
+        assert _sha(_fvecs_bytes(pts)) == REF[key]["input_sha256"]
+        val, sid = orc.rips_filtration(pts, np.float32(eps), dim)
+        if ids:
+            lines = ["%f\t%d" % (float(v), int(i)) for v, i in zip(val, sid)]
+        else:
+            lines = ["%f\t<%s>" % (float(v), ",".join(map(str, _verts(int(i), n)))) for v, i in zip(val, sid)]
+        assert len(lines) == REF[key]["simplices"]
+        assert _sha(("\n".join(lines) + "\n").encode()) == REF[key]["output_sha256"]
+
+
+@pytest.mark.parametrize("ref_key,orc_file", [("systolic_gmm_int_1Mx128", "full_gmm_int_1Mx128.json"),
+                                              ("systolic_edit_40k", "full_edit_40k.json")])
+def test_full_size_reference_pairs_equal_oracle_digests(ref_key, orc_file):
+    """Two independent programs at full size: the reference's systolic driver
+    (8 MPI ranks, cover trees) and the oracle's brute force."""
+    if ref_key not in REF or not os.path.exists(os.path.join(GOLD, orc_file)):
+        pytest.skip(f"{ref_key} / {orc_file} not generated")
+    o = json.load(open(os.path.join(GOLD, orc_file)))
+    r = REF[ref_key]
+    assert r["edges_undirected"] == o["edges"]
+    assert r["sha256_pairs_int64"] == o["sha256_ij_int64"]
+    n = o.get("n", o.get("strings"))
+    assert r["header"] == f"% {n} {n} {2 * o['edges'] + n}"
