@@ -49,6 +49,8 @@ public:
         edges.swap(all);
     }
 
+    void redistribute_edges(SingleComm) { redistribute_edges(); }
+
     // "% n n m" then "u v w" per edge, w as %.6f (include/graph.hpp:1-18).
     void write_file(const char* fname) const {
         FILE* f = std::fopen(fname, "w");
@@ -81,6 +83,9 @@ public:
         std::fwrite(s.data(), 1, s.size(), f);
         std::fclose(f);
     }
+
+    // The communicator form the drivers call (include/graph.hpp:20-57).
+    void write_file(const char* fname, SingleComm) const { write_file_stream(fname); }
 
     Index num_edges() const { return (Index)edges.size(); }
     const EdgeVector& get_edges() const { return edges; }

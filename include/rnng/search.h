@@ -1,10 +1,13 @@
 // Epsilon-range queries behind the reference's search API (include/search.h):
 // CoverTree's build / radius_query / radius_query_batched /
-// has_radius_neighbor and BruteForce::radius_query keep their signatures and
-// their functor contract (called once per emitted pair, before the call
-// returns) but run as GPU epsilon joins: no tree is built.  L2Distance<float>
-// runs the BF16 MFMA filter with the exact FP32 recheck, EditDistance<char>
-// the exact integer edit-distance kernels.
+// has_radius_neighbor and BruteForce::radius_query keep their signatures,
+// return values (the number of pairs found) and functor contract (called once
+// per found pair, before the call returns), but run as GPU epsilon joins: no
+// tree is built.  L2Distance<float> runs the MFMA filter with its exact
+// recheck (or the uniform grid for d <= 3), EditDistance<char> the exact
+// edit-distance kernels.  A Voronoi cell's queries against itself and its
+// ghosts at the radius its diagram was built for come from that one batched
+// build.
 #ifndef RNNG_HOST_SEARCH_H_
 #define RNNG_HOST_SEARCH_H_
 
@@ -12,66 +15,84 @@
 
 namespace rnng_host {
 
-struct GraphOwner {
-    rnng_graph_t h = nullptr;
-    ~GraphOwner() {
-        if (h) rnng_graph_destroy(h);
+// One device buffer per thread for single float queries, overwritten by each
+// query: no device allocation per query once it exists for the dimension.
+// Never freed (it lives until the process ends, past the runtime's teardown).
+struct QueryBuffer {
+    rnng_dataset_t h = nullptr;
+    int32_t d = 0;
+    rnng_dataset_t set(const float* x, int32_t dim) {
+        if (h && d == dim) {
+            check(rnng_dataset_upload_f32(h, x, 1), "dataset_upload_f32");
+            return h;
+        }
+        if (h) rnng_dataset_destroy(h);
+        h = nullptr;
+        check(rnng_dataset_create_f32(x, 1, dim, &h), "dataset_create_f32");
+        d = dim;
+        return h;
     }
 };
-
-// (i, j, w) of a device graph, sorted by (i, j).
-inline void graph_edges(rnng_graph_t g, std::vector<int64_t>& ij, std::vector<float>& w) {
-    int64_t ne = 0;
-    check(rnng_graph_num_edges(g, &ne), "graph_num_edges");
-    ij.resize((size_t)(2 * ne));
-    w.resize((size_t)ne);
-    check(rnng_graph_copy(g, ij.data(), w.data(), ne, 1), "graph_copy");
+inline QueryBuffer& query_buffer() {
+    static thread_local QueryBuffer* q = new QueryBuffer();
+    return *q;
 }
 
-// Pairs (i in A, j in B) within radius; self = (A is B): pairs i < j only.
+// Pairs (i, j), i < j, of one container within radius.
 template <class Atom>
-void eps_join(const PointContainer<Atom>& a, const PointContainer<Atom>& b, bool self, Real radius,
-              std::vector<int64_t>& ij, std::vector<float>& w) {
+void self_join(const PointContainer<Atom>& a, Real radius, std::vector<int64_t>& ij, std::vector<float>& w) {
     GraphOwner g;
     if constexpr (std::is_same<Atom, float>::value) {
         // d <= 3: uniform grid (same edges, pairs of adjacent cells only)
-        const int32_t mode = a.num_points() > 0 && a[0].size() <= 3 ? RNNG_MODE_GRID : RNNG_MODE_MFMA;
-        if (self) check(rnng_graph_build(a.device_points(), radius, mode, &g.h, nullptr), "graph_build");
-        else
-            check(rnng_graph_build_bipartite(a.device_points(), b.device_points(), radius, RNNG_MODE_MFMA, &g.h,
-                                             nullptr),
-                  "graph_build_bipartite");
-        graph_edges(g.h, ij, w);
+        const int32_t mode = a.num_points() > 0 && a.dim() <= 3 ? RNNG_MODE_GRID : RNNG_MODE_MFMA;
+        check(rnng_graph_build(a.device_points(), radius, mode, &g.h, nullptr), "graph_build");
     } else {
-        if (self) {
-            check(rnng_edit_graph_build(a.device_strings(), radius, &g.h, nullptr), "edit_graph_build");
-            graph_edges(g.h, ij, w);
-            return;
-        }
-        // cross pairs of the self join over A followed by B
-        std::vector<char> atoms(a.atoms());
-        atoms.insert(atoms.end(), b.atoms().begin(), b.atoms().end());
-        IndexVector offs(a.atom_offsets());
-        for (size_t k = 1; k < b.atom_offsets().size(); ++k) offs.push_back(a.num_atoms() + b.atom_offsets()[k]);
-        const int64_t na = a.num_points(), n = na + b.num_points();
-        StringsHandle s;
-        check(rnng_strings_create(atoms.data(), offs.data(), n, &s.h), "strings_create");
-        check(rnng_edit_graph_build(s.h, radius, &g.h, nullptr), "edit_graph_build");
-        std::vector<int64_t> all;
-        std::vector<float> aw;
-        graph_edges(g.h, all, aw);
-        ij.clear();
-        w.clear();
-        for (size_t e = 0; e < aw.size(); ++e) {
-            int64_t i = all[2 * e], j = all[2 * e + 1];
-            if (i > j) std::swap(i, j);
-            if (i < na && j >= na) {
-                ij.push_back(i);
-                ij.push_back(j - na);
-                w.push_back(aw[e]);
-            }
-        }
+        check(rnng_edit_graph_build(a.device_strings(), radius, &g.h, nullptr), "edit_graph_build");
     }
+    graph_edges(g.h, ij, w);
+}
+
+// Pairs (i in a, j in b) within radius.
+template <class Atom>
+void cross_join(const PointContainer<Atom>& a, const PointContainer<Atom>& b, Real radius, std::vector<int64_t>& ij,
+                std::vector<float>& w) {
+    ij.clear();
+    w.clear();
+    if (a.num_points() == 0 || b.num_points() == 0) return;
+    GraphOwner g;
+    if constexpr (std::is_same<Atom, float>::value) {
+        if (a.dim() != b.dim()) fail("query points of another dimension");
+        check(rnng_graph_build_bipartite(a.device_points(), b.device_points(), radius, RNNG_MODE_MFMA, &g.h, nullptr),
+              "graph_build_bipartite");
+    } else {
+        check(rnng_edit_graph_build_bipartite(a.device_strings(), b.device_strings(), radius, &g.h, nullptr),
+              "edit_graph_build_bipartite");
+    }
+    graph_edges(g.h, ij, w);
+}
+
+// Points of a within radius of one query point: (i, 0) pairs.
+template <class Atom>
+void query_join(const PointContainer<Atom>& a, const Point<Atom>& q, Real radius, std::vector<int64_t>& ij,
+                std::vector<float>& w) {
+    ij.clear();
+    w.clear();
+    if (a.num_points() == 0) return;
+    GraphOwner g;
+    if constexpr (std::is_same<Atom, float>::value) {
+        if (q.size() != a.dim()) fail("query point of another dimension");
+        rnng_dataset_t qd = query_buffer().set(q.begin(), (int32_t)q.size());
+        // one column: the plain kernel's single launch decides every pair
+        check(rnng_graph_build_bipartite(a.device_points(), qd, radius, RNNG_MODE_PLAIN, &g.h, nullptr),
+              "graph_build_bipartite");
+    } else {
+        const int64_t offs[2] = {0, q.size()};
+        StringsHandle s;
+        check(rnng_strings_create(q.begin(), offs, 1, &s.h), "strings_create");
+        check(rnng_edit_graph_build_bipartite(a.device_strings(), s.h, radius, &g.h, nullptr),
+              "edit_graph_build_bipartite");
+    }
+    graph_edges(g.h, ij, w);
 }
 
 template <class Distance, class Atom>
@@ -85,52 +106,81 @@ constexpr void require_gpu_metric() {
 
 class CoverTree {
 public:
-    CoverTree() : cover(1.5), leaf_size(10) {}
-    CoverTree(Real cover, Index leaf_size) : cover(cover), leaf_size(leaf_size) {}
+    CoverTree(Real cover = 1.5, Index leaf_size = 1) : cover(cover), leaf_size(leaf_size) {}
 
-    // Keeps the point set; the queries run against it on the GPU.
+    // Keeps nothing: the queries run against the point set on the GPU.
     template <class Atom, class Distance>
     void build(const PointContainer<Atom>&, Distance&) {
         rnng_host::require_gpu_metric<Distance, Atom>();
     }
 
-    // functor(points[i], query, d) for every point within radius of query
-    // (include/search.hpp:223-264).
+    // functor(points[i], query, d) for every point within radius of query;
+    // returns their number (include/search.hpp:223-264).
     template <class Atom, class Distance, class Functor>
-    void radius_query(const PointContainer<Atom>& points, Distance& distance, const Point<Atom>& query, Real radius,
-                      const Functor& functor) const {
-        PointContainer<Atom> batch;
-        batch.push_back(query);
-        radius_query_batched(points, distance, batch, radius,
-                             [&](const Point<Atom>& p, const Point<Atom>&, Real d) { functor(p, query, d); });
-    }
-
-    // functor(points[i], batch[j], d) for every pair within radius
-    // (include/search.hpp:266-340); when batch is the point set itself this
-    // includes both orders of every pair and each point with itself.
-    template <class Atom, class Distance, class Functor>
-    void radius_query_batched(const PointContainer<Atom>& points, Distance&, const PointContainer<Atom>& batch,
-                              Real radius, const Functor& functor) const {
+    Index radius_query(const PointContainer<Atom>& points, Distance&, const Point<Atom>& query, Real radius,
+                       const Functor& functor) const {
         rnng_host::require_gpu_metric<Distance, Atom>();
-        const bool self = &points == &batch;
         std::vector<int64_t> ij;
         std::vector<float> w;
-        rnng_host::eps_join(points, batch, self, radius, ij, w);
-        for (size_t e = 0; e < w.size(); ++e) {
-            functor(points[ij[2 * e]], batch[ij[2 * e + 1]], w[e]);
-            if (self) functor(points[ij[2 * e + 1]], batch[ij[2 * e]], w[e]);
-        }
-        if (self && radius >= 0)
+        rnng_host::query_join(points, query, radius, ij, w);
+        for (size_t e = 0; e < w.size(); ++e) functor(points[ij[2 * e]], query, w[e]);
+        return (Index)w.size();
+    }
+
+    // functor(points[i], batch[j], d) for every pair within radius; returns
+    // their number (include/search.hpp:266-340).  When batch is the point set
+    // itself that includes both orders of every pair and each point with
+    // itself at distance 0.
+    template <class Atom, class Distance, class Functor>
+    Index radius_query_batched(const PointContainer<Atom>& points, Distance&, const PointContainer<Atom>& batch,
+                               Real radius, const Functor& functor) const {
+        rnng_host::require_gpu_metric<Distance, Atom>();
+        std::vector<int64_t> ij;
+        std::vector<float> w;
+        if (&points == &batch) {
+            if (!(radius >= 0)) return 0;
+            rnng_host::self_join(points, radius, ij, w);
+            for (size_t e = 0; e < w.size(); ++e) {
+                functor(points[ij[2 * e]], batch[ij[2 * e + 1]], w[e]);
+                functor(points[ij[2 * e + 1]], batch[ij[2 * e]], w[e]);
+            }
             for (Index i = 0; i < points.num_points(); ++i) functor(points[i], batch[i], Real(0));
+            return (Index)(2 * w.size()) + points.num_points();
+        }
+        rnng_host::cross_join(points, batch, radius, ij, w);
+        for (size_t e = 0; e < w.size(); ++e) functor(points[ij[2 * e]], batch[ij[2 * e + 1]], w[e]);
+        return (Index)w.size();
+    }
+
+    // A Voronoi cell against itself or its ghosts at the radius its diagram's
+    // batched build used: the same pairs, read from that build.
+    template <class Atom, class Distance, class Functor>
+    Index radius_query_batched(const VoronoiCell<Atom>& cell, Distance& distance, const PointContainer<Atom>& batch,
+                               Real radius, const Functor& functor) const {
+        const rnng_host::CellPairs* cp = cell.served_pairs();
+        const PointContainer<Atom>& members = cell;
+        if (!cp || cp->radius != radius || (&batch != &members && &batch != &cell.ghosts()))
+            return radius_query_batched(members, distance, batch, radius, functor);
+        Index found = 0;
+        if (&batch == &members) {
+            for (const auto& [i, j, d] : cp->members) {
+                functor(cell[i], cell[j], d);
+                functor(cell[j], cell[i], d);
+            }
+            for (Index i = 0; i < cell.num_points(); ++i) functor(cell[i], cell[i], Real(0));
+            found = (Index)(2 * cp->members.size()) + cell.num_points();
+        } else {
+            for (const auto& [i, g, d] : cp->ghosts) functor(cell[i], batch[g], d);
+            found = (Index)cp->ghosts.size();
+        }
+        return found;
     }
 
     // Whether some point lies within radius of query (include/search.hpp:342-381).
     template <class Atom, class Distance>
     bool has_radius_neighbor(const PointContainer<Atom>& points, Distance& distance, const Point<Atom>& query,
                              Real radius) const {
-        bool found = false;
-        radius_query(points, distance, query, radius, [&](const Point<Atom>&, const Point<Atom>&, Real) { found = true; });
-        return found;
+        return radius_query(points, distance, query, radius, [](const Point<Atom>&, const Point<Atom>&, Real) {}) > 0;
     }
 
 private:
@@ -142,14 +192,15 @@ private:
 // the CoverTree queries reproduce.
 class BruteForce {
 public:
+    BruteForce() {}
     template <class Atom, class Distance>
     void build(const PointContainer<Atom>&, Distance&) {
         rnng_host::require_gpu_metric<Distance, Atom>();
     }
     template <class Atom, class Distance, class Functor>
-    void radius_query(const PointContainer<Atom>& points, Distance& distance, const Point<Atom>& query, Real radius,
-                      const Functor& functor) const {
-        CoverTree().radius_query(points, distance, query, radius, functor);
+    Index radius_query(const PointContainer<Atom>& points, Distance& distance, const Point<Atom>& query, Real radius,
+                       const Functor& functor) const {
+        return CoverTree().radius_query(points, distance, query, radius, functor);
     }
 };
 

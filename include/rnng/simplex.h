@@ -54,6 +54,23 @@ struct Simplex {
         std::sort(verts.begin(), verts.end());
         return verts;
     }
+    // The facets: each vertex left out in turn (include/simplex.hpp:81-111).
+    void get_facets(std::vector<Simplex>& facets, Index n) const {
+        facets.clear();
+        const IndexVector v = getverts(n);
+        for (size_t i = 0; i < v.size() && v.size() > 1; ++i) {
+            IndexVector f;
+            for (size_t j = 0; j < v.size(); ++j)
+                if (j != i) f.push_back(v[j]);
+            facets.emplace_back(f);
+        }
+    }
+    void get_facet_ids(IndexVector& ids, Index n) const {
+        std::vector<Simplex> facets;
+        get_facets(facets, n);
+        ids.clear();
+        for (const auto& f : facets) ids.push_back(f.getid());
+    }
     std::string get_simplex_repr(Index n) const {
         const IndexVector v = getverts(n);
         std::stringstream ss;
@@ -84,7 +101,7 @@ struct WeightedSimplex : public Simplex {
 inline void merge_and_write_filtration(const char* fname, const std::vector<WeightedSimplex>& simplices,
                                        Index num_vertices, bool use_ids) {
     std::vector<WeightedSimplex> s(simplices);
-    std::sort(s.begin(), s.end());
+    if (!std::is_sorted(s.begin(), s.end())) std::sort(s.begin(), s.end());
     s.erase(std::unique(s.begin(), s.end()), s.end());
     FILE* f = std::fopen(fname, "w");
     if (!f) {
@@ -96,6 +113,12 @@ inline void merge_and_write_filtration(const char* fname, const std::vector<Weig
         else std::fprintf(f, "%f\t%s\n", x.value, x.get_simplex_repr(num_vertices).c_str());
     }
     std::fclose(f);
+}
+
+// The communicator form (rank 0 gathers and writes): one process writes.
+inline void merge_and_write_filtration(const char* fname, const std::vector<WeightedSimplex>& simplices,
+                                       Index num_vertices, bool use_ids, SingleComm) {
+    merge_and_write_filtration(fname, simplices, num_vertices, use_ids);
 }
 
 #endif

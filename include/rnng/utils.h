@@ -33,6 +33,12 @@ template <class Atom> struct EditDistance;
 
 namespace rnng_host {
 
+[[noreturn]] inline void fail(const std::string& msg) {
+    std::fprintf(stderr, "rnng: %s\n", msg.c_str());
+    std::fflush(stderr);
+    std::abort();
+}
+
 inline void check(int rc, const char* what) {
     if (rc != RNNG_OK) {
         std::fprintf(stderr, "rnng: %s failed (%d): %s\n", what, rc, rnng_last_error());
@@ -51,6 +57,13 @@ constexpr bool is_edit() {
 }
 
 }  // namespace rnng_host
+
+// One process's communicator.  The reference passes an MPI_Comm to every call
+// that moves data between ranks (reads, the systolic ring, cell exchange,
+// edge redistribution, the filtration merge); with one process those calls
+// take this token, keep the reference's argument lists and move nothing.
+struct SingleComm {};
+inline constexpr SingleComm COMM_SINGLE{};
 
 // Knuth's selection sampling with the reference's engine and distribution
 // (include/utils.hpp:1-29): `size` ascending ids from [0, range).

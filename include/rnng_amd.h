@@ -93,6 +93,13 @@ int rnng_init(int device);
  * point i is points[i*d .. i*d+d). */
 int rnng_dataset_create_f32(const float *points, int64_t n, int32_t d, rnng_dataset_t *out);
 
+/* Overwrites a dataset created by rnng_dataset_create_f32 (or read from a
+ * file) with n points of its dimension, n no more than it was created with: one
+ * device buffer reused for changing query batches, with no allocation (the
+ * reference reuses its receive buffers the same way, PointContainer::sendrecv,
+ * include/point.hpp:369-417). */
+int rnng_dataset_upload_f32(rnng_dataset_t ds, const float *points, int64_t n);
+
 /* Wrap points that already live in device memory (not copied, not freed). */
 int rnng_dataset_wrap_device_f32(const float *dev_points, int64_t n, int32_t d,
                                  rnng_dataset_t *out);
@@ -220,6 +227,15 @@ int rnng_strings_destroy(rnng_strings_t s);
  * 512 atoms (RNNG_ERR_UNSUPPORTED otherwise). */
 int rnng_edit_graph_build(rnng_strings_t s, float eps, rnng_graph_t *out, rnng_stats *stats);
 
+/* All pairs (i, j), i in a, j in b, whose Levenshtein distance is <= eps, the
+ * weight the distance as a float: the edit-distance form of
+ * rnng_graph_build_bipartite (include/search.hpp:266-340 and :342-381 over
+ * two string sets: cell x ghosts, a single query).  a and b may be the same
+ * set (then every ordered pair, i == j included).  Limits as
+ * rnng_edit_graph_build. */
+int rnng_edit_graph_build_bipartite(rnng_strings_t a, rnng_strings_t b, float eps, rnng_graph_t *out,
+                                    rnng_stats *stats);
+
 /* One-shot host form with the count-then-fill contract of rnng_l2_eps_pairs_f32. */
 int rnng_edit_eps_pairs(const char *atoms, const int64_t *offsets, int64_t n, float eps,
                         int64_t *ij, float *w, int64_t cap, int64_t *needed, rnng_stats *stats);
@@ -250,6 +266,19 @@ int rnng_filtration_size(rnng_filtration_t f, int64_t *n, int64_t *per_dim);
 
 /* Copy the first cap simplices in filtration order (either array may be NULL). */
 int rnng_filtration_copy(rnng_filtration_t f, float *value, int64_t *id, int64_t cap);
+
+/* Relabels the vertices of a filtration built on n points (a Voronoi complex's
+ * cell and ghost points; n must be that number): vertex v becomes ids[v]
+ * (distinct, >= 0), every
+ * simplex id is rebuilt from the relabelled vertices and the filtration is
+ * re-sorted by (value, id) -- the global ids VoronoiComplex::build_filtration
+ * gives its simplices (include/point.hpp:1218-1231).  vertex_flags (n bytes,
+ * may be NULL): simplex_flags (may be NULL; one byte per simplex, in the new
+ * order) receives their OR over each simplex's vertices (the interior flag,
+ * :1163-1175).  Labels past the id limits of rnng_rips_build give
+ * RNNG_ERR_UNSUPPORTED. */
+int rnng_filtration_relabel(rnng_filtration_t f, const int64_t *ids, int64_t n, const uint8_t *vertex_flags,
+                            uint8_t *simplex_flags);
 
 int rnng_filtration_destroy(rnng_filtration_t f);
 

@@ -69,6 +69,7 @@ _SIGS = {
     "rnng_init": (C.c_int, [C.c_int]),
     "rnng_dataset_create_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
     "rnng_dataset_wrap_device_f32": (C.c_int, [_vp, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+    "rnng_dataset_upload_f32": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_dataset_destroy": (C.c_int, [_vp]),
     "rnng_dataset_shape": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
     "rnng_dataset_copy_f32": (C.c_int, [_vp, _vp]),
@@ -94,10 +95,12 @@ _SIGS = {
     "rnng_rips_build": (C.c_int, [_vp, C.c_float, C.c_int32, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_filtration_size": (C.c_int, [_vp, _i64p, _vp]),
     "rnng_filtration_copy": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "rnng_filtration_relabel": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
     "rnng_filtration_destroy": (C.c_int, [_vp]),
     "rnng_strings_create": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(_vp)]),
     "rnng_strings_destroy": (C.c_int, [_vp]),
     "rnng_edit_graph_build": (C.c_int, [_vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),
+    "rnng_edit_graph_build_bipartite": (C.c_int, [_vp, _vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_edit_eps_pairs": (C.c_int, [_vp, _vp, C.c_int64, C.c_float, _vp, _vp, C.c_int64, _i64p,
                                       C.POINTER(Stats)]),
     "rnng_selection_sample": (C.c_int, [C.c_int64, C.c_int64, _vp, C.c_int32]),

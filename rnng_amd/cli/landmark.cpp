@@ -49,37 +49,56 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     double t0 = wall(), t;
-    PointContainer<float> points;
-    const Index n = points.read_file(infile);
-    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)n);
+    // the landmark driver's sequence (drivers/landmark.cpp:70-250) on the class API
+    PointContainer<float> mypoints;
+    L2Distance<float> distance;
+    const Index num_points = mypoints.read_file(infile);
+    if (verbosity >= 1)
+        std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)num_points);
+    if (num_centers > num_points || num_centers < 1) {
+        std::fprintf(stderr, "error: need 1 <= centers <= points (-m)\n");
+        return 1;
+    }
     t = wall();
     IndexVector landmarks;
-    selection_sample(n, std::min(num_centers, n), landmarks, rng_seed);
-    PointContainer<float> centers;
-    centers.indexed_gather(points, landmarks);
-    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] picked centers [centers=%lld]\n", wall() - t, (long long)landmarks.size());
+    selection_sample(num_points, num_centers, landmarks, rng_seed);
+    PointContainer<float> mycenters, centers;
+    mycenters.indexed_gather(mypoints, landmarks);
+    centers.allgather(mycenters, COMM_SINGLE);
+    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] picked centers [centers=%lld]\n", wall() - t, (long long)num_centers);
     t = wall();
-    L2Distance<float> distance;
-    VoronoiDiagram<float> diagram(points, centers, distance);
+    VoronoiDiagram<float> diagram(mypoints, centers, distance);
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] computed point partitioning\n", wall() - t);
     t = wall();
-    std::vector<VoronoiCell<float>> cells;
-    diagram.coalesce_cells(cells);
+    std::vector<VoronoiCell<float>> mycells;
+    diagram.coalesce_cells(mycells, COMM_SINGLE);
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] coalesced voronoi cells\n", wall() - t);
     t = wall();
-    diagram.add_ghost_points_systolic(cells, distance, radius, cover, leaf_size);
+    diagram.add_ghost_points_systolic(mycells, distance, radius, cover, leaf_size, COMM_SINGLE);
     Index ghosts = 0;
-    for (const auto& cell : cells) ghosts += cell.num_ghosts();
+    for (const auto& cell : mycells) ghosts += cell.num_ghosts();
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] added ghost points [ghosts=%lld]\n", wall() - t, (long long)ghosts);
     t = wall();
-    Graph graph = Graph::from_device(diagram.graph_handle(radius), n);
+    using Edge = std::tuple<Index, Index, Real>;
+    std::vector<Edge> myedges;
+    auto functor = [&](const Point<float>& p, const Point<float>& q, Real dist) { myedges.emplace_back(q.id(), p.id(), dist); };
+    for (const VoronoiCell<float>& cell : mycells) {
+        CoverTree tree(cover, leaf_size);
+        tree.build(cell, distance);
+        tree.radius_query_batched(cell, distance, cell, radius, functor);
+        tree.radius_query_batched(cell, distance, cell.ghosts(), radius, functor);
+    }
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] found neighbors\n", wall() - t);
+    t = wall();
+    Graph graph(myedges, num_points);
+    graph.redistribute_edges(COMM_SINGLE);
     if (verbosity >= 1)
-        std::fprintf(stderr, "[time=%.3f] redistributed edges [points=%lld,edges=%lld,density=%.3f]\n", 0.0,
-                     (long long)n, (long long)graph.num_edges(), (graph.num_edges() + 0.0) / (double)std::max<Index>(n, 1));
+        std::fprintf(stderr, "[time=%.3f] redistributed edges [points=%lld,edges=%lld,density=%.3f]\n", wall() - t,
+                     (long long)num_points, (long long)graph.num_edges(),
+                     (graph.num_edges() + 0.0) / (double)std::max<Index>(num_points, 1));
     if (outfile) {
         t = wall();
-        graph.write_file_stream(outfile);
+        graph.write_file(outfile, COMM_SINGLE);
         if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] wrote edges to file '%s'\n", wall() - t, outfile);
     }
     std::fprintf(stderr, "[time=%.3f] complete\n", wall() - t0);

@@ -1,8 +1,9 @@
-// Vietoris-Rips filtration on one GPU: the rips driver's flags and merged
-// output file "<outfile>.txt" (drivers/rips.cpp, include/simplex.hpp:130-196).
-// The union of the per-cell complexes is the Rips complex of the whole point
-// set, built here directly; maxdim up to 3 (tetrahedra for up to 55,111
-// points, where the reference's simplex ids are exact).
+// Vietoris-Rips filtration on one GPU: the rips driver's flags, sequence and
+// output files (drivers/rips.cpp, include/point.hpp:1126-1290,
+// include/simplex.hpp:130-196): landmark cells with ghosts, one complex per
+// cell built on the GPU, the merged "<outfile>.txt"; maxdim up to 3
+// (tetrahedra for up to 55,111 points, where the reference's simplex ids are
+// exact).
 #include "cli_common.h"
 
 static Real radius = -1;
@@ -19,15 +20,15 @@ static bool use_ids = true;
 
 static void usage(const char* prog, int err) {
     std::fprintf(stderr, "Usage: %s [options] -i <points> -r <radius>\n", prog);
-    std::fprintf(stderr, "Options: -m INT   number of centers [%lld] (accepted; one GPU holds every cell)\n", (long long)num_centers);
+    std::fprintf(stderr, "Options: -m INT   number of centers [%lld]\n", (long long)num_centers);
     std::fprintf(stderr, "         -d INT   maximum simplex dimension [%lld] (0..3)\n", (long long)maxdim);
     std::fprintf(stderr, "         -c FLOAT cover tree base [%.2f] (accepted; no tree is built)\n", cover);
     std::fprintf(stderr, "         -l INT   leaf size [%lld] (accepted; no tree is built)\n", (long long)leaf_size);
     std::fprintf(stderr, "         -v INT   verbosity level [%d]\n", verbosity);
     std::fprintf(stderr, "         -D STR   distance metric [%s] (l2)\n", metric);
     std::fprintf(stderr, "         -o FILE  output filtration prefix (writes FILE.txt)\n");
-    std::fprintf(stderr, "         -s INT   random number seed (accepted)\n");
-    std::fprintf(stderr, "         -F       fix number of centers (accepted)\n");
+    std::fprintf(stderr, "         -s INT   random number seed\n");
+    std::fprintf(stderr, "         -F       fix number of centers (one rank: always fixed)\n");
     std::fprintf(stderr, "         -V       write vertex lists instead of simplex ids\n");
     std::fprintf(stderr, "         -h       help message\n");
     std::exit(err);
@@ -57,36 +58,60 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     double t0 = wall(), t;
-    PointContainer<float> points;
-    const Index n = points.read_file(infile);
-    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)n);
-    t = wall();
-    rnng_filtration_t f = nullptr;
-    rnng_host::check(rnng_rips_build(points.device_points(), radius, (int32_t)maxdim, &f, nullptr), "rips_build");
-    int64_t ns = 0, per[4] = {0, 0, 0, 0};
-    rnng_host::check(rnng_filtration_size(f, &ns, per), "filtration_size");
-    std::vector<float> val((size_t)ns);
-    std::vector<int64_t> ids((size_t)ns);
-    rnng_host::check(rnng_filtration_copy(f, val.data(), ids.data(), ns), "filtration_copy");
-    rnng_filtration_destroy(f);
+    // the rips driver's sequence (drivers/rips.cpp:70-250) on the class API
+    PointContainer<float> mypoints;
+    L2Distance<float> distance;
+    const Index num_points = mypoints.read_file(infile);
     if (verbosity >= 1)
-        std::fprintf(stderr, "[time=%.3f] built filtration [vertices=%lld,edges=%lld,triangles=%lld,tetrahedra=%lld]\n",
-                     wall() - t, (long long)per[0], (long long)per[1], (long long)per[2], (long long)per[3]);
+        std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)num_points);
+    if (num_centers > num_points || num_centers < 1) {
+        std::fprintf(stderr, "error: need 1 <= centers <= points (-m)\n");
+        return 1;
+    }
+    t = wall();
+    IndexVector landmarks;
+    selection_sample(num_points, num_centers, landmarks, rng_seed);
+    PointContainer<float> mycenters, centers;
+    mycenters.indexed_gather(mypoints, landmarks);
+    centers.allgather(mycenters, COMM_SINGLE);
+    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] picked centers [centers=%lld]\n", wall() - t, (long long)num_centers);
+    t = wall();
+    VoronoiDiagram<float> diagram(mypoints, centers, distance);
+    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] computed point partitioning\n", wall() - t);
+    t = wall();
+    std::vector<VoronoiCell<float>> mycells;
+    diagram.coalesce_cells(mycells, COMM_SINGLE);
+    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] coalesced voronoi cells\n", wall() - t);
+    t = wall();
+    diagram.add_ghost_points_systolic_rips(mycells, distance, radius, cover, leaf_size, COMM_SINGLE);
+    if (verbosity >= 1) std::printf("[time=%.3f] added ghost points\n", wall() - t);
+    t = wall();
+    std::vector<VoronoiComplex<float>> complexes;
+    for (const VoronoiCell<float>& cell : mycells) {
+        complexes.emplace_back(cell, radius, maxdim);
+        complexes.back().build_filtration(distance, num_points, cover, leaf_size);
+    }
+    if (verbosity >= 1) std::printf("[time=%.3f] built complexes\n", wall() - t);
+    if (outfile) {
+        // per-cell files: the driver passes use_ids where write_filtration_file
+        // takes the vertex count (drivers/rips.cpp:204), so they hold ids
+        t = wall();
+        for (size_t i = 0; i < complexes.size(); ++i) {
+            const std::string s = std::string(outfile) + ".rank0.cell" + std::to_string(i) + ".txt";
+            complexes[i].write_filtration_file(s.c_str(), use_ids);
+        }
+        if (verbosity >= 1) std::printf("[time=%.3f] wrote complex files\n", wall() - t);
+    }
     std::fprintf(stderr, "[time=%.3f] complete\n", wall() - t0);
     if (outfile) {
         t = wall();
-        std::string s = std::string(outfile) + ".txt";
-        // already sorted by (value, id) and unique by id on the device
-        FILE* fo = std::fopen(s.c_str(), "w");
-        if (!fo) {
-            std::fprintf(stderr, "cannot open %s\n", s.c_str());
-            return 1;
-        }
-        for (int64_t k = 0; k < ns; ++k) {
-            if (use_ids) std::fprintf(fo, "%f\t%lld\n", val[(size_t)k], (long long)ids[(size_t)k]);
-            else std::fprintf(fo, "%f\t%s\n", val[(size_t)k], Simplex(ids[(size_t)k]).get_simplex_repr(n).c_str());
-        }
-        std::fclose(fo);
+        std::vector<WeightedSimplex> mysimplices;
+        Index total = 0;
+        for (const auto& c : complexes) total += c.num_simplices();
+        mysimplices.reserve((size_t)total);
+        for (const auto& c : complexes) mysimplices.insert(mysimplices.end(), c.begin(), c.end());
+        const std::string s = std::string(outfile) + ".txt";
+        merge_and_write_filtration(s.c_str(), mysimplices, num_points, use_ids, COMM_SINGLE);
         if (verbosity >= 1) std::printf("[time=%.3f] wrote merged simplices to '%s'\n", wall() - t, s.c_str());
     }
     return 0;

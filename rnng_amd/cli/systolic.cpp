@@ -22,32 +22,41 @@ static void usage(const char* prog, int err) {
     std::exit(err);
 }
 
-template <class Atom>
+// The systolic driver's sequence (drivers/systolic.cpp:70-195) on the class
+// API: read, build the search structure, query the rank's block, build the
+// graph and redistribute it, write.  With one rank the ring has one step, in
+// which the rank queries its own block, so the queries run as one batched self
+// query.
+template <class Atom, class Distance>
 static int run() {
     double t0 = wall(), t;
-    PointContainer<Atom> points;
-    Index n;
-    if constexpr (std::is_same<Atom, float>::value) n = points.read_file(infile);
-    else n = points.read_seqs(infile);
-    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)n);
+    PointContainer<Atom> mypoints;
+    Distance distance;
+    Index num_points;
+    if constexpr (std::is_same<Atom, float>::value) num_points = mypoints.read_file(infile);
+    else num_points = mypoints.read_seqs(infile, COMM_SINGLE);
+    if (verbosity >= 1)
+        std::fprintf(stderr, "[time=%.3f] read input file '%s' [size=%lld]\n", wall() - t0, infile, (long long)num_points);
     t = wall();
-    rnng_host::GraphOwner g;
-    rnng_stats st{};
-    if constexpr (std::is_same<Atom, float>::value) {
-        const int32_t mode = n > 0 && points[0].size() <= 3 ? RNNG_MODE_GRID : RNNG_MODE_MFMA;
-        rnng_host::check(rnng_graph_build(points.device_points(), radius, mode, &g.h, &st), "graph_build");
-    }
-    else
-        rnng_host::check(rnng_edit_graph_build(points.device_strings(), radius, &g.h, &st), "edit_graph_build");
+    CoverTree search(cover, leaf_size);
+    search.build(mypoints, distance);
+    if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] built cover tree\n", wall() - t);
+    t = wall();
+    using Edge = std::tuple<Index, Index, Real>;
+    std::vector<Edge> myedges;
+    auto functor = [&](const Point<Atom>& p, const Point<Atom>& q, Real dist) { myedges.emplace_back(q.id(), p.id(), dist); };
+    search.radius_query_batched(mypoints, distance, mypoints, radius, functor);
     if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] found neighbors\n", wall() - t);
     t = wall();
-    Graph graph = Graph::from_device(g.h, n);
+    Graph graph(myedges, num_points);
+    graph.redistribute_edges(COMM_SINGLE);
     if (verbosity >= 1)
         std::fprintf(stderr, "[time=%.3f] redistributed edges [points=%lld,edges=%lld,density=%.3f]\n", wall() - t,
-                     (long long)n, (long long)graph.num_edges(), (graph.num_edges() + 0.0) / (double)std::max<Index>(n, 1));
+                     (long long)num_points, (long long)graph.num_edges(),
+                     (graph.num_edges() + 0.0) / (double)std::max<Index>(num_points, 1));
     if (outfile) {
         t = wall();
-        graph.write_file_stream(outfile);
+        graph.write_file(outfile, COMM_SINGLE);
         if (verbosity >= 1) std::fprintf(stderr, "[time=%.3f] wrote edges to file '%s'\n", wall() - t, outfile);
     }
     std::fprintf(stderr, "[time=%.3f] complete\n", wall() - t0);
@@ -68,8 +77,8 @@ int main(int argc, char* argv[]) {
         else usage(argv[0], 1);
     }
     if (!infile || radius < 0) usage(argv[0], 1);
-    if (!std::strcmp(metric, "l2")) return run<float>();
-    if (!std::strcmp(metric, "edit")) return run<char>();
+    if (!std::strcmp(metric, "l2")) return run<float, L2Distance<float>>();
+    if (!std::strcmp(metric, "edit")) return run<char, EditDistance<char>>();
     std::fprintf(stderr, "unknown metric '%s'\n", metric);
     return 1;
 }

@@ -37,6 +37,7 @@
 #include "mfma.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <numeric>
 #include <vector>
@@ -55,8 +56,10 @@ struct rnng_strings_s {
     rnng::DevBuf codes;     // uint8 dense alphabet codes, strings in sorted order, 16-byte aligned
     rnng::DevBuf coff;      // int64 [n]: offset of each sorted position's codes
     int planes = 1;         // bits of a code
+    int64_t codes_len = 0;  // bytes of codes (strings, alignment padding and tail)
     int64_t n_pad = 0;
     std::vector<int32_t> len_sorted;
+    bool seen[256] = {false};   // bytes that occur in the set (codes are their ranks)
 };
 
 namespace rnng {
@@ -170,9 +173,13 @@ __global__ void k_debug_mfma_i8(const i32x4_t* __restrict__ a, const i32x4_t* __
 // candidate, and a load per set bit would stall the whole workgroup at the
 // next barrier, so row lengths sit in LDS and column lengths are fetched a
 // tile ahead.
+// Rows and columns may come from two string sets (a bipartite join: rows nr
+// strings of one, columns nc of the other, each in its own length order); a
+// self join passes the same set twice and keeps a < b only.
 __global__ __launch_bounds__(256) void k_edit_filter(
-    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ nrm,
-    const int* __restrict__ slen, const int4* __restrict__ work, int n, int k, int bias,
+    const i32x4_t* __restrict__ xa, const i32x4_t* __restrict__ yb, const int* __restrict__ rnrm,
+    const int* __restrict__ cnrm, const int* __restrict__ rslen, const int* __restrict__ cslen,
+    const int4* __restrict__ work, int nr, int nc, int self_join, int k, int bias,
     int2* __restrict__ cand, unsigned long long cap, unsigned long long* __restrict__ counter) {
     constexpr int NCH = EKS * 64;   // 16-byte fragments per column tile
     __shared__ i32x4_t bs[2][NCH];
@@ -196,12 +203,12 @@ __global__ __launch_bounds__(256) void k_edit_filter(
 #pragma unroll
     for (int r = 0; r < ERT; ++r)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) xr[r][q] = nrm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
-    int yc = nrm[wi.y * 32 + (lane & 31)] + bias;
+        for (int q = 0; q < 16; ++q) xr[r][q] = rnrm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
+    int yc = cnrm[wi.y * 32 + (lane & 31)] + bias;
     // the lengths of the wave's 64 rows sit in LDS (read by the candidate
     // path), each lane keeps the length of its column, fetched a tile ahead
-    srl[wave][lane] = rt0 * 32 + lane < n ? slen[rt0 * 32 + lane] : 0;
-    int clen = wi.y * 32 + (lane & 31) < n ? slen[wi.y * 32 + (lane & 31)] : 0;
+    srl[wave][lane] = rt0 * 32 + lane < nr ? rslen[rt0 * 32 + lane] : 0;
+    int clen = wi.y * 32 + (lane & 31) < nc ? cslen[wi.y * 32 + (lane & 31)] : 0;
     bs[0][tid] = yb[(size_t)wi.y * NCH + tid];
     bs[0][tid + 256] = yb[(size_t)wi.y * NCH + tid + 256];
     __syncthreads();
@@ -215,8 +222,8 @@ __global__ __launch_bounds__(256) void k_edit_filter(
             p0 = yb[(size_t)(ct + 1) * NCH + tid];
             p1 = yb[(size_t)(ct + 1) * NCH + tid + 256];
             const int cn = (ct + 1) * 32 + (lane & 31);
-            ycn = nrm[cn] + bias;
-            clen_next = cn < n ? slen[cn] : 0;
+            ycn = cnrm[cn] + bias;
+            clen_next = cn < nc ? cslen[cn] : 0;
         }
         i32x16_t acc[ERT];
 #pragma unroll
@@ -248,7 +255,7 @@ __global__ __launch_bounds__(256) void k_edit_filter(
                 for (int q = 0; q < 16; ++q) mask |= (acc[r][q] < nyc ? 1u : 0u) << (r * 16 + q + opaque);
             const int col = ct * 32 + (lane & 31);
             const int rowb = 4 * (lane >> 5);
-            if (col >= n) mask = 0;
+            if (col >= nc) mask = 0;
 #pragma unroll 1
             while (mask) {
                 const int rq = __builtin_ctz(mask);
@@ -256,7 +263,10 @@ __global__ __launch_bounds__(256) void k_edit_filter(
                 const int q = rq & 15;
                 const int rl_i = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);   // row within the wave
                 const int row = rt0 * 32 + rl_i;
-                if (row < col && clen - srl[wave][rl_i] <= k) {
+                // a self join's lengths rise with the sorted position, so
+                // clen >= the row's length there; padding rows never pass
+                const int dl = clen - srl[wave][rl_i];
+                if (self_join ? (row < col && dl <= k) : (row < nr && dl <= k && -dl <= k)) {
                     const unsigned slot = atomicAdd(&scount, 1u);
                     const int2 pr = make_int2(row, col);
                     if (slot < ESTAGE) {
@@ -281,6 +291,25 @@ __global__ __launch_bounds__(256) void k_edit_filter(
     __syncthreads();
     for (unsigned i = tid; i < nst; i += 256)
         if (sbase + i < cap) cand[sbase + i] = stage[i];
+}
+
+// One side (rows or columns) of an edit join, strings in length order:
+// atoms/soff address the raw bytes, codes/coff the dense alphabet codes
+// (16-byte aligned per string), order maps a sorted position to its id.
+struct EditSide {
+    const uint8_t* atoms;
+    const int64_t* soff;
+    const uint8_t* codes;
+    const int64_t* coff;
+    const int* slen;
+    const int* order;
+};
+
+// The output edge of candidate (a, b): a self join's ids in ascending order,
+// a bipartite join's as (row id, column id).
+__device__ __forceinline__ int2 edge_ids(const EditSide& R, const EditSide& C, int2 q, int self_join) {
+    const int i = R.order[q.x], j = C.order[q.y];
+    return self_join ? make_int2(min(i, j), max(i, j)) : make_int2(i, j);
 }
 
 // Banded Levenshtein, band half-width KB >= k.  D[o + KB] holds the DP cell
@@ -333,13 +362,11 @@ __device__ int banded_edit(const uint8_t* __restrict__ s, int m, const uint8_t* 
     return r;
 }
 
+// Candidates are sorted positions (a, b) of the row and column sides.
 template <int KB>
-// Candidates are sorted positions (a, b); soff/slen address the strings in
-// that order and order[] gives the original ids of the output edge.
 __global__ __launch_bounds__(256) void k_edit_verify(
-    const uint8_t* __restrict__ atoms, const int64_t* __restrict__ soff, const int* __restrict__ slen,
-    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
-    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    const EditSide R, const EditSide C, int self_join, const int2* __restrict__ cand, unsigned long long ccap,
+    int k, int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
     unsigned long long* __restrict__ counters) {
     const unsigned long long nc = min(counters[0], ccap);
     const int lane = lane_id();
@@ -351,15 +378,14 @@ __global__ __launch_bounds__(256) void k_edit_verify(
         int dist = 0;
         if (c < nc) {
             const int2 q = cand[c];
-            const int m = slen[q.x], nn = slen[q.y];
+            const int m = R.slen[q.x], nn = C.slen[q.y];
             if (m - nn <= k && nn - m <= k) {   // |m - n| <= dist
-                dist = banded_edit<KB>(atoms + soff[q.x], m, atoms + soff[q.y], nn, k);
+                const uint8_t* sr = R.atoms + R.soff[q.x];
+                const uint8_t* sc = C.atoms + C.soff[q.y];
+                dist = m <= nn ? banded_edit<KB>(sr, m, sc, nn, k) : banded_edit<KB>(sc, nn, sr, m, k);
                 edge = dist <= k;
             }
-            if (edge) {
-                const int i = order[q.x], j = order[q.y];
-                p = make_int2(min(i, j), max(i, j));
-            }
+            if (edge) p = edge_ids(R, C, q, self_join);
         }
         const unsigned long long msk = __ballot(edge);
         if (msk) {
@@ -473,9 +499,8 @@ __device__ int banded_edit_bp(const uint8_t* __restrict__ p, int m, const uint8_
 
 template <class VT, int NP>
 __global__ __launch_bounds__(256) void k_edit_verify_bp(
-    const uint8_t* __restrict__ codes, const int64_t* __restrict__ coff, const int* __restrict__ slen,
-    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
-    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    const EditSide R, const EditSide C, int self_join, const int2* __restrict__ cand, unsigned long long ccap,
+    int k, int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
     unsigned long long* __restrict__ counters) {
     const unsigned long long nc = min(counters[0], ccap);
     const int lane = lane_id();
@@ -487,13 +512,13 @@ __global__ __launch_bounds__(256) void k_edit_verify_bp(
         int dist = 0;
         if (c < nc) {
             const int2 q = cand[c];
-            const int m = slen[q.x], nn = slen[q.y];
-            dist = banded_edit_bp<VT, NP>(codes + coff[q.x], m, codes + coff[q.y], nn, k);
+            const int m = R.slen[q.x], nn = C.slen[q.y];
+            // the shorter string is the pattern (a self join's rows always are)
+            const uint8_t* pr_ = R.codes + R.coff[q.x];
+            const uint8_t* pc_ = C.codes + C.coff[q.y];
+            dist = m <= nn ? banded_edit_bp<VT, NP>(pr_, m, pc_, nn, k) : banded_edit_bp<VT, NP>(pc_, nn, pr_, m, k);
             edge = dist <= k;
-            if (edge) {
-                const int i = order[q.x], j = order[q.y];
-                pr = make_int2(min(i, j), max(i, j));
-            }
+            if (edge) pr = edge_ids(R, C, q, self_join);
         }
         const unsigned long long msk = __ballot(edge);
         if (msk) {
@@ -587,9 +612,8 @@ __device__ int myers_edit(const uint8_t* __restrict__ p, int m, const uint8_t* _
 
 template <int W, int NP>
 __global__ __launch_bounds__(256) void k_edit_verify_myers(
-    const uint8_t* __restrict__ codes, const int64_t* __restrict__ coff, const int* __restrict__ slen,
-    const int* __restrict__ order, const int2* __restrict__ cand, unsigned long long ccap, int k,
-    int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    const EditSide R, const EditSide C, int self_join, const int2* __restrict__ cand, unsigned long long ccap,
+    int k, int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
     unsigned long long* __restrict__ counters) {
     const unsigned long long nc = min(counters[0], ccap);
     const int lane = lane_id();
@@ -601,17 +625,15 @@ __global__ __launch_bounds__(256) void k_edit_verify_myers(
         int dist = 0;
         if (c < nc) {
             const int2 q = cand[c];
-            const int m = slen[q.x], nn = slen[q.y];
+            const int m = R.slen[q.x], nn = C.slen[q.y];
             if (m - nn <= k && nn - m <= k) {
                 // the shorter string is the pattern (the distance is symmetric)
-                dist = m <= nn ? myers_edit<W, NP>(codes + coff[q.x], m, codes + coff[q.y], nn, k)
-                               : myers_edit<W, NP>(codes + coff[q.y], nn, codes + coff[q.x], m, k);
+                const uint8_t* pr_ = R.codes + R.coff[q.x];
+                const uint8_t* pc_ = C.codes + C.coff[q.y];
+                dist = m <= nn ? myers_edit<W, NP>(pr_, m, pc_, nn, k) : myers_edit<W, NP>(pc_, nn, pr_, m, k);
                 edge = dist <= k;
             }
-            if (edge) {
-                const int i = order[q.x], j = order[q.y];
-                pr = make_int2(min(i, j), max(i, j));
-            }
+            if (edge) pr = edge_ids(R, C, q, self_join);
         }
         const unsigned long long msk = __ballot(edge);
         if (msk) {
@@ -647,56 +669,61 @@ struct EditWs {
 std::mutex g_edit_mu;
 EditWs g_ews;
 
+// The two sides of a join and what the verify kernels are chosen by.
+struct EditJoin {
+    EditSide R, C;
+    int self_join = 1;
+    int planes = 1;
+    int maxlen = 0;   // longest string of either side
+};
+
 template <int KB>
-void launch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
-    k_edit_verify<KB><<<2048, 256, 0, stream()>>>(
-        S->atoms.as<uint8_t>(), S->soff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
-        ws.cand.as<int2>(), ws.cand_cap, k,
-        g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
-        ws.counters.as<unsigned long long>());
+void launch_verify(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    k_edit_verify<KB><<<2048, 256, 0, stream()>>>(J.R, J.C, J.self_join, ws.cand.as<int2>(), ws.cand_cap, k,
+                                                  g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
+                                                  ws.counters.as<unsigned long long>());
 }
 
 template <class VT, int NP>
-void launch_verify_bp(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
-    k_edit_verify_bp<VT, NP><<<2048, 256, 0, stream()>>>(
-        S->codes.as<uint8_t>(), S->coff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
-        ws.cand.as<int2>(), ws.cand_cap, k, g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
-        ws.counters.as<unsigned long long>());
+void launch_verify_bp(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    k_edit_verify_bp<VT, NP><<<2048, 256, 0, stream()>>>(J.R, J.C, J.self_join, ws.cand.as<int2>(), ws.cand_cap, k,
+                                                         g->ij.as<int2>(), g->w.as<float>(),
+                                                         (unsigned long long)g->cap,
+                                                         ws.counters.as<unsigned long long>());
 }
 
 template <class VT>
-void launch_verify_planes(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
-    if (S->planes <= 2) launch_verify_bp<VT, 2>(S, ws, k, g);
-    else if (S->planes <= 3) launch_verify_bp<VT, 3>(S, ws, k, g);
-    else if (S->planes <= 4) launch_verify_bp<VT, 4>(S, ws, k, g);
-    else launch_verify_bp<VT, 8>(S, ws, k, g);
+void launch_verify_planes(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    if (J.planes <= 2) launch_verify_bp<VT, 2>(J, ws, k, g);
+    else if (J.planes <= 3) launch_verify_bp<VT, 3>(J, ws, k, g);
+    else if (J.planes <= 4) launch_verify_bp<VT, 4>(J, ws, k, g);
+    else launch_verify_bp<VT, 8>(J, ws, k, g);
 }
 
 template <int W, int NP>
-void launch_verify_myers(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
-    k_edit_verify_myers<W, NP><<<2048, 256, 0, stream()>>>(
-        S->codes.as<uint8_t>(), S->coff.as<int64_t>(), S->slen.as<int>(), S->order.as<int>(),
-        ws.cand.as<int2>(), ws.cand_cap, k, g->ij.as<int2>(), g->w.as<float>(), (unsigned long long)g->cap,
-        ws.counters.as<unsigned long long>());
+void launch_verify_myers(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    k_edit_verify_myers<W, NP><<<2048, 256, 0, stream()>>>(J.R, J.C, J.self_join, ws.cand.as<int2>(), ws.cand_cap,
+                                                           k, g->ij.as<int2>(), g->w.as<float>(),
+                                                           (unsigned long long)g->cap,
+                                                           ws.counters.as<unsigned long long>());
 }
 
 template <int W>
-void launch_myers_planes(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
-    if (S->planes <= 2) launch_verify_myers<W, 2>(S, ws, k, g);
-    else if (S->planes <= 3) launch_verify_myers<W, 3>(S, ws, k, g);
-    else if (S->planes <= 4) launch_verify_myers<W, 4>(S, ws, k, g);
-    else launch_verify_myers<W, 8>(S, ws, k, g);
+void launch_myers_planes(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    if (J.planes <= 2) launch_verify_myers<W, 2>(J, ws, k, g);
+    else if (J.planes <= 3) launch_verify_myers<W, 3>(J, ws, k, g);
+    else if (J.planes <= 4) launch_verify_myers<W, 4>(J, ws, k, g);
+    else launch_verify_myers<W, 8>(J, ws, k, g);
 }
 
 constexpr int kMyersMaxLen = 512;   // longest string of a radius above 32 (8 words)
 
-int dispatch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g) {
+int dispatch_verify(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
     if (k > 32) {
-        const int maxlen = S->len_sorted.empty() ? 0 : S->len_sorted.back();
-        if (maxlen <= 64) launch_myers_planes<1>(S, ws, k, g);
-        else if (maxlen <= 128) launch_myers_planes<2>(S, ws, k, g);
-        else if (maxlen <= 256) launch_myers_planes<4>(S, ws, k, g);
-        else if (maxlen <= kMyersMaxLen) launch_myers_planes<8>(S, ws, k, g);
+        if (J.maxlen <= 64) launch_myers_planes<1>(J, ws, k, g);
+        else if (J.maxlen <= 128) launch_myers_planes<2>(J, ws, k, g);
+        else if (J.maxlen <= 256) launch_myers_planes<4>(J, ws, k, g);
+        else if (J.maxlen <= kMyersMaxLen) launch_myers_planes<8>(J, ws, k, g);
         else {
             set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
             return RNNG_ERR_UNSUPPORTED;
@@ -705,58 +732,116 @@ int dispatch_verify(const rnng_strings_s* S, EditWs& ws, int k, rnng_graph_s* g)
         return RNNG_OK;
     }
     if (k <= 15) {
-        launch_verify_planes<uint32_t>(S, ws, k, g);
+        launch_verify_planes<uint32_t>(J, ws, k, g);
         RNNG_HIP_TRY(hipGetLastError());
         return RNNG_OK;
     }
     if (k <= 31) {
-        launch_verify_planes<uint64_t>(S, ws, k, g);
+        launch_verify_planes<uint64_t>(J, ws, k, g);
         RNNG_HIP_TRY(hipGetLastError());
         return RNNG_OK;
     }
-    if (k <= 4) launch_verify<4>(S, ws, k, g);
-    else if (k <= 8) launch_verify<8>(S, ws, k, g);
-    else if (k <= 12) launch_verify<12>(S, ws, k, g);
-    else if (k <= 16) launch_verify<16>(S, ws, k, g);
-    else if (k <= 24) launch_verify<24>(S, ws, k, g);
-    else if (k <= 32) launch_verify<32>(S, ws, k, g);
-    else {
-        set_error("edit-distance radius above 32 is not supported");
-        return RNNG_ERR_UNSUPPORTED;
-    }
+    launch_verify<32>(J, ws, k, g);
     RNNG_HIP_TRY(hipGetLastError());
     return RNNG_OK;
 }
 
-int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
+__global__ void k_recode(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n,
+                         const uint8_t* __restrict__ lut) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = lut[src[i]];
+}
+
+EditSide side_of(const rnng_strings_s* S, const uint8_t* codes) {
+    EditSide e;
+    e.atoms = S->atoms.as<uint8_t>();
+    e.soff = S->soff.as<int64_t>();
+    e.codes = codes;
+    e.coff = S->coff.as<int64_t>();
+    e.slen = S->slen.as<int>();
+    e.order = S->order.as<int>();
+    return e;
+}
+
+// Pairs (a in A, b in B) with Levenshtein distance <= eps; a self join (A is
+// B) keeps a < b.  Strings of both sets are in length order, so each row
+// block's partners form one contiguous column range.
+int edit_join(const rnng_strings_s* A, const rnng_strings_s* B, bool self, float eps, rnng_graph_s* g,
+              rnng_stats* st) {
     EditWs& ws = g_ews;
-    const int n = (int)S->n;
-    if (st) st->pair_evals += (int64_t)n * (n - 1) / 2;
-    if (!(eps >= 0.0f) || n < 2) {
+    const int nr = (int)A->n, nc = (int)B->n;
+    if (st) st->pair_evals += self ? (int64_t)nr * (nr - 1) / 2 : (int64_t)nr * nc;
+    if (!(eps >= 0.0f) || (self ? nr < 2 : (nr == 0 || nc == 0))) {
         RNNG_TRY(edge_buffers(g, 1));
         g->n_edges = 0;
         return RNNG_OK;
     }
     const int k = eps >= 1e9f ? 1000000000 : (int)floorf(eps);
-    if (k > 32 && !S->len_sorted.empty() && S->len_sorted.back() > kMyersMaxLen) {
+    EditJoin J;
+    J.self_join = self ? 1 : 0;
+    J.maxlen = std::max(A->len_sorted.empty() ? 0 : A->len_sorted.back(),
+                        B->len_sorted.empty() ? 0 : B->len_sorted.back());
+    if (k > 32 && J.maxlen > kMyersMaxLen) {
         set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
         return RNNG_ERR_UNSUPPORTED;
     }
+    // codes of both sides in one alphabet: each set numbers the bytes it
+    // holds, so two sets with different byte sets are renumbered by the union
+    LocalBuf rc_a, rc_b, lut;
+    const uint8_t* ca = A->codes.as<uint8_t>();
+    const uint8_t* cb = B->codes.as<uint8_t>();
+    J.planes = A->planes;
+    if (!self && std::memcmp(A->seen, B->seen, sizeof(A->seen)) != 0) {
+        uint8_t uni[256], la[256] = {0}, lb[256] = {0};
+        int sigma = 0;
+        for (int c = 0; c < 256; ++c) uni[c] = (A->seen[c] || B->seen[c]) ? (uint8_t)sigma++ : 0;
+        int qa = 0, qb = 0;
+        for (int c = 0; c < 256; ++c) {
+            if (A->seen[c]) la[qa++] = uni[c];
+            if (B->seen[c]) lb[qb++] = uni[c];
+        }
+        J.planes = 1;
+        while ((1 << J.planes) < sigma) ++J.planes;
+        RNNG_TRY(lut.reserve(512));
+        RNNG_HIP_TRY(hipMemcpyAsync(lut.ptr, la, 256, hipMemcpyHostToDevice, stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(lut.as<uint8_t>() + 256, lb, 256, hipMemcpyHostToDevice, stream()));
+        RNNG_TRY(rc_a.reserve((size_t)A->codes_len));
+        RNNG_TRY(rc_b.reserve((size_t)B->codes_len));
+        k_recode<<<blocks_for(A->codes_len, 256), 256, 0, stream()>>>(ca, rc_a.as<uint8_t>(), A->codes_len,
+                                                                      lut.as<uint8_t>());
+        k_recode<<<blocks_for(B->codes_len, 256), 256, 0, stream()>>>(cb, rc_b.as<uint8_t>(), B->codes_len,
+                                                                      lut.as<uint8_t>() + 256);
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));   // the host tables above are on the stack
+        ca = rc_a.as<uint8_t>();
+        cb = rc_b.as<uint8_t>();
+    }
+    J.R = side_of(A, ca);
+    J.C = side_of(B, cb);
     // the clamped 3-gram L1 of two strings is at most 512, so any radius above
     // 100 passes every length-compatible pair (and -(6k + 1) stays in range)
     const int kf = std::min(k, 100);
-    const int kw = std::min(k, 1 << 30);
+    const int64_t kw = std::min(k, 1 << 30);
     // work list: row block x column-tile chunk over the length-compatible range
     ws.work_host.clear();
-    const int nrb = (int)(S->n_pad / ERB);
+    const int nrb = (int)(A->n_pad / ERB);
+    const std::vector<int32_t>& lb_ = B->len_sorted;
     for (int rb = 0; rb < nrb; ++rb) {
-        if ((int64_t)rb * ERB >= n) break;
-        const int a_last = std::min(n - 1, rb * ERB + ERB - 1);
-        const int maxlen = (int)std::min<int64_t>((int64_t)S->len_sorted[a_last] + kw, INT32_MAX);
-        const int b_end = (int)(std::upper_bound(S->len_sorted.begin(), S->len_sorted.end(), maxlen) -
-                                S->len_sorted.begin());
+        if ((int64_t)rb * ERB >= nr) break;
+        const int a_first = rb * ERB, a_last = std::min(nr - 1, rb * ERB + ERB - 1);
+        const int64_t maxlen = (int64_t)A->len_sorted[a_last] + kw;
+        const int b_end = (int)(std::upper_bound(lb_.begin(), lb_.end(), (int32_t)std::min<int64_t>(maxlen, INT32_MAX)) -
+                                lb_.begin());
+        int ct0 = rb * (ERB / 32);
+        if (!self) {
+            const int64_t minlen = (int64_t)A->len_sorted[a_first] - kw;
+            const int b_begin = (int)(std::lower_bound(lb_.begin(), lb_.end(),
+                                                       (int32_t)std::max<int64_t>(minlen, INT32_MIN)) -
+                                      lb_.begin());
+            ct0 = b_begin / 32;
+        }
         const int ct_end = (b_end + 31) / 32;
-        for (int ct = rb * (ERB / 32); ct < ct_end; ct += ECT)
+        for (int ct = ct0; ct < ct_end; ct += ECT)
             ws.work_host.push_back(make_int4(rb, ct, std::min(ct_end, ct + ECT), 0));
     }
     xcd_permute(ws.work_host);
@@ -765,13 +850,14 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
     if (!ws.work_host.empty())
         RNNG_HIP_TRY(hipMemcpyAsync(ws.work.ptr, ws.work_host.data(), ws.work_host.size() * sizeof(int4),
                                     hipMemcpyHostToDevice, stream()));
+    const int nmax = std::max(nr, nc);
     if (ws.cand_cap == 0) {
-        ws.cand_cap = std::max<unsigned long long>(1ull << 20, 128ull * (unsigned long long)n);
+        ws.cand_cap = std::max<unsigned long long>(1ull << 20, 128ull * (unsigned long long)nmax);
         RNNG_TRY(ws.cand.reserve(ws.cand_cap * sizeof(int2)));
     }
     if (g->cap == 0)
         RNNG_TRY(edge_buffers(g, g_edge_hint > 0 ? g_edge_hint
-                                                 : std::max<int64_t>(1 << 20, 64 * (int64_t)n)));
+                                                 : std::max<int64_t>(1 << 20, 64 * (int64_t)nmax)));
     Events ev;
     RNNG_TRY(ev.create());
     for (int attempt = 0; attempt < 3; ++attempt) {
@@ -779,12 +865,12 @@ int edit_build(rnng_strings_s* S, float eps, rnng_graph_s* g, rnng_stats* st) {
         RNNG_HIP_TRY(hipEventRecord(ev.e[0], stream()));
         if (!ws.work_host.empty())
             k_edit_filter<<<(unsigned)ws.work_host.size(), 256, 0, stream()>>>(
-                S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(), S->mnrm.as<int>(), S->slen.as<int>(),
-                ws.work.as<int4>(), n, k, -(6 * kf + 1), ws.cand.as<int2>(), ws.cand_cap,
-                ws.counters.as<unsigned long long>());
+                A->mxa.as<i32x4_t>(), B->myb.as<i32x4_t>(), A->mnrm.as<int>(), B->mnrm.as<int>(),
+                A->slen.as<int>(), B->slen.as<int>(), ws.work.as<int4>(), nr, nc, J.self_join, k, -(6 * kf + 1),
+                ws.cand.as<int2>(), ws.cand_cap, ws.counters.as<unsigned long long>());
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
-        RNNG_TRY(dispatch_verify(S, ws, k, g));
+        RNNG_TRY(dispatch_verify(J, ws, k, g));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
         unsigned long long cnt[8];
         RNNG_HIP_TRY(hipMemcpyAsync(cnt, ws.counters.ptr, sizeof(cnt), hipMemcpyDeviceToHost, stream()));
@@ -874,7 +960,7 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
     up(S->soff, soff.data(), soff.size() * sizeof(int64_t));
     // dense alphabet codes for the bit-parallel verify (same layout as atoms)
     {
-        bool seen[256] = {false};
+        bool* seen = S->seen;
         const uint8_t* src = reinterpret_cast<const uint8_t*>(atoms) + base;
         for (int64_t i = 0; i < S->total; ++i) seen[src[i]] = true;
         uint8_t code[256];
@@ -893,6 +979,7 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
         std::vector<uint8_t> cs((size_t)pos + 256, 0);
         for (int64_t a = 0; a < n; ++a)
             for (int32_t i = 0; i < len[(size_t)a]; ++i) cs[(size_t)(co[(size_t)a] + i)] = code[src[soff[(size_t)a] + i]];
+        S->codes_len = (int64_t)cs.size();
         up(S->codes, cs.data(), cs.size());
         up(S->coff, co.data(), co.size() * sizeof(int64_t));
     }
@@ -980,7 +1067,26 @@ int rnng_edit_graph_build(rnng_strings_t S, float eps, rnng_graph_t* out, rnng_s
     RNNG_TRY(ensure_init());
     std::lock_guard<std::mutex> lk(g_edit_mu);
     rnng_graph_s* g = new rnng_graph_s();
-    const int rc = edit_build(S, eps, g, st);
+    const int rc = edit_join(S, S, true, eps, g, st);
+    if (rc != RNNG_OK) {
+        rnng_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return RNNG_OK;
+}
+
+int rnng_edit_graph_build_bipartite(rnng_strings_t a, rnng_strings_t b, float eps, rnng_graph_t* out,
+                                    rnng_stats* st) {
+    if (!a || !b || !out) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    *out = nullptr;
+    RNNG_TRY(ensure_init());
+    std::lock_guard<std::mutex> lk(g_edit_mu);
+    rnng_graph_s* g = new rnng_graph_s();
+    const int rc = edit_join(a, b, false, eps, g, st);
     if (rc != RNNG_OK) {
         rnng_graph_destroy(g);
         return rc;

@@ -164,6 +164,7 @@ int load(const char* path, int64_t start, int64_t count, rnng_dataset_s* ds) {
         return RNNG_ERR_FORMAT;
     }
     ds->n = count;
+    ds->cap = std::max<int64_t>(count, 1);
     ds->d = std::max(fi.d, 1);
     ds->owned = true;
     const size_t bytes = (size_t)std::max<int64_t>(count, 1) * ds->d * sizeof(float);

@@ -6,6 +6,7 @@
 struct rnng_dataset_s {
     float* X = nullptr;       // n*d floats in HBM, row-major
     int64_t n = 0;
+    int64_t cap = 0;          // points the owned buffer holds
     int32_t d = 0;
     bool owned = false;
 };
@@ -48,7 +49,8 @@ inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + bs - 1)
 // the stream has drained.
 int sort_pairs_u64_f32(uint64_t* keys, float* vals, int64_t n);
 int sort_pairs_u64_i64(uint64_t* keys, int64_t* vals, int64_t n);
-int sort_by_value_then_id(float* value, int64_t* id, int64_t n);
+// payload (may be null): one byte per element, moved with it.
+int sort_by_value_then_id(float* value, int64_t* id, int64_t n, uint8_t* payload = nullptr);
 // Stable LSD sorts on key bits [0, end_bit).
 int sort_pairs_u32_i32(uint32_t* keys, int32_t* vals, int64_t n, unsigned end_bit);
 int sort_pairs_u32_i64(uint32_t* keys, int64_t* vals, int64_t n, unsigned end_bit);

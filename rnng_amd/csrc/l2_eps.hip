@@ -2403,6 +2403,7 @@ int rnng_dataset_create_f32(const float* points, int64_t n, int32_t d, rnng_data
     RNNG_TRY(ensure_init());
     rnng_dataset_s* ds = new rnng_dataset_s();
     ds->n = n;
+    ds->cap = std::max<int64_t>(n, 1);
     ds->d = d;
     ds->owned = true;
     const size_t bytes = (size_t)std::max<int64_t>(n, 1) * d * sizeof(float);
@@ -2422,6 +2423,24 @@ int rnng_dataset_create_f32(const float* points, int64_t n, int32_t d, rnng_data
         }
     }
     *out = ds;
+    return RNNG_OK;
+}
+
+int rnng_dataset_upload_f32(rnng_dataset_t ds, const float* points, int64_t n) {
+    if (!ds || n < 0 || (n > 0 && !points)) {
+        set_error("rnng_dataset_upload_f32: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    if (!ds->owned || n > ds->cap) {
+        set_error("rnng_dataset_upload_f32: the dataset is not the library's or holds fewer points");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    if (n > 0)
+        RNNG_HIP_TRY(hipMemcpyAsync(ds->X, points, (size_t)n * ds->d * sizeof(float), hipMemcpyHostToDevice,
+                                    stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    ds->n = n;
     return RNNG_OK;
 }
 

@@ -53,6 +53,50 @@ __device__ __forceinline__ int64_t binom3(int64_t v) { return (v * (v - 1) / 2) 
 // C(v, 4) = C(v, 3) (v - 3) / 4, exact while C(v, 3) (v - 3) < 2^63
 __device__ __forceinline__ int64_t binom4(int64_t v) { return binom3(v) * (v - 3) / 4; }
 
+// C(v, k) for the simplex dimensions of a filtration (k = 1..4).
+__device__ __forceinline__ int64_t binomk(int64_t v, int k) {
+    return k == 1 ? v : k == 2 ? binom2(v) : k == 3 ? binom3(v) : binom4(v);
+}
+
+// Relabelling of a filtration built on n points: each simplex's vertices
+// (decoded from its id, include/simplex.hpp:43-79: the largest l < n with
+// C(l, i + 1) <= the remaining uid, for i = dim .. 0) become map[v], and the
+// id is rebuilt from them (include/simplex.hpp:5-19); sflag = OR of vflag over
+// the simplex's local vertices.
+__global__ void k_relabel(int64_t* __restrict__ id, int64_t ns, const int64_t* __restrict__ map, int64_t n,
+                          const uint8_t* __restrict__ vflag, uint8_t* __restrict__ sflag) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int64_t x = id[s];
+    const int p = (int)((uint64_t)x >> 60) & 15;
+    int64_t uid = x & 0x0FFFFFFFFFFFFFFFll;
+    int64_t g[4];
+    uint8_t f = 0;
+    for (int i = p; i >= 0; --i) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t mid = lo + ((hi - lo) >> 1);
+            if (binomk(mid, i + 1) > uid) hi = mid;
+            else lo = mid + 1;
+        }
+        const int64_t l = hi - 1;
+        uid -= binomk(l, i + 1);
+        g[i] = map[l];
+        if (vflag) f |= vflag[l];
+    }
+    // ascending global vertices (at most four: insertion sort)
+    for (int i = 1; i <= p; ++i)
+        for (int j = i; j > 0 && g[j - 1] > g[j]; --j) {
+            const int64_t t = g[j];
+            g[j] = g[j - 1];
+            g[j - 1] = t;
+        }
+    int64_t nu = 0;
+    for (int i = 0; i <= p; ++i) nu += binomk(g[i], i + 1);
+    id[s] = nu | ((int64_t)p << 60);
+    if (sflag) sflag[s] = f;
+}
+
 // Largest vertex count whose 4-clique ids the reference computes without
 // overflow: its Binom multiplies v (v-1) (v-2) (v-3) in int64
 // (include/utils.hpp:126-127), which stays below 2^63 for v <= 55110.
@@ -547,6 +591,50 @@ int rnng_filtration_copy(rnng_filtration_t f, float* value, int64_t* id, int64_t
     if (value)
         RNNG_HIP_TRY(hipMemcpyAsync(value, f->value.ptr, (size_t)k * sizeof(float), hipMemcpyDeviceToHost, stream()));
     if (id) RNNG_HIP_TRY(hipMemcpyAsync(id, f->id.ptr, (size_t)k * sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    return RNNG_OK;
+}
+
+int rnng_filtration_relabel(rnng_filtration_t f, const int64_t* ids, int64_t n, const uint8_t* vertex_flags,
+                            uint8_t* simplex_flags) {
+    if (!f || n < 0 || (n > 0 && !ids)) {
+        set_error("rnng_filtration_relabel: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    if (f->n == 0) return RNNG_OK;
+    if (f->counts[0] != n) {
+        set_error("rnng_filtration_relabel: n must be the number of points the filtration was built on");
+        return RNNG_ERR_ARG;
+    }
+    std::vector<int64_t> sorted_ids(ids, ids + n);
+    std::sort(sorted_ids.begin(), sorted_ids.end());
+    if (n > 0 && (sorted_ids.front() < 0 || std::adjacent_find(sorted_ids.begin(), sorted_ids.end()) != sorted_ids.end())) {
+        set_error("rnng_filtration_relabel: labels must be distinct and non-negative");
+        return RNNG_ERR_ARG;
+    }
+    // the relabelled ids must be what the reference's int64 binomials give
+    const int64_t top = n > 0 ? sorted_ids.back() + 1 : 0;
+    if ((f->counts[3] > 0 && top > kMaxTetraVertices) || (f->counts[2] > 0 && top > 1800000) ||
+        top > 0x7fffffffll) {
+        set_error("rnng_filtration_relabel: labels too large for the simplex ids of this dimension");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    LocalBuf dmap, dvf, dsf;
+    RNNG_TRY(dmap.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int64_t)));
+    if (n > 0) RNNG_HIP_TRY(hipMemcpyAsync(dmap.ptr, ids, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, stream()));
+    if (vertex_flags) {
+        RNNG_TRY(dvf.reserve((size_t)std::max<int64_t>(n, 1)));
+        if (n > 0) RNNG_HIP_TRY(hipMemcpyAsync(dvf.ptr, vertex_flags, (size_t)n, hipMemcpyHostToDevice, stream()));
+    }
+    RNNG_TRY(dsf.reserve((size_t)f->n));
+    k_relabel<<<blocks_for(f->n, 256), 256, 0, stream()>>>(f->id.as<int64_t>(), f->n, dmap.as<int64_t>(), n,
+                                                           vertex_flags ? dvf.as<uint8_t>() : nullptr,
+                                                           dsf.as<uint8_t>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(sort_by_value_then_id(f->value.as<float>(), f->id.as<int64_t>(), f->n, dsf.as<uint8_t>()));
+    if (simplex_flags)
+        RNNG_HIP_TRY(hipMemcpyAsync(simplex_flags, dsf.ptr, (size_t)f->n, hipMemcpyDeviceToHost, stream()));
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }

@@ -116,8 +116,19 @@ int sort_pairs_u64_f32_swap(DevBuf& keys, DevBuf& vals, int64_t n, unsigned end_
     return sort_pairs_swap_impl<uint64_t, float>(keys, vals, n, end_bit);
 }
 
-// Sorts (value, id) lexicographically: stable LSD passes by id, then by value.
-int sort_by_value_then_id(float* value, int64_t* id, int64_t n) {
+namespace {
+
+__global__ void k_gather_u8(const uint8_t* __restrict__ src, const int64_t* __restrict__ idx,
+                            uint8_t* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+}  // namespace
+
+// Sorts (value, id) lexicographically: stable LSD passes by id, then by value;
+// payload (may be null) follows its element.
+int sort_by_value_then_id(float* value, int64_t* id, int64_t n, uint8_t* payload) {
     if (n <= 1) return RNNG_OK;
     LocalBuf perm, keys64, bits, tmpv, tmpi;
     RNNG_TRY(perm.reserve((size_t)n * sizeof(int64_t)));
@@ -141,6 +152,14 @@ int sort_by_value_then_id(float* value, int64_t* id, int64_t n) {
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_HIP_TRY(hipMemcpyAsync(value, tmpv.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, stream()));
     RNNG_HIP_TRY(hipMemcpyAsync(id, tmpi.ptr, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToDevice, stream()));
+    if (payload) {
+        LocalBuf tmpp;
+        RNNG_TRY(tmpp.reserve((size_t)n));
+        k_gather_u8<<<blocks_for(n, 256), 256, 0, stream()>>>(payload, perm.as<int64_t>(), tmpp.as<uint8_t>(), n);
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_HIP_TRY(hipMemcpyAsync(payload, tmpp.ptr, (size_t)n, hipMemcpyDeviceToDevice, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    }
     RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }

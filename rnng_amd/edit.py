@@ -54,6 +54,14 @@ class StringSet:
         check(_lib.lib().rnng_edit_graph_build(self._h, np.float32(eps), C.byref(h), C.byref(st)))
         return Graph(h, st)
 
+    def bipartite(self, other: "StringSet", eps: float, stats: Stats | None = None) -> Graph:
+        """All (i in self, j in other) with edit distance <= eps."""
+        h = C.c_void_p()
+        st = stats if stats is not None else Stats()
+        check(_lib.lib().rnng_edit_graph_build_bipartite(self._h, other._h, np.float32(eps), C.byref(h),
+                                                         C.byref(st)))
+        return Graph(h, st)
+
 
 def eps_pairs(atoms, offsets, eps: float, stats: Stats | None = None):
     """All i < j with edit distance <= eps, sorted.  Returns (ij int64 [E,2],

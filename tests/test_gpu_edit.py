@@ -165,3 +165,49 @@ def test_full_size_dna_properties():
         s = atoms[offs[i]:offs[i + 1]].tobytes()
         t = atoms[offs[j]:offs[j + 1]].tobytes()
         assert oracle.edit_distance(s, t) == d[e]
+
+
+def _bipartite_ref(orc, A, B, eps):
+    out = []
+    for i, s in enumerate(A):
+        for j, t in enumerate(B):
+            if abs(len(s) - len(t)) <= eps:
+                d = orc.edit_distance(s, t)
+                if d <= eps:
+                    out.append((i, j, d))
+    if not out:
+        return np.zeros((0, 2), np.int64), np.zeros(0, np.int32)
+    a = np.array(out, np.int64)
+    return a[:, :2], a[:, 2].astype(np.int32)
+
+
+@pytest.mark.parametrize("eps", [0, 3, 8, 31, 32, 40])
+def test_bipartite_matches_oracle(orc, eps):
+    """Rows of one string set against columns of another (cell x ghosts, a
+    single query): every (i, j) within eps, including pairs whose row string
+    is the longer one, and two sets with different byte alphabets."""
+    from rnng_amd import synth
+    edit = _edit()
+    atoms, offs = synth.dna_mutants(n_seeds=8, per_seed=30)
+    S = [atoms[offs[i]:offs[i + 1]].tobytes() for i in range(offs.shape[0] - 1)]
+    rng = np.random.default_rng(eps)
+    A = [S[i] for i in rng.choice(len(S), 90, replace=False)]
+    B = [S[i] for i in rng.choice(len(S), 70, replace=False)]
+    # a few strings in another alphabet on one side only, near a DNA string
+    for _ in range(6):
+        t = bytearray(B[int(rng.integers(0, len(B)))])
+        for _ in range(3):
+            t[int(rng.integers(0, len(t)))] = ord("N")
+        A.append(bytes(t))
+    flush_print(f"edit bipartite {len(A)} x {len(B)} eps={eps}")
+    SA, SB = edit.StringSet(*edit.as_csr(A)), edit.StringSet(*edit.as_csr(B))
+    g = SA.bipartite(SB, eps)
+    ij, w = g.edges(sorted=True)
+    g.close()
+    SA.close()
+    SB.close()
+    ij_ref, d_ref = _bipartite_ref(orc, A, B, eps)
+    assert np.array_equal(ij, ij_ref)
+    assert np.array_equal(w.astype(np.int32), d_ref)
+    if eps >= 8:
+        assert ij.shape[0] > 50
