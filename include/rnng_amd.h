@@ -219,6 +219,19 @@ typedef struct rnng_strings_s *rnng_strings_t;   /* device-resident string set *
 int rnng_strings_create(const char *atoms, const int64_t *offsets, int64_t n,
                         rnng_strings_t *out);
 
+/* A .seqs file straight into HBM: one string per line, a trailing '\r'
+ * dropped, empty lines skipped, ids in file order (include/point.hpp:170-205);
+ * strings [start, start + count) of the file (count < 0: to the end).  The
+ * file travels in chunks through two pinned host buffers (the next chunk read
+ * while the previous one is in flight); line splitting and the set's device
+ * layout are built on the GPU.  RNNG_ERR_ARG for a range beyond the file. */
+int rnng_strings_read_file(const char *path, int64_t start, int64_t count, rnng_strings_t *out);
+
+/* Strings and atoms of a set; its atoms and offsets (n + 1, from 0) copied to
+ * the host in id order (either may be NULL). */
+int rnng_strings_size(rnng_strings_t s, int64_t *n, int64_t *atoms);
+int rnng_strings_copy(rnng_strings_t s, char *atoms, int64_t *offsets);
+
 int rnng_strings_destroy(rnng_strings_t s);
 
 /* All pairs i < j whose Levenshtein distance (unit costs, the reference's

@@ -98,6 +98,9 @@ _SIGS = {
     "rnng_filtration_relabel": (C.c_int, [_vp, _vp, C.c_int64, _vp, _vp]),
     "rnng_filtration_destroy": (C.c_int, [_vp]),
     "rnng_strings_create": (C.c_int, [_vp, _vp, C.c_int64, C.POINTER(_vp)]),
+    "rnng_strings_read_file": (C.c_int, [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(_vp)]),
+    "rnng_strings_size": (C.c_int, [_vp, _i64p, _i64p]),
+    "rnng_strings_copy": (C.c_int, [_vp, _vp, _vp]),
     "rnng_strings_destroy": (C.c_int, [_vp]),
     "rnng_edit_graph_build": (C.c_int, [_vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),
     "rnng_edit_graph_build_bipartite": (C.c_int, [_vp, _vp, C.c_float, C.POINTER(_vp), C.POINTER(Stats)]),

@@ -33,14 +33,16 @@
 //      banded dynamic programme with the same band; k > 32 uses
 //      k_edit_verify_myers, the same recurrence unbanded over all rows of the
 //      shorter string (up to 8 64-bit words: strings of at most 512 atoms).
-#include "internal.hpp"
-#include "mfma.hpp"
-
 #include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <numeric>
 #include <vector>
+
+#include <rocprim/rocprim.hpp>
+
+#include "internal.hpp"
+#include "mfma.hpp"
 
 struct rnng_strings_s {
     int64_t n = 0;
@@ -899,6 +901,269 @@ int edit_join(const rnng_strings_s* A, const rnng_strings_s* B, bool self, float
     return RNNG_ERR_HIP;
 }
 
+// ---------------------------------------------------------------------------
+// String sets from device-resident atoms (original order) and offsets
+// ---------------------------------------------------------------------------
+__global__ void k_str_lengths(const int64_t* __restrict__ off, int64_t n, unsigned* __restrict__ len,
+                              int* __restrict__ id) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        len[i] = (unsigned)(off[i + 1] - off[i]);
+        id[i] = (int)i;
+    }
+}
+
+__global__ void k_str_sorted(const int* __restrict__ order, const int64_t* __restrict__ off,
+                             const unsigned* __restrict__ slen, int64_t n, int64_t* __restrict__ soff,
+                             int64_t* __restrict__ alen) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) {
+        soff[a] = off[order[a]];
+        alen[a] = ((int64_t)slen[a] + 15) / 16 * 16;
+    }
+}
+
+// bytes that occur: one LDS table per block, one global OR per byte value
+__global__ __launch_bounds__(256) void k_byte_seen(const uint8_t* __restrict__ atoms, int64_t total,
+                                                   unsigned* __restrict__ seen) {
+    __shared__ unsigned s[256];
+    s[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256)
+        s[atoms[i]] = 1u;
+    __syncthreads();
+    if (s[threadIdx.x]) atomicOr(&seen[threadIdx.x], 1u);
+}
+
+// dense codes of each string in length order, one wave per string
+__global__ __launch_bounds__(256) void k_write_codes(const uint8_t* __restrict__ atoms,
+                                                     const int64_t* __restrict__ soff,
+                                                     const unsigned* __restrict__ slen,
+                                                     const int64_t* __restrict__ coff, int64_t n,
+                                                     const uint8_t* __restrict__ lut, uint8_t* __restrict__ codes) {
+    const int lane = lane_id();
+    for (int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; a < n;
+         a += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const uint8_t* src = atoms + soff[a];
+        uint8_t* dst = codes + coff[a];
+        const unsigned len = slen[a];
+        for (unsigned i = lane; i < len; i += 64) dst[i] = lut[src[i]];
+    }
+}
+
+__global__ void k_fill_bytes(uint32_t* __restrict__ p, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) p[i] = 0u;
+}
+
+// S->atoms (total bytes) and S->offsets (n + 1, from 0) hold the strings in
+// their original order; builds the length order (a stable sort, so equal
+// lengths keep their ids' order), the dense alphabet codes, the 3-gram
+// profiles and the filter fragments on the device.
+int strings_from_device(rnng_strings_s* S, int64_t n, int64_t total) {
+    S->n = n;
+    S->total = total;
+    RNNG_TRY(S->order.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+    RNNG_TRY(S->slen.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int32_t)));
+    RNNG_TRY(S->soff.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int64_t)));
+    LocalBuf alen, seen, lut;
+    RNNG_TRY(alen.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int64_t)));
+    RNNG_TRY(S->coff.reserve((size_t)std::max<int64_t>(n, 1) * sizeof(int64_t)));
+    if (n > 0) {
+        k_str_lengths<<<blocks_for(n, 256), 256, 0, stream()>>>(S->offsets.as<int64_t>(), n, S->slen.as<unsigned>(),
+                                                                 S->order.as<int>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(sort_pairs_u32_i32(S->slen.as<uint32_t>(), S->order.as<int32_t>(), n, 32));
+        k_str_sorted<<<blocks_for(n, 256), 256, 0, stream()>>>(S->order.as<int>(), S->offsets.as<int64_t>(),
+                                                                S->slen.as<unsigned>(), n, S->soff.as<int64_t>(),
+                                                                alen.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(exclusive_scan_i64(alen.as<int64_t>(), S->coff.as<int64_t>(), n));
+        S->len_sorted.resize((size_t)n);
+        RNNG_HIP_TRY(hipMemcpyAsync(S->len_sorted.data(), S->slen.ptr, (size_t)n * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, stream()));
+    }
+    // alphabet: the bytes that occur, numbered in byte order
+    RNNG_TRY(seen.reserve(256 * sizeof(unsigned)));
+    k_fill_bytes<<<1, 256, 0, stream()>>>(seen.as<uint32_t>(), 256);
+    if (total > 0)
+        k_byte_seen<<<(unsigned)std::min<int64_t>(4096, (total + 255) / 256), 256, 0, stream()>>>(
+            S->atoms.as<uint8_t>(), total, seen.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    unsigned hs[256];
+    int64_t last[2] = {0, 0};
+    RNNG_HIP_TRY(hipMemcpyAsync(hs, seen.ptr, sizeof(hs), hipMemcpyDeviceToHost, stream()));
+    if (n > 0) {
+        RNNG_HIP_TRY(hipMemcpyAsync(&last[0], S->coff.as<int64_t>() + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(&last[1], alen.as<int64_t>() + (n - 1), sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    stream()));
+    }
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    uint8_t code[256];
+    int sigma = 0;
+    for (int c = 0; c < 256; ++c) {
+        S->seen[c] = hs[c] != 0;
+        code[c] = S->seen[c] ? (uint8_t)sigma++ : 0;
+    }
+    S->planes = 1;
+    while ((1 << S->planes) < sigma) ++S->planes;
+    // codes: each string 16-byte aligned in length order; the verify kernels
+    // read up to 128 bytes past a string's end
+    S->codes_len = last[0] + last[1] + 256;
+    RNNG_TRY(S->codes.reserve((size_t)(S->codes_len + 3) / 4 * 4));
+    RNNG_TRY(lut.reserve(256));
+    RNNG_HIP_TRY(hipMemcpyAsync(lut.ptr, code, 256, hipMemcpyHostToDevice, stream()));
+    k_fill_bytes<<<blocks_for((S->codes_len + 3) / 4, 256), 256, 0, stream()>>>(S->codes.as<uint32_t>(),
+                                                                               (S->codes_len + 3) / 4);
+    if (n > 0)
+        k_write_codes<<<(unsigned)std::min<int64_t>(8192, (n + 3) / 4), 256, 0, stream()>>>(
+            S->atoms.as<uint8_t>(), S->soff.as<int64_t>(), S->slen.as<unsigned>(), S->coff.as<int64_t>(), n,
+            lut.as<uint8_t>(), S->codes.as<uint8_t>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(S->prof.reserve(std::max<size_t>(16, (size_t)n * 64)));
+    if (n > 0) {
+        k_profiles<<<blocks_for(n, 256), 256, 0, stream()>>>(S->atoms.as<uint8_t>(), S->soff.as<int64_t>(),
+                                                              S->slen.as<int>(), (int)n, S->prof.as<uint4>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    // MFMA filter fragments, padded to whole row blocks
+    S->n_pad = (n + ERB - 1) / ERB * ERB;
+    if (n > 0) {
+        // 64 bins x TLEV levels = one int8 per slot: n_pad x 256 bytes per form
+        const size_t fb = (size_t)S->n_pad * 64 * TLEV;
+        RNNG_TRY(S->mxa.reserve(fb));
+        RNNG_TRY(S->myb.reserve(fb));
+        RNNG_TRY(S->mnrm.reserve((size_t)S->n_pad * sizeof(int)));
+        k_edit_frags<<<blocks_for(S->n_pad * EKS * 2, 256), 256, 0, stream()>>>(
+            S->prof.as<uint4>(), (int)n, S->n_pad, S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(), S->mnrm.as<int>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    return RNNG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// .seqs files (include/point.hpp:170-205): one string per line, a trailing
+// '\r' dropped, empty lines skipped, ids in file order
+// ---------------------------------------------------------------------------
+__global__ void k_newline_flags(const uint8_t* __restrict__ b, int64_t size, uint8_t* __restrict__ f) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < size) f[i] = b[i] == (uint8_t)'\n' ? 1 : 0;
+}
+
+// line k spans [nl[k-1] + 1, nl[k]) (the first from 0, the last to the end)
+__global__ void k_line_spans(const uint8_t* __restrict__ b, int64_t size, const int64_t* __restrict__ nl,
+                             int64_t nnl, longlong2* __restrict__ span, uint8_t* __restrict__ keep) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnl) return;
+    const int64_t s = k == 0 ? 0 : nl[k - 1] + 1;
+    int64_t e = k < nnl ? nl[k] : size;
+    if (e > s && b[e - 1] == (uint8_t)'\r') --e;
+    span[k] = make_longlong2(s, e);
+    keep[k] = e > s ? 1 : 0;
+}
+
+__global__ void k_span_lengths(const longlong2* __restrict__ span, int64_t start, int64_t n,
+                               int64_t* __restrict__ len) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) len[i] = i < n ? span[start + i].y - span[start + i].x : 0;
+}
+
+// one wave per line: its bytes to the string's place in the atom buffer
+__global__ __launch_bounds__(256) void k_gather_lines(const uint8_t* __restrict__ b,
+                                                      const longlong2* __restrict__ span, int64_t start, int64_t n,
+                                                      const int64_t* __restrict__ off, uint8_t* __restrict__ atoms) {
+    const int lane = lane_id();
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < n;
+         i += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+        const longlong2 sp = span[start + i];
+        uint8_t* dst = atoms + off[i];
+        for (int64_t j = sp.x + lane; j < sp.y; j += 64) dst[j - sp.x] = b[j];
+    }
+}
+
+int read_seqs_device(const char* path, int64_t start, int64_t count, rnng_strings_s* S) {
+    int64_t size = 0;
+    RNNG_TRY(upload_file_bytes(path, nullptr, &size, 0));
+    LocalBuf raw, flags, nl, cnt, tmp, span, keep, spank;
+    RNNG_TRY(raw.reserve((size_t)std::max<int64_t>(size, 16)));
+    RNNG_TRY(upload_file_bytes(path, raw.as<uint8_t>(), &size, (int64_t)raw.bytes));
+    // newline positions
+    RNNG_TRY(flags.reserve((size_t)std::max<int64_t>(size, 1)));
+    RNNG_TRY(nl.reserve((size_t)std::max<int64_t>(size, 1) * sizeof(int64_t)));
+    RNNG_TRY(cnt.reserve(2 * sizeof(unsigned long long)));
+    if (size > 0) {
+        k_newline_flags<<<blocks_for(size, 256), 256, 0, stream()>>>(raw.as<uint8_t>(), size, flags.as<uint8_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    unsigned long long hc[2] = {0, 0};
+    if (size > 0) {
+        rocprim::counting_iterator<int64_t> pos(0);
+        size_t tb = 0;
+        RNNG_HIP_TRY(rocprim::select(nullptr, tb, pos, flags.as<uint8_t>(), nl.as<int64_t>(),
+                                     cnt.as<unsigned long long>(), (size_t)size, stream()));
+        RNNG_TRY(tmp.reserve(tb));
+        RNNG_HIP_TRY(rocprim::select(tmp.ptr, tb, pos, flags.as<uint8_t>(), nl.as<int64_t>(),
+                                     cnt.as<unsigned long long>(), (size_t)size, stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(hc, cnt.ptr, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    }
+    const int64_t nnl = (int64_t)hc[0];
+    // non-empty lines
+    const int64_t nseg = nnl + 1;
+    RNNG_TRY(span.reserve((size_t)nseg * sizeof(longlong2)));
+    RNNG_TRY(spank.reserve((size_t)nseg * sizeof(longlong2)));
+    RNNG_TRY(keep.reserve((size_t)nseg));
+    k_line_spans<<<blocks_for(nseg, 256), 256, 0, stream()>>>(raw.as<uint8_t>(), size, nl.as<int64_t>(), nnl,
+                                                              span.as<longlong2>(), keep.as<uint8_t>());
+    RNNG_HIP_TRY(hipGetLastError());
+    {
+        size_t tb = 0;
+        RNNG_HIP_TRY(rocprim::select(nullptr, tb, span.as<longlong2>(), keep.as<uint8_t>(), spank.as<longlong2>(),
+                                     cnt.as<unsigned long long>() + 1, (size_t)nseg, stream()));
+        LocalBuf t2;
+        RNNG_TRY(t2.reserve(tb));
+        RNNG_HIP_TRY(rocprim::select(t2.ptr, tb, span.as<longlong2>(), keep.as<uint8_t>(), spank.as<longlong2>(),
+                                     cnt.as<unsigned long long>() + 1, (size_t)nseg, stream()));
+        RNNG_HIP_TRY(hipMemcpyAsync(&hc[1], cnt.as<unsigned long long>() + 1, sizeof(unsigned long long),
+                                    hipMemcpyDeviceToHost, stream()));
+        RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    }
+    const int64_t lines = (int64_t)hc[1];
+    if (start < 0 || start > lines) {
+        set_error("start beyond the last string of the file");
+        return RNNG_ERR_ARG;
+    }
+    if (count < 0) count = lines - start;
+    if (start + count > lines) {
+        set_error("start + count beyond the last string of the file");
+        return RNNG_ERR_ARG;
+    }
+    if (count > 0x7fffffff - 64) {
+        set_error("string sets are limited to 2^31 - 64 strings");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    const int64_t n = count;
+    LocalBuf len;
+    RNNG_TRY(len.reserve((size_t)(n + 1) * sizeof(int64_t)));
+    RNNG_TRY(S->offsets.reserve((size_t)(n + 1) * sizeof(int64_t)));
+    k_span_lengths<<<blocks_for(n + 1, 256), 256, 0, stream()>>>(spank.as<longlong2>(), start, n, len.as<int64_t>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(exclusive_scan_i64(len.as<int64_t>(), S->offsets.as<int64_t>(), n + 1));
+    int64_t total = 0;
+    RNNG_HIP_TRY(hipMemcpyAsync(&total, S->offsets.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    RNNG_TRY(S->atoms.reserve((size_t)std::max<int64_t>(total, 16)));
+    if (n > 0) {
+        k_gather_lines<<<(unsigned)std::min<int64_t>(8192, (n + 3) / 4), 256, 0, stream()>>>(
+            raw.as<uint8_t>(), spank.as<longlong2>(), start, n, S->offsets.as<int64_t>(), S->atoms.as<uint8_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    return strings_from_device(S, n, total);
+}
+
 }  // namespace
 }  // namespace rnng
 
@@ -917,108 +1182,77 @@ int rnng_strings_create(const char* atoms, const int64_t* offsets, int64_t n, rn
         return RNNG_ERR_UNSUPPORTED;
     }
     for (int64_t i = 0; i < n; ++i)
-        if (offsets[i + 1] < offsets[i]) {
-            set_error("rnng_strings_create: offsets must be non-decreasing");
+        if (offsets[i + 1] < offsets[i] || offsets[i + 1] - offsets[i] > 0x7fffffff) {
+            set_error("rnng_strings_create: offsets must be non-decreasing, strings below 2^31 atoms");
             return RNNG_ERR_ARG;
         }
     RNNG_TRY(ensure_init());
     rnng_strings_s* S = new rnng_strings_s();
-    S->n = n;
     const int64_t base = n > 0 ? offsets[0] : 0;
-    S->total = n > 0 ? offsets[n] - base : 0;
-    std::vector<int64_t> off(n + 1);
-    for (int64_t i = 0; i <= n; ++i) off[i] = n > 0 ? offsets[i] - base : 0;
-    std::vector<int32_t> ord(n);
-    std::iota(ord.begin(), ord.end(), 0);
-    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
-        return off[a + 1] - off[a] < off[b + 1] - off[b];
-    });
-    std::vector<int32_t> len(n);
-    std::vector<int64_t> soff(n);
-    for (int64_t a = 0; a < n; ++a) {
-        len[a] = (int32_t)(off[ord[a] + 1] - off[ord[a]]);
-        soff[a] = off[ord[a]];
+    const int64_t total = n > 0 ? offsets[n] - base : 0;
+    std::vector<int64_t> off((size_t)n + 1);
+    for (int64_t i = 0; i <= n; ++i) off[(size_t)i] = n > 0 ? offsets[i] - base : 0;
+    int rc = S->atoms.reserve((size_t)std::max<int64_t>(total, 16));
+    if (rc == RNNG_OK) rc = S->offsets.reserve(off.size() * sizeof(int64_t));
+    if (rc == RNNG_OK && total > 0 &&
+        hipMemcpyAsync(S->atoms.ptr, atoms + base, (size_t)total, hipMemcpyHostToDevice, stream()) != hipSuccess) {
+        set_error("rnng_strings_create: upload failed");
+        rc = RNNG_ERR_HIP;
     }
-    S->len_sorted = len;
-    int rc = RNNG_OK;
-    auto up = [&](DevBuf& b, const void* src, size_t bytes) {
-        if (rc != RNNG_OK) return;
-        rc = b.reserve(std::max<size_t>(bytes, 16));
-        if (rc == RNNG_OK && bytes) {
-            hipError_t e = hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, stream());
-            if (e == hipSuccess) e = hipStreamSynchronize(stream());
-            if (e != hipSuccess) {
-                set_error(std::string("hipMemcpy: ") + hipGetErrorString(e));
-                rc = RNNG_ERR_HIP;
-            }
-        }
-    };
-    up(S->atoms, n > 0 ? atoms + base : nullptr, (size_t)S->total);
-    up(S->offsets, off.data(), off.size() * sizeof(int64_t));
-    up(S->order, ord.data(), ord.size() * sizeof(int32_t));
-    up(S->slen, len.data(), len.size() * sizeof(int32_t));
-    up(S->soff, soff.data(), soff.size() * sizeof(int64_t));
-    // dense alphabet codes for the bit-parallel verify (same layout as atoms)
-    {
-        bool* seen = S->seen;
-        const uint8_t* src = reinterpret_cast<const uint8_t*>(atoms) + base;
-        for (int64_t i = 0; i < S->total; ++i) seen[src[i]] = true;
-        uint8_t code[256];
-        int sigma = 0;
-        for (int c = 0; c < 256; ++c) code[c] = seen[c] ? (uint8_t)sigma++ : 0;
-        S->planes = 1;
-        while ((1 << S->planes) < sigma) ++S->planes;
-        // strings in sorted order, each starting on a 16-byte boundary; the
-        // verify kernel reads up to 128 bytes past a string's end
-        std::vector<int64_t> co((size_t)n);
-        int64_t pos = 0;
-        for (int64_t a = 0; a < n; ++a) {
-            co[(size_t)a] = pos;
-            pos += (len[(size_t)a] + 15) / 16 * 16;
-        }
-        std::vector<uint8_t> cs((size_t)pos + 256, 0);
-        for (int64_t a = 0; a < n; ++a)
-            for (int32_t i = 0; i < len[(size_t)a]; ++i) cs[(size_t)(co[(size_t)a] + i)] = code[src[soff[(size_t)a] + i]];
-        S->codes_len = (int64_t)cs.size();
-        up(S->codes, cs.data(), cs.size());
-        up(S->coff, co.data(), co.size() * sizeof(int64_t));
+    if (rc == RNNG_OK &&
+        hipMemcpyAsync(S->offsets.ptr, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream()) !=
+            hipSuccess) {
+        set_error("rnng_strings_create: upload failed");
+        rc = RNNG_ERR_HIP;
     }
-    if (rc == RNNG_OK) rc = S->prof.reserve(std::max<size_t>(16, (size_t)n * 64));
-    if (rc == RNNG_OK && n > 0) {
-        k_profiles<<<blocks_for(n, 256), 256, 0, stream()>>>(S->atoms.as<uint8_t>(), S->soff.as<int64_t>(),
-                                                              S->slen.as<int>(), (int)n, S->prof.as<uint4>());
-        hipError_t e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(stream());
-        if (e != hipSuccess) {
-            set_error(std::string("k_profiles: ") + hipGetErrorString(e));
-            rc = RNNG_ERR_HIP;
-        }
-    }
-    // MFMA filter fragments, padded to whole row blocks
-    S->n_pad = (n + ERB - 1) / ERB * ERB;
-    if (rc == RNNG_OK && n > 0) {
-        // 64 bins x TLEV levels = one int8 per slot: n_pad x 256 bytes per form
-        const size_t fb = (size_t)S->n_pad * 64 * TLEV;
-        rc = S->mxa.reserve(fb);
-        if (rc == RNNG_OK) rc = S->myb.reserve(fb);
-        if (rc == RNNG_OK) rc = S->mnrm.reserve((size_t)S->n_pad * sizeof(int));
-        if (rc == RNNG_OK) {
-            k_edit_frags<<<blocks_for(S->n_pad * EKS * 2, 256), 256, 0, stream()>>>(
-                S->prof.as<uint4>(), (int)n, S->n_pad, S->mxa.as<i32x4_t>(), S->myb.as<i32x4_t>(),
-                S->mnrm.as<int>());
-            hipError_t e = hipGetLastError();
-            if (e == hipSuccess) e = hipStreamSynchronize(stream());
-            if (e != hipSuccess) {
-                set_error(std::string("k_edit_frags: ") + hipGetErrorString(e));
-                rc = RNNG_ERR_HIP;
-            }
-        }
-    }
+    if (rc == RNNG_OK) rc = strings_from_device(S, n, total);
     if (rc != RNNG_OK) {
         rnng_strings_destroy(S);
         return rc;
     }
     *out = S;
+    return RNNG_OK;
+}
+
+int rnng_strings_read_file(const char* path, int64_t start, int64_t count, rnng_strings_t* out) {
+    if (!path || !out) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    *out = nullptr;
+    RNNG_TRY(ensure_init());
+    rnng_strings_s* S = new rnng_strings_s();
+    const int rc = read_seqs_device(path, start, count, S);
+    if (rc != RNNG_OK) {
+        rnng_strings_destroy(S);
+        return rc;
+    }
+    *out = S;
+    return RNNG_OK;
+}
+
+int rnng_strings_size(rnng_strings_t S, int64_t* n, int64_t* atoms) {
+    if (!S || !n) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    *n = S->n;
+    if (atoms) *atoms = S->total;
+    return RNNG_OK;
+}
+
+int rnng_strings_copy(rnng_strings_t S, char* atoms, int64_t* offsets) {
+    if (!S) {
+        set_error("null argument");
+        return RNNG_ERR_ARG;
+    }
+    RNNG_TRY(ensure_init());
+    if (atoms && S->total > 0)
+        RNNG_HIP_TRY(hipMemcpyAsync(atoms, S->atoms.ptr, (size_t)S->total, hipMemcpyDeviceToHost, stream()));
+    if (offsets)
+        RNNG_HIP_TRY(hipMemcpyAsync(offsets, S->offsets.ptr, (size_t)(S->n + 1) * sizeof(int64_t),
+                                    hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
     return RNNG_OK;
 }
 

@@ -238,6 +238,55 @@ int load(const char* path, int64_t start, int64_t count, rnng_dataset_s* ds) {
 }
 
 }  // namespace
+
+int upload_file_bytes(const char* path, uint8_t* dst, int64_t* bytes_out, int64_t max_bytes) {
+    Fd f;
+    f.fd = open(path, O_RDONLY);
+    if (f.fd < 0) {
+        set_error(std::string("cannot open ") + path);
+        return RNNG_ERR_ARG;
+    }
+    struct stat st;
+    if (fstat(f.fd, &st) != 0) {
+        set_error(std::string("cannot stat ") + path);
+        return RNNG_ERR_ARG;
+    }
+    const int64_t size = (int64_t)st.st_size;
+    *bytes_out = size;
+    if (!dst) return RNNG_OK;   // size query
+    if (size > max_bytes) {
+        set_error("upload_file_bytes: destination too small");
+        return RNNG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> pin_lock(g_pin_mu);
+    hipEvent_t done[2] = {nullptr, nullptr};
+    struct EvGuard {
+        hipEvent_t* e;
+        ~EvGuard() {
+            for (int b = 0; b < 2; ++b)
+                if (e[b]) (void)hipEventDestroy(e[b]);
+        }
+    } evg{done};
+    struct Drain {
+        ~Drain() { (void)hipStreamSynchronize(stream()); }
+    } drain;
+    for (int b = 0; b < 2; ++b) {
+        if (!g_pinned[b]) RNNG_HIP_TRY(hipHostMalloc(&g_pinned[b], (size_t)kChunk, hipHostMallocDefault));
+        RNNG_HIP_TRY(hipEventCreateWithFlags(&done[b], hipEventDisableTiming));
+    }
+    bool used[2] = {false, false};
+    for (int64_t off = 0, it = 0; off < size; off += kChunk, ++it) {
+        const int b = (int)(it & 1);
+        const int64_t nb = std::min(kChunk, size - off);
+        if (used[b]) RNNG_HIP_TRY(hipEventSynchronize(done[b]));   // buffer b's last DMA has finished
+        RNNG_TRY(read_exact(f.fd, g_pinned[b], (size_t)nb, off));
+        RNNG_HIP_TRY(hipMemcpyAsync(dst + off, g_pinned[b], (size_t)nb, hipMemcpyHostToDevice, stream()));
+        RNNG_HIP_TRY(hipEventRecord(done[b], stream()));
+        used[b] = true;
+    }
+    return RNNG_OK;
+}
+
 }  // namespace rnng
 
 using namespace rnng;

@@ -81,6 +81,12 @@ int launch_plain_self(const float* X, int n, int d, float T, int2* ij, float* w,
 // (noalloc.hip); called by device initialisation with that device current.
 int noalloc_attach(int device);
 
+// A whole file into device memory through two pinned host buffers, the next
+// chunk read while the previous one is in flight (ingest.hip).  *bytes gets
+// the file size; dst null: size query only; a file above max_bytes is an
+// error.  Returns after the copies have landed.
+int upload_file_bytes(const char* path, uint8_t* dst, int64_t* bytes, int64_t max_bytes);
+
 // Uniform-grid self join for d <= 3 (grid.hip); eps must be finite.
 int grid_graph_build(const rnng_dataset_s* A, float eps, rnng_graph_s* g, rnng_stats* st);
 

@@ -37,6 +37,23 @@ class StringSet:
                                              C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_file(cls, path: str, start: int = 0, count: int | None = None) -> "StringSet":
+        """A .seqs file read straight into HBM (rnng_strings_read_file)."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        check(_lib.lib().rnng_strings_read_file(path.encode(), int(start), -1 if count is None else int(count),
+                                                C.byref(h)))
+        self._h = h
+        n, na = C.c_int64(), C.c_int64()
+        check(_lib.lib().rnng_strings_size(h, C.byref(n), C.byref(na)))
+        self.n = n.value
+        self.atoms = np.empty(na.value, np.uint8)
+        self.offsets = np.empty(self.n + 1, np.int64)
+        check(_lib.lib().rnng_strings_copy(h, self.atoms.ctypes.data_as(C.c_void_p),
+                                           self.offsets.ctypes.data_as(C.c_void_p)))
+        return self
+
     def close(self) -> None:
         if self._h:
             _lib.lib().rnng_strings_destroy(self._h)

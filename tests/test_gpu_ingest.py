@@ -81,3 +81,71 @@ def test_device_load_errors(tmp_path):
     with pytest.raises(RnngError) as ei:
         io.load_device(v)
     assert ei.value.code == RNNG_ERR_FORMAT
+
+
+def _seqs_reference(data: bytes):
+    """The reference's reader (include/point.hpp:170-205) restated: getline on
+    '\\n', one trailing '\\r' dropped, empty lines skipped."""
+    parts = data.split(b"\n")
+    if parts and parts[-1] == b"":
+        parts.pop()
+    out = []
+    for ln in parts:
+        if ln.endswith(b"\r"):
+            ln = ln[:-1]
+        if ln:
+            out.append(ln)
+    return out
+
+
+def _csr(strings):
+    offs = np.zeros(len(strings) + 1, np.int64)
+    offs[1:] = np.cumsum([len(s) for s in strings])
+    return np.frombuffer(b"".join(strings), np.uint8), offs
+
+
+@pytest.mark.parametrize("case", ["edges", "dna", "big"])
+def test_seqs_device_read_equals_reference_reader(tmp_path, case):
+    from rnng_amd import edit, synth
+    rng = np.random.default_rng(11)
+    if case == "edges":
+        data = (b"ACGT\r\n\r\n\n\rAC\r\r\nTTTT\n\r\n" + bytes(range(1, 256)).replace(b"\n", b"") + b"\nlast-no-newline")
+    elif case == "dna":
+        atoms, offs = synth.dna_mutants(n_seeds=300, per_seed=20)
+        lines = [atoms[offs[i]:offs[i + 1]].tobytes() for i in range(offs.shape[0] - 1)]
+        data = b"".join(l + (b"\r\n" if i % 3 == 0 else b"\n") + (b"\n" if i % 7 == 0 else b"")
+                        for i, l in enumerate(lines))
+    else:
+        # above two 32 MiB pinned chunks: the double-buffered upload wraps
+        lines = [bytes(rng.choice(list(b"ACGT"), int(L))) for L in rng.integers(100, 151, 600_000)]
+        data = b"\n".join(lines) + b"\n"
+    f = tmp_path / "s.seqs"
+    f.write_bytes(data)
+    want = _seqs_reference(data)
+    flush_print(f"seqs {case}: {len(data)} bytes, {len(want)} strings")
+    for start, count in [(0, None), (len(want) // 3, None), (1, max(1, len(want) // 4)), (len(want), 0)]:
+        S = edit.StringSet.from_file(str(f), start, count)
+        sub = want[start:] if count is None else want[start:start + count]
+        a, o = _csr(sub)
+        assert S.n == len(sub)
+        assert np.array_equal(S.offsets, o) and S.atoms.tobytes() == a.tobytes()
+        S.close()
+    from rnng_amd import _lib
+    import ctypes as C
+    h = C.c_void_p()
+    assert _lib.lib().rnng_strings_read_file(str(f).encode(), len(want) + 1, -1, C.byref(h)) == _lib.RNNG_ERR_ARG
+
+
+def test_seqs_device_read_graph_equals_host_created(tmp_path):
+    from rnng_amd import edit, synth
+    atoms, offs = synth.dna_mutants(n_seeds=40, per_seed=50)
+    f = tmp_path / "s.seqs"
+    f.write_bytes(b"".join(atoms[offs[i]:offs[i + 1]].tobytes() + b"\n" for i in range(offs.shape[0] - 1)))
+    A = edit.StringSet.from_file(str(f))
+    B = edit.StringSet(atoms, offs)
+    ga, gb = A.graph(8.0), B.graph(8.0)
+    ija, wa = ga.edges()
+    ijb, wb = gb.edges()
+    assert ija.shape[0] > 1000 and np.array_equal(ija, ijb) and wa.tobytes() == wb.tobytes()
+    for x in (ga, gb, A, B):
+        x.close()
