@@ -50,6 +50,78 @@ def cpu_baseline(pts, eps, sample):
                       f"in {dt:.2f} s; edges/s {edges / dt:.4g}"}
 
 
+def reference_baseline(pts, eps, sample):
+    """The reference's own systolic driver (built from its sources into
+    oracle/_ref by oracle/Makefile) on the first `sample` points, one MPI
+    rank per core: its cover-tree build plus the ring of radius queries, as
+    its own timers report them (drivers/systolic.cpp:149-156).  None when
+    the binary or MPICH's launcher is absent."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "systolic")
+    mpiexec = os.path.join(os.environ.get("RNNG_MPI_HOME", "/opt/conda"), "bin", "mpiexec.hydra")
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    ranks = min(len(os.sched_getaffinity(0)), 16)
+    sub = np.ascontiguousarray(pts[:sample], np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "sample.fvecs")
+        rec = np.empty((sub.shape[0], sub.shape[1] + 1), np.float32)
+        rec[:, 0] = np.array([sub.shape[1]], np.int32).view(np.float32)[0]
+        rec[:, 1:] = sub
+        rec.tofile(path)
+        t0 = time.perf_counter()
+        r = subprocess.run([mpiexec, "-n", str(ranks), exe, "-i", path, "-r", "%.9g" % float(eps),
+                            "-o", os.path.join(tmp, "out.txt")], capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        log(f"reference driver failed ({r.returncode}): {r.stderr[-500:]}")
+        return None
+    timers = {}
+    for line in (r.stdout + r.stderr).splitlines():
+        if line.startswith("[time="):
+            t, _, what = line[6:].partition("] ")
+            timers[what.split(" [")[0].split(" '")[0]] = float(t)
+    search = timers.get("built cover tree", 0.0) + timers.get("found neighbors", 0.0)
+    if search <= 0:
+        return None
+    pairs = sample * (sample - 1) // 2
+    return {"value": pairs / search, "unit": "pair-evals/s", "cores": ranks, "kind": "reference",
+            "sample": f"the reference's systolic driver (cover trees, {ranks} MPI ranks) on the first {sample} "
+                      f"points: tree build {timers.get('built cover tree', 0):.3f} s + ring of radius queries "
+                      f"{timers.get('found neighbors', 0):.3f} s ({pairs} pairs, all-pairs equivalent); "
+                      f"whole run with file I/O {wall:.2f} s"}
+
+
+def edge_digest(g):
+    """Edge count and SHA-256 of the (i, j)-sorted pairs (int64) and of their
+    weights (float32): the form of the oracle's frozen full-size digests.
+    The sort runs on the device at copy time, outside any timed step."""
+    import hashlib
+    ij, w = g.edges(sorted=True)
+    return {"edges": int(ij.shape[0]), "sha256_ij_int64": hashlib.sha256(ij.tobytes()).hexdigest(),
+            "sha256_w_float32": hashlib.sha256(w.tobytes()).hexdigest()}
+
+
+def same_edges(a, b):
+    return all(a[k] == b[k] for k in ("edges", "sha256_ij_int64", "sha256_w_float32"))
+
+
+def oracle_digest(name, pts, eps):
+    """The committed oracle digest of this exact input (None if absent or for
+    other points or another eps)."""
+    import hashlib
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not os.path.exists(path):
+        return None
+    ref = json.load(open(path))
+    if (ref["n"], ref["d"]) != pts.shape or ref["eps_bits"] != int(np.float32(eps).view(np.uint32)):
+        return None
+    if hashlib.sha256(np.ascontiguousarray(pts, np.float32).tobytes()).hexdigest() != ref["sha256_points_float32"]:
+        return None
+    return ref
+
+
 def measured_traffic(profile, n):
     """HBM bytes per filter launch from the committed rocprofv3 summary (two
     --pmc passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known
@@ -66,15 +138,24 @@ def measured_traffic(profile, n):
 def bench_cells(args):
     """Voronoi-cell workload: 2,000,000 x 96 unit vectors (vMF-like mixture of
     1,024 directions, kappa 50), 4,096 farthest-point landmarks, eps for mean
-    degree ~25.  One step = ghost test + batched per-cell query + dedup over
-    the resident cells; the landmark selection and the cell grouping are
-    timed once."""
+    degree ~25 (--cells-data int: the 1,000,000 x 128 integer-valued GMM
+    instead, on the exact int8 filter).  One step = ghost test + batched
+    per-cell query + dedup over the resident cells; the landmark selection and
+    the cell grouping are timed once.  The all-pairs MFMA self join runs on the
+    same points beside it, and both edge sets are compared by digest."""
     from rnng_amd import Stats, costmodel, l2, synth, voronoi
     have_model = costmodel.use_default()
     log(f"cost model: {'shipped model' if have_model else 'none (count-then-fill)'}")
     t0 = time.perf_counter()
-    n = args.n if args.n != 1_000_000 else 2_000_000
-    pts, eps = synth.points_vmf(n=n)
+    if args.cells_data == "int":
+        pts, eps = synth.points_gmm_int(n=args.n)
+        data = "synthetic (seeded 256-component Gaussian mixture, integer-valued)"
+        what = "fp32 integer-valued [0,255] (mean degree ~30)"
+    else:
+        n = args.n if args.n != 1_000_000 else 2_000_000
+        pts, eps = synth.points_vmf(n=n)
+        data = "synthetic (seeded vMF-like mixture)"
+        what = "unit-norm fp32 (mean degree ~25)"
     n, d = pts.shape
     log(f"generated {n}x{d} eps={eps} in {time.perf_counter() - t0:.1f}s")
     ds = l2.Dataset(pts)
@@ -85,6 +166,9 @@ def bench_cells(args):
     sizes = np.diff(ptr)
     log(f"fps landmarks + cells: {t_lm * 1e3:.1f} ms; cell sizes min {sizes.min()} "
         f"median {int(np.median(sizes))} max {sizes.max()}")
+    if args.schedule:
+        order, form = (int(x) for x in args.schedule.split(","))
+        v.set_schedule(order, form)
     for w in range(args.warmup):
         g = v.graph(eps)
         log(f"warmup {w}: {g.num_edges} edges, stats {g.stats.as_dict()}")
@@ -92,24 +176,33 @@ def bench_cells(args):
     st = Stats()
     t0 = time.perf_counter()
     ne = 0
+    g = None
     for _ in range(args.steps):
+        if g is not None:
+            g.close()
         g = v.graph(eps, "mfma", st)
         ne = g.num_edges
-        g.close()
     dt = time.perf_counter() - t0
     K = args.steps
     log(f"cells: {K} steps {dt / K * 1e3:.2f} ms/step, {ne} edges, stats {st.as_dict()}")
+    dig = edge_digest(g)
+    g.close()
     tv, tx, tms = v.tuning()
+    order, form, cfgs = v.schedule()
+    int8 = st.int8_filter == K
     P_all = n * (n - 1) // 2
     out = {
-        "metric": "ε-graph edges/s via Voronoi cells + ghosts, 2M×96",
+        "metric": f"ε-graph edges/s via Voronoi cells + ghosts, {n // 1000000}M×{d}",
         "value": ne / (dt / K), "unit": "edges/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
         "ms_per_step": dt / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16 mfma (fp32 accumulate) + fp32 recheck", "data": "synthetic (seeded vMF-like mixture)",
-        "config": {"workload": f"eps-graph through {args.cells} Voronoi cells, {n} points x {d} dims "
-                               f"unit-norm fp32, eps={float(eps):.6g} (mean degree ~25)",
+        "dtype": ("int8 mfma filter (int32 accumulate, exact on this integer data) + fp32 reference-order recheck"
+                  if int8 else "bf16 mfma filter (fp32 accumulate) + fp32 reference-order recheck"),
+        "data": data,
+        "config": {"workload": f"eps-graph through {args.cells} Voronoi cells, {n} points x {d} dims {what}, "
+                               f"eps={float(eps):.6g}",
                    "n": n, "d": d, "eps": float(eps), "cells": args.cells},
         "edges": ne,
+        "edge_set": dict(dig, note="sorted on the device at copy time, outside the timed step"),
         "all_pairs_equiv_per_s": P_all / (dt / K),
         "cell_pair_evals": st.pair_evals // K,
         "ghost_tests": st.ghost_tests // K, "ghost_candidates": st.ghost_candidates // K,
@@ -117,27 +210,47 @@ def bench_cells(args):
         "band_pairs": st.band_pairs // K, "band_flips": st.band_flips // K, "refills": st.refills,
         "cost_model": {"used": have_model, "filter_passes_per_step": st.filter_passes / K,
                        "predicted_candidates": st.predicted_candidates // K,
+                       "predicted_over_actual": (st.predicted_candidates / max(1, st.candidates)),
                        "predicted_edges": st.predicted_edges // K},
         "landmarks_and_cells_ms": t_lm * 1e3,
-        "autotuned_filter": {"variant": tv, "xcd_order": tx,
-                             "config_ms_on_an_eighth": [round(x, 3) for x in tms.tolist()]},
+        "schedule": {"cell_order": ["landmark order", "center chain"][order],
+                     "work_form": ["runs of column tiles", "tile lists", "tile lists on a persistent queue"][form],
+                     "forced": bool(args.schedule), "filter_variant": None if int8 else tv,
+                     "xcd_order": tx,
+                     "tuned_ms_on_an_eighth": {f"order{o}/form{f}/xcd{x}": round(t, 3) for o, f, x, t in cfgs}},
         "cell_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
         "stage_ms": {"ghost_test": st.ghost_ms / K, "prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck_dedup": st.recheck_ms / K, "device_total": st.total_ms / K},
     }
-    # on this workload every cell pair is adjacent, so the algorithmic work is
-    # every unordered pair once at 2 d flops (the padded tiles the filter runs
-    # are cell_pair_evals)
-    ach = 2.0 * d * P_all / (st.filter_ms / K * 1e-3) / 1e12
-    kname = {11: "k_l2_mfma_filter_ring<bf16, G=1, S=3>", 12: "k_l2_mfma_filter_ring<bf16, G=2, S=3>",
-             13: "k_l2_mfma_filter_ring<bf16, G=2, S=2>", 14: "k_l2_mfma_filter_ring<bf16, G=3, S=2>",
-             16: "k_l2_mfma_filter_ring<bf16, G=4, S=2, 12 waves>",
-             17: "k_l2_mfma_filter_ring<bf16, G=8, S=2, 12 waves>",
-             8: "k_l2_mfma_filter_g<G=2>"}.get(tv, "k_l2_mfma_filter")
-    out["roofline"] = {"kernel": f"{kname} (autotuned variant {tv})", "bound": "mfma", "achieved": ach,
-                       "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0, "traffic": None,
-                       "work": "2 d flops per unordered pair n(n-1)/2, fill-pass filter time from HIP events"}
+    # the filter's work: 2 d ops per pair of the padded 32 x 32 tiles it runs
+    # (cell_pair_evals; on the vMF data every cell pair is adjacent, so that
+    # is every unordered pair plus padding)
+    peak = PEAK_INT8_TOPS if int8 else PEAK_BF16_TFLOPS
+    ach = 2.0 * d * (st.pair_evals / K) / (st.filter_ms / K * 1e-3) / 1e12
+    kname = ("k_l2_mfma_filter_ring<int8>" if int8 else
+             {12: "k_l2_mfma_filter_ring<bf16, G=2, S=3>", 13: "k_l2_mfma_filter_ring<bf16, G=2, S=2>",
+              14: "k_l2_mfma_filter_ring<bf16, G=3, S=2>",
+              17: "k_l2_mfma_filter_ring<bf16, G=8, S=2, 12 waves>"}.get(tv, "k_l2_mfma_filter_ring"))
+    if form > 0:
+        kname += ", tile lists" + (", persistent" if form == 2 else "")
+    out["roofline"] = {"kernel": kname, "bound": "mfma", "achieved": ach, "peak": peak,
+                       "unit": "TOP/s (int8)" if int8 else "TFLOP/s", "frac": ach / peak, "traffic": None,
+                       "work": "2 d ops per pair of the tiles the filter runs, fill-pass filter time from HIP events"}
     v.close()
+    if not args.no_plain:
+        # the all-pairs MFMA self join on the same resident points
+        sa = Stats()
+        for rep in range(2):
+            t1 = time.perf_counter()
+            g = ds.graph(eps, "mfma", sa)
+            ta = time.perf_counter() - t1
+            if rep == 1:
+                adig = edge_digest(g)
+            g.close()
+        out["all_pairs"] = {"ms": ta * 1e3, "filter_ms": sa.filter_ms / 2, "edges": adig["edges"],
+                            "same_edge_set": same_edges(adig, dig),
+                            "cells_speedup": ta / (dt / K)}
+        log(f"all pairs: {ta * 1e3:.1f} ms, {adig['edges']} edges, same set {same_edges(adig, dig)}")
     ds.close()
     print(json.dumps(out), flush=True)
 
@@ -255,13 +368,23 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=150_000)
+    ap.add_argument("--cpu-sample", type=int, default=150_000,
+                    help="points of the oracle's brute-force baseline (without the reference driver)")
+    ap.add_argument("--ref-sample", type=int, default=250_000,
+                    help="points of the reference driver's baseline")
+    ap.add_argument("--port-baseline", action="store_true",
+                    help="time the oracle's brute force instead of the reference's driver")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--workload", choices=["l2", "cells", "edit", "rips"], default="l2",
                     help="l2: 1M x 128 all-pairs (default, the headline); cells: 2M x 96 through "
                          "Voronoi cells; edit: 200k strings; rips: 300k x 3 2-skeleton")
     ap.add_argument("--cells", type=int, default=4096, help="landmarks of the cells workload")
+    ap.add_argument("--cells-data", choices=["vmf", "int"], default="vmf",
+                    help="cells workload points: 2M x 96 unit vectors, or the 1M x 128 integer GMM")
+    ap.add_argument("--schedule", default="", metavar="ORDER,FORM",
+                    help="force the cell order (0 landmark, 1 center chain) and work form (0 runs, "
+                         "1 tile lists, 2 persistent queue) instead of the autotuner's choice")
     ap.add_argument("--bf16", action="store_true",
                     help="run the timed steps on the BF16 filter even where the exact int8 one applies")
     args = ap.parse_args()
@@ -308,15 +431,22 @@ def main():
     sync()
     t0 = time.perf_counter()
     ne = 0
+    g = None
     for _ in range(args.steps):
+        if g is not None:
+            g.close()
         g = ds.graph(eps, "mfma", st)
         ne = g.num_edges
-        g.close()
     sync()
     dt = time.perf_counter() - t0
     K = args.steps
     ms = dt / K * 1e3
     log(f"mfma: {K} steps {ms:.2f} ms/step, {ne} edges, stats {st.as_dict()}")
+    # the last step's edges against the oracle's frozen digest of this input
+    dig = edge_digest(g)
+    g.close()
+    ref = oracle_digest("full_gmm_int_1Mx128.json", pts, eps)
+    log(f"edge digest {dig}; oracle digest {'present' if ref else 'absent'}")
 
     filt_s = st.filter_ms / K / 1e3
     flops = 2.0 * d * P
@@ -336,14 +466,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("int8 mfma (int32 accumulate, exact on this integer data) + fp32 recheck" if int8
-                  else "bf16 mfma (fp32 accumulate) + fp32 recheck"),
+        "dtype": ("int8 mfma filter (int32 accumulate) + exact integer recheck: exact on this integer data"
+                  if int8 else "bf16 mfma filter (fp32 accumulate) + fp32 reference-order recheck"),
         "data": "synthetic (seeded 256-component Gaussian mixture, integer-valued)",
         "config": {"workload": f"eps-graph self join, {n} points x {d} dims fp32 integer-valued "
                                f"[0,255], eps={float(eps):.6g} (mean degree ~30)",
                    "n": n, "d": d, "eps": float(eps), "mode": "mfma"},
         "edges": ne,
         "edges_per_s": ne / (dt / K),
+        "edge_set": dict(dig, equals_oracle_digest=(same_edges(dig, ref) if ref else None),
+                         note="the (i, j) sort of the reference's Graph constructor runs on the device "
+                              "when the edges are copied out, outside the timed step"),
         "candidates": st.candidates // K,
         "band_pairs": st.band_pairs // K,
         "band_flips": st.band_flips // K,
@@ -360,14 +493,16 @@ def main():
         _lib.check(_lib.lib().rnng_set_int8_filter(0))
         try:
             sb = Stats()
-            for _ in range(2):
+            for rep in range(2):
                 g = ds.graph(eps, "mfma", sb)
                 be = g.num_edges
+                if rep == 1:
+                    bdig = edge_digest(g)
                 g.close()
             bf = sb.filter_ms / 2
             out["bf16_filter"] = {"filter_ms": bf, "achieved_tflops": flops / (bf / 1e3) / 1e12,
                                   "frac_of_bf16_peak": flops / (bf / 1e3) / 1e12 / PEAK_BF16_TFLOPS,
-                                  "edges_equal": be == ne}
+                                  "same_edge_set": same_edges(bdig, dig)}
             log(f"bf16 filter: {bf:.2f} ms, {be} edges")
         finally:
             _lib.check(_lib.lib().rnng_set_int8_filter(1))
@@ -378,12 +513,13 @@ def main():
         t1 = time.perf_counter()
         g = ds.graph(eps, "plain", sp)
         pe = g.num_edges
-        g.close()
         sync()
         tp = time.perf_counter() - t1
+        pdig = edge_digest(g)
+        g.close()
         log(f"plain: {tp * 1e3:.1f} ms, {pe} edges, stats {sp.as_dict()}")
         out["plain"] = {"ms": tp * 1e3, "pair_evals_per_s": P / tp, "edges_per_s": pe / tp,
-                        "edges_equal_count": pe == ne,
+                        "same_edge_set": same_edges(pdig, dig),
                         # 3 non-fused VALU ops per dim per pair against the FP32
                         # VALU op rate (256 CU x 128 lanes/clk x 2.4 GHz = 7.86e13/s;
                         # the 157.3 TFLOP/s peak counts an FMA as 2 flops)
@@ -391,7 +527,9 @@ def main():
         out["speedup_vs_plain"] = tp / (dt / K)
 
     if not args.no_cpu:
-        cb = cpu_baseline(pts, eps, min(args.cpu_sample, n))
+        cb = reference_baseline(pts, eps, min(args.ref_sample, n)) if not args.port_baseline else None
+        if cb is None:
+            cb = cpu_baseline(pts, eps, min(args.cpu_sample, n))
         log(f"cpu baseline: {cb}")
         out["cpu_baseline"] = cb
     ds.close()

@@ -68,6 +68,8 @@ typedef struct rnng_stats {
     int64_t int8_filter;          /* MFMA builds that ran the exact int8 filter (integer data) */
     int64_t plain_fallbacks;      /* MFMA builds run by the plain kernel: coordinates no filter can
                                      bound (NaN, infinities, d * range^2 > 1e36) */
+    double call_ms;               /* host wall time of the build calls (graph and Voronoi builds) */
+    double host_ms;               /* Voronoi: host time spent on the cell layout and work lists */
 } rnng_stats;
 
 /* Opaque handles. */
@@ -367,6 +369,24 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
  * zeros if not tuned; integer data runs the int8 filter untuned). */
 int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
 
+/* Schedule of the cell filter (no counterpart in the reference): the cell
+ * order of the row layout (0 landmark order, 1 the centers' nearest-neighbour
+ * chain, so that neighbouring cells share rows of the layout) and the work
+ * form (0 runs of consecutive column tiles, one workgroup each; 1 each row
+ * block's needed column tiles as one list, cut into items of up to 256 tiles;
+ * 2 those items on a persistent grid of resident workgroups that take them
+ * from a queue, heaviest first by tiles and the cost model's predicted
+ * candidates).  By default the autotuner times every (order, form, filter
+ * shape) on every 8th item of the first build and keeps the fastest;
+ * set_schedule forces an order and/or form (-1 leaves it to the tuner) from
+ * the next build on.  rnng_voronoi_schedule reports the order and form in use
+ * and the tuner's configurations (order << 8 | form << 4 | XCD order) with
+ * their times in ms (up to cap; *n_configs = how many were timed).  Every
+ * schedule gives the same edges. */
+int rnng_voronoi_set_schedule(rnng_voronoi_t v, int32_t order, int32_t sched);
+int rnng_voronoi_schedule(rnng_voronoi_t v, int32_t *order, int32_t *sched, int32_t *configs, double *config_ms,
+                          int32_t cap, int32_t *n_configs);
+
 /* Ghosts found by the last rnng_voronoi_graph_build: their count, and the
  * (cell, point) pairs sorted by cell then point into pairs[2*cap].
  * Replaces: VoronoiCell::ghosts(), include/point.h:120. */
@@ -455,6 +475,27 @@ int rnng_get_filter_kernels(int32_t *int8_code, int32_t *bf16_variant);
  * eighth of the list; positions past the list hold empty items (first tile
  * not below the end tile) that the filter kernels skip. */
 int rnng_debug_xcd_permute(const int32_t *items, int64_t n, int32_t *out, int64_t cap, int64_t *n_out);
+
+/* The cell filter's work lists by the host builder (rnng_debug_cell_work, no
+ * device work) and by the device builder the graph build uses
+ * (rnng_debug_cell_work_device): for m cells of cnt[c] points with the
+ * m x ceil(m / 32) adjacency bitset adj (bit b of row a: cells a and b may hold
+ * an edge), laid out in position order perm (position -> cell), with bt row
+ * tiles per row block: start[c] = first slot of cell c (may be NULL); sched 0:
+ * items (row block, first tile, end tile, 0) of runs of consecutive needed
+ * cells' column tiles, cut at multiples of 256 tiles, in order of that chunk
+ * then row block; 1 / 2: items (row block, first, end, 0) indexing ctl, each
+ * row block's needed column tiles in ascending order, cut into items of at
+ * most 256, in row-block order (2: by cost, heaviest first: tiles + the row
+ * block's rb_cand (NULL: zeros) times the item's share of its tiles / 64).
+ * *covered = pairs the tiles cover.  Both builders give the same arrays. */
+int rnng_debug_cell_work(const int32_t *cnt, int32_t m, const uint32_t *adj, const int32_t *perm, int32_t bt,
+                         int32_t sched, const float *rb_cand, int32_t *start, int32_t *items, int64_t cap_items,
+                         int32_t *ctl, int64_t cap_ctl, int64_t *n_items, int64_t *n_ctl, int64_t *covered);
+int rnng_debug_cell_work_device(const int32_t *cnt, int32_t m, const uint32_t *adj, const int32_t *perm,
+                                int32_t bt, int32_t sched, const float *rb_cand, int32_t *start, int32_t *items,
+                                int64_t cap_items, int32_t *ctl, int64_t cap_ctl, int64_t *n_items, int64_t *n_ctl,
+                                int64_t *covered);
 
 /* Runs one 32x32x32 int8 MFMA (the edit filter's) on caller data: a and b
  * hold 16 int8 per lane (64 lanes), d gets the 16 int32 accumulators of

@@ -54,6 +54,8 @@ class Stats(C.Structure):
         ("predicted_edges", C.c_int64),
         ("int8_filter", C.c_int64),
         ("plain_fallbacks", C.c_int64),
+        ("call_ms", C.c_double),
+        ("host_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -117,6 +119,9 @@ _SIGS = {
                                            C.POINTER(Stats)]),
     "rnng_voronoi_set_options": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "rnng_voronoi_tuning": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _vp]),
+    "rnng_voronoi_set_schedule": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "rnng_voronoi_schedule": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _vp, _vp, C.c_int32,
+                                        C.POINTER(C.c_int32)]),
     "rnng_voronoi_num_ghosts": (C.c_int, [_vp, _i64p]),
     "rnng_voronoi_ghosts": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_voronoi_destroy": (C.c_int, [_vp]),
@@ -134,6 +139,10 @@ _SIGS = {
     "rnng_debug_sqrt": (C.c_int, [_vp, _vp, C.c_int64]),
     "rnng_debug_copy": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
     "rnng_debug_xcd_permute": (C.c_int, [_vp, C.c_int64, _vp, C.c_int64, _i64p]),
+    "rnng_debug_cell_work": (C.c_int, [_vp, C.c_int32, _vp, _vp, C.c_int32, C.c_int32, _vp, _vp, _vp, C.c_int64,
+                                       _vp, C.c_int64, _i64p, _i64p, _i64p]),
+    "rnng_debug_cell_work_device": (C.c_int, [_vp, C.c_int32, _vp, _vp, C.c_int32, C.c_int32, _vp, _vp, _vp,
+                                              C.c_int64, _vp, C.c_int64, _i64p, _i64p, _i64p]),
 }
 
 _lib = None

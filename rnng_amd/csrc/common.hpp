@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -65,9 +66,12 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 // filter kernel reads its item through this helper and, when it returns
 // false, leaves before any other memory access (the whole workgroup takes the
 // same branch, so no barrier is skipped by part of it).
-__device__ __forceinline__ bool load_work_item(const int4* __restrict__ work, int4& wi) {
-    wi = work[blockIdx.x];
+__device__ __forceinline__ bool load_work_item(const int4* __restrict__ work, unsigned item, int4& wi) {
+    wi = work[item];
     return wi.y < wi.z;
+}
+__device__ __forceinline__ bool load_work_item(const int4* __restrict__ work, int4& wi) {
+    return load_work_item(work, blockIdx.x, wi);
 }
 
 // Exclusive prefix of per-lane counts over the 64-lane wave and one atomic

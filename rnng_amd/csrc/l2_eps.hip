@@ -1219,21 +1219,52 @@ __device__ __forceinline__ void ring_barrier(int n) {
     asm volatile("" ::: "memory");
 }
 
+// The column tiles of a tile-list work item, held in registers for the
+// whole item: lane l of r[k] holds ctl[first + 64 k + l] (items have at most
+// 256 tiles).  tile_at(pos) reads the tile at list position pos (uniform
+// across the wave) with one lane read, no memory access.
+struct TileIdx {
+    int r[4];
+    int first;
+};
+
+__device__ __forceinline__ TileIdx load_tile_idx(const int* __restrict__ ctl, int first, int end, int lane) {
+    TileIdx ti;
+    ti.first = first;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = first + 64 * k + lane;
+        ti.r[k] = p < end ? ctl[p] : 0;
+    }
+    return ti;
+}
+
+__device__ __forceinline__ int tile_at(const TileIdx& ti, int pos) {
+    const int o = __builtin_amdgcn_readfirstlane(pos - ti.first);
+    const int k = o >> 6;
+    const int v = k == 0 ? ti.r[0] : k == 1 ? ti.r[1] : k == 2 ? ti.r[2] : ti.r[3];
+    return __builtin_amdgcn_readlane(v, o & 63);
+}
+
 // This wave's share of the loads of one group (G column tiles from t0,
 // clamped to the last tile): fragment chunks j = wave, wave + 4, ... straight
 // into LDS (global_load_lds_dwordx4), and, from wave 3, the group's column
 // start terms (global_load_lds_dword, two tiles per instruction, (G + 1) / 2
 // instructions).
-template <class T, int KS, int G, int NW = 4>
+// With a tile list (TL), t0 and last are positions in the work item's list
+// of column tiles, whose indices ti holds.
+template <class T, int KS, int G, int NW = 4, bool TL = false>
 __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict__ yb,
                                               const typename T::term* __restrict__ cterm, typename T::frag* dst,
-                                              typename T::term* cdst, int t0, int last, int wave, int lane) {
+                                              typename T::term* cdst, int t0, int last, int wave, int lane,
+                                              const TileIdx& ti) {
     constexpr int NJ = G * KS;
 #pragma unroll
     for (int jj = 0; jj < (NJ + NW - 1) / NW; ++jj) {
         const int j = jj * NW + wave;
         if (j >= NJ) break;
-        const int tile = min(t0 + j / KS, last);
+        const int pos = min(t0 + j / KS, last);
+        const int tile = TL ? tile_at(ti, pos) : pos;
         const typename T::frag* src = yb + ((size_t)tile * KS + (j % KS)) * 64 + lane;
         __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)(dst + j * 64),
                                          16, 0, 0);
@@ -1241,7 +1272,15 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
     if (wave == NW - 1) {
 #pragma unroll
         for (int c2 = 0; c2 < (G + 1) / 2; ++c2) {
-            const typename T::term* src = cterm + min(t0 + 2 * c2 + (lane >> 5), last) * 32 + (lane & 31);
+            int tile;
+            if constexpr (TL) {
+                const int t_lo = tile_at(ti, min(t0 + 2 * c2, last));
+                const int t_hi = tile_at(ti, min(t0 + 2 * c2 + 1, last));
+                tile = (lane >> 5) ? t_hi : t_lo;
+            } else {
+                tile = min(t0 + 2 * c2 + (lane >> 5), last);
+            }
+            const typename T::term* src = cterm + tile * 32 + (lane & 31);
             __builtin_amdgcn_global_load_lds((const void*)src,
                                              (__attribute__((address_space(3))) void*)(cdst + 64 * c2), 4, 0, 0);
         }
@@ -1254,12 +1293,19 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
 // time to land; one barrier per group.  The stages are distinct LDS objects
 // and the loop is unrolled by S, so the compiler sees that reads of one stage
 // never alias the asynchronous loads into the others.
-template <class T, int KS, int G, int MINW, int S = 3, int NW = 4>
+//
+// Work items (row block, first, end, 0) cover column tiles first .. end - 1,
+// or with a tile list (TL) the tiles ctl[first .. end - 1], at most 256 of
+// them (held in registers for the item's whole run).  A persistent
+// grid (PQ) takes items queue[0]++ (in list order) in each workgroup until
+// the list is exhausted; otherwise workgroup b runs item b.
+template <class T, int KS, int G, int MINW, int S = 3, int NW = 4, bool TL = false, bool PQ = false>
 __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
     const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
     const int4* __restrict__ work, int n_rows, int n_cols, int self_join, int2* __restrict__ cand,
-    unsigned long long cap, unsigned long long* __restrict__ counter) {
+    unsigned long long cap, unsigned long long* __restrict__ counter, const int* __restrict__ ctl,
+    unsigned n_items, unsigned* __restrict__ queue) {
     using frag = typename T::frag;
     using term = typename T::term;
     using accv = typename T::accv;
@@ -1275,12 +1321,24 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     __shared__ int2 stage[STG];
     __shared__ unsigned scount;
     __shared__ unsigned long long sbase;
-    int4 wi;
-    if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
+    __shared__ unsigned s_item;
     const int tid = threadIdx.x;
-    if (tid == 0) scount = 0u;
-    const int wave = tid >> 6;
+    // wave-uniform (scalar) in the tile-list forms, whose list positions
+    // derive from it
+    const int wave = TL ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
     const int lane = tid & 63;
+    unsigned item = blockIdx.x;
+    for (;; ) {
+    if constexpr (PQ) {
+        if (tid == 0) s_item = atomicAdd(queue, 1u);
+        __syncthreads();
+        item = s_item;
+        __syncthreads();   // every wave has read s_item before the next round writes it
+        if (item >= n_items) break;
+    }
+    int4 wi;
+    if (load_work_item(work, item, wi)) {   // padding items touch no memory
+    if (tid == 0) scount = 0u;
     const int rt0 = wi.x * (NW * MRT) + wave * MRT;
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
@@ -1289,13 +1347,17 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
                                                    (wave == NW - 1 ? (G + 1) / 2 : 0));
 
     // issue order: rows, group 0, group 1 (the barrier's count relies on it)
+    // the item's tile indices first (the group loads wait for them), then
+    // the rows
+    TileIdx ti{};
+    if constexpr (TL) ti = load_tile_idx(ctl, wi.y, wi.z, lane);
     frag a[MRT][KS];
     accv xr[MRT];
     mf_load_rows<T, KS>(xa, rterm, rt0, lane, a, xr);
     asm volatile("" ::: "memory");
-    mf_group_load<T, KS, G, NW>(yb, cterm, bs0, cs0, wi.y, last, wave, lane);
+    mf_group_load<T, KS, G, NW, TL>(yb, cterm, bs0, cs0, wi.y, last, wave, lane, ti);
     asm volatile("" ::: "memory");
-    if (S == 3 && ng > 1) mf_group_load<T, KS, G, NW>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane);
+    if (S == 3 && ng > 1) mf_group_load<T, KS, G, NW, TL>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane, ti);
     // rows and group 0 landed (with three stages group 1 may still be in
     // flight); scount is set
     ring_barrier(S == 3 && ng > 1 ? nld : 0);
@@ -1306,16 +1368,18 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     auto step = [&](const frag* cur, const term* ccur, frag* nxt, term* cnxt, int g) {
         const int t0 = wi.y + g * G;
         const bool issue = g + S - 1 < ng;
-        if (issue) mf_group_load<T, KS, G, NW>(yb, cterm, nxt, cnxt, t0 + (S - 1) * G, last, wave, lane);
+        if (issue) mf_group_load<T, KS, G, NW, TL>(yb, cterm, nxt, cnxt, t0 + (S - 1) * G, last, wave, lane, ti);
 #pragma unroll
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
             const term nyc = -ccur[cc * 32 + (lane & 31)];
             accv acc[MRT];
             mf_tile<T, KS>(cur + cc * NCH, a, xr, lane, acc);
-            if (__any(mf_min<T>(acc) < nyc))
-                mf_emit<T>(acc, nyc, t0 + cc, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand,
-                           cap, counter);
+            if (__any(mf_min<T>(acc) < nyc)) {
+                const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
+                mf_emit<T>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
+                           counter);
+            }
         }
         ring_barrier(S == 3 && issue ? nld : 0);
     };
@@ -1343,6 +1407,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     __syncthreads();
     for (unsigned i = tid; i < nst; i += 64 * NW)
         if (sbase + i < cap) cand[sbase + i] = stage[i];
+    }   // item
+    if constexpr (!PQ) break;
+    }   // items of a persistent workgroup
 }
 
 // Output per candidate c: flag[c] = 1 if it is an edge, w[c] = its float
@@ -1647,7 +1714,8 @@ void launch_filter_g(const FilterArgs& a) {
 template <int KS, int G, int S = 3, int NW = 4>
 void launch_filter_ring(const FilterArgs& a) {
     k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S, NW><<<a.n_items, 64 * NW, 0, stream()>>>(
-        a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+        a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter, nullptr,
+        a.n_items, nullptr);
 }
 
 template <int KS>
@@ -1699,14 +1767,17 @@ void launch_filter_i8(const I8Args& a) {
                                                                   a.n_cols, a.self_join, a.cand, a.cap, a.counter);
     else if constexpr (CODE <= 2)
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 3><<<a.n_items, 256, 0, stream()>>>(
-            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+            nullptr, a.n_items, nullptr);
     else if constexpr (CODE <= 4)
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<a.n_items, 256, 0, stream()>>>(
-            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+            nullptr, a.n_items, nullptr);
     else {
         constexpr int G = CODE == 5 ? 4 : (CODE == 6 || KS8 >= 5) ? 6 : 8;
         k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8><<<a.n_items, 512, 0, stream()>>>(
-            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter);
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+            nullptr, a.n_items, nullptr);
     }
 }
 
@@ -1796,6 +1867,84 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
     }
     RNNG_HIP_TRY(hipGetLastError());
     return RNNG_OK;
+}
+
+namespace {
+
+template <class T, int KS, int G, int MINW, int NW, bool PQ>
+int launch_tiles(const TileFilterArgs& a) {
+    auto kern = k_l2_mfma_filter_ring<T, KS, G, MINW, 2, NW, true, PQ>;
+    unsigned grid = a.n_items;
+    if constexpr (PQ) {
+        // resident workgroups of this shape on the whole device (once)
+        static int slots = 0;
+        if (slots == 0) {
+            int per_cu = 0, cus = 0, dev = 0;
+            RNNG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * NW, 0));
+            RNNG_HIP_TRY(hipGetDevice(&dev));
+            RNNG_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            slots = std::max(1, per_cu * cus);
+        }
+        grid = std::min<unsigned>(grid, (unsigned)slots);
+        if (a.dry_run) return RNNG_OK;
+        RNNG_TRY(launch_fill_u32(a.queue, 1, 0u));
+    }
+    if (a.dry_run) return RNNG_OK;
+    using frag = typename T::frag;
+    using term = typename T::term;
+    kern<<<grid, 64 * NW, 0, stream()>>>(
+        static_cast<const frag*>(a.xa), static_cast<const frag*>(a.yb), static_cast<const term*>(a.rterm),
+        static_cast<const term*>(a.cterm), a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter, a.ctl,
+        a.n_items, a.queue);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
+template <int KS, bool PQ>
+int launch_tiles_bf16(const TileFilterArgs& a) {
+    if (a.variant == 13) return launch_tiles<MfBF16, KS, 2, 1, 4, PQ>(a);
+    return launch_tiles<MfBF16, KS, (KS >= 9 ? 6 : 8), 1, 12, PQ>(a);
+}
+
+template <bool PQ>
+int dispatch_tiles(const TileFilterArgs& a) {
+    if (a.i8) {
+        switch (a.ks) {
+            case 1: return launch_tiles<MfI8, 1, 3, 4, 4, PQ>(a);
+            case 2: return launch_tiles<MfI8, 2, 3, 4, 4, PQ>(a);
+            case 3: return launch_tiles<MfI8, 3, 3, 4, 4, PQ>(a);
+            case 4: return launch_tiles<MfI8, 4, 3, 4, 4, PQ>(a);
+            case 5: return launch_tiles<MfI8, 5, 3, 1, 4, PQ>(a);
+            default: break;
+        }
+        set_error("int8 tile-list filter supports d <= 160");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    switch (a.ks) {
+        case 1: return launch_tiles_bf16<1, PQ>(a);
+        case 2: return launch_tiles_bf16<2, PQ>(a);
+        case 3: return launch_tiles_bf16<3, PQ>(a);
+        case 4: return launch_tiles_bf16<4, PQ>(a);
+        case 5: return launch_tiles_bf16<5, PQ>(a);
+        case 6: return launch_tiles_bf16<6, PQ>(a);
+        case 7: return launch_tiles_bf16<7, PQ>(a);
+        case 8: return launch_tiles_bf16<8, PQ>(a);
+        case 9: return launch_tiles_bf16<9, PQ>(a);
+        default: break;
+    }
+    set_error("BF16 tile-list filter supports d <= 144");
+    return RNNG_ERR_UNSUPPORTED;
+}
+
+}  // namespace
+
+int filter_dispatch_tiles(const TileFilterArgs& a) {
+    if (a.n_items == 0 && !a.dry_run) return RNNG_OK;
+    if (!a.ctl || (a.persistent && !a.queue) || (!a.i8 && a.variant != 17 && a.variant != 13)) {
+        set_error("filter_dispatch_tiles: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    return a.persistent ? dispatch_tiles<true>(a) : dispatch_tiles<false>(a);
 }
 
 int filter_i8_code() { return i8_group(); }
@@ -2467,12 +2616,18 @@ int rnng_dataset_destroy(rnng_dataset_t ds) {
 }
 
 int rnng_graph_build(rnng_dataset_t ds, float eps, int32_t mode, rnng_graph_t* out, rnng_stats* st) {
-    return graph_build(ds, ds, true, eps, mode, out, st);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int s = graph_build(ds, ds, true, eps, mode, out, st);
+    if (st) st->call_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return s;
 }
 
 int rnng_graph_build_bipartite(rnng_dataset_t a, rnng_dataset_t b, float eps, int32_t mode,
                                rnng_graph_t* out, rnng_stats* st) {
-    return graph_build(a, b, false, eps, mode, out, st);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int s = graph_build(a, b, false, eps, mode, out, st);
+    if (st) st->call_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return s;
 }
 
 int rnng_graph_num_edges(rnng_graph_t g, int64_t* n_edges) {

@@ -88,12 +88,41 @@ struct I8Args {
     unsigned long long* counter = nullptr;
 };
 int filter_dispatch_i8(const I8Args& a, int KS8, int code);
+
+// The ring filter over tile lists (the cell path): work items (row block,
+// first, end, 0) index ctl, the column tiles of each row block in order.
+// int8 (i8): the four-wave shape with two stages of 3 tiles (code 3); BF16:
+// variant 17 or 13.  persistent: a grid of resident workgroups takes items
+// from *queue (cleared by the launch) in list order; otherwise one workgroup
+// per item.  xa/yb/rterm/cterm are the int8 or BF16 forms.
+struct TileFilterArgs {
+    bool i8 = false;
+    const void* xa = nullptr;
+    const void* yb = nullptr;
+    const void* rterm = nullptr;
+    const void* cterm = nullptr;
+    const int4* work = nullptr;
+    unsigned n_items = 0;
+    const int* ctl = nullptr;
+    bool persistent = false;
+    unsigned* queue = nullptr;
+    int n_rows = 0, n_cols = 0, self_join = 0;
+    int2* cand = nullptr;
+    unsigned long long cap = 0;
+    unsigned long long* counter = nullptr;
+    int ks = 0;        // k-steps (KS8 for int8)
+    int variant = 17;  // BF16 variant (17 or 13)
+    bool dry_run = false;   // size the persistent grid (occupancy query) without launching
+};
+int filter_dispatch_tiles(const TileFilterArgs& a);
 int filter_i8_code();
 int filter_block_tiles(int variant);
 int filter_i8_row_tiles(int code);   // row tiles per work item of an int8 kernel code
 int filter_default_variant();
 int filter_num_variants();
 constexpr int kFilterChunk = 256;   // column tiles per chunk of the work list
+constexpr int kMaxListItem = 256;   // column tiles per tile-list work item (four registers per lane)
+constexpr int kMaxCellWords = 16384;   // cell bitset words the device work-list builder holds in LDS (x2)
 
 // Reorders a work list so that XCD x (workgroups are dealt to the 8 XCDs
 // round-robin) runs the contiguous x-th eighth; pads with empty items.

@@ -62,11 +62,24 @@ struct rnng_voronoi_s {
     rnng::DevBuf feat;
     rnng::DevBuf counts;
     // graph-build workspace, grown on demand and kept across builds
-    rnng::LocalBuf rad, cnt_dev, adj_dev, rs, slot, items, yd, counters, cand, cid, cflag, cw, tune_items;
+    rnng::LocalBuf rad, cnt_dev, adj_dev, slot, items, yd, counters, cand, cid, cflag, cw, tune_items;
     rnng::MfmaPrep prep;
     // filter configuration chosen by the autotuner on the first build
     int tuned_variant = -1, tuned_xcd = 1;
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // schedule: cell order (0 landmark order, 1 nearest-neighbour chain of the
+    // centers) and work form (0 runs of column tiles, 1 tile lists, 2 tile
+    // lists on a persistent queue in order of predicted cost); -1: the
+    // autotuner's choice.  forced_*: set by rnng_voronoi_set_schedule.
+    int order = -1, sched = -1, forced_order = -1, forced_sched = -1;
+    std::vector<int> sched_cfg;           // tuner configurations (order << 8 | sched << 4 | variant slot)
+    std::vector<double> sched_ms;         // and their times on every 8th work item
+    std::vector<int> chain;               // positions -> cells along the chain (computed once)
+    rnng::LocalBuf start, ctl, queue, cdist;
+    // the row layout on the device (position -> cell, cell -> position, padded
+    // row starts, tile -> position) and the work-list builder's workspace
+    rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell;
+    rnng::LocalBuf wl_need, wl_cnt, wl_off, wl_icnt, wl_ioff, wl_stage, wl_stage2, wl_keys, wl_tot, wl_rbc;
     int use_graph = -1, autotune = -1;   // -1: from RNNG_CELL_GRAPH / RNNG_CELL_TUNE (default on)
     // captured fill pass (filter, id mapping, recheck, reference semantics,
     // per-cell counts) and the arguments it was captured with
@@ -503,98 +516,227 @@ __global__ __launch_bounds__(256) void k_cell_adjacency(const float* __restrict_
     }
 }
 
-// Slot map of the cell-ordered layout: slots rs[b] .. rs[b] + n_b hold the
-// members of cell b in coalesce order, the rest of the cell's 32-padded range
-// is -1.
-__global__ __launch_bounds__(256) void k_slot_map(const int* __restrict__ cptr, const int* __restrict__ cids,
-                                                  const int* __restrict__ rs, int* __restrict__ slot) {
-    const int b = blockIdx.x;
-    const int nb = cptr[b + 1] - cptr[b], nr = rs[b + 1] - rs[b];
-    for (int k = threadIdx.x; k < nr; k += 256) slot[rs[b] + k] = k < nb ? cids[cptr[b] + k] : -1;
-}
-
-// Candidate slots -> point ids.
-__global__ void k_cand_ids(const int2* __restrict__ cand, const unsigned long long* __restrict__ counter,
-                           unsigned long long cap, const int* __restrict__ slot, int2* __restrict__ ids) {
-    const unsigned long long nc = min(counter[0], cap);
-    for (unsigned long long c = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-         c += (unsigned long long)gridDim.x * blockDim.x) {
-        const int2 s = cand[c];
-        ids[c] = make_int2(slot[s.x], slot[s.y]);
+// Squared FP32 distances between all centers (an ordering aid, not a
+// decision): one workgroup per center a, row a of the m x m matrix.
+__global__ __launch_bounds__(256) void k_center_dist(const float* __restrict__ C, int m, int d,
+                                                     float* __restrict__ out) {
+    extern __shared__ float ca[];
+    const int a = blockIdx.x;
+    for (int t = threadIdx.x; t < d; t += 256) ca[t] = C[(size_t)a * d + t];
+    __syncthreads();
+    for (int b = threadIdx.x; b < m; b += 256) {
+        const float* cb = C + (size_t)b * d;
+        float acc = 0.0f;
+        for (int k = 0; k < d; ++k) {
+            const float t = ca[k] - cb[k];
+            acc += t * t;
+        }
+        out[(size_t)a * m + b] = acc;
     }
 }
 
-// The reference's semantics for an edge (p, q) within eps found across cells
-// a = cell(p) != b = cell(q): the landmark driver emits it iff q is a ghost of
-// a (L2(c_a, q) <= fl(delta_q + 2 eps)) or p a ghost of b.  Clears the edge
-// flag otherwise; writes each ghost (cell << 32 | point) found; edges are
-// made (min, max).  counters[4] = center tests, counters[5] = ghost hits.
-__global__ void k_ref_semantics(const float* __restrict__ X, const float* __restrict__ C, int d,
-                                const int* __restrict__ cell, const float* __restrict__ delta, float eps,
-                                int2* __restrict__ ids, uint8_t* __restrict__ flag,
-                                unsigned long long cap, unsigned long long* __restrict__ counters,
-                                uint64_t* __restrict__ gkeys, int vec4) {
+// Slot map of a cell layout: slots start[b] .. start[b] + n_b hold the
+// members of cell b in coalesce order, the rest of its 32-padded range -1.
+__global__ __launch_bounds__(256) void k_slot_map_at(const int* __restrict__ cptr, const int* __restrict__ cids,
+                                                     const int* __restrict__ start, int* __restrict__ slot) {
+    const int b = blockIdx.x;
+    const int nb = cptr[b + 1] - cptr[b], nr = (nb + 31) & ~31;
+    for (int k = threadIdx.x; k < nr; k += 256) slot[start[b] + k] = k < nb ? cids[cptr[b] + k] : -1;
+}
+
+// The fill pass's work per candidate.  Each wave takes a contiguous range of
+// the candidates, RC_CELL per round, with their two point rows staged once in
+// its own part of LDS:
+//  * candidate slots -> point ids, written (min, max);
+//  * the FP32 distance in the reference's order (sequential, not fused), the
+//    edge decision acc <= T and the float weight; candidates within 1e-5 eps
+//    of eps are band pairs (counters[2]), whose FP64 decision is compared
+//    (flips: counters[3]);
+//  * the reference's semantics for an edge (p, q) across cells
+//    a = cell(p) != b = cell(q): the landmark driver emits it iff q is a
+//    ghost of a (L2(c_a, q) <= fl(delta_q + 2 eps)) or p a ghost of b; the
+//    flag is cleared otherwise and each ghost found is written
+//    (cell << 32 | point); counters[4] = center tests, counters[5] = ghosts;
+//  * per-cell (candidates, edges) by the cell of the row point p.
+// Counters are kept per wave and added once: the ghost keys go through a
+// per-wave LDS buffer, the per-cell counts accumulate while the row cell
+// stays the same (consecutive candidates come from one row block).  Reads
+// the candidate count from the device (the grid is sized by cap), so the
+// pass can be captured.
+constexpr int RC_CELL = 16;
+static_assert(3 * RC_CELL <= 64 && (RC_CELL & (RC_CELL - 1)) == 0, "three sums per candidate in one wave");
+constexpr int GBUF = 256;   // ghost keys buffered per wave
+__host__ __device__ constexpr int cell_rows_ld(int d) { return ((d + 3) & ~3) + 4; }
+
+template <bool VEC4>
+__global__ __launch_bounds__(256) void k_cell_recheck(
+    const float* __restrict__ X, const float* __restrict__ C, int d, const int2* __restrict__ cand,
+    unsigned long long cap, const int* __restrict__ slot, const int* __restrict__ cell,
+    const float* __restrict__ delta, float T, float eps, int2* __restrict__ ids, uint8_t* __restrict__ flag,
+    float* __restrict__ cw, unsigned long long* __restrict__ counters, uint64_t* __restrict__ gkeys,
+    unsigned* __restrict__ cc) {
 #pragma clang fp contract(off)
+    extern __shared__ float rows_all[];   // [4 waves][2 * RC_CELL][ld]: p rows, then q rows
+    __shared__ uint64_t gbuf_all[4][GBUF];
     const unsigned long long nc = min(counters[0], cap);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int ld = cell_rows_ld(d);
+    float* rows = rows_all + (size_t)wave * (2 * RC_CELL) * ld;
+    uint64_t* gbuf = gbuf_all[wave];
+    const double band = 1e-5 * (double)eps;
     const float two_eps = 2.0f * eps;
-    const int lane = threadIdx.x & 63;
-    // rounds are uniform over the wave (its 64 candidates), so the counters and
-    // ghost slots are reserved once per wave (ballot + mbcnt)
-    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
-         base += (unsigned long long)gridDim.x * blockDim.x) {
-        const unsigned long long c = base + threadIdx.x;
-        bool tested = false, g[2] = {false, false};
-        int cab[2] = {0, 0}, qv[2] = {0, 0};
-        if (c < nc && flag[c]) {
-            const int2 e = ids[c];
-            ids[c] = make_int2(min(e.x, e.y), max(e.x, e.y));
-            const int a = cell[e.x], b = cell[e.y];
-            if (a != b) {
-                tested = true;
+    // this wave's range of candidates
+    const unsigned long long nw = (unsigned long long)gridDim.x * 4;
+    const unsigned long long per = ((nc + nw - 1) / nw + RC_CELL - 1) / RC_CELL * RC_CELL;
+    const unsigned long long beg = ((unsigned long long)blockIdx.x * 4 + wave) * per;
+    const unsigned long long end = min(nc, beg + per);
+    unsigned long long tests = 0;
+    int gn = 0;                              // keys in gbuf
+    int run_cell = -1;                       // per-cell counts of the current row cell
+    unsigned run_cand = 0, run_edges = 0;
+    auto flush_ghosts = [&]() {
+        unsigned long long o = 0;
+        if (lane == 0 && gn) o = atomicAdd(&counters[5], (unsigned long long)gn);
+        o = __shfl(o, 0, 64);
+        for (int i = lane; i < gn; i += 64) gkeys[o + i] = gbuf[i];
+        gn = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
+    for (unsigned long long c0 = beg; c0 < end; c0 += RC_CELL) {
+        const bool live = lane < RC_CELL && c0 + lane < end;
+        int p = -1, q = -1;
+        if (live) {
+            const int2 sl = cand[c0 + lane];
+            p = slot[sl.x];
+            q = slot[sl.y];
+        }
+        if (VEC4 && d <= 128) {
+            // row r = 2 i + (lane >> 5), float4 column lane & 31
+            const int half = lane >> 5, k4 = lane & 31, nk4 = d >> 2;
+            float4 v[RC_CELL];
 #pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const int cc = s ? b : a, q = s ? e.x : e.y;
-                    const float* cp = C + (size_t)cc * d;
-                    const float* qp = X + (size_t)q * d;
-                    float acc = 0.0f;   // the reference's order: sequential, not fused
-                    if (vec4) {
-                        for (int k = 0; k < d; k += 4) {
-                            const float4 u = *reinterpret_cast<const float4*>(cp + k);
-                            const float4 v = *reinterpret_cast<const float4*>(qp + k);
-                            float t = u.x - v.x;
-                            acc = acc + t * t;
-                            t = u.y - v.y;
-                            acc = acc + t * t;
-                            t = u.z - v.z;
-                            acc = acc + t * t;
-                            t = u.w - v.w;
-                            acc = acc + t * t;
-                        }
-                    } else {
-                        for (int k = 0; k < d; ++k) {
-                            const float t = cp[k] - qp[k];
-                            acc = acc + t * t;
-                        }
-                    }
-                    g[s] = sqrtf(acc) <= delta[q] + two_eps;
-                    cab[s] = cc;
-                    qv[s] = q;
-                }
-                if (!g[0] && !g[1]) flag[c] = 0;
+            for (int i = 0; i < RC_CELL; ++i) {
+                const int r = 2 * i + half;
+                const int src = r < RC_CELL ? r : r - RC_CELL;
+                const int id = __shfl(r < RC_CELL ? p : q, src, 64);
+                v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (id >= 0 && k4 < nk4) v[i] = *reinterpret_cast<const float4*>(X + (size_t)id * d + 4 * k4);
+            }
+#pragma unroll
+            for (int i = 0; i < RC_CELL; ++i) {
+                const int r = 2 * i + half;
+                if (k4 < nk4) *reinterpret_cast<float4*>(rows + r * ld + 4 * k4) = v[i];
+            }
+        } else {
+            for (int r = 0; r < 2 * RC_CELL; ++r) {
+                const int src = r < RC_CELL ? r : r - RC_CELL;
+                const int id = __shfl(r < RC_CELL ? p : q, src, 64);
+                if (id < 0) continue;
+                const float* row = X + (size_t)id * d;
+                for (int k = lane; k < d; k += 64) rows[r * ld + k] = row[k];
             }
         }
-        const unsigned long long mt = __ballot(tested);
-        if (mt && lane == 0) atomicAdd(&counters[4], 2ull * (unsigned long long)__popcll(mt));
+        __builtin_amdgcn_wave_barrier();   // this wave's staged rows (LDS is in order within a wave)
+        // three sequential sums per candidate i, one lane each, one
+        // instruction stream: lane i the edge distance (row p from global
+        // memory against row q in LDS), lane 16 + i center a against q, lane
+        // 32 + i center b against p (computed for every candidate across
+        // cells, needed only for its edges)
+        const int ci = lane & (RC_CELL - 1), role = lane / RC_CELL;
+        const int pi = __shfl(p, ci, 64), qi = __shfl(q, ci, 64);
+        const int a = live ? cell[p] : -1, b = live ? cell[q] : -1;
+        const int ai = __shfl(a, ci, 64), bi = __shfl(b, ci, 64);
+        (void)qi;
+        float acc = 0.0f;
+        if (pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
+            const float* u = role == 0 ? X + (size_t)pi * d : C + (size_t)(role == 1 ? ai : bi) * d;
+            const float* y = rows + (role == 2 ? ci : RC_CELL + ci) * ld;
+            if (VEC4) {
+                for (int k = 0; k < d; k += 4) {
+                    const float4 uv = *reinterpret_cast<const float4*>(u + k);
+                    const float4 yv = *reinterpret_cast<const float4*>(y + k);
+                    float t;
+                    t = uv.x - yv.x; acc = acc + t * t;
+                    t = uv.y - yv.y; acc = acc + t * t;
+                    t = uv.z - yv.z; acc = acc + t * t;
+                    t = uv.w - yv.w; acc = acc + t * t;
+                }
+            } else {
+                for (int k = 0; k < d; ++k) {
+                    const float t = u[k] - y[k];
+                    acc = acc + t * t;
+                }
+            }
+        }
+        const float sa = __shfl(acc, RC_CELL + ci, 64), sb = __shfl(acc, 2 * RC_CELL + ci, 64);
+        bool edge = false, tested = false, g0 = false, g1 = false;
+        if (live) {
+            const float* x = rows + lane * ld;              // p
+            const float* y = rows + (RC_CELL + lane) * ld;  // q
+            edge = acc <= T;
+            const float w = sqrtf(acc);
+            if (fabs((double)w - (double)eps) <= band) {
+                atomicAdd(&counters[2], 1ull);
+                double s2 = 0.0;
+                for (int k = 0; k < d; ++k) {
+                    const double t = (double)x[k] - (double)y[k];
+                    s2 += t * t;
+                }
+                if ((sqrt(s2) <= (double)eps) != edge) atomicAdd(&counters[3], 1ull);
+            }
+            if (edge && a != b) {
+                tested = true;
+                g0 = sqrtf(sa) <= delta[q] + two_eps;
+                g1 = sqrtf(sb) <= delta[p] + two_eps;
+                if (!g0 && !g1) edge = false;
+            }
+            ids[c0 + lane] = make_int2(min(p, q), max(p, q));
+            flag[c0 + lane] = edge ? 1 : 0;
+            cw[c0 + lane] = w;
+        }
+        __builtin_amdgcn_wave_barrier();   // every lane has read the staged rows
+        tests += 2ull * (unsigned long long)__popcll(__ballot(tested));
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const unsigned long long mg = __ballot(g[s]);
+            const bool gs = s ? g1 : g0;
+            const unsigned long long mg = __ballot(gs);
             if (mg) {
-                unsigned long long o = 0;
-                if (lane == 0) o = atomicAdd(&counters[5], (unsigned long long)__popcll(mg));
-                o = __shfl(o, 0, 64) +
-                    __builtin_amdgcn_mbcnt_hi((unsigned)(mg >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mg, 0u));
-                if (g[s]) gkeys[o] = ((uint64_t)(uint32_t)cab[s] << 32) | (uint32_t)qv[s];
+                const int k = __popcll(mg);
+                if (gn + k > GBUF) flush_ghosts();
+                const int o = gn + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mg >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((unsigned)mg, 0u));
+                if (gs) gbuf[o] = ((uint64_t)(uint32_t)(s ? b : a) << 32) | (uint32_t)(s ? p : q);
+                gn += k;
+                __builtin_amdgcn_wave_barrier();
             }
+        }
+        // per-cell counts: the lanes of each distinct row cell in turn
+        bool pending = live;
+        for (;;) {
+            const unsigned long long pm = __ballot(pending);
+            if (!pm) break;
+            const int a0 = __shfl(a, __builtin_ctzll(pm), 64);
+            const bool mine = pending && a == a0;
+            const unsigned nm = (unsigned)__popcll(__ballot(mine)), ne = (unsigned)__popcll(__ballot(mine && edge));
+            if (a0 != run_cell) {
+                if (lane == 0 && run_cell >= 0) {
+                    atomicAdd(&cc[2 * run_cell], run_cand);
+                    if (run_edges) atomicAdd(&cc[2 * run_cell + 1], run_edges);
+                }
+                run_cell = a0;
+                run_cand = run_edges = 0;
+            }
+            run_cand += nm;
+            run_edges += ne;
+            pending = pending && !mine;
+        }
+    }
+    flush_ghosts();
+    if (lane == 0) {
+        if (tests) atomicAdd(&counters[4], tests);
+        if (run_cell >= 0) {
+            atomicAdd(&cc[2 * run_cell], run_cand);
+            if (run_edges) atomicAdd(&cc[2 * run_cell + 1], run_edges);
         }
     }
 }
@@ -690,35 +832,6 @@ __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__
         feat[a * 4 + 1] = (float)(log2(1.0 + 0.5 * n + (double)scol[0]) / 24.0);
         feat[a * 4 + 2] = (float)(log2(1.0 + fmin(rr / e, 1e30)) / 8.0);
         feat[a * 4 + 3] = (float)(log2(1.0 + n * kk) / 24.0);
-    }
-}
-
-// Per-cell (candidates, edges) by the cell of the candidate's row point.
-__global__ void k_cell_counts(const int2* __restrict__ cand, const uint8_t* __restrict__ flag,
-                              const unsigned long long* __restrict__ counters, unsigned long long cap,
-                              const int* __restrict__ slot, const int* __restrict__ cell,
-                              unsigned* __restrict__ cc) {
-    const unsigned long long nc = min(counters[0], cap);
-    const int lane = threadIdx.x & 63;
-    for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
-         base += (unsigned long long)gridDim.x * blockDim.x) {
-        const unsigned long long c = base + threadIdx.x;
-        const bool live = c < nc;
-        const int a = live ? cell[slot[cand[c].x]] : -1;
-        const bool f = live && flag[c];
-        // candidates come grouped by row block, so a wave's rows usually share
-        // one cell: one pair of atomics for the wave then
-        const int a0 = __builtin_amdgcn_readfirstlane(a);
-        if (__all(!live || a == a0)) {
-            const unsigned long long ml = __ballot(live), mf = __ballot(f);
-            if (lane == 0 && ml) {   // a wave past the end has no live lane (a0 = -1)
-                atomicAdd(&cc[2 * a0], (unsigned)__popcll(ml));
-                if (mf) atomicAdd(&cc[2 * a0 + 1], (unsigned)__popcll(mf));
-            }
-        } else if (live) {
-            atomicAdd(&cc[2 * a], 1u);
-            if (f) atomicAdd(&cc[2 * a + 1], 1u);
-        }
     }
 }
 
@@ -944,6 +1057,415 @@ int64_t cell_work(const std::vector<int>& rs, const std::vector<unsigned>& adj, 
     return covered;
 }
 
+// Cells along a nearest-neighbour chain of their centers (greedy from cell
+// 0, each step to the closest unvisited center), so that cells next to each
+// other in the row layout are near each other: a row block spanning two
+// cells then needs the column tiles of two overlapping adjacency sets.
+int center_chain(rnng_voronoi_s* v, int m, int d) {
+    v->chain.assign(1, 0);
+    if (m <= 1) return RNNG_OK;
+    RNNG_TRY(v->cdist.reserve((size_t)m * m * sizeof(float)));
+    k_center_dist<<<m, 256, (size_t)d * sizeof(float), stream()>>>(v->C.as<float>(), m, d, v->cdist.as<float>());
+    RNNG_HIP_TRY(hipGetLastError());
+    std::vector<float> D;
+    RNNG_TRY(download(D, v->cdist, (size_t)m * m));
+    std::vector<char> seen((size_t)m, 0);
+    seen[0] = 1;
+    int cur = 0;
+    for (int k = 1; k < m; ++k) {
+        const float* row = D.data() + (size_t)cur * m;
+        int best = -1;
+        float bd = INFINITY;
+        for (int b = 0; b < m; ++b)
+            if (!seen[(size_t)b] && (best < 0 || row[b] < bd)) {
+                best = b;
+                bd = row[b];
+            }
+        seen[(size_t)best] = 1;
+        v->chain.push_back(best);
+        cur = best;
+    }
+    return RNNG_OK;
+}
+
+// The row layout of one cell order: positions in order, each cell's members
+// padded to 32 slots.
+struct CellLayout {
+    std::vector<int> perm;      // position -> cell
+    std::vector<int> pos;       // cell -> position
+    std::vector<int> rs;        // position prefix of padded sizes (m + 1)
+    std::vector<int> start;     // cell -> first slot
+    std::vector<int> tcell;     // tile -> position
+    std::vector<unsigned> adj;  // adjacency between positions (host work lists only)
+};
+
+// With adj (cell adjacency), also relabels it to positions for the host
+// work lists.
+void make_layout(const std::vector<int>& perm, const std::vector<int>& cnt, const std::vector<unsigned>* adj,
+                 int m, int words, CellLayout& L) {
+    L.perm = perm;
+    L.pos.assign((size_t)m, 0);
+    for (int k = 0; k < m; ++k) L.pos[(size_t)perm[(size_t)k]] = k;
+    std::vector<int> R((size_t)m);
+    for (int k = 0; k < m; ++k) R[(size_t)k] = (cnt[(size_t)perm[(size_t)k]] + 31) & ~31;
+    L.rs = prefix(R);
+    L.start.resize((size_t)m);
+    for (int c = 0; c < m; ++c) L.start[(size_t)c] = L.rs[(size_t)L.pos[(size_t)c]];
+    L.tcell.assign((size_t)std::max(1, L.rs[(size_t)m] / 32), 0);
+    for (int k = 0; k < m; ++k)
+        for (int t = L.rs[(size_t)k] / 32; t < L.rs[(size_t)k + 1] / 32; ++t) L.tcell[(size_t)t] = k;
+    L.adj.clear();
+    if (!adj) return;
+    L.adj.assign((size_t)m * words, 0u);
+    for (int k = 0; k < m; ++k) {
+        const unsigned* row = adj->data() + (size_t)perm[(size_t)k] * words;
+        unsigned* out = L.adj.data() + (size_t)k * words;
+        for (int w = 0; w < words; ++w)
+            for (unsigned bits = row[w]; bits; bits &= bits - 1) {
+                const int c = 32 * w + __builtin_ctz(bits);
+                const int j = L.pos[(size_t)c];
+                out[j >> 5] |= 1u << (j & 31);
+            }
+    }
+}
+
+// Tile-list work of the cell-ordered self join: for every row block, the
+// column tiles of the same needed cells as cell_work (its own cells from the
+// block's first tile on and every cell adjacent to one of them) concatenated
+// in ctl, cut into items of at most `seg` tiles: (row block, first, end)
+// index ctl.  Items in row-block order.  Returns the pairs the tiles cover.
+// (The host form of the device builder below, kept as its reference.)
+int64_t cell_work_tiles(const std::vector<int>& rs, const std::vector<unsigned>& adj, int words, int m, int bt,
+                        int n_rowblocks, int seg, std::vector<int4>& items, std::vector<int>& ctl) {
+    seg = std::min(std::max(seg, 1), kMaxListItem);   // the kernel holds an item's tiles in registers
+    const int ntiles = rs[m] / 32;
+    std::vector<int> tcell(std::max(1, ntiles));
+    for (int b = 0; b < m; ++b)
+        for (int t = rs[b] / 32; t < rs[b + 1] / 32; ++t) tcell[t] = b;
+    std::vector<unsigned> need(words);
+    items.clear();
+    ctl.clear();
+    int64_t covered = 0;
+    for (int rb = 0; rb < n_rowblocks; ++rb) {
+        const int t0 = rb * bt, t1 = std::min(ntiles, t0 + bt);
+        if (t0 >= ntiles) break;
+        std::fill(need.begin(), need.end(), 0u);
+        for (int a = tcell[t0]; a <= tcell[t1 - 1]; ++a) {
+            need[a >> 5] |= 1u << (a & 31);
+            const unsigned* row = adj.data() + (size_t)a * words;
+            for (int w = 0; w < words; ++w) need[w] |= row[w];
+        }
+        const int p0 = (int)ctl.size();
+        for (int w = tcell[t0] >> 5; w < words; ++w) {
+            unsigned bits = need[w];
+            if (w == (tcell[t0] >> 5)) bits &= ~0u << (tcell[t0] & 31);
+            for (; bits; bits &= bits - 1) {
+                const int b = 32 * w + __builtin_ctz(bits);
+                if (b >= m) break;
+                for (int t = std::max(rs[b] / 32, t0); t < rs[b + 1] / 32; ++t) ctl.push_back(t);
+            }
+        }
+        const int p1 = (int)ctl.size();
+        covered += (int64_t)(p1 - p0) * 32 * (t1 - t0) * 32;
+        for (int p = p0; p < p1; p += seg) items.push_back(make_int4(rb, p, std::min(p1, p + seg), 0));
+    }
+    return covered;
+}
+
+// Cost of a tile-list item of `tiles` column tiles whose row block has
+// rb_tiles tiles and rb_cand predicted candidates (float arithmetic, the same
+// on the host and the device).
+__host__ __device__ inline float item_cost(int tiles, float rb_cand, int rb_tiles) {
+    return rb_tiles > 0 ? (float)tiles + rb_cand * (float)tiles / (float)rb_tiles / 64.0f : (float)tiles;
+}
+
+// Items of a persistent queue, heaviest first: cost = column tiles + the
+// item's share (by tiles) of its row block's predicted candidates / 64 (a
+// tile's candidate path costs about as much as the tile itself at 64
+// candidates).  rb_cand[rb]: the cost model's candidate estimate of row block
+// rb (zeros without a model: longest items first).
+void order_by_cost(std::vector<int4>& items, const std::vector<float>& rb_cand) {
+    std::vector<int> rb_tiles(rb_cand.size(), 0);
+    for (const int4& w : items)
+        if ((size_t)w.x < rb_tiles.size()) rb_tiles[(size_t)w.x] += w.z - w.y;
+    std::vector<std::pair<float, int>> key(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const int4 w = items[i];
+        key[i] = {-item_cost(w.z - w.y, (size_t)w.x < rb_cand.size() ? rb_cand[(size_t)w.x] : 0.0f,
+                             (size_t)w.x < rb_tiles.size() ? rb_tiles[(size_t)w.x] : 0),
+                  (int)i};
+    }
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int4> out(items.size());
+    for (size_t i = 0; i < items.size(); ++i) out[i] = items[(size_t)key[i].second];
+    items.swap(out);
+}
+
+// ---- the cell filter's work lists, built on the device ----
+// The same lists as cell_work (runs) and cell_work_tiles (tile lists), from
+// the adjacency on the device and the layout's arrays: perm (position ->
+// cell), pos (cell -> position), rs (padded row starts by position, m + 1),
+// tcell (tile -> position).
+
+// Needed positions of row block rb: the cells of its tiles and every cell
+// adjacent to one of them, from the block's first cell on (position space).
+__global__ __launch_bounds__(256) void k_rb_need(const unsigned* __restrict__ adj, int words, int m,
+                                                 const int* __restrict__ perm, const int* __restrict__ pos,
+                                                 const int* __restrict__ tcell, int ntiles, int bt,
+                                                 unsigned* __restrict__ need) {
+    extern __shared__ unsigned nbits[];   // [words] by cell, [words] by position
+    unsigned* ucell = nbits;
+    unsigned* upos = nbits + words;
+    const int rb = blockIdx.x;
+    const int t0 = rb * bt, t1 = min(ntiles, t0 + bt);
+    const int k0 = tcell[t0], k1 = tcell[t1 - 1];
+    for (int w = threadIdx.x; w < words; w += 256) {
+        unsigned u = 0u;
+        for (int k = k0; k <= k1; ++k) u |= adj[(size_t)perm[k] * words + w];
+        ucell[w] = u;
+        upos[w] = 0u;
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < words; w += 256)
+        for (unsigned bits = ucell[w]; bits; bits &= bits - 1) {
+            const int c = 32 * w + __builtin_ctz(bits);
+            if (c < m) {
+                const int k = pos[c];
+                atomicOr(&upos[k >> 5], 1u << (k & 31));
+            }
+        }
+    for (int k = k0 + (int)threadIdx.x; k <= k1; k += 256) atomicOr(&upos[k >> 5], 1u << (k & 31));
+    __syncthreads();
+    for (int w = threadIdx.x; w < words; w += 256) {
+        unsigned u = upos[w];
+        if (w < (k0 >> 5)) u = 0u;
+        else if (w == (k0 >> 5)) u &= ~0u << (k0 & 31);
+        need[(size_t)rb * words + w] = u;
+    }
+}
+
+// The pieces of a row block's work in column chunk c (kFilterChunk tiles):
+// its runs of consecutive needed positions as tile ranges [lo, hi), clipped
+// to the chunk and to the block's first tile t0.
+template <class F>
+__device__ __forceinline__ void chunk_pieces(const unsigned* __restrict__ need, const int* __restrict__ rs,
+                                             const int* __restrict__ tcell, int ntiles, int t0, int c, F&& emit) {
+    const int lo_t = c * kFilterChunk, hi_t = min(ntiles, lo_t + kFilterChunk);
+    const int tl = max(lo_t, t0);
+    if (tl >= hi_t) return;
+    const int kS = tcell[tl], kE = tcell[hi_t - 1];
+    int lo = -1, hi = -1;
+    for (int k = kS; k <= kE; ++k) {
+        if ((need[k >> 5] >> (k & 31)) & 1u) {
+            if (lo < 0) lo = hi = max(rs[k] >> 5, tl);
+            hi = max(hi, min(rs[k + 1] >> 5, hi_t));
+        } else {
+            if (lo >= 0 && hi > lo) emit(lo, hi);
+            lo = -1;
+        }
+    }
+    if (lo >= 0 && hi > lo) emit(lo, hi);
+}
+
+// Pair i of (chunk, row block): runs (tiles_form 0, i = c * nrb + rb) count
+// their pieces, tile lists (i = rb * nch + c) their tiles; counts[P] = 0 for
+// the scan.  tot[0] += the pairs the pieces cover.
+__global__ __launch_bounds__(256) void k_piece_counts(const unsigned* __restrict__ need, int words,
+                                                      const int* __restrict__ rs, const int* __restrict__ tcell,
+                                                      int ntiles, int bt, int nrb, int nch, int tiles_form,
+                                                      int64_t* __restrict__ counts,
+                                                      unsigned long long* __restrict__ tot) {
+    __shared__ unsigned long long part[4];
+    const int64_t P = (int64_t)nrb * nch;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    unsigned long long cov = 0;
+    if (i < P) {
+        const int rb = tiles_form ? (int)(i / nch) : (int)(i % nrb);
+        const int c = tiles_form ? (int)(i % nch) : (int)(i / nrb);
+        const int t0 = rb * bt, rows = (min(ntiles, t0 + bt) - t0) * 32;
+        int64_t pieces = 0, tiles = 0;
+        chunk_pieces(need + (size_t)rb * words, rs, tcell, ntiles, t0, c, [&](int lo, int hi) {
+            ++pieces;
+            tiles += hi - lo;
+        });
+        counts[i] = tiles_form ? tiles : pieces;
+        cov = (unsigned long long)tiles * 32ull * (unsigned long long)rows;
+    } else if (i == P) {
+        counts[i] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) cov += __shfl_xor(cov, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cov;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long b = part[0] + part[1] + part[2] + part[3];
+        if (b) atomicAdd(&tot[0], b);
+    }
+}
+
+// Writes the pieces of pair i at off[i]: runs as items (rb, lo, hi, 0), tile
+// lists as their tiles in ctl.
+__global__ __launch_bounds__(256) void k_piece_write(const unsigned* __restrict__ need, int words,
+                                                     const int* __restrict__ rs, const int* __restrict__ tcell,
+                                                     int ntiles, int bt, int nrb, int nch, int tiles_form,
+                                                     const int64_t* __restrict__ off, int4* __restrict__ items,
+                                                     int* __restrict__ ctl) {
+    const int64_t P = (int64_t)nrb * nch;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= P) return;
+    const int rb = tiles_form ? (int)(i / nch) : (int)(i % nrb);
+    const int c = tiles_form ? (int)(i % nch) : (int)(i / nrb);
+    int64_t o = off[i];
+    chunk_pieces(need + (size_t)rb * words, rs, tcell, ntiles, rb * bt, c, [&](int lo, int hi) {
+        if (tiles_form) {
+            for (int t = lo; t < hi; ++t) ctl[o++] = t;
+        } else {
+            items[o++] = make_int4(rb, lo, hi, 0);
+        }
+    });
+}
+
+// Tile lists: items per row block (its tiles cut into kMaxListItem), icnt[nrb] = 0.
+__global__ void k_list_item_counts(const int64_t* __restrict__ off, int nrb, int nch, int64_t* __restrict__ icnt) {
+    const int rb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rb > nrb) return;
+    if (rb == nrb) {
+        icnt[rb] = 0;
+        return;
+    }
+    const int64_t T = off[(int64_t)(rb + 1) * nch] - off[(int64_t)rb * nch];
+    icnt[rb] = (T + kMaxListItem - 1) / kMaxListItem;
+}
+
+// Tile lists: the items of row block rb at ioff[rb]; with keys, the sort key
+// of the persistent queue's order (cost descending, then list order).
+__global__ void k_list_items(const int64_t* __restrict__ off, const int64_t* __restrict__ ioff, int nrb, int nch,
+                             const float* __restrict__ rbc, int4* __restrict__ items, uint64_t* __restrict__ keys) {
+    const int rb = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rb >= nrb) return;
+    const int64_t b0 = off[(int64_t)rb * nch];
+    const int T = (int)(off[(int64_t)(rb + 1) * nch] - b0);
+    int64_t it = ioff[rb];
+    for (int p = 0; p < T; p += kMaxListItem, ++it) {
+        const int e = min(T, p + kMaxListItem);
+        items[it] = make_int4(rb, (int)(b0 + p), (int)(b0 + e), 0);
+        if (keys) {
+            const float cost = item_cost(e - p, rbc ? rbc[rb] : 0.0f, T);
+            keys[it] = ((uint64_t)~__float_as_uint(cost) << 32) | (uint32_t)it;
+        }
+    }
+}
+
+__global__ void k_gather_items(const int4* __restrict__ src, const uint64_t* __restrict__ keys, int64_t n,
+                               int4* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(uint32_t)keys[i]];
+}
+
+// The launch list: every stride-th of the n items, in XCD order when xcd
+// (workgroup b runs on XCD b % 8 and takes item (b % 8) * per + b / 8 of the
+// selection, per = ceil(M / 8)); n_out slots, the ones past the selection
+// empty items.
+__global__ void k_items_select(const int4* __restrict__ src, int64_t n, int stride, int xcd, int64_t n_out,
+                               int4* __restrict__ dst) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_out) return;
+    const int64_t M = (n + stride - 1) / stride;
+    const int64_t sel = xcd ? (b % 8) * ((M + 7) / 8) + b / 8 : b;
+    dst[b] = sel < M ? src[sel * stride] : make_int4(0, 0, 0, 0);
+}
+
+int64_t launch_list_size(int64_t n, int stride, int xcd) {
+    const int64_t M = (n + stride - 1) / stride;
+    return xcd ? 8 * ((M + 7) / 8) : M;
+}
+
+// Builds the launch list of schedule sched (0 runs, 1 tile lists, 2 tile
+// lists by cost) for row blocks of bt tiles into out (n_out items; the tile
+// lists' tiles into v->ctl): every stride-th item, in XCD order if xcd (runs
+// and plain tile lists).  rbc: per row block predicted candidates (sched 2).
+// *covered = the pairs of every item (before the stride).
+int device_work_list(rnng_voronoi_s* v, int m, int words, int ntiles, int bt, int sched, int xcd, int stride,
+                     const std::vector<float>& rbc, DevBuf& out, int64_t* n_out, int64_t* covered) {
+    const int nrb = (ntiles + bt - 1) / bt;
+    const int nch = (ntiles + kFilterChunk - 1) / kFilterChunk;
+    const int64_t P = (int64_t)nrb * nch;
+    const int tiles_form = sched > 0 ? 1 : 0;
+    RNNG_TRY(v->wl_need.reserve((size_t)nrb * words * sizeof(unsigned)));
+    RNNG_TRY(v->wl_cnt.reserve((size_t)(P + 1) * sizeof(int64_t)));
+    RNNG_TRY(v->wl_off.reserve((size_t)(P + 1) * sizeof(int64_t)));
+    RNNG_TRY(v->wl_tot.reserve(4 * sizeof(int64_t)));
+    k_rb_need<<<nrb, 256, (size_t)2 * words * sizeof(unsigned), stream()>>>(
+        v->adj_dev.as<unsigned>(), words, m, v->lay_perm.as<int>(), v->lay_pos.as<int>(), v->lay_tcell.as<int>(),
+        ntiles, bt, v->wl_need.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(launch_fill_u64(v->wl_tot.as<unsigned long long>(), 1, 0ull));
+    k_piece_counts<<<(unsigned)((P + 1 + 255) / 256), 256, 0, stream()>>>(
+        v->wl_need.as<unsigned>(), words, v->lay_rs.as<int>(), v->lay_tcell.as<int>(), ntiles, bt, nrb, nch,
+        tiles_form, v->wl_cnt.as<int64_t>(), v->wl_tot.as<unsigned long long>());
+    RNNG_HIP_TRY(hipGetLastError());
+    RNNG_TRY(exclusive_scan_i64(v->wl_cnt.as<int64_t>(), v->wl_off.as<int64_t>(), P + 1));
+    if (tiles_form) {
+        RNNG_TRY(v->wl_icnt.reserve((size_t)(nrb + 1) * sizeof(int64_t)));
+        RNNG_TRY(v->wl_ioff.reserve((size_t)(nrb + 1) * sizeof(int64_t)));
+        k_list_item_counts<<<(nrb + 1 + 255) / 256, 256, 0, stream()>>>(v->wl_off.as<int64_t>(), nrb, nch,
+                                                                          v->wl_icnt.as<int64_t>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(exclusive_scan_i64(v->wl_icnt.as<int64_t>(), v->wl_ioff.as<int64_t>(), nrb + 1));
+    }
+    // totals: pieces or tiles, tile-list items, covered pairs
+    int64_t tot[3] = {0, 0, 0};
+    RNNG_HIP_TRY(hipMemcpyAsync(&tot[0], v->wl_off.as<int64_t>() + P, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                stream()));
+    if (tiles_form)
+        RNNG_HIP_TRY(hipMemcpyAsync(&tot[1], v->wl_ioff.as<int64_t>() + nrb, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    stream()));
+    RNNG_HIP_TRY(hipMemcpyAsync(&tot[2], v->wl_tot.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, stream()));
+    RNNG_HIP_TRY(hipStreamSynchronize(stream()));
+    *covered = tot[2];
+    if (tiles_form && tot[0] > INT32_MAX) {
+        set_error("cell tile lists: more than 2^31 tiles");
+        return RNNG_ERR_UNSUPPORTED;
+    }
+    const int64_t n_items = tiles_form ? tot[1] : tot[0];
+    RNNG_TRY(v->wl_stage.reserve((size_t)std::max<int64_t>(1, n_items) * sizeof(int4)));
+    if (tiles_form) RNNG_TRY(v->ctl.reserve((size_t)std::max<int64_t>(1, tot[0]) * sizeof(int)));
+    k_piece_write<<<(unsigned)((P + 255) / 256), 256, 0, stream()>>>(
+        v->wl_need.as<unsigned>(), words, v->lay_rs.as<int>(), v->lay_tcell.as<int>(), ntiles, bt, nrb, nch,
+        tiles_form, v->wl_off.as<int64_t>(), v->wl_stage.as<int4>(), v->ctl.as<int>());
+    RNNG_HIP_TRY(hipGetLastError());
+    const DevBuf* src = &v->wl_stage;
+    if (tiles_form) {
+        uint64_t* keys = nullptr;
+        if (sched == 2) {
+            RNNG_TRY(v->wl_keys.reserve((size_t)std::max<int64_t>(1, n_items) * sizeof(uint64_t)));
+            RNNG_TRY(upload(v->wl_rbc, rbc));
+            keys = v->wl_keys.as<uint64_t>();
+        }
+        k_list_items<<<(nrb + 255) / 256, 256, 0, stream()>>>(v->wl_off.as<int64_t>(), v->wl_ioff.as<int64_t>(), nrb,
+                                                               nch, rbc.empty() ? nullptr : v->wl_rbc.as<float>(),
+                                                               v->wl_stage.as<int4>(), keys);
+        RNNG_HIP_TRY(hipGetLastError());
+        if (sched == 2 && n_items > 0) {
+            RNNG_TRY(sort_keys_u64(keys, n_items, 64));
+            RNNG_TRY(v->wl_stage2.reserve((size_t)n_items * sizeof(int4)));
+            k_gather_items<<<(unsigned)((n_items + 255) / 256), 256, 0, stream()>>>(
+                v->wl_stage.as<int4>(), keys, n_items, v->wl_stage2.as<int4>());
+            RNNG_HIP_TRY(hipGetLastError());
+            src = &v->wl_stage2;
+        }
+    }
+    const int64_t no = launch_list_size(n_items, stride, sched == 2 ? 0 : xcd);
+    RNNG_TRY(out.reserve((size_t)std::max<int64_t>(1, no) * sizeof(int4)));
+    if (no > 0) {
+        k_items_select<<<(unsigned)((no + 255) / 256), 256, 0, stream()>>>(src->as<int4>(), n_items, stride,
+                                                                          sched == 2 ? 0 : xcd, no, out.as<int4>());
+        RNNG_HIP_TRY(hipGetLastError());
+    }
+    if (!tiles_form) RNNG_TRY(v->ctl.reserve(sizeof(int)));
+    *n_out = no;
+    return RNNG_OK;
+}
+
 int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st) {
     const int n = (int)v->ds->n, m = (int)v->m, d = v->ds->d;
     const int KS = mfma_ksteps(d);
@@ -975,21 +1497,12 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     k_cell_adjacency<<<m, 256, (size_t)d * sizeof(float), stream()>>>(
         v->C.as<float>(), m, d, cnt_dev.as<int>(), rad.as<unsigned>(), eps, words, adj_dev.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
-    std::vector<unsigned> adj;
-    RNNG_TRY(download(adj, adj_dev, (size_t)m * words));
-    // ---- cell-ordered slot layout and work list ----
-    std::vector<int> R(m);
-    for (int b = 0; b < m; ++b) R[b] = (v->cnt[b] + 31) & ~31;
-    const std::vector<int> rs = prefix(R);
-    const int n_pad = std::max(kPadRows, (rs[m] + kPadRows - 1) / kPadRows * kPadRows);
-    LocalBuf& d_rs = v->rs;
+    if (words > kMaxCellWords) {
+        set_error("Voronoi graph: at most 32 * 20000 cells");
+        return RNNG_ERR_UNSUPPORTED;
+    }
     LocalBuf& slot = v->slot;
     LocalBuf& d_items = v->items;
-    RNNG_TRY(upload(d_rs, rs));
-    RNNG_TRY(slot.reserve((size_t)n_pad * sizeof(int)));
-    RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
-    k_slot_map<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), d_rs.as<int>(), slot.as<int>());
-    RNNG_HIP_TRY(hipGetLastError());
     // ---- cost model: per-cell features -> predicted buffer sizes ----
     RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
     k_cell_features<<<m, 256, (size_t)(FS * d + FM * (d + 1)) * sizeof(float), stream()>>>(
@@ -998,6 +1511,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     RNNG_HIP_TRY(hipGetLastError());
     const float* model = cost_model_device();
     double pred_cand = 0.0, pred_edges = 0.0;
+    std::vector<double> pc((size_t)m, 0.0);   // predicted candidates per cell (0 without a model)
     if (model) {
         LocalBuf& yd = v->yd;
         RNNG_TRY(yd.reserve((size_t)m * 2 * sizeof(float)));
@@ -1005,24 +1519,55 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         std::vector<float> y;
         RNNG_TRY(download(y, yd, (size_t)m * 2));
         for (int a = 0; a < m; ++a) {
-            pred_cand += std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
+            pc[(size_t)a] = std::exp2(24.0 * std::min(std::max((double)y[2 * a], 0.0), 1.5)) - 1.0;
+            pred_cand += pc[(size_t)a];
             pred_edges += std::exp2(24.0 * std::min(std::max((double)y[2 * a + 1], 0.0), 1.5)) - 1.0;
         }
         // a non-finite output (NaN passes the clamps) sizes nothing: count-then-fill
         if (!std::isfinite(pred_cand) || !std::isfinite(pred_edges)) {
             model = nullptr;
             pred_cand = pred_edges = 0.0;
+            std::fill(pc.begin(), pc.end(), 0.0);
         }
     }
 
     RNNG_HIP_TRY(hipEventRecord(ev.e[1], stream()));
-    // ---- BF16 fragments of the cell-ordered points ----
+    // ---- cell order: the row layout, its slot map and the fragments ----
     MfmaPrep& P = v->prep;
-    RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
+    CellLayout L;
+    int n_pad = 0, layout_order = -1;
+    auto apply_layout = [&](int order) -> int {
+        if (order == layout_order) return RNNG_OK;
+        std::vector<int> perm((size_t)m);
+        if (order == 1) {
+            if ((int)v->chain.size() != m) RNNG_TRY(center_chain(v, m, d));
+            perm = v->chain;
+        } else {
+            std::iota(perm.begin(), perm.end(), 0);
+        }
+        make_layout(perm, v->cnt, nullptr, m, words, L);
+        n_pad = std::max(kPadRows, (L.rs[m] + kPadRows - 1) / kPadRows * kPadRows);
+        RNNG_TRY(upload(v->start, L.start));
+        RNNG_TRY(upload(v->lay_perm, L.perm));
+        RNNG_TRY(upload(v->lay_pos, L.pos));
+        RNNG_TRY(upload(v->lay_rs, L.rs));
+        RNNG_TRY(upload(v->lay_tcell, L.tcell));
+        RNNG_TRY(slot.reserve((size_t)n_pad * sizeof(int)));
+        RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
+        k_slot_map_at<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), v->start.as<int>(),
+                                               slot.as<int>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
+        layout_order = order;
+        return RNNG_OK;
+    };
+    RNNG_TRY(apply_layout(v->order >= 0 ? v->order : 0));
     RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
+    const auto host0 = std::chrono::steady_clock::now();
 
     LocalBuf& counters = v->counters;
     RNNG_TRY(counters.reserve(8 * sizeof(unsigned long long)));
+    RNNG_TRY(v->queue.reserve(sizeof(unsigned)));
     auto filter_args = [&](const DevBuf& work, size_t n_items, int2* cand_ptr, unsigned long long cap_) {
         FilterArgs fa;
         fa.xa = P.xa.as<bf16x8_t>();
@@ -1041,9 +1586,33 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         return fa;
     };
     // the filter over a work list: the exact int8 kernel when the prep built
-    // int8 forms, else the BF16 variant var
-    auto run_filter = [&](const DevBuf& work, size_t n_items, int2* cand_ptr, unsigned long long cap_,
-                          int var) -> int {
+    // int8 forms, else the BF16 variant var; runs of column tiles (sched 0)
+    // or tile lists ctl (1; 2 on a persistent queue)
+    auto run_filter = [&](int sched_, const DevBuf& work, size_t n_items, const DevBuf& ctl_, int2* cand_ptr,
+                          unsigned long long cap_, int var) -> int {
+        if (sched_ > 0 || sched_ == -2) {
+            TileFilterArgs ta;
+            ta.dry_run = sched_ == -2;
+            ta.i8 = P.i8;
+            ta.xa = P.xa.ptr;
+            ta.yb = P.yb.ptr;
+            ta.rterm = P.xn.ptr;
+            ta.cterm = P.ycn.ptr;
+            ta.work = work.as<int4>();
+            ta.n_items = (unsigned)n_items;
+            ta.ctl = ctl_.as<int>();
+            ta.persistent = sched_ == 2 || sched_ == -2;
+            ta.queue = v->queue.as<unsigned>();
+            ta.n_rows = n_pad;
+            ta.n_cols = n_pad;
+            ta.self_join = 1;
+            ta.cand = cand_ptr;
+            ta.cap = cap_;
+            ta.counter = counters.as<unsigned long long>();
+            ta.ks = P.i8 ? P.KS8 : KS;
+            ta.variant = var;
+            return filter_dispatch_tiles(ta);
+        }
         if (P.i8) {
             I8Args ia;
             ia.xa = P.xa.as<i32x4_t>();
@@ -1062,50 +1631,90 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         }
         return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
     };
-    auto work_list = [&](int var, int xcd, std::vector<int4>& out) {
-        const int bt = P.i8 ? filter_i8_row_tiles(cell_i8_code()) : filter_block_tiles(var);
-        const int64_t cov = cell_work(rs, adj, words, m, bt, (rs[m] + bt * 32 - 1) / (bt * 32), out);
-        if (xcd) xcd_permute(out);
-        return cov;
+    // predicted candidates of every row block (the persistent queue's order)
+    auto rb_cands = [&](int bt) {
+        std::vector<float> out((size_t)((L.rs[m] + bt * 32 - 1) / (bt * 32)), 0.0f);
+        std::vector<double> acc(out.size(), 0.0);
+        for (int k = 0; k < m; ++k) {
+            const int c = L.perm[(size_t)k];
+            const int tiles = (L.rs[k + 1] - L.rs[k]) / 32;
+            for (int t = L.rs[k] / 32; t < L.rs[k + 1] / 32; ++t) acc[(size_t)(t / bt)] += pc[(size_t)c] / tiles;
+        }
+        for (size_t i = 0; i < out.size(); ++i) out[i] = (float)acc[i];
+        return out;
     };
-    // ---- autotuner: every (variant, XCD order) on every 8th work item ----
+    // the launch list of a schedule on the current layout, built on the device
+    auto work_list = [&](int sched_, int var, int xcd, int stride, DevBuf& out, int64_t* n_out) -> int64_t {
+        const int bt = P.i8 ? filter_i8_row_tiles(cell_i8_code()) : filter_block_tiles(var);
+        int64_t cov = 0;
+        const std::vector<float> rbc = sched_ == 2 ? rb_cands(bt) : std::vector<float>();
+        const int rc = device_work_list(v, m, words, L.rs[m] / 32, bt, sched_, xcd, stride, rbc, out, n_out, &cov);
+        return rc != RNNG_OK ? -1 : cov;
+    };
+    // ---- autotuner: cell order x work form x filter shape, on every 8th item ----
     if (v->tuned_variant < 0) {
         v->tuned_variant = filter_default_variant();
         v->tuned_xcd = 1;
-        // the int8 filter has one default shape; the tuner times BF16 variants
-        if (!P.i8 && (v->autotune >= 0 ? v->autotune : tuning_enabled())) {
-            double best = 1e30;
-            int k = 0;
-            for (int var : kTuneVariants) {
-                for (int xcd = 0; xcd < 2; ++xcd, ++k) {
-                    std::vector<int4> full, sub;
-                    work_list(var, 0, full);
-                    for (size_t i = 0; i < full.size(); i += 8) sub.push_back(full[i]);
-                    if (xcd) xcd_permute(sub);
-                    RNNG_TRY(upload(v->tune_items, sub));
-                    double t = 0.0;
-                    for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
-                        RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
-                        RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
-                        RNNG_TRY(run_filter(v->tune_items, sub.size(), nullptr, 0ull, var));
-                        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
-                        RNNG_HIP_TRY(hipEventSynchronize(ev.e[4]));
-                        t = ev.ms(3, 4);
-                    }
-                    v->tune_ms[k] = t;
-                    if (t < best) {
-                        best = t;
-                        v->tuned_variant = var;
-                        v->tuned_xcd = xcd;
+        v->order = v->forced_order >= 0 ? v->forced_order : 0;
+        v->sched = v->forced_sched >= 0 ? v->forced_sched : 0;
+        const bool forced = v->forced_order >= 0 || v->forced_sched >= 0;
+        if (!forced && (v->autotune >= 0 ? v->autotune : tuning_enabled())) {
+            struct Cfg {
+                int order, sched, var, xcd;
+            };
+            std::vector<Cfg> cfgs;
+            for (int order = 0; order < 2; ++order) {
+                if (P.i8) {
+                    cfgs.push_back({order, 0, 0, 1});
+                    cfgs.push_back({order, 1, 0, 1});
+                    cfgs.push_back({order, 2, 0, 0});
+                } else {
+                    for (int var : kTuneVariants)
+                        for (int xcd = 0; xcd < 2; ++xcd) cfgs.push_back({order, 0, var, xcd});
+                    for (int var : {17, 13}) {
+                        cfgs.push_back({order, 1, var, 1});
+                        cfgs.push_back({order, 2, var, 0});
                     }
                 }
             }
+            v->sched_cfg.clear();
+            v->sched_ms.clear();
+            double best = 1e30;
+            for (size_t k = 0; k < cfgs.size(); ++k) {
+                const Cfg c = cfgs[k];
+                RNNG_TRY(apply_layout(c.order));
+                int64_t n_sub = 0;
+                if (work_list(c.sched, c.var, c.xcd, 8, v->tune_items, &n_sub) < 0) return RNNG_ERR_HIP;
+                double t = 0.0;
+                for (int rep = 0; rep < 2; ++rep) {   // first run warms caches and code
+                    RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
+                    RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
+                    RNNG_TRY(run_filter(c.sched, v->tune_items, (size_t)n_sub, v->ctl, nullptr, 0ull, c.var));
+                    RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
+                    RNNG_HIP_TRY(hipEventSynchronize(ev.e[4]));
+                    t = ev.ms(3, 4);
+                }
+                v->sched_cfg.push_back(c.order << 8 | c.sched << 4 | (c.xcd & 1));
+                v->sched_ms.push_back(t);
+                if (!P.i8 && c.order == 0 && c.sched == 0 && k < 8) v->tune_ms[k] = t;
+                if (t < best) {
+                    best = t;
+                    v->order = c.order;
+                    v->sched = c.sched;
+                    v->tuned_xcd = c.xcd;
+                    if (!P.i8) v->tuned_variant = c.var;
+                }
+            }
         }
+        if (!P.i8 && v->sched > 0 && v->tuned_variant != 17 && v->tuned_variant != 13) v->tuned_variant = 17;
+        RNNG_TRY(apply_layout(v->order));
     }
     const int variant = v->tuned_variant;
-    std::vector<int4> items;
-    const int64_t covered = work_list(variant, v->tuned_xcd, items);
-    RNNG_TRY(upload(d_items, items));
+    const int sched = v->sched;
+    int64_t n_launch = 0;
+    const int64_t covered = work_list(sched, variant, v->tuned_xcd, 1, d_items, &n_launch);
+    if (covered < 0) return RNNG_ERR_HIP;
+    const size_t n_items = (size_t)n_launch;
 
     LocalBuf& cand = v->cand;
     LocalBuf& cid = v->cid;
@@ -1132,29 +1741,23 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         RNNG_TRY(v->counts.reserve((size_t)m * 2 * sizeof(unsigned)));
         return RNNG_OK;
     };
-    // The fill pass: filter into capacity c, then candidate ids, FP32 recheck,
-    // reference semantics and per-cell counts, all reading the candidate count
-    // from the device (grids are sized by c), so the pass can be captured.
+    // The fill pass: filter into capacity c, then one kernel for candidate ids,
+    // FP32 recheck, reference semantics and per-cell counts, reading the
+    // candidate count from the device (its grid is sized by c), so the pass
+    // can be captured.
+    const bool vec4 = d % 4 == 0 && ((uintptr_t)v->ds->X % 16) == 0 && ((uintptr_t)v->C.ptr % 16) == 0;
     auto fill_pass = [&](unsigned long long c) -> int {
         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
-        RNNG_TRY(run_filter(d_items, items.size(), cand.as<int2>(), c, variant));
-        const unsigned eg =
-            (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(8192, (c + 255) / 256));
-        k_cand_ids<<<eg, 256, 0, stream()>>>(cand.as<int2>(), counters.as<unsigned long long>(), c, slot.as<int>(),
-                                             cid.as<int2>());
-        RNNG_HIP_TRY(hipGetLastError());
-        RNNG_TRY(launch_recheck(v->ds->X, v->ds->X, d, cid.as<int2>(), c, c, T, eps, cflag.as<uint8_t>(),
-                                cw.as<float>(), counters.as<unsigned long long>()));
-        const int rs_vec4 = (d % 4 == 0 && ((uintptr_t)v->ds->X % 16) == 0 && ((uintptr_t)v->C.ptr % 16) == 0) ? 1 : 0;
-        k_ref_semantics<<<eg, 256, 0, stream()>>>(v->ds->X, v->C.as<float>(), d, v->cell.as<int>(),
-                                                  v->delta.as<float>(), eps, cid.as<int2>(), cflag.as<uint8_t>(), c,
-                                                  counters.as<unsigned long long>(), v->gkeys.as<uint64_t>(),
-                                                  rs_vec4);
-        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(run_filter(sched, d_items, n_items, v->ctl, cand.as<int2>(), c, variant));
         RNNG_TRY(launch_fill_u32(v->counts.as<unsigned>(), 2 * m, 0u));
-        k_cell_counts<<<eg, 256, 0, stream()>>>(cand.as<int2>(), cflag.as<uint8_t>(),
-                                                counters.as<unsigned long long>(), c, slot.as<int>(),
-                                                v->cell.as<int>(), v->counts.as<unsigned>());
+        const unsigned rg =
+            (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(2048, (c + 63) / 64));
+        const size_t lds = (size_t)4 * 2 * RC_CELL * cell_rows_ld(d) * sizeof(float);
+        auto kern = vec4 ? k_cell_recheck<true> : k_cell_recheck<false>;
+        kern<<<rg, 256, lds, stream()>>>(v->ds->X, v->C.as<float>(), d, cand.as<int2>(), c, slot.as<int>(),
+                                         v->cell.as<int>(), v->delta.as<float>(), T, eps, cid.as<int2>(),
+                                         cflag.as<uint8_t>(), cw.as<float>(), counters.as<unsigned long long>(),
+                                         v->gkeys.as<uint64_t>(), v->counts.as<unsigned>());
         RNNG_HIP_TRY(hipGetLastError());
         return RNNG_OK;
     };
@@ -1168,6 +1771,10 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         (void)hipGetLastError();
         cap = 0;
     }
+    // the persistent grid's size comes from an occupancy query, made here
+    // rather than inside a stream capture
+    if (sched == 2) RNNG_TRY(run_filter(-2, d_items, n_items, v->ctl, nullptr, 0ull, variant));
+    const double host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
     if (model && cap > 0) {
         // one pass sized by the model; captured once and replayed while the
         // buffers, the work list and eps are unchanged
@@ -1177,11 +1784,13 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         const std::vector<uint64_t> key = {
             (uint64_t)(uintptr_t)cand.ptr, (uint64_t)(uintptr_t)cid.ptr,  (uint64_t)(uintptr_t)cflag.ptr,
             (uint64_t)(uintptr_t)cw.ptr,   (uint64_t)(uintptr_t)v->gkeys.ptr, (uint64_t)(uintptr_t)v->counts.ptr,
-            (uint64_t)(uintptr_t)d_items.ptr, (uint64_t)items.size(), (uint64_t)(uintptr_t)P.xa.ptr,
+            (uint64_t)(uintptr_t)d_items.ptr, (uint64_t)n_items, (uint64_t)(uintptr_t)P.xa.ptr,
             (uint64_t)(uintptr_t)P.yb.ptr, (uint64_t)(uintptr_t)P.xn.ptr, (uint64_t)(uintptr_t)P.ycn.ptr,
             (uint64_t)(uintptr_t)P.params.ptr, (uint64_t)(uintptr_t)slot.ptr,
             (uint64_t)(uintptr_t)counters.ptr, (uint64_t)(uintptr_t)v->ds->X, (uint64_t)(uintptr_t)v->C.ptr,
-            (uint64_t)(uintptr_t)v->cell.ptr, (uint64_t)(uintptr_t)v->delta.ptr, cap, (uint64_t)variant,
+            (uint64_t)(uintptr_t)v->cell.ptr, (uint64_t)(uintptr_t)v->delta.ptr, (uint64_t)(uintptr_t)v->ctl.ptr,
+            (uint64_t)(uintptr_t)v->queue.ptr, (uint64_t)sched, (uint64_t)v->order,
+            cap, (uint64_t)variant,
             (uint64_t)n_pad, (uint64_t)eb, (uint64_t)tb, (uint64_t)m, (uint64_t)d};
         if (v->use_graph >= 0 ? v->use_graph : graph_enabled()) {
             if (!v->fill_exec || v->fill_key != key) {
@@ -1218,7 +1827,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         // count-then-fill: a counting pass (capacity 0, filter only)
         RNNG_TRY(launch_fill_u64(counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
-        RNNG_TRY(run_filter(d_items, items.size(), nullptr, 0ull, variant));
+        RNNG_TRY(run_filter(sched, d_items, n_items, v->ctl, nullptr, 0ull, variant));
         RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
         ++passes;
         RNNG_TRY(read_count());
@@ -1291,6 +1900,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         st->filter_ms += ev.ms(3, 4);   // the fill pass (filter through per-cell counts)
         st->recheck_ms += ev.ms(4, 5);  // compaction and ghost list
         st->total_ms += ev.ms(0, 2) + ev.ms(3, 5);
+        st->host_ms += host_ms;
         (void)graphed;
     }
     return RNNG_OK;
@@ -1458,8 +2068,10 @@ int rnng_voronoi_graph_build(rnng_voronoi_t v, float eps, int32_t mode, rnng_gra
         set_error("voronoi_graph_build: the batched cell query runs in MFMA mode");
         return RNNG_ERR_UNSUPPORTED;
     }
+    const auto t0 = std::chrono::steady_clock::now();
     auto* g = new rnng_graph_s();
     const int s = voronoi_graph(v, eps, g, st);
+    if (st) st->call_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (s != RNNG_OK) {
         g->ij.release();
         g->w.release();
@@ -1497,6 +2109,122 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
         v->fill_key.clear();
     }
     if (v->autotune >= 0) v->tuned_variant = -1;   // choose again at the next build
+    return RNNG_OK;
+}
+
+namespace {
+
+// The work lists of one layout by the host builder (device 0) or the device
+// builder (device 1, stride 1, no XCD order).
+int debug_cell_work(const int32_t* cnt, int32_t m, const uint32_t* adj, const int32_t* perm, int32_t bt,
+                    int32_t sched, const float* rb_cand, int32_t device, int32_t* start, int32_t* items,
+                    int64_t cap_items, int32_t* ctl, int64_t cap_ctl, int64_t* n_items, int64_t* n_ctl,
+                    int64_t* covered) {
+    if (!cnt || !adj || !perm || m <= 0 || bt <= 0 || sched < 0 || sched > 2 || !n_items || !n_ctl || !covered) {
+        set_error("rnng_debug_cell_work: bad arguments");
+        return RNNG_ERR_ARG;
+    }
+    const int words = (m + 31) / 32;
+    std::vector<int> cn(cnt, cnt + m), pm(perm, perm + m);
+    std::vector<unsigned> ad(adj, adj + (size_t)m * words);
+    CellLayout L;
+    make_layout(pm, cn, &ad, m, words, L);
+    const int nrb = (L.rs[m] + bt * 32 - 1) / (bt * 32);
+    std::vector<float> rbc;
+    if (sched == 2) rbc = rb_cand ? std::vector<float>(rb_cand, rb_cand + nrb) : std::vector<float>((size_t)nrb, 0.0f);
+    std::vector<int4> it;
+    std::vector<int> cl;
+    int64_t cov = 0;
+    if (device) {
+        if (L.rs[m] == 0 || words > kMaxCellWords) {
+            set_error("rnng_debug_cell_work: no tiles, or too many cells");
+            return RNNG_ERR_ARG;
+        }
+        RNNG_TRY(ensure_init());
+        rnng_voronoi_s tmp;
+        RNNG_TRY(upload(tmp.adj_dev, ad));
+        RNNG_TRY(upload(tmp.lay_perm, L.perm));
+        RNNG_TRY(upload(tmp.lay_pos, L.pos));
+        RNNG_TRY(upload(tmp.lay_rs, L.rs));
+        RNNG_TRY(upload(tmp.lay_tcell, L.tcell));
+        LocalBuf out;
+        int64_t no = 0;
+        RNNG_TRY(device_work_list(&tmp, m, words, L.rs[m] / 32, bt, sched, 0, 1, rbc, out, &no, &cov));
+        RNNG_TRY(download(it, out, (size_t)no));
+        if (sched > 0) {
+            int64_t nt = 0;
+            for (const int4& w : it) nt = std::max<int64_t>(nt, w.z);
+            RNNG_TRY(download(cl, tmp.ctl, (size_t)nt));
+        }
+        const int rc = hipStreamSynchronize(stream()) == hipSuccess ? RNNG_OK : RNNG_ERR_HIP;
+        tmp.adj_dev.release();
+        RNNG_TRY(rc);
+    } else {
+        if (sched == 0) cov = cell_work(L.rs, L.adj, words, m, bt, nrb, it);
+        else cov = cell_work_tiles(L.rs, L.adj, words, m, bt, nrb, kFilterChunk, it, cl);
+        if (sched == 2) order_by_cost(it, rbc);
+    }
+    *n_items = (int64_t)it.size();
+    *n_ctl = (int64_t)cl.size();
+    *covered = cov;
+    if (start)
+        for (int c = 0; c < m; ++c) start[c] = L.start[(size_t)c];
+    if ((int64_t)it.size() > cap_items || (int64_t)cl.size() > cap_ctl) {
+        set_error("rnng_debug_cell_work: output capacity too small");
+        return RNNG_ERR_CAPACITY;
+    }
+    for (size_t i = 0; i < it.size(); ++i) {
+        items[4 * i] = it[i].x;
+        items[4 * i + 1] = it[i].y;
+        items[4 * i + 2] = it[i].z;
+        items[4 * i + 3] = it[i].w;
+    }
+    for (size_t i = 0; i < cl.size(); ++i) ctl[i] = cl[i];
+    return RNNG_OK;
+}
+
+}  // namespace
+
+int rnng_debug_cell_work(const int32_t* cnt, int32_t m, const uint32_t* adj, const int32_t* perm, int32_t bt,
+                         int32_t sched, const float* rb_cand, int32_t* start, int32_t* items, int64_t cap_items,
+                         int32_t* ctl, int64_t cap_ctl, int64_t* n_items, int64_t* n_ctl, int64_t* covered) {
+    return debug_cell_work(cnt, m, adj, perm, bt, sched, rb_cand, 0, start, items, cap_items, ctl, cap_ctl, n_items,
+                           n_ctl, covered);
+}
+
+int rnng_debug_cell_work_device(const int32_t* cnt, int32_t m, const uint32_t* adj, const int32_t* perm, int32_t bt,
+                                int32_t sched, const float* rb_cand, int32_t* start, int32_t* items,
+                                int64_t cap_items, int32_t* ctl, int64_t cap_ctl, int64_t* n_items, int64_t* n_ctl,
+                                int64_t* covered) {
+    return debug_cell_work(cnt, m, adj, perm, bt, sched, rb_cand, 1, start, items, cap_items, ctl, cap_ctl, n_items,
+                           n_ctl, covered);
+}
+
+int rnng_voronoi_set_schedule(rnng_voronoi_t v, int32_t order, int32_t sched) {
+    RNNG_TRY(check_voronoi(v));
+    if (order > 1 || sched > 2) {
+        set_error("rnng_voronoi_set_schedule: order 0..1, sched 0..2 (-1: the autotuner's)");
+        return RNNG_ERR_ARG;
+    }
+    v->forced_order = order < 0 ? -1 : order;
+    v->forced_sched = sched < 0 ? -1 : sched;
+    v->tuned_variant = -1;   // lay out and choose again at the next build
+    v->order = -1;
+    v->sched = -1;
+    return RNNG_OK;
+}
+
+int rnng_voronoi_schedule(rnng_voronoi_t v, int32_t* order, int32_t* sched, int32_t* configs, double* config_ms,
+                          int32_t cap, int32_t* n_configs) {
+    RNNG_TRY(check_voronoi(v));
+    if (order) *order = v->order;
+    if (sched) *sched = v->sched;
+    const int nc = (int)v->sched_cfg.size();
+    if (n_configs) *n_configs = nc;
+    for (int k = 0; k < std::min(nc, (int)cap); ++k) {
+        if (configs) configs[k] = v->sched_cfg[(size_t)k];
+        if (config_ms) config_ms[k] = v->sched_ms[(size_t)k];
+    }
     return RNNG_OK;
 }
 

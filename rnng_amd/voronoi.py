@@ -102,6 +102,22 @@ class Voronoi:
         check(_lib.lib().rnng_voronoi_tuning(self._h, C.byref(var), C.byref(xcd), _vp(ms)))
         return var.value, xcd.value, ms
 
+    def set_schedule(self, order: int = -1, sched: int = -1) -> None:
+        """Force the cell order (0 landmark order, 1 center chain) and work form
+        (0 runs, 1 tile lists, 2 tile lists on a persistent queue); -1: tuner."""
+        check(_lib.lib().rnng_voronoi_set_schedule(self._h, int(order), int(sched)))
+
+    def schedule(self):
+        """(order, form, [(order, form, xcd, ms) of every tuned configuration])."""
+        o, s, nc = C.c_int32(), C.c_int32(), C.c_int32()
+        cfg = np.zeros(64, np.int32)
+        ms = np.zeros(64, np.float64)
+        check(_lib.lib().rnng_voronoi_schedule(self._h, C.byref(o), C.byref(s), _vp(cfg), _vp(ms), 64,
+                                               C.byref(nc)))
+        k = min(nc.value, 64)
+        return o.value, s.value, [(int(c) >> 8, (int(c) >> 4) & 15, int(c) & 1, float(t))
+                                  for c, t in zip(cfg[:k], ms[:k])]
+
     def ghosts(self) -> np.ndarray:
         """(cell, point) pairs of the last graph build, sorted."""
         n = C.c_int64()

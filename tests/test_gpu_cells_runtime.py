@@ -77,3 +77,55 @@ def test_autotuner_choice_keeps_results(orc):
     assert np.array_equal(ij0, ij1) and w0.tobytes() == w1.tobytes() and np.array_equal(gh0, gh1)
     v.close()
     ds.close()
+
+
+def _int_data(n=12000, d=128):
+    from rnng_amd import synth
+    pts, _ = synth.points_gmm_int(n=n, d=d, k=24, seed=5)
+    return pts, synth.radius_for_degree(pts, 20.0)
+
+
+@pytest.mark.parametrize("kind", ["float", "int"])
+def test_every_schedule_gives_the_oracle_edges(orc, kind):
+    """Cell order (landmark order, center chain) x work form (runs of column
+    tiles, tile lists, tile lists on a persistent queue), forced one at a
+    time, eager and with the captured fill pass (cost model on): every build
+    equals the all-pairs oracle bit for bit, with the same ghosts."""
+    from rnng_amd import costmodel, l2, voronoi
+    pts, eps = _data(seed=4, n=24000, d=48) if kind == "float" else _int_data()
+    ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
+    ds = l2.Dataset(pts)
+    v = voronoi.Voronoi(ds, fps=96, first=0)
+    p = costmodel.load()
+    gh0 = None
+    try:
+        for model in ([None, p] if p is not None else [None]):
+            costmodel.set_cost_model(model)
+            for order in (0, 1):
+                for sched in (0, 1, 2):
+                    for use_graph in ((0, 1) if model is not None else (0,)):
+                        flush_print(f"{kind} model={model is not None} order={order} sched={sched} "
+                                    f"graph={use_graph}")
+                        v.set_options(use_graph=use_graph, autotune=0)
+                        v.set_schedule(order, sched)
+                        for rep in range(2 if use_graph else 1):
+                            ij, w, gh, st = _edges(v, eps)
+                            assert v.schedule()[:2] == (order, sched)
+                            assert st.int8_filter == (1 if kind == "int" else 0)
+                            assert np.array_equal(ij, ij_ref) and w.tobytes() == w_ref.tobytes()
+                            if gh0 is None:
+                                gh0 = gh
+                            assert np.array_equal(gh, gh0)
+        costmodel.set_cost_model(None)
+        v.set_schedule(-1, -1)
+        v.set_options(use_graph=0, autotune=1)
+        ij, w, gh, _ = _edges(v, eps)
+        order, sched, cfgs = v.schedule()
+        flush_print(f"tuned: order {order} form {sched} "
+                    + " ".join(f"{o}/{s}/{x}:{t:.3f}" for o, s, x, t in cfgs))
+        assert len(cfgs) >= 6 and all(t > 0 for *_, t in cfgs)
+        assert np.array_equal(ij, ij_ref) and w.tobytes() == w_ref.tobytes() and np.array_equal(gh, gh0)
+    finally:
+        costmodel.set_cost_model(None)
+        v.close()
+        ds.close()
