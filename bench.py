@@ -263,6 +263,31 @@ def _secondary_line(metric, value, unit, K, warmup, ms, workload, extra):
     return out
 
 
+# int32 VALU lane-operations per second: 256 CUs x 128 lanes x 2.4 GHz, the
+# rate behind the 157.3 TFLOP/s FP32 figure (an FMA counted as 2 flops); the
+# verify kernel's counters show 66 lane-ops per CU-clock, so the VALU issues
+# more than 64 lanes per clock
+PEAK_VALU_INT32_TOPS = 78.64
+# VALU instructions per text column in the banded bit-parallel verify
+# (k <= 15: 32-bit band words, two code bit-planes), counted in its gfx950 code
+VERIFY_OPS_PER_COLUMN = 39
+
+
+def _verify_roofline(st, K):
+    """The banded bit-parallel verify (k_edit_verify_bp): its text columns
+    (counted by the kernel over the candidates inside the length window) at
+    VERIFY_OPS_PER_COLUMN lane-operations each, over its HIP-event time."""
+    if st.kernel_units <= 0 or st.kernel_ms <= 0:
+        return None
+    ops = st.kernel_units / K * VERIFY_OPS_PER_COLUMN
+    ach = ops / (st.kernel_ms / K * 1e-3) / 1e12
+    return {"kernel": "k_edit_verify_bp<uint32, 2 planes>", "bound": "valu (int32)", "achieved": ach,
+            "peak": PEAK_VALU_INT32_TOPS, "unit": "Tops/s (int32 lane-ops)", "frac": ach / PEAK_VALU_INT32_TOPS,
+            "traffic": None, "columns": st.kernel_units // K, "ms": st.kernel_ms / K,
+            "work": f"{VERIFY_OPS_PER_COLUMN} VALU lane-ops per text column of each candidate inside the "
+                    "length window"}
+
+
 def bench_edit(args):
     """Edit-distance workload: 200,000 DNA-like strings (2,000 random seeds of
     length 100-150, 100 mutants each at 1%/0.5%/0.5% substitution/insertion/
@@ -301,7 +326,8 @@ def bench_edit(args):
          "stage_ms": {"filter": filt_ms, "verify": st.recheck_ms / K, "device_total": st.total_ms / K},
          "roofline": {"kernel": "k_edit_filter", "bound": "mfma", "achieved": achieved, "peak": 5000.0,
                       "unit": "TOP/s (int8)", "frac": achieved / 5000.0, "traffic": None,
-                      "work": "2 x 256 int8 ops per length-window pair, filter time from HIP events"}})
+                      "work": "2 x 256 int8 ops per length-window pair, filter time from HIP events"},
+         "verify_roofline": _verify_roofline(st, K)})
     if not args.no_cpu:
         import oracle
         cores = min(len(os.sched_getaffinity(0)), 16)
@@ -338,6 +364,13 @@ def bench_rips(args):
         t1 = time.perf_counter()
         f.copy()
         copy_s = time.perf_counter() - t1
+    # the triangle fill (k_triangles<true>): per triangle 12 bytes written
+    # (u32 value group + i64 id); the lower CSR read once (per edge 8 bytes of
+    # (group, vertex) + 4 of row + 8 of output offset, 8 per row pointer)
+    n, E, T = pts.shape[0], int(per[1]), int(per[2])
+    fill_bytes = 12.0 * T + 20.0 * E + 8.0 * (n + 1)
+    fill_ms = st.kernel_ms / K
+    ach = fill_bytes / (fill_ms * 1e-3) / 1e9 if fill_ms > 0 else None
     out = _secondary_line(
         "Rips 2-skeleton simplices/s, 300k x 3", total / dt, "simplices/s", K, args.warmup, dt * 1e3,
         "Vietoris-Rips 2-skeleton, 300000 points x 3 dims (torus/sphere/box), eps=0.08",
@@ -346,7 +379,12 @@ def bench_rips(args):
          "stage_ms": {"graph_grid": st.filter_ms / K, "cliques": st.recheck_ms / K,
                       "order": st.prep_ms / K, "device_total": st.total_ms / K},
          "copy_to_host_ms": copy_s * 1e3,
-         "simplices_per_s_with_copy": total / (dt + copy_s)})
+         "simplices_per_s_with_copy": total / (dt + copy_s),
+         "roofline": None if ach is None else {
+             "kernel": "k_triangles<true> (triangle fill)", "bound": "hbm", "achieved": ach, "peak": 8000.0,
+             "unit": "GB/s", "frac": ach / 8000.0, "traffic": None, "ms": fill_ms,
+             "work": "12 B per triangle written + 20 B per lower edge and 8 B per row pointer read, "
+                     "fill time from HIP events"}})
     if not args.no_cpu:
         import oracle
         cores = min(len(os.sched_getaffinity(0)), 16)

@@ -70,6 +70,10 @@ typedef struct rnng_stats {
                                      bound (NaN, infinities, d * range^2 > 1e36) */
     double call_ms;               /* host wall time of the build calls (graph and Voronoi builds) */
     double host_ms;               /* Voronoi: host time spent on the cell layout and work lists */
+    double kernel_ms;             /* one stage kernel's device time: the banded / bit-parallel verify of
+                                     an edit build, the triangle fill of a Rips build */
+    int64_t kernel_units;         /* its work: text columns the verify ran (pairs inside the length
+                                     window), triangles the fill wrote */
 } rnng_stats;
 
 /* Opaque handles. */

@@ -56,6 +56,8 @@ class Stats(C.Structure):
         ("plain_fallbacks", C.c_int64),
         ("call_ms", C.c_double),
         ("host_ms", C.c_double),
+        ("kernel_ms", C.c_double),
+        ("kernel_units", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

@@ -10,7 +10,8 @@ candidate buffer from the summed per-cell estimates and runs the filter once;
 without it the build counts first and fills in a second pass.
 
 `python -m rnng_amd.costmodel OUT.npz` collects per-cell counts from 256
-synthetic cell builds of 3k-300k points and 24 of 0.8M-2M points on the GPU and
+synthetic cell builds of 3k-300k points and 32 of 0.8M-2M points (random or
+farthest-point landmarks at every size) on the GPU and
 trains the network on them (12,000 full-batch Adam steps, lr 3e-3): the shipped
 cost_model.npz.
 """
@@ -159,7 +160,7 @@ def synthetic_run(rng: np.random.Generator, big: float = 0.0, huge: float = 0.0)
     eps = synth.radius_for_degree(pts, degree, sample=min(n, 1500), seed=int(rng.integers(1 << 30)))
     sizes = [2048, 4096] if very else ([256, 512, 1024, 2048] if large else [8, 16, 32, 64, 128, 256, 512])
     m = int(min(n // 8, rng.choice(sizes)))
-    return pts, eps, max(m, 2), bool(rng.integers(0, 2)) and not very
+    return pts, eps, max(m, 2), bool(rng.integers(0, 2))
 
 
 def collect(runs: int = 256, seed: int = 1, log=None, big: float = 0.0, huge: float = 0.0):
@@ -194,7 +195,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description="train the cell-query cost model on the GPU")
     ap.add_argument("out")
     ap.add_argument("--runs", type=int, default=256, help="runs of 3k-300k points")
-    ap.add_argument("--huge-runs", type=int, default=24, help="additional runs of 0.8M-2M points")
+    ap.add_argument("--huge-runs", type=int, default=32, help="additional runs of 0.8M-2M points")
     ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)

@@ -384,7 +384,10 @@ __global__ __launch_bounds__(256) void k_edit_verify(
             if (m - nn <= k && nn - m <= k) {   // |m - n| <= dist
                 const uint8_t* sr = R.atoms + R.soff[q.x];
                 const uint8_t* sc = C.atoms + C.soff[q.y];
-                dist = m <= nn ? banded_edit<KB>(sr, m, sc, nn, k) : banded_edit<KB>(sc, nn, sr, m, k);
+                // the shorter string is the pattern; one call, so lanes of either order run
+                // the same code
+                const bool sw = m > nn;
+                dist = banded_edit<KB>(sw ? sc : sr, sw ? nn : m, sw ? sr : sc, sw ? m : nn, k);
                 edge = dist <= k;
             }
             if (edge) p = edge_ids(R, C, q, self_join);
@@ -504,8 +507,35 @@ __global__ __launch_bounds__(256) void k_edit_verify_bp(
     const EditSide R, const EditSide C, int self_join, const int2* __restrict__ cand, unsigned long long ccap,
     int k, int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
     unsigned long long* __restrict__ counters) {
+    // each wave's edges gather in its own LDS buffer and take output slots
+    // with one atomic per EBUF edges (the counters share one cache line)
+    constexpr int EBUF = 256;
+    __shared__ int2 bij_all[4][EBUF];
+    __shared__ float bw_all[4][EBUF];
     const unsigned long long nc = min(counters[0], ccap);
-    const int lane = lane_id();
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    int2* bij = bij_all[wave];
+    float* bw = bw_all[wave];
+    int en = 0;                        // edges in the buffer (the same in every lane)
+    unsigned long long cols_run = 0;   // text columns this lane's candidates ran
+    auto flush = [&]() {
+        unsigned lo = 0, hi = 0;
+        if (lane == 0 && en) {
+            const unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)en);
+            lo = (unsigned)b0;
+            hi = (unsigned)(b0 >> 32);
+        }
+        lo = __shfl(lo, 0, 64);
+        hi = __shfl(hi, 0, 64);
+        const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
+        for (int i = lane; i < en; i += 64)
+            if (b0 + i < ecap) {
+                eij[b0 + i] = bij[i];
+                ew[b0 + i] = bw[i];
+            }
+        en = 0;
+        __builtin_amdgcn_wave_barrier();
+    };
     for (unsigned long long base = (unsigned long long)blockIdx.x * blockDim.x; base < nc;
          base += (unsigned long long)gridDim.x * blockDim.x) {
         const unsigned long long c = base + threadIdx.x;
@@ -515,32 +545,34 @@ __global__ __launch_bounds__(256) void k_edit_verify_bp(
         if (c < nc) {
             const int2 q = cand[c];
             const int m = R.slen[q.x], nn = C.slen[q.y];
-            // the shorter string is the pattern (a self join's rows always are)
+            // the shorter string is the pattern; one call, so lanes of either order run the
+            // same code
             const uint8_t* pr_ = R.codes + R.coff[q.x];
             const uint8_t* pc_ = C.codes + C.coff[q.y];
-            dist = m <= nn ? banded_edit_bp<VT, NP>(pr_, m, pc_, nn, k) : banded_edit_bp<VT, NP>(pc_, nn, pr_, m, k);
+            const bool sw = m > nn;
+            dist = banded_edit_bp<VT, NP>(sw ? pc_ : pr_, sw ? nn : m, sw ? pr_ : pc_, sw ? m : nn, k);
             edge = dist <= k;
             if (edge) pr = edge_ids(R, C, q, self_join);
+            if (m - nn <= k && nn - m <= k) cols_run += (unsigned long long)max(m, nn);
         }
         const unsigned long long msk = __ballot(edge);
         if (msk) {
-            unsigned lo = 0, hi = 0;
-            if (lane == 0) {
-                unsigned long long b0 = atomicAdd(&counters[1], (unsigned long long)__popcll(msk));
-                lo = (unsigned)b0;
-                hi = (unsigned)(b0 >> 32);
+            const int ne = __popcll(msk);
+            if (en + ne > EBUF) flush();
+            const int o = en + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
+            if (edge) {
+                bij[o] = pr;
+                bw[o] = (float)dist;
             }
-            lo = __shfl(lo, 0, 64);
-            hi = __shfl(hi, 0, 64);
-            const unsigned long long b0 = ((unsigned long long)hi << 32) | lo;
-            const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(msk >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((unsigned)msk, 0u));
-            if (edge && b0 + rank < ecap) {
-                eij[b0 + rank] = pr;
-                ew[b0 + rank] = (float)dist;
-            }
+            en += ne;
+            __builtin_amdgcn_wave_barrier();
         }
     }
+    flush();
+    // text columns the band ran (counters[2]): one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) cols_run += __shfl_xor(cols_run, o, 64);
+    if (lane == 0 && cols_run) atomicAdd(&counters[2], cols_run);
 }
 
 // Exact Levenshtein distance of pattern p (rows, length m <= 64 W) and text t
@@ -629,10 +661,12 @@ __global__ __launch_bounds__(256) void k_edit_verify_myers(
             const int2 q = cand[c];
             const int m = R.slen[q.x], nn = C.slen[q.y];
             if (m - nn <= k && nn - m <= k) {
-                // the shorter string is the pattern (the distance is symmetric)
+                // the shorter string is the pattern (the distance is symmetric); one call,
+                // so lanes of either order run the same code
                 const uint8_t* pr_ = R.codes + R.coff[q.x];
                 const uint8_t* pc_ = C.codes + C.coff[q.y];
-                dist = m <= nn ? myers_edit<W, NP>(pr_, m, pc_, nn, k) : myers_edit<W, NP>(pc_, nn, pr_, m, k);
+                const bool sw = m > nn;
+                dist = myers_edit<W, NP>(sw ? pc_ : pr_, sw ? nn : m, sw ? pr_ : pc_, sw ? m : nn, k);
                 edge = dist <= k;
             }
             if (edge) pr = edge_ids(R, C, q, self_join);
@@ -887,6 +921,8 @@ int edit_join(const rnng_strings_s* A, const rnng_strings_s* B, bool self, float
                 st->filter_ms += ev.ms(0, 1);
                 st->recheck_ms += ev.ms(1, 2);
                 st->total_ms += ev.ms(0, 2);
+                st->kernel_ms += ev.ms(1, 2);
+                st->kernel_units += (int64_t)cnt[2];
             }
             return RNNG_OK;
         }

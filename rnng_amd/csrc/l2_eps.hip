@@ -1049,14 +1049,21 @@ __device__ __forceinline__ void mf_tile(const typename T::frag* b, const typenam
     }
 }
 
+// Minimum of a lane's 16 MRT values: four independent chains of eight (the
+// compiler fuses pairs into 3-input minimums), then combined, so the
+// dependent depth after the last MFMA is 6 minimums rather than 16.
 template <class T>
 __device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)[MRT]) {
-    typename T::term mn = acc[0][0];
+    static_assert(MRT == 2, "four chains of eight values");
+    typename T::term c[4];
 #pragma unroll
-    for (int r = 0; r < MRT; ++r)
+    for (int h = 0; h < 4; ++h) {
+        const int r = h >> 1, q0 = (h & 1) * 8;
+        c[h] = acc[r][q0];
 #pragma unroll
-        for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = T::vmin(mn, acc[r][q]);
-    return mn;
+        for (int q = 1; q < 8; ++q) c[h] = T::vmin(c[h], acc[r][q0 + q]);
+    }
+    return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
 }
 
 // Rare path: candidates of one tile pair, staged in LDS.  Each lane builds

@@ -458,10 +458,12 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
     if (T > 0) {
         RNNG_TRY(tg.reserve((size_t)T * sizeof(uint32_t)));
         RNNG_TRY(tid.reserve((size_t)T * sizeof(int64_t)));
+        RNNG_HIP_TRY(hipEventRecord(ev.e[4], stream()));
         k_triangles<true><<<blocks_for(8 * E, 256), 256, 0, stream()>>>(
             row.as<int>(), rowptr.as<int64_t>(), cv.as<uint64_t>(), E, nullptr, offT.as<int64_t>(),
             tg.as<uint32_t>(), tid.as<int64_t>());
         RNNG_HIP_TRY(hipGetLastError());
+        RNNG_HIP_TRY(hipEventRecord(ev.e[5], stream()));
     }
     if (Q > 0) {
         RNNG_TRY(qg.reserve((size_t)Q * sizeof(uint32_t)));
@@ -539,6 +541,10 @@ int rips_build(const rnng_dataset_s* ds, float eps, int maxdim, rnng_filtration_
         st->filter_ms += ev.ms(0, 1);     // epsilon graph
         st->recheck_ms += ev.ms(1, 2);    // CSR, value groups, cliques
         st->prep_ms += ev.ms(2, 3);       // (value, id) order
+        if (T > 0) {
+            st->kernel_ms += ev.ms(4, 5);  // the triangle fill
+            st->kernel_units += T;
+        }
         st->total_ms += ev.ms(0, 3);
     }
     return RNNG_OK;
