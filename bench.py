@@ -213,7 +213,8 @@ def bench_cells(args):
                        "predicted_over_actual": (st.predicted_candidates / max(1, st.candidates)),
                        "predicted_edges": st.predicted_edges // K},
         "landmarks_and_cells_ms": t_lm * 1e3,
-        "schedule": {"cell_order": ["landmark order", "center chain"][order],
+        "schedule": {"row_order": ["cells in landmark order", "cells along the center chain",
+                                   "flat (points in their own order, every pair)"][order],
                      "work_form": ["runs of column tiles", "tile lists", "tile lists on a persistent queue"][form],
                      "forced": bool(args.schedule), "filter_variant": None if int8 else tv,
                      "xcd_order": tx,
@@ -421,8 +422,9 @@ def main():
     ap.add_argument("--cells-data", choices=["vmf", "int"], default="vmf",
                     help="cells workload points: 2M x 96 unit vectors, or the 1M x 128 integer GMM")
     ap.add_argument("--schedule", default="", metavar="ORDER,FORM",
-                    help="force the cell order (0 landmark, 1 center chain) and work form (0 runs, "
-                         "1 tile lists, 2 persistent queue) instead of the autotuner's choice")
+                    help="force the row order (0 cells in landmark order, 1 cells along the center chain, "
+                         "2 flat) and work form (0 runs, 1 tile lists, 2 persistent queue; flat: 0 only) "
+                         "instead of the autotuner's choice")
     ap.add_argument("--bf16", action="store_true",
                     help="run the timed steps on the BF16 filter even where the exact int8 one applies")
     args = ap.parse_args()

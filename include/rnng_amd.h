@@ -373,14 +373,15 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
  * zeros if not tuned; integer data runs the int8 filter untuned). */
 int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
 
-/* Schedule of the cell filter (no counterpart in the reference): the cell
- * order of the row layout (0 landmark order, 1 the centers' nearest-neighbour
- * chain, so that neighbouring cells share rows of the layout) and the work
- * form (0 runs of consecutive column tiles, one workgroup each; 1 each row
- * block's needed column tiles as one list, cut into items of up to 256 tiles;
- * 2 those items on a persistent grid of resident workgroups that take them
- * from a queue, heaviest first by tiles and the cost model's predicted
- * candidates).  By default the autotuner times every (order, form, filter
+/* Schedule of the cell filter (no counterpart in the reference): the row
+ * order of the layout (0 cells in landmark order, 1 cells along the centers'
+ * nearest-neighbour chain, so that neighbouring cells share rows of the
+ * layout; 2 flat: the points in their own order and every pair in the work,
+ * for diagrams whose adjacency prunes little, runs only) and the work form
+ * (0 runs of consecutive column tiles, one workgroup each; 1 each row block's
+ * needed column tiles as one list, cut into items of up to 256 tiles; 2 those
+ * items on a persistent grid of resident workgroups that take them from a
+ * queue, heaviest first by tiles and the cost model's predicted candidates).  By default the autotuner times every (order, form, filter
  * shape) on every 8th item of the first build and keeps the fastest;
  * set_schedule forces an order and/or form (-1 leaves it to the tuner) from
  * the next build on.  rnng_voronoi_schedule reports the order and form in use

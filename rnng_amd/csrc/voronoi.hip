@@ -67,10 +67,12 @@ struct rnng_voronoi_s {
     // filter configuration chosen by the autotuner on the first build
     int tuned_variant = -1, tuned_xcd = 1;
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    // schedule: cell order (0 landmark order, 1 nearest-neighbour chain of the
-    // centers) and work form (0 runs of column tiles, 1 tile lists, 2 tile
-    // lists on a persistent queue in order of predicted cost); -1: the
-    // autotuner's choice.  forced_*: set by rnng_voronoi_set_schedule.
+    // schedule: row order (0 cells in landmark order, 1 cells along the
+    // nearest-neighbour chain of the centers, 2 flat: the points in their own
+    // order as one block, every pair in the work) and work form (0 runs of
+    // column tiles, 1 tile lists, 2 tile lists on a persistent queue in order
+    // of predicted cost); -1: the autotuner's choice.  forced_*: set by
+    // rnng_voronoi_set_schedule.
     int order = -1, sched = -1, forced_order = -1, forced_sched = -1;
     std::vector<int> sched_cfg;           // tuner configurations (order << 8 | sched << 4 | variant slot)
     std::vector<double> sched_ms;         // and their times on every 8th work item
@@ -78,7 +80,7 @@ struct rnng_voronoi_s {
     rnng::LocalBuf start, ctl, queue, cdist;
     // the row layout on the device (position -> cell, cell -> position, padded
     // row starts, tile -> position) and the work-list builder's workspace
-    rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell;
+    rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell, flat_adj;
     rnng::LocalBuf wl_need, wl_cnt, wl_off, wl_icnt, wl_ioff, wl_stage, wl_stage2, wl_keys, wl_tot, wl_rbc;
     int use_graph = -1, autotune = -1;   // -1: from RNNG_CELL_GRAPH / RNNG_CELL_TUNE (default on)
     // captured fill pass (filter, id mapping, recheck, reference semantics,
@@ -533,6 +535,12 @@ __global__ __launch_bounds__(256) void k_center_dist(const float* __restrict__ C
         }
         out[(size_t)a * m + b] = acc;
     }
+}
+
+// Slot map of the flat order: slot i holds point i, the padding -1.
+__global__ void k_slot_identity(int n, int n_pad, int* __restrict__ slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_pad) slot[i] = i < n ? i : -1;
 }
 
 // Slot map of a cell layout: slots start[b] .. start[b] + n_b hold the
@@ -1384,8 +1392,9 @@ int64_t launch_list_size(int64_t n, int stride, int xcd) {
 // lists' tiles into v->ctl): every stride-th item, in XCD order if xcd (runs
 // and plain tile lists).  rbc: per row block predicted candidates (sched 2).
 // *covered = the pairs of every item (before the stride).
-int device_work_list(rnng_voronoi_s* v, int m, int words, int ntiles, int bt, int sched, int xcd, int stride,
-                     const std::vector<float>& rbc, DevBuf& out, int64_t* n_out, int64_t* covered) {
+int device_work_list(rnng_voronoi_s* v, const unsigned* adj, int m, int words, int ntiles, int bt, int sched,
+                     int xcd, int stride, const std::vector<float>& rbc, DevBuf& out, int64_t* n_out,
+                     int64_t* covered) {
     const int nrb = (ntiles + bt - 1) / bt;
     const int nch = (ntiles + kFilterChunk - 1) / kFilterChunk;
     const int64_t P = (int64_t)nrb * nch;
@@ -1395,7 +1404,7 @@ int device_work_list(rnng_voronoi_s* v, int m, int words, int ntiles, int bt, in
     RNNG_TRY(v->wl_off.reserve((size_t)(P + 1) * sizeof(int64_t)));
     RNNG_TRY(v->wl_tot.reserve(4 * sizeof(int64_t)));
     k_rb_need<<<nrb, 256, (size_t)2 * words * sizeof(unsigned), stream()>>>(
-        v->adj_dev.as<unsigned>(), words, m, v->lay_perm.as<int>(), v->lay_pos.as<int>(), v->lay_tcell.as<int>(),
+        adj, words, m, v->lay_perm.as<int>(), v->lay_pos.as<int>(), v->lay_tcell.as<int>(),
         ntiles, bt, v->wl_need.as<unsigned>());
     RNNG_HIP_TRY(hipGetLastError());
     RNNG_TRY(launch_fill_u64(v->wl_tot.as<unsigned long long>(), 1, 0ull));
@@ -1536,26 +1545,47 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     MfmaPrep& P = v->prep;
     CellLayout L;
     int n_pad = 0, layout_order = -1;
+    // blocks of the current layout (m cells, or the one block of the flat
+    // order) and their adjacency
+    int lm = m, lwords = words;
+    const unsigned* ladj = adj_dev.as<unsigned>();
     auto apply_layout = [&](int order) -> int {
         if (order == layout_order) return RNNG_OK;
-        std::vector<int> perm((size_t)m);
-        if (order == 1) {
-            if ((int)v->chain.size() != m) RNNG_TRY(center_chain(v, m, d));
-            perm = v->chain;
+        if (order == 2) {
+            // flat: every pair of the points in their own order (one block
+            // adjacent to itself)
+            make_layout(std::vector<int>{0}, std::vector<int>{n}, nullptr, 1, 1, L);
+            lm = 1;
+            lwords = 1;
+            RNNG_TRY(upload(v->flat_adj, std::vector<unsigned>{1u}));
+            ladj = v->flat_adj.as<unsigned>();
         } else {
-            std::iota(perm.begin(), perm.end(), 0);
+            std::vector<int> perm((size_t)m);
+            if (order == 1) {
+                if ((int)v->chain.size() != m) RNNG_TRY(center_chain(v, m, d));
+                perm = v->chain;
+            } else {
+                std::iota(perm.begin(), perm.end(), 0);
+            }
+            make_layout(perm, v->cnt, nullptr, m, words, L);
+            lm = m;
+            lwords = words;
+            ladj = adj_dev.as<unsigned>();
         }
-        make_layout(perm, v->cnt, nullptr, m, words, L);
-        n_pad = std::max(kPadRows, (L.rs[m] + kPadRows - 1) / kPadRows * kPadRows);
-        RNNG_TRY(upload(v->start, L.start));
+        n_pad = std::max(kPadRows, (L.rs[lm] + kPadRows - 1) / kPadRows * kPadRows);
         RNNG_TRY(upload(v->lay_perm, L.perm));
         RNNG_TRY(upload(v->lay_pos, L.pos));
         RNNG_TRY(upload(v->lay_rs, L.rs));
         RNNG_TRY(upload(v->lay_tcell, L.tcell));
         RNNG_TRY(slot.reserve((size_t)n_pad * sizeof(int)));
-        RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
-        k_slot_map_at<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), v->start.as<int>(),
-                                               slot.as<int>());
+        if (order == 2) {
+            k_slot_identity<<<blocks_for(n_pad, 256), 256, 0, stream()>>>(n, n_pad, slot.as<int>());
+        } else {
+            RNNG_TRY(upload(v->start, L.start));
+            RNNG_TRY(launch_fill_u32(slot.as<unsigned>(), n_pad, 0xffffffffu));
+            k_slot_map_at<<<m, 256, 0, stream()>>>(v->cptr.as<int>(), v->cids.as<int>(), v->start.as<int>(),
+                                                   slot.as<int>());
+        }
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
         layout_order = order;
@@ -1633,9 +1663,9 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     };
     // predicted candidates of every row block (the persistent queue's order)
     auto rb_cands = [&](int bt) {
-        std::vector<float> out((size_t)((L.rs[m] + bt * 32 - 1) / (bt * 32)), 0.0f);
+        std::vector<float> out((size_t)((L.rs[lm] + bt * 32 - 1) / (bt * 32)), 0.0f);
         std::vector<double> acc(out.size(), 0.0);
-        for (int k = 0; k < m; ++k) {
+        for (int k = 0; k < lm && lm == m; ++k) {
             const int c = L.perm[(size_t)k];
             const int tiles = (L.rs[k + 1] - L.rs[k]) / 32;
             for (int t = L.rs[k] / 32; t < L.rs[k + 1] / 32; ++t) acc[(size_t)(t / bt)] += pc[(size_t)c] / tiles;
@@ -1648,7 +1678,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         const int bt = P.i8 ? filter_i8_row_tiles(cell_i8_code()) : filter_block_tiles(var);
         int64_t cov = 0;
         const std::vector<float> rbc = sched_ == 2 ? rb_cands(bt) : std::vector<float>();
-        const int rc = device_work_list(v, m, words, L.rs[m] / 32, bt, sched_, xcd, stride, rbc, out, n_out, &cov);
+        const int rc =
+            device_work_list(v, ladj, lm, lwords, L.rs[lm] / 32, bt, sched_, xcd, stride, rbc, out, n_out, &cov);
         return rc != RNNG_OK ? -1 : cov;
     };
     // ---- autotuner: cell order x work form x filter shape, on every 8th item ----
@@ -1676,6 +1707,12 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
                         cfgs.push_back({order, 2, var, 0});
                     }
                 }
+            }
+            // flat order (every pair, no cell padding): runs only
+            if (P.i8) {
+                cfgs.push_back({2, 0, 0, 1});
+            } else {
+                for (int var : {17, 13}) cfgs.push_back({2, 0, var, 1});
             }
             v->sched_cfg.clear();
             v->sched_ms.clear();
@@ -2149,7 +2186,8 @@ int debug_cell_work(const int32_t* cnt, int32_t m, const uint32_t* adj, const in
         RNNG_TRY(upload(tmp.lay_tcell, L.tcell));
         LocalBuf out;
         int64_t no = 0;
-        RNNG_TRY(device_work_list(&tmp, m, words, L.rs[m] / 32, bt, sched, 0, 1, rbc, out, &no, &cov));
+        RNNG_TRY(device_work_list(&tmp, tmp.adj_dev.as<unsigned>(), m, words, L.rs[m] / 32, bt, sched, 0, 1, rbc,
+                                  out, &no, &cov));
         RNNG_TRY(download(it, out, (size_t)no));
         if (sched > 0) {
             int64_t nt = 0;
@@ -2202,8 +2240,9 @@ int rnng_debug_cell_work_device(const int32_t* cnt, int32_t m, const uint32_t* a
 
 int rnng_voronoi_set_schedule(rnng_voronoi_t v, int32_t order, int32_t sched) {
     RNNG_TRY(check_voronoi(v));
-    if (order > 1 || sched > 2) {
-        set_error("rnng_voronoi_set_schedule: order 0..1, sched 0..2 (-1: the autotuner's)");
+    if (order > 2 || sched > 2 || (order == 2 && sched > 0)) {
+        set_error("rnng_voronoi_set_schedule: order 0..2, sched 0..2, the flat order 2 with runs (sched 0) only "
+                  "(-1: the autotuner's)");
         return RNNG_ERR_ARG;
     }
     v->forced_order = order < 0 ? -1 : order;

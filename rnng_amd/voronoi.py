@@ -103,8 +103,10 @@ class Voronoi:
         return var.value, xcd.value, ms
 
     def set_schedule(self, order: int = -1, sched: int = -1) -> None:
-        """Force the cell order (0 landmark order, 1 center chain) and work form
-        (0 runs, 1 tile lists, 2 tile lists on a persistent queue); -1: tuner."""
+        """Force the row order (0 cells in landmark order, 1 cells along the
+        center chain, 2 flat: points in their own order, every pair; runs only)
+        and work form (0 runs, 1 tile lists, 2 tile lists on a persistent
+        queue); -1: tuner."""
         check(_lib.lib().rnng_voronoi_set_schedule(self._h, int(order), int(sched)))
 
     def schedule(self):
