@@ -462,8 +462,9 @@ int rnng_set_int8_filter(int32_t enable);
  * measurement and for tests that run every shape).  int8_code: 0 column tiles
  * staged through registers, 1/2 direct-to-LDS ring with three stages of 1/2
  * tiles, 3/4 two stages of 3/4 tiles, 5/6/7 two stages of 4/6/8 tiles shared
- * by eight waves (all-pairs builds; the cell path runs code 3 for codes above
- * 4).  bf16_variant 0..17 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
+ * by eight waves of two row tiles, 8 two stages of 8 tiles shared by four waves
+ * of four row tiles (two waves per SIMD) (all-pairs builds; the cell path runs
+ * code 3 for codes above 4).  bf16_variant 0..17 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
  * tiles, 15 two stages of 4 tiles shared by 8 waves, 16/17 two stages of 4/8
  * tiles shared by 12 waves, the others register-staged or group-load shapes).
  * A negative value leaves that setting unchanged; the defaults (7, 17) come from

@@ -1021,77 +1021,80 @@ struct MfBF16 {
     __device__ static unsigned below(term v, term t) { return v < t ? 1u : 0u; }
 };
 
-constexpr int MRT = 2;   // row tiles per wave
+constexpr int MRT = 2;   // row tiles per wave (the ring kernel also runs 4)
 
-template <class T, int KS>
+template <class T, int KS, int M = MRT>
 __device__ __forceinline__ void mf_load_rows(const typename T::frag* __restrict__ xa,
                                              const typename T::term* __restrict__ rterm, int rt0, int lane,
-                                             typename T::frag (&a)[MRT][KS], typename T::accv (&xr)[MRT]) {
+                                             typename T::frag (&a)[M][KS], typename T::accv (&xr)[M]) {
 #pragma unroll
-    for (int r = 0; r < MRT; ++r)
+    for (int r = 0; r < M; ++r)
 #pragma unroll
         for (int s = 0; s < KS; ++s) a[r][s] = xa[((size_t)(rt0 + r) * KS + s) * 64 + lane];
 #pragma unroll
-    for (int r = 0; r < MRT; ++r)
+    for (int r = 0; r < M; ++r)
 #pragma unroll
         for (int q = 0; q < 16; ++q) xr[r][q] = rterm[(rt0 + r) * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3)];
 }
 
 // one column tile held in LDS (b)
-template <class T, int KS>
-__device__ __forceinline__ void mf_tile(const typename T::frag* b, const typename T::frag (&a)[MRT][KS],
-                                        const typename T::accv (&xr)[MRT], int lane, typename T::accv (&acc)[MRT]) {
+template <class T, int KS, int M = MRT>
+__device__ __forceinline__ void mf_tile(const typename T::frag* b, const typename T::frag (&a)[M][KS],
+                                        const typename T::accv (&xr)[M], int lane, typename T::accv (&acc)[M]) {
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const typename T::frag bb = b[s * 64 + lane];
 #pragma unroll
-        for (int r = 0; r < MRT; ++r) acc[r] = T::mfma(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
+        for (int r = 0; r < M; ++r) acc[r] = T::mfma(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
     }
 }
 
-// Minimum of a lane's 16 MRT values: four independent chains of eight (the
+// Minimum of a lane's 16 M values: four independent chains of 4 M (the
 // compiler fuses pairs into 3-input minimums), then combined, so the
-// dependent depth after the last MFMA is 6 minimums rather than 16.
-template <class T>
-__device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)[MRT]) {
-    static_assert(MRT == 2, "four chains of eight values");
+// dependent depth after the last MFMA is a quarter of the values plus two.
+template <class T, int M = MRT>
+__device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)[M]) {
+    static_assert(M == 2 || M == 4, "four chains");
+    constexpr int L = 4 * M;   // values per chain
     typename T::term c[4];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-        const int r = h >> 1, q0 = (h & 1) * 8;
-        c[h] = acc[r][q0];
+        c[h] = acc[(h * L) >> 4][(h * L) & 15];
 #pragma unroll
-        for (int q = 1; q < 8; ++q) c[h] = T::vmin(c[h], acc[r][q0 + q]);
+        for (int q = 1; q < L; ++q) c[h] = T::vmin(c[h], acc[(h * L + q) >> 4][(h * L + q) & 15]);
     }
     return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
 }
 
 // Rare path: candidates of one tile pair, staged in LDS.  Each lane builds
-// the mask of its 32 values below nyc (value i = r*16+q of a lane is row
+// the mask of its 16 M values below nyc (value i = r*16+q of a lane is row
 // rowb + 8*(i >> 2) + (i & 3), rising with i), drops the rows at or past its
 // limit (the row/column limit of a self join, padding) as a bit count, then
 // the lanes walk their own set bits together: one ballot and one LDS
 // reservation per round.  The mask's shifts depend on an opaque zero so they
 // stay out of the hot loop.
-template <class T>
-__device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], typename T::term nyc, int ct, int rt0,
+template <class T, int M = MRT>
+__device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[M], typename T::term nyc, int ct, int rt0,
                                         int lane, int n_rows, int n_cols, int self_join, int2* stage, unsigned stage_cap,
                                         unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
                                         unsigned long long* __restrict__ counter) {
+    static_assert(M == 2 || M == 4, "32 or 64 values per lane");
+    using mask_t = typename std::conditional<M == 4, unsigned long long, unsigned>::type;
+    constexpr int NV = 16 * M;   // values per lane
     const int rowb = rt0 * 32 + 4 * (lane >> 5);
     int opaque;
     asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
-    unsigned mask = 0;
+    mask_t mask = 0;
 #pragma unroll
-    for (int r = 0; r < MRT; ++r)
+    for (int r = 0; r < M; ++r)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) mask |= T::below(acc[r][q], nyc) << (r * 16 + q + opaque);
+        for (int q = 0; q < 16; ++q) mask |= (mask_t)T::below(acc[r][q], nyc) << (r * 16 + q + opaque);
     const int col = ct * 32 + (lane & 31);
     const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
     // values with row offset 8*(i >> 2) + (i & 3) < L are the first k
     const int L = lim - rowb;
-    const int k = L <= 0 ? 0 : (L >= 64 ? 32 : 4 * (L >> 3) + min(4, L & 7));
-    mask &= k >= 32 ? ~0u : ((1u << k) - 1u);
+    const int k = L <= 0 ? 0 : (L >= 2 * NV ? NV : 4 * (L >> 3) + min(4, L & 7));
+    mask &= k >= NV ? ~(mask_t)0 : (((mask_t)1 << k) - (mask_t)1);
 #pragma unroll 1
     for (;;) {
         const bool p = mask != 0;
@@ -1099,7 +1102,7 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[MRT], type
         if (m == 0) break;
         int row = 0;
         if (p) {
-            const int rq = __builtin_ctz(mask);
+            const int rq = M == 4 ? __builtin_ctzll((unsigned long long)mask) : __builtin_ctz((unsigned)mask);
             mask &= mask - 1;
             row = rowb + 8 * (rq >> 2) + (rq & 3);
         }
@@ -1306,7 +1309,7 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
 // them (held in registers for the item's whole run).  A persistent
 // grid (PQ) takes items queue[0]++ (in list order) in each workgroup until
 // the list is exhausted; otherwise workgroup b runs item b.
-template <class T, int KS, int G, int MINW, int S = 3, int NW = 4, bool TL = false, bool PQ = false>
+template <class T, int KS, int G, int MINW, int S = 3, int NW = 4, bool TL = false, bool PQ = false, int M = MRT>
 __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
     const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
@@ -1346,7 +1349,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     int4 wi;
     if (load_work_item(work, item, wi)) {   // padding items touch no memory
     if (tid == 0) scount = 0u;
-    const int rt0 = wi.x * (NW * MRT) + wave * MRT;
+    const int rt0 = wi.x * (NW * M) + wave * M;
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
     // loads this wave issues per group (the same in every lane)
@@ -1358,9 +1361,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     // the rows
     TileIdx ti{};
     if constexpr (TL) ti = load_tile_idx(ctl, wi.y, wi.z, lane);
-    frag a[MRT][KS];
-    accv xr[MRT];
-    mf_load_rows<T, KS>(xa, rterm, rt0, lane, a, xr);
+    frag a[M][KS];
+    accv xr[M];
+    mf_load_rows<T, KS, M>(xa, rterm, rt0, lane, a, xr);
     asm volatile("" ::: "memory");
     mf_group_load<T, KS, G, NW, TL>(yb, cterm, bs0, cs0, wi.y, last, wave, lane, ti);
     asm volatile("" ::: "memory");
@@ -1380,12 +1383,12 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
             const term nyc = -ccur[cc * 32 + (lane & 31)];
-            accv acc[MRT];
-            mf_tile<T, KS>(cur + cc * NCH, a, xr, lane, acc);
-            if (__any(mf_min<T>(acc) < nyc)) {
+            accv acc[M];
+            mf_tile<T, KS, M>(cur + cc * NCH, a, xr, lane, acc);
+            if (__any(mf_min<T, M>(acc) < nyc)) {
                 const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
-                mf_emit<T>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
-                           counter);
+                mf_emit<T, M>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
+                              counter);
             }
         }
         ring_barrier(S == 3 && issue ? nld : 0);
@@ -1780,17 +1783,24 @@ void launch_filter_i8(const I8Args& a) {
         k_l2_mfma_filter_ring<MfI8, KS8, CODE, MINW, 2><<<a.n_items, 256, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
             nullptr, a.n_items, nullptr);
-    else {
+    else if constexpr (CODE <= 7) {
         constexpr int G = CODE == 5 ? 4 : (CODE == 6 || KS8 >= 5) ? 6 : 8;
         k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8><<<a.n_items, 512, 0, stream()>>>(
+            a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+            nullptr, a.n_items, nullptr);
+    } else {
+        // four waves of four row tiles each (the same 16 row tiles per item),
+        // two waves per SIMD with twice the registers
+        constexpr int G = KS8 >= 5 ? 6 : 8;
+        k_l2_mfma_filter_ring<MfI8, KS8, G, 2, 2, 4, false, false, 4><<<a.n_items, 256, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
             nullptr, a.n_items, nullptr);
     }
 }
 
-constexpr int kI8Codes = 8;
+constexpr int kI8Codes = 9;
 
-// row tiles per work item of an int8 kernel code (two per wave)
+// row tiles per work item of an int8 kernel code (codes 5-8: 16 row tiles)
 int i8_row_tiles(int code) { return code >= 5 ? 16 : 8; }
 
 int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
@@ -1836,6 +1846,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
         case 1 * 16 + 5: launch_filter_i8<1, 5>(a); break;
         case 1 * 16 + 6: launch_filter_i8<1, 6>(a); break;
         case 1 * 16 + 7: launch_filter_i8<1, 7>(a); break;
+        case 1 * 16 + 8: launch_filter_i8<1, 8>(a); break;
         case 2 * 16 + 0: launch_filter_i8<2, 0>(a); break;
         case 2 * 16 + 1: launch_filter_i8<2, 1>(a); break;
         case 2 * 16 + 2: launch_filter_i8<2, 2>(a); break;
@@ -1844,6 +1855,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
         case 2 * 16 + 5: launch_filter_i8<2, 5>(a); break;
         case 2 * 16 + 6: launch_filter_i8<2, 6>(a); break;
         case 2 * 16 + 7: launch_filter_i8<2, 7>(a); break;
+        case 2 * 16 + 8: launch_filter_i8<2, 8>(a); break;
         case 3 * 16 + 0: launch_filter_i8<3, 0>(a); break;
         case 3 * 16 + 1: launch_filter_i8<3, 1>(a); break;
         case 3 * 16 + 2: launch_filter_i8<3, 2>(a); break;
@@ -1852,6 +1864,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
         case 3 * 16 + 5: launch_filter_i8<3, 5>(a); break;
         case 3 * 16 + 6: launch_filter_i8<3, 6>(a); break;
         case 3 * 16 + 7: launch_filter_i8<3, 7>(a); break;
+        case 3 * 16 + 8: launch_filter_i8<3, 8>(a); break;
         case 4 * 16 + 0: launch_filter_i8<4, 0>(a); break;
         case 4 * 16 + 1: launch_filter_i8<4, 1>(a); break;
         case 4 * 16 + 2: launch_filter_i8<4, 2>(a); break;
@@ -1860,6 +1873,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
         case 4 * 16 + 5: launch_filter_i8<4, 5>(a); break;
         case 4 * 16 + 6: launch_filter_i8<4, 6>(a); break;
         case 4 * 16 + 7: launch_filter_i8<4, 7>(a); break;
+        case 4 * 16 + 8: launch_filter_i8<4, 8>(a); break;
         case 5 * 16 + 0: launch_filter_i8<5, 0>(a); break;
         case 5 * 16 + 1: launch_filter_i8<5, 1>(a); break;
         case 5 * 16 + 2: launch_filter_i8<5, 2>(a); break;
@@ -1868,6 +1882,7 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
         case 5 * 16 + 5: launch_filter_i8<5, 5>(a); break;
         case 5 * 16 + 6: launch_filter_i8<5, 6>(a); break;
         case 5 * 16 + 7: launch_filter_i8<5, 7>(a); break;
+        case 5 * 16 + 8: launch_filter_i8<5, 8>(a); break;
         default:
             set_error("int8 filter supports d <= 160");
             return RNNG_ERR_UNSUPPORTED;

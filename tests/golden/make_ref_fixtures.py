@@ -18,6 +18,12 @@ Jobs:
               and 1,500 points (eps 0.35, -d 3, vertex lists -V)
   edit40k     systolic driver, -D edit, the first 40,000 of the 200,000 strings
   gmm_int     systolic driver, the 1,000,000 x 128 integer-valued GMM (full size)
+  c2scale     systolic driver on the first 50,000 / 100,000 / 200,000 points of that
+              GMM at its eps (timing: how the cover trees scale)
+  c4sub       landmark driver on the first 200,000 of the 2,000,000 x 96 vMF points at
+              their eps, 410 random centers (-F; the full workload has 4,096)
+  c5cells     rips driver on the 300,000 x 3 torus/sphere/box points, eps 0.08, -d 2,
+              256 cells (-m 32 per rank; one cell of 300,000 points is out of reach)
 
 The reference writes its output through MPI-IO; MPICH comes from MPI_HOME
 (default /opt/conda).  Nothing here is imported by the product.
@@ -167,7 +173,52 @@ def job_gmm_int(res, wd):
     res["systolic_gmm_int_1Mx128"] = _entry("systolic", 8, flags, inp, out, wall, timers, _graph_summary(out))
 
 
-JOBS = {"c1": job_c1, "edit600": job_edit600, "rips": job_rips, "edit40k": job_edit40k, "gmm_int": job_gmm_int}
+def job_c2scale(res, wd):
+    from rnng_amd import synth
+    pts, eps = synth.points_gmm_int()
+    flags = ["-r", "%.9g" % float(eps)]
+    for n in (50_000, 100_000, 200_000):
+        inp = os.path.join(wd, f"gmm_int_{n}.fvecs")
+        _write_fvecs(inp, pts[:n])
+        out = os.path.join(wd, f"gmm_int_{n}.txt")
+        wall, timers = _run("systolic", 8, ["-i", inp, "-o", out] + flags, wd)
+        res[f"systolic_gmm_int_{n // 1000}k"] = _entry("systolic", 8, flags, inp, out, wall, timers,
+                                                       _graph_summary(out))
+        os.remove(out)
+
+
+def job_c4sub(res, wd):
+    from rnng_amd import synth
+    pts, eps = synth.points_vmf()
+    inp = os.path.join(wd, "vmf200k.fvecs")
+    _write_fvecs(inp, pts[:200_000])
+    del pts
+    flags = ["-r", "%.9g" % float(eps), "-m", "410", "-F", "-s", "7"]
+    out = os.path.join(wd, "vmf200k_landmark.txt")
+    wall, timers = _run("landmark", 8, ["-i", inp, "-o", out] + flags, wd)
+    res["landmark_vmf_200kx96"] = _entry("landmark", 8, flags, inp, out, wall, timers, _graph_summary(out))
+    os.remove(out)
+
+
+def job_c5cells(res, wd):
+    from rnng_amd import synth
+    pts, eps = synth.points_rips()
+    inp = os.path.join(wd, "rips300k.fvecs")
+    _write_fvecs(inp, pts)
+    flags = ["-r", "%.9g" % float(np.float32(eps)), "-d", "2", "-m", "32", "-s", "11"]
+    base = os.path.join(wd, "rips300k")
+    wall, timers = _run("rips", 8, ["-i", inp, "-o", base] + flags, wd)
+    out = base + ".txt"
+    with open(out) as f:
+        nlines = sum(1 for _ in f)
+    res["rips_300k_d2_256cells"] = _entry("rips", 8, flags, inp, out, wall, timers, {"simplices": nlines})
+    for fn in os.listdir(wd):
+        if fn.startswith("rips300k"):
+            os.remove(os.path.join(wd, fn))
+
+
+JOBS = {"c1": job_c1, "edit600": job_edit600, "rips": job_rips, "edit40k": job_edit40k, "gmm_int": job_gmm_int,
+        "c2scale": job_c2scale, "c4sub": job_c4sub, "c5cells": job_c5cells}
 
 
 def main(names):

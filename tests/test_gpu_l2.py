@@ -246,7 +246,7 @@ def test_int8_filter_bit_exact(orc, d, lo, hi):
 
 
 def test_every_filter_kernel_shape(orc):
-    """Every selectable filter kernel (int8 codes 0-7, BF16 variants 0-17, set
+    """Every selectable filter kernel (int8 codes 0-8, BF16 variants 0-17, set
     through rnng_set_filter_kernels) gives the oracle's edges bit for bit, on
     integer data (the int8 path) and float data (the BF16 path), for self joins
     and a bipartite join, at d = 17 (scalar loads, partial k-steps) and 128."""
@@ -279,7 +279,7 @@ def test_every_filter_kernel_shape(orc):
             _check_same(ijb, wb, ijb_ref, wb_ref)
 
     try:
-        for code in range(8):
+        for code in range(9):
             flush_print(f"int8 filter kernel code {code}")
             _lib.check(lib.rnng_set_filter_kernels(code, -1))
             check(ints, 1)
@@ -287,7 +287,7 @@ def test_every_filter_kernel_shape(orc):
             flush_print(f"BF16 filter variant {var}")
             _lib.check(lib.rnng_set_filter_kernels(-1, var))
             check(floats, 0)
-        assert lib.rnng_set_filter_kernels(8, -1) != 0 and lib.rnng_set_filter_kernels(-1, 18) != 0
+        assert lib.rnng_set_filter_kernels(9, -1) != 0 and lib.rnng_set_filter_kernels(-1, 18) != 0
     finally:
         _lib.check(lib.rnng_set_filter_kernels(code0.value, var0.value))
 
