@@ -14,6 +14,8 @@ than against another device path.
   full_edit_40k.json         the first 40,000 strings of the 200,000-string DNA
                              set, eps 8: sorted pairs and int32 distances
   full_vmf_2Mx96.json        2,000,000 x 96 vMF unit vectors: sorted pairs and weights
+  full_vmf_200kx96.json      the first 200,000 of those points at the same eps (the
+                             reference's landmark driver was run on this subset)
 
 Each file also holds the SHA-256 of the generated input, so a change of the
 generator shows up as a fixture mismatch instead of a parity failure.
@@ -65,6 +67,19 @@ def vmf():
         "sha256_points_float32": _sha(pts, np.float32), "oracle_s": round(time.time() - t, 1)})
 
 
+def vmf200k():
+    import oracle
+    from rnng_amd import synth
+    pts, eps = synth.points_vmf()
+    pts = np.ascontiguousarray(pts[:200_000])
+    t = time.time()
+    ij, w = oracle.l2_eps_pairs(pts, eps, cap=2_000_000)
+    _dump("full_vmf_200kx96.json", {
+        "n": int(pts.shape[0]), "d": int(pts.shape[1]), "eps_bits": int(np.float32(eps).view(np.uint32)),
+        "edges": int(ij.shape[0]), "sha256_ij_int64": _sha(ij, np.int64), "sha256_w_float32": _sha(w, np.float32),
+        "sha256_points_float32": _sha(pts, np.float32), "oracle_s": round(time.time() - t, 1)})
+
+
 def rips():
     import oracle
     from rnng_amd import synth
@@ -94,7 +109,7 @@ def edit():
         "sha256_atoms": _sha(sub_atoms, np.uint8), "oracle_s": round(time.time() - t, 1)})
 
 
-JOBS = {"gmm_int": gmm_int, "rips": rips, "edit": edit, "vmf": vmf}
+JOBS = {"gmm_int": gmm_int, "rips": rips, "edit": edit, "vmf": vmf, "vmf200k": vmf200k}
 
 if __name__ == "__main__":
     for name in (sys.argv[1:] or ["gmm_int", "rips", "edit"]):

@@ -160,3 +160,34 @@ def test_vmf_2Mx96_cells_equal_plain_and_ghosts_follow_the_reference(orc):
         assert orc.l2_eps_pairs_count(pts[q][None, :], eps, Y=pts[members]) > 0
     v.close()
     ds.close()
+
+
+def test_vmf_200k_cells_with_the_reference_centers_equal_its_landmark_output():
+    """The reference's landmark driver on the first 200,000 of the 2M x 96 vMF
+    points (410 centers picked by its selection_sample with seed 7, 8 MPI
+    ranks): the cell path with the same centers gives its sorted pairs bit for
+    bit (and the oracle's brute force, when that digest is present)."""
+    import json as _json
+    from rnng_amd import l2, synth, voronoi
+    ref = _json.load(open(os.path.join(GOLD, "ref_outputs.json"))).get("landmark_vmf_200kx96")
+    if ref is None:
+        pytest.skip("reference landmark run not recorded")
+    pts, eps = synth.points_vmf()
+    pts = np.ascontiguousarray(pts[:200_000])
+    centers = voronoi.selection_sample(pts.shape[0], 410, 7)
+    ds = l2.Dataset(pts)
+    v = voronoi.Voronoi(ds, centers=centers)
+    try:
+        flush_print("vmf 200k x 96, 410 reference centers")
+        g = v.graph(eps)
+        ij, w = g.edges(sorted=True)
+        g.close()
+        assert ij.shape[0] == ref["edges_undirected"]
+        assert _sha(ij, np.int64) == ref["sha256_pairs_int64"]
+        path = os.path.join(GOLD, "full_vmf_200kx96.json")
+        if os.path.exists(path):
+            fx = _json.load(open(path))
+            assert _sha(ij, np.int64) == fx["sha256_ij_int64"] and _sha(w, np.float32) == fx["sha256_w_float32"]
+    finally:
+        v.close()
+        ds.close()

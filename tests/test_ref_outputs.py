@@ -113,10 +113,13 @@ def test_rips_oracle_equals_reference_files(orc):
 
 
 @pytest.mark.parametrize("ref_key,orc_file", [("systolic_gmm_int_1Mx128", "full_gmm_int_1Mx128.json"),
-                                              ("systolic_edit_40k", "full_edit_40k.json")])
+                                              ("systolic_edit_40k", "full_edit_40k.json"),
+                                              ("landmark_vmf_200kx96", "full_vmf_200kx96.json")])
 def test_full_size_reference_pairs_equal_oracle_digests(ref_key, orc_file):
-    """Two independent programs at full size: the reference's systolic driver
-    (8 MPI ranks, cover trees) and the oracle's brute force."""
+    """Two independent programs at full size: the reference's drivers (8 MPI
+    ranks, cover trees; the landmark driver with 410 random centers, whose ghost
+    criterion keeps every pair on these unit vectors) and the oracle's brute
+    force."""
     if ref_key not in REF or not os.path.exists(os.path.join(GOLD, orc_file)):
         pytest.skip(f"{ref_key} / {orc_file} not generated")
     o = json.load(open(os.path.join(GOLD, orc_file)))
@@ -125,3 +128,14 @@ def test_full_size_reference_pairs_equal_oracle_digests(ref_key, orc_file):
     assert r["sha256_pairs_int64"] == o["sha256_ij_int64"]
     n = o.get("n", o.get("strings"))
     assert r["header"] == f"% {n} {n} {2 * o['edges'] + n}"
+
+
+def test_full_rips_reference_simplex_count_equals_oracle():
+    """The reference's rips driver on the 300,000 x 3 points (256 cells, merged
+    filtration of 4.6 GB) wrote as many simplices as the oracle's filtration
+    holds (whose digest the GPU matches)."""
+    key = "rips_300k_d2_256cells"
+    if key not in REF or not os.path.exists(os.path.join(GOLD, "full_rips_300kx3.json")):
+        pytest.skip("not generated")
+    o = json.load(open(os.path.join(GOLD, "full_rips_300kx3.json")))
+    assert REF[key]["simplices"] == sum(o["per_dim"])
