@@ -368,9 +368,10 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
  * the autotuner times 17 = twelve waves sharing two LDS stages of 8 column
  * tiles, 13 / 14 = four waves with two stages of 2 / 3 tiles, 12 = four waves
  * with three stages of 2 tiles; xcd_order: the XCD-ordered work list) and the
- * autotuner's timings of the 8 (variant x order) candidates in ms, in that
- * variant order, each without and with the XCD order (config_ms may be NULL;
- * zeros if not tuned; integer data runs the int8 filter untuned). */
+ * autotuner's timings in ms of those 4 variants on runs in landmark order,
+ * each without and with the XCD order (config_ms may be NULL; zeros if not
+ * tuned, and on integer data, whose int8 filter has one cell shape).
+ * rnng_voronoi_schedule reports every configuration the tuner timed. */
 int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
 
 /* Schedule of the cell filter (no counterpart in the reference): the row
