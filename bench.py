@@ -217,8 +217,10 @@ def bench_cells(args):
                                    "flat (points in their own order, every pair)"][order],
                      "work_form": ["runs of column tiles", "tile lists", "tile lists on a persistent queue"][form],
                      "forced": bool(args.schedule), "filter_variant": None if int8 else tv,
+                     "tuned_best_shape": (min(cfgs, key=lambda c: c[3])[4] if cfgs else None),
                      "xcd_order": tx,
-                     "tuned_ms_on_an_eighth": {f"order{o}/form{f}/xcd{x}": round(t, 3) for o, f, x, t in cfgs}},
+                     "tuned_ms_on_an_eighth": {f"order{o}/form{f}/xcd{x}/shape{k}": round(t, 3)
+                                               for o, f, x, t, k in cfgs}},
         "cell_sizes": {"min": int(sizes.min()), "median": int(np.median(sizes)), "max": int(sizes.max())},
         "stage_ms": {"ghost_test": st.ghost_ms / K, "prep": st.prep_ms / K, "filter": st.filter_ms / K,
                      "recheck_dedup": st.recheck_ms / K, "device_total": st.total_ms / K},

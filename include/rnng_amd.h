@@ -386,7 +386,9 @@ int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, 
  * shape) on every 8th item of the first build and keeps the fastest;
  * set_schedule forces an order and/or form (-1 leaves it to the tuner) from
  * the next build on.  rnng_voronoi_schedule reports the order and form in use
- * and the tuner's configurations (order << 8 | form << 4 | XCD order) with
+ * and the tuner's configurations (filter shape << 16 | order << 8 | form << 4 |
+ * XCD order; the shape is the BF16 variant, or for integer data the int8 code,
+ * 0 for the cell default) with
  * their times in ms (up to cap; *n_configs = how many were timed).  Every
  * schedule gives the same edges. */
 int rnng_voronoi_set_schedule(rnng_voronoi_t v, int32_t order, int32_t sched);

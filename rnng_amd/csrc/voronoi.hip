@@ -66,6 +66,7 @@ struct rnng_voronoi_s {
     rnng::MfmaPrep prep;
     // filter configuration chosen by the autotuner on the first build
     int tuned_variant = -1, tuned_xcd = 1;
+    int tuned_i8 = 0;   // int8 cell kernel code chosen by the tuner (0: the default, code 3)
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // schedule: row order (0 cells in landmark order, 1 cells along the
     // nearest-neighbour chain of the centers, 2 flat: the points in their own
@@ -1657,7 +1658,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             ia.cand = cand_ptr;
             ia.cap = cap_;
             ia.counter = counters.as<unsigned long long>();
-            return filter_dispatch_i8(ia, P.KS8, cell_i8_code());
+            return filter_dispatch_i8(ia, P.KS8, var > 0 ? var : cell_i8_code());
         }
         return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
     };
@@ -1675,7 +1676,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     };
     // the launch list of a schedule on the current layout, built on the device
     auto work_list = [&](int sched_, int var, int xcd, int stride, DevBuf& out, int64_t* n_out) -> int64_t {
-        const int bt = P.i8 ? filter_i8_row_tiles(cell_i8_code()) : filter_block_tiles(var);
+        const int bt = P.i8 ? filter_i8_row_tiles(var > 0 ? var : cell_i8_code()) : filter_block_tiles(var);
         int64_t cov = 0;
         const std::vector<float> rbc = sched_ == 2 ? rb_cands(bt) : std::vector<float>();
         const int rc =
@@ -1685,6 +1686,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     // ---- autotuner: cell order x work form x filter shape, on every 8th item ----
     if (v->tuned_variant < 0) {
         v->tuned_variant = filter_default_variant();
+        v->tuned_i8 = 0;
         v->tuned_xcd = 1;
         v->order = v->forced_order >= 0 ? v->forced_order : 0;
         v->sched = v->forced_sched >= 0 ? v->forced_sched : 0;
@@ -1696,7 +1698,10 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             std::vector<Cfg> cfgs;
             for (int order = 0; order < 2; ++order) {
                 if (P.i8) {
+                    // the four-wave kernel (256-row blocks) on every form, the
+                    // eight-wave one (code 7, 512-row blocks) on runs
                     cfgs.push_back({order, 0, 0, 1});
+                    cfgs.push_back({order, 0, 7, 1});
                     cfgs.push_back({order, 1, 0, 1});
                     cfgs.push_back({order, 2, 0, 0});
                 } else {
@@ -1711,6 +1716,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             // flat order (every pair, no cell padding): runs only
             if (P.i8) {
                 cfgs.push_back({2, 0, 0, 1});
+                cfgs.push_back({2, 0, 7, 1});
             } else {
                 for (int var : {17, 13}) cfgs.push_back({2, 0, var, 1});
             }
@@ -1731,7 +1737,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
                     RNNG_HIP_TRY(hipEventSynchronize(ev.e[4]));
                     t = ev.ms(3, 4);
                 }
-                v->sched_cfg.push_back(c.order << 8 | c.sched << 4 | (c.xcd & 1));
+                v->sched_cfg.push_back(c.var << 16 | c.order << 8 | c.sched << 4 | (c.xcd & 1));
                 v->sched_ms.push_back(t);
                 if (!P.i8 && c.order == 0 && c.sched == 0 && k < 8) v->tune_ms[k] = t;
                 if (t < best) {
@@ -1739,14 +1745,15 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
                     v->order = c.order;
                     v->sched = c.sched;
                     v->tuned_xcd = c.xcd;
-                    if (!P.i8) v->tuned_variant = c.var;
+                    if (P.i8) v->tuned_i8 = c.var;
+                    else v->tuned_variant = c.var;
                 }
             }
         }
         if (!P.i8 && v->sched > 0 && v->tuned_variant != 17 && v->tuned_variant != 13) v->tuned_variant = 17;
         RNNG_TRY(apply_layout(v->order));
     }
-    const int variant = v->tuned_variant;
+    const int variant = P.i8 ? v->tuned_i8 : v->tuned_variant;
     const int sched = v->sched;
     int64_t n_launch = 0;
     const int64_t covered = work_list(sched, variant, v->tuned_xcd, 1, d_items, &n_launch);
@@ -2145,7 +2152,10 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
         v->fill_exec = nullptr;
         v->fill_key.clear();
     }
-    if (v->autotune >= 0) v->tuned_variant = -1;   // choose again at the next build
+    if (v->autotune >= 0) {   // choose again at the next build
+        v->tuned_variant = -1;
+        v->tuned_i8 = 0;
+    }
     return RNNG_OK;
 }
 
@@ -2248,6 +2258,7 @@ int rnng_voronoi_set_schedule(rnng_voronoi_t v, int32_t order, int32_t sched) {
     v->forced_order = order < 0 ? -1 : order;
     v->forced_sched = sched < 0 ? -1 : sched;
     v->tuned_variant = -1;   // lay out and choose again at the next build
+    v->tuned_i8 = 0;
     v->order = -1;
     v->sched = -1;
     return RNNG_OK;

@@ -110,14 +110,16 @@ class Voronoi:
         check(_lib.lib().rnng_voronoi_set_schedule(self._h, int(order), int(sched)))
 
     def schedule(self):
-        """(order, form, [(order, form, xcd, ms) of every tuned configuration])."""
+        """(order, form, [(order, form, xcd, ms, shape) of every tuned
+        configuration]); shape: the BF16 variant or the int8 code (0: the cell
+        default)."""
         o, s, nc = C.c_int32(), C.c_int32(), C.c_int32()
         cfg = np.zeros(64, np.int32)
         ms = np.zeros(64, np.float64)
         check(_lib.lib().rnng_voronoi_schedule(self._h, C.byref(o), C.byref(s), _vp(cfg), _vp(ms), 64,
                                                C.byref(nc)))
         k = min(nc.value, 64)
-        return o.value, s.value, [(int(c) >> 8, (int(c) >> 4) & 15, int(c) & 1, float(t))
+        return o.value, s.value, [((int(c) >> 8) & 255, (int(c) >> 4) & 15, int(c) & 1, float(t), int(c) >> 16)
                                   for c, t in zip(cfg[:k], ms[:k])]
 
     def ghosts(self) -> np.ndarray:

@@ -122,8 +122,8 @@ def test_every_schedule_gives_the_oracle_edges(orc, kind):
         ij, w, gh, _ = _edges(v, eps)
         order, sched, cfgs = v.schedule()
         flush_print(f"tuned: order {order} form {sched} "
-                    + " ".join(f"{o}/{s}/{x}:{t:.3f}" for o, s, x, t in cfgs))
-        assert len(cfgs) >= 6 and all(t > 0 for *_, t in cfgs)
+                    + " ".join(f"{o}/{s}/{x}/{k}:{t:.3f}" for o, s, x, t, k in cfgs))
+        assert len(cfgs) >= 6 and all(c[3] > 0 for c in cfgs)
         assert np.array_equal(ij, ij_ref) and w.tobytes() == w_ref.tobytes() and np.array_equal(gh, gh0)
     finally:
         costmodel.set_cost_model(None)
