@@ -163,22 +163,27 @@ __global__ void k_edge_groups(const uint32_t* __restrict__ bits, const uint32_t*
 }
 
 // Triangles (u, v, w), u < v < w, through lower edge e = (v, w): the common
-// smaller neighbours u < v of w and v.  Eight lanes per edge: the shorter of
-// the two sorted lists (lower(w) below v, lower(v)) is read in rounds of
-// eight elements, each lane binary-searches its element in the other list,
-// and a round's matches are compacted in lane order, so each edge's
-// triangles come out ascending in u and the group's stores are contiguous.
-// cv packs (group << 32 | vertex), so a probe that matches also yields the
-// edge's value group.  FILL = false counts, FILL = true writes group and id at
+// smaller neighbours u < v of w and v.  Eight lanes per edge: the vertex keys
+// of the longer of the two sorted lists (lower(w) below v, lower(v)) are first
+// copied into the group's LDS slot (up to LCAP, coalesced; a longer list is
+// searched in global memory), then the shorter one is read in rounds of eight
+// elements, each lane binary-searches its element in the longer list, and a
+// round's matches are compacted in lane order, so each edge's triangles come
+// out ascending in u and the group's stores are contiguous.  cv packs
+// (group << 32 | vertex), so a probe that matches also yields the edge's
+// value group.  FILL = false counts, FILL = true writes group and id at
 // offset[e].
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_triangles(
     const int* __restrict__ row, const int64_t* __restrict__ rowptr, const uint64_t* __restrict__ cv, int64_t E,
     int64_t* __restrict__ count, const int64_t* __restrict__ offset, uint32_t* __restrict__ tg,
     int64_t* __restrict__ tid) {
+    constexpr int LCAP = 64;
+    __shared__ uint32_t slot_all[32][LCAP];   // vertex keys, one slot per 8-lane group of the workgroup
     const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3;
     const int j = threadIdx.x & 7;
     const int gshift = threadIdx.x & 56;   // first lane of the group in the wave
+    uint32_t* slot = slot_all[threadIdx.x >> 3];
     if (e >= E) return;                    // the 8 lanes of a group leave together
     const uint64_t ce = cv[e];
     const int v = (int)(uint32_t)ce, w = row[e];
@@ -186,6 +191,14 @@ __global__ __launch_bounds__(256) void k_triangles(
     const int64_t a0 = rowptr[w], a1 = e, b0 = rowptr[v], b1 = rowptr[v + 1];
     const bool sw = (a1 - a0) > (b1 - b0);
     const int64_t s0 = sw ? b0 : a0, s1 = sw ? b1 : a1, l0 = sw ? a0 : b0, l1 = sw ? a1 : b1;
+    const int64_t L = l1 - l0;
+    const bool staged = L <= LCAP;
+    if (staged && s1 > s0)
+        for (int64_t k = j; k < L; k += 8) slot[k] = (uint32_t)cv[l0 + k];
+    // the group's lanes read what the others wrote: LDS is in order within a wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int64_t out = FILL ? offset[e] : 0;
     const int64_t base = binom2(v) + binom3(w) + kP2;
     for (int64_t r = s0; r < s1; r += 8) {
@@ -195,15 +208,24 @@ __global__ __launch_bounds__(256) void k_triangles(
         if (k < s1) {
             se = cv[k];
             const uint32_t key = (uint32_t)se;
-            int64_t lo = l0, hi = l1;
-            while (lo < hi) {
-                const int64_t mid = (lo + hi) >> 1;
-                if ((uint32_t)cv[mid] < key) lo = mid + 1;
-                else hi = mid;
-            }
-            if (lo < l1) {
-                le = cv[lo];
-                hit = (uint32_t)le == key;
+            int64_t lo = 0, hi = L;
+            if (staged) {
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (slot[mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                hit = lo < L && slot[lo] == key;
+                // the group of a match from global memory (only the fill needs it)
+                if (FILL && hit) le = cv[l0 + lo];
+            } else {
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if ((uint32_t)cv[l0 + mid] < key) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo < L) le = cv[l0 + lo];
+                hit = lo < L && (uint32_t)le == key;
             }
         }
         const uint32_t gm = (uint32_t)(__ballot(hit) >> gshift) & 0xffu;
