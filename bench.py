@@ -233,7 +233,9 @@ def bench_cells(args):
     kname = ("k_l2_mfma_filter_ring<int8>" if int8 else
              {12: "k_l2_mfma_filter_ring<bf16, G=2, S=3>", 13: "k_l2_mfma_filter_ring<bf16, G=2, S=2>",
               14: "k_l2_mfma_filter_ring<bf16, G=3, S=2>",
-              17: "k_l2_mfma_filter_ring<bf16, G=8, S=2, 12 waves>"}.get(tv, "k_l2_mfma_filter_ring"))
+              17: "k_l2_mfma_filter_ring<bf16, G=8, S=2, 12 waves>",
+              18: "k_l2_mfma_filter_ring<bf16 16x16, G=8, S=2, 12 waves>",
+              19: "k_l2_mfma_filter_ring<bf16 16x16, G=2, S=2>"}.get(tv, "k_l2_mfma_filter_ring"))
     if form > 0:
         kname += ", tile lists" + (", persistent" if form == 2 else "")
     out["roofline"] = {"kernel": kname, "bound": "mfma", "achieved": ach, "peak": peak,

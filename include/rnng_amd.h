@@ -367,10 +367,13 @@ int rnng_voronoi_set_options(rnng_voronoi_t v, int32_t use_graph, int32_t autotu
 /* Filter configuration in use (the BF16 variant of rnng_set_filter_kernels:
  * the autotuner times 17 = twelve waves sharing two LDS stages of 8 column
  * tiles, 13 / 14 = four waves with two stages of 2 / 3 tiles, 12 = four waves
- * with three stages of 2 tiles; xcd_order: the XCD-ordered work list) and the
- * autotuner's timings in ms of those 4 variants on runs in landmark order,
- * each without and with the XCD order (config_ms may be NULL; zeros if not
- * tuned, and on integer data, whose int8 filter has one cell shape).
+ * with three stages of 2 tiles, or, when the kernels in use are the 16x16
+ * forms and the k-step count is even, 19 and 18 (the 16x16 forms of 13 and
+ * 17); xcd_order: the XCD-ordered work list) and the autotuner's timings in ms
+ * of those variants on runs in landmark order, each without and with the XCD
+ * order (8 values: 4 variants, or 19 and 18 in the first 4 and zeros after;
+ * config_ms may be NULL; zeros if not tuned, and on integer data, whose int8
+ * filter has one cell shape).
  * rnng_voronoi_schedule reports every configuration the tuner timed. */
 int rnng_voronoi_tuning(rnng_voronoi_t v, int32_t *variant, int32_t *xcd_order, double *config_ms);
 
@@ -466,13 +469,17 @@ int rnng_set_int8_filter(int32_t enable);
  * staged through registers, 1/2 direct-to-LDS ring with three stages of 1/2
  * tiles, 3/4 two stages of 3/4 tiles, 5/6/7 two stages of 4/6/8 tiles shared
  * by eight waves of two row tiles, 8 two stages of 8 tiles shared by four waves
- * of four row tiles (two waves per SIMD) (all-pairs builds; the cell path runs
- * code 3 for codes above 4).  bf16_variant 0..17 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
+ * of four row tiles (two waves per SIMD), 9/10 the 16x16 MFMA forms
+ * (v_mfma_i32_16x16x64_i8, 16-row fragment layout) of 7/3 (all-pairs builds;
+ * the cell path runs code 3 for codes 5-8, and code 10 for 9 and 10).
+ * bf16_variant 0..19 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
  * tiles, 15 two stages of 4 tiles shared by 8 waves, 16/17 two stages of 4/8
- * tiles shared by 12 waves, the others register-staged or group-load shapes).
- * A negative value leaves that setting unchanged; the defaults (7, 17) come from
- * RNNG_L2_I8_GROUP / RNNG_L2_VARIANT when set.  Every shape gives the same
- * edges. */
+ * tiles shared by 12 waves, 18/19 the 16x16 forms (v_mfma_f32_16x16x32_bf16)
+ * of 17/13, the others register-staged or group-load shapes).  The 16x16 forms
+ * need an even k-step count (int8: ceil(d / 32), BF16: ceil(d / 16)) and run
+ * as their 32x32 forms otherwise.  A negative value leaves that setting
+ * unchanged; the defaults (9, 19) come from RNNG_L2_I8_GROUP / RNNG_L2_VARIANT
+ * when set.  Every shape gives the same edges. */
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant);
 int rnng_get_filter_kernels(int32_t *int8_code, int32_t *bf16_variant);
 

@@ -322,7 +322,7 @@ constexpr float kPadNorm = 1.2676506e30f;   // 2^100
 __global__ __launch_bounds__(256) void k_prep_frags(
     const float* __restrict__ X, int n, int n_pad, int d, int KS, const float* __restrict__ mu,
     const int* __restrict__ idmap, uint16_t* __restrict__ xa, uint16_t* __restrict__ yb,
-    float* __restrict__ xn, float* __restrict__ yn, unsigned* __restrict__ stat) {
+    float* __restrict__ xn, float* __restrict__ yn, unsigned* __restrict__ stat, int l16) {
     extern __shared__ float xs[];          // [PP][d + 1]: x~ = bf16(x - mu) as float
     __shared__ int preal[PP];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -383,7 +383,10 @@ __global__ __launch_bounds__(256) void k_prep_frags(
             xv[e] = f32_to_bf16_rn(xf);
             yv[e] = f32_to_bf16_rn(-2.0f * xf);
         }
-        const size_t slot = ((size_t)((p0 >> 5) + tt) * KS * 64 + rem) * 8;
+        // chunk ch = 2 st + (l >> 5) (dims 8 ch .. 8 ch + 7) of tile point l & 31
+        const int ch = 2 * st + (l >> 5), pt = l & 31;
+        const int word = l16 ? (((pt >> 4) * (KS / 2) + (ch >> 2)) * 64 + 16 * (ch & 3) + (pt & 15)) : rem;
+        const size_t slot = ((size_t)((p0 >> 5) + tt) * KS * 64 + word) * 8;
         if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
         if (yb) *(uint4*)(yb + slot) = *(const uint4*)yv;
     }
@@ -396,11 +399,11 @@ __global__ __launch_bounds__(256) void k_prep_frags(
 }
 
 int launch_prep(const float* X, int n, int n_pad, int d, int KS, const float* mu, const int* idmap,
-                uint16_t* xa, uint16_t* yb, float* xn, float* yn, unsigned* stat) {
+                uint16_t* xa, uint16_t* yb, float* xn, float* yn, unsigned* stat, bool l16) {
     if (n_pad <= 0) return RNNG_OK;
     const size_t lds = (size_t)PP * (d + 1) * sizeof(float);
     k_prep_frags<<<blocks_for(n_pad, PP), 256, lds, stream()>>>(X, n, n_pad, d, KS, mu, idmap, xa, yb, xn, yn,
-                                                                stat);
+                                                                stat, l16 ? 1 : 0);
     RNNG_HIP_TRY(hipGetLastError());
     return RNNG_OK;
 }
@@ -869,7 +872,7 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
                                                  const unsigned* __restrict__ colmin, int Tq,
                                                  i32x4_t* __restrict__ xa, i32x4_t* __restrict__ yb,
                                                  int* __restrict__ rterm, int* __restrict__ cterm,
-                                                 int* __restrict__ hsq, i32x4_t* __restrict__ xrow) {
+                                                 int* __restrict__ hsq, i32x4_t* __restrict__ xrow, int l16) {
     __shared__ int off[160];      // o_k + 128 per dimension (d <= 160)
     __shared__ int sh[4][64], ssx[4][64];
     const int tile = blockIdx.x;
@@ -914,7 +917,10 @@ __global__ __launch_bounds__(256) void k_prep_i8(const float* __restrict__ X, in
             wa[j >> 2] |= (unsigned)(xv & 0xff) << (8 * (j & 3));
             wb[j >> 2] |= (unsigned)(yv & 0xff) << (8 * (j & 3));
         }
-        const size_t idx = ((size_t)tile * KS8 + s) * 64 + lane;
+        // chunk c = 2 s + half (dims 16 c .. 16 c + 15) of tile point p = lane & 31
+        const int c = 2 * s + half, p = lane & 31;
+        const int word = l16 ? (((p >> 4) * (KS8 / 2) + (c >> 2)) * 64 + 16 * (c & 3) + (p & 15)) : s * 64 + lane;
+        const size_t idx = (size_t)tile * KS8 * 64 + word;
         if (xa) xa[idx] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
         if (yb) yb[idx] = i32x4_t{(int)wb[0], (int)wb[1], (int)wb[2], (int)wb[3]};
         if (xrow) xrow[(size_t)a * (2 * KS8) + 2 * s + half] = i32x4_t{(int)wa[0], (int)wa[1], (int)wa[2], (int)wa[3]};
@@ -1004,8 +1010,13 @@ __global__ __launch_bounds__(256) void k_recheck_i8(const i32x4_t* __restrict__ 
 struct MfI8 {
     using frag = i32x4_t;
     using accv = i32x16_t;
+    using accv16 = i32x4_t;
     using term = int;
+    static constexpr term kHigh = 0x7fffffff, kLow = -0x7fffffff - 1;
     __device__ static accv mfma(frag a, frag b, accv c) { return __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0); }
+    __device__ static accv16 mfma16(frag a, frag b, accv16 c) {
+        return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+    }
     __device__ static term vmin(term a, term b) { return min(a, b); }
     // v < t; v - t stays inside int32 for these start terms
     __device__ static unsigned below(term v, term t) { return (unsigned)(v - t) >> 31; }
@@ -1013,9 +1024,14 @@ struct MfI8 {
 struct MfBF16 {
     using frag = bf16x8_t;
     using accv = f32x16_t;
+    using accv16 = f32x4_t;
     using term = float;
+    static constexpr term kHigh = 3.402823466e38f, kLow = -3.402823466e38f;
     __device__ static accv mfma(frag a, frag b, accv c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+    __device__ static accv16 mfma16(frag a, frag b, accv16 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
     __device__ static term vmin(term a, term b) { return fminf(a, b); }
     __device__ static unsigned below(term v, term t) { return v < t ? 1u : 0u; }
@@ -1066,6 +1082,47 @@ __device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)
     return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
 }
 
+// The lanes walk their own set bits of mask together (value i of a lane is
+// row rowb + RS * (i >> 2) + (i & 3)): one ballot and one LDS reservation per
+// round, candidates staged in LDS, past the stage one global reservation per
+// round for the wave's overflow.
+template <class mask_t, int RS>
+__device__ __forceinline__ void mf_walk(mask_t mask, int rowb, int col, int lane, int2* stage, unsigned stage_cap,
+                                        unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
+                                        unsigned long long* __restrict__ counter) {
+#pragma unroll 1
+    for (;;) {
+        const bool p = mask != 0;
+        const unsigned long long m = __ballot(p);
+        if (m == 0) break;
+        int row = 0;
+        if (p) {
+            const int rq = sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)mask) : __builtin_ctz((unsigned)mask);
+            mask &= mask - 1;
+            row = rowb + RS * (rq >> 2) + (rq & 3);
+        }
+        unsigned slot = 0;
+        if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
+        slot = __shfl(slot, 0, 64) +
+               __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (p && slot < stage_cap) stage[slot] = make_int2(row, col);
+        // past the stage (dense tiles, e.g. clusters in cell order): one global
+        // reservation per round for the wave's overflowing candidates
+        const bool ovf = p && slot >= stage_cap;
+        const unsigned long long mo = __ballot(ovf);
+        if (mo) {
+            unsigned long long gb = 0;
+            if (lane == 0) gb = atomicAdd(counter, (unsigned long long)__popcll(mo));
+            gb = __shfl(gb, 0, 64);
+            if (ovf) {
+                const unsigned long long g =
+                    gb + __builtin_amdgcn_mbcnt_hi((unsigned)(mo >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mo, 0u));
+                if (g < cap) cand[g] = make_int2(row, col);
+            }
+        }
+    }
+}
+
 // Rare path: candidates of one tile pair, staged in LDS.  Each lane builds
 // the mask of its 16 M values below nyc (value i = r*16+q of a lane is row
 // rowb + 8*(i >> 2) + (i & 3), rising with i), drops the rows at or past its
@@ -1095,37 +1152,78 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[M], typena
     const int L = lim - rowb;
     const int k = L <= 0 ? 0 : (L >= 2 * NV ? NV : 4 * (L >> 3) + min(4, L & 7));
     mask &= k >= NV ? ~(mask_t)0 : (((mask_t)1 << k) - (mask_t)1);
-#pragma unroll 1
-    for (;;) {
-        const bool p = mask != 0;
-        const unsigned long long m = __ballot(p);
-        if (m == 0) break;
-        int row = 0;
-        if (p) {
-            const int rq = M == 4 ? __builtin_ctzll((unsigned long long)mask) : __builtin_ctz((unsigned)mask);
-            mask &= mask - 1;
-            row = rowb + 8 * (rq >> 2) + (rq & 3);
-        }
-        unsigned slot = 0;
-        if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
-        slot = __shfl(slot, 0, 64) +
-               __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (p && slot < stage_cap) stage[slot] = make_int2(row, col);
-        // past the stage (dense tiles, e.g. clusters in cell order): one global
-        // reservation per round for the wave's overflowing candidates
-        const bool ovf = p && slot >= stage_cap;
-        const unsigned long long mo = __ballot(ovf);
-        if (mo) {
-            unsigned long long gb = 0;
-            if (lane == 0) gb = atomicAdd(counter, (unsigned long long)__popcll(mo));
-            gb = __shfl(gb, 0, 64);
-            if (ovf) {
-                const unsigned long long g =
-                    gb + __builtin_amdgcn_mbcnt_hi((unsigned)(mo >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mo, 0u));
-                if (g < cap) cand[g] = make_int2(row, col);
-            }
-        }
+    mf_walk<mask_t, 8>(mask, rowb, col, lane, stage, stage_cap, scount, cand, cap, counter);
+}
+
+// 16x16 MFMA forms of the filter (v_mfma_i32_16x16x64_i8 and
+// v_mfma_f32_16x16x32_bf16), on the 16-row fragment layout: a 32-point tile
+// is two 16-point halves of KH = KS / 2 k-steps each; word (half hb, k-step
+// s, lane l) holds point 16 hb + (l & 15) and its k-chunk l >> 4 of k-step s
+// (16 int8 or 8 bf16 values; the same 16-byte words as the 32x32 layout,
+// permuted).  A wave keeps its 64 rows as four 16-row blocks: a[r][s] and the
+// row start terms in the 16x16 accumulator layout (lane l: rows 4 (l >> 4) +
+// q of the block, column l & 15).  A 32-column tile runs as two 16-column
+// halves with their own column terms; each half's test is made while the
+// next half's MFMAs run.
+template <class T, int KS>
+__device__ __forceinline__ void mf16_load_rows(const typename T::frag* __restrict__ xa,
+                                               const typename T::term* __restrict__ rterm, int rt0, int lane,
+                                               typename T::frag (&a)[4][KS / 2], typename T::accv16 (&xr)[4]) {
+    constexpr int KH = KS / 2;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int s = 0; s < KH; ++s) a[r][s] = xa[((size_t)(rt0 + (r >> 1)) * KS + (r & 1) * KH + s) * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xr[r][q] = rterm[rt0 * 32 + 16 * r + 4 * (lane >> 4) + q];
+}
+
+// one 16-column half held in LDS (b: its first k-step)
+template <class T, int KS>
+__device__ __forceinline__ void mf16_half(const typename T::frag* b, const typename T::frag (&a)[4][KS / 2],
+                                          const typename T::accv16 (&xr)[4], int lane,
+                                          typename T::accv16 (&acc)[4]) {
+#pragma unroll
+    for (int s = 0; s < KS / 2; ++s) {
+        const typename T::frag bb = b[s * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = T::mfma16(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
     }
+}
+
+// minimum of a lane's 16 values (four chains of four)
+template <class T>
+__device__ __forceinline__ typename T::term mf16_min(const typename T::accv16 (&acc)[4]) {
+    typename T::term c[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) c[h] = T::vmin(T::vmin(acc[h][0], acc[h][1]), T::vmin(acc[h][2], acc[h][3]));
+    return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
+}
+
+// Rare path of a half: value i = 4 r + q of a lane is row rowb + 16 (i >> 2) +
+// (i & 3) (rising with i) of column col; the rows at or past the lane's limit
+// are dropped as a bit count, as in mf_emit.
+template <class T>
+__device__ __forceinline__ void mf16_emit(const typename T::accv16 (&acc)[4], typename T::term nyc, int col, int rt0,
+                                          int lane, int n_rows, int n_cols, int self_join, int2* stage,
+                                          unsigned stage_cap, unsigned* scount, int2* __restrict__ cand,
+                                          unsigned long long cap, unsigned long long* __restrict__ counter) {
+    const int rowb = rt0 * 32 + 4 * (lane >> 4);
+    int opaque;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
+    unsigned mask = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) mask |= T::below(acc[r][q], nyc) << (r * 4 + q + opaque);
+    const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
+    // values with row offset 16*(i >> 2) + (i & 3) < L are the first k
+    const int L = lim - rowb;
+    const int k = L <= 0 ? 0 : (L >= 64 ? 16 : 4 * (L >> 4) + min(4, L & 15));
+    mask &= (1u << k) - 1u;
+    mf_walk<unsigned, 16>(mask, rowb, col, lane, stage, stage_cap, scount, cand, cap, counter);
 }
 
 // The int8 counterpart of k_l2_mfma_filter<KS, 2, true, 1>: same work list,
@@ -1309,7 +1407,11 @@ __device__ __forceinline__ void mf_group_load(const typename T::frag* __restrict
 // them (held in registers for the item's whole run).  A persistent
 // grid (PQ) takes items queue[0]++ (in list order) in each workgroup until
 // the list is exhausted; otherwise workgroup b runs item b.
-template <class T, int KS, int G, int MINW, int S = 3, int NW = 4, bool TL = false, bool PQ = false, int M = MRT>
+//
+// L16: the 16x16 MFMA forms on the 16-row fragment layout (M = 2 row tiles
+// per wave as four 16-row blocks; KS even).
+template <class T, int KS, int G, int MINW, int S = 3, int NW = 4, bool TL = false, bool PQ = false, int M = MRT,
+          bool L16 = false>
 __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     const typename T::frag* __restrict__ xa, const typename T::frag* __restrict__ yb,
     const typename T::term* __restrict__ rterm, const typename T::term* __restrict__ cterm,
@@ -1319,10 +1421,12 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     using frag = typename T::frag;
     using term = typename T::term;
     using accv = typename T::accv;
+    using accv16 = typename T::accv16;
     constexpr int NCH = KS * 64;
     constexpr int NJ = G * KS;
     constexpr int NC = 64 * ((G + 1) / 2);   // start terms per stage (two tiles per load)
     static_assert(G >= 1 && G <= 16 && (S == 2 || S == 3), "ring shape");
+    static_assert(!L16 || (M == 2 && KS % 2 == 0), "16x16 forms: two row tiles per wave, even k-steps");
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
     // candidates staged in LDS; the int8 shape with four tiles per stage keeps
@@ -1361,9 +1465,21 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     // the rows
     TileIdx ti{};
     if constexpr (TL) ti = load_tile_idx(ctl, wi.y, wi.z, lane);
-    frag a[M][KS];
-    accv xr[M];
-    mf_load_rows<T, KS, M>(xa, rterm, rt0, lane, a, xr);
+    frag a[L16 ? 1 : M][KS];
+    accv xr[L16 ? 1 : M];
+    frag a16[L16 ? 4 : 1][L16 ? KS / 2 : 1];
+    accv16 xr16[L16 ? 4 : 1];
+    // L16: the half whose test is pending (initially none: no value is below kLow)
+    accv16 accp[L16 ? 4 : 1];
+    term nycp = T::kLow;
+    int colp = 0;
+    if constexpr (L16) {
+        mf16_load_rows<T, KS>(xa, rterm, rt0, lane, a16, xr16);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accp[r] = accv16{T::kHigh, T::kHigh, T::kHigh, T::kHigh};
+    } else {
+        mf_load_rows<T, KS, M>(xa, rterm, rt0, lane, a, xr);
+    }
     asm volatile("" ::: "memory");
     mf_group_load<T, KS, G, NW, TL>(yb, cterm, bs0, cs0, wi.y, last, wave, lane, ti);
     asm volatile("" ::: "memory");
@@ -1382,13 +1498,31 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
 #pragma unroll
         for (int cc = 0; cc < G; ++cc) {
             if (cc > 0 && t0 + cc > last) break;
-            const term nyc = -ccur[cc * 32 + (lane & 31)];
-            accv acc[M];
-            mf_tile<T, KS, M>(cur + cc * NCH, a, xr, lane, acc);
-            if (__any(mf_min<T, M>(acc) < nyc)) {
+            if constexpr (L16) {
                 const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
-                mf_emit<T, M>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
-                              counter);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const term nyc = -ccur[cc * 32 + 16 * h + (lane & 15)];
+                    accv16 acc[4];
+                    mf16_half<T, KS>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
+                    // the previous half's test, under this half's MFMAs
+                    if (__any(mf16_min<T>(accp) < nycp))
+                        mf16_emit<T>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount,
+                                     cand, cap, counter);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) accp[r] = acc[r];
+                    nycp = nyc;
+                    colp = ct * 32 + 16 * h + (lane & 15);
+                }
+            } else {
+                const term nyc = -ccur[cc * 32 + (lane & 31)];
+                accv acc[M];
+                mf_tile<T, KS, M>(cur + cc * NCH, a, xr, lane, acc);
+                if (__any(mf_min<T, M>(acc) < nyc)) {
+                    const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
+                    mf_emit<T, M>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand,
+                                  cap, counter);
+                }
             }
         }
         ring_barrier(S == 3 && issue ? nld : 0);
@@ -1410,6 +1544,11 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
             step(bs1, cs1, bs0, cs0, g);
             if (++g >= ng) break;
         }
+    }
+    if constexpr (L16) {   // the last half's test
+        if (__any(mf16_min<T>(accp) < nycp))
+            mf16_emit<T>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
+                         counter);
     }
     __syncthreads();   // every wave's staged candidates and count
     const unsigned nst = min(scount, STG);
@@ -1624,18 +1763,20 @@ struct FilterVariant {
 // Variants 7, 8, 9 and 10 are the group kernel (direct-to-LDS loads) with G = 1..4;
 // 11 and 12 the direct-to-LDS ring (three stages of 1 or 2 tiles), 13 and 14
 // the ring with two stages of 2 or 3 tiles; 15 two stages of 4 tiles shared by
-// 8 waves, 16 and 17 two stages of 4 or 8 tiles shared by 12 waves.
+// 8 waves, 16 and 17 two stages of 4 or 8 tiles shared by 12 waves; 18 and 19
+// the 16x16 MFMA forms of 17 and 13 (16-row fragment layout).
 constexpr FilterVariant kVariants[] = {{2, false, 1, 4}, {2, true, 1, 4}, {4, true, 1, 4}, {3, true, 1, 4},
                                        {2, true, 2, 4},  {1, true, 4, 4}, {1, true, 2, 4},
                                        {2, true, 1, 4},  {2, true, 2, 4}, {2, true, 3, 4}, {2, true, 4, 4},
                                        {2, true, 1, 4},  {2, true, 2, 4}, {2, true, 2, 4}, {2, true, 3, 4},
-                                       {2, true, 4, 8},  {2, true, 4, 12}, {2, true, 8, 12}};
-constexpr int kNumVariants = 18;
+                                       {2, true, 4, 8},  {2, true, 4, 12}, {2, true, 8, 12},
+                                       {2, true, 8, 12}, {2, true, 2, 4}};
+constexpr int kNumVariants = 20;
 int g_variant = -1;   // -1: from RNNG_L2_VARIANT or the default
 
 int filter_variant() {
     if (g_variant < 0) {
-        g_variant = 17;
+        g_variant = 19;
         if (const char* e = getenv("RNNG_L2_VARIANT")) {
             const int v = atoi(e);
             if (v >= 0 && v < kNumVariants) g_variant = v;
@@ -1721,15 +1862,19 @@ void launch_filter_g(const FilterArgs& a) {
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.prm, a.cand, a.cap, a.counter);
 }
 
-template <int KS, int G, int S = 3, int NW = 4>
+template <int KS, int G, int S = 3, int NW = 4, bool L16 = false>
 void launch_filter_ring(const FilterArgs& a) {
-    k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S, NW><<<a.n_items, 64 * NW, 0, stream()>>>(
+    k_l2_mfma_filter_ring<MfBF16, KS, G, 1, S, NW, false, false, MRT, L16><<<a.n_items, 64 * NW, 0, stream()>>>(
         a.xa, a.yb, a.xn, a.ycn, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter, nullptr,
         a.n_items, nullptr);
 }
 
 template <int KS>
 void launch_filter_v(const FilterArgs& a, int variant) {
+    if constexpr (KS % 2 == 0) {
+        if (variant == 18) return launch_filter_ring<KS, (KS >= 9 ? 6 : 8), 2, 12, true>(a);
+        if (variant == 19) return launch_filter_ring<KS, 2, 2, 4, true>(a);
+    }
     switch (variant) {
         case 11: launch_filter_ring<KS, 1>(a); return;
         case 12: launch_filter_ring<KS, 2>(a); return;
@@ -1768,7 +1913,9 @@ float ord2f_host(unsigned u) {
 // through registers); 1, 2 = the direct-to-LDS ring with 1 or 2 tiles per
 // stage and three stages; 3, 4 = 3 or 4 tiles per stage and two stages; 5, 6,
 // 7 = two stages of 4, 6 or 8 tiles shared by eight waves (512 threads,
-// 16 row tiles per work item; 6 tiles at KS8 = 5 for two workgroups per CU).
+// 16 row tiles per work item; 6 tiles at KS8 = 5 for two workgroups per CU);
+// 8 = four waves of four row tiles; 9, 10 = the 16x16 forms of 7 and 3 (the
+// default: code 9).
 template <int KS8, int CODE>
 void launch_filter_i8(const I8Args& a) {
     constexpr int MINW = KS8 <= 4 ? 4 : 1;
@@ -1788,26 +1935,37 @@ void launch_filter_i8(const I8Args& a) {
         k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8><<<a.n_items, 512, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
             nullptr, a.n_items, nullptr);
-    } else {
+    } else if constexpr (CODE == 8) {
         // four waves of four row tiles each (the same 16 row tiles per item),
         // two waves per SIMD with twice the registers
         constexpr int G = KS8 >= 5 ? 6 : 8;
         k_l2_mfma_filter_ring<MfI8, KS8, G, 2, 2, 4, false, false, 4><<<a.n_items, 256, 0, stream()>>>(
             a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
             nullptr, a.n_items, nullptr);
+    } else if constexpr (KS8 % 2 == 0) {
+        if constexpr (CODE == 9) {   // 16x16 form of code 7
+            constexpr int G = KS8 >= 5 ? 6 : 8;
+            k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8, false, false, MRT, true><<<a.n_items, 512, 0, stream()>>>(
+                a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+                nullptr, a.n_items, nullptr);
+        } else {                     // 16x16 form of code 3
+            k_l2_mfma_filter_ring<MfI8, KS8, 3, MINW, 2, 4, false, false, MRT, true><<<a.n_items, 256, 0, stream()>>>(
+                a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+                nullptr, a.n_items, nullptr);
+        }
     }
 }
 
-constexpr int kI8Codes = 9;
+constexpr int kI8Codes = 11;
 
-// row tiles per work item of an int8 kernel code (codes 5-8: 16 row tiles)
-int i8_row_tiles(int code) { return code >= 5 ? 16 : 8; }
+// row tiles per work item of an int8 kernel code (codes 5-9: 16 row tiles)
+int i8_row_tiles(int code) { return code >= 5 && code <= 9 ? 16 : 8; }
 
 int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
 
 int i8_group() {
     if (g_i8_group < 0) {
-        g_i8_group = 7;
+        g_i8_group = 9;
         if (const char* e = getenv("RNNG_L2_I8_GROUP")) {
             const int v = atoi(e);
             if (v >= 0 && v < kI8Codes) g_i8_group = v;
@@ -1837,7 +1995,15 @@ I8Args i8_args(const Workspace& ws, int nx, int ny, int self_join) {
 
 int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
     if (a.n_items == 0) return RNNG_OK;
+    if (a.l16 != (code == 9 || code == 10) || (a.l16 && KS8 % 2 != 0)) {
+        set_error("int8 filter: the kernel does not match the fragment layout");
+        return RNNG_ERR_ARG;
+    }
     switch (KS8 * 16 + code) {
+        case 2 * 16 + 9: launch_filter_i8<2, 9>(a); break;
+        case 2 * 16 + 10: launch_filter_i8<2, 10>(a); break;
+        case 4 * 16 + 9: launch_filter_i8<4, 9>(a); break;
+        case 4 * 16 + 10: launch_filter_i8<4, 10>(a); break;
         case 1 * 16 + 0: launch_filter_i8<1, 0>(a); break;
         case 1 * 16 + 1: launch_filter_i8<1, 1>(a); break;
         case 1 * 16 + 2: launch_filter_i8<1, 2>(a); break;
@@ -1893,9 +2059,9 @@ int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
 
 namespace {
 
-template <class T, int KS, int G, int MINW, int NW, bool PQ>
+template <class T, int KS, int G, int MINW, int NW, bool PQ, bool L16 = false>
 int launch_tiles(const TileFilterArgs& a) {
-    auto kern = k_l2_mfma_filter_ring<T, KS, G, MINW, 2, NW, true, PQ>;
+    auto kern = k_l2_mfma_filter_ring<T, KS, G, MINW, 2, NW, true, PQ, MRT, L16>;
     unsigned grid = a.n_items;
     if constexpr (PQ) {
         // resident workgroups of this shape on the whole device (once)
@@ -1924,12 +2090,20 @@ int launch_tiles(const TileFilterArgs& a) {
 
 template <int KS, bool PQ>
 int launch_tiles_bf16(const TileFilterArgs& a) {
+    if constexpr (KS % 2 == 0) {
+        if (a.variant == 19) return launch_tiles<MfBF16, KS, 2, 1, 4, PQ, true>(a);
+        if (a.variant == 18) return launch_tiles<MfBF16, KS, (KS >= 9 ? 6 : 8), 1, 12, PQ, true>(a);
+    }
     if (a.variant == 13) return launch_tiles<MfBF16, KS, 2, 1, 4, PQ>(a);
     return launch_tiles<MfBF16, KS, (KS >= 9 ? 6 : 8), 1, 12, PQ>(a);
 }
 
 template <bool PQ>
 int dispatch_tiles(const TileFilterArgs& a) {
+    if (a.i8 && a.l16) {   // the 16x16 form of the four-wave shape
+        if (a.ks == 2) return launch_tiles<MfI8, 2, 3, 4, 4, PQ, true>(a);
+        if (a.ks == 4) return launch_tiles<MfI8, 4, 3, 4, 4, PQ, true>(a);
+    }
     if (a.i8) {
         switch (a.ks) {
             case 1: return launch_tiles<MfI8, 1, 3, 4, 4, PQ>(a);
@@ -1962,7 +2136,9 @@ int dispatch_tiles(const TileFilterArgs& a) {
 
 int filter_dispatch_tiles(const TileFilterArgs& a) {
     if (a.n_items == 0 && !a.dry_run) return RNNG_OK;
-    if (!a.ctl || (a.persistent && !a.queue) || (!a.i8 && a.variant != 17 && a.variant != 13)) {
+    const bool bf_l32 = a.variant == 17 || a.variant == 13, bf_l16 = a.variant == 18 || a.variant == 19;
+    if (!a.ctl || (a.persistent && !a.queue) || (!a.i8 && !(a.l16 ? bf_l16 : bf_l32)) ||
+        (a.l16 && a.ks % 2 != 0)) {
         set_error("filter_dispatch_tiles: bad arguments");
         return RNNG_ERR_ARG;
     }
@@ -2127,8 +2303,10 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     }
     const int KS8 = (d + 31) / 32;
 
-    const int variant = filter_variant();
-    const int rb_tiles = i8 ? i8_row_tiles(i8_group()) : filter_block_tiles(variant);   // row tiles per workgroup
+    // the kernel that runs, and whether its fragments take the 16-row layout
+    int variant = 0, i8_code = 0;
+    const bool l16 = i8 ? l16_i8_code(i8_group(), KS8, &i8_code) : l16_variant(filter_variant(), KS, &variant);
+    const int rb_tiles = i8 ? i8_row_tiles(i8_code) : filter_block_tiles(variant);   // row tiles per workgroup
     const int rb_rows = 32 * rb_tiles;
     const int n_rowtiles = ((nx + rb_rows - 1) / rb_rows) * rb_tiles;
     const int n_coltiles = (ny + 31) / 32;
@@ -2177,14 +2355,17 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
             k_prep_i8<<<npad / 32, 256, 0, stream()>>>(A->X, nx, npad, d, KS8, nullptr, vec_ok(A->X),
                                                        ws.colmin.as<unsigned>(),
                                                        Tq, ws.xa.as<i32x4_t>(), ws.yb.as<i32x4_t>(), ws.xn.as<int>(),
-                                                       ws.ycn.as<int>(), ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
+                                                       ws.ycn.as<int>(), ws.hx.as<int>(), ws.xr8.as<i32x4_t>(),
+                                                       l16 ? 1 : 0);
         } else {
             k_prep_i8<<<npad_x / 32, 256, 0, stream()>>>(A->X, nx, npad_x, d, KS8, nullptr, vec_ok(A->X),
                                                          ws.colmin.as<unsigned>(), Tq, ws.xa.as<i32x4_t>(), nullptr,
-                                                         ws.xn.as<int>(), nullptr, ws.hx.as<int>(), ws.xr8.as<i32x4_t>());
+                                                         ws.xn.as<int>(), nullptr, ws.hx.as<int>(), ws.xr8.as<i32x4_t>(),
+                                                         l16 ? 1 : 0);
             k_prep_i8<<<npad_y / 32, 256, 0, stream()>>>(B->X, ny, npad_y, d, KS8, nullptr, vec_ok(B->X),
                                                          ws.colmin.as<unsigned>(), Tq, nullptr, ws.yb.as<i32x4_t>(),
-                                                         nullptr, ws.ycn.as<int>(), ws.hy.as<int>(), ws.yr8.as<i32x4_t>());
+                                                         nullptr, ws.ycn.as<int>(), ws.hy.as<int>(), ws.yr8.as<i32x4_t>(),
+                                                         l16 ? 1 : 0);
         }
         RNNG_HIP_TRY(hipGetLastError());
     } else if (self_join) {
@@ -2192,15 +2373,15 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
         // row norms cover npad_x rows, the column norms every padded row)
         RNNG_TRY(launch_prep(A->X, nx, npad, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
                              ws.yb.as<uint16_t>(), npad == npad_x ? ws.xn.as<float>() : nullptr,
-                             ws.yn.as<float>(), ws.stat.as<unsigned>()));
+                             ws.yn.as<float>(), ws.stat.as<unsigned>(), l16));
         if (npad != npad_x)
             RNNG_HIP_TRY(hipMemcpyAsync(ws.xn.ptr, ws.yn.ptr, (size_t)npad_x * sizeof(float),
                                         hipMemcpyDeviceToDevice, stream()));
     } else {
         RNNG_TRY(launch_prep(A->X, nx, npad_x, d, KS, ws.mu.as<float>(), nullptr, ws.xa.as<uint16_t>(),
-                             nullptr, ws.xn.as<float>(), nullptr, ws.stat.as<unsigned>()));
+                             nullptr, ws.xn.as<float>(), nullptr, ws.stat.as<unsigned>(), l16));
         RNNG_TRY(launch_prep(B->X, ny, npad_y, d, KS, ws.mu.as<float>(), nullptr, nullptr,
-                             ws.yb.as<uint16_t>(), nullptr, ws.yn.as<float>(), ws.stat.as<unsigned>()));
+                             ws.yb.as<uint16_t>(), nullptr, ws.yn.as<float>(), ws.stat.as<unsigned>(), l16));
     }
     RNNG_HIP_TRY(hipGetLastError());
     if (!i8) {
@@ -2217,8 +2398,15 @@ int run_mfma(const rnng_dataset_s* A, const rnng_dataset_s* B, bool self_join, f
     for (int attempt = 0; attempt < 3; ++attempt) {
         RNNG_TRY(launch_fill_u64(ws.counters.as<unsigned long long>(), 8, 0ull));
         RNNG_HIP_TRY(hipEventRecord(ev.e[2], stream()));
-        if (i8) RNNG_TRY(filter_dispatch_i8(i8_args(ws, nx, ny, self_join ? 1 : 0), KS8, i8_group()));
-        else RNNG_TRY(filter_dispatch(ws_args(ws, nx, ny, self_join ? 1 : 0), KS, variant));
+        if (i8) {
+            I8Args ia = i8_args(ws, nx, ny, self_join ? 1 : 0);
+            ia.l16 = l16;
+            RNNG_TRY(filter_dispatch_i8(ia, KS8, i8_code));
+        } else {
+            FilterArgs fa = ws_args(ws, nx, ny, self_join ? 1 : 0);
+            fa.l16 = l16;
+            RNNG_TRY(filter_dispatch(fa, KS, variant));
+        }
         RNNG_HIP_TRY(hipEventRecord(ev.e[3], stream()));
         unsigned long long ncand = 0;
         RNNG_HIP_TRY(hipMemcpyAsync(&ncand, ws.counters.ptr, sizeof(ncand), hipMemcpyDeviceToHost, stream()));
@@ -2298,6 +2486,10 @@ int filter_dispatch(const FilterArgs& a, int KS, int variant) {
         set_error("unknown filter variant");
         return RNNG_ERR_ARG;
     }
+    if (a.l16 != (variant == 18 || variant == 19) || (a.l16 && KS % 2 != 0)) {
+        set_error("BF16 filter: the kernel does not match the fragment layout");
+        return RNNG_ERR_ARG;
+    }
     if (a.n_items == 0) return RNNG_OK;
     switch (KS) {
         case 1: launch_filter_v<1>(a, variant); break;
@@ -2318,6 +2510,18 @@ int filter_dispatch(const FilterArgs& a, int KS, int variant) {
 }
 
 int filter_block_tiles(int variant) { return kVariants[variant].nw * kVariants[variant].rt; }
+
+bool l16_i8_code(int code, int ks8, int* run_code) {
+    const bool l16 = code == 9 || code == 10;
+    *run_code = l16 && ks8 % 2 != 0 ? (code == 9 ? 7 : 3) : code;
+    return l16 && ks8 % 2 == 0;
+}
+
+bool l16_variant(int variant, int ks, int* run_variant) {
+    const bool l16 = variant == 18 || variant == 19;
+    *run_variant = l16 && ks % 2 != 0 ? (variant == 18 ? 17 : 13) : variant;
+    return l16 && ks % 2 == 0;
+}
 int filter_default_variant() { return filter_variant(); }
 int filter_num_variants() { return kNumVariants; }
 
@@ -2409,6 +2613,11 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
     if (!P.stats_ready) RNNG_TRY(mfma_input_check(P, X, n, d));
     P.stats_ready = false;
     P.KS = KS;
+    {
+        // the 16-row layout when the kernels in use are the 16x16 forms
+        int run = 0;
+        P.l16 = P.i8 ? l16_i8_code(i8_group(), (d + 31) / 32, &run) : l16_variant(filter_variant(), KS, &run);
+    }
     RNNG_TRY(P.xa.reserve(std::max<size_t>(16, (size_t)n_rowpad * KS * 16 * 2)));
     RNNG_TRY(P.yb.reserve(std::max<size_t>(16, (size_t)n_colpad * KS * 16 * 2)));
     RNNG_TRY(P.xn.reserve(std::max<size_t>(16, (size_t)n_rowpad * sizeof(float))));
@@ -2424,17 +2633,18 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
                 k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,
                                                                P.colmin.as<unsigned>(), Tq, P.xa.as<i32x4_t>(),
                                                                P.yb.as<i32x4_t>(), P.xn.as<int>(), P.ycn.as<int>(),
-                                                               nullptr, nullptr);
+                                                               nullptr, nullptr, P.l16 ? 1 : 0);
         } else {
             if (n_rowpad > 0)
                 k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,
                                                                P.colmin.as<unsigned>(), Tq, P.xa.as<i32x4_t>(),
-                                                               nullptr, P.xn.as<int>(), nullptr, nullptr, nullptr);
+                                                               nullptr, P.xn.as<int>(), nullptr, nullptr, nullptr,
+                                                               P.l16 ? 1 : 0);
             if (n_colpad > 0)
                 k_prep_i8<<<n_colpad / 32, 256, 0, stream()>>>(X, n, n_colpad, d, P.KS8, colmap, vec4,
                                                                P.colmin.as<unsigned>(), Tq, nullptr,
                                                                P.yb.as<i32x4_t>(), nullptr, P.ycn.as<int>(), nullptr,
-                                                               nullptr);
+                                                               nullptr, P.l16 ? 1 : 0);
         }
         RNNG_HIP_TRY(hipGetLastError());
         return RNNG_OK;
@@ -2442,12 +2652,12 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
     if (rowmap == colmap && n_rowpad == n_colpad) {
         // one pass writes both forms
         RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(),
-                             P.yb.as<uint16_t>(), P.xn.as<float>(), P.yn.as<float>(), P.stat.as<unsigned>()));
+                             P.yb.as<uint16_t>(), P.xn.as<float>(), P.yn.as<float>(), P.stat.as<unsigned>(), P.l16));
     } else {
         RNNG_TRY(launch_prep(X, n, n_rowpad, d, KS, P.mu.as<float>(), rowmap, P.xa.as<uint16_t>(), nullptr,
-                             P.xn.as<float>(), nullptr, P.stat.as<unsigned>()));
+                             P.xn.as<float>(), nullptr, P.stat.as<unsigned>(), P.l16));
         RNNG_TRY(launch_prep(X, n, n_colpad, d, KS, P.mu.as<float>(), colmap, nullptr, P.yb.as<uint16_t>(),
-                             nullptr, P.yn.as<float>(), P.stat.as<unsigned>()));
+                             nullptr, P.yn.as<float>(), P.stat.as<unsigned>(), P.l16));
     }
     k_finalize_c0<<<1, 1, 0, stream()>>>(P.stat.as<unsigned>(), P.range.as<unsigned>(), d, KS, Tb,
                                          P.params.as<MfmaParams>());
@@ -2549,7 +2759,7 @@ int rnng_set_int8_filter(int32_t enable) {
 
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant) {
     if (int8_code >= kI8Codes || bf16_variant >= kNumVariants) {
-        set_error("rnng_set_filter_kernels: int8 code 0..4, BF16 variant 0..14 (negative: unchanged)");
+        set_error("rnng_set_filter_kernels: int8 code 0..10, BF16 variant 0..19 (negative: unchanged)");
         return RNNG_ERR_ARG;
     }
     std::lock_guard<std::mutex> lock(g_ws_mu);   // not in the middle of a build

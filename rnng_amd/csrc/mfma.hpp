@@ -8,6 +8,7 @@
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 typedef int i32x16_t __attribute__((ext_vector_type(16)));
 
@@ -31,6 +32,9 @@ struct MfmaPrep {
     bool i8 = false;
     int KS8 = 0;                                     // 32-wide int8 k-steps
     bool stats_ready = false;                        // mfma_input_check ran for the next prepare
+    // fragments in the 16-row layout of the 16x16 MFMA kernels (set before
+    // mfma_prepare_mapped; only kernels of that layout may then run)
+    bool l16 = false;
 };
 
 // Number of 16-wide k-steps for dimension d.
@@ -69,6 +73,7 @@ struct FilterArgs {
     int2* cand = nullptr;
     unsigned long long cap = 0;
     unsigned long long* counter = nullptr;
+    bool l16 = false;   // fragments in the 16-row layout (variants 18, 19)
 };
 int filter_dispatch(const FilterArgs& a, int KS, int variant);
 
@@ -86,13 +91,14 @@ struct I8Args {
     int2* cand = nullptr;
     unsigned long long cap = 0;
     unsigned long long* counter = nullptr;
+    bool l16 = false;   // fragments in the 16-row layout (codes 9, 10)
 };
 int filter_dispatch_i8(const I8Args& a, int KS8, int code);
 
 // The ring filter over tile lists (the cell path): work items (row block,
 // first, end, 0) index ctl, the column tiles of each row block in order.
-// int8 (i8): the four-wave shape with two stages of 3 tiles (code 3); BF16:
-// variant 17 or 13.  persistent: a grid of resident workgroups takes items
+// int8 (i8): the four-wave shape with two stages of 3 tiles (code 3, or its
+// 16x16 form code 10 with l16); BF16: variant 17 or 13 (18 or 19 with l16).  persistent: a grid of resident workgroups takes items
 // from *queue (cleared by the launch) in list order; otherwise one workgroup
 // per item.  xa/yb/rterm/cterm are the int8 or BF16 forms.
 struct TileFilterArgs {
@@ -113,9 +119,17 @@ struct TileFilterArgs {
     int ks = 0;        // k-steps (KS8 for int8)
     int variant = 17;  // BF16 variant (17 or 13)
     bool dry_run = false;   // size the persistent grid (occupancy query) without launching
+    bool l16 = false;  // fragments in the 16-row layout: the 16x16 MFMA forms of the same shapes
 };
 int filter_dispatch_tiles(const TileFilterArgs& a);
 int filter_i8_code();
+// The 16x16 MFMA kernels (16-row fragment layout): int8 codes 9 and 10 are
+// the 16x16 forms of codes 7 and 3, BF16 variants 18 and 19 those of 17 and
+// 13; they need an even number of k-steps.  l16_i8_code / l16_variant give
+// the kernel that runs for k-step count ks (the 32x32 form for odd ks), and
+// whether its fragments take the 16-row layout.
+bool l16_i8_code(int code, int ks8, int* run_code);
+bool l16_variant(int variant, int ks, int* run_variant);
 int filter_block_tiles(int variant);
 int filter_i8_row_tiles(int code);   // row tiles per work item of an int8 kernel code
 int filter_default_variant();

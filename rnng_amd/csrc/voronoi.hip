@@ -66,7 +66,8 @@ struct rnng_voronoi_s {
     rnng::MfmaPrep prep;
     // filter configuration chosen by the autotuner on the first build
     int tuned_variant = -1, tuned_xcd = 1;
-    int tuned_i8 = 0;   // int8 cell kernel code chosen by the tuner (0: the default, code 3)
+    int tuned_i8 = 0;   // int8 cell kernel code chosen by the tuner (0: the default, code 3 or 10)
+    bool tuned_l16 = false;   // the fragment layout the configuration was chosen on
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // schedule: row order (0 cells in landmark order, 1 cells along the
     // nearest-neighbour chain of the centers, 2 flat: the points in their own
@@ -987,15 +988,24 @@ int check_voronoi(const rnng_voronoi_s* v) {
 // Autotuned filter configurations: (variant, XCD ordering).  Variant 17 = the
 // direct-to-LDS ring with two stages of 8 column tiles shared by 12 waves (24
 // row tiles per work item), 13 and 14 = two stages of 2 or 3 tiles shared by 4
-// waves, 12 = three stages of 2 tiles (all 2 row tiles per wave).
+// waves, 12 = three stages of 2 tiles (all 2 row tiles per wave); on the
+// 16-row fragment layout their 16x16 forms 18 (of 17) and 19 (of 13).
 constexpr int kTuneVariants[4] = {17, 13, 14, 12};
+constexpr int kTuneVariants16[2] = {19, 18};
 constexpr int kPadRows = 1536;   // multiple of every row block (256, 512, 768 rows)
 
-// int8 kernel of the cell path: the eight-wave shapes (codes above 4) cover
+// int8 kernel of the cell path: the eight-wave shapes (codes 5-9) cover
 // 512-row blocks, whose column-tile unions over short cell runs cost more than
 // their longer groups save (1M x 128 integer points in 4,096 cells: 43.0 ms
-// against 34.7 ms for code 3), so those codes map to code 3 here.
-int cell_i8_code() { return filter_i8_code() > 4 ? 3 : filter_i8_code(); }
+// against 34.7 ms for code 3), so those codes map to the four-wave code 3, or
+// on the 16-row layout its 16x16 form, code 10.
+int cell_i8_code(bool l16) {
+    if (l16) return 10;
+    const int c = filter_i8_code();
+    return c > 4 ? 3 : c;
+}
+// the eight-wave int8 shape the tuner also times on runs
+int cell_i8_wide(bool l16) { return l16 ? 9 : 7; }
 
 int tuning_enabled() {
     static int on = -1;
@@ -1642,6 +1652,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             ta.counter = counters.as<unsigned long long>();
             ta.ks = P.i8 ? P.KS8 : KS;
             ta.variant = var;
+            ta.l16 = P.l16;
             return filter_dispatch_tiles(ta);
         }
         if (P.i8) {
@@ -1658,9 +1669,12 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             ia.cand = cand_ptr;
             ia.cap = cap_;
             ia.counter = counters.as<unsigned long long>();
-            return filter_dispatch_i8(ia, P.KS8, var > 0 ? var : cell_i8_code());
+            ia.l16 = P.l16;
+            return filter_dispatch_i8(ia, P.KS8, var > 0 ? var : cell_i8_code(P.l16));
         }
-        return filter_dispatch(filter_args(work, n_items, cand_ptr, cap_), KS, var);
+        FilterArgs fa = filter_args(work, n_items, cand_ptr, cap_);
+        fa.l16 = P.l16;
+        return filter_dispatch(fa, KS, var);
     };
     // predicted candidates of every row block (the persistent queue's order)
     auto rb_cands = [&](int bt) {
@@ -1676,7 +1690,7 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     };
     // the launch list of a schedule on the current layout, built on the device
     auto work_list = [&](int sched_, int var, int xcd, int stride, DevBuf& out, int64_t* n_out) -> int64_t {
-        const int bt = P.i8 ? filter_i8_row_tiles(var > 0 ? var : cell_i8_code()) : filter_block_tiles(var);
+        const int bt = P.i8 ? filter_i8_row_tiles(var > 0 ? var : cell_i8_code(P.l16)) : filter_block_tiles(var);
         int64_t cov = 0;
         const std::vector<float> rbc = sched_ == 2 ? rb_cands(bt) : std::vector<float>();
         const int rc =
@@ -1684,8 +1698,13 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         return rc != RNNG_OK ? -1 : cov;
     };
     // ---- autotuner: cell order x work form x filter shape, on every 8th item ----
+    // (again when the kernels in use changed the fragment layout)
+    if (v->tuned_variant >= 0 && v->tuned_l16 != P.l16) v->tuned_variant = -1;
     if (v->tuned_variant < 0) {
-        v->tuned_variant = filter_default_variant();
+        v->tuned_l16 = P.l16;
+        // the default variant in the layout the prep took (its 32x32 form for
+        // an odd k-step count)
+        (void)l16_variant(filter_default_variant(), KS, &v->tuned_variant);
         v->tuned_i8 = 0;
         v->tuned_xcd = 1;
         v->order = v->forced_order >= 0 ? v->forced_order : 0;
@@ -1699,11 +1718,18 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             for (int order = 0; order < 2; ++order) {
                 if (P.i8) {
                     // the four-wave kernel (256-row blocks) on every form, the
-                    // eight-wave one (code 7, 512-row blocks) on runs
+                    // eight-wave one (code 7 or 9, 512-row blocks) on runs
                     cfgs.push_back({order, 0, 0, 1});
-                    cfgs.push_back({order, 0, 7, 1});
+                    cfgs.push_back({order, 0, cell_i8_wide(P.l16), 1});
                     cfgs.push_back({order, 1, 0, 1});
                     cfgs.push_back({order, 2, 0, 0});
+                } else if (P.l16) {
+                    for (int var : kTuneVariants16)
+                        for (int xcd = 0; xcd < 2; ++xcd) cfgs.push_back({order, 0, var, xcd});
+                    for (int var : kTuneVariants16) {
+                        cfgs.push_back({order, 1, var, 1});
+                        cfgs.push_back({order, 2, var, 0});
+                    }
                 } else {
                     for (int var : kTuneVariants)
                         for (int xcd = 0; xcd < 2; ++xcd) cfgs.push_back({order, 0, var, xcd});
@@ -1716,9 +1742,10 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             // flat order (every pair, no cell padding): runs only
             if (P.i8) {
                 cfgs.push_back({2, 0, 0, 1});
-                cfgs.push_back({2, 0, 7, 1});
+                cfgs.push_back({2, 0, cell_i8_wide(P.l16), 1});
             } else {
-                for (int var : {17, 13}) cfgs.push_back({2, 0, var, 1});
+                for (int var : P.l16 ? std::vector<int>{19, 18} : std::vector<int>{17, 13})
+                    cfgs.push_back({2, 0, var, 1});
             }
             v->sched_cfg.clear();
             v->sched_ms.clear();
@@ -1750,7 +1777,11 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
                 }
             }
         }
-        if (!P.i8 && v->sched > 0 && v->tuned_variant != 17 && v->tuned_variant != 13) v->tuned_variant = 17;
+        // tile lists run the ring shapes 17 and 13 (18 and 19 on the 16-row layout)
+        if (!P.i8 && v->sched > 0) {
+            const int a = P.l16 ? 18 : 17, b = P.l16 ? 19 : 13;
+            if (v->tuned_variant != a && v->tuned_variant != b) v->tuned_variant = a;
+        }
         RNNG_TRY(apply_layout(v->order));
     }
     const int variant = P.i8 ? v->tuned_i8 : v->tuned_variant;
