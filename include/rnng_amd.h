@@ -242,8 +242,7 @@ int rnng_strings_destroy(rnng_strings_t s);
 
 /* All pairs i < j whose Levenshtein distance (unit costs, the reference's
  * EditDistance functor, drivers/systolic.cpp:271-299) is <= eps; the weight
- * is the distance as a float.  Any eps; above 32 every string must have at most
- * 512 atoms (RNNG_ERR_UNSUPPORTED otherwise). */
+ * is the distance as a float.  Any eps and any string lengths. */
 int rnng_edit_graph_build(rnng_strings_t s, float eps, rnng_graph_t *out, rnng_stats *stats);
 
 /* All pairs (i, j), i in a, j in b, whose Levenshtein distance is <= eps, the

@@ -32,7 +32,12 @@
 //      for DNA).  k = 32 (a 65-bit window) uses k_edit_verify, Ukkonen's
 //      banded dynamic programme with the same band; k > 32 uses
 //      k_edit_verify_myers, the same recurrence unbanded over all rows of the
-//      shorter string (up to 8 64-bit words: strings of at most 512 atoms).
+//      shorter string (up to 8 64-bit words, 512 atoms, one thread per
+//      candidate), and for longer strings k_edit_verify_long, the same
+//      recurrence over any length with one wave per candidate: lane w holds
+//      word w of a band of 64 words and works on column j - w at step j (the
+//      word-to-word carries pass one lane per step), bands of 4,096 rows in
+//      turn with the carries between bands kept per column in memory.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -660,7 +665,8 @@ __global__ __launch_bounds__(256) void k_edit_verify_myers(
         if (c < nc) {
             const int2 q = cand[c];
             const int m = R.slen[q.x], nn = C.slen[q.y];
-            if (m - nn <= k && nn - m <= k) {
+            // (a shorter string of more than 64 W atoms: k_edit_verify_long)
+            if (m - nn <= k && nn - m <= k && min(m, nn) <= 64 * W) {
                 // the shorter string is the pattern (the distance is symmetric); one call,
                 // so lanes of either order run the same code
                 const uint8_t* pr_ = R.codes + R.coff[q.x];
@@ -692,13 +698,154 @@ __global__ __launch_bounds__(256) void k_edit_verify_myers(
     }
 }
 
+// The multiword recurrence of myers_edit for a pattern of any length, as a
+// wavefront over one wave: in band b (pattern words 64 b .. 64 b + 63, rows
+// 4096 b + 1 ..), lane w holds word 64 b + w and at step s works on text
+// column s - w, taking the addition carry and the two horizontal deltas of
+// the word below from lane w - 1's previous step.  Lane 0 takes them from the
+// DP's top row (band 0: D[0][j] = j, a +1 horizontal delta) or from the band
+// below, whose last word left them per column in scr (three bit-planes of
+// (n + 31) / 32 words, written and read in column order with device-coherent
+// accesses and a fence between bands).  The score follows row m (on the lane
+// holding word (m - 1) / 64 of the last band): D[m][n], or a value > k as soon
+// as D[m][j] - (n - j) > k.  Every argument is wave-uniform; pattern bytes are
+// read at most 31 past the pattern's end, as in myers_edit.
+template <int NP>
+__device__ int myers_wave(const uint8_t* __restrict__ p, int m, const uint8_t* __restrict__ t, int n, int k,
+                          int lane, unsigned* __restrict__ scr) {
+    if (m == 0) return n;
+    const int W = (m + 63) >> 6;           // pattern words
+    const int nb = (W + 63) >> 6;          // bands of 64 words
+    const int wl = (m - 1) >> 6, bl = (m - 1) & 63;
+    const int nw32 = (n + 31) >> 5;
+    int score = m;                         // D[m][0]; kept by row m's lane
+    for (int b = 0; b < nb; ++b) {
+        const int w = 64 * b + lane;
+        const bool active = w < W;
+        const int last = min(63, W - 1 - 64 * b);   // last active lane of the band
+        const bool top = b == nb - 1;
+        const int wlane = wl - 64 * b;     // row m's lane (top band)
+        uint64_t pl[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) pl[q] = 0;
+        if (active) {
+            unsigned lo[NP], hi[NP];
+            plane_words<NP>(p + 64 * w, lo);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) hi[q] = 0u;
+            if (64 * w + 32 < m) plane_words<NP>(p + 64 * w + 32, hi);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) pl[q] = (uint64_t)lo[q] | ((uint64_t)hi[q] << 32);
+        }
+        uint64_t Pv = ~0ull, Mv = 0ull;
+        unsigned cin = 0u;                 // carries from lane - 1 (bit 0 addition, 1 +1 delta, 2 -1 delta)
+        unsigned acc0 = 0u, acc1 = 0u, acc2 = 0u;   // last lane: carries for the band above
+        unsigned bin0 = 0u, bin1 = 0u, bin2 = 0u;   // lane 0: carries from the band below
+        const int steps = n + last;
+        for (int s = 0; s < steps; ++s) {
+            const int col = s - lane;
+            const bool live = active && col >= 0 && col < n;
+            unsigned in = cin;
+            if (lane == 0) {
+                if (b == 0) {
+                    in = 2u;
+                } else if (col < n) {
+                    const int sh = col & 31;
+                    if (sh == 0) {
+                        const unsigned* src = scr + (col >> 5);
+                        bin0 = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        bin1 = __hip_atomic_load(src + nw32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        bin2 = __hip_atomic_load(src + 2 * nw32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    in = ((bin0 >> sh) & 1u) | (((bin1 >> sh) & 1u) << 1) | (((bin2 >> sh) & 1u) << 2);
+                }
+            }
+            unsigned out = 0u;
+            if (live) {
+                const unsigned tc = t[col];
+                uint64_t Eq = ~0ull;
+#pragma unroll
+                for (int q = 0; q < NP; ++q) Eq &= pl[q] ^ ((uint64_t)((tc >> q) & 1u) - 1ull);
+                const uint64_t Xv = Eq | Mv;
+                const uint64_t a = Eq & Pv;
+                const uint64_t s1 = a + Pv;
+                const uint64_t s2 = s1 + (uint64_t)(in & 1u);
+                const unsigned cyo = (unsigned)((s1 < a) | (s2 < s1));
+                const uint64_t Xh = (s2 ^ Pv) | Eq;
+                const uint64_t Ph = Mv | ~(Xh | Pv);
+                const uint64_t Mh = Pv & Xh;
+                if (top && lane == wlane) score += (int)((Ph >> bl) & 1ull) - (int)((Mh >> bl) & 1ull);
+                const uint64_t Phs = (Ph << 1) | (uint64_t)((in >> 1) & 1u);
+                const uint64_t Mhs = (Mh << 1) | (uint64_t)((in >> 2) & 1u);
+                Pv = Mhs | ~(Xv | Phs);
+                Mv = Phs & Xv;
+                out = cyo | ((unsigned)(Ph >> 63) << 1) | ((unsigned)(Mh >> 63) << 2);
+                if (!top && lane == 63) {
+                    const int sh = col & 31;
+                    acc0 |= (out & 1u) << sh;
+                    acc1 |= ((out >> 1) & 1u) << sh;
+                    acc2 |= ((out >> 2) & 1u) << sh;
+                    if (sh == 31 || col == n - 1) {
+                        unsigned* dst = scr + (col >> 5);
+                        __hip_atomic_store(dst, acc0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(dst + nw32, acc1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(dst + 2 * nw32, acc2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        acc0 = acc1 = acc2 = 0u;
+                    }
+                }
+            }
+            cin = __shfl_up(out, 1, 64);
+            if (top && (s & 31) == 31) {
+                // row m's lane has finished column s - wlane
+                const int sc = __shfl(score, wlane, 64);
+                const int cw = s - wlane;
+                if (cw >= 0 && cw < n && sc - (n - 1 - cw) > k) return k + 1;
+            }
+        }
+        if (!top) __threadfence();
+    }
+    return __shfl(score, wl - 64 * (nb - 1), 64);
+}
+
+// Candidates whose shorter string has more than short_max atoms (the others
+// are k_edit_verify_myers's), one wave each.  scr: 3 * scr_words words per
+// wave (only patterns of more than 4,096 atoms use it).
+template <int NP>
+__global__ __launch_bounds__(256) void k_edit_verify_long(
+    const EditSide R, const EditSide C, int self_join, const int2* __restrict__ cand, unsigned long long ccap,
+    int k, int short_max, int2* __restrict__ eij, float* __restrict__ ew, unsigned long long ecap,
+    unsigned long long* __restrict__ counters, unsigned* __restrict__ scratch, int64_t scr_words) {
+    const unsigned long long nc = min(counters[0], ccap);
+    const int lane = lane_id();
+    const unsigned wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const unsigned nwaves = gridDim.x * (blockDim.x >> 6);
+    unsigned* scr = scratch ? scratch + (size_t)wave * 3 * (size_t)scr_words : nullptr;
+    for (unsigned long long c = wave; c < nc; c += nwaves) {
+        const int2 q = cand[c];
+        const int m0 = R.slen[q.x], n0 = C.slen[q.y];
+        if (m0 - n0 > k || n0 - m0 > k || min(m0, n0) <= short_max) continue;
+        // the shorter string is the pattern
+        const bool sw = m0 > n0;
+        const uint8_t* pr_ = R.codes + R.coff[q.x];
+        const uint8_t* pc_ = C.codes + C.coff[q.y];
+        const int dist = myers_wave<NP>(sw ? pc_ : pr_, sw ? n0 : m0, sw ? pr_ : pc_, sw ? m0 : n0, k, lane, scr);
+        if (lane == 0 && dist <= k) {
+            const unsigned long long e = atomicAdd(&counters[1], 1ull);
+            if (e < ecap) {
+                eij[e] = edge_ids(R, C, q, self_join);
+                ew[e] = (float)dist;
+            }
+        }
+    }
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
 }
 
 struct EditWs {
-    DevBuf counters, cand, work;
+    DevBuf counters, cand, work, scratch;
     unsigned long long cand_cap = 0;
     std::vector<int4> work_host;
 };
@@ -752,19 +899,40 @@ void launch_myers_planes(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) 
     else launch_verify_myers<W, 8>(J, ws, k, g);
 }
 
-constexpr int kMyersMaxLen = 512;   // longest string of a radius above 32 (8 words)
+constexpr int kMyersMaxLen = 512;   // longest pattern of the thread-per-candidate kernel (8 words)
+constexpr int kLongWaves = 1024;     // waves of k_edit_verify_long
+
+template <int NP>
+int launch_verify_long(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
+    // carries between bands of 4,096 rows: three bit-planes of a text's
+    // columns per wave, for patterns longer than one band
+    int64_t words = 0;
+    unsigned* scr = nullptr;
+    if (J.maxlen > 4096) {
+        words = ((int64_t)J.maxlen + 31) / 32;
+        RNNG_TRY(ws.scratch.reserve((size_t)kLongWaves * 3 * (size_t)words * sizeof(unsigned)));
+        scr = ws.scratch.as<unsigned>();
+    }
+    k_edit_verify_long<NP><<<kLongWaves / 4, 256, 0, stream()>>>(
+        J.R, J.C, J.self_join, ws.cand.as<int2>(), ws.cand_cap, k, kMyersMaxLen, g->ij.as<int2>(), g->w.as<float>(),
+        (unsigned long long)g->cap, ws.counters.as<unsigned long long>(), scr, words);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
 
 int dispatch_verify(const EditJoin& J, EditWs& ws, int k, rnng_graph_s* g) {
     if (k > 32) {
         if (J.maxlen <= 64) launch_myers_planes<1>(J, ws, k, g);
         else if (J.maxlen <= 128) launch_myers_planes<2>(J, ws, k, g);
         else if (J.maxlen <= 256) launch_myers_planes<4>(J, ws, k, g);
-        else if (J.maxlen <= kMyersMaxLen) launch_myers_planes<8>(J, ws, k, g);
-        else {
-            set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
-            return RNNG_ERR_UNSUPPORTED;
-        }
+        else launch_myers_planes<8>(J, ws, k, g);
         RNNG_HIP_TRY(hipGetLastError());
+        if (J.maxlen > kMyersMaxLen) {   // pairs whose shorter string is longer than 512 atoms
+            if (J.planes <= 2) return launch_verify_long<2>(J, ws, k, g);
+            if (J.planes <= 3) return launch_verify_long<3>(J, ws, k, g);
+            if (J.planes <= 4) return launch_verify_long<4>(J, ws, k, g);
+            return launch_verify_long<8>(J, ws, k, g);
+        }
         return RNNG_OK;
     }
     if (k <= 15) {
@@ -817,10 +985,6 @@ int edit_join(const rnng_strings_s* A, const rnng_strings_s* B, bool self, float
     J.self_join = self ? 1 : 0;
     J.maxlen = std::max(A->len_sorted.empty() ? 0 : A->len_sorted.back(),
                         B->len_sorted.empty() ? 0 : B->len_sorted.back());
-    if (k > 32 && J.maxlen > kMyersMaxLen) {
-        set_error("edit-distance radius above 32 needs strings of at most 512 atoms");
-        return RNNG_ERR_UNSUPPORTED;
-    }
     // codes of both sides in one alphabet: each set numbers the bytes it
     // holds, so two sets with different byte sets are renumbered by the union
     LocalBuf rc_a, rc_b, lut;

@@ -63,12 +63,48 @@ def test_edge_cases(orc):
     ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, 40)
     ij, d = edit.eps_pairs(atoms, offs, 40)
     assert np.array_equal(ij, ij_ref) and np.array_equal(d, d_ref)
-    # ... for strings of at most 512 atoms
-    from rnng_amd import RnngError
-    a2, o2 = edit.as_csr([b"A" * 513, b"A" * 500])
-    with pytest.raises(RnngError):
-        edit.eps_pairs(a2, o2, 40)
+    # ... and strings longer than 512 atoms with the one-wave-per-pair kernel
+    a2, o2 = edit.as_csr([b"A" * 513, b"A" * 500, b"A" * 540 + b"C"])
+    ij, d = edit.eps_pairs(a2, o2, 40)
+    assert [tuple(p) for p in ij] == [(0, 1), (0, 2)] and list(d) == [13, 28]   # (1, 2) is at 41
     assert edit.eps_pairs(a2, o2, 20)[0].shape[0] == 1
+
+
+@pytest.mark.parametrize("lens,eps", [((480, 900), 40), ((480, 900), float("inf")), ((3900, 4400), 60),
+                                      ((4000, 9000), 120), ((8100, 8300), float("inf"))])
+def test_long_strings_radius_above_32(orc, lens, eps):
+    """Strings longer than 512 atoms at radii above 32: pairs whose shorter
+    string has more than 512 atoms run one wave per pair (bands of 4,096
+    pattern rows, the carries between bands kept per column), the others the
+    thread-per-pair kernel; pairs and distances equal the oracle's full DP.
+    An infinite radius gives every pair with its exact distance."""
+    lo, hi = lens
+    rng = np.random.default_rng(lo + hi)
+    alphabet = list(b"ACGT")
+    seeds = [rng.choice(alphabet, int(rng.integers(lo, hi + 1))).astype(np.uint8).tobytes() for _ in range(3)]
+    strs = []
+    for s in seeds:
+        strs.append(s)
+        for _ in range(5):
+            t = bytearray(s)
+            for _ in range(int(rng.integers(0, max(2, int(eps) if eps != float("inf") else 200)))):
+                r = rng.random()
+                if t and r < 0.4:
+                    t[int(rng.integers(0, len(t)))] = int(rng.choice(alphabet))
+                elif t and r < 0.7:
+                    del t[int(rng.integers(0, len(t)))]
+                else:
+                    t.insert(int(rng.integers(0, len(t) + 1)), int(rng.choice(alphabet)))
+            strs.append(bytes(t))
+    atoms, offs = _edit().as_csr(strs)
+    flush_print(f"long strings {lo}-{hi} eps {eps}: {len(strs)} strings")
+    ij_ref, d_ref = orc.edit_eps_pairs(atoms, offs, eps)
+    ij, d = _edit().eps_pairs(atoms, offs, eps)
+    assert np.array_equal(ij, ij_ref)
+    assert np.array_equal(d, d_ref)
+    assert ij.shape[0] > 0
+    if eps == float("inf"):
+        assert ij.shape[0] == len(strs) * (len(strs) - 1) // 2
 
 
 @pytest.mark.parametrize("alphabet,maxlen", [(b"ACGT", 150), (b"ACGT", 300), (b"abcdefghijklmnopqrstu", 64),
