@@ -146,3 +146,91 @@ def test_multiword_bit_parallel_equals_full_dp():
         else:
             assert b > k, (s, t, k, d, b)
     assert within > 150
+
+
+def wavefront_bit_parallel(p: bytes, t: bytes, k: int, lanes: int, past_end: bytes) -> int:
+    """The recurrence of k_edit_verify_long (shorter string above 512 atoms),
+    with `lanes` lanes per wave (64 on the GPU): bands of `lanes` pattern
+    words; at step s lane w of a band works on text column s - w with the
+    carries lane w - 1 produced at step s - 1; lane 0 takes them from the top
+    row (band 0) or from the carries the band below's last lane left per
+    column; row m's lane keeps the score and the wave stops after every 32nd
+    step once D[m][j] - (n - j) > k."""
+    m, n = len(p), len(t)
+    if m == 0:
+        return n
+    W = (m + 63) >> 6
+    nb = (W + lanes - 1) // lanes
+    pp = p + past_end[: 64 * W - m]
+    wl, bl = (m - 1) >> 6, (m - 1) & 63
+    score = m
+    below = [0] * n                         # carries of the band below, per column
+    for b in range(nb):
+        last = min(lanes - 1, W - 1 - lanes * b)
+        top = b == nb - 1
+        wlane = wl - lanes * b
+        eq = [None] * lanes
+        for lane in range(last + 1):
+            w = lanes * b + lane
+            eq[lane] = {c: sum(1 << r for r in range(64) if pp[64 * w + r] == c) for c in set(t)}
+        Pv, Mv = [M64] * lanes, [0] * lanes
+        cin = [0] * lanes
+        above = [0] * n
+        for s in range(n + last):
+            outs = [0] * lanes
+            for lane in range(last + 1):
+                col = s - lane
+                if not 0 <= col < n:
+                    continue
+                inn = cin[lane] if lane else (2 if b == 0 else below[col])
+                Eq = eq[lane][t[col]]
+                Xv = Eq | Mv[lane]
+                a = Eq & Pv[lane]
+                s1 = (a + Pv[lane]) & M64
+                s2 = (s1 + (inn & 1)) & M64
+                cyo = int(s1 < a or s2 < s1)
+                Xh = (s2 ^ Pv[lane]) | Eq
+                Ph = (Mv[lane] | ~(Xh | Pv[lane])) & M64
+                Mh = Pv[lane] & Xh
+                if top and lane == wlane:
+                    score += ((Ph >> bl) & 1) - ((Mh >> bl) & 1)
+                Phs = ((Ph << 1) | ((inn >> 1) & 1)) & M64
+                Mhs = ((Mh << 1) | ((inn >> 2) & 1)) & M64
+                Pv[lane] = (Mhs | ~(Xv | Phs)) & M64
+                Mv[lane] = Phs & Xv
+                outs[lane] = cyo | ((Ph >> 63) << 1) | ((Mh >> 63) << 2)
+                if not top and lane == lanes - 1:
+                    above[col] = outs[lane]
+            cin = [0] + outs[:-1]
+            if top and s % 32 == 31:
+                cw = s - wlane
+                if 0 <= cw < n and score - (n - 1 - cw) > k:
+                    return k + 1
+        below = above
+    return score
+
+
+def test_wavefront_bit_parallel_equals_full_dp():
+    """Equal to the full dynamic programme whenever the distance is <= k,
+    above k otherwise: one band and several (4-lane waves: bands of 256
+    rows, patterns up to 900 atoms), and two 64-lane bands (4,200 atoms),
+    with garbage codes past the pattern's end."""
+    rng = random.Random(11)
+    within = 0
+    cases = [(4, rng.randint(60, 900)) for _ in range(40)] + [(64, 4200)]
+    for lanes, m in cases:
+        alpha = rng.choice([b"ACGT", b"abcdefghijklmnopqrstu"]) if lanes == 4 else b"ACGT"
+        s = bytes(rng.choice(alpha) for _ in range(m))
+        t = _mutate(rng, s, alpha, rng.randint(0, 80))
+        if len(t) < len(s):
+            s, t = t, s
+        d = oracle.edit_distance(s, t)
+        past_end = bytes(rng.choice(alpha) for _ in range(64))
+        for k in (max(0, d - 1), d, 10 ** 9):
+            b = wavefront_bit_parallel(s, t, k, lanes, past_end)
+            if d <= k:
+                assert b == d, (len(s), len(t), k, d, b)
+                within += 1
+            else:
+                assert b > k, (len(s), len(t), k, d, b)
+    assert within >= 80
