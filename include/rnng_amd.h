@@ -469,8 +469,10 @@ int rnng_set_int8_filter(int32_t enable);
  * tiles, 3/4 two stages of 3/4 tiles, 5/6/7 two stages of 4/6/8 tiles shared
  * by eight waves of two row tiles, 8 two stages of 8 tiles shared by four waves
  * of four row tiles (two waves per SIMD), 9/10 the 16x16 MFMA forms
- * (v_mfma_i32_16x16x64_i8, 16-row fragment layout) of 7/3 (all-pairs builds;
- * the cell path runs code 3 for codes 5-8, and code 10 for 9 and 10).
+ * (v_mfma_i32_16x16x64_i8, 16-row fragment layout) of 7/3, 11/12 16x16 forms
+ * with four row tiles per wave (four waves, two per SIMD) / three (twelve
+ * waves, three per SIMD) (all-pairs builds; the cell path runs code 3 for
+ * codes 5-8, and code 10 for 9-12).
  * bf16_variant 0..19 (11/12 three-stage ring, 13/14 two-stage ring of 2/3
  * tiles, 15 two stages of 4 tiles shared by 8 waves, 16/17 two stages of 4/8
  * tiles shared by 12 waves, 18/19 the 16x16 forms (v_mfma_f32_16x16x32_bf16)

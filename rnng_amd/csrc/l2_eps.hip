@@ -1165,64 +1165,72 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[M], typena
 // q of the block, column l & 15).  A 32-column tile runs as two 16-column
 // halves with their own column terms; each half's test is made while the
 // next half's MFMAs run.
-template <class T, int KS>
+template <class T, int KS, int M>
 __device__ __forceinline__ void mf16_load_rows(const typename T::frag* __restrict__ xa,
                                                const typename T::term* __restrict__ rterm, int rt0, int lane,
-                                               typename T::frag (&a)[4][KS / 2], typename T::accv16 (&xr)[4]) {
+                                               typename T::frag (&a)[2 * M][KS / 2],
+                                               typename T::accv16 (&xr)[2 * M]) {
     constexpr int KH = KS / 2;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 2 * M; ++r)
 #pragma unroll
         for (int s = 0; s < KH; ++s) a[r][s] = xa[((size_t)(rt0 + (r >> 1)) * KS + (r & 1) * KH + s) * 64 + lane];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 2 * M; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) xr[r][q] = rterm[rt0 * 32 + 16 * r + 4 * (lane >> 4) + q];
 }
 
 // one 16-column half held in LDS (b: its first k-step)
-template <class T, int KS>
-__device__ __forceinline__ void mf16_half(const typename T::frag* b, const typename T::frag (&a)[4][KS / 2],
-                                          const typename T::accv16 (&xr)[4], int lane,
-                                          typename T::accv16 (&acc)[4]) {
+template <class T, int KS, int M>
+__device__ __forceinline__ void mf16_half(const typename T::frag* b, const typename T::frag (&a)[2 * M][KS / 2],
+                                          const typename T::accv16 (&xr)[2 * M], int lane,
+                                          typename T::accv16 (&acc)[2 * M]) {
 #pragma unroll
     for (int s = 0; s < KS / 2; ++s) {
         const typename T::frag bb = b[s * 64 + lane];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = T::mfma16(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
+        for (int r = 0; r < 2 * M; ++r) acc[r] = T::mfma16(a[r][s], bb, s == 0 ? xr[r] : acc[r]);
     }
 }
 
-// minimum of a lane's 16 values (four chains of four)
-template <class T>
-__device__ __forceinline__ typename T::term mf16_min(const typename T::accv16 (&acc)[4]) {
+// minimum of a lane's 8 M values (four chains)
+template <class T, int M>
+__device__ __forceinline__ typename T::term mf16_min(const typename T::accv16 (&acc)[2 * M]) {
+    constexpr int NV = 8 * M, L = NV / 4;   // values, per chain
     typename T::term c[4];
 #pragma unroll
-    for (int h = 0; h < 4; ++h) c[h] = T::vmin(T::vmin(acc[h][0], acc[h][1]), T::vmin(acc[h][2], acc[h][3]));
+    for (int h = 0; h < 4; ++h) {
+        c[h] = acc[(h * L) >> 2][(h * L) & 3];
+#pragma unroll
+        for (int q = 1; q < L; ++q) c[h] = T::vmin(c[h], acc[(h * L + q) >> 2][(h * L + q) & 3]);
+    }
     return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
 }
 
 // Rare path of a half: value i = 4 r + q of a lane is row rowb + 16 (i >> 2) +
 // (i & 3) (rising with i) of column col; the rows at or past the lane's limit
 // are dropped as a bit count, as in mf_emit.
-template <class T>
-__device__ __forceinline__ void mf16_emit(const typename T::accv16 (&acc)[4], typename T::term nyc, int col, int rt0,
-                                          int lane, int n_rows, int n_cols, int self_join, int2* stage,
+template <class T, int M>
+__device__ __forceinline__ void mf16_emit(const typename T::accv16 (&acc)[2 * M], typename T::term nyc, int col,
+                                          int rt0, int lane, int n_rows, int n_cols, int self_join, int2* stage,
                                           unsigned stage_cap, unsigned* scount, int2* __restrict__ cand,
                                           unsigned long long cap, unsigned long long* __restrict__ counter) {
+    static_assert(M >= 2 && M <= 4, "16 to 32 values per lane");
+    constexpr int NV = 8 * M;
     const int rowb = rt0 * 32 + 4 * (lane >> 4);
     int opaque;
     asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
     unsigned mask = 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 2 * M; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) mask |= T::below(acc[r][q], nyc) << (r * 4 + q + opaque);
     const int lim = col < n_cols ? (self_join ? min(n_rows, col) : n_rows) : 0;
     // values with row offset 16*(i >> 2) + (i & 3) < L are the first k
     const int L = lim - rowb;
-    const int k = L <= 0 ? 0 : (L >= 64 ? 16 : 4 * (L >> 4) + min(4, L & 15));
-    mask &= (1u << k) - 1u;
+    const int k = L <= 0 ? 0 : (L >= 4 * NV ? NV : 4 * (L >> 4) + min(4, L & 15));
+    mask &= k >= 32 ? ~0u : (1u << k) - 1u;
     mf_walk<unsigned, 16>(mask, rowb, col, lane, stage, stage_cap, scount, cand, cap, counter);
 }
 
@@ -1426,7 +1434,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     constexpr int NJ = G * KS;
     constexpr int NC = 64 * ((G + 1) / 2);   // start terms per stage (two tiles per load)
     static_assert(G >= 1 && G <= 16 && (S == 2 || S == 3), "ring shape");
-    static_assert(!L16 || (M == 2 && KS % 2 == 0), "16x16 forms: two row tiles per wave, even k-steps");
+    static_assert(!L16 || (M >= 2 && M <= 4 && KS % 2 == 0), "16x16 forms: 2-4 row tiles per wave, even k-steps");
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
     // candidates staged in LDS; the int8 shape with four tiles per stage keeps
@@ -1467,16 +1475,16 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     if constexpr (TL) ti = load_tile_idx(ctl, wi.y, wi.z, lane);
     frag a[L16 ? 1 : M][KS];
     accv xr[L16 ? 1 : M];
-    frag a16[L16 ? 4 : 1][L16 ? KS / 2 : 1];
-    accv16 xr16[L16 ? 4 : 1];
+    frag a16[L16 ? 2 * M : 1][L16 ? KS / 2 : 1];
+    accv16 xr16[L16 ? 2 * M : 1];
     // L16: the half whose test is pending (initially none: no value is below kLow)
-    accv16 accp[L16 ? 4 : 1];
+    accv16 accp[L16 ? 2 * M : 1];
     term nycp = T::kLow;
     int colp = 0;
     if constexpr (L16) {
-        mf16_load_rows<T, KS>(xa, rterm, rt0, lane, a16, xr16);
+        mf16_load_rows<T, KS, M>(xa, rterm, rt0, lane, a16, xr16);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) accp[r] = accv16{T::kHigh, T::kHigh, T::kHigh, T::kHigh};
+        for (int r = 0; r < 2 * M; ++r) accp[r] = accv16{T::kHigh, T::kHigh, T::kHigh, T::kHigh};
     } else {
         mf_load_rows<T, KS, M>(xa, rterm, rt0, lane, a, xr);
     }
@@ -1503,14 +1511,14 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const term nyc = -ccur[cc * 32 + 16 * h + (lane & 15)];
-                    accv16 acc[4];
-                    mf16_half<T, KS>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
+                    accv16 acc[2 * M];
+                    mf16_half<T, KS, M>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
                     // the previous half's test, under this half's MFMAs
-                    if (__any(mf16_min<T>(accp) < nycp))
-                        mf16_emit<T>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount,
-                                     cand, cap, counter);
+                    if (__any(mf16_min<T, M>(accp) < nycp))
+                        mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount,
+                                        cand, cap, counter);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) accp[r] = acc[r];
+                    for (int r = 0; r < 2 * M; ++r) accp[r] = acc[r];
                     nycp = nyc;
                     colp = ct * 32 + 16 * h + (lane & 15);
                 }
@@ -1546,9 +1554,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
         }
     }
     if constexpr (L16) {   // the last half's test
-        if (__any(mf16_min<T>(accp) < nycp))
-            mf16_emit<T>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
-                         counter);
+        if (__any(mf16_min<T, M>(accp) < nycp))
+            mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
+                            counter);
     }
     __syncthreads();   // every wave's staged candidates and count
     const unsigned nst = min(scount, STG);
@@ -1915,7 +1923,8 @@ float ord2f_host(unsigned u) {
 // 7 = two stages of 4, 6 or 8 tiles shared by eight waves (512 threads,
 // 16 row tiles per work item; 6 tiles at KS8 = 5 for two workgroups per CU);
 // 8 = four waves of four row tiles; 9, 10 = the 16x16 forms of 7 and 3 (the
-// default: code 9).
+// default: code 9); 11, 12 = 16x16 forms with four row tiles per wave (four
+// waves, two per SIMD) and three (twelve waves, three per SIMD).
 template <int KS8, int CODE>
 void launch_filter_i8(const I8Args& a) {
     constexpr int MINW = KS8 <= 4 ? 4 : 1;
@@ -1948,18 +1957,27 @@ void launch_filter_i8(const I8Args& a) {
             k_l2_mfma_filter_ring<MfI8, KS8, G, MINW, 2, 8, false, false, MRT, true><<<a.n_items, 512, 0, stream()>>>(
                 a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
                 nullptr, a.n_items, nullptr);
-        } else {                     // 16x16 form of code 3
+        } else if constexpr (CODE == 10) {   // 16x16 form of code 3
             k_l2_mfma_filter_ring<MfI8, KS8, 3, MINW, 2, 4, false, false, MRT, true><<<a.n_items, 256, 0, stream()>>>(
+                a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+                nullptr, a.n_items, nullptr);
+        } else if constexpr (CODE == 11) {   // 16x16, four row tiles per wave, four waves (two per SIMD)
+            k_l2_mfma_filter_ring<MfI8, KS8, 8, 2, 2, 4, false, false, 4, true><<<a.n_items, 256, 0, stream()>>>(
+                a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
+                nullptr, a.n_items, nullptr);
+        } else {                             // 16x16, three row tiles per wave, twelve waves (three per SIMD)
+            k_l2_mfma_filter_ring<MfI8, KS8, 8, 3, 2, 12, false, false, 3, true><<<a.n_items, 768, 0, stream()>>>(
                 a.xa, a.yb, a.rterm, a.cterm, a.work, a.n_rows, a.n_cols, a.self_join, a.cand, a.cap, a.counter,
                 nullptr, a.n_items, nullptr);
         }
     }
 }
 
-constexpr int kI8Codes = 11;
+constexpr int kI8Codes = 13;
 
-// row tiles per work item of an int8 kernel code (codes 5-9: 16 row tiles)
-int i8_row_tiles(int code) { return code >= 5 && code <= 9 ? 16 : 8; }
+// row tiles per work item of an int8 kernel code (codes 5-9 and 11: 16 row
+// tiles, 12: 36)
+int i8_row_tiles(int code) { return code == 12 ? 36 : (code >= 5 && code <= 9) || code == 11 ? 16 : 8; }
 
 int g_i8_group = -1;   // int8 filter kernel code (launch_filter_i8); -1: not read yet
 
@@ -1995,15 +2013,19 @@ I8Args i8_args(const Workspace& ws, int nx, int ny, int self_join) {
 
 int filter_dispatch_i8(const I8Args& a, int KS8, int code) {
     if (a.n_items == 0) return RNNG_OK;
-    if (a.l16 != (code == 9 || code == 10) || (a.l16 && KS8 % 2 != 0)) {
+    if (a.l16 != (code >= 9 && code <= 12) || (a.l16 && KS8 % 2 != 0)) {
         set_error("int8 filter: the kernel does not match the fragment layout");
         return RNNG_ERR_ARG;
     }
     switch (KS8 * 16 + code) {
         case 2 * 16 + 9: launch_filter_i8<2, 9>(a); break;
         case 2 * 16 + 10: launch_filter_i8<2, 10>(a); break;
+        case 2 * 16 + 11: launch_filter_i8<2, 11>(a); break;
+        case 2 * 16 + 12: launch_filter_i8<2, 12>(a); break;
         case 4 * 16 + 9: launch_filter_i8<4, 9>(a); break;
         case 4 * 16 + 10: launch_filter_i8<4, 10>(a); break;
+        case 4 * 16 + 11: launch_filter_i8<4, 11>(a); break;
+        case 4 * 16 + 12: launch_filter_i8<4, 12>(a); break;
         case 1 * 16 + 0: launch_filter_i8<1, 0>(a); break;
         case 1 * 16 + 1: launch_filter_i8<1, 1>(a); break;
         case 1 * 16 + 2: launch_filter_i8<1, 2>(a); break;
@@ -2512,8 +2534,8 @@ int filter_dispatch(const FilterArgs& a, int KS, int variant) {
 int filter_block_tiles(int variant) { return kVariants[variant].nw * kVariants[variant].rt; }
 
 bool l16_i8_code(int code, int ks8, int* run_code) {
-    const bool l16 = code == 9 || code == 10;
-    *run_code = l16 && ks8 % 2 != 0 ? (code == 9 ? 7 : 3) : code;
+    const bool l16 = code >= 9 && code <= 12;
+    *run_code = l16 && ks8 % 2 != 0 ? (code == 10 ? 3 : 7) : code;
     return l16 && ks8 % 2 == 0;
 }
 
@@ -2759,7 +2781,7 @@ int rnng_set_int8_filter(int32_t enable) {
 
 int rnng_set_filter_kernels(int32_t int8_code, int32_t bf16_variant) {
     if (int8_code >= kI8Codes || bf16_variant >= kNumVariants) {
-        set_error("rnng_set_filter_kernels: int8 code 0..10, BF16 variant 0..19 (negative: unchanged)");
+        set_error("rnng_set_filter_kernels: int8 code 0..12, BF16 variant 0..19 (negative: unchanged)");
         return RNNG_ERR_ARG;
     }
     std::lock_guard<std::mutex> lock(g_ws_mu);   // not in the middle of a build

@@ -134,7 +134,7 @@ int filter_block_tiles(int variant);
 int filter_i8_row_tiles(int code);   // row tiles per work item of an int8 kernel code
 int filter_default_variant();
 int filter_num_variants();
-constexpr int kFilterChunk = 256;   // column tiles per chunk of the work list
+constexpr int kFilterChunk = 192;   // column tiles per chunk of the work list
 constexpr int kMaxListItem = 256;   // column tiles per tile-list work item (four registers per lane)
 constexpr int kMaxCellWords = 16384;   // cell bitset words the device work-list builder holds in LDS (x2)
 

@@ -246,11 +246,11 @@ def test_int8_filter_bit_exact(orc, d, lo, hi):
 
 
 def test_every_filter_kernel_shape(orc):
-    """Every selectable filter kernel (int8 codes 0-10, BF16 variants 0-19, set
+    """Every selectable filter kernel (int8 codes 0-12, BF16 variants 0-19, set
     through rnng_set_filter_kernels) gives the oracle's edges bit for bit, on
     integer data (the int8 path) and float data (the BF16 path), for self joins
     and a bipartite join, at d = 17 (scalar loads, partial k-steps), 64 and 128.
-    The 16x16 kernels (int8 codes 9, 10; BF16 variants 18, 19) run on the
+    The 16x16 kernels (int8 codes 9-12; BF16 variants 18, 19) run on the
     16-row fragment layout where the k-step count is even (int8 at d = 64 and
     128, BF16 at every d here) and as their 32x32 forms otherwise."""
     from rnng_amd import Stats, _lib
@@ -282,7 +282,7 @@ def test_every_filter_kernel_shape(orc):
             _check_same(ijb, wb, ijb_ref, wb_ref)
 
     try:
-        for code in range(11):
+        for code in range(13):
             flush_print(f"int8 filter kernel code {code}")
             _lib.check(lib.rnng_set_filter_kernels(code, -1))
             check(ints, 1)
@@ -290,7 +290,7 @@ def test_every_filter_kernel_shape(orc):
             flush_print(f"BF16 filter variant {var}")
             _lib.check(lib.rnng_set_filter_kernels(-1, var))
             check(floats, 0)
-        assert lib.rnng_set_filter_kernels(11, -1) != 0 and lib.rnng_set_filter_kernels(-1, 20) != 0
+        assert lib.rnng_set_filter_kernels(13, -1) != 0 and lib.rnng_set_filter_kernels(-1, 20) != 0
     finally:
         _lib.check(lib.rnng_set_filter_kernels(code0.value, var0.value))
 
