@@ -2240,7 +2240,7 @@ int debug_cell_work(const int32_t* cnt, int32_t m, const uint32_t* adj, const in
         RNNG_TRY(rc);
     } else {
         if (sched == 0) cov = cell_work(L.rs, L.adj, words, m, bt, nrb, it);
-        else cov = cell_work_tiles(L.rs, L.adj, words, m, bt, nrb, kFilterChunk, it, cl);
+        else cov = cell_work_tiles(L.rs, L.adj, words, m, bt, nrb, kMaxListItem, it, cl);
         if (sched == 2) order_by_cost(it, rbc);
     }
     *n_items = (int64_t)it.size();
