@@ -418,12 +418,17 @@ int rnng_voronoi_destroy(rnng_voronoi_t v);
 /* y[batch][2] = the network on x[batch][4] (the inference kernel). */
 int rnng_mlp_infer(const float *params, const float *x, int64_t batch, float *y);
 
-/* Loss = mean over batch and outputs of w (y - target)^2, w = 4 where
- * y < target (an estimate below the count costs a second filter pass, one
- * above it only memory), and its gradient with respect to the parameters
- * (forward + backward kernels). */
+/* Loss = mean over batch and outputs of w (y - target)^2, w = the
+ * under-weight where y < target (an estimate below the count costs a second
+ * filter pass, one above it only memory) and 1 elsewhere, and its gradient
+ * with respect to the parameters (forward + backward kernels). */
 int rnng_mlp_loss_grad(const float *params, const float *x, const float *target, int64_t batch,
                        float *loss, float *grad);
+
+/* The under-weight of the loss (process-wide, for rnng_mlp_loss_grad and
+ * rnng_mlp_train; default 4; 0 < w < 1e6). */
+int rnng_mlp_set_under_weight(float w);
+float rnng_mlp_under_weight(void);
 
 /* `iters` full-batch Adam steps (lr, betas 0.9/0.999) on the GPU, updating
  * params in place; final_loss (may be NULL) = loss of the result. */

@@ -308,9 +308,9 @@ def mlp_forward(p, x):
 MLP_UNDER = 4.0   # loss weight of an output below its target
 
 
-def mlp_loss_grad(p, x, t):
-    """(loss, grad) of mean(w (y - t)^2) over batch and outputs, w = 4 where
-    y < t, else 1; ReLU'(0) = 0."""
+def mlp_loss_grad(p, x, t, under=MLP_UNDER):
+    """(loss, grad) of mean(w (y - t)^2) over batch and outputs, w = under
+    (default 4) where y < t, else 1; ReLU'(0) = 0."""
     W1, b1, W2, b2, W3, b3 = mlp_unpack(p)
     x = np.asarray(x, dtype=np.float64)
     t = np.asarray(t, dtype=np.float64)
@@ -321,7 +321,7 @@ def mlp_loss_grad(p, x, t):
     h2 = np.maximum(z2, 0.0)
     y = h2 @ W3.T + b3
     r = y - t
-    wr = np.where(r < 0, MLP_UNDER, 1.0)
+    wr = np.where(r < 0, under, 1.0)
     loss = float(np.mean(wr * r * r))
     dy = 2.0 * wr * r / (B * MLP_O)
     gW3 = dy.T @ h2

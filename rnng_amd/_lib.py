@@ -131,6 +131,8 @@ _SIGS = {
     "rnng_mlp_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_int64, _vp, _vp]),
     "rnng_mlp_train": (C.c_int, [_vp, _vp, _vp, C.c_int64, C.c_int32, C.c_float, _vp]),
     "rnng_set_cost_model": (C.c_int, [_vp]),
+    "rnng_mlp_set_under_weight": (C.c_int, [C.c_float]),
+    "rnng_mlp_under_weight": (C.c_float, []),
     "rnng_voronoi_cell_stats": (C.c_int, [_vp, _vp, _vp]),
     "rnng_sq_threshold": (C.c_float, [C.c_float]),
     "rnng_debug_mfma_layout": (C.c_int, [_vp, _vp]),

@@ -71,6 +71,16 @@ def loss_grad(params, x, t):
     return float(loss.value), g
 
 
+def set_under_weight(w: float) -> None:
+    """Loss weight of an estimate below its target (default 4), for
+    loss_grad and train."""
+    check(_lib.lib().rnng_mlp_set_under_weight(C.c_float(w)))
+
+
+def under_weight() -> float:
+    return float(_lib.lib().rnng_mlp_under_weight())
+
+
 def train(params, x, t, iters: int = 3000, lr: float = 3e-3):
     """Full-batch Adam on the GPU.  Returns (trained params, final loss)."""
     p = np.array(params, dtype=np.float32, copy=True)
@@ -200,6 +210,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--big", type=float, default=0.25, help="share of 100k-300k point runs")
+    ap.add_argument("--under", type=float, default=4.0, help="loss weight of an estimate below its count")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -211,6 +222,7 @@ def main(argv=None):
         Xh, Th = collect(a.huge_runs, a.seed + 4, log, huge=1.0)
         X, T = np.concatenate([X, Xh]), np.concatenate([T, Th])
     log(f"collected {X.shape[0]} cells from {a.runs + a.huge_runs} runs in {time.perf_counter() - t0:.1f}s")
+    set_under_weight(a.under)
     p, loss = train(init_params(a.seed), X, T, iters=a.iters, lr=a.lr)
     log(f"trained: loss {loss:.3e}")
     np.savez(a.out, params=p, features=X, targets=T)

@@ -7,9 +7,11 @@
 //
 // Parameters, packed: W1[64][4], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
 // (row-major, out x in).  Loss: mean over batch and outputs of w (y - t)^2
-// with w = MLP_UNDER (4) where y < t: an estimate below the count costs a
-// second filter pass, one above it only buffer memory.
+// with w = the under-weight (rnng_mlp_set_under_weight, default MLP_UNDER =
+// 4) where y < t: an estimate below the count costs a second filter pass, one
+// above it only buffer memory.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <vector>
@@ -19,6 +21,9 @@
 
 namespace rnng {
 namespace {
+
+std::atomic<float> g_under{MLP_UNDER};   // loss weight of an estimate below its target
+float under_weight() { return g_under.load(); }
 
 constexpr int OW1 = 0, OB1 = OW1 + MLP_H * MLP_I, OW2 = OB1 + MLP_H, OB2 = OW2 + MLP_H * MLP_H,
               OW3 = OB2 + MLP_H, OB3 = OW3 + MLP_O * MLP_H;
@@ -75,7 +80,7 @@ constexpr int NPL = MLP_NPARAMS + 1;
 __global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ P, const float* __restrict__ X,
                                                       const float* __restrict__ T, const float* __restrict__ Y,
                                                       const float* __restrict__ A1, const float* __restrict__ A2,
-                                                      int B, float* __restrict__ part) {
+                                                      int B, float under, float* __restrict__ part) {
     __shared__ float xs[BS][MLP_I];
     __shared__ float h1[BS][MLP_H + 1];
     __shared__ float h2[BS][MLP_H + 1];
@@ -101,7 +106,7 @@ __global__ __launch_bounds__(256) void k_mlp_backward(const float* __restrict__ 
 #pragma unroll
         for (int o = 0; o < MLP_O; ++o) {
             const float r = ok ? Y[(size_t)(s0 + s) * MLP_O + o] - T[(size_t)(s0 + s) * MLP_O + o] : 0.0f;
-            const float wr = r < 0.0f ? MLP_UNDER : 1.0f;
+            const float wr = r < 0.0f ? under : 1.0f;
             dy[s][o] = scale * wr * r;
             l += wr * r * r;
         }
@@ -216,7 +221,8 @@ struct Batch {
         k_mlp_forward<<<blocks_for(B, 256), 256, 0, stream()>>>(P.as<float>(), X.as<float>(), B, Y.as<float>(),
                                                                  A1.as<float>(), A2.as<float>());
         k_mlp_backward<<<nblocks, 256, 0, stream()>>>(P.as<float>(), X.as<float>(), T.as<float>(), Y.as<float>(),
-                                                      A1.as<float>(), A2.as<float>(), B, part.as<float>());
+                                                      A1.as<float>(), A2.as<float>(), B, under_weight(),
+                                                      part.as<float>());
         k_mlp_sum<<<dim3(blocks_for(NPL, 256), ngroups), 256, 0, stream()>>>(part.as<float>(), nblocks, GR,
                                                                               part2.as<float>());
         k_mlp_sum<<<dim3(blocks_for(NPL, 256), 1), 256, 0, stream()>>>(part2.as<float>(), ngroups, ngroups,
@@ -317,6 +323,17 @@ int rnng_mlp_train(float* params, const float* x, const float* target, int64_t b
     RNNG_HIP_TRY(hipMemcpyAsync(params, b.P.ptr, MLP_NPARAMS * sizeof(float), hipMemcpyDeviceToHost, stream()));
     return b.read(nullptr, final_loss, (int)batch);
 }
+
+int rnng_mlp_set_under_weight(float w) {
+    if (!(w > 0.0f) || !(w < 1e6f)) {
+        set_error("mlp_set_under_weight: need 0 < w < 1e6");
+        return RNNG_ERR_ARG;
+    }
+    g_under.store(w);
+    return RNNG_OK;
+}
+
+float rnng_mlp_under_weight(void) { return g_under.load(); }
 
 int rnng_set_cost_model(const float* params) {
     RNNG_TRY(ensure_init());

@@ -26,16 +26,23 @@ def test_infer_matches_oracle(orc):
     assert np.allclose(y, ref, rtol=1e-4, atol=1e-5)
 
 
-@pytest.mark.parametrize("batch", [1, 31, 32, 1000, 4133])
-def test_loss_grad_matches_oracle(orc, batch):
+@pytest.mark.parametrize("batch,under", [(1, 4.0), (31, 4.0), (32, 4.0), (1000, 4.0), (4133, 4.0), (1000, 1.0),
+                                         (4133, 2.5)])
+def test_loss_grad_matches_oracle(orc, batch, under):
     cm = _cm()
     rng = np.random.default_rng(batch)
     p = cm.init_params(batch)
     p += (rng.standard_normal(p.size) * 0.05).astype(np.float32)
     x = rng.uniform(0, 1, size=(batch, 4)).astype(np.float32)
     t = rng.uniform(0, 1, size=(batch, 2)).astype(np.float32)
-    loss, g = cm.loss_grad(p, x, t)
-    loss_ref, g_ref = orc.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    w0 = cm.under_weight()
+    try:
+        cm.set_under_weight(under)
+        loss, g = cm.loss_grad(p, x, t)
+    finally:
+        cm.set_under_weight(w0)
+    loss_ref, g_ref = orc.mlp_loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                                        under=under)
     assert np.isclose(loss, loss_ref, rtol=1e-4)
     # FP32 sums over the batch against float64: tolerance scaled by the gradient size
     tol = 1e-4 * np.abs(g_ref).max() + 1e-7
