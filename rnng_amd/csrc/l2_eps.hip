@@ -1181,11 +1181,24 @@ __device__ __forceinline__ void mf16_load_rows(const typename T::frag* __restric
         for (int q = 0; q < 4; ++q) xr[r][q] = rterm[rt0 * 32 + 16 * r + 4 * (lane >> 4) + q];
 }
 
-// one 16-column half held in LDS (b: its first k-step)
+// one 16-column half whose column fragments bb (one per k-step) were read
+// from LDS
 template <class T, int KS, int M>
-__device__ __forceinline__ void mf16_half(const typename T::frag* b, const typename T::frag (&a)[2 * M][KS / 2],
-                                          const typename T::accv16 (&xr)[2 * M], int lane,
-                                          typename T::accv16 (&acc)[2 * M]) {
+__device__ __forceinline__ void mf16_half(const typename T::frag (&bb)[KS / 2],
+                                          const typename T::frag (&a)[2 * M][KS / 2],
+                                          const typename T::accv16 (&xr)[2 * M], typename T::accv16 (&acc)[2 * M]) {
+#pragma unroll
+    for (int s = 0; s < KS / 2; ++s)
+#pragma unroll
+        for (int r = 0; r < 2 * M; ++r) acc[r] = T::mfma16(a[r][s], bb[s], s == 0 ? xr[r] : acc[r]);
+}
+
+// one 16-column half held in LDS (b: its first k-step), each k-step's
+// fragment read just before its MFMAs
+template <class T, int KS, int M>
+__device__ __forceinline__ void mf16_half_lds(const typename T::frag* b, const typename T::frag (&a)[2 * M][KS / 2],
+                                              const typename T::accv16 (&xr)[2 * M], int lane,
+                                              typename T::accv16 (&acc)[2 * M]) {
 #pragma unroll
     for (int s = 0; s < KS / 2; ++s) {
         const typename T::frag bb = b[s * 64 + lane];
@@ -1435,6 +1448,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     constexpr int NC = 64 * ((G + 1) / 2);   // start terms per stage (two tiles per load)
     static_assert(G >= 1 && G <= 16 && (S == 2 || S == 3), "ring shape");
     static_assert(!L16 || (M >= 2 && M <= 4 && KS % 2 == 0), "16x16 forms: 2-4 row tiles per wave, even k-steps");
+    // 16x16 forms: read a tile's fragments for both halves at once where the
+    // registers allow it (int8 with two row tiles per wave, all-pairs lists)
+    constexpr bool PRE = std::is_same<T, MfI8>::value && !TL && M == 2;
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
     // candidates staged in LDS; the int8 shape with four tiles per stage keeps
@@ -1446,8 +1462,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     __shared__ unsigned s_item;
     const int tid = threadIdx.x;
     // wave-uniform (scalar) in the tile-list forms, whose list positions
-    // derive from it
-    const int wave = TL ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
+    // derive from it, and in the 16x16 forms (its load addresses then stay in
+    // scalar registers)
+    const int wave = TL || L16 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
     const int lane = tid & 63;
     unsigned item = blockIdx.x;
     for (;; ) {
@@ -1508,11 +1525,22 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
             if (cc > 0 && t0 + cc > last) break;
             if constexpr (L16) {
                 const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
+                // with PRE both halves' column fragments are read before the
+                // first half's MFMAs (the second half's reads land under
+                // them); else each k-step's just before its MFMAs
+                frag bb[PRE ? 2 : 1][KS / 2];
+                if constexpr (PRE) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int s = 0; s < KS / 2; ++s) bb[h][s] = cur[cc * NCH + (h * (KS / 2) + s) * 64 + lane];
+                }
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const term nyc = -ccur[cc * 32 + 16 * h + (lane & 15)];
                     accv16 acc[2 * M];
-                    mf16_half<T, KS, M>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
+                    if constexpr (PRE) mf16_half<T, KS, M>(bb[h], a16, xr16, acc);
+                    else mf16_half_lds<T, KS, M>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
                     // the previous half's test, under this half's MFMAs
                     if (__any(mf16_min<T, M>(accp) < nycp))
                         mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount,
