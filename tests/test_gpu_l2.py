@@ -57,7 +57,7 @@ def test_small_gmm_config_bit_exact(orc, mode):
 
 
 @pytest.mark.parametrize("n,d", [(1, 3), (2, 3), (33, 3), (1000, 16), (3001, 17), (2500, 37),
-                                 (2000, 96), (1800, 128), (700, 130)])
+                                 (2000, 96), (1800, 128), (700, 130), (1, 64), (2, 64), (33, 64), (65, 128)])
 @pytest.mark.parametrize("mode", ["plain", "mfma"])
 def test_random_shapes_bit_exact(orc, n, d, mode):
     rng = np.random.default_rng(n * 7 + d)
@@ -67,6 +67,27 @@ def test_random_shapes_bit_exact(orc, n, d, mode):
     ij_ref, w_ref = orc.l2_eps_pairs(pts, eps)
     ij, w = _l2().eps_pairs(pts, eps, mode)
     _check_same(ij, w, ij_ref, w_ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 31, 33, 64, 513])
+@pytest.mark.parametrize("d", [64, 128])
+def test_int8_small_and_ragged_sizes(orc, n, d):
+    """The int8 filter (16x16 forms at an even k-step count) at sizes below,
+    at and past one 32-point tile, self join and bipartite against a ragged
+    second set."""
+    rng = np.random.default_rng(n * 3 + d)
+    X = rng.integers(0, 60, size=(n, d)).astype(F)
+    Y = rng.integers(0, 60, size=(n + 7, d)).astype(F)
+    if n > 1:
+        X[n - 1] = X[0]
+    eps = F(np.sqrt(600.0 * d) * 0.95)   # about a tenth of the pairs
+    flush_print(f"int8 small n={n} d={d}")
+    ij_ref, w_ref = orc.l2_eps_pairs(X, eps)
+    ij, w = _l2().eps_pairs(X, eps, "mfma")
+    _check_same(ij, w, ij_ref, w_ref)
+    ijb_ref, wb_ref = orc.l2_eps_pairs(X, eps, Y)
+    ijb, wb = _l2().eps_bipartite(X, Y, eps, "mfma")
+    _check_same(ijb, wb, ijb_ref, wb_ref)
 
 
 @pytest.mark.parametrize("mode", ["plain", "mfma"])
