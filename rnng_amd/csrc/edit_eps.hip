@@ -423,9 +423,7 @@ __global__ __launch_bounds__(256) void k_edit_verify(
 // byte and the multiply by 0x01020408 gathers the four into bits 24..27
 // (distinct partial products, no carries).
 template <int NP>
-__device__ __forceinline__ void plane_words(const uint8_t* __restrict__ c32, unsigned (&w)[NP]) {
-    const uint4 a = *reinterpret_cast<const uint4*>(c32);
-    const uint4 b = *reinterpret_cast<const uint4*>(c32 + 16);
+__device__ __forceinline__ void plane_words_of(const uint4 a, const uint4 b, unsigned (&w)[NP]) {
     const unsigned d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int q = 0; q < NP; ++q) {
@@ -434,6 +432,11 @@ __device__ __forceinline__ void plane_words(const uint8_t* __restrict__ c32, uns
         for (int r = 0; r < 8; ++r) v |= (((d[r] >> q) & 0x01010101u) * 0x01020408u) >> 24 << (4 * r);
         w[q] = v;
     }
+}
+
+template <int NP>
+__device__ __forceinline__ void plane_words(const uint8_t* __restrict__ c32, unsigned (&w)[NP]) {
+    plane_words_of<NP>(*reinterpret_cast<const uint4*>(c32), *reinterpret_cast<const uint4*>(c32 + 16), w);
 }
 
 // Bit-parallel banded Levenshtein of pattern p (rows, length m) against
@@ -460,9 +463,16 @@ __device__ int banded_edit_bp(const uint8_t* __restrict__ p, int m, const uint8_
     VT Pv = W, Mv = 0;
     int T = 0;       // value of the cell just above the window
     int R = 1;       // first row of the window
+    // the text's next 32 codes are read a group ahead (the loads land under
+    // the current group's columns)
+    uint4 ta = *reinterpret_cast<const uint4*>(t), tb = *reinterpret_cast<const uint4*>(t + 16);
     for (int g = 0; 32 * g < n; ++g) {
         unsigned tw[NP], ew[NP];
-        plane_words<NP>(t + 32 * g, tw);
+        plane_words_of<NP>(ta, tb, tw);
+        if (32 * (g + 1) < n) {
+            ta = *reinterpret_cast<const uint4*>(t + 32 * (g + 1));
+            tb = *reinterpret_cast<const uint4*>(t + 32 * (g + 1) + 16);
+        }
 #pragma unroll
         for (int q = 0; q < NP; ++q) ew[q] = __builtin_amdgcn_alignbit(pn[q], pc[q], (unsigned)k);
         const int cols = min(32, n - 32 * g);
