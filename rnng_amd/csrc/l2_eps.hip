@@ -1082,25 +1082,18 @@ __device__ __forceinline__ typename T::term mf_min(const typename T::accv (&acc)
     return T::vmin(T::vmin(c[0], c[1]), T::vmin(c[2], c[3]));
 }
 
-// The lanes walk their own set bits of mask together (value i of a lane is
-// row rowb + RS * (i >> 2) + (i & 3)): one ballot and one LDS reservation per
-// round, candidates staged in LDS, past the stage one global reservation per
-// round for the wave's overflow.
-template <class mask_t, int RS>
-__device__ __forceinline__ void mf_walk(mask_t mask, int rowb, int col, int lane, int2* stage, unsigned stage_cap,
-                                        unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
-                                        unsigned long long* __restrict__ counter) {
-#pragma unroll 1
-    for (;;) {
-        const bool p = mask != 0;
-        const unsigned long long m = __ballot(p);
-        if (m == 0) break;
-        int row = 0;
-        if (p) {
-            const int rq = sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)mask) : __builtin_ctz((unsigned)mask);
-            mask &= mask - 1;
-            row = rowb + RS * (rq >> 2) + (rq & 3);
-        }
+// Where the candidate walk puts a wave's candidates of one round (m: the
+// wave's ballot, p: this lane has one).
+// A workgroup's LDS stage shared by its waves: one LDS reservation per round,
+// past the stage one global reservation per round for the wave's overflow.
+struct SharedSink {
+    int2* stage;
+    unsigned stage_cap;
+    unsigned* scount;
+    int2* __restrict__ cand;
+    unsigned long long cap;
+    unsigned long long* __restrict__ counter;
+    __device__ __forceinline__ void put(bool p, unsigned long long m, int row, int col, int lane) {
         unsigned slot = 0;
         if (lane == 0) slot = atomicAdd(scount, (unsigned)__popcll(m));
         slot = __shfl(slot, 0, 64) +
@@ -1121,6 +1114,60 @@ __device__ __forceinline__ void mf_walk(mask_t mask, int rowb, int col, int lane
             }
         }
     }
+};
+
+// A wave's own slice of the LDS stage (st, cap_st >= 64 entries), its fill n
+// counted in a scalar register: a round needs no LDS reservation, and one
+// that would not fit first writes the slice out under one global reservation
+// (flush, also at the end of a work item).  No other wave touches the slice.
+struct WaveSink {
+    int2* st;
+    unsigned n;
+    unsigned cap_st;
+    int2* __restrict__ cand;
+    unsigned long long cap;
+    unsigned long long* __restrict__ counter;
+    __device__ __forceinline__ void flush(int lane) {
+        if (n == 0) return;
+        unsigned long long gb = 0;
+        if (lane == 0) gb = atomicAdd(counter, (unsigned long long)n);
+        gb = __shfl(gb, 0, 64);
+        // the slice's entries were written by other lanes of this wave: its
+        // LDS operations complete in order; the fence keeps the compiler's too
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (unsigned i = lane; i < n; i += 64)
+            if (gb + i < cap) cand[gb + i] = st[i];
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        n = 0;
+    }
+    __device__ __forceinline__ void put(bool p, unsigned long long m, int row, int col, int lane) {
+        const unsigned c = (unsigned)__popcll(m);
+        if (n + c > cap_st) flush(lane);
+        const unsigned slot =
+            n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (p) st[slot] = make_int2(row, col);
+        n += c;
+    }
+};
+
+// The lanes walk their own set bits of mask together (value i of a lane is
+// row rowb + RS * (i >> 2) + (i & 3)): one ballot per round, each round's
+// candidates to the sink.
+template <class mask_t, int RS, class Sink>
+__device__ __forceinline__ void mf_walk(mask_t mask, int rowb, int col, int lane, Sink& sk) {
+#pragma unroll 1
+    for (;;) {
+        const bool p = mask != 0;
+        const unsigned long long m = __ballot(p);
+        if (m == 0) break;
+        int row = 0;
+        if (p) {
+            const int rq = sizeof(mask_t) == 8 ? __builtin_ctzll((unsigned long long)mask) : __builtin_ctz((unsigned)mask);
+            mask &= mask - 1;
+            row = rowb + RS * (rq >> 2) + (rq & 3);
+        }
+        sk.put(p, m, row, col, lane);
+    }
 }
 
 // Rare path: candidates of one tile pair, staged in LDS.  Each lane builds
@@ -1130,11 +1177,9 @@ __device__ __forceinline__ void mf_walk(mask_t mask, int rowb, int col, int lane
 // the lanes walk their own set bits together: one ballot and one LDS
 // reservation per round.  The mask's shifts depend on an opaque zero so they
 // stay out of the hot loop.
-template <class T, int M = MRT>
+template <class T, int M = MRT, class Sink>
 __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[M], typename T::term nyc, int ct, int rt0,
-                                        int lane, int n_rows, int n_cols, int self_join, int2* stage, unsigned stage_cap,
-                                        unsigned* scount, int2* __restrict__ cand, unsigned long long cap,
-                                        unsigned long long* __restrict__ counter) {
+                                        int lane, int n_rows, int n_cols, int self_join, Sink& sk) {
     static_assert(M == 2 || M == 4, "32 or 64 values per lane");
     using mask_t = typename std::conditional<M == 4, unsigned long long, unsigned>::type;
     constexpr int NV = 16 * M;   // values per lane
@@ -1152,7 +1197,7 @@ __device__ __forceinline__ void mf_emit(const typename T::accv (&acc)[M], typena
     const int L = lim - rowb;
     const int k = L <= 0 ? 0 : (L >= 2 * NV ? NV : 4 * (L >> 3) + min(4, L & 7));
     mask &= k >= NV ? ~(mask_t)0 : (((mask_t)1 << k) - (mask_t)1);
-    mf_walk<mask_t, 8>(mask, rowb, col, lane, stage, stage_cap, scount, cand, cap, counter);
+    mf_walk<mask_t, 8>(mask, rowb, col, lane, sk);
 }
 
 // 16x16 MFMA forms of the filter (v_mfma_i32_16x16x64_i8 and
@@ -1224,11 +1269,9 @@ __device__ __forceinline__ typename T::term mf16_min(const typename T::accv16 (&
 // Rare path of a half: value i = 4 r + q of a lane is row rowb + 16 (i >> 2) +
 // (i & 3) (rising with i) of column col; the rows at or past the lane's limit
 // are dropped as a bit count, as in mf_emit.
-template <class T, int M>
+template <class T, int M, class Sink>
 __device__ __forceinline__ void mf16_emit(const typename T::accv16 (&acc)[2 * M], typename T::term nyc, int col,
-                                          int rt0, int lane, int n_rows, int n_cols, int self_join, int2* stage,
-                                          unsigned stage_cap, unsigned* scount, int2* __restrict__ cand,
-                                          unsigned long long cap, unsigned long long* __restrict__ counter) {
+                                          int rt0, int lane, int n_rows, int n_cols, int self_join, Sink& sk) {
     static_assert(M >= 2 && M <= 4, "16 to 32 values per lane");
     constexpr int NV = 8 * M;
     const int rowb = rt0 * 32 + 4 * (lane >> 4);
@@ -1244,7 +1287,7 @@ __device__ __forceinline__ void mf16_emit(const typename T::accv16 (&acc)[2 * M]
     const int L = lim - rowb;
     const int k = L <= 0 ? 0 : (L >= 4 * NV ? NV : 4 * (L >> 4) + min(4, L & 15));
     mask &= k >= 32 ? ~0u : (1u << k) - 1u;
-    mf_walk<unsigned, 16>(mask, rowb, col, lane, stage, stage_cap, scount, cand, cap, counter);
+    mf_walk<unsigned, 16>(mask, rowb, col, lane, sk);
 }
 
 // The int8 counterpart of k_l2_mfma_filter<KS, 2, true, 1>: same work list,
@@ -1297,9 +1340,10 @@ __global__ __launch_bounds__(256, KS8 <= 4 ? 4 : 1) void k_l2_mfma_filter_i8(
         }
         i32x16_t acc[MRT];
         mf_tile<MfI8, KS8>(bs[buf], a, xr, lane, acc);
-        if (__any(mf_min<MfI8>(acc) < -yc))
-            mf_emit<MfI8>(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, stage, LSTAGE, &scount, cand, cap,
-                          counter);
+        if (__any(mf_min<MfI8>(acc) < -yc)) {
+            SharedSink sk{stage, (unsigned)LSTAGE, &scount, cand, cap, counter};
+            mf_emit<MfI8>(acc, -yc, ct, rt0, lane, n_rows, n_cols, self_join, sk);
+        }
         if (has_next) {
 #pragma unroll
             for (int j = 0; j < PER; ++j) {
@@ -1453,12 +1497,12 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     constexpr bool PRE = std::is_same<T, MfI8>::value && !TL && M == 2;
     __shared__ frag bs0[G * NCH], bs1[G * NCH], bs2[S == 3 ? G * NCH : 1];
     __shared__ term cs0[NC], cs1[NC], cs2[S == 3 ? NC : 1];
-    // candidates staged in LDS; the int8 shape with four tiles per stage keeps
-    // 512 so that four workgroups fit a CU (the rest go straight to memory)
-    constexpr unsigned STG = (std::is_same<T, MfI8>::value && G >= 4) ? 512u : (unsigned)LSTAGE;
-    __shared__ int2 stage[STG];
-    __shared__ unsigned scount;
-    __shared__ unsigned long long sbase;
+    // candidates staged in LDS, WS per wave (at least one round of 64); the
+    // four-wave int8 shapes with four tiles per stage keep 512 in all so that
+    // four workgroups fit a CU
+    constexpr unsigned STG = (std::is_same<T, MfI8>::value && G >= 4 && NW <= 4) ? 512u : (unsigned)LSTAGE;
+    constexpr unsigned WS = (STG / NW) / 64 * 64 < 64 ? 64u : (STG / NW) / 64 * 64;
+    __shared__ int2 stage[NW * WS];
     __shared__ unsigned s_item;
     const int tid = threadIdx.x;
     // wave-uniform (scalar) in the tile-list forms, whose list positions
@@ -1477,7 +1521,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     }
     int4 wi;
     if (load_work_item(work, item, wi)) {   // padding items touch no memory
-    if (tid == 0) scount = 0u;
+    WaveSink sk{stage + wave * WS, 0u, WS, cand, cap, counter};
     const int rt0 = wi.x * (NW * M) + wave * M;
     const int last = wi.z - 1;
     const int ng = (wi.z - wi.y + G - 1) / G;
@@ -1510,7 +1554,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     asm volatile("" ::: "memory");
     if (S == 3 && ng > 1) mf_group_load<T, KS, G, NW, TL>(yb, cterm, bs1, cs1, wi.y + G, last, wave, lane, ti);
     // rows and group 0 landed (with three stages group 1 may still be in
-    // flight); scount is set
+    // flight)
     ring_barrier(S == 3 && ng > 1 ? nld : 0);
 
     // group g from stage cur; the loads of group g + S - 1 go to stage nxt
@@ -1543,8 +1587,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
                     else mf16_half_lds<T, KS, M>(cur + cc * NCH + h * (KS / 2) * 64, a16, xr16, lane, acc);
                     // the previous half's test, under this half's MFMAs
                     if (__any(mf16_min<T, M>(accp) < nycp))
-                        mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount,
-                                        cand, cap, counter);
+                        mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, sk);
 #pragma unroll
                     for (int r = 0; r < 2 * M; ++r) accp[r] = acc[r];
                     nycp = nyc;
@@ -1556,8 +1599,7 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
                 mf_tile<T, KS, M>(cur + cc * NCH, a, xr, lane, acc);
                 if (__any(mf_min<T, M>(acc) < nyc)) {
                     const int ct = TL ? tile_at(ti, t0 + cc) : t0 + cc;
-                    mf_emit<T, M>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand,
-                                  cap, counter);
+                    mf_emit<T, M>(acc, nyc, ct, rt0, lane, n_rows, n_cols, self_join, sk);
                 }
             }
         }
@@ -1583,15 +1625,9 @@ __global__ __launch_bounds__(64 * NW, MINW) void k_l2_mfma_filter_ring(
     }
     if constexpr (L16) {   // the last half's test
         if (__any(mf16_min<T, M>(accp) < nycp))
-            mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, stage, STG, &scount, cand, cap,
-                            counter);
+            mf16_emit<T, M>(accp, nycp, colp, rt0, lane, n_rows, n_cols, self_join, sk);
     }
-    __syncthreads();   // every wave's staged candidates and count
-    const unsigned nst = min(scount, STG);
-    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
-    __syncthreads();
-    for (unsigned i = tid; i < nst; i += 64 * NW)
-        if (sbase + i < cap) cand[sbase + i] = stage[i];
+    sk.flush(lane);   // the wave's staged candidates
     }   // item
     if constexpr (!PQ) break;
     }   // items of a persistent workgroup

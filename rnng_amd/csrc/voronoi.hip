@@ -18,11 +18,10 @@
 // The cell query runs as one self join over the points in cell order (each
 // cell padded to 32-point tiles) on the BF16 MFMA filter, restricted to the
 // tiles of cell pairs that can hold a reference pair: p in cell a and q in
-// cell b (a != b) are paired by the reference only if q is a ghost of a or p
-// a ghost of b, which needs L2(c_a, q) <= delta_q + 2 eps (or the mirror
-// image) and so L2(c_a, c_b) <= 2 max(R_a, R_b) + 2 eps with R the largest
-// delta of a cell.  Cell pairs beyond that bound (with a 1e-3 relative margin,
-// far above the FP32 distance error) are skipped.  Every evaluated pair is
+// cell b (a != b) are paired by the reference only if L2(p, q) <= eps, and so
+// only if L2(c_a, c_b) <= R_a + R_b + eps with R the largest delta of a cell
+// (triangle inequality).  Cell pairs beyond that bound (with a 1e-3 relative
+// margin, far above the FP32 distance error) are skipped.  Every evaluated pair is
 // evaluated once; a pair within eps across cells is kept exactly when the
 // reference keeps it (the exact FP32 center test of either side passes), and
 // the passing sides are the ghosts: the ghost existence test of
@@ -476,11 +475,15 @@ __global__ void k_cell_radius(const int* __restrict__ cell, const float* __restr
 }
 
 // Cell adjacency bit matrix adj[a][b / 32] (m x words): cells a != b, both
-// non-empty, with L2(c_a, c_b) <= (2 max(R_a, R_b) + 2 eps)(1 + 1e-3).  For
-// q in cell b and p in cell a, the reference can pair p and q only if
-// L2(c_a, q) <= delta_q + 2 eps or L2(c_b, p) <= delta_p + 2 eps, and either
-// implies L2(c_a, c_b) <= 2 max(R_a, R_b) + 2 eps up to rounding far below the
-// margin.  One workgroup per cell a.
+// non-empty, with L2(c_a, c_b) <= (R_a + R_b + eps)(1 + 1e-3) + 1e-15.  Any
+// edge the reference emits joins p in cell a and q in cell b with
+// L2(p, q) <= eps, so L2(c_a, c_b) <= L2(c_a, p) + L2(p, q) + L2(q, c_b) <=
+// R_a + eps + R_b.  The FP32 values here and in the reference's own distances
+// are within a few 1e-6 relative of the true ones (sums of squares), far below
+// the relative margin; the absolute one covers squares that underflow (below
+// 1e-17 in a distance for any d up to 10^4).  This is tighter than the bound
+// the ghost criterion alone gives (2 max(R_a, R_b) + 2 eps).  One workgroup
+// per cell a.
 __global__ __launch_bounds__(256) void k_cell_adjacency(const float* __restrict__ C, int m, int d,
                                                         const int* __restrict__ cnt,
                                                         const unsigned* __restrict__ rad, float eps, int words,
@@ -508,7 +511,7 @@ __global__ __launch_bounds__(256) void k_cell_adjacency(const float* __restrict_
                 acc = acc + t * t;
             }
             const double rb = (double)ord2f_v(rad[b]);
-            const double bound = (2.0 * fmax(ra, rb) + 2.0 * (double)eps) * (1.0 + 1e-3) + 1e-30;
+            const double bound = (ra + rb + (double)eps) * (1.0 + 1e-3) + 1e-15;
             keep = (double)sqrtf(acc) <= bound;
         }
         const unsigned long long bal = __ballot(keep);
