@@ -78,7 +78,7 @@ struct rnng_voronoi_s {
     std::vector<int> sched_cfg;           // tuner configurations (order << 8 | sched << 4 | variant slot)
     std::vector<double> sched_ms;         // and their times on every 8th work item
     std::vector<int> chain;               // positions -> cells along the chain (computed once)
-    rnng::LocalBuf start, ctl, queue, cdist;
+    rnng::LocalBuf start, ctl, queue, cdist, nbc;
     // the row layout on the device (position -> cell, cell -> position, padded
     // row starts, tile -> position) and the work-list builder's workspace
     rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell, flat_adj;
@@ -482,63 +482,88 @@ __global__ void k_cell_radius(const int* __restrict__ cell, const float* __restr
 // are within a few 1e-6 relative of the true ones (sums of squares), far below
 // the relative margin; the absolute one covers squares that underflow (below
 // 1e-17 in a distance for any d up to 10^4).  This is tighter than the bound
-// the ghost criterion alone gives (2 max(R_a, R_b) + 2 eps).  One workgroup
-// per cell a.
-__global__ __launch_bounds__(256) void k_cell_adjacency(const float* __restrict__ C, int m, int d,
-                                                        const int* __restrict__ cnt,
-                                                        const unsigned* __restrict__ rad, float eps, int words,
-                                                        unsigned* __restrict__ adj) {
+// the ghost criterion alone gives (2 max(R_a, R_b) + 2 eps).
+//
+// With ADJ the adjacency bits (their 64 x 64 block of the matrix: two words
+// of each of 64 rows), else the squared FP32 distances (an ordering aid) into
+// out[a * m + b].  One workgroup per 64 x 64 block of center pairs, the two
+// blocks' centers staged in LDS 32 dimensions at a time, each thread 4 x 4
+// pairs; every pair's sum runs over the dimensions in order.
+constexpr int CPB = 64;   // centers per block side
+constexpr int CPK = 32;   // dimensions per LDS round
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_center_pairs(const float* __restrict__ C, int m, int d,
+                                                      const int* __restrict__ cnt, const unsigned* __restrict__ rad,
+                                                      float eps, int words, unsigned* __restrict__ adj,
+                                                      float* __restrict__ out) {
 #pragma clang fp contract(off)
-    extern __shared__ float ca[];
-    const int a = blockIdx.x;
-    unsigned* row = adj + (size_t)a * words;
-    if (cnt[a] == 0) {
-        for (int w = threadIdx.x; w < words; w += 256) row[w] = 0u;
-        return;
+    __shared__ float ca[CPK][CPB + 1], cb[CPK][CPB + 1];
+    __shared__ unsigned bits[CPB][2];
+    const int a0 = blockIdx.y * CPB, b0 = blockIdx.x * CPB;
+    const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k0 = 0; k0 < d; k0 += CPK) {
+        const int kn = min(CPK, d - k0);
+        __syncthreads();
+        for (int e = tid; e < CPB * CPK; e += 256) {
+            const int r = e / CPK, k = e % CPK;
+            const int a = a0 + r, b = b0 + r;
+            ca[k][r] = (a < m && k < kn) ? C[(size_t)a * d + k0 + k] : 0.0f;
+            cb[k][r] = (b < m && k < kn) ? C[(size_t)b * d + k0 + k] : 0.0f;
+        }
+        __syncthreads();
+        for (int k = 0; k < kn; ++k) {
+            float va[4], vb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) va[i] = ca[k][ta * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vb[j] = cb[k][tb * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float t = va[i] - vb[j];
+                    acc[i][j] = acc[i][j] + t * t;
+                }
+        }
     }
-    for (int t = threadIdx.x; t < d; t += 256) ca[t] = C[(size_t)a * d + t];
-    __syncthreads();
-    const double ra = (double)ord2f_v(rad[a]);
-    const int lane = threadIdx.x & 63;
-    for (int b0 = 0; b0 < words * 32; b0 += 256) {
-        const int b = b0 + threadIdx.x;
-        bool keep = false;
-        if (b < m && b != a && cnt[b] > 0) {
-            const float* cb = C + (size_t)b * d;
-            float acc = 0.0f;
-            for (int k = 0; k < d; ++k) {
-                const float t = ca[k] - cb[k];
-                acc = acc + t * t;
+    if constexpr (!ADJ) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int a = a0 + ta * 4 + i, b = b0 + tb * 4 + j;
+                if (a < m && b < m) out[(size_t)a * m + b] = acc[i][j];
             }
-            const double rb = (double)ord2f_v(rad[b]);
-            const double bound = (ra + rb + (double)eps) * (1.0 + 1e-3) + 1e-15;
-            keep = (double)sqrtf(acc) <= bound;
+    } else {
+        if (tid < 2 * CPB) bits[tid >> 1][tid & 1] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int a = a0 + ta * 4 + i;
+            if (a >= m || cnt[a] == 0) continue;
+            const double ra = (double)ord2f_v(rad[a]);
+            unsigned w = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int b = b0 + tb * 4 + j;
+                if (b < m && b != a && cnt[b] > 0) {
+                    const double rb = (double)ord2f_v(rad[b]);
+                    const double bound = (ra + rb + (double)eps) * (1.0 + 1e-3) + 1e-15;
+                    if ((double)sqrtf(acc[i][j]) <= bound) w |= 1u << (((tb * 4) & 31) + j);
+                }
+            }
+            if (w) atomicOr(&bits[ta * 4 + i][(tb * 4) >> 5], w);
         }
-        const unsigned long long bal = __ballot(keep);
-        // lanes 0 and 32 write the two 32-bit words of this wave's 64 cells
-        if ((lane & 31) == 0) {
-            const int w = (b0 + (threadIdx.x & ~31)) >> 5;
-            if (w < words) row[w] = (unsigned)(bal >> (lane & 32));
+        __syncthreads();
+        if (tid < 2 * CPB) {
+            const int a = a0 + (tid >> 1), wd = (b0 >> 5) + (tid & 1);
+            if (a < m && wd < words) adj[(size_t)a * words + wd] = bits[tid >> 1][tid & 1];
         }
-    }
-}
-
-// Squared FP32 distances between all centers (an ordering aid, not a
-// decision): one workgroup per center a, row a of the m x m matrix.
-__global__ __launch_bounds__(256) void k_center_dist(const float* __restrict__ C, int m, int d,
-                                                     float* __restrict__ out) {
-    extern __shared__ float ca[];
-    const int a = blockIdx.x;
-    for (int t = threadIdx.x; t < d; t += 256) ca[t] = C[(size_t)a * d + t];
-    __syncthreads();
-    for (int b = threadIdx.x; b < m; b += 256) {
-        const float* cb = C + (size_t)b * d;
-        float acc = 0.0f;
-        for (int k = 0; k < d; ++k) {
-            const float t = ca[k] - cb[k];
-            acc += t * t;
-        }
-        out[(size_t)a * m + b] = acc;
     }
 }
 
@@ -760,7 +785,7 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uin
     if (e < n) flag[e] = (e == 0 || keys[e] != keys[e - 1]) ? 1 : 0;
 }
 
-// Cost-model features of cell a (one workgroup per cell):
+// Cost-model features of cell a:
 //   f0 = log2(1 + n) / 16                   members
 //   f1 = log2(1 + n/2 + paired) / 24         column points its tiles pair with
 //                                            (every member of adjacent later cells)
@@ -769,30 +794,82 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uin
 //                                            mean count of members within eps of
 //                                            the first min(32, n) members (exact FP32)
 constexpr int FS = 32;   // sampled members per cell
-constexpr int FM = 64;   // members staged in LDS per round
 
-__global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__ X, int d,
-                                                       const unsigned* __restrict__ adj, int words,
-                                                       const int* __restrict__ cnt, const int* __restrict__ cptr,
-                                                       const int* __restrict__ cids, const unsigned* __restrict__ rad,
-                                                       float eps, float T, int m, float* __restrict__ feat) {
+// Counts of f3: thread i takes member position i of the cell order against
+// its cell's first min(FS, n) members (the samples), eight samples at a time
+// (squared FP32 distances summed over the dimensions in order, <= T); the
+// counts go to nb[cell] per wave (per lane where a wave spans two cells).
+template <bool VEC4>
+__global__ __launch_bounds__(256) void k_cell_nb(const float* __restrict__ X, int d, int n,
+                                                 const int* __restrict__ cids, const int* __restrict__ cell,
+                                                 const int* __restrict__ cptr, const int* __restrict__ cnt, float T,
+                                                 unsigned* __restrict__ nb) {
 #pragma clang fp contract(off)
-    extern __shared__ float fl[];    // [FS][d] sampled members, then [FM][d + 1] staged members
-    float* smp = fl;
-    float* mem = fl + FS * d;
-    const int ld = d + 1;
-    const int a = blockIdx.x;
-    __shared__ long long scol[256];
-    __shared__ long long snb[256];
-    const int n = cnt[a], base = cptr[a];
-    const int S = min(FS, n);
-    for (int e = threadIdx.x; e < S * d; e += 256) {
-        const int s = e / d, k = e % d;
-        smp[e] = X[(size_t)cids[base + s] * d + k];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int c = -1;
+    unsigned cntv = 0;
+    if (i < n) {
+        const int id = cids[i];
+        c = cell[id];
+        const int base = cptr[c], S = min(FS, cnt[c]);
+        const float* q = X + (size_t)id * d;
+        for (int s0 = 0; s0 < S; s0 += 8) {
+            const float* smp[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) smp[j] = X + (size_t)cids[base + min(s0 + j, S - 1)] * d;
+            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if constexpr (VEC4) {
+                for (int k = 0; k < d; k += 4) {
+                    const float4 qv = *(const float4*)(q + k);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float4 sv = *(const float4*)(smp[j] + k);
+                        float t = sv.x - qv.x;
+                        acc[j] = acc[j] + t * t;
+                        t = sv.y - qv.y;
+                        acc[j] = acc[j] + t * t;
+                        t = sv.z - qv.z;
+                        acc[j] = acc[j] + t * t;
+                        t = sv.w - qv.w;
+                        acc[j] = acc[j] + t * t;
+                    }
+                }
+            } else {
+                for (int k = 0; k < d; ++k) {
+                    const float qv = q[k];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float t = smp[j][k] - qv;
+                        acc[j] = acc[j] + t * t;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cntv += (s0 + j < S && acc[j] <= T) ? 1u : 0u;
+        }
     }
+    // one atomic per wave when its lanes share a cell (the common case)
+    const int c0 = __shfl(c, 0, 64);
+    if (__all(c == c0)) {
+        unsigned v = cntv;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0 && c0 >= 0 && v) atomicAdd(&nb[c0], v);
+    } else if (c >= 0 && cntv) {
+        atomicAdd(&nb[c], cntv);
+    }
+}
+
+// The features of cell a (one wave per cell): f1's paired points from the
+// adjacency row, f3 from the counts of k_cell_nb.
+__global__ __launch_bounds__(256) void k_cell_feat(const unsigned* __restrict__ adj, int words,
+                                                   const int* __restrict__ cnt, const unsigned* __restrict__ rad,
+                                                   const unsigned* __restrict__ nbc, float eps, int m,
+                                                   float* __restrict__ feat) {
+    const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (a >= m) return;
     long long col = 0;
     const unsigned* row = adj + (size_t)a * words;
-    for (int w = threadIdx.x; w < words; w += 256) {
+    for (int w = lane; w < words; w += 64) {
         unsigned bits = row[w];
         if (w == (a >> 5)) bits &= ~0u << (a & 31);      // later cells only
         else if (w < (a >> 5)) bits = 0u;
@@ -802,47 +879,15 @@ __global__ __launch_bounds__(256) void k_cell_features(const float* __restrict__
             if (b > a && b < m) col += cnt[b];
         }
     }
-    // member r = tid / 4 of each round against samples 8 (tid % 4) .. + 7
-    long long nb = 0;
-    const int r = threadIdx.x >> 2, s0 = (threadIdx.x & 3) * 8;
-    for (int m0 = 0; m0 < n; m0 += FM) {
-        const int nm = min(FM, n - m0);
-        __syncthreads();
-        for (int e = threadIdx.x; e < nm * d; e += 256) {
-            const int rr = e / d, k = e % d;
-            mem[rr * ld + k] = X[(size_t)cids[base + m0 + rr] * d + k];
-        }
-        __syncthreads();
-        if (r < nm) {
-            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            for (int k = 0; k < d; ++k) {
-                const float q = mem[r * ld + k];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const float t = smp[(s0 + j < S ? s0 + j : 0) * d + k] - q;
-                    acc[j] = acc[j] + t * t;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) nb += (s0 + j < S && acc[j] <= T) ? 1 : 0;
-        }
-    }
-    scol[threadIdx.x] = col;
-    snb[threadIdx.x] = nb;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            scol[threadIdx.x] += scol[threadIdx.x + o];
-            snb[threadIdx.x] += snb[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
+    for (int o = 32; o > 0; o >>= 1) col += __shfl_xor(col, o, 64);
+    if (lane == 0) {
+        const int n = cnt[a];
+        const int S = min(FS, n);
         const double e = fmax((double)eps, 1e-30);
         const double rr = n ? (double)ord2f_v(rad[a]) : 0.0;
-        const double kk = S ? (double)snb[0] / S : 0.0;
+        const double kk = S ? (double)nbc[a] / S : 0.0;
         feat[a * 4 + 0] = (float)(log2(1.0 + n) / 16.0);
-        feat[a * 4 + 1] = (float)(log2(1.0 + 0.5 * n + (double)scol[0]) / 24.0);
+        feat[a * 4 + 1] = (float)(log2(1.0 + 0.5 * n + (double)col) / 24.0);
         feat[a * 4 + 2] = (float)(log2(1.0 + fmin(rr / e, 1e30)) / 8.0);
         feat[a * 4 + 3] = (float)(log2(1.0 + n * kk) / 24.0);
     }
@@ -1087,7 +1132,9 @@ int center_chain(rnng_voronoi_s* v, int m, int d) {
     v->chain.assign(1, 0);
     if (m <= 1) return RNNG_OK;
     RNNG_TRY(v->cdist.reserve((size_t)m * m * sizeof(float)));
-    k_center_dist<<<m, 256, (size_t)d * sizeof(float), stream()>>>(v->C.as<float>(), m, d, v->cdist.as<float>());
+    const dim3 cgrid((unsigned)((m + CPB - 1) / CPB), (unsigned)((m + CPB - 1) / CPB));
+    k_center_pairs<false><<<cgrid, 256, 0, stream()>>>(v->C.as<float>(), m, d, nullptr, nullptr, 0.0f, 0, nullptr,
+                                                       v->cdist.as<float>());
     RNNG_HIP_TRY(hipGetLastError());
     std::vector<float> D;
     RNNG_TRY(download(D, v->cdist, (size_t)m * m));
@@ -1517,8 +1564,12 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     k_cell_radius<<<blocks_for(n, 256), 256, 0, stream()>>>(v->cell.as<int>(), v->delta.as<float>(), n,
                                                             rad.as<unsigned>());
     RNNG_TRY(upload(cnt_dev, v->cnt));
-    k_cell_adjacency<<<m, 256, (size_t)d * sizeof(float), stream()>>>(
-        v->C.as<float>(), m, d, cnt_dev.as<int>(), rad.as<unsigned>(), eps, words, adj_dev.as<unsigned>());
+    {
+        const dim3 cgrid((unsigned)((m + CPB - 1) / CPB), (unsigned)((m + CPB - 1) / CPB));
+        k_center_pairs<true><<<cgrid, 256, 0, stream()>>>(v->C.as<float>(), m, d, cnt_dev.as<int>(),
+                                                          rad.as<unsigned>(), eps, words, adj_dev.as<unsigned>(),
+                                                          nullptr);
+    }
     RNNG_HIP_TRY(hipGetLastError());
     if (words > kMaxCellWords) {
         set_error("Voronoi graph: at most 32 * 20000 cells");
@@ -1528,9 +1579,20 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     LocalBuf& d_items = v->items;
     // ---- cost model: per-cell features -> predicted buffer sizes ----
     RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
-    k_cell_features<<<m, 256, (size_t)(FS * d + FM * (d + 1)) * sizeof(float), stream()>>>(
-        v->ds->X, d, adj_dev.as<unsigned>(), words, cnt_dev.as<int>(), v->cptr.as<int>(), v->cids.as<int>(),
-        rad.as<unsigned>(), eps, T, m, v->feat.as<float>());
+    RNNG_TRY(v->nbc.reserve((size_t)m * sizeof(unsigned)));
+    RNNG_TRY(launch_fill_u32(v->nbc.as<unsigned>(), m, 0u));
+    if (d % 4 == 0)
+        k_cell_nb<true><<<blocks_for(n, 256), 256, 0, stream()>>>(v->ds->X, d, n, v->cids.as<int>(),
+                                                                  v->cell.as<int>(), v->cptr.as<int>(),
+                                                                  cnt_dev.as<int>(), T, v->nbc.as<unsigned>());
+    else
+        k_cell_nb<false><<<blocks_for(n, 256), 256, 0, stream()>>>(v->ds->X, d, n, v->cids.as<int>(),
+                                                                   v->cell.as<int>(), v->cptr.as<int>(),
+                                                                   cnt_dev.as<int>(), T, v->nbc.as<unsigned>());
+    RNNG_HIP_TRY(hipGetLastError());
+    k_cell_feat<<<(m + 3) / 4, 256, 0, stream()>>>(adj_dev.as<unsigned>(), words, cnt_dev.as<int>(),
+                                                   rad.as<unsigned>(), v->nbc.as<unsigned>(), eps, m,
+                                                   v->feat.as<float>());
     RNNG_HIP_TRY(hipGetLastError());
     const float* model = cost_model_device();
     double pred_cand = 0.0, pred_edges = 0.0;

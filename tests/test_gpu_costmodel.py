@@ -150,3 +150,69 @@ def test_wild_model_outputs_keep_the_edge_set():
             assert st.predicted_candidates == 0
         else:
             assert 0 < st.predicted_candidates and st.refills == 0
+
+
+def _features_ref(X, C, cell, delta, cptr, cids, eps, T):
+    """The four per-cell features restated in numpy: FP32 sums over the
+    dimensions in order, the triangle-inequality cell adjacency in double,
+    the sampled within-cell counts against the FP32 squared threshold."""
+    m, d = C.shape
+    cnt = np.diff(cptr).astype(np.int64)
+    R = np.zeros(m, np.float32)
+    np.maximum.at(R, cell, delta)
+    acc = np.zeros((m, m), np.float32)
+    for k in range(d):
+        t = C[:, None, k] - C[None, :, k]
+        acc = acc + t * t
+    D = np.sqrt(acc).astype(np.float64)
+    R64 = R.astype(np.float64)
+    bound = (R64[:, None] + R64[None, :] + np.float64(np.float32(eps))) * (1.0 + 1e-3) + 1e-15
+    adj = (D <= bound) & (cnt[:, None] > 0) & (cnt[None, :] > 0)
+    np.fill_diagonal(adj, False)
+    col = np.triu(adj, 1).astype(np.int64) @ cnt
+    nb = np.zeros(m, np.int64)
+    for a in range(m):
+        mem = cids[cptr[a]:cptr[a + 1]]
+        smp = mem[:min(32, mem.size)]
+        s2 = np.zeros((mem.size, smp.size), np.float32)
+        for k in range(d):
+            t = X[smp, k][None, :] - X[mem, k][:, None]
+            s2 = s2 + t * t
+        nb[a] = int(np.count_nonzero(s2 <= T))
+    n = cnt.astype(np.float64)
+    S = np.minimum(32, cnt)
+    kk = np.where(S > 0, nb / np.maximum(S, 1), 0.0)
+    e = max(float(np.float32(eps)), 1e-30)
+    rr = np.where(cnt > 0, R64, 0.0)
+    f = np.stack([np.log2(1 + n) / 16, np.log2(1 + 0.5 * n + col) / 24,
+                  np.log2(1 + np.minimum(rr / e, 1e30)) / 8, np.log2(1 + n * kk) / 24], 1)
+    return f.astype(np.float32)
+
+
+@pytest.mark.parametrize("d", [20, 19])
+def test_cell_features_match_the_restatement(orc, d):
+    """The cost model's inputs for every cell (members, paired points of
+    adjacent later cells, radius, sampled neighbour count) against the numpy
+    restatement, on both load paths of the count kernel (d % 4 == 0 or not)."""
+    from rnng_amd import l2, synth, voronoi
+    cm = _cm()
+    rng = np.random.default_rng(d)
+    k = 12
+    mu = rng.uniform(0, 10, size=(k, d)).astype(np.float32)
+    X = (mu[rng.integers(0, k, 4000)] + 0.7 * rng.standard_normal((4000, d))).astype(np.float32)
+    eps = synth.radius_for_degree(X, 12.0)
+    ds = l2.Dataset(X)
+    v = voronoi.Voronoi(ds, fps=48)
+    try:
+        g = v.graph(eps)
+        f, _ = cm.cell_stats(v)
+        g.close()
+        cell, delta = v.assignment()
+        cptr, cids = v.cells()
+        C = X[v.centers()]
+        ref = _features_ref(X, C, cell, delta, cptr, cids, eps, orc.sq_threshold(eps))
+    finally:
+        v.close()
+        ds.close()
+    flush_print(f"d={d}: features max |diff| {np.abs(f - ref).max():.3g}")
+    assert np.allclose(f, ref, rtol=0, atol=1e-6), np.argwhere(~np.isclose(f, ref, rtol=0, atol=1e-6))[:5]
