@@ -2689,6 +2689,20 @@ int mfma_input_check(MfmaPrep& P, const float* X, int n, int d) {
     return RNNG_OK;
 }
 
+int mfma_i8_rows(const MfmaPrep& P, const float* X, int n, int d, const int* idmap, int npad, int* hsq,
+                 i32x4_t* xrow) {
+    if (!P.i8 || npad <= 0) return RNNG_OK;
+    if (npad % 32 != 0) {
+        set_error("int8 rows: positions must be a multiple of 32");
+        return RNNG_ERR_ARG;
+    }
+    const int vec4 = (d % 4 == 0 && ((uintptr_t)X % 16) == 0) ? 1 : 0;
+    k_prep_i8<<<npad / 32, 256, 0, stream()>>>(X, n, npad, d, P.KS8, idmap, vec4, P.colmin.as<unsigned>(), 0,
+                                               nullptr, nullptr, nullptr, nullptr, hsq, xrow, 0);
+    RNNG_HIP_TRY(hipGetLastError());
+    return RNNG_OK;
+}
+
 int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
                         int n_rowpad, const int* colmap, int n_colpad) {
     const int KS = mfma_ksteps(d);
@@ -2715,11 +2729,14 @@ int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, con
         const int Tq = Tb >= 1073741823.0f ? 1073741823 : (Tb < 0.0f ? -1 : (int)std::floor(Tb));
         const int vec4 = (d % 4 == 0 && ((uintptr_t)X % 16) == 0) ? 1 : 0;
         if (rowmap == colmap && n_rowpad == n_colpad) {
+            RNNG_TRY(P.xrow.reserve(std::max<size_t>(16, (size_t)n_rowpad * P.KS8 * 32)));
+            RNNG_TRY(P.hsq.reserve(std::max<size_t>(16, (size_t)n_rowpad * sizeof(int))));
             if (n_rowpad > 0)
                 k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,
                                                                P.colmin.as<unsigned>(), Tq, P.xa.as<i32x4_t>(),
                                                                P.yb.as<i32x4_t>(), P.xn.as<int>(), P.ycn.as<int>(),
-                                                               nullptr, nullptr, P.l16 ? 1 : 0);
+                                                               P.hsq.as<int>(), P.xrow.as<i32x4_t>(),
+                                                               P.l16 ? 1 : 0);
         } else {
             if (n_rowpad > 0)
                 k_prep_i8<<<n_rowpad / 32, 256, 0, stream()>>>(X, n, n_rowpad, d, P.KS8, rowmap, vec4,

@@ -31,6 +31,9 @@ struct MfmaPrep {
     // int32 start terms (xn rows, ycn columns) for the exact int8 filter
     bool i8 = false;
     int KS8 = 0;                                     // 32-wide int8 k-steps
+    // int8 forms of a mapped self join: x' rows (2 KS8 16-byte words) and the
+    // exact |x'|^2 by padded position, for an exact integer recheck
+    LocalBuf xrow, hsq;
     bool stats_ready = false;                        // mfma_input_check ran for the next prepare
     // fragments in the 16-row layout of the 16x16 MFMA kernels (set before
     // mfma_prepare_mapped; only kernels of that layout may then run)
@@ -55,6 +58,12 @@ int mfma_input_check(MfmaPrep& P, const float* X, int n, int d);
 // builds the int8 forms instead and sets P.i8 (threshold floor(Tb)).
 int mfma_prepare_mapped(MfmaPrep& P, const float* X, int n, int d, float Tb, const int* rowmap,
                         int n_rowpad, const int* colmap, int n_colpad);
+
+// After an int8 mfma_prepare_mapped: x' rows (2 P.KS8 16-byte words each) and
+// exact |x'|^2 of the npad positions of idmap (npad a multiple of 32, -1:
+// padding), with the offsets of that preparation.
+int mfma_i8_rows(const MfmaPrep& P, const float* X, int n, int d, const int* idmap, int npad, int* hsq,
+                 i32x4_t* xrow);
 
 constexpr int LSTAGE = 1024;   // candidates staged in LDS per work item
 

@@ -79,6 +79,9 @@ struct rnng_voronoi_s {
     std::vector<double> sched_ms;         // and their times on every 8th work item
     std::vector<int> chain;               // positions -> cells along the chain (computed once)
     rnng::LocalBuf start, ctl, queue, cdist, nbc;
+    // int8 rows of the centers (the exact integer recheck): center ids padded
+    // to 32 with -1, x' rows and |x'|^2
+    rnng::LocalBuf cmap, crow, chsq;
     // the row layout on the device (position -> cell, cell -> position, padded
     // row starts, tile -> position) and the work-list builder's workspace
     rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell, flat_adj;
@@ -606,13 +609,21 @@ static_assert(3 * RC_CELL <= 64 && (RC_CELL & (RC_CELL - 1)) == 0, "three sums p
 constexpr int GBUF = 256;   // ghost keys buffered per wave
 __host__ __device__ constexpr int cell_rows_ld(int d) { return ((d + 3) & ~3) + 4; }
 
-template <bool VEC4>
+//
+// I8 (integer data, the int8 filter's preparation): the three sums from the
+// int8 x' rows by position (xrow, hsq) and of the centers (crow, chsq; nw8
+// 16-byte words per row) as |u|^2 + |y|^2 - 2 u.y with 4-way int8 dot
+// products.  Each is the exact integer the reference's FP32 sum yields on
+// such data (every partial sum an integer below 2^24), so every decision and
+// weight is the same as from the FP32 rows; no rows are staged.
+template <bool VEC4, bool I8>
 __global__ __launch_bounds__(256) void k_cell_recheck(
     const float* __restrict__ X, const float* __restrict__ C, int d, const int2* __restrict__ cand,
     unsigned long long cap, const int* __restrict__ slot, const int* __restrict__ cell,
     const float* __restrict__ delta, float T, float eps, int2* __restrict__ ids, uint8_t* __restrict__ flag,
     float* __restrict__ cw, unsigned long long* __restrict__ counters, uint64_t* __restrict__ gkeys,
-    unsigned* __restrict__ cc) {
+    unsigned* __restrict__ cc, const i32x4_t* __restrict__ xrow, const int* __restrict__ hsq,
+    const i32x4_t* __restrict__ crow, const int* __restrict__ chsq, int nw8) {
 #pragma clang fp contract(off)
     extern __shared__ float rows_all[];   // [4 waves][2 * RC_CELL][ld]: p rows, then q rows
     __shared__ uint64_t gbuf_all[4][GBUF];
@@ -642,13 +653,16 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
     };
     for (unsigned long long c0 = beg; c0 < end; c0 += RC_CELL) {
         const bool live = lane < RC_CELL && c0 + lane < end;
-        int p = -1, q = -1;
+        int p = -1, q = -1, sp = 0, sq = 0;
         if (live) {
             const int2 sl = cand[c0 + lane];
-            p = slot[sl.x];
-            q = slot[sl.y];
+            sp = sl.x;
+            sq = sl.y;
+            p = slot[sp];
+            q = slot[sq];
         }
-        if (VEC4 && d <= 128) {
+        if (I8) {
+        } else if (VEC4 && d <= 128) {
             // row r = 2 i + (lane >> 5), float4 column lane & 31
             const int half = lane >> 5, k4 = lane & 31, nk4 = d >> 2;
             float4 v[RC_CELL];
@@ -686,7 +700,21 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
         const int ai = __shfl(a, ci, 64), bi = __shfl(b, ci, 64);
         (void)qi;
         float acc = 0.0f;
-        if (pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
+        if (I8 && pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
+            const int spi = __shfl(sp, ci, 64), sqi = __shfl(sq, ci, 64);
+            const int cu = role == 1 ? ai : bi, sy = role == 2 ? spi : sqi;
+            const i32x4_t* u = role == 0 ? xrow + (size_t)spi * nw8 : crow + (size_t)cu * nw8;
+            const i32x4_t* y = xrow + (size_t)sy * nw8;
+            int dot = 0;
+            for (int j = 0; j < nw8; ++j) {
+                const i32x4_t uv = u[j], yv = y[j];
+                dot = __builtin_amdgcn_sdot4(uv.x, yv.x, dot, false);
+                dot = __builtin_amdgcn_sdot4(uv.y, yv.y, dot, false);
+                dot = __builtin_amdgcn_sdot4(uv.z, yv.z, dot, false);
+                dot = __builtin_amdgcn_sdot4(uv.w, yv.w, dot, false);
+            }
+            acc = (float)((role == 0 ? hsq[spi] : chsq[cu]) + hsq[sy] - 2 * dot);
+        } else if (!I8 && pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
             const float* u = role == 0 ? X + (size_t)pi * d : C + (size_t)(role == 1 ? ai : bi) * d;
             const float* y = rows + (role == 2 ? ci : RC_CELL + ci) * ld;
             if (VEC4) {
@@ -716,9 +744,13 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
             if (fabs((double)w - (double)eps) <= band) {
                 atomicAdd(&counters[2], 1ull);
                 double s2 = 0.0;
-                for (int k = 0; k < d; ++k) {
-                    const double t = (double)x[k] - (double)y[k];
-                    s2 += t * t;
+                if (I8) {
+                    s2 = (double)acc;   // the exact squared distance
+                } else {
+                    for (int k = 0; k < d; ++k) {
+                        const double t = (double)x[k] - (double)y[k];
+                        s2 += t * t;
+                    }
                 }
                 if ((sqrt(s2) <= (double)eps) != edge) atomicAdd(&counters[3], 1ull);
             }
@@ -1664,6 +1696,16 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         }
         RNNG_HIP_TRY(hipGetLastError());
         RNNG_TRY(mfma_prepare_mapped(P, v->ds->X, n, d, Tb, slot.as<int>(), n_pad, slot.as<int>(), n_pad));
+        if (P.i8) {
+            const int m_pad = (m + 31) / 32 * 32;
+            std::vector<int> cm((size_t)m_pad, -1);
+            std::copy(v->centers.begin(), v->centers.end(), cm.begin());
+            RNNG_TRY(upload(v->cmap, cm));
+            RNNG_TRY(v->crow.reserve((size_t)m_pad * P.KS8 * 32));
+            RNNG_TRY(v->chsq.reserve((size_t)m_pad * sizeof(int)));
+            RNNG_TRY(mfma_i8_rows(P, v->ds->X, n, d, v->cmap.as<int>(), m_pad, v->chsq.as<int>(),
+                                  v->crow.as<i32x4_t>()));
+        }
         layout_order = order;
         return RNNG_OK;
     };
@@ -1892,12 +1934,16 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
         RNNG_TRY(launch_fill_u32(v->counts.as<unsigned>(), 2 * m, 0u));
         const unsigned rg =
             (unsigned)std::max<unsigned long long>(1, std::min<unsigned long long>(2048, (c + 63) / 64));
-        const size_t lds = (size_t)4 * 2 * RC_CELL * cell_rows_ld(d) * sizeof(float);
-        auto kern = vec4 ? k_cell_recheck<true> : k_cell_recheck<false>;
+        const bool i8 = P.i8;
+        const size_t lds = i8 ? 0 : (size_t)4 * 2 * RC_CELL * cell_rows_ld(d) * sizeof(float);
+        auto kern = i8 ? k_cell_recheck<false, true> : (vec4 ? k_cell_recheck<true, false> : k_cell_recheck<false, false>);
         kern<<<rg, 256, lds, stream()>>>(v->ds->X, v->C.as<float>(), d, cand.as<int2>(), c, slot.as<int>(),
                                          v->cell.as<int>(), v->delta.as<float>(), T, eps, cid.as<int2>(),
                                          cflag.as<uint8_t>(), cw.as<float>(), counters.as<unsigned long long>(),
-                                         v->gkeys.as<uint64_t>(), v->counts.as<unsigned>());
+                                         v->gkeys.as<uint64_t>(), v->counts.as<unsigned>(),
+                                         i8 ? P.xrow.as<i32x4_t>() : nullptr, i8 ? P.hsq.as<int>() : nullptr,
+                                         i8 ? v->crow.as<i32x4_t>() : nullptr, i8 ? v->chsq.as<int>() : nullptr,
+                                         i8 ? 2 * P.KS8 : 0);
         RNNG_HIP_TRY(hipGetLastError());
         return RNNG_OK;
     };
@@ -1931,7 +1977,9 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
             (uint64_t)(uintptr_t)v->cell.ptr, (uint64_t)(uintptr_t)v->delta.ptr, (uint64_t)(uintptr_t)v->ctl.ptr,
             (uint64_t)(uintptr_t)v->queue.ptr, (uint64_t)sched, (uint64_t)v->order,
             cap, (uint64_t)variant,
-            (uint64_t)n_pad, (uint64_t)eb, (uint64_t)tb, (uint64_t)m, (uint64_t)d};
+            (uint64_t)n_pad, (uint64_t)eb, (uint64_t)tb, (uint64_t)m, (uint64_t)d,
+            (uint64_t)(uintptr_t)P.xrow.ptr, (uint64_t)(uintptr_t)P.hsq.ptr, (uint64_t)(uintptr_t)v->crow.ptr,
+            (uint64_t)(uintptr_t)v->chsq.ptr, (uint64_t)P.i8, (uint64_t)P.KS8};
         if (v->use_graph >= 0 ? v->use_graph : graph_enabled()) {
             if (!v->fill_exec || v->fill_key != key) {
                 if (v->fill_exec) {
