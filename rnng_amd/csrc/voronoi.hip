@@ -82,6 +82,9 @@ struct rnng_voronoi_s {
     // int8 rows of the centers (the exact integer recheck): center ids padded
     // to 32 with -1, x' rows and |x'|^2
     rnng::LocalBuf cmap, crow, chsq;
+    // member chunks (cell, first member) of the feature counts, built once
+    rnng::LocalBuf nbchunks;
+    int64_t n_nbchunks = 0;
     // the row layout on the device (position -> cell, cell -> position, padded
     // row starts, tile -> position) and the work-list builder's workspace
     rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell, flat_adj;
@@ -610,12 +613,28 @@ constexpr int GBUF = 256;   // ghost keys buffered per wave
 __host__ __device__ constexpr int cell_rows_ld(int d) { return ((d + 3) & ~3) + 4; }
 
 //
-// I8 (integer data, the int8 filter's preparation): the three sums from the
+// I8 (integer data, the int8 filter's preparation): one candidate per lane
+// (64 per round), the three sums from the
 // int8 x' rows by position (xrow, hsq) and of the centers (crow, chsq; nw8
 // 16-byte words per row) as |u|^2 + |y|^2 - 2 u.y with 4-way int8 dot
 // products.  Each is the exact integer the reference's FP32 sum yields on
 // such data (every partial sum an integer below 2^24), so every decision and
-// weight is the same as from the FP32 rows; no rows are staged.
+// weight is the same as from the FP32 rows; no rows are staged, and the
+// center sums are made only for edges across cells.
+__device__ __forceinline__ int i8_sq_dist(const i32x4_t* __restrict__ u, const i32x4_t* __restrict__ y, int h,
+                                          int nw8) {
+    int dot = 0;
+#pragma unroll 4
+    for (int j = 0; j < nw8; ++j) {
+        const i32x4_t uv = u[j], yv = y[j];
+        dot = __builtin_amdgcn_sdot4(uv.x, yv.x, dot, false);
+        dot = __builtin_amdgcn_sdot4(uv.y, yv.y, dot, false);
+        dot = __builtin_amdgcn_sdot4(uv.z, yv.z, dot, false);
+        dot = __builtin_amdgcn_sdot4(uv.w, yv.w, dot, false);
+    }
+    return h - 2 * dot;
+}
+
 template <bool VEC4, bool I8>
 __global__ __launch_bounds__(256) void k_cell_recheck(
     const float* __restrict__ X, const float* __restrict__ C, int d, const int2* __restrict__ cand,
@@ -636,7 +655,8 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
     const float two_eps = 2.0f * eps;
     // this wave's range of candidates
     const unsigned long long nw = (unsigned long long)gridDim.x * 4;
-    const unsigned long long per = ((nc + nw - 1) / nw + RC_CELL - 1) / RC_CELL * RC_CELL;
+    constexpr int RC = I8 ? 64 : RC_CELL;   // candidates per round
+    const unsigned long long per = ((nc + nw - 1) / nw + RC - 1) / RC * RC;
     const unsigned long long beg = ((unsigned long long)blockIdx.x * 4 + wave) * per;
     const unsigned long long end = min(nc, beg + per);
     unsigned long long tests = 0;
@@ -651,8 +671,8 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
         gn = 0;
         __builtin_amdgcn_wave_barrier();
     };
-    for (unsigned long long c0 = beg; c0 < end; c0 += RC_CELL) {
-        const bool live = lane < RC_CELL && c0 + lane < end;
+    for (unsigned long long c0 = beg; c0 < end; c0 += RC) {
+        const bool live = lane < RC && c0 + lane < end;
         int p = -1, q = -1, sp = 0, sq = 0;
         if (live) {
             const int2 sl = cand[c0 + lane];
@@ -700,21 +720,9 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
         const int ai = __shfl(a, ci, 64), bi = __shfl(b, ci, 64);
         (void)qi;
         float acc = 0.0f;
-        if (I8 && pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
-            const int spi = __shfl(sp, ci, 64), sqi = __shfl(sq, ci, 64);
-            const int cu = role == 1 ? ai : bi, sy = role == 2 ? spi : sqi;
-            const i32x4_t* u = role == 0 ? xrow + (size_t)spi * nw8 : crow + (size_t)cu * nw8;
-            const i32x4_t* y = xrow + (size_t)sy * nw8;
-            int dot = 0;
-            for (int j = 0; j < nw8; ++j) {
-                const i32x4_t uv = u[j], yv = y[j];
-                dot = __builtin_amdgcn_sdot4(uv.x, yv.x, dot, false);
-                dot = __builtin_amdgcn_sdot4(uv.y, yv.y, dot, false);
-                dot = __builtin_amdgcn_sdot4(uv.z, yv.z, dot, false);
-                dot = __builtin_amdgcn_sdot4(uv.w, yv.w, dot, false);
-            }
-            acc = (float)((role == 0 ? hsq[spi] : chsq[cu]) + hsq[sy] - 2 * dot);
-        } else if (!I8 && pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
+        if (I8) {
+            if (live) acc = (float)i8_sq_dist(xrow + (size_t)sp * nw8, xrow + (size_t)sq * nw8, hsq[sp] + hsq[sq], nw8);
+        } else if (pi >= 0 && role < 3 && (role == 0 || ai != bi)) {
             const float* u = role == 0 ? X + (size_t)pi * d : C + (size_t)(role == 1 ? ai : bi) * d;
             const float* y = rows + (role == 2 ? ci : RC_CELL + ci) * ld;
             if (VEC4) {
@@ -734,7 +742,11 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
                 }
             }
         }
-        const float sa = __shfl(acc, RC_CELL + ci, 64), sb = __shfl(acc, 2 * RC_CELL + ci, 64);
+        float sa = 0.0f, sb = 0.0f;
+        if (!I8) {
+            sa = __shfl(acc, RC_CELL + ci, 64);
+            sb = __shfl(acc, 2 * RC_CELL + ci, 64);
+        }
         bool edge = false, tested = false, g0 = false, g1 = false;
         if (live) {
             const float* x = rows + lane * ld;              // p
@@ -756,6 +768,10 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
             }
             if (edge && a != b) {
                 tested = true;
+                if (I8) {
+                    sa = (float)i8_sq_dist(crow + (size_t)a * nw8, xrow + (size_t)sq * nw8, chsq[a] + hsq[sq], nw8);
+                    sb = (float)i8_sq_dist(crow + (size_t)b * nw8, xrow + (size_t)sp * nw8, chsq[b] + hsq[sp], nw8);
+                }
                 g0 = sqrtf(sa) <= delta[q] + two_eps;
                 g1 = sqrtf(sb) <= delta[p] + two_eps;
                 if (!g0 && !g1) edge = false;
@@ -827,34 +843,39 @@ __global__ void k_unique_flags(const uint64_t* __restrict__ keys, int64_t n, uin
 //                                            the first min(32, n) members (exact FP32)
 constexpr int FS = 32;   // sampled members per cell
 
-// Counts of f3: thread i takes member position i of the cell order against
-// its cell's first min(FS, n) members (the samples), eight samples at a time
-// (squared FP32 distances summed over the dimensions in order, <= T); the
-// counts go to nb[cell] per wave (per lane where a wave spans two cells).
+// Counts of f3: one workgroup per chunk (cell, first member) of up to 256
+// members of a cell, one thread per member against the cell's first
+// min(FS, n) members (the samples), sixteen at a time (squared FP32 distances
+// summed over the dimensions in order, <= T), the counts added to nb[cell]
+// per wave.  The samples' addresses are uniform over the workgroup, so their
+// values arrive through scalar loads as instruction operands.
 template <bool VEC4>
-__global__ __launch_bounds__(256) void k_cell_nb(const float* __restrict__ X, int d, int n,
-                                                 const int* __restrict__ cids, const int* __restrict__ cell,
+__global__ __launch_bounds__(256) void k_cell_nb(const float* __restrict__ X, int d,
+                                                 const int2* __restrict__ chunks, const int* __restrict__ cids,
                                                  const int* __restrict__ cptr, const int* __restrict__ cnt, float T,
                                                  unsigned* __restrict__ nb) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    int c = -1;
+    const int2 ch = chunks[blockIdx.x];
+    const int c = __builtin_amdgcn_readfirstlane(ch.x);
+    const int nc = __builtin_amdgcn_readfirstlane(cnt[c]), base = __builtin_amdgcn_readfirstlane(cptr[c]);
+    const int S = min(FS, nc);
+    const int i = __builtin_amdgcn_readfirstlane(ch.y) + threadIdx.x;
     unsigned cntv = 0;
-    if (i < n) {
-        const int id = cids[i];
-        c = cell[id];
-        const int base = cptr[c], S = min(FS, cnt[c]);
-        const float* q = X + (size_t)id * d;
-        for (int s0 = 0; s0 < S; s0 += 8) {
-            const float* smp[8];
+    if (i < nc) {
+        const float* q = X + (size_t)cids[base + i] * d;
+        for (int s0 = 0; s0 < S; s0 += 16) {
+            const float* smp[16];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) smp[j] = X + (size_t)cids[base + min(s0 + j, S - 1)] * d;
-            float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < 16; ++j)
+                smp[j] = X + (size_t)__builtin_amdgcn_readfirstlane(cids[base + min(s0 + j, S - 1)]) * d;
+            float acc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0f;
             if constexpr (VEC4) {
                 for (int k = 0; k < d; k += 4) {
                     const float4 qv = *(const float4*)(q + k);
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
+                    for (int j = 0; j < 16; ++j) {
                         const float4 sv = *(const float4*)(smp[j] + k);
                         float t = sv.x - qv.x;
                         acc[j] = acc[j] + t * t;
@@ -870,25 +891,18 @@ __global__ __launch_bounds__(256) void k_cell_nb(const float* __restrict__ X, in
                 for (int k = 0; k < d; ++k) {
                     const float qv = q[k];
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
+                    for (int j = 0; j < 16; ++j) {
                         const float t = smp[j][k] - qv;
                         acc[j] = acc[j] + t * t;
                     }
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 8; ++j) cntv += (s0 + j < S && acc[j] <= T) ? 1u : 0u;
+            for (int j = 0; j < 16; ++j) cntv += (s0 + j < S && acc[j] <= T) ? 1u : 0u;
         }
     }
-    // one atomic per wave when its lanes share a cell (the common case)
-    const int c0 = __shfl(c, 0, 64);
-    if (__all(c == c0)) {
-        unsigned v = cntv;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if ((threadIdx.x & 63) == 0 && c0 >= 0 && v) atomicAdd(&nb[c0], v);
-    } else if (c >= 0 && cntv) {
-        atomicAdd(&nb[c], cntv);
-    }
+    for (int o = 32; o > 0; o >>= 1) cntv += __shfl_xor(cntv, o, 64);
+    if ((threadIdx.x & 63) == 0 && cntv) atomicAdd(&nb[c], cntv);
 }
 
 // The features of cell a (one wave per cell): f1's paired points from the
@@ -1613,14 +1627,23 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     RNNG_TRY(v->feat.reserve((size_t)m * 4 * sizeof(float)));
     RNNG_TRY(v->nbc.reserve((size_t)m * sizeof(unsigned)));
     RNNG_TRY(launch_fill_u32(v->nbc.as<unsigned>(), m, 0u));
-    if (d % 4 == 0)
-        k_cell_nb<true><<<blocks_for(n, 256), 256, 0, stream()>>>(v->ds->X, d, n, v->cids.as<int>(),
-                                                                  v->cell.as<int>(), v->cptr.as<int>(),
-                                                                  cnt_dev.as<int>(), T, v->nbc.as<unsigned>());
-    else
-        k_cell_nb<false><<<blocks_for(n, 256), 256, 0, stream()>>>(v->ds->X, d, n, v->cids.as<int>(),
-                                                                   v->cell.as<int>(), v->cptr.as<int>(),
-                                                                   cnt_dev.as<int>(), T, v->nbc.as<unsigned>());
+    const bool xv4 = d % 4 == 0 && ((uintptr_t)v->ds->X % 16) == 0;
+    {
+        // chunks of up to 256 members of a cell (the cells are fixed per handle)
+        if (v->n_nbchunks == 0) {
+            std::vector<int2> ch;
+            for (int a = 0; a < m; ++a)
+                for (int o = 0; o < v->cnt[(size_t)a]; o += 256) ch.push_back(make_int2(a, o));
+            RNNG_TRY(upload(v->nbchunks, ch));
+            v->n_nbchunks = (int64_t)ch.size();
+        }
+        if (v->n_nbchunks > 0) {
+            auto kern = xv4 ? k_cell_nb<true> : k_cell_nb<false>;
+            kern<<<(unsigned)v->n_nbchunks, 256, 0, stream()>>>(v->ds->X, d, v->nbchunks.as<int2>(),
+                                                                v->cids.as<int>(), v->cptr.as<int>(),
+                                                                cnt_dev.as<int>(), T, v->nbc.as<unsigned>());
+        }
+    }
     RNNG_HIP_TRY(hipGetLastError());
     k_cell_feat<<<(m + 3) / 4, 256, 0, stream()>>>(adj_dev.as<unsigned>(), words, cnt_dev.as<int>(),
                                                    rad.as<unsigned>(), v->nbc.as<unsigned>(), eps, m,
@@ -2045,7 +2068,8 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     const int64_t nhits = (int64_t)cnt[5];
     int64_t ng = 0;
     if (nhits > 0) {
-        RNNG_TRY(sort_keys_u64(v->gkeys.as<uint64_t>(), nhits, 64));
+        // keys (cell << 32 | point), cell < m: the bits above 32 + bits_for(m) are zero
+        RNNG_TRY(sort_keys_u64(v->gkeys.as<uint64_t>(), nhits, 32 + bits_for(m)));
         LocalBuf uflag, uniq, nsel;
         RNNG_TRY(uflag.reserve((size_t)nhits));
         RNNG_TRY(uniq.reserve((size_t)nhits * sizeof(uint64_t)));

@@ -189,11 +189,12 @@ def _features_ref(X, C, cell, delta, cptr, cids, eps, T):
     return f.astype(np.float32)
 
 
-@pytest.mark.parametrize("d", [20, 19])
+@pytest.mark.parametrize("d", [20, 19, 144])
 def test_cell_features_match_the_restatement(orc, d):
     """The cost model's inputs for every cell (members, paired points of
     adjacent later cells, radius, sampled neighbour count) against the numpy
-    restatement, on both load paths of the count kernel (d % 4 == 0 or not)."""
+    restatement, on both load paths of the count kernel (d % 4 == 0 or not)
+    and at the cell graph's largest d."""
     from rnng_amd import l2, synth, voronoi
     cm = _cm()
     rng = np.random.default_rng(d)
