@@ -192,13 +192,11 @@ __global__ __launch_bounds__(256) void k_edit_filter(
     __shared__ i32x4_t bs[2][NCH];
     __shared__ int2 stage[ESTAGE];
     __shared__ int srl[4][64];
-    __shared__ unsigned scount;
-    __shared__ unsigned long long sbase;
     int4 wi;
     if (!load_work_item(work, wi)) return;   // padding item: no memory is touched
     const int tid = threadIdx.x;
-    if (tid == 0) scount = 0u;
     const int wave = tid >> 6, lane = tid & 63;
+    WaveSink sk{stage + wave * (ESTAGE / 4), 0u, (unsigned)(ESTAGE / 4), cand, cap, counter};
     const int rt0 = wi.x * (4 * ERT) + wave * ERT;
 
     i32x4_t a[ERT][EKS];
@@ -248,11 +246,12 @@ __global__ __launch_bounds__(256) void k_edit_filter(
             for (int q = (r == 0 ? 1 : 0); q < 16; ++q) mn = min(mn, acc[r][q]);
 
         if (__any(mn < nyc)) {
-            // Candidate path: each lane walks its own set sign bits (a short
-            // divergent loop: a tile holds a few survivors), reads the row's
-            // length from LDS and takes stage slots with LDS atomics.  The
-            // mask's shifts depend on an opaque zero, so the compiler cannot
-            // hoist them into every tile of the hot loop.
+            // Candidate path: the lanes walk their own set sign bits together,
+            // one per lane and round, each checking its row's length (from
+            // LDS) against the column's; a round's survivors go to the wave's
+            // slice of the stage under one ballot.  The mask's shifts depend
+            // on an opaque zero, so the compiler cannot hoist them into every
+            // tile of the hot loop.
             int opaque;
             asm volatile("s_mov_b32 %0, 0" : "=s"(opaque));
             unsigned mask = 0;
@@ -264,25 +263,23 @@ __global__ __launch_bounds__(256) void k_edit_filter(
             const int rowb = 4 * (lane >> 5);
             if (col >= nc) mask = 0;
 #pragma unroll 1
-            while (mask) {
-                const int rq = __builtin_ctz(mask);
-                mask &= mask - 1;
-                const int q = rq & 15;
-                const int rl_i = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);   // row within the wave
-                const int row = rt0 * 32 + rl_i;
-                // a self join's lengths rise with the sorted position, so
-                // clen >= the row's length there; padding rows never pass
-                const int dl = clen - srl[wave][rl_i];
-                if (self_join ? (row < col && dl <= k) : (row < nr && dl <= k && -dl <= k)) {
-                    const unsigned slot = atomicAdd(&scount, 1u);
-                    const int2 pr = make_int2(row, col);
-                    if (slot < ESTAGE) {
-                        stage[slot] = pr;
-                    } else {
-                        const unsigned long long gslot = atomicAdd(counter, 1ull);
-                        if (gslot < cap) cand[gslot] = pr;
-                    }
+            for (;;) {
+                const bool has = mask != 0;
+                if (!__any(has)) break;
+                bool pass = false;
+                int row = 0;
+                if (has) {
+                    const int rq = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    const int q = rq & 15;
+                    const int rl_i = rowb + (rq >> 4) * 32 + 8 * (q >> 2) + (q & 3);   // row within the wave
+                    row = rt0 * 32 + rl_i;
+                    // a self join's lengths rise with the sorted position, so
+                    // clen >= the row's length there; padding rows never pass
+                    const int dl = clen - srl[wave][rl_i];
+                    pass = self_join ? (row < col && dl <= k) : (row < nr && dl <= k && -dl <= k);
                 }
+                sk.put(pass, __ballot(pass), row, col, lane);
             }
         }
         if (has_next) {
@@ -293,11 +290,7 @@ __global__ __launch_bounds__(256) void k_edit_filter(
         yc = ycn;
         clen = clen_next;
     }
-    const unsigned nst = min(scount, (unsigned)ESTAGE);
-    if (tid == 0) sbase = nst ? atomicAdd(counter, (unsigned long long)nst) : 0ull;
-    __syncthreads();
-    for (unsigned i = tid; i < nst; i += 256)
-        if (sbase + i < cap) cand[sbase + i] = stage[i];
+    sk.flush(lane);   // the wave's staged candidates
 }
 
 // One side (rows or columns) of an edit join, strings in length order:

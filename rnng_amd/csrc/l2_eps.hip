@@ -1116,40 +1116,6 @@ struct SharedSink {
     }
 };
 
-// A wave's own slice of the LDS stage (st, cap_st >= 64 entries), its fill n
-// counted in a scalar register: a round needs no LDS reservation, and one
-// that would not fit first writes the slice out under one global reservation
-// (flush, also at the end of a work item).  No other wave touches the slice.
-struct WaveSink {
-    int2* st;
-    unsigned n;
-    unsigned cap_st;
-    int2* __restrict__ cand;
-    unsigned long long cap;
-    unsigned long long* __restrict__ counter;
-    __device__ __forceinline__ void flush(int lane) {
-        if (n == 0) return;
-        unsigned long long gb = 0;
-        if (lane == 0) gb = atomicAdd(counter, (unsigned long long)n);
-        gb = __shfl(gb, 0, 64);
-        // the slice's entries were written by other lanes of this wave: its
-        // LDS operations complete in order; the fence keeps the compiler's too
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        for (unsigned i = lane; i < n; i += 64)
-            if (gb + i < cap) cand[gb + i] = st[i];
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        n = 0;
-    }
-    __device__ __forceinline__ void put(bool p, unsigned long long m, int row, int col, int lane) {
-        const unsigned c = (unsigned)__popcll(m);
-        if (n + c > cap_st) flush(lane);
-        const unsigned slot =
-            n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (p) st[slot] = make_int2(row, col);
-        n += c;
-    }
-};
-
 // The lanes walk their own set bits of mask together (value i of a lane is
 // row rowb + RS * (i >> 2) + (i & 3)): one ballot per round, each round's
 // candidates to the sink.

@@ -85,6 +85,7 @@ struct rnng_voronoi_s {
     // member chunks (cell, first member) of the feature counts, built once
     rnng::LocalBuf nbchunks;
     int64_t n_nbchunks = 0;
+    bool rad_ready = false;   // rad (largest delta per cell) and cnt_dev hold this handle's cells
     // the row layout on the device (position -> cell, cell -> position, padded
     // row starts, tile -> position) and the work-list builder's workspace
     rnng::LocalBuf lay_perm, lay_pos, lay_rs, lay_tcell, flat_adj;
@@ -1604,12 +1605,16 @@ int voronoi_graph(rnng_voronoi_s* v, float eps, rnng_graph_s* g, rnng_stats* st)
     LocalBuf& rad = v->rad;
     LocalBuf& cnt_dev = v->cnt_dev;
     LocalBuf& adj_dev = v->adj_dev;
-    RNNG_TRY(rad.reserve((size_t)m * sizeof(unsigned)));
     RNNG_TRY(adj_dev.reserve((size_t)m * words * sizeof(unsigned)));
-    RNNG_TRY(launch_fill_u32(rad.as<unsigned>(), m, 0u));
-    k_cell_radius<<<blocks_for(n, 256), 256, 0, stream()>>>(v->cell.as<int>(), v->delta.as<float>(), n,
-                                                            rad.as<unsigned>());
-    RNNG_TRY(upload(cnt_dev, v->cnt));
+    if (!v->rad_ready) {   // cell radii and sizes: fixed per handle
+        RNNG_TRY(rad.reserve((size_t)m * sizeof(unsigned)));
+        RNNG_TRY(launch_fill_u32(rad.as<unsigned>(), m, 0u));
+        k_cell_radius<<<blocks_for(n, 256), 256, 0, stream()>>>(v->cell.as<int>(), v->delta.as<float>(), n,
+                                                                rad.as<unsigned>());
+        RNNG_HIP_TRY(hipGetLastError());
+        RNNG_TRY(upload(cnt_dev, v->cnt));
+        v->rad_ready = true;
+    }
     {
         const dim3 cgrid((unsigned)((m + CPB - 1) / CPB), (unsigned)((m + CPB - 1) / CPB));
         k_center_pairs<true><<<cgrid, 256, 0, stream()>>>(v->C.as<float>(), m, d, cnt_dev.as<int>(),
