@@ -112,6 +112,26 @@ def _graph_case(orc, pts, centers, eps, check_ghosts=True):
     return st
 
 
+@pytest.mark.parametrize("scale", [1.0, 0.37])
+def test_cells_at_the_adjacency_bound(orc, scale):
+    """Two cells on a line, members at 0..R from their centers, whose closest
+    cross pair is exactly eps apart: the centers are then R_a + R_b + eps
+    apart, the largest separation the cell adjacency keeps, and that pair
+    must be an edge (integer coordinates take the int8 filter, scaled ones
+    the BF16 filter); every edge and ghost as the oracle's."""
+    d = 16
+    pts = np.zeros((82, d), np.float32)
+    pts[:41, 0] = np.arange(-20, 21)
+    pts[41:, 0] = np.arange(29, 70)
+    pts[:, 1] = np.tile([0.0, 1.0], 41)    # a second coordinate: the pairs are not all collinear
+    pts *= np.float32(scale)
+    eps = float(orc.l2(pts[40], pts[41]))   # the closest cross pair, (20, 0) and (29, 1)
+    centers = np.array([20, 61])            # x = 0 and x = 49
+    flush_print(f"adjacency bound scale={scale} eps={eps}")
+    st = _graph_case(orc, pts, centers, eps)
+    assert st.edges > 0 and st.ghosts > 0
+
+
 def test_gmm_small_landmark_graph(orc):
     from rnng_amd import synth
     _, vor = _mods()
