@@ -769,12 +769,28 @@ __global__ __launch_bounds__(256) void k_cell_recheck(
             }
             if (edge && a != b) {
                 tested = true;
+                const float rq = delta[q] + two_eps, rp = delta[p] + two_eps;
                 if (I8) {
-                    sa = (float)i8_sq_dist(crow + (size_t)a * nw8, xrow + (size_t)sq * nw8, chsq[a] + hsq[sq], nw8);
-                    sb = (float)i8_sq_dist(crow + (size_t)b * nw8, xrow + (size_t)sp * nw8, chsq[b] + hsq[sp], nw8);
+                    // L2(c_a, q) <= L2(c_a, p) + L2(p, q) = delta_p + w up to
+                    // rounding: where that bound, widened by 3 kappa (kappa =
+                    // 4 (d + 8) 2^-24 covers the relative error of an FP32 sum
+                    // of d squares and its square root, for every value here)
+                    // and 1e-20 (underflow), is within the reference's
+                    // threshold, its test passes and the sum is not made
+                    const double wid = 1.0 + 12.0 * (double)(d + 8) * 5.9604644775390625e-08;
+                    const double w64 = (double)w;
+                    g0 = ((double)delta[p] + w64) * wid + 1e-20 <= (double)rq;
+                    g1 = ((double)delta[q] + w64) * wid + 1e-20 <= (double)rp;
+                    if (!g0)
+                        g0 = sqrtf((float)i8_sq_dist(crow + (size_t)a * nw8, xrow + (size_t)sq * nw8,
+                                                     chsq[a] + hsq[sq], nw8)) <= rq;
+                    if (!g1)
+                        g1 = sqrtf((float)i8_sq_dist(crow + (size_t)b * nw8, xrow + (size_t)sp * nw8,
+                                                     chsq[b] + hsq[sp], nw8)) <= rp;
+                } else {
+                    g0 = sqrtf(sa) <= rq;
+                    g1 = sqrtf(sb) <= rp;
                 }
-                g0 = sqrtf(sa) <= delta[q] + two_eps;
-                g1 = sqrtf(sb) <= delta[p] + two_eps;
                 if (!g0 && !g1) edge = false;
             }
             ids[c0 + lane] = make_int2(min(p, q), max(p, q));
