@@ -390,11 +390,25 @@ __global__ __launch_bounds__(256) void k_prep_frags(
         if (xa) *(uint4*)(xa + slot) = *(const uint4*)xv;
         if (yb) *(uint4*)(yb + slot) = *(const uint4*)yv;
     }
+    // the workgroup's maxima (non-negative floats: their bits order as
+    // unsigned), then one atomic each, and only where the running maximum (a
+    // relaxed read) is lower: every workgroup's atomics on the same two words
+    // otherwise serialise at one L2 channel
     e_max = wave_max(e_max);
     n_max = wave_max(n_max);
+    __shared__ float wmax[2][4];
     if (lane == 0) {
-        atomicMax(&stat[ST_MAXE], __float_as_uint(e_max));
-        atomicMax(&stat[ST_MAXN], __float_as_uint(n_max));
+        wmax[0][wave] = e_max;
+        wmax[1][wave] = n_max;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned eb = __float_as_uint(fmaxf(fmaxf(wmax[0][0], wmax[0][1]), fmaxf(wmax[0][2], wmax[0][3])));
+        const unsigned nb = __float_as_uint(fmaxf(fmaxf(wmax[1][0], wmax[1][1]), fmaxf(wmax[1][2], wmax[1][3])));
+        if (eb > __hip_atomic_load(&stat[ST_MAXE], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&stat[ST_MAXE], eb);
+        if (nb > __hip_atomic_load(&stat[ST_MAXN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&stat[ST_MAXN], nb);
     }
 }
 
