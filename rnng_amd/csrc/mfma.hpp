@@ -1,5 +1,6 @@
-// Shared definitions of the BF16 MFMA candidate filter (l2_eps.hip and the
-// batched Voronoi-cell kernel in voronoi.hip).
+// Shared definitions of the MFMA candidate filters (l2_eps.hip, the batched
+// Voronoi-cell path in voronoi.hip, and the edit-distance filter in
+// edit_eps.hip, which shares the per-wave candidate stage).
 #pragma once
 
 #include <vector>
@@ -38,6 +39,7 @@ struct WaveSink {
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         n = 0;
     }
+    // one round: p = this lane has candidate (row, col), m = the wave's ballot of p
     __device__ __forceinline__ void put(bool p, unsigned long long m, int row, int col, int lane) {
         const unsigned c = (unsigned)__popcll(m);
         if (n + c > cap_st) flush(lane);
