@@ -487,8 +487,9 @@ __global__ void k_cell_radius(const int* __restrict__ cell, const float* __restr
 // L2(p, q) <= eps, so L2(c_a, c_b) <= L2(c_a, p) + L2(p, q) + L2(q, c_b) <=
 // R_a + eps + R_b.  The FP32 values here and in the reference's own distances
 // are within a few 1e-6 relative of the true ones (sums of squares), far below
-// the relative margin; the absolute one covers squares that underflow (below
-// 1e-17 in a distance for any d up to 10^4).  This is tighter than the bound
+// the relative margin; the absolute one covers squares that underflow (about
+// 1e-18 in a distance at the cell path's d <= 144, even with subnormals
+// flushed).  This is tighter than the bound
 // the ghost criterion alone gives (2 max(R_a, R_b) + 2 eps).
 //
 // With ADJ the adjacency bits (their 64 x 64 block of the matrix: two words
