@@ -20,8 +20,9 @@
 // tiles of cell pairs that can hold a reference pair: p in cell a and q in
 // cell b (a != b) are paired by the reference only if L2(p, q) <= eps, and so
 // only if L2(c_a, c_b) <= R_a + R_b + eps with R the largest delta of a cell
-// (triangle inequality).  Cell pairs beyond that bound (with a 1e-3 relative
-// margin, far above the FP32 distance error) are skipped.  Every evaluated pair is
+// (triangle inequality) and L2(c_a, c_b) <= 2 (min(R_a, R_b) + eps) (each
+// point's own center is its nearest).  Cell pairs beyond these bounds (with a
+// 1e-3 relative margin, far above the FP32 distance error) are skipped.  Every evaluated pair is
 // evaluated once; a pair within eps across cells is kept exactly when the
 // reference keeps it (the exact FP32 center test of either side passes), and
 // the passing sides are the ghosts: the ghost existence test of
@@ -482,10 +483,13 @@ __global__ void k_cell_radius(const int* __restrict__ cell, const float* __restr
 }
 
 // Cell adjacency bit matrix adj[a][b / 32] (m x words): cells a != b, both
-// non-empty, with L2(c_a, c_b) <= (R_a + R_b + eps)(1 + 1e-3) + 1e-15.  Any
-// edge the reference emits joins p in cell a and q in cell b with
-// L2(p, q) <= eps, so L2(c_a, c_b) <= L2(c_a, p) + L2(p, q) + L2(q, c_b) <=
-// R_a + eps + R_b.  The FP32 values here and in the reference's own distances
+// non-empty, with L2(c_a, c_b) <= min(R_a + R_b + eps, 2 min(R_a, R_b) +
+// 2 eps)(1 + 1e-3) + 1e-15.  Any edge the reference emits joins p in cell a
+// and q in cell b with L2(p, q) <= eps, so L2(c_a, c_b) <= L2(c_a, p) +
+// L2(p, q) + L2(q, c_b) <= R_a + eps + R_b; and q is no nearer to c_a than to
+// c_b (its cell is the nearest center), so L2(c_a, c_b) <= L2(c_a, q) +
+// L2(q, c_b) <= 2 L2(c_a, q) <= 2 (R_a + eps), and likewise 2 (R_b + eps).
+// The FP32 values here and in the reference's own distances
 // are within a few 1e-6 relative of the true ones (sums of squares), far below
 // the relative margin; the absolute one covers squares that underflow (about
 // 1e-18 in a distance at the cell path's d <= 144, even with subnormals
@@ -561,7 +565,8 @@ __global__ __launch_bounds__(256) void k_center_pairs(const float* __restrict__ 
                 const int b = b0 + tb * 4 + j;
                 if (b < m && b != a && cnt[b] > 0) {
                     const double rb = (double)ord2f_v(rad[b]);
-                    const double bound = (ra + rb + (double)eps) * (1.0 + 1e-3) + 1e-15;
+                    const double bound =
+                        fmin(ra + rb + (double)eps, 2.0 * (fmin(ra, rb) + (double)eps)) * (1.0 + 1e-3) + 1e-15;
                     if ((double)sqrtf(acc[i][j]) <= bound) w |= 1u << (((tb * 4) & 31) + j);
                 }
             }

@@ -101,3 +101,61 @@ def test_oracle_cells_rule(orc):
     # a center that is not a member of its own cell (duplicate center) leaves the order alone
     ptr, ids = orc.voronoi_cells(np.array([1, 1, 0]), np.array([0, 1]), 2)
     assert ids.tolist() == [2, 1, 0]
+
+
+@pytest.mark.parametrize("scale,d,m,seed", [(1.0, 8, 40, 1), (1e-3, 16, 25, 2), (3e4, 5, 60, 3), (1.0, 33, 12, 4)])
+def test_cell_adjacency_bound_keeps_every_pair_within_eps(orc, scale, d, m, seed):
+    """The cell adjacency of the graph build, restated (FP32 center distances
+    summed in order, R = a cell's largest FP32 member distance, the bound
+    (R_a + R_b + eps)(1 + 1e-3) + 1e-15 compared in double): every pair the
+    oracle finds within eps lies in one cell or in two adjacent cells, and on
+    clustered data the bound still prunes some cell pairs."""
+    rng = np.random.default_rng(seed)
+    mu = rng.uniform(0, 10, size=(6, d))
+    X = (mu[rng.integers(0, 6, 3000)] + 0.6 * rng.standard_normal((3000, d))) * scale
+    X = X.astype(F)
+    from rnng_amd import synth
+    eps = F(synth.radius_for_degree(X, 15.0))
+    ij, _ = orc.l2_eps_pairs(X, eps)
+    centers = rng.choice(X.shape[0], m, replace=False)
+    cell, delta = orc.voronoi_assign(X, X[centers])
+    R = np.zeros(m, np.float64)
+    np.maximum.at(R, cell, delta.astype(np.float64))
+    adj = _adjacency(X[centers], R, eps)
+    a, b = cell[ij[:, 0]], cell[ij[:, 1]]
+    assert ij.shape[0] > 1000
+    assert np.all((a == b) | adj[a, b])
+    assert not adj.all()   # the bound prunes on this data
+
+
+def _adjacency(C, R, eps, wrong=False):
+    """The graph build's cell adjacency restated: FP32 center distances summed
+    in order, min(R_a + R_b + eps, 2 (min(R_a, R_b) + eps)) with a 1e-3
+    relative and 1e-15 absolute margin, compared in double.  wrong: the
+    triangle bound without eps (a planted mistake)."""
+    D = _np_l2(C[:, None, :], C[None, :, :]).astype(np.float64)
+    Ra, Rb = R[:, None], R[None, :]
+    if wrong:
+        return D <= Ra + Rb
+    return D <= np.minimum(Ra + Rb + float(eps), 2.0 * (np.minimum(Ra, Rb) + float(eps))) * (1.0 + 1e-3) + 1e-15
+
+
+def test_cell_adjacency_bound_at_its_limit(orc):
+    """Two cells on a line whose closest cross pair is exactly eps apart, the
+    centers R_a + R_b + eps apart: the pair needs the two cells adjacent, which
+    the bound keeps, and the same bound without eps (a planted mistake) does
+    not."""
+    X = np.zeros((82, 4), F)
+    X[:41, 0] = np.arange(-20, 21)
+    X[41:, 0] = np.arange(29, 70)
+    X[:, 1] = np.tile([0.0, 1.0], 41)
+    eps = orc.l2(X[40], X[41])
+    ij, _ = orc.l2_eps_pairs(X, eps)
+    centers = np.array([20, 61])
+    cell, delta = orc.voronoi_assign(X, X[centers])
+    R = np.zeros(2, np.float64)
+    np.maximum.at(R, cell, delta.astype(np.float64))
+    a, b = cell[ij[:, 0]], cell[ij[:, 1]]
+    assert np.any(a != b)
+    assert np.all((a == b) | _adjacency(X[centers], R, eps)[a, b])
+    assert not np.all((a == b) | _adjacency(X[centers], R, eps, wrong=True)[a, b])
